@@ -1,0 +1,219 @@
+"""eigenrand_amd: EigenRand's sampling hot path on MI355X (gfx950).
+
+Host-side mirror of the reference's generator API over the C ABI
+(include/eigenrand_amd.h). Names follow `Eigen::Rand`:
+
+    urng = Vmt19937_64(42)                       # Eigen::Rand::Vmt19937_64 urng{42};
+    x = normal(1024, 1024, urng)                 # Rand::normal<MatrixXf>(1024, 1024, urng)
+    y = gamma(256, 256, urng, 3.7, 1.0)          # Rand::gamma<MatrixXf>(..., 3.7, 1.0)
+
+A Vmt19937_64 object may hold many engines (a batch of independent
+`Vmt19937_64` objects); a fill of `rows x cols` then produces one block per
+engine, exactly as if each engine had filled its own matrix. Matrices are
+returned column-major like Eigen's default: element (r, c) of block g is at
+linear index c*rows + r.
+
+Device memory and streams come from PyTorch (ROCm build); the sampling itself
+runs in the native library, which must be built first.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import ErdError, InvalidArgument
+
+__all__ = [
+    "Vmt19937_64", "ErdError", "InvalidArgument", "STATE_BYTES",
+    "randbits_u64", "uniform_real", "uniform_real_f64", "normal", "normal_vop",
+    "exponential", "cauchy", "student_t", "gamma", "chi_squared",
+]
+
+STATE_BYTES = 10016  # erd_engine_state_bytes()
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream_ptr(stream):
+    torch = _torch()
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+class Vmt19937_64:
+    """A set of `n` Vmt19937_64 engines resident in GPU memory.
+
+    Engine g is `Vmt19937_64(seed + g*stride)`: its 4 MT19937-64 lanes are
+    seeded seed+g*stride .. seed+g*stride+3 (PacketRandomEngine.h:230-239).
+    With the default stride 4, n engines cover lanes seed .. seed+4n-1.
+    """
+
+    default_seed = 5489
+
+    def __init__(self, seed=default_seed, n=1, stride=4, seeds=None, stream=None, _handle=None,
+                 _parent=None):
+        self._parent = _parent
+        if _handle is not None:
+            self._h = _handle
+            return
+        h = C.c_void_p()
+        if seeds is not None:
+            arr = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
+            _lib.call("erd_engines_create", C.byref(h), arr.ctypes.data_as(C.c_void_p),
+                      int(arr.size), _stream_ptr(stream))
+        else:
+            _lib.call("erd_engines_create_range", C.byref(h), int(seed) % (1 << 64),
+                      int(stride) % (1 << 64), int(n), _stream_ptr(stream))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self):
+        return int(_lib.load().erd_engines_count(self._h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.erd_engines_destroy(h)
+            self._h = None
+
+    def view(self, first, n):
+        """Engines [first, first+n) of this set, sharing their state."""
+        h = C.c_void_p()
+        _lib.call("erd_engines_view", C.byref(h), self._h, int(first), int(n))
+        return Vmt19937_64(_handle=h, _parent=self)
+
+    def snapshot(self, stream=None):
+        """Engine records as a uint8 array (n * STATE_BYTES)."""
+        buf = np.empty(len(self) * STATE_BYTES, dtype=np.uint8)
+        _lib.call("erd_engines_snapshot", self._h, buf.ctypes.data_as(C.c_void_p), _stream_ptr(stream))
+        return buf
+
+    def restore(self, buf, stream=None):
+        buf = np.ascontiguousarray(np.asarray(buf, dtype=np.uint8))
+        if buf.size != len(self) * STATE_BYTES:
+            raise InvalidArgument("erd_engines_restore", 1)
+        _lib.call("erd_engines_restore", self._h, buf.ctypes.data_as(C.c_void_p), _stream_ptr(stream))
+
+    def copy(self, stream=None):
+        """An independent engine set in the same state (`auto c = urng;`)."""
+        out = Vmt19937_64(seed=0, n=len(self), stream=stream)
+        _lib.call("erd_engines_copy", out._h, self._h, _stream_ptr(stream))
+        return out
+
+    def discard(self, calls, stream=None):
+        _lib.call("erd_engines_discard", self._h, int(calls), _stream_ptr(stream))
+
+    def status(self, stream=None):
+        v = C.c_uint32()
+        _lib.call("erd_engines_status", self._h, C.byref(v), _stream_ptr(stream))
+        return int(v.value)
+
+
+def _alloc(urng, per_engine, dtype, out):
+    torch = _torch()
+    n = len(urng)
+    if out is None:
+        out = torch.empty(n * per_engine, dtype=dtype, device="cuda")
+    else:
+        if out.dtype != dtype or not out.is_cuda or not out.is_contiguous() or out.numel() != n * per_engine:
+            raise InvalidArgument("output tensor", 1)
+    return out
+
+
+def _shape(flat, urng, rows, cols):
+    n = len(urng)
+    if n == 1:
+        return flat.view(cols, rows).t()
+    return flat.view(n, cols, rows).transpose(1, 2)
+
+
+def _fill(name, urng, rows, cols, dtype, out, stream, *params):
+    per = int(rows) * int(cols)
+    buf = _alloc(urng, per, dtype, out)
+    _lib.call(name, urng.handle, _ptr(buf), per, *params, _stream_ptr(stream))
+    return _shape(buf, urng, rows, cols) if out is None else buf
+
+
+def randbits_u64(draws, urng, out=None, stream=None):
+    """`draws` calls of Vmt19937_64::operator(); returns (n, 4*draws) uint64."""
+    torch = _torch()
+    per = 4 * int(draws)
+    buf = _alloc(urng, per, torch.uint64, out)
+    _lib.call("erd_fill_u64", urng.handle, _ptr(buf), per, _stream_ptr(stream))
+    return buf.view(len(urng), per)
+
+
+def uniform_real(rows, cols, urng, min=0.0, max=1.0, out=None, stream=None):
+    """Rand::uniformReal<MatrixXf>(rows, cols, urng[, min, max])."""
+    return _fill("erd_fill_uniform_f32", urng, rows, cols, _torch().float32, out, stream,
+                 float(min), float(max))
+
+
+def uniform_real_f64(rows, cols, urng, min=0.0, max=1.0, out=None, stream=None):
+    """Rand::uniformReal<MatrixXd>(rows, cols, urng[, min, max])."""
+    return _fill("erd_fill_uniform_f64", urng, rows, cols, _torch().float64, out, stream,
+                 float(min), float(max))
+
+
+def normal(rows, cols, urng, mean=None, stdev=None, out=None, stream=None):
+    """Rand::normal<MatrixXf>(rows, cols, urng) (StdNormalGen) or, with
+    mean/stdev, Rand::normal<MatrixXf>(rows, cols, urng, mean, stdev) (NormalGen)."""
+    torch = _torch()
+    if mean is None and stdev is None:
+        return _fill("erd_fill_std_normal_f32", urng, rows, cols, torch.float32, out, stream)
+    return _fill("erd_fill_normal_f32", urng, rows, cols, torch.float32, out, stream,
+                 float(0.0 if mean is None else mean), float(1.0 if stdev is None else stdev))
+
+
+def normal_vop(urng, mean, stdev, out=None, stream=None):
+    """Rand::normal(urng, mean_array, stdev_array) (NormalVGen); mean/stdev are
+    CUDA float32 tensors of n*per_engine elements (engine g uses its block)."""
+    torch = _torch()
+    mean = mean.contiguous()
+    stdev = stdev.contiguous()
+    if mean.numel() != stdev.numel() or mean.numel() % len(urng):
+        raise InvalidArgument("erd_fill_normal_vop_f32", 1)
+    per = mean.numel() // len(urng)
+    buf = _alloc(urng, per, torch.float32, out)
+    _lib.call("erd_fill_normal_vop_f32", urng.handle, _ptr(buf), per, _ptr(mean), _ptr(stdev),
+              _stream_ptr(stream))
+    return buf
+
+
+def exponential(rows, cols, urng, lmbda=1.0, out=None, stream=None):
+    """Rand::exponential<MatrixXf>(rows, cols, urng, lambda)."""
+    return _fill("erd_fill_exponential_f32", urng, rows, cols, _torch().float32, out, stream,
+                 float(lmbda))
+
+
+def cauchy(rows, cols, urng, a=0.0, b=1.0, out=None, stream=None):
+    """Rand::cauchy<MatrixXf>(rows, cols, urng, a, b)."""
+    return _fill("erd_fill_cauchy_f32", urng, rows, cols, _torch().float32, out, stream,
+                 float(a), float(b))
+
+
+def student_t(rows, cols, urng, n=1.0, out=None, stream=None):
+    """Rand::studentT<MatrixXf>(rows, cols, urng, n)."""
+    return _fill("erd_fill_studentt_f32", urng, rows, cols, _torch().float32, out, stream, float(n))
+
+
+def gamma(rows, cols, urng, alpha=1.0, beta=1.0, out=None, stream=None):
+    """Rand::gamma<MatrixXf>(rows, cols, urng, alpha, beta)."""
+    return _fill("erd_fill_gamma_f32", urng, rows, cols, _torch().float32, out, stream,
+                 float(alpha), float(beta))
+
+
+def chi_squared(rows, cols, urng, n=1.0, out=None, stream=None):
+    """Rand::chiSquared<MatrixXf>(rows, cols, urng, n)."""
+    return _fill("erd_fill_chisquared_f32", urng, rows, cols, _torch().float32, out, stream, float(n))

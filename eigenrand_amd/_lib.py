@@ -1,0 +1,109 @@
+"""ctypes binding of the C ABI in include/eigenrand_amd.h.
+
+The shared library is built in-tree (eigenrand_amd/liberd_amd.so). There is no
+fallback: if the library is missing or cannot be loaded, every entry point
+raises, so a product call can never silently run on the CPU.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liberd_amd.so")
+
+ERD_OK = 0
+STATUS_NAMES = {
+    0: "ERD_OK", 1: "ERD_ERR_INVALID", 2: "ERD_ERR_NONFINITE", 3: "ERD_ERR_NOT_PSD",
+    4: "ERD_ERR_ITER_CAP", 5: "ERD_ERR_OOM", 6: "ERD_ERR_HIP", 7: "ERD_ERR_UNSUPPORTED",
+}
+
+
+class ErdError(RuntimeError):
+    """A C ABI call returned a status other than ERD_OK."""
+
+    def __init__(self, fn, status):
+        self.status = status
+        super().__init__(f"{fn} failed: {STATUS_NAMES.get(status, status)}")
+
+
+class InvalidArgument(ErdError, ValueError):
+    pass
+
+
+P = C.c_void_p
+I64 = C.c_int64
+U64 = C.c_uint64
+F32 = C.c_float
+F64 = C.c_double
+INT = C.c_int
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+_SIGS = {
+    "erd_status_string": [INT],
+    "erd_engine_state_bytes": [],
+    "erd_engines_create": [C.POINTER(P), P, I64, P],
+    "erd_engines_create_range": [C.POINTER(P), U64, U64, I64, P],
+    "erd_engines_view": [C.POINTER(P), P, I64, I64],
+    "erd_engines_destroy": [P],
+    "erd_engines_count": [P],
+    "erd_engines_device_ptr": [P],
+    "erd_engines_snapshot": [P, P, P],
+    "erd_engines_restore": [P, P, P],
+    "erd_engines_copy": [P, P, P],
+    "erd_engines_discard": [P, I64, P],
+    "erd_engines_status": [P, C.POINTER(C.c_uint32), P],
+    "erd_fill_u64": [P, P, I64, P],
+    "erd_fill_uniform_f32": [P, P, I64, F32, F32, P],
+    "erd_fill_uniform_f64": [P, P, I64, F64, F64, P],
+    "erd_fill_std_normal_f32": [P, P, I64, P],
+    "erd_fill_normal_f32": [P, P, I64, F32, F32, P],
+    "erd_fill_normal_vop_f32": [P, P, I64, P, P, P],
+    "erd_fill_exponential_f32": [P, P, I64, F32, P],
+    "erd_fill_cauchy_f32": [P, P, I64, F32, F32, P],
+    "erd_fill_studentt_f32": [P, P, I64, F32, P],
+    "erd_fill_gamma_f32": [P, P, I64, F32, F32, P],
+    "erd_fill_chisquared_f32": [P, P, I64, F32, P],
+    "erd_device_alloc": [C.POINTER(P), C.c_size_t],
+    "erd_device_free": [P],
+    "erd_memcpy_d2h": [P, P, C.c_size_t, P],
+    "erd_memcpy_h2d": [P, P, C.c_size_t, P],
+    "erd_stream_synchronize": [P],
+}
+_RESTYPES = {
+    "erd_status_string": C.c_char_p,
+    "erd_engine_state_bytes": C.c_size_t,
+    "erd_engines_count": I64,
+    "erd_engines_device_ptr": P,
+}
+
+_lib = None
+
+
+def load():
+    """Load the native library (once) and declare every signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"eigenrand_amd native library not built ({LIB_PATH}); run __graft_entry__.build()")
+    lib = C.CDLL(LIB_PATH)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPES.get(name, INT)
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Call a status-returning entry point and raise on failure."""
+    rc = getattr(load(), name)(*args)
+    if rc != ERD_OK:
+        if rc == 1:
+            raise InvalidArgument(name, rc)
+        raise ErdError(name, rc)
+    return rc
+
+
+def declared_symbols():
+    return list(_SIGS)

@@ -1,0 +1,252 @@
+// The C ABI declared in include/eigenrand_amd.h: argument checks, engine-set
+// lifetime, and dispatch to the kernel launchers.
+#include <new>
+#include <vector>
+#include <cmath>
+#include <cstring>
+#include "eigenrand_amd.h"
+#include "erd_device.hpp"
+#include "erd_kernels.hpp"
+#include "erd_capi_internal.hpp"
+
+using erd::EngineState;
+
+namespace {
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int hip_status(hipError_t e) {
+    if (e == hipSuccess) return ERD_OK;
+    if (e == hipErrorOutOfMemory) return ERD_ERR_OOM;
+    return ERD_ERR_HIP;
+}
+
+inline bool valid_set(const erd_engines* e) { return e && e->states && e->count > 0; }
+
+// Common checks of a fill call: a valid set, a multiple of the packet width,
+// and a device output pointer whenever something is written.
+inline int check_fill(const erd_engines* e, const void* out, int64_t per, int64_t width) {
+    if (!valid_set(e)) return ERD_ERR_INVALID;
+    if (per < 0 || per % width != 0) return ERD_ERR_INVALID;
+    if (per > 0 && !out) return ERD_ERR_INVALID;
+    return ERD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* erd_status_string(int status) {
+    switch (status) {
+        case ERD_OK: return "ok";
+        case ERD_ERR_INVALID: return "invalid argument";
+        case ERD_ERR_NONFINITE: return "non-finite weights";
+        case ERD_ERR_NOT_PSD: return "matrix is not positive semi-definite";
+        case ERD_ERR_ITER_CAP: return "rejection loop hit the iteration cap";
+        case ERD_ERR_OOM: return "device out of memory";
+        case ERD_ERR_HIP: return "HIP runtime error";
+        case ERD_ERR_UNSUPPORTED: return "unsupported parameters";
+        default: return "unknown status";
+    }
+}
+
+size_t erd_engine_state_bytes(void) { return sizeof(EngineState); }
+
+int erd_engines_create(erd_engines** out, const uint64_t* seeds, int64_t n, void* stream) {
+    if (!out || !seeds || n <= 0) return ERD_ERR_INVALID;
+    *out = nullptr;
+    erd_engines* e = new (std::nothrow) erd_engines();
+    if (!e) return ERD_ERR_OOM;
+    uint64_t* d_seeds = nullptr;
+    hipError_t err = hipMalloc(&e->states, sizeof(EngineState) * (size_t)n);
+    if (err == hipSuccess) err = hipMalloc(&e->status, sizeof(uint32_t));
+    if (err == hipSuccess) err = hipMalloc(&d_seeds, sizeof(uint64_t) * (size_t)n);
+    if (err == hipSuccess) err = hipMemsetAsync(e->status, 0, sizeof(uint32_t), S(stream));
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(d_seeds, seeds, sizeof(uint64_t) * (size_t)n, hipMemcpyHostToDevice, S(stream));
+    if (err == hipSuccess) err = erd::launch_seed(e->states, d_seeds, n, S(stream));
+    if (err == hipSuccess) err = hipStreamSynchronize(S(stream));
+    if (d_seeds) (void)hipFree(d_seeds);
+    if (err != hipSuccess) {
+        if (e->states) (void)hipFree(e->states);
+        if (e->status) (void)hipFree(e->status);
+        delete e;
+        return hip_status(err);
+    }
+    e->count = n;
+    e->owns = true;
+    *out = e;
+    return ERD_OK;
+}
+
+int erd_engines_create_range(erd_engines** out, uint64_t seed0, uint64_t stride, int64_t n,
+                             void* stream) {
+    if (!out || n <= 0) return ERD_ERR_INVALID;
+    std::vector<uint64_t> seeds((size_t)n);
+    for (int64_t g = 0; g < n; ++g) seeds[(size_t)g] = seed0 + (uint64_t)g * stride;
+    return erd_engines_create(out, seeds.data(), n, stream);
+}
+
+int erd_engines_view(erd_engines** out, erd_engines* parent, int64_t first, int64_t n) {
+    if (!out || !valid_set(parent) || first < 0 || n <= 0 || first + n > parent->count)
+        return ERD_ERR_INVALID;
+    erd_engines* e = new (std::nothrow) erd_engines();
+    if (!e) return ERD_ERR_OOM;
+    e->states = parent->states + first;
+    e->status = parent->status;
+    e->count = n;
+    e->owns = false;
+    *out = e;
+    return ERD_OK;
+}
+
+int erd_engines_destroy(erd_engines* e) {
+    if (!e) return ERD_OK;
+    hipError_t err = hipSuccess;
+    if (e->owns) {
+        err = hipFree(e->states);
+        hipError_t e2 = hipFree(e->status);
+        if (err == hipSuccess) err = e2;
+    }
+    delete e;
+    return hip_status(err);
+}
+
+int64_t erd_engines_count(const erd_engines* e) { return e ? e->count : 0; }
+
+void* erd_engines_device_ptr(erd_engines* e) { return e ? (void*)e->states : nullptr; }
+
+int erd_engines_snapshot(const erd_engines* e, void* host_dst, void* stream) {
+    if (!valid_set(e) || !host_dst) return ERD_ERR_INVALID;
+    hipError_t err = hipMemcpyAsync(host_dst, e->states, sizeof(EngineState) * (size_t)e->count,
+                                    hipMemcpyDeviceToHost, S(stream));
+    if (err == hipSuccess) err = hipStreamSynchronize(S(stream));
+    return hip_status(err);
+}
+
+int erd_engines_restore(erd_engines* e, const void* host_src, void* stream) {
+    if (!valid_set(e) || !host_src) return ERD_ERR_INVALID;
+    // Reject records whose block position or cache flag is out of range: a
+    // kernel would index LDS with them.
+    const EngineState* rec = static_cast<const EngineState*>(host_src);
+    for (int64_t g = 0; g < e->count; ++g) {
+        if (rec[g].idx < 0 || rec[g].idx > erd::kN) return ERD_ERR_INVALID;
+        if (rec[g].half_valid != 0 && rec[g].half_valid != 1) return ERD_ERR_INVALID;
+    }
+    hipError_t err = hipMemcpyAsync(e->states, host_src, sizeof(EngineState) * (size_t)e->count,
+                                    hipMemcpyHostToDevice, S(stream));
+    if (err == hipSuccess) err = hipStreamSynchronize(S(stream));
+    return hip_status(err);
+}
+
+int erd_engines_copy(erd_engines* dst, const erd_engines* src, void* stream) {
+    if (!valid_set(dst) || !valid_set(src) || dst->count != src->count) return ERD_ERR_INVALID;
+    return hip_status(hipMemcpyAsync(dst->states, src->states, sizeof(EngineState) * (size_t)src->count,
+                                     hipMemcpyDeviceToDevice, S(stream)));
+}
+
+int erd_engines_discard(erd_engines* e, int64_t calls, void* stream) {
+    if (!valid_set(e) || calls < 0) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_discard(e->states, e->count, calls, S(stream)));
+}
+
+int erd_engines_status(erd_engines* e, uint32_t* status_out, void* stream) {
+    if (!valid_set(e) || !status_out) return ERD_ERR_INVALID;
+    uint32_t v = 0;
+    hipError_t err = hipMemcpyAsync(&v, e->status, sizeof(uint32_t), hipMemcpyDeviceToHost, S(stream));
+    if (err == hipSuccess) err = hipStreamSynchronize(S(stream));
+    if (err == hipSuccess) err = hipMemsetAsync(e->status, 0, sizeof(uint32_t), S(stream));
+    if (err == hipSuccess) err = hipStreamSynchronize(S(stream));
+    *status_out = v;
+    return hip_status(err);
+}
+
+int erd_fill_u64(erd_engines* e, uint64_t* out, int64_t per, void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_u64(e->states, e->count, out, per, S(stream)));
+}
+
+int erd_fill_uniform_f32(erd_engines* e, float* out, int64_t per, float a, float b, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    const int affine = !(a == 0.0f && b == 1.0f);
+    return hip_status(erd::launch_fill_uniform_f32(e->states, e->count, out, per, a, b, affine, S(stream)));
+}
+
+int erd_fill_uniform_f64(erd_engines* e, double* out, int64_t per, double a, double b, void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    const int affine = !(a == 0.0 && b == 1.0);
+    return hip_status(erd::launch_fill_uniform_f64(e->states, e->count, out, per, a, b, affine, S(stream)));
+}
+
+int erd_fill_std_normal_f32(erd_engines* e, float* out, int64_t per, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_normal_f32(e->states, e->count, out, per, 0.f, 1.f, 0, S(stream)));
+}
+
+int erd_fill_normal_f32(erd_engines* e, float* out, int64_t per, float mean, float sd, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_normal_f32(e->states, e->count, out, per, mean, sd, 1, S(stream)));
+}
+
+int erd_fill_normal_vop_f32(erd_engines* e, float* out, int64_t per, const float* mean,
+                            const float* sd, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    if (per > 0 && (!mean || !sd)) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_fill_normal_vop_f32(e->states, e->count, out, per, mean, sd, S(stream)));
+}
+
+int erd_fill_exponential_f32(erd_engines* e, float* out, int64_t per, float lambda, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_exponential_f32(e->states, e->count, out, per, lambda, S(stream)));
+}
+
+int erd_fill_cauchy_f32(erd_engines* e, float* out, int64_t per, float a, float b, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_cauchy_f32(e->states, e->count, out, per, a, b, S(stream)));
+}
+
+int erd_fill_studentt_f32(erd_engines* e, float* out, int64_t per, float n, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_studentt_f32(e->states, e->count, out, per, n, S(stream)));
+}
+
+int erd_fill_gamma_f32(erd_engines* e, float* out, int64_t per, float alpha, float beta, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    if (!(alpha > 0.0f) || !std::isfinite(alpha)) return ERD_ERR_INVALID;
+    if (alpha == 1.0f || ((float)(int)alpha == alpha && alpha < 20.0f))
+        return hip_status(erd::launch_fill_gamma_fixed_f32(e->states, e->count, out, per, alpha, beta, S(stream)));
+    return ERD_ERR_UNSUPPORTED;
+}
+
+int erd_fill_chisquared_f32(erd_engines* e, float* out, int64_t per, float n, void* stream) {
+    return erd_fill_gamma_f32(e, out, per, n * 0.5f, 2.0f, stream);
+}
+
+int erd_device_alloc(void** ptr, size_t bytes) {
+    if (!ptr) return ERD_ERR_INVALID;
+    return hip_status(hipMalloc(ptr, bytes));
+}
+int erd_device_free(void* ptr) { return hip_status(hipFree(ptr)); }
+int erd_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+    hipError_t err = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(stream));
+    if (err == hipSuccess) err = hipStreamSynchronize(S(stream));
+    return hip_status(err);
+}
+int erd_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+    hipError_t err = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(stream));
+    if (err == hipSuccess) err = hipStreamSynchronize(S(stream));
+    return hip_status(err);
+}
+int erd_stream_synchronize(void* stream) { return hip_status(hipStreamSynchronize(S(stream))); }
+
+}  // extern "C"

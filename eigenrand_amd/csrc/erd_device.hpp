@@ -1,0 +1,258 @@
+// Device building blocks for the EigenRand sampling path on gfx950.
+//
+// Engine model: one Vmt19937_64 (EigenRand/PacketRandomEngine.h:585-599, AVX2
+// layout) = 4 independent MT19937-64 lanes. One workgroup of one wave64 owns
+// one engine for the whole fill: its 4x312 u64 state lives in LDS, the twist is
+// done lane-parallel in three dependency phases, tempering happens in
+// registers as words are consumed.
+//
+// Float math: every function reproduces the op sequence of the reference's
+// unfused packet code (AVX2 without FMA), so the library is compiled with
+// -ffp-contract=off and correctly rounded fp32 sqrt/div.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace erd {
+
+constexpr int kN = 312;                 // MT19937-64 state words per lane
+constexpr int kM = 156;
+constexpr int kLanes = 4;               // u64 lanes per Vmt19937_64 call (AVX2)
+constexpr int kWords = kN * kLanes;     // 1248 u64 per engine
+constexpr uint64_t kA = 0xb5026f5aa96619e9ull;
+constexpr uint64_t kUM = 0xFFFFFFFF80000000ull;
+constexpr uint64_t kLM = 0x000000007FFFFFFFull;
+constexpr uint64_t kF = 6364136223846793005ull;
+
+constexpr double kPi = 3.1415926535897932;   // EigenRand/Dists/Basic.h:23
+constexpr double kE = 2.7182818284590452;    // EigenRand/Dists/Basic.h:24
+
+// Per-engine record in HBM. The layout is part of the C ABI (snapshot/restore).
+struct EngineState {
+    uint64_t mt[kWords];     // word i of lane l at mt[i*4 + l] (packet-major)
+    int32_t idx;             // next word 0..312; 312 = twist before next draw
+    int32_t half_valid;      // MersenneTwister::half() cache flag
+    uint64_t half_cache[2];  // upper 128 bits of the draw that filled the cache
+    uint64_t reserved;
+};
+static_assert(sizeof(EngineState) == 10016, "engine record layout");
+
+// Error bits a kernel can raise (OR-ed into a device status word).
+constexpr uint32_t kStatusIterCap = 1u;
+
+__device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t as_u(float f) { return __float_as_uint(f); }
+
+// ----------------------------------------------------------- MT19937-64 ---
+__device__ __forceinline__ uint64_t temper(uint64_t y) {
+    // PacketRandomEngine.h:303-308
+    y ^= (y >> 29) & 0x5555555555555555ull;
+    y ^= (y << 17) & 0x71d67fffeda60000ull;
+    y ^= (y << 37) & 0xfff7eee000000000ull;
+    y ^= y >> 43;
+    return y;
+}
+
+__device__ __forceinline__ uint64_t twist_word(uint64_t cur, uint64_t next, uint64_t far) {
+    uint64_t y = (cur & kUM) | (next & kLM);
+    uint64_t mag = (uint64_t)(-(int64_t)(y & 1)) & kA;
+    return far ^ (y >> 1) ^ mag;
+}
+
+// In-place twist of the 4 lanes held in LDS (packet-major, so consecutive
+// threads touch consecutive u64 words: conflict-free ds_read_b64).
+// Phase 1 (i < 156) reads only old words; phase 2 (156 <= i < 311) reads the
+// phase-1 result at i-156; phase 3 (i = 311) reads new words 0 and 155.
+// Within a phase a word i is read (as i+1) before it is rewritten because the
+// store depends on the load and later loop trips only touch higher i.
+__device__ __forceinline__ void twist_lds(uint64_t* mt, int lane) {
+    for (int t = lane; t < kM * kLanes; t += 64)
+        mt[t] = twist_word(mt[t], mt[t + kLanes], mt[t + kM * kLanes]);
+    __syncthreads();
+    for (int t = kM * kLanes + lane; t < (kN - 1) * kLanes; t += 64)
+        mt[t] = twist_word(mt[t], mt[t + kLanes], mt[t - kM * kLanes]);
+    __syncthreads();
+    if (lane < kLanes) {
+        int t = (kN - 1) * kLanes + lane;
+        mt[t] = twist_word(mt[t], mt[lane], mt[t - kM * kLanes]);
+    }
+    __syncthreads();
+}
+
+// Coalesced 16-byte copies of the 9984-byte word block between HBM and LDS.
+__device__ __forceinline__ void load_state(uint64_t* lds, const EngineState* st, int lane) {
+    const uint4* src = reinterpret_cast<const uint4*>(st->mt);
+    uint4* dst = reinterpret_cast<uint4*>(lds);
+    for (int i = lane; i < kWords / 2; i += 64) dst[i] = src[i];
+    __syncthreads();
+}
+__device__ __forceinline__ void store_state(EngineState* st, const uint64_t* lds, int lane) {
+    const uint4* src = reinterpret_cast<const uint4*>(lds);
+    uint4* dst = reinterpret_cast<uint4*>(st->mt);
+    for (int i = lane; i < kWords / 2; i += 64) dst[i] = src[i];
+}
+
+// ------------------------------------------------------ bits -> uniforms ---
+// bit_to_ur_float: MorePacketMath.h:205-215.
+__device__ __forceinline__ float ur_f32(uint32_t w) {
+    return as_f((w & 0x7FFFFFu) | 0x3F800000u) - 1.0f;
+}
+// nonzero_uniform_real: RandUtils.h:47-51 (epsilon/8).
+__device__ __forceinline__ float nzur_f32(uint32_t w) {
+    return ur_f32(w) + 1.1920928955078125e-07f / 8;
+}
+// bit_to_ur_double: MorePacketMath.h:217-227.
+__device__ __forceinline__ double ur_f64(uint64_t w) {
+    return __longlong_as_double((long long)((w & 0xFFFFFFFFFFFFFull) | (1023ull << 52))) - 1.0;
+}
+
+// ------------------------------------------------------------ float math ---
+#define ERD_F(x) ((float)(x))
+
+// ln(x): cephes logf with ln2 split in q1/q2, as the reference writes it
+// (EigenRand/arch/NEON/MorePacketMath.h:440-521), unfused.
+__device__ __forceinline__ float plog(float x0) {
+    const bool invalid = !(0.0f <= x0);
+    const bool iszero = (x0 == 0.0f);
+    const float min_norm = as_f(0x00800000u);
+    float x = (x0 < min_norm) ? min_norm : x0;
+    int32_t emm0 = (int32_t)(as_u(x) >> 23) - 0x7f;
+    x = as_f((as_u(x) & 0x807FFFFFu) | 0x3F000000u);
+    float e = (float)emm0 + 1.0f;
+    const bool mask = x < 0.707106781186547524f;
+    float tmp = mask ? x : 0.0f;
+    x = x - 1.0f;
+    e = e - (mask ? 1.0f : 0.0f);
+    x = x + tmp;
+    const float x2 = x * x;
+    const float x3 = x2 * x;
+    float y = 7.0376836292E-2f * x + -1.1514610310E-1f;
+    float y1 = -1.2420140846E-1f * x + 1.4249322787E-1f;
+    float y2 = 2.0000714765E-1f * x + -2.4999993993E-1f;
+    y = y * x + 1.1676998740E-1f;
+    y1 = y1 * x + -1.6668057665E-1f;
+    y2 = y2 * x + 3.3333331174E-1f;
+    y = y * x3 + y1;
+    y = y * x3 + y2;
+    y = y * x3;
+    y1 = e * -2.12194440e-4f;
+    tmp = x2 * 0.5f;
+    y = y + y1;
+    x = x - tmp;
+    y2 = e * 0.693359375f;
+    x = x + y;
+    x = x + y2;
+    if (iszero) return as_f(0xff800000u);
+    if (invalid) return as_f(0xFFFFFFFFu);
+    return x;
+}
+
+// plog restricted to inputs in [2^-126, +inf) finite: the special-case masks
+// of plog are constant there, so the result is identical and cheaper.
+__device__ __forceinline__ float plog_pos(float x) {
+    int32_t emm0 = (int32_t)(as_u(x) >> 23) - 0x7f;
+    x = as_f((as_u(x) & 0x807FFFFFu) | 0x3F000000u);
+    float e = (float)emm0 + 1.0f;
+    const bool mask = x < 0.707106781186547524f;
+    float tmp = mask ? x : 0.0f;
+    x = x - 1.0f;
+    e = e - (mask ? 1.0f : 0.0f);
+    x = x + tmp;
+    const float x2 = x * x;
+    const float x3 = x2 * x;
+    float y = 7.0376836292E-2f * x + -1.1514610310E-1f;
+    float y1 = -1.2420140846E-1f * x + 1.4249322787E-1f;
+    float y2 = 2.0000714765E-1f * x + -2.4999993993E-1f;
+    y = y * x + 1.1676998740E-1f;
+    y1 = y1 * x + -1.6668057665E-1f;
+    y2 = y2 * x + 3.3333331174E-1f;
+    y = y * x3 + y1;
+    y = y * x3 + y2;
+    y = y * x3;
+    y1 = e * -2.12194440e-4f;
+    tmp = x2 * 0.5f;
+    y = y + y1;
+    x = x - tmp;
+    y2 = e * 0.693359375f;
+    x = x + y;
+    return x + y2;
+}
+
+// exp(x): cephes expf (ln2 split C1/C2, degree-5 polynomial, scale by 2^m),
+// unfused; same op sequence as the oracle's pexp.
+__device__ __forceinline__ float pexp(float x0) {
+    float x = x0 < 88.723f ? x0 : 88.723f;
+    x = x > -88.723f ? x : -88.723f;
+    const float m = floorf(x * 1.44269504088896341f + 0.5f);
+    float r = m * -0.693359375f + x;
+    r = m * 2.12194440e-4f + r;
+    const float r2 = r * r;
+    const float r3 = r2 * r;
+    float y = 1.9875691500E-4f * r + 1.3981999507E-3f;
+    float y1 = 4.1665795894E-2f * r + 1.6666665459E-1f;
+    const float y2 = r + 1.0f;
+    y = y * r + 8.3334519073E-3f;
+    y1 = y1 * r + 5.0000001201E-1f;
+    y = y * r3 + y1;
+    y = y * r2 + y2;
+    const float out = ldexpf(y, (int)m);
+    if (!(x0 == x0)) return x0;
+    return out > x0 ? out : x0;
+}
+
+// sincos: the reference's cephes scheme, MorePacketMath.h:297-393 (float).
+__device__ __forceinline__ void psincos(float xin, float& s, float& c) {
+    uint32_t sign_sin = as_u(xin) & 0x80000000u;
+    float x = as_f(as_u(xin) & 0x7FFFFFFFu);
+    float y = x * ERD_F(1.27323954473516);
+    int32_t j = (int32_t)y;
+    j = (j + 1) & ~1;
+    y = (float)j;
+    const uint32_t swap_sin = (uint32_t)(j & 4) << 29;
+    const bool poly_mask = (j & 2) == 0;
+    x = x + y * ERD_F(-0.78515625);
+    x = x + y * ERD_F(-2.4187564849853515625e-4);
+    x = x + y * ERD_F(-3.77489497744594108e-8);
+    const uint32_t sign_cos = (uint32_t)(4 & ~(j - 2)) << 29;
+    sign_sin ^= swap_sin;
+    const float z = x * x;
+    float yc = ERD_F(2.443315711809948E-005) * z;
+    yc = yc + ERD_F(-1.388731625493765E-003);
+    yc = yc * z;
+    yc = yc + ERD_F(4.166664568298827E-002);
+    yc = yc * z;
+    yc = yc * z;
+    yc = yc - z * ERD_F(0.5);
+    yc = yc + 1.0f;
+    float ys = ERD_F(-1.9515295891E-4) * z;
+    ys = ys + ERD_F(8.3321608736E-3);
+    ys = ys * z;
+    ys = ys + ERD_F(-1.6666654611E-1);
+    ys = ys * z;
+    ys = ys * x;
+    ys = ys + x;
+    // The reference blends with and/andnot then adds; for finite polynomial
+    // values that equals a select with -0 mapped to +0 (adding a masked +0).
+    const float rs = (poly_mask ? ys : yc) + 0.0f;
+    const float rc = (poly_mask ? yc : ys) + 0.0f;
+    s = as_f(as_u(rs) ^ sign_sin);
+    c = as_f(as_u(rc) ^ sign_cos);
+}
+
+// lgamma approximation: MorePacketMath.h:158-170.
+__device__ __forceinline__ float plgamma_approx(float x) {
+    const float x_3 = x + 3.0f;
+    float ret = (x_3 + -0.5f) * plog(x_3);
+    ret = ret - x_3;
+    ret = ret + ERD_F(0.9189385332046727);
+    ret = ret + ERD_F(1 / 12.) / x_3;
+    ret = ret - plog(((x_3 - 1.0f) * (x_3 - 2.0f)) * x);
+    return ret;
+}
+
+__device__ __forceinline__ uint32_t lo32(uint64_t w) { return (uint32_t)w; }
+__device__ __forceinline__ uint32_t hi32(uint64_t w) { return (uint32_t)(w >> 32); }
+
+}  // namespace erd

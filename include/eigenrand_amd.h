@@ -1,0 +1,125 @@
+/*
+ * eigenrand_amd: C ABI of the MI355X (gfx950) implementation of EigenRand's
+ * sampling hot path.
+ *
+ * The reference exposes this path as C++ templates that Eigen's assignment
+ * loop drives packet by packet; this header is the flat, C-callable boundary a
+ * binding (ctypes, cgo, JNI, a C++ shim) would target instead. Every symbol is
+ * `extern "C"`, takes plain pointers and sizes, and returns an erd_status.
+ *
+ * Conventions
+ *  - An erd_engines handle is a set of N Vmt19937_64 engines resident in
+ *    device memory (reference: EigenRand/PacketRandomEngine.h:585-599, AVX2
+ *    layout: each engine = 4 MT19937-64 lanes, lane l seeded seed+l).
+ *  - A fill with `per_engine` outputs writes N contiguous blocks: engine g
+ *    fills out[g*per_engine, (g+1)*per_engine) in the reference's packet order,
+ *    exactly as `x = Rand::xxx<ArrayXf>(per_engine, 1, engine_g, ...)` would.
+ *    Each call behaves like a free function of the reference (a fresh
+ *    generator per engine block: queued rejection leftovers and the normal
+ *    generator's cached half-packet are dropped at the end of the block), and
+ *    advances the engines exactly as the reference advances them.
+ *  - Output/parameter pointers are DEVICE pointers; `stream` is a hipStream_t
+ *    (NULL = default stream). Calls are asynchronous on that stream.
+ *  - per_engine must be a multiple of the packet width: 8 for float fills
+ *    (Packet8f), 4 for int32/double/u64 fills. The reference's scalar tail
+ *    path for other sizes is not part of this boundary (ERD_ERR_INVALID).
+ */
+#ifndef EIGENRAND_AMD_H
+#define EIGENRAND_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum erd_status {
+    ERD_OK = 0,
+    ERD_ERR_INVALID = 1,     /* bad argument (size, pointer, parameter range)          */
+    ERD_ERR_NONFINITE = 2,   /* NaN/Inf weights (reference: Discrete.h:88 throws)       */
+    ERD_ERR_NOT_PSD = 3,     /* covariance not positive semi-definite (MvNormal.h:31-35) */
+    ERD_ERR_ITER_CAP = 4,    /* a rejection loop hit the device iteration cap           */
+    ERD_ERR_OOM = 5,         /* device allocation failed                                */
+    ERD_ERR_HIP = 6,         /* a HIP runtime call failed                               */
+    ERD_ERR_UNSUPPORTED = 7  /* parameter combination outside this boundary             */
+} erd_status;
+
+typedef struct erd_engines erd_engines;
+
+/* ---------------------------------------------------------------- misc --- */
+const char* erd_status_string(int status);
+size_t erd_engine_state_bytes(void); /* bytes of one engine record (10016) */
+
+/* ------------------------------------------------------------- engines --- */
+/* Vmt19937_64(seed) for each engine: MersenneTwister(uint64_t x0)
+ * (PacketRandomEngine.h:230-239). `seeds` is a HOST array of n_engines. */
+int erd_engines_create(erd_engines** out, const uint64_t* seeds, int64_t n_engines, void* stream);
+/* Engine g seeded seed0 + g*stride (stride 4 gives distinct lanes seed0.. ). */
+int erd_engines_create_range(erd_engines** out, uint64_t seed0, uint64_t stride, int64_t n_engines,
+                             void* stream);
+/* Non-owning handle on engines [first, first+n) of `parent`. */
+int erd_engines_view(erd_engines** out, erd_engines* parent, int64_t first, int64_t n);
+int erd_engines_destroy(erd_engines* e);
+int64_t erd_engines_count(const erd_engines* e);
+/* Device address of the engine records (n * erd_engine_state_bytes()). */
+void* erd_engines_device_ptr(erd_engines* e);
+/* Copy engine records to/from HOST memory (the copy semantics of
+ * AlignedArray / `auto copy = engine;`, PacketRandomEngine.h:89-118). */
+int erd_engines_snapshot(const erd_engines* e, void* host_dst, void* stream);
+int erd_engines_restore(erd_engines* e, const void* host_src, void* stream);
+/* dst engines := src engines (same count), device to device. */
+int erd_engines_copy(erd_engines* dst, const erd_engines* src, void* stream);
+/* MersenneTwister::discard(calls) (PacketRandomEngine.h:317-323) on every engine. */
+int erd_engines_discard(erd_engines* e, int64_t calls, void* stream);
+/* Read and clear the device status bits raised by kernels on this set (sync). */
+int erd_engines_status(erd_engines* e, uint32_t* status_out, void* stream);
+
+/* ---------------------------------------------------- raw engine draws --- */
+/* Vmt19937_64::operator() (PacketRandomEngine.h:295-310): out[4k+l] = lane l
+ * of draw k; per_engine = 4 * draws. */
+int erd_fill_u64(erd_engines* e, uint64_t* out, int64_t per_engine, void* stream);
+
+/* ---------------------------------------------------------- univariate --- */
+/* uniformReal<float>: StdUniformRealGen (Basic.h:584-590); with (a,b) it is
+ * UniformRealGen (Basic.h:626-634): u*(b-a)+a. Identical values for (0,1). */
+int erd_fill_uniform_f32(erd_engines* e, float* out, int64_t per_engine, float a, float b,
+                         void* stream);
+/* uniformReal<double> (arch/AVX/RandUtils.h:212-225): 4 doubles per draw. */
+int erd_fill_uniform_f64(erd_engines* e, double* out, int64_t per_engine, double a, double b,
+                         void* stream);
+/* normal<float>(rows, cols, urng): StdNormalGen (NormalExp.h:60-82). */
+int erd_fill_std_normal_f32(erd_engines* e, float* out, int64_t per_engine, void* stream);
+/* normal<float>(rows, cols, urng, mean, stdev): NormalGen (NormalExp.h:127-135). */
+int erd_fill_normal_f32(erd_engines* e, float* out, int64_t per_engine, float mean, float stdev,
+                        void* stream);
+/* normal(urng, mean_array, stdev_array): NormalVGen (NormalExp.h:138-163,
+ * free function 1294-1335). mean/stdev are device arrays of n*per_engine. */
+int erd_fill_normal_vop_f32(erd_engines* e, float* out, int64_t per_engine, const float* mean,
+                            const float* stdev, void* stream);
+/* exponential<float>(…, lambda): ExponentialGen (NormalExp.h:356-399). */
+int erd_fill_exponential_f32(erd_engines* e, float* out, int64_t per_engine, float lambda,
+                             void* stream);
+/* cauchy<float>(…, a, b): CauchyGen (NormalExp.h:809-856). */
+int erd_fill_cauchy_f32(erd_engines* e, float* out, int64_t per_engine, float a, float b,
+                        void* stream);
+/* studentT<float>(…, n): StudentTGen (NormalExp.h:241-302). */
+int erd_fill_studentt_f32(erd_engines* e, float* out, int64_t per_engine, float n, void* stream);
+/* gamma<float>(…, alpha, beta): GammaGen (NormalExp.h:434-606), all regimes. */
+int erd_fill_gamma_f32(erd_engines* e, float* out, int64_t per_engine, float alpha, float beta,
+                       void* stream);
+/* chiSquared<float>(…, n) = GammaGen(n*0.5, 2) (NormalExp.h:767-802). */
+int erd_fill_chisquared_f32(erd_engines* e, float* out, int64_t per_engine, float n, void* stream);
+
+/* -------------------------------------------------------- device memory --- */
+/* Small helpers so callers without another GPU runtime can use the library. */
+int erd_device_alloc(void** ptr, size_t bytes);
+int erd_device_free(void* ptr);
+int erd_memcpy_d2h(void* host_dst, const void* dev_src, size_t bytes, void* stream);
+int erd_memcpy_h2d(void* dev_dst, const void* host_src, size_t bytes, void* stream);
+int erd_stream_synchronize(void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EIGENRAND_AMD_H */

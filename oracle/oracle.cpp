@@ -1,0 +1,663 @@
+// Oracle (test infrastructure only): a plain C++ restatement of EigenRand's
+// packet algorithms for the sampling hot path, lane-emulated at the canonical
+// AVX2 layout (Packet8f = 8 float lanes fed by one Vmt19937_64 call of 4 u64
+// lanes; int32 distributions use Packet4i/Packet4f fed by half-calls).
+// Built with -ffp-contract=off so every float op rounds exactly as the
+// reference's unfused packet ops do.
+//
+// Only tests/, the smoke check and the CPU baseline leg of the benchmark may
+// load this library; it is never on the product path.
+#include <cstdint>
+#include <cstring>
+#include <cmath>
+#include <vector>
+#include <thread>
+#include <algorithm>
+#include <memory>
+#include <limits>
+#include "orc_math.h"
+
+using namespace orc;
+
+namespace {
+
+// ---------------------------------------------------------------- engine ---
+// MT19937-64 constants: EigenRand/PacketRandomEngine.h:421-426 (Pmt19937_64).
+constexpr int kN = 312, kM = 156;
+constexpr uint64_t kA = 0xb5026f5aa96619e9ull;
+constexpr uint64_t kUM = 0xFFFFFFFF80000000ull;  // (~0 << 31)
+constexpr uint64_t kLM = 0x7FFFFFFFull;
+constexpr uint64_t kF = 6364136223846793005ull;
+
+// Vmt19937_64 under AVX2 = Pmt19937_64<Packet8i>: 4 u64 lanes, lane l is an
+// independent mt19937_64 seeded x0+l (PacketRandomEngine.h:230-265).
+// Layout shared with the device engine state: word i of lane l at mt[i*4+l].
+struct Vmt {
+    uint64_t mt[kN * 4];
+    int32_t idx;          // next word; kN means "twist before the next draw"
+    int32_t half_valid;   // MersenneTwister::half() cache (PacketRandomEngine.h:325-336)
+    uint64_t half_cache[2];
+    uint64_t reserved;    // pads the record to a multiple of 16 bytes
+};
+static_assert(sizeof(Vmt) == 10016, "engine state layout");
+
+void vmt_seed(Vmt& e, uint64_t x0) {
+    // seed(): state[i] = f*(prev ^ prev>>62) + i (PacketRandomEngine.h:256-265,
+    // pmuluadd64 arch/AVX/MorePacketMath.h:684-694).
+    for (int l = 0; l < 4; ++l) {
+        uint64_t prev = x0 + (uint64_t)l;
+        e.mt[l] = prev;
+        for (int i = 1; i < kN; ++i) {
+            prev = (prev ^ (prev >> 62)) * kF + (uint64_t)i;
+            e.mt[i * 4 + l] = prev;
+        }
+    }
+    e.idx = kN;
+    e.half_valid = 0;
+    e.half_cache[0] = e.half_cache[1] = 0;
+    e.reserved = 0;
+}
+
+void vmt_twist(Vmt& e) {
+    // refill_upper/refill_lower (PacketRandomEngine.h:340-404) write the next
+    // 312 words of each lane; in single-buffer form that is the standard twist
+    // x[i] = x[i+m] ^ twist(x[i], x[i+1]), indices mod n, in increasing i.
+    // Written as the usual three loops (no modulo) with the 4 lanes innermost.
+    uint64_t* x = e.mt;
+    auto tw = [](uint64_t cur, uint64_t nxt, uint64_t far) {
+        uint64_t y = (cur & kUM) | (nxt & kLM);
+        return far ^ (y >> 1) ^ ((0 - (y & 1)) & kA);
+    };
+    int i = 0;
+    for (; i < kN - kM; ++i)
+        for (int l = 0; l < 4; ++l) x[i * 4 + l] = tw(x[i * 4 + l], x[(i + 1) * 4 + l], x[(i + kM) * 4 + l]);
+    for (; i < kN - 1; ++i)
+        for (int l = 0; l < 4; ++l) x[i * 4 + l] = tw(x[i * 4 + l], x[(i + 1) * 4 + l], x[(i + kM - kN) * 4 + l]);
+    for (int l = 0; l < 4; ++l) x[i * 4 + l] = tw(x[i * 4 + l], x[l], x[(kM - 1) * 4 + l]);
+}
+
+inline uint64_t temper(uint64_t y) {
+    // operator() tempering: PacketRandomEngine.h:303-308
+    y ^= (y >> 29) & 0x5555555555555555ull;
+    y ^= (y << 17) & 0x71d67fffeda60000ull;
+    y ^= (y << 37) & 0xfff7eee000000000ull;
+    y ^= y >> 43;
+    return y;
+}
+
+inline void vmt_draw(Vmt& e, uint64_t out[4]) {
+    if (e.idx >= kN) { vmt_twist(e); e.idx = 0; }
+    for (int l = 0; l < 4; ++l) out[l] = temper(e.mt[e.idx * 4 + l]);
+    e.idx++;
+}
+
+// half(): low 128 bits now, high 128 bits cached (PacketRandomEngine.h:325-336).
+inline void vmt_half(Vmt& e, uint64_t out[2]) {
+    if (e.half_valid) {
+        e.half_valid = 0;
+        out[0] = e.half_cache[0]; out[1] = e.half_cache[1];
+        return;
+    }
+    uint64_t p[4];
+    vmt_draw(e, p);
+    out[0] = p[0]; out[1] = p[1];
+    e.half_cache[0] = p[2]; e.half_cache[1] = p[3];
+    e.half_valid = 1;
+}
+
+// Packet8f uniform_real: rawbits_34 = one full call (arch/AVX/RandUtils.h:
+// 154-157, 206-209); float j = 32-bit word j = u64 lane j/2, low word first.
+inline void ur8(Vmt& e, float u[8]) {
+    uint64_t p[4];
+    vmt_draw(e, p);
+    for (int j = 0; j < 8; ++j) u[j] = bit_to_ur_float((uint32_t)(p[j >> 1] >> (32 * (j & 1))));
+}
+inline void nzur8(Vmt& e, float u[8]) {
+    ur8(e, u);
+    for (int j = 0; j < 8; ++j) u[j] = u[j] + 1.1920928955078125e-07f / 8;
+}
+// Packet4f uniform_real via half() (arch/AVX/RandUtils.h:21-38).
+inline void ur4(Vmt& e, float u[4]) {
+    uint64_t p[2];
+    vmt_half(e, p);
+    for (int j = 0; j < 4; ++j) u[j] = bit_to_ur_float((uint32_t)(p[j >> 1] >> (32 * (j & 1))));
+}
+
+// ---------------------------------------------------------- CompressMask ---
+// Lane-compaction FIFO of arch/AVX/PacketFilter.h:147-184 (W=8) and
+// arch/SSE/PacketFilter.h:118-150 (W=4): a fully accepted packet is emitted as
+// is and leaves the queue untouched; otherwise accepted lanes are appended in
+// lane order and the first W queued values are emitted once W are available.
+template <int W>
+bool compress_append(float value[W], const bool mask[W], float rest[W], int& rest_cnt) {
+    int cnt = 0;
+    for (int i = 0; i < W; ++i) cnt += mask[i];
+    if (cnt == W) return true;
+    float q[2 * W];
+    int n = 0;
+    for (int i = 0; i < rest_cnt; ++i) q[n++] = rest[i];
+    for (int i = 0; i < W; ++i) if (mask[i]) q[n++] = value[i];
+    if (n >= W) {
+        for (int i = 0; i < W; ++i) value[i] = q[i];
+        for (int i = W; i < n; ++i) rest[i - W] = q[i];
+        rest_cnt = n - W;
+        return true;
+    }
+    for (int i = 0; i < n; ++i) rest[i] = q[i];
+    rest_cnt = n;
+    return false;
+}
+
+constexpr float kPiF = (float)kPi;
+
+// ------------------------------------------------------------ generators ---
+// StdNormalGen::packetOp (EigenRand/Dists/NormalExp.h:60-82): Box-Muller with
+// the cos half cached for the next packet.
+struct StdNormal8 {
+    bool valid = false;
+    float cache[8];
+    void op(Vmt& e, float out[8]) {
+        if (valid) { valid = false; std::memcpy(out, cache, sizeof(cache)); return; }
+        valid = true;
+        float u1[8], u2[8];
+        ur8(e, u1);
+        ur8(e, u2);
+        for (int j = 0; j < 8; ++j) {
+            float a = 1.0f - u1[j];
+            float radius = std::sqrt(-2.0f * plog(a));
+            float theta = F(2 * kPi) * u2[j];
+            float s, c;
+            psincos(theta, s, c);
+            cache[j] = radius * c;
+            out[j] = radius * s;
+        }
+    }
+};
+
+// GammaGen::packetOp (NormalExp.h:527-605); ctor constants 453-458.
+struct Gamma8 {
+    float alpha, beta, px, sq;
+    float rest[8];
+    int rest_cnt = 0;
+    Gamma8(float a, float b) : alpha(a), beta(b) {
+        px = (float)(kE / (alpha + kE));
+        sq = std::sqrt(2 * alpha - 1);
+    }
+    void op(Vmt& e, float out[8]) {
+        if (alpha < 1) {
+            for (;;) {
+                float ux[8], vx[8], u3[8], cands[8];
+                bool acc[8];
+                ur8(e, ux);
+                nzur8(e, vx);
+                float xx[8], qx[8];
+                for (int j = 0; j < 8; ++j) {
+                    float lv = plog(vx[j]);
+                    float x1 = pexp((1 / alpha) * lv);
+                    float q1 = pexp(-x1);
+                    float x2 = 1.0f - plog(vx[j]);
+                    float q2 = pexp(plog(x2) * (alpha - 1));
+                    bool c = ux[j] < px;
+                    xx[j] = c ? x1 : x2;
+                    qx[j] = c ? q1 : q2;
+                }
+                ur8(e, u3);
+                for (int j = 0; j < 8; ++j) { cands[j] = beta * xx[j]; acc[j] = u3[j] < qx[j]; }
+                if (compress_append<8>(cands, acc, rest, rest_cnt)) { std::memcpy(out, cands, 32); return; }
+            }
+        }
+        if (alpha == 1) {
+            float u[8];
+            ur8(e, u);
+            for (int j = 0; j < 8; ++j) out[j] = beta * -(plog(1.0f - u[j]) / 1.0f);
+            return;
+        }
+        int count;
+        if ((count = (int)alpha) == alpha && count < 20) {
+            float yx[8], u[8];
+            nzur8(e, yx);
+            while (--count) {
+                nzur8(e, u);
+                for (int j = 0; j < 8; ++j) yx[j] = yx[j] * u[j];
+            }
+            for (int j = 0; j < 8; ++j) out[j] = -(beta * plog(yx[j]));
+            return;
+        }
+        for (;;) {
+            const float alpha_1 = alpha - 1;
+            float u1[8], u2[8], cands[8];
+            bool acc[8];
+            ur8(e, u1);
+            float yx[8], xx[8];
+            bool c[8];
+            for (int j = 0; j < 8; ++j) {
+                float ys, yc;
+                psincos(kPiF * u1[j], ys, yc);
+                yx[j] = ys / yc;
+                xx[j] = sq * yx[j] + alpha_1;
+                c[j] = 0.0f < xx[j];
+            }
+            ur8(e, u2);
+            for (int j = 0; j < 8; ++j) {
+                float ub = (yx[j] * yx[j] + 1.0f) *
+                           pexp(alpha_1 * plog(xx[j] / alpha_1) - yx[j] * sq);
+                acc[j] = c[j] && (u2[j] <= ub);
+                cands[j] = beta * xx[j];
+            }
+            if (compress_append<8>(cands, acc, rest, rest_cnt)) { std::memcpy(out, cands, 32); return; }
+        }
+    }
+};
+
+// PoissonGen (EigenRand/Dists/Discrete.h:788-904) at Packet4i/Packet4f.
+struct Poisson4 {
+    double mean, ne_mean, sqrt_tmean, log_mean, g1;
+    float rest[4];
+    int rest_cnt = 0;
+    explicit Poisson4(double m) : mean(m), ne_mean(std::exp(-m)) {
+        sqrt_tmean = std::sqrt(2 * mean);
+        log_mean = std::log(mean);
+        g1 = mean * log_mean - std::lgamma(mean + 1);
+    }
+    void op(Vmt& e, int32_t out[4]) {
+        if (mean < 12) {
+            int32_t res[4] = {0, 0, 0, 0};
+            float val[4] = {1, 1, 1, 1};
+            const float thr = (float)ne_mean;
+            for (;;) {
+                float u[4];
+                ur4(e, u);
+                bool any = false;
+                bool c[4];
+                for (int j = 0; j < 4; ++j) { val[j] = val[j] * u[j]; c[j] = thr < val[j]; any |= c[j]; }
+                if (!any) break;
+                for (int j = 0; j < 4; ++j) res[j] += c[j] ? 1 : 0;
+            }
+            std::memcpy(out, res, 16);
+            return;
+        }
+        const float psq = (float)sqrt_tmean, pmean = (float)mean, plm = (float)log_mean, pg1 = (float)g1;
+        for (;;) {
+            float u1[4], u2[4], fres[4], p12[4];
+            bool c1[4], acc[4];
+            ur4(e, u1);
+            for (int j = 0; j < 4; ++j) {
+                float s, c;
+                psincos(kPiF * u1[j], s, c);
+                float yx = s / c;
+                fres[j] = ptruncate(psq * yx + pmean);
+                float p1 = (yx * yx + 1.0f) * F(0.9);
+                float p2 = pexp((fres[j] * plm - plgamma_approx(fres[j] + 1.0f)) - pg1);
+                c1[j] = 0.0f <= fres[j];
+                p12[j] = p1 * p2;
+            }
+            ur4(e, u2);
+            for (int j = 0; j < 4; ++j) acc[j] = c1[j] && (u2[j] <= p12[j]);
+            if (compress_append<4>(fres, acc, rest, rest_cnt)) {
+                for (int j = 0; j < 4; ++j) out[j] = (int32_t)fres[j];
+                return;
+            }
+        }
+    }
+};
+
+// BinomialGen (Discrete.h:911-1053) at Packet4i/Packet4f; the rejection queue
+// is the one owned by its PoissonGen member.
+struct Binomial4 {
+    Poisson4 poisson;
+    int32_t trials;
+    double p, small_p, g1 = 0, sqrt_v = 0, log_small_p = 0, log_small_q = 0;
+    Binomial4(int32_t n, double pp)
+        : poisson(n * std::min(pp, 1 - pp)), trials(n), p(pp), small_p(std::min(pp, 1 - pp)) {
+        if (!(trials < 25 || poisson.mean < 1.0)) {
+            g1 = std::lgamma(trials + 1);
+            sqrt_v = std::sqrt(2 * poisson.mean * (1 - small_p));
+            log_small_p = std::log(small_p);
+            log_small_q = std::log(1 - small_p);
+        }
+    }
+    void op(Vmt& e, int32_t out[4]) {
+        int32_t res[4];
+        if (trials < 25) {
+            const float pp = (float)p;
+            for (int j = 0; j < 4; ++j) res[j] = trials;
+            for (int i = 0; i < trials; ++i) {
+                float u[4];
+                ur4(e, u);
+                for (int j = 0; j < 4; ++j) res[j] += (pp <= u[j]) ? -1 : 0;
+            }
+            std::memcpy(out, res, 16);
+            return;
+        } else if (poisson.mean < 1.0) {
+            poisson.op(e, res);
+        } else {
+            const float ptrials = (float)trials, psqrt_v = (float)sqrt_v, pmean = (float)poisson.mean,
+                        plp = (float)log_small_p, plq = (float)log_small_q, pg1 = (float)g1;
+            for (;;) {
+                float u1[4], u2[4], fres[4], p12[4];
+                bool c1[4], acc[4];
+                ur4(e, u1);
+                for (int j = 0; j < 4; ++j) {
+                    float s, c;
+                    psincos(kPiF * u1[j], s, c);
+                    float ys = s / c;
+                    fres[j] = ptruncate(psqrt_v * ys + pmean);
+                    float p1 = (F(1.2) * psqrt_v) * (1.0f + ys * ys);
+                    float p2 = pexp((((pg1 - plgamma_approx(fres[j] + 1.0f)) -
+                                      plgamma_approx((ptrials + 1.0f) - fres[j])) +
+                                     fres[j] * plp) +
+                                    (ptrials - fres[j]) * plq);
+                    c1[j] = (0.0f <= fres[j]) && (fres[j] <= ptrials);
+                    p12[j] = p1 * p2;
+                }
+                ur4(e, u2);
+                for (int j = 0; j < 4; ++j) acc[j] = c1[j] && (u2[j] <= p12[j]);
+                if (compress_append<4>(fres, acc, poisson.rest, poisson.rest_cnt)) {
+                    for (int j = 0; j < 4; ++j) res[j] = (int32_t)fres[j];
+                    break;
+                }
+            }
+        }
+        for (int j = 0; j < 4; ++j) out[j] = (p == small_p) ? res[j] : trials - res[j];
+    }
+};
+
+// BinomialVGen<int32_t>::packetOp (Discrete.h:1124-1192): per-lane float
+// parameters, lockstep Knuth loop for lanes with mean < 1, then lockstep
+// tangent rejection keeping each lane's first accepted candidate.
+void binomialv4(Vmt& e, const int32_t trials[4], const float p[4], int32_t out[4]) {
+    float psmall[4], ptr[4], pmean[4], pg1[4], psq[4], plp[4], plq[4];
+    bool valid[4];
+    float valid_res[4] = {0, 0, 0, 0};
+    for (int j = 0; j < 4; ++j) {
+        psmall[j] = (p[j] < 1.0f - p[j]) ? p[j] : 1.0f - p[j];   // pmin(p, 1-p)
+        ptr[j] = (float)trials[j];
+        pmean[j] = ptr[j] * psmall[j];
+        pg1[j] = plgamma_approx(ptr[j] + 1.0f);
+        psq[j] = std::sqrt((2.0f * pmean[j]) * (1.0f - psmall[j]));
+        plp[j] = plog(psmall[j]);
+        plq[j] = plog(1.0f - psmall[j]);
+        valid[j] = pmean[j] < 1.0f;
+    }
+    bool any_valid = valid[0] || valid[1] || valid[2] || valid[3];
+    if (any_valid) {
+        int32_t res[4] = {0, 0, 0, 0};
+        float val[4] = {1, 1, 1, 1}, nem[4];
+        for (int j = 0; j < 4; ++j) nem[j] = pexp(-pmean[j]);
+        for (;;) {
+            float u[4];
+            ur4(e, u);
+            bool any = false, c[4];
+            for (int j = 0; j < 4; ++j) {
+                val[j] = val[j] * u[j];
+                c[j] = (nem[j] < val[j]) && valid[j];
+                any |= c[j];
+            }
+            if (!any) break;
+            for (int j = 0; j < 4; ++j) res[j] += c[j] ? 1 : 0;
+        }
+        for (int j = 0; j < 4; ++j) valid_res[j] = (float)res[j];
+    }
+    bool all_valid = valid[0] && valid[1] && valid[2] && valid[3];
+    if (!all_valid) {
+        for (;;) {
+            float u1[4], u2[4], fres[4], p12[4];
+            bool c1[4];
+            ur4(e, u1);
+            for (int j = 0; j < 4; ++j) {
+                float s, c;
+                psincos(kPiF * u1[j], s, c);
+                float ys = s / c;
+                fres[j] = ptruncate(psq[j] * ys + pmean[j]);
+                float pp1 = (F(1.2) * psq[j]) * (1.0f + ys * ys);
+                float pp2 = pexp((((pg1[j] - plgamma_approx(fres[j] + 1.0f)) -
+                                   plgamma_approx((ptr[j] + 1.0f) - fres[j])) +
+                                  fres[j] * plp[j]) +
+                                 (ptr[j] - fres[j]) * plq[j]);
+                c1[j] = (0.0f <= fres[j]) && (fres[j] <= ptr[j]);
+                p12[j] = pp1 * pp2;
+            }
+            ur4(e, u2);
+            bool all = true;
+            for (int j = 0; j < 4; ++j) {
+                bool fv = c1[j] && (u2[j] <= p12[j]);
+                if (fv && !valid[j]) valid_res[j] = fres[j];
+                valid[j] = valid[j] || fv;
+                all = all && valid[j];
+            }
+            if (all) break;
+        }
+    }
+    for (int j = 0; j < 4; ++j) {
+        int32_t k = (int32_t)valid_res[j];
+        out[j] = (p[j] == psmall[j]) ? k : trials[j] - k;
+    }
+}
+
+// --------------------------------------------------------------- helpers ---
+template <class Fn>
+void for_engines(int64_t n_engines, int threads, Fn fn) {
+    if (threads <= 1 || n_engines <= 1) {
+        for (int64_t g = 0; g < n_engines; ++g) fn(g);
+        return;
+    }
+    std::vector<std::thread> ts;
+    for (int t = 0; t < threads; ++t) {
+        ts.emplace_back([=, &fn]() {
+            for (int64_t g = t; g < n_engines; g += threads) fn(g);
+        });
+    }
+    for (auto& th : ts) th.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_state_bytes() { return (int)sizeof(Vmt); }
+
+void orc_seed(void* states, const uint64_t* seeds, int64_t n_engines) {
+    Vmt* e = (Vmt*)states;
+    for (int64_t g = 0; g < n_engines; ++g) vmt_seed(e[g], seeds[g]);
+}
+
+// Raw u64 draws: out[g*calls*4 + 4k + l] = lane l of call k of engine g.
+void orc_fill_u64(void* states, int64_t n_engines, uint64_t* out, int64_t calls, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        uint64_t* o = out + g * calls * 4;
+        for (int64_t k = 0; k < calls; ++k) vmt_draw(e[g], o + 4 * k);
+    });
+}
+
+// P8_mt19937_64 / ParallelRandomEngineAdaptor<uint64_t, Vmt19937_64, 8>
+// (PacketRandomEngine.h:429-517, 600-609): two engines seeded s and s+4,
+// buffer = [engine0 call, engine1 call]; out[8k + 4i + l].
+void orc_fill_p8_u64(void* two_states, uint64_t* out, int64_t n_refills) {
+    Vmt* e = (Vmt*)two_states;
+    for (int64_t k = 0; k < n_refills; ++k) {
+        vmt_draw(e[0], out + 8 * k);
+        vmt_draw(e[1], out + 8 * k + 4);
+    }
+}
+
+// half() draws: out[g*halves*2 + 2h + {0,1}].
+void orc_fill_half_u64(void* states, int64_t n_engines, uint64_t* out, int64_t halves) {
+    Vmt* e = (Vmt*)states;
+    for (int64_t g = 0; g < n_engines; ++g)
+        for (int64_t h = 0; h < halves; ++h) vmt_half(e[g], out + (g * halves + h) * 2);
+}
+
+// uniformReal: StdUniformRealGen (Basic.h:584-590) when (a,b)=(0,1);
+// UniformRealGen (Basic.h:626-634) otherwise: u*(b-a)+a.
+void orc_fill_uniform_f32(void* states, int64_t n_engines, float* out, int64_t per_engine,
+                          float a, float b, int affine, int threads) {
+    Vmt* e = (Vmt*)states;
+    const float slope = b - a;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        float* o = out + g * per_engine;
+        float u[8];
+        for (int64_t i = 0; i < per_engine; i += 8) {
+            ur8(e[g], u);
+            for (int j = 0; j < 8; ++j) o[i + j] = affine ? u[j] * slope + a : u[j];
+        }
+    });
+}
+
+// uniformReal<double>: UniformRealUtils<Packet4d>::uniform_real
+// (arch/AVX/RandUtils.h:212-225) = bit_to_ur_double of one full call.
+void orc_fill_uniform_f64(void* states, int64_t n_engines, double* out, int64_t per_engine,
+                          double a, double b, int affine, int threads) {
+    Vmt* e = (Vmt*)states;
+    const double slope = b - a;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        double* o = out + g * per_engine;
+        uint64_t p[4];
+        for (int64_t i = 0; i < per_engine; i += 4) {
+            vmt_draw(e[g], p);
+            for (int j = 0; j < 4; ++j) {
+                double u = bit_to_ur_double(p[j]);
+                o[i + j] = affine ? u * slope + a : u;
+            }
+        }
+    });
+}
+
+// normal: StdNormalGen when affine=0, NormalGen z*sd+mean (NormalExp.h:
+// 127-135) when affine=1. A fresh generator per engine block.
+void orc_fill_normal_f32(void* states, int64_t n_engines, float* out, int64_t per_engine,
+                         float mean, float sd, int affine, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        float* o = out + g * per_engine;
+        StdNormal8 gen;
+        for (int64_t i = 0; i < per_engine; i += 8) {
+            gen.op(e[g], o + i);
+            if (affine) for (int j = 0; j < 8; ++j) o[i + j] = o[i + j] * sd + mean;
+        }
+    });
+}
+
+// NormalVGen (NormalExp.h:138-163): z*sd[i]+mean[i].
+void orc_fill_normal_vop_f32(void* states, int64_t n_engines, float* out, int64_t per_engine,
+                             const float* mean, const float* sd, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        float* o = out + g * per_engine;
+        const float* m = mean + g * per_engine;
+        const float* s = sd + g * per_engine;
+        StdNormal8 gen;
+        for (int64_t i = 0; i < per_engine; i += 8) {
+            gen.op(e[g], o + i);
+            for (int j = 0; j < 8; ++j) o[i + j] = o[i + j] * s[i + j] + m[i + j];
+        }
+    });
+}
+
+// ExponentialGen (NormalExp.h:391-398): -(log(1-u)/lambda); pnegate flips the sign bit.
+void orc_fill_exponential_f32(void* states, int64_t n_engines, float* out, int64_t per_engine,
+                              float lambda, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        float* o = out + g * per_engine;
+        float u[8];
+        for (int64_t i = 0; i < per_engine; i += 8) {
+            ur8(e[g], u);
+            for (int j = 0; j < 8; ++j) o[i + j] = -(plog(1.0f - u[j]) / lambda);
+        }
+    });
+}
+
+// CauchyGen (NormalExp.h:843-855): a + b*tan(pi*(u-0.5)) via psincos.
+void orc_fill_cauchy_f32(void* states, int64_t n_engines, float* out, int64_t per_engine,
+                         float a, float b, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        float* o = out + g * per_engine;
+        float u[8];
+        for (int64_t i = 0; i < per_engine; i += 8) {
+            ur8(e[g], u);
+            for (int j = 0; j < 8; ++j) {
+                float s, c;
+                psincos(kPiF * (u[j] - F(0.5)), s, c);
+                o[i + j] = a + b * (s / c);
+            }
+        }
+    });
+}
+
+// StudentTGen (NormalExp.h:285-301). The reference calls Eigen's psin; the
+// sine of the reference's own psincos stands in for it here and on the GPU.
+void orc_fill_studentt_f32(void* states, int64_t n_engines, float* out, int64_t per_engine,
+                           float n, int threads) {
+    Vmt* e = (Vmt*)states;
+    const float k = -2 / n;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        float* o = out + g * per_engine;
+        float u1[8], u2[8];
+        for (int64_t i = 0; i < per_engine; i += 8) {
+            ur8(e[g], u1);
+            ur8(e[g], u2);
+            for (int j = 0; j < 8; ++j) {
+                float a = 1.0f - u1[j];
+                float radius = std::sqrt(n * (pexp(plog(a) * k) - 1.0f));
+                float theta = F(2 * kPi) * u2[j];
+                float s, c;
+                psincos(theta, s, c);
+                o[i + j] = radius * s;
+            }
+        }
+    });
+}
+
+// GammaGen; ChiSquaredGen(n) = GammaGen(n*0.5, 2) (NormalExp.h:781-784).
+void orc_fill_gamma_f32(void* states, int64_t n_engines, float* out, int64_t per_engine,
+                        float alpha, float beta, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        float* o = out + g * per_engine;
+        Gamma8 gen(alpha, beta);
+        for (int64_t i = 0; i < per_engine; i += 8) gen.op(e[g], o + i);
+    });
+}
+
+void orc_fill_poisson_i32(void* states, int64_t n_engines, int32_t* out, int64_t per_engine,
+                          double mean, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        int32_t* o = out + g * per_engine;
+        Poisson4 gen(mean);
+        for (int64_t i = 0; i < per_engine; i += 4) gen.op(e[g], o + i);
+    });
+}
+
+void orc_fill_binomial_i32(void* states, int64_t n_engines, int32_t* out, int64_t per_engine,
+                           int32_t trials, double p, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        int32_t* o = out + g * per_engine;
+        Binomial4 gen(trials, p);
+        for (int64_t i = 0; i < per_engine; i += 4) gen.op(e[g], o + i);
+    });
+}
+
+void orc_fill_binomial_vop_i32(void* states, int64_t n_engines, int32_t* out, int64_t per_engine,
+                               const int32_t* trials, const float* p, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        int64_t base = g * per_engine;
+        for (int64_t i = 0; i < per_engine; i += 4)
+            binomialv4(e[g], trials + base + i, p + base + i, out + base + i);
+    });
+}
+
+// Elementwise access to the packet math, for pinning it against libm.
+void orc_plog(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = plog(x[i]); }
+void orc_pexp(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = pexp(x[i]); }
+void orc_psincos(const float* x, float* s, float* c, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) psincos(x[i], s[i], c[i]);
+}
+void orc_plgamma(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = plgamma_approx(x[i]); }
+void orc_bit_to_ur_float(const uint32_t* w, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = bit_to_ur_float(w[i]); }
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+"""numpy/ctypes front end of the oracle library (test infrastructure only).
+
+Only tests/, `__graft_entry__.smoke()` and the CPU-baseline leg of bench.py
+import this module; the product path never does.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+STATE_BYTES = 10016
+
+_lib = None
+
+P = C.c_void_p
+I64 = C.c_int64
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (make -C oracle)")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.orc_state_bytes.restype = C.c_int
+        assert _lib.orc_state_bytes() == STATE_BYTES
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Engines:
+    """Host copy of n Vmt19937_64 engine records (same layout as the device)."""
+
+    def __init__(self, seeds=None, n=None, seed0=0, stride=4, raw=None):
+        if raw is not None:
+            self.raw = np.array(raw, dtype=np.uint8, copy=True)
+            return
+        if seeds is None:
+            seeds = [seed0 + g * stride for g in range(n)]
+        seeds = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
+        self.raw = np.zeros(seeds.size * STATE_BYTES, dtype=np.uint8)
+        lib().orc_seed(_p(self.raw), _p(seeds), I64(seeds.size))
+
+    def __len__(self):
+        return self.raw.size // STATE_BYTES
+
+    def copy(self):
+        return Engines(raw=self.raw)
+
+    def subset(self, first, n):
+        return Engines(raw=self.raw[first * STATE_BYTES:(first + n) * STATE_BYTES])
+
+    def records(self):
+        """Structured view: mt (n,312,4) u64, idx, half_valid, half_cache (n,2)."""
+        dt = np.dtype([("mt", "<u8", (312, 4)), ("idx", "<i4"), ("half_valid", "<i4"),
+                       ("half_cache", "<u8", (2,)), ("reserved", "<u8")])
+        return self.raw.view(dt)
+
+
+def fill_u64(e, draws, threads=1):
+    out = np.empty(len(e) * draws * 4, dtype=np.uint64)
+    lib().orc_fill_u64(_p(e.raw), I64(len(e)), _p(out), I64(draws), C.c_int(threads))
+    return out
+
+
+def fill_half_u64(e, halves):
+    out = np.empty(len(e) * halves * 2, dtype=np.uint64)
+    lib().orc_fill_half_u64(_p(e.raw), I64(len(e)), _p(out), I64(halves))
+    return out
+
+
+def fill_p8_u64(e2, refills):
+    assert len(e2) == 2
+    out = np.empty(refills * 8, dtype=np.uint64)
+    lib().orc_fill_p8_u64(_p(e2.raw), _p(out), I64(refills))
+    return out
+
+
+def _f32(e, per, name, *args, threads=1):
+    out = np.empty(len(e) * per, dtype=np.float32)
+    getattr(lib(), name)(_p(e.raw), I64(len(e)), _p(out), I64(per), *args, C.c_int(threads))
+    return out
+
+
+def fill_uniform_f32(e, per, a=0.0, b=1.0, threads=1):
+    affine = int(not (a == 0.0 and b == 1.0))
+    return _f32(e, per, "orc_fill_uniform_f32", C.c_float(a), C.c_float(b), C.c_int(affine),
+                threads=threads)
+
+
+def fill_uniform_f64(e, per, a=0.0, b=1.0, threads=1):
+    out = np.empty(len(e) * per, dtype=np.float64)
+    affine = int(not (a == 0.0 and b == 1.0))
+    lib().orc_fill_uniform_f64(_p(e.raw), I64(len(e)), _p(out), I64(per), C.c_double(a),
+                               C.c_double(b), C.c_int(affine), C.c_int(threads))
+    return out
+
+
+def fill_normal_f32(e, per, mean=None, sd=None, threads=1):
+    affine = int(mean is not None or sd is not None)
+    return _f32(e, per, "orc_fill_normal_f32", C.c_float(0.0 if mean is None else mean),
+                C.c_float(1.0 if sd is None else sd), C.c_int(affine), threads=threads)
+
+
+def fill_normal_vop_f32(e, mean, sd, threads=1):
+    mean = np.ascontiguousarray(mean, dtype=np.float32)
+    sd = np.ascontiguousarray(sd, dtype=np.float32)
+    per = mean.size // len(e)
+    out = np.empty(mean.size, dtype=np.float32)
+    lib().orc_fill_normal_vop_f32(_p(e.raw), I64(len(e)), _p(out), I64(per), _p(mean), _p(sd),
+                                  C.c_int(threads))
+    return out
+
+
+def fill_exponential_f32(e, per, lmbda=1.0, threads=1):
+    return _f32(e, per, "orc_fill_exponential_f32", C.c_float(lmbda), threads=threads)
+
+
+def fill_cauchy_f32(e, per, a=0.0, b=1.0, threads=1):
+    return _f32(e, per, "orc_fill_cauchy_f32", C.c_float(a), C.c_float(b), threads=threads)
+
+
+def fill_studentt_f32(e, per, n=1.0, threads=1):
+    return _f32(e, per, "orc_fill_studentt_f32", C.c_float(n), threads=threads)
+
+
+def fill_gamma_f32(e, per, alpha=1.0, beta=1.0, threads=1):
+    return _f32(e, per, "orc_fill_gamma_f32", C.c_float(alpha), C.c_float(beta), threads=threads)
+
+
+def fill_poisson_i32(e, per, mean, threads=1):
+    out = np.empty(len(e) * per, dtype=np.int32)
+    lib().orc_fill_poisson_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), C.c_double(mean),
+                               C.c_int(threads))
+    return out
+
+
+def fill_binomial_i32(e, per, trials, p, threads=1):
+    out = np.empty(len(e) * per, dtype=np.int32)
+    lib().orc_fill_binomial_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), C.c_int32(trials),
+                                C.c_double(p), C.c_int(threads))
+    return out
+
+
+def fill_binomial_vop_i32(e, trials, p, threads=1):
+    trials = np.ascontiguousarray(trials, dtype=np.int32)
+    p = np.ascontiguousarray(p, dtype=np.float32)
+    per = trials.size // len(e)
+    out = np.empty(trials.size, dtype=np.int32)
+    lib().orc_fill_binomial_vop_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), _p(trials), _p(p),
+                                    C.c_int(threads))
+    return out
+
+
+def _elementwise(name, x, nout=1):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    outs = [np.empty_like(x) for _ in range(nout)]
+    getattr(lib(), name)(_p(x), *[_p(o) for o in outs], I64(x.size))
+    return outs[0] if nout == 1 else tuple(outs)
+
+
+def plog(x):
+    return _elementwise("orc_plog", x)
+
+
+def pexp(x):
+    return _elementwise("orc_pexp", x)
+
+
+def psincos(x):
+    return _elementwise("orc_psincos", x, 2)
+
+
+def plgamma(x):
+    return _elementwise("orc_plgamma", x)
+
+
+def bit_to_ur_float(w):
+    w = np.ascontiguousarray(w, dtype=np.uint32)
+    out = np.empty(w.size, dtype=np.float32)
+    lib().orc_bit_to_ur_float(_p(w), _p(out), I64(w.size))
+    return out

@@ -1,0 +1,174 @@
+"""GPU parity of the engine and the fixed-consumption fills against the oracle.
+
+Integer/bit outputs must be identical; float fills are compared bit for bit
+too (the kernels follow the oracle's op sequence exactly).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(x):
+    x = np.asarray(x)
+    return x.view(np.uint32) if x.dtype == np.float32 else x.view(np.uint64)
+
+
+def assert_same(got, exp):
+    got = np.asarray(got).ravel()
+    exp = np.asarray(exp).ravel()
+    assert got.shape == exp.shape
+    bad = np.nonzero(bits(got) != bits(exp))[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first at {bad[:5]}: {got[bad[:5]]} vs {exp[bad[:5]]}"
+
+
+def flat(t):
+    """Column-major matrix view (or flat buffer) back to the linear fill order."""
+    if t.dim() == 2:
+        return t.t().contiguous().view(-1).cpu().numpy()
+    if t.dim() == 3:
+        return t.transpose(1, 2).contiguous().view(-1).cpu().numpy()
+    return t.contiguous().view(-1).cpu().numpy()
+
+
+def test_raw_stream_matches_std(er, mt_golden):
+    urng = er.Vmt19937_64(42)
+    d = er.randbits_u64(1000, urng).cpu().numpy().reshape(1000, 4)
+    for lane in range(4):
+        np.testing.assert_array_equal(d[:, lane], mt_golden["streams"][42 + lane])
+
+
+def test_raw_stream_known_answer(er, mt_golden):
+    urng = er.Vmt19937_64()  # default seed 5489
+    d = er.randbits_u64(10000, urng).cpu().numpy().reshape(10000, 4)
+    assert int(d[9999, 0]) == 9981545732273789042
+
+
+def test_engine_records_match_oracle(er, orc):
+    urng = er.Vmt19937_64(7, n=3, stride=5)
+    ref = orc.Engines(seeds=[7, 12, 17])
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+    er.randbits_u64(700, urng)
+    orc.fill_u64(ref, 700)
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+
+
+def test_config1_sequence(er, orc):
+    """Vmt19937_64(42): 2^20 raw u64, then uniformReal and normal 1024x1024."""
+    urng = er.Vmt19937_64(42)
+    ref = orc.Engines(seeds=[42])
+    raw = er.randbits_u64(1 << 18, urng).cpu().numpy()
+    assert_same(raw, orc.fill_u64(ref, 1 << 18))
+    u = er.uniform_real(1024, 1024, urng)
+    assert_same(flat(u), orc.fill_uniform_f32(ref, 1 << 20))
+    z = er.normal(1024, 1024, urng)
+    assert_same(flat(z), orc.fill_normal_f32(ref, 1 << 20))
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+
+
+CASES = [
+    ("uniform", lambda er, n, g: er.uniform_real(n, 1, g), lambda o, e, n: o.fill_uniform_f32(e, n)),
+    ("uniform_ab", lambda er, n, g: er.uniform_real(n, 1, g, -3.0, 5.5),
+     lambda o, e, n: o.fill_uniform_f32(e, n, -3.0, 5.5)),
+    ("uniform_f64", lambda er, n, g: er.uniform_real_f64(n, 1, g), lambda o, e, n: o.fill_uniform_f64(e, n)),
+    ("std_normal", lambda er, n, g: er.normal(n, 1, g), lambda o, e, n: o.fill_normal_f32(e, n)),
+    ("normal", lambda er, n, g: er.normal(n, 1, g, 2.0, 3.0), lambda o, e, n: o.fill_normal_f32(e, n, 2.0, 3.0)),
+    ("exponential", lambda er, n, g: er.exponential(n, 1, g, 1.5),
+     lambda o, e, n: o.fill_exponential_f32(e, n, 1.5)),
+    ("cauchy", lambda er, n, g: er.cauchy(n, 1, g, 0.5, 2.0), lambda o, e, n: o.fill_cauchy_f32(e, n, 0.5, 2.0)),
+    ("student_t", lambda er, n, g: er.student_t(n, 1, g, 5.0), lambda o, e, n: o.fill_studentt_f32(e, n, 5.0)),
+    ("gamma_1", lambda er, n, g: er.gamma(n, 1, g, 1.0, 2.0), lambda o, e, n: o.fill_gamma_f32(e, n, 1.0, 2.0)),
+    ("gamma_3", lambda er, n, g: er.gamma(n, 1, g, 3.0, 1.5), lambda o, e, n: o.fill_gamma_f32(e, n, 3.0, 1.5)),
+    ("gamma_19", lambda er, n, g: er.gamma(n, 1, g, 19.0, 1.0), lambda o, e, n: o.fill_gamma_f32(e, n, 19.0, 1.0)),
+    ("chi_squared_4", lambda er, n, g: er.chi_squared(n, 1, g, 4.0), lambda o, e, n: o.fill_gamma_f32(e, n, 2.0, 2.0)),
+]
+
+
+@pytest.mark.parametrize("name,gpu,cpu", CASES, ids=[c[0] for c in CASES])
+def test_fill_matches_oracle(er, orc, name, gpu, cpu):
+    # 5 engines; sizes chosen so that engines start fills at odd block offsets
+    # and normal fills end on an unused cos half-packet (8 mod 16).
+    seeds = [3, 1000, 2 ** 40 + 5, 16380, 77]
+    urng = er.Vmt19937_64(seeds=seeds)
+    ref = orc.Engines(seeds=seeds)
+    for n in (8, 24, 312 * 8 + 8, 4000, 8):
+        if name == "uniform_f64":
+            n = n // 2
+        out = gpu(er, n, urng)
+        exp = cpu(orc, ref, n)
+        assert_same(flat(out), exp)
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+
+
+def test_normal_vop_matches_oracle(er, orc):
+    import torch
+    seeds = [11, 12, 13]
+    per = 3000 * 8
+    rng = np.random.default_rng(5)
+    mean = rng.uniform(-10, 10, 3 * per).astype(np.float32)
+    sd = rng.uniform(0.1, 5, 3 * per).astype(np.float32)
+    urng = er.Vmt19937_64(seeds=seeds)
+    ref = orc.Engines(seeds=seeds)
+    er.randbits_u64(3, urng)
+    orc.fill_u64(ref, 3)
+    out = er.normal_vop(urng, torch.from_numpy(mean).cuda(), torch.from_numpy(sd).cuda())
+    assert_same(out.cpu().numpy(), orc.fill_normal_vop_f32(ref, mean, sd))
+
+
+def test_snapshot_restore_copy_discard(er, orc):
+    urng = er.Vmt19937_64(99, n=4)
+    er.randbits_u64(100, urng)
+    snap = urng.snapshot()
+    a = er.normal(512, 1, urng)
+    urng.restore(snap)
+    b = er.normal(512, 1, urng)
+    assert_same(flat(a), flat(b))
+    c = urng.copy()
+    x = er.uniform_real(256, 1, urng)
+    y = er.uniform_real(256, 1, c)
+    assert_same(flat(x), flat(y))
+    # discard(n) == drawing n packets
+    d = urng.copy()
+    urng.discard(1000)
+    er.randbits_u64(1000, d)
+    np.testing.assert_array_equal(urng.snapshot(), d.snapshot())
+
+
+def test_views_fill_their_engines(er, orc):
+    urng = er.Vmt19937_64(0, n=6)
+    ref = orc.Engines(n=6, seed0=0)
+    v = urng.view(2, 3)
+    out = er.uniform_real(64, 1, v)
+    exp = orc.fill_uniform_f32(ref.subset(2, 3), 64)
+    assert_same(flat(out), exp)
+
+
+def test_invalid_arguments_raise(er):
+    urng = er.Vmt19937_64(1)
+    with pytest.raises(er.InvalidArgument):
+        er.normal(7, 1, urng)  # not a multiple of the 8-float packet
+    with pytest.raises(er.InvalidArgument):
+        er.gamma(8, 1, urng, -1.0, 1.0)
+
+
+def test_full_size_normal_sampled_parity_and_moments(er, orc):
+    """2^28 normals from 4096 engines seeded 0,4,...,16380 (16384 MT lanes
+    seeded 0..16383): four engine blocks bit-exact vs the oracle, and the
+    whole output's mean/variance/kurtosis within sampling error."""
+    import torch
+    n_eng, per = 4096, 65536
+    urng = er.Vmt19937_64(0, n=n_eng, stride=4)
+    out = er.normal(per, 1, urng)  # (n, per, 1)
+    out = out.reshape(n_eng, per)
+    for g in (0, 1, 2047, 4095):
+        ref = orc.Engines(seeds=[4 * g])
+        assert_same(out[g].cpu().numpy(), orc.fill_normal_f32(ref, per))
+    x = out.reshape(-1).double()
+    m = x.mean().item()
+    v = x.var().item()
+    k = ((x - m) ** 4).mean().item() / v ** 2
+    n = x.numel()
+    assert abs(m) < 5 / n ** 0.5
+    assert abs(v - 1) < 5 * (2 / n) ** 0.5
+    assert abs(k - 3) < 5 * (24 / n) ** 0.5
+    assert torch.isfinite(out).all()
