@@ -214,14 +214,25 @@ __global__ __launch_bounds__(64) void k_fill_fixed(EngineState* __restrict__ sta
             break;
         }
         // Keep the words of the unit that straddles the boundary.
+        // (Up to (C-1)*4 words, i.e. KEEP per thread; read all, then write.)
+        constexpr int KEEP = ((C > 1 ? C - 1 : 1) * kLanes + 63) / 64;
         const int left = avail - consumed;
-        uint64_t keep = 0;
-        if (lane < left * kLanes) {
-            const int v = consumed + (lane >> 2), l = lane & 3;
-            keep = (v < nc) ? carry[v * kLanes + l] : mt[(pos + v - nc) * kLanes + l];
+        uint64_t keep[KEEP];
+#pragma unroll
+        for (int k = 0; k < KEEP; ++k) {
+            const int i = lane + 64 * k;
+            keep[k] = 0;
+            if (i < left * kLanes) {
+                const int v = consumed + (i >> 2), l = i & 3;
+                keep[k] = (v < nc) ? carry[v * kLanes + l] : mt[(pos + v - nc) * kLanes + l];
+            }
         }
         __syncthreads();
-        if (lane < left * kLanes) carry[lane] = keep;
+#pragma unroll
+        for (int k = 0; k < KEEP; ++k) {
+            const int i = lane + 64 * k;
+            if (i < left * kLanes) carry[i] = keep[k];
+        }
         __syncthreads();
         nc = left;
         pos = kN;
