@@ -26,7 +26,8 @@ from ._lib import ErdError, InvalidArgument
 __all__ = [
     "Vmt19937_64", "ErdError", "InvalidArgument", "STATE_BYTES",
     "randbits_u64", "uniform_real", "uniform_real_f64", "normal", "normal_vop",
-    "exponential", "cauchy", "student_t", "gamma", "chi_squared",
+    "exponential", "cauchy", "student_t", "gamma", "chi_squared", "poisson", "binomial",
+    "binomial_vop",
 ]
 
 STATE_BYTES = 10016  # erd_engine_state_bytes()
@@ -217,3 +218,29 @@ def gamma(rows, cols, urng, alpha=1.0, beta=1.0, out=None, stream=None):
 def chi_squared(rows, cols, urng, n=1.0, out=None, stream=None):
     """Rand::chiSquared<MatrixXf>(rows, cols, urng, n)."""
     return _fill("erd_fill_chisquared_f32", urng, rows, cols, _torch().float32, out, stream, float(n))
+
+
+def poisson(rows, cols, urng, mean=1.0, out=None, stream=None):
+    """Rand::poisson<MatrixXi>(rows, cols, urng, mean) (int32)."""
+    return _fill("erd_fill_poisson_i32", urng, rows, cols, _torch().int32, out, stream, float(mean))
+
+
+def binomial(rows, cols, urng, trials=1, p=0.5, out=None, stream=None):
+    """Rand::binomial<MatrixXi>(rows, cols, urng, trials, p) (int32)."""
+    return _fill("erd_fill_binomial_i32", urng, rows, cols, _torch().int32, out, stream, int(trials),
+                 float(p))
+
+
+def binomial_vop(urng, trials, p, out=None, stream=None):
+    """Rand::binomial(urng, trials_array, p_array) (BinomialVGen<int32_t>);
+    trials int32 and p float32 CUDA tensors of n*per_engine elements."""
+    torch = _torch()
+    trials = trials.contiguous()
+    p = p.contiguous()
+    if trials.numel() != p.numel() or trials.numel() % len(urng):
+        raise InvalidArgument("erd_fill_binomial_vop_i32", 1)
+    per = trials.numel() // len(urng)
+    buf = _alloc(urng, per, torch.int32, out)
+    _lib.call("erd_fill_binomial_vop_i32", urng.handle, _ptr(buf), per, _ptr(trials), _ptr(p),
+              _stream_ptr(stream))
+    return buf

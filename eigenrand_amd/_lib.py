@@ -62,6 +62,9 @@ _SIGS = {
     "erd_fill_studentt_f32": [P, P, I64, F32, P],
     "erd_fill_gamma_f32": [P, P, I64, F32, F32, P],
     "erd_fill_chisquared_f32": [P, P, I64, F32, P],
+    "erd_fill_poisson_i32": [P, P, I64, F64, P],
+    "erd_fill_binomial_i32": [P, P, I64, C.c_int32, F64, P],
+    "erd_fill_binomial_vop_i32": [P, P, I64, P, P, P],
     "erd_device_alloc": [C.POINTER(P), C.c_size_t],
     "erd_device_free": [P],
     "erd_memcpy_d2h": [P, P, C.c_size_t, P],

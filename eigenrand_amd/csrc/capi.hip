@@ -4,6 +4,7 @@
 #include <vector>
 #include <cmath>
 #include <cstring>
+#include <algorithm>
 #include "eigenrand_amd.h"
 #include "erd_device.hpp"
 #include "erd_kernels.hpp"
@@ -225,7 +226,63 @@ int erd_fill_gamma_f32(erd_engines* e, float* out, int64_t per, float alpha, flo
     if (!(alpha > 0.0f) || !std::isfinite(alpha)) return ERD_ERR_INVALID;
     if (alpha == 1.0f || ((float)(int)alpha == alpha && alpha < 20.0f))
         return hip_status(erd::launch_fill_gamma_fixed_f32(e->states, e->count, out, per, alpha, beta, S(stream)));
-    return ERD_ERR_UNSUPPORTED;
+    return hip_status(erd::launch_fill_gamma_reject_f32(e->states, e->count, out, per, alpha, beta,
+                                                        e->status, S(stream)));
+}
+
+int erd_fill_poisson_i32(erd_engines* e, int32_t* out, int64_t per, double mean, void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    if (!(mean >= 0.0) || !std::isfinite(mean)) return ERD_ERR_INVALID;
+    if (mean < 12) {
+        return hip_status(erd::launch_fill_poisson_knuth_i32(e->states, e->count, out, per, mean, 0, 0,
+                                                             e->status, S(stream)));
+    }
+    // PoissonGen constructor (Discrete.h:807-813), in double with libm.
+    erd::PoissonParams pp;
+    pp.mean = mean;
+    pp.ne_mean = std::exp(-mean);
+    pp.sqrt_tmean = std::sqrt(2 * mean);
+    pp.log_mean = std::log(mean);
+    pp.g1 = mean * pp.log_mean - std::lgamma(mean + 1);
+    return hip_status(erd::launch_fill_poisson_reject_i32(e->states, e->count, out, per, pp, e->status,
+                                                          S(stream)));
+}
+
+int erd_fill_binomial_i32(erd_engines* e, int32_t* out, int64_t per, int32_t trials, double p,
+                          void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    if (trials < 0 || !(p >= 0.0 && p <= 1.0)) return ERD_ERR_INVALID;
+    // BinomialGen constructor (Discrete.h:929-941).
+    erd::BinomialParams bp;
+    bp.trials = trials;
+    bp.p = p;
+    bp.small_p = std::min(p, 1 - p);
+    bp.mean = trials * bp.small_p;
+    bp.g1 = bp.sqrt_v = bp.log_small_p = bp.log_small_q = 0;
+    if (trials < 25)
+        return hip_status(erd::launch_fill_binomial_small_i32(e->states, e->count, out, per, trials, p,
+                                                              S(stream)));
+    const int flip = (p == bp.small_p) ? 0 : 1;
+    if (bp.mean < 1.0)
+        return hip_status(erd::launch_fill_poisson_knuth_i32(e->states, e->count, out, per, bp.mean,
+                                                             trials, flip, e->status, S(stream)));
+    bp.g1 = std::lgamma(trials + 1);
+    bp.sqrt_v = std::sqrt(2 * bp.mean * (1 - bp.small_p));
+    bp.log_small_p = std::log(bp.small_p);
+    bp.log_small_q = std::log(1 - bp.small_p);
+    return hip_status(erd::launch_fill_binomial_reject_i32(e->states, e->count, out, per, bp, e->status,
+                                                           S(stream)));
+}
+
+int erd_fill_binomial_vop_i32(erd_engines* e, int32_t* out, int64_t per, const int32_t* trials,
+                              const float* p, void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    if (per > 0 && (!trials || !p)) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_fill_binomial_vop_i32(e->states, e->count, out, per, trials, p,
+                                                        e->status, S(stream)));
 }
 
 int erd_fill_chisquared_f32(erd_engines* e, float* out, int64_t per, float n, void* stream) {

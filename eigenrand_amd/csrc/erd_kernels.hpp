@@ -8,6 +8,17 @@ namespace erd {
 
 struct EngineState;
 
+// PoissonGen constructor constants, computed on the host in double with libm
+// (Discrete.h:807-813).
+struct PoissonParams {
+    double mean, ne_mean, sqrt_tmean, log_mean, g1;
+};
+// BinomialGen constructor constants (Discrete.h:929-941).
+struct BinomialParams {
+    int32_t trials;
+    double p, small_p, mean, g1, sqrt_v, log_small_p, log_small_q;
+};
+
 // engine.hip
 hipError_t launch_seed(EngineState* st, const uint64_t* d_seeds, int64_t n, hipStream_t s);
 hipError_t launch_discard(EngineState* st, int64_t n, int64_t calls, hipStream_t s);
@@ -30,5 +41,24 @@ hipError_t launch_fill_studentt_f32(EngineState* st, int64_t n, float* out, int6
                                     hipStream_t s);
 hipError_t launch_fill_gamma_fixed_f32(EngineState* st, int64_t n, float* out, int64_t per,
                                        float alpha, float beta, hipStream_t s);
+
+// fill_reject.hip (CompressMask rejection samplers)
+hipError_t launch_fill_gamma_reject_f32(EngineState* st, int64_t n, float* out, int64_t per,
+                                        float alpha, float beta, uint32_t* status, hipStream_t s);
+hipError_t launch_fill_poisson_reject_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
+                                          const PoissonParams& pp, uint32_t* status, hipStream_t s);
+hipError_t launch_fill_binomial_reject_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
+                                           const BinomialParams& bp, uint32_t* status,
+                                           hipStream_t s);
+
+// fill_lockstep.hip (packet-lockstep samplers on half draws)
+hipError_t launch_fill_poisson_knuth_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
+                                         double mean, int32_t trials, int flip, uint32_t* status,
+                                         hipStream_t s);
+hipError_t launch_fill_binomial_small_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
+                                          int32_t trials, double p, hipStream_t s);
+hipError_t launch_fill_binomial_vop_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
+                                        const int32_t* trials, const float* p, uint32_t* status,
+                                        hipStream_t s);
 
 }  // namespace erd

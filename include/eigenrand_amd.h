@@ -111,6 +111,18 @@ int erd_fill_gamma_f32(erd_engines* e, float* out, int64_t per_engine, float alp
 /* chiSquared<float>(…, n) = GammaGen(n*0.5, 2) (NormalExp.h:767-802). */
 int erd_fill_chisquared_f32(erd_engines* e, float* out, int64_t per_engine, float n, void* stream);
 
+/* ------------------------------------------------------ integer (int32) --- */
+/* poisson<int32>(…, mean): PoissonGen (Discrete.h:788-904). mean >= 0. */
+int erd_fill_poisson_i32(erd_engines* e, int32_t* out, int64_t per_engine, double mean, void* stream);
+/* binomial<int32>(…, trials, p): BinomialGen (Discrete.h:911-1053). */
+int erd_fill_binomial_i32(erd_engines* e, int32_t* out, int64_t per_engine, int32_t trials, double p,
+                          void* stream);
+/* binomial(urng, trials_array, p_array): BinomialVGen<int32_t> (Discrete.h:
+ * 1061-1193, free function 1805-1831). trials (int32) and p (float) are device
+ * arrays of n*per_engine elements. */
+int erd_fill_binomial_vop_i32(erd_engines* e, int32_t* out, int64_t per_engine, const int32_t* trials,
+                              const float* p, void* stream);
+
 /* -------------------------------------------------------- device memory --- */
 /* Small helpers so callers without another GPU runtime can use the library. */
 int erd_device_alloc(void** ptr, size_t bytes);

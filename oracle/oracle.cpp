@@ -661,3 +661,26 @@ void orc_plgamma(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < 
 void orc_bit_to_ur_float(const uint32_t* w, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = bit_to_ur_float(w[i]); }
 
 }  // extern "C"
+
+extern "C" {
+// Runs the CompressMask queue over candidate packets (test hook): values and
+// masks are n_cand x W; writes up to max_packets emitted packets to out and
+// returns the number of candidates consumed when the last one was emitted.
+int64_t orc_compress_stream(int W, const float* values, const uint8_t* masks, int64_t n_cand,
+                            float* out, int64_t max_packets) {
+    float rest[8];
+    int rest_cnt = 0;
+    int64_t emitted = 0;
+    for (int64_t c = 0; c < n_cand && emitted < max_packets; ++c) {
+        float v[8];
+        bool m[8];
+        for (int j = 0; j < W; ++j) { v[j] = values[c * W + j]; m[j] = masks[c * W + j] != 0; }
+        bool full = (W == 8) ? compress_append<8>(v, m, rest, rest_cnt) : compress_append<4>(v, m, rest, rest_cnt);
+        if (full) {
+            for (int j = 0; j < W; ++j) out[emitted * W + j] = v[j];
+            if (++emitted == max_packets) return c + 1;
+        }
+    }
+    return -1;
+}
+}

@@ -185,3 +185,14 @@ def bit_to_ur_float(w):
     out = np.empty(w.size, dtype=np.float32)
     lib().orc_bit_to_ur_float(_p(w), _p(out), I64(w.size))
     return out
+
+
+def compress_stream(W, values, masks, max_packets):
+    values = np.ascontiguousarray(values, dtype=np.float32)
+    masks = np.ascontiguousarray(masks, dtype=np.uint8)
+    n = values.size // W
+    out = np.full(max_packets * W, np.nan, dtype=np.float32)
+    f = lib().orc_compress_stream
+    f.restype = C.c_int64
+    used = f(C.c_int(W), _p(values), _p(masks), I64(n), _p(out), I64(max_packets))
+    return out.reshape(max_packets, W), used

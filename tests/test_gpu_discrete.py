@@ -1,0 +1,80 @@
+"""GPU parity of the rejection (CompressMask) and lockstep samplers: gamma in
+its GS and tangent regimes, Poisson, binomial and BinomialVGen. Integer
+outputs and engine records must be identical to the oracle's; sequences mix
+half-draw consumers with full-draw fills so that fills start with a cached
+half draw and at odd block offsets."""
+import numpy as np
+import pytest
+
+from test_gpu_fixed import assert_same, flat
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = [5, 2 ** 33 + 1, 16380, 42, 999]
+
+
+@pytest.mark.parametrize("alpha,beta", [(0.2, 1.0), (0.5, 1.0), (0.99, 2.0), (3.7, 1.0), (10.5, 1.0),
+                                        (25.0, 0.5), (1.5, 3.0)])
+def test_gamma_reject_matches_oracle(er, orc, alpha, beta):
+    urng = er.Vmt19937_64(seeds=SEEDS)
+    ref = orc.Engines(seeds=SEEDS)
+    for n in (8, 16, 2504, 4096, 8):
+        out = er.gamma(n, 1, urng, alpha, beta)
+        assert_same(flat(out), orc.fill_gamma_f32(ref, n, alpha, beta))
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+    assert urng.status() == 0
+
+
+@pytest.mark.parametrize("mean", [0.3, 1.0, 4.0, 11.5, 12.0, 16.0, 100.0, 3000.0])
+def test_poisson_matches_oracle(er, orc, mean):
+    urng = er.Vmt19937_64(seeds=SEEDS)
+    ref = orc.Engines(seeds=SEEDS)
+    for n in (4, 12, 2500, 4):
+        out = er.poisson(n, 1, urng, mean)
+        assert_same(flat(out), orc.fill_poisson_i32(ref, n, mean))
+        np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+        # a full-draw fill in between leaves any cached half draw in place
+        u = er.uniform_real(8, 1, urng)
+        assert_same(flat(u), orc.fill_uniform_f32(ref, 8))
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+
+
+@pytest.mark.parametrize("trials,p", [(20, 0.5), (10, 0.9), (1, 0.3), (0, 0.5), (24, 1.0),
+                                      (50, 0.01), (30, 0.98), (100, 0.75), (1000, 0.3), (25, 0.5)])
+def test_binomial_matches_oracle(er, orc, trials, p):
+    urng = er.Vmt19937_64(seeds=SEEDS)
+    ref = orc.Engines(seeds=SEEDS)
+    for n in (4, 8, 1252, 4):
+        out = er.binomial(n, 1, urng, trials, p)
+        assert_same(flat(out), orc.fill_binomial_i32(ref, n, trials, p))
+        np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+        z = er.normal(8, 1, urng)
+        assert_same(flat(z), orc.fill_normal_f32(ref, 8))
+
+
+def test_binomial_vop_matches_oracle(er, orc):
+    """Config-3 parameter ranges: n ~ U{1..1000}, p ~ U(0,1), int32 bit-exact."""
+    import torch
+    rng = np.random.default_rng(2024)
+    per = 4096
+    n = len(SEEDS) * per
+    trials = rng.integers(1, 1001, n).astype(np.int32)
+    p = rng.uniform(0, 1, n).astype(np.float32)
+    urng = er.Vmt19937_64(seeds=SEEDS)
+    ref = orc.Engines(seeds=SEEDS)
+    for _ in range(2):
+        out = er.binomial_vop(urng, torch.from_numpy(trials).cuda(), torch.from_numpy(p).cuda())
+        assert_same(out.cpu().numpy(), orc.fill_binomial_vop_i32(ref, trials, p))
+        np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+
+
+def test_binomial_vop_edge_parameters(er, orc):
+    import torch
+    vals_n = np.array([0, 1, 1, 24, 25, 1000, 1000, 7], dtype=np.int32)
+    vals_p = np.array([0.5, 0.0, 1.0, 0.5, 0.04, 1e-4, 0.9999, 0.5], dtype=np.float32)
+    trials = np.tile(vals_n, 64)
+    p = np.tile(vals_p, 64)
+    urng = er.Vmt19937_64(seeds=[3])
+    ref = orc.Engines(seeds=[3])
+    out = er.binomial_vop(urng, torch.from_numpy(trials).cuda(), torch.from_numpy(p).cuda())
+    assert_same(out.cpu().numpy(), orc.fill_binomial_vop_i32(ref, trials, p))
