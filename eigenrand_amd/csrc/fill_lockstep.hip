@@ -33,6 +33,7 @@ struct HalfCursor {
     int pos;
     int nc;
     bool twisted;
+    bool used;  // a block slot was consumed by this fill
 
     __device__ int avail() const { return nc + (kHalfSlots - pos); }
     // Tempered 32-bit word of float lane j (0..3) in virtual slot v.
@@ -42,6 +43,7 @@ struct HalfCursor {
     }
     __device__ void advance(int n) {
         if (n >= nc) {
+            if (n > nc) used = true;
             pos += n - nc;
             nc = 0;
         }
@@ -67,6 +69,7 @@ __device__ __forceinline__ void cursor_begin(HalfCursor& cur, uint64_t* mt, uint
     cur.pos = 2 * st->idx;
     cur.nc = 0;
     cur.twisted = false;
+    cur.used = false;
     if (st->half_valid) {
         if (lane < 2) carry[lane] = st->half_cache[lane];
         cur.nc = 1;
@@ -89,6 +92,12 @@ __device__ __forceinline__ void cursor_end(const HalfCursor& cur, EngineState* s
         } else {
             st->idx = cur.pos / 2;
             st->half_valid = 0;
+            // The reference keeps the last split draw's high half in its cache
+            // member after handing it out; mirror it so records compare equal.
+            if (cur.used) {
+                st->half_cache[0] = temper(cur.mt[2 * (cur.pos - 1)]);
+                st->half_cache[1] = temper(cur.mt[2 * (cur.pos - 1) + 1]);
+            }
         }
     }
 }

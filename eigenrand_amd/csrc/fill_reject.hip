@@ -291,6 +291,12 @@ __global__ __launch_bounds__(64) void k_fill_reject(EngineState* __restrict__ st
         } else {
             st->idx = bpos / 2;
             st->half_valid = 0;
+            // The reference keeps the last split draw's high half in its cache
+            // member after handing it out; mirror it so records compare equal.
+            if (twisted || end_v > nc) {
+                st->half_cache[0] = temper(mt[2 * (bpos - 1)]);
+                st->half_cache[1] = temper(mt[2 * (bpos - 1) + 1]);
+            }
         }
     }
 }
