@@ -27,7 +27,7 @@ __all__ = [
     "Vmt19937_64", "ErdError", "InvalidArgument", "STATE_BYTES",
     "randbits_u64", "uniform_real", "uniform_real_f64", "normal", "normal_vop",
     "exponential", "cauchy", "student_t", "gamma", "chi_squared", "poisson", "binomial",
-    "binomial_vop",
+    "binomial_vop", "gamma_vop", "poisson_vop", "run_requests",
 ]
 
 STATE_BYTES = 10016  # erd_engine_state_bytes()
@@ -244,3 +244,47 @@ def binomial_vop(urng, trials, p, out=None, stream=None):
     _lib.call("erd_fill_binomial_vop_i32", urng.handle, _ptr(buf), per, _ptr(trials), _ptr(p),
               _stream_ptr(stream))
     return buf
+
+
+def gamma_vop(urng, alpha, beta, out=None, stream=None):
+    """Per-element gamma(alpha[i], beta[i]) (float32 CUDA tensors); the
+    lockstep vectorize-over-params design of BinomialVGen applied to GammaGen."""
+    torch = _torch()
+    alpha = alpha.contiguous()
+    beta = beta.contiguous()
+    if alpha.numel() != beta.numel() or alpha.numel() % len(urng):
+        raise InvalidArgument("erd_fill_gamma_vop_f32", 1)
+    per = alpha.numel() // len(urng)
+    buf = _alloc(urng, per, torch.float32, out)
+    _lib.call("erd_fill_gamma_vop_f32", urng.handle, _ptr(buf), per, _ptr(alpha), _ptr(beta),
+              _stream_ptr(stream))
+    return buf
+
+
+def poisson_vop(urng, mean, out=None, stream=None):
+    """Per-element poisson(mean[i]) (float32 CUDA tensor) -> int32."""
+    torch = _torch()
+    mean = mean.contiguous()
+    if mean.numel() % len(urng):
+        raise InvalidArgument("erd_fill_poisson_vop_i32", 1)
+    per = mean.numel() // len(urng)
+    buf = _alloc(urng, per, torch.int32, out)
+    _lib.call("erd_fill_poisson_vop_i32", urng.handle, _ptr(buf), per, _ptr(mean), _stream_ptr(stream))
+    return buf
+
+
+def run_requests(urng, requests, stream=None):
+    """Run a batch of heterogeneous fills (erd_run_requests). `requests` is a
+    list of dicts: dist (name), engine, count, out (tensor), optional in0/in1
+    (tensors), a, b."""
+    arr = (_lib.Request * len(requests))()
+    for i, r in enumerate(requests):
+        arr[i].dist = _lib.DIST[r["dist"]]
+        arr[i].engine = int(r.get("engine", 0))
+        arr[i].count = int(r["count"])
+        arr[i].out = r["out"].data_ptr() if r.get("out") is not None else None
+        arr[i].in0 = r["in0"].data_ptr() if r.get("in0") is not None else None
+        arr[i].in1 = r["in1"].data_ptr() if r.get("in1") is not None else None
+        arr[i].a = float(r.get("a", 0.0))
+        arr[i].b = float(r.get("b", 0.0))
+    _lib.call("erd_run_requests", urng.handle, C.cast(arr, C.c_void_p), len(requests), _stream_ptr(stream))

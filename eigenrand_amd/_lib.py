@@ -65,6 +65,9 @@ _SIGS = {
     "erd_fill_poisson_i32": [P, P, I64, F64, P],
     "erd_fill_binomial_i32": [P, P, I64, C.c_int32, F64, P],
     "erd_fill_binomial_vop_i32": [P, P, I64, P, P, P],
+    "erd_fill_gamma_vop_f32": [P, P, I64, P, P, P],
+    "erd_fill_poisson_vop_i32": [P, P, I64, P, P],
+    "erd_run_requests": [P, P, I64, P],
     "erd_device_alloc": [C.POINTER(P), C.c_size_t],
     "erd_device_free": [P],
     "erd_memcpy_d2h": [P, P, C.c_size_t, P],
@@ -110,3 +113,17 @@ def call(name, *args):
 
 def declared_symbols():
     return list(_SIGS)
+
+
+class Request(C.Structure):
+    """erd_request (include/eigenrand_amd.h)."""
+    _fields_ = [("dist", C.c_int32), ("engine", C.c_int32), ("count", C.c_int64),
+                ("out", C.c_void_p), ("in0", C.c_void_p), ("in1", C.c_void_p),
+                ("a", C.c_double), ("b", C.c_double)]
+
+
+DIST = {
+    "u64": 0, "uniform": 1, "uniform_f64": 2, "std_normal": 3, "normal": 4, "normal_vop": 5,
+    "exponential": 6, "cauchy": 7, "student_t": 8, "gamma": 9, "poisson": 10, "binomial": 11,
+    "binomial_vop": 12, "gamma_vop": 13, "poisson_vop": 14, "discard": 15,
+}

@@ -307,3 +307,147 @@ int erd_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
 int erd_stream_synchronize(void* stream) { return hip_status(hipStreamSynchronize(S(stream))); }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- VoP + batched requests ---
+#include "erd_segment.hpp"
+
+extern "C" {
+
+int erd_fill_gamma_vop_f32(erd_engines* e, float* out, int64_t per, const float* alpha,
+                           const float* beta, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    if (per > 0 && (!alpha || !beta)) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_fill_gamma_vop_f32(e->states, e->count, out, per, alpha, beta,
+                                                     e->status, S(stream)));
+}
+
+int erd_fill_poisson_vop_i32(erd_engines* e, int32_t* out, int64_t per, const float* mean,
+                             void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    if (per > 0 && !mean) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_fill_poisson_vop_i32(e->states, e->count, out, per, mean, e->status,
+                                                       S(stream)));
+}
+
+}  // extern "C"
+
+namespace erd {
+
+// Public request -> internal segment (constructor constants on the host).
+int make_segment(const erd_request& r, Segment& s) {
+    s = Segment{};
+    s.kind = r.dist;
+    s.count = r.count;
+    s.out = r.out;
+    s.in0 = r.in0;
+    s.in1 = r.in1;
+    if (r.count < 0) return ERD_ERR_INVALID;
+    int64_t width = 8;
+    switch (r.dist) {
+        case ERD_DIST_U64: width = 4; break;
+        case ERD_DIST_UNIFORM_F32:
+            s.d[0] = (float)r.a; s.d[1] = (float)r.b;
+            s.i0 = !((float)r.a == 0.0f && (float)r.b == 1.0f);
+            break;
+        case ERD_DIST_UNIFORM_F64:
+            width = 4;
+            s.d[0] = r.a; s.d[1] = r.b;
+            s.i0 = !(r.a == 0.0 && r.b == 1.0);
+            break;
+        case ERD_DIST_STD_NORMAL_F32: break;
+        case ERD_DIST_NORMAL_F32: s.d[0] = (float)r.a; s.d[1] = (float)r.b; break;
+        case ERD_DIST_NORMAL_VOP_F32:
+            if (r.count > 0 && (!r.in0 || !r.in1)) return ERD_ERR_INVALID;
+            break;
+        case ERD_DIST_EXPONENTIAL_F32: s.d[0] = (float)r.a; break;
+        case ERD_DIST_CAUCHY_F32: s.d[0] = (float)r.a; s.d[1] = (float)r.b; break;
+        case ERD_DIST_STUDENT_T_F32: s.d[0] = (float)r.a; break;
+        case ERD_DIST_GAMMA_F32:
+            if (!((float)r.a > 0.0f) || !std::isfinite((float)r.a)) return ERD_ERR_INVALID;
+            s.d[0] = (float)r.a; s.d[1] = (float)r.b;
+            break;
+        case ERD_DIST_POISSON_I32:
+            width = 4;
+            if (!(r.a >= 0.0) || !std::isfinite(r.a)) return ERD_ERR_INVALID;
+            s = seg_poisson((int32_t*)r.out, r.count, r.a);
+            break;
+        case ERD_DIST_BINOMIAL_I32: {
+            width = 4;
+            const int32_t trials = (int32_t)r.a;
+            if (trials < 0 || (double)trials != r.a || !(r.b >= 0.0 && r.b <= 1.0)) return ERD_ERR_INVALID;
+            s = seg_binomial((int32_t*)r.out, r.count, trials, r.b);
+            break;
+        }
+        case ERD_DIST_BINOMIAL_VOP_I32:
+        case ERD_DIST_POISSON_VOP_I32:
+            width = 4;
+            if (r.count > 0 && (!r.in0 || (r.dist == ERD_DIST_BINOMIAL_VOP_I32 && !r.in1)))
+                return ERD_ERR_INVALID;
+            break;
+        case ERD_DIST_GAMMA_VOP_F32:
+            if (r.count > 0 && (!r.in0 || !r.in1)) return ERD_ERR_INVALID;
+            break;
+        case ERD_DIST_DISCARD: width = 1; break;
+        default: return ERD_ERR_INVALID;
+    }
+    if (r.count % width != 0) return ERD_ERR_INVALID;
+    if (r.dist != ERD_DIST_DISCARD && r.count > 0 && !r.out) return ERD_ERR_INVALID;
+    return ERD_OK;
+}
+
+// Groups requests by engine (keeping each engine's order), uploads the
+// program and launches it. Device buffers come from the stream-ordered
+// allocator and are released on the same stream.
+int run_requests(erd_engines* e, const erd_request* reqs, int64_t n, hipStream_t stream) {
+    if (!e || !e->states || n < 0 || (n > 0 && !reqs)) return ERD_ERR_INVALID;
+    if (n == 0) return ERD_OK;
+    std::vector<int64_t> order((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        if (reqs[i].engine < 0 || reqs[i].engine >= e->count) return ERD_ERR_INVALID;
+        order[(size_t)i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t x, int64_t y) { return reqs[x].engine < reqs[y].engine; });
+    std::vector<Segment> segs((size_t)n);
+    std::vector<int32_t> engine_of, begin;
+    for (int64_t k = 0; k < n; ++k) {
+        const erd_request& r = reqs[order[(size_t)k]];
+        int rc = make_segment(r, segs[(size_t)k]);
+        if (rc) return rc;
+        if (engine_of.empty() || engine_of.back() != r.engine) {
+            engine_of.push_back(r.engine);
+            begin.push_back((int32_t)k);
+        }
+    }
+    begin.push_back((int32_t)n);
+    const int32_t groups = (int32_t)engine_of.size();
+    const size_t seg_bytes = sizeof(Segment) * (size_t)n;
+    const size_t idx_bytes = sizeof(int32_t) * (size_t)(2 * groups + 1);
+    char* dbuf = nullptr;
+    hipError_t err = hipMallocAsync((void**)&dbuf, seg_bytes + idx_bytes, stream);
+    if (err != hipSuccess) return err == hipErrorOutOfMemory ? ERD_ERR_OOM : ERD_ERR_HIP;
+    std::vector<char> host(seg_bytes + idx_bytes);
+    std::memcpy(host.data(), segs.data(), seg_bytes);
+    std::memcpy(host.data() + seg_bytes, engine_of.data(), sizeof(int32_t) * (size_t)groups);
+    std::memcpy(host.data() + seg_bytes + sizeof(int32_t) * (size_t)groups, begin.data(),
+                sizeof(int32_t) * (size_t)(groups + 1));
+    err = hipMemcpyAsync(dbuf, host.data(), host.size(), hipMemcpyHostToDevice, stream);
+    // The copy is staged from pageable memory, so `host` may be released once
+    // it returns; synchronise the stream to make that unconditional.
+    if (err == hipSuccess) err = hipStreamSynchronize(stream);
+    if (err == hipSuccess)
+        err = launch_program(e->states, groups, (const int32_t*)(dbuf + seg_bytes),
+                             (const int32_t*)(dbuf + seg_bytes) + groups, (const Segment*)dbuf,
+                             e->status, stream);
+    hipError_t e2 = hipFreeAsync(dbuf, stream);
+    if (err == hipSuccess) err = e2;
+    return err == hipSuccess ? ERD_OK : ERD_ERR_HIP;
+}
+
+}  // namespace erd
+
+extern "C" int erd_run_requests(erd_engines* e, const erd_request* requests, int64_t n, void* stream) {
+    return erd::run_requests(e, requests, n, S(stream));
+}

@@ -41,6 +41,7 @@ static_assert(sizeof(EngineState) == 10016, "engine record layout");
 
 // Error bits a kernel can raise (OR-ed into a device status word).
 constexpr uint32_t kStatusIterCap = 1u;
+constexpr uint32_t kStatusBadSegment = 2u;
 
 __device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t as_u(float f) { return __float_as_uint(f); }

@@ -1,0 +1,271 @@
+// Per-distribution arithmetic shared by the fill kernels and the segment
+// program kernel: fixed-consumption unit functors (op_fixed) and rejection
+// candidate policies (op_reject). Each follows the reference's packetOp line
+// by line in the unfused float order.
+#pragma once
+#include "erd_device.hpp"
+
+namespace erd {
+
+// ============================================================ fixed units ===
+// Each functor: CALLS full draws per unit; units(per) = units for `per`
+// outputs; emit(per, u, l, w) writes the outputs of unit u from the tempered
+// words w[0..CALLS) of u64 lane l; rebase(off) moves its pointers to the block
+// of one engine.
+
+struct RawU64 {  // out[4u + l] = lane l of call u (PacketRandomEngine.h:295-310)
+    static constexpr int CALLS = 1;
+    uint64_t* out;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 4; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        out[4 * u + l] = w[0];
+    }
+};
+
+struct UniformF32 {  // Basic.h:584-590 / 626-634; float j of a call = word j
+    static constexpr int CALLS = 1;
+    float* out;
+    float a, slope;
+    int affine;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        float u0 = ur_f32(lo32(w[0])), u1 = ur_f32(hi32(w[0]));
+        if (affine) { u0 = u0 * slope + a; u1 = u1 * slope + a; }
+        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(u0, u1);
+    }
+};
+
+struct UniformF64 {  // arch/AVX/RandUtils.h:212-225: 4 doubles per call
+    static constexpr int CALLS = 1;
+    double* out;
+    double a, slope;
+    int affine;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 4; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        double v = ur_f64(w[0]);
+        if (affine) v = v * slope + a;
+        out[4 * u + l] = v;
+    }
+};
+
+// Box-Muller pair of StdNormalGen::packetOp (NormalExp.h:60-82): calls A, B
+// give u1 = 1 - U(A), u2 = U(B); the first packet is r*sin, the cached second
+// packet r*cos. A fill of 8 mod 16 outputs ends with an unused cos packet.
+template <int MODE>  // 0 = StdNormalGen, 1 = NormalGen(mean, sd), 2 = NormalVGen
+struct NormalF32 {
+    static constexpr int CALLS = 2;
+    float* out;
+    float mean, sd;
+    const float* vmean;
+    const float* vsd;
+    __device__ void rebase(int64_t off) {
+        out += off;
+        if (MODE == 2) { vmean += off; vsd += off; }
+    }
+    __device__ static int64_t units(int64_t n) { return (n + 15) / 16; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        float sn[2], cs[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float u1 = 1.0f - ur_f32(h ? hi32(w[0]) : lo32(w[0]));
+            const float u2 = ur_f32(h ? hi32(w[1]) : lo32(w[1]));
+            const float radius = sqrtf(-2.0f * plog_pos(u1));
+            float s, c;
+            psincos(ERD_F(2 * kPi) * u2, s, c);
+            sn[h] = radius * s;
+            cs[h] = radius * c;
+        }
+        const int64_t base = 16 * u + 2 * l;
+        const bool full = 16 * u + 16 <= per;
+        if (MODE == 1) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) { sn[h] = sn[h] * sd + mean; cs[h] = cs[h] * sd + mean; }
+        } else if (MODE == 2) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                sn[h] = sn[h] * vsd[base + h] + vmean[base + h];
+                if (full) cs[h] = cs[h] * vsd[base + 8 + h] + vmean[base + 8 + h];
+            }
+        }
+        *reinterpret_cast<float2*>(out + base) = make_float2(sn[0], sn[1]);
+        if (full) *reinterpret_cast<float2*>(out + base + 8) = make_float2(cs[0], cs[1]);
+    }
+};
+
+struct ExponentialF32 {  // NormalExp.h:391-398
+    static constexpr int CALLS = 1;
+    float* out;
+    float lambda;
+    float scale;  // 1 for ExponentialGen; beta for GammaGen(alpha == 1) (NormalExp.h:561-565)
+    int gamma;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        float v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            float x = -(plog_pos(1.0f - ur_f32(h ? hi32(w[0]) : lo32(w[0]))) / lambda);
+            v[h] = gamma ? scale * x : x;
+        }
+        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
+    }
+};
+
+struct CauchyF32 {  // NormalExp.h:843-855
+    static constexpr int CALLS = 1;
+    float* out;
+    float a, b;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        float v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            float s, c;
+            psincos(ERD_F(kPi) * (ur_f32(h ? hi32(w[0]) : lo32(w[0])) - 0.5f), s, c);
+            v[h] = a + b * (s / c);
+        }
+        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
+    }
+};
+
+struct StudentTF32 {  // NormalExp.h:285-301 (sine from the reference's psincos)
+    static constexpr int CALLS = 2;
+    float* out;
+    float n, k;  // k = -2/n in float
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n_) { return n_ / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        float v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float u1 = 1.0f - ur_f32(h ? hi32(w[0]) : lo32(w[0]));
+            const float u2 = ur_f32(h ? hi32(w[1]) : lo32(w[1]));
+            const float radius = sqrtf(n * (pexp(plog_pos(u1) * k) - 1.0f));
+            float s, c;
+            psincos(ERD_F(2 * kPi) * u2, s, c);
+            v[h] = radius * s;
+        }
+        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
+    }
+};
+
+// GammaGen with integer alpha < 20: -(beta * log(prod of alpha nonzero
+// uniforms)), alpha calls per packet (NormalExp.h:567-578).
+template <int ALPHA>
+struct GammaIntF32 {
+    static constexpr int CALLS = ALPHA;
+    float* out;
+    float beta;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        float v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            float yx = nzur_f32(h ? hi32(w[0]) : lo32(w[0]));
+#pragma unroll
+            for (int c = 1; c < ALPHA; ++c) yx = yx * nzur_f32(h ? hi32(w[c]) : lo32(w[c]));
+            v[h] = -(beta * plog(yx));
+        }
+        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
+    }
+};
+
+
+// ====================================================== rejection policies ===
+// W: lanes per packet; SW: words per slot (4 full draw, 2 half draw);
+// K: slots per candidate; eval(w, h, v, acc): lane 2p+h of a candidate from
+// word p of each of its K slots; store(out, i, v): write output element i.
+
+struct GammaGS {  // NormalExp.h:536-558
+    static constexpr int W = 8, SW = 4, K = 3;
+    using Out = float;
+    float beta, px, inv_alpha, am1;
+    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc) const {
+        const float ux = ur_f32(h ? hi32(w[0]) : lo32(w[0]));
+        const float vx = nzur_f32(h ? hi32(w[1]) : lo32(w[1]));
+        const float lv = plog_pos(vx);
+        const float x1 = pexp(inv_alpha * lv);
+        const float q1 = pexp(-x1);
+        const float x2 = 1.0f - lv;
+        const float q2 = pexp(plog_pos(x2) * am1);
+        const bool c = ux < px;
+        const float xx = c ? x1 : x2;
+        const float qx = c ? q1 : q2;
+        const float u3 = ur_f32(h ? hi32(w[2]) : lo32(w[2]));
+        v = beta * xx;
+        acc = u3 < qx;
+    }
+    __device__ void store(Out* o, int64_t i, float v) const { o[i] = v; }
+};
+
+struct GammaTangent {  // NormalExp.h:580-603
+    static constexpr int W = 8, SW = 4, K = 2;
+    using Out = float;
+    float beta, sq, am1;
+    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc) const {
+        float ys, yc;
+        psincos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), ys, yc);
+        const float yx = ys / yc;
+        const float xx = sq * yx + am1;
+        const bool c = 0.0f < xx;
+        const float ux = ur_f32(h ? hi32(w[1]) : lo32(w[1]));
+        const float ub = (yx * yx + 1.0f) * pexp(am1 * plog(xx / am1) - yx * sq);
+        acc = c && (ux <= ub);
+        v = beta * xx;
+    }
+    __device__ void store(Out* o, int64_t i, float v) const { o[i] = v; }
+};
+
+struct PoissonTangent {  // Discrete.h:875-902 (Packet4f via half())
+    static constexpr int W = 4, SW = 2, K = 2;
+    using Out = int32_t;
+    float psq, pmean, plm, pg1;
+    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc) const {
+        float s, c;
+        psincos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), s, c);
+        const float yx = s / c;
+        const float fres = truncf(psq * yx + pmean);
+        const float p1 = (yx * yx + 1.0f) * ERD_F(0.9);
+        const float p2 = pexp((fres * plm - plgamma_approx(fres + 1.0f)) - pg1);
+        const bool c1 = 0.0f <= fres;
+        const float u2 = ur_f32(h ? hi32(w[1]) : lo32(w[1]));
+        acc = c1 && (u2 <= p1 * p2);
+        v = fres;
+    }
+    __device__ void store(Out* o, int64_t i, float v) const { o[i] = (int32_t)v; }
+};
+
+struct BinomialTangent {  // Discrete.h:1011-1051
+    static constexpr int W = 4, SW = 2, K = 2;
+    using Out = int32_t;
+    float ptrials, psqrt_v, pmean, plp, plq, pg1;
+    int32_t trials;
+    int flip;  // p != small_p: result is trials - k
+    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc) const {
+        float s, c;
+        psincos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), s, c);
+        const float ys = s / c;
+        const float fres = truncf(psqrt_v * ys + pmean);
+        const float p1 = (ERD_F(1.2) * psqrt_v) * (1.0f + ys * ys);
+        const float p2 = pexp((((pg1 - plgamma_approx(fres + 1.0f)) -
+                                plgamma_approx((ptrials + 1.0f) - fres)) +
+                               fres * plp) +
+                              (ptrials - fres) * plq);
+        const bool c1 = (0.0f <= fres) && (fres <= ptrials);
+        const float u2 = ur_f32(h ? hi32(w[1]) : lo32(w[1]));
+        acc = c1 && (u2 <= p1 * p2);
+        v = fres;
+    }
+    __device__ void store(Out* o, int64_t i, float v) const {
+        const int32_t k = (int32_t)v;
+        o[i] = flip ? trials - k : k;
+    }
+};
+
+
+}  // namespace erd

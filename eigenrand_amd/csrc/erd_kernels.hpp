@@ -7,6 +7,7 @@
 namespace erd {
 
 struct EngineState;
+struct Segment;
 
 // PoissonGen constructor constants, computed on the host in double with libm
 // (Discrete.h:807-813).
@@ -60,5 +61,18 @@ hipError_t launch_fill_binomial_small_i32(EngineState* st, int64_t n, int32_t* o
 hipError_t launch_fill_binomial_vop_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
                                         const int32_t* trials, const float* p, uint32_t* status,
                                         hipStream_t s);
+
+hipError_t launch_fill_gamma_vop_f32(EngineState* st, int64_t n, float* out, int64_t per,
+                                     const float* alpha, const float* beta, uint32_t* status,
+                                     hipStream_t s);
+hipError_t launch_fill_poisson_vop_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
+                                       const float* mean, uint32_t* status, hipStream_t s);
+
+// program.hip (segment programs)
+hipError_t launch_program(EngineState* st, int32_t n_groups, const int32_t* d_engine_of,
+                          const int32_t* d_begin, const Segment* d_segs, uint32_t* status,
+                          hipStream_t s);
+Segment seg_poisson(int32_t* out, int64_t count, double mean);
+Segment seg_binomial(int32_t* out, int64_t count, int32_t trials, double p);
 
 }  // namespace erd

@@ -1,0 +1,708 @@
+// Sampling ops on an engine resident in LDS.
+//
+// A kernel loads one Vmt19937_64 record into LDS (Eng), runs one or more ops
+// on it, and writes it back. Ops follow the reference's consumption exactly:
+//   op_fixed    - a fixed number of full draws per output unit
+//   op_reject   - candidate packets through the CompressMask queue
+//   op_knuth / op_binomial_small / op_binomial_vop - packet-lockstep loops
+// Full-draw ops use Eng::idx only; half-draw ops (Packet4f via half()) also
+// read and update the engine's cached half draw, as the reference does.
+// Every op is called by all 64 threads of the wave with wave-uniform
+// arguments; `lane` is the thread index.
+#pragma once
+#include "erd_device.hpp"
+#include "erd_slots.hpp"
+
+namespace erd {
+
+struct Eng {
+    uint64_t* mt;            // LDS, kWords
+    int idx;                 // next draw in the block (kN: twist first)
+    int half_valid;
+    uint64_t half_cache[2];  // tempered upper half of the last split draw
+    bool twisted;            // LDS block differs from the HBM record
+};
+
+__device__ __forceinline__ void eng_load(Eng& e, uint64_t* mt, const EngineState* st, int lane) {
+    load_state(mt, st, lane);
+    e.mt = mt;
+    e.idx = st->idx;
+    e.half_valid = st->half_valid;
+    e.half_cache[0] = st->half_cache[0];
+    e.half_cache[1] = st->half_cache[1];
+    e.twisted = false;
+}
+
+__device__ __forceinline__ void eng_store(const Eng& e, EngineState* st, int lane) {
+    __syncthreads();
+    if (e.twisted) store_state(st, e.mt, lane);
+    if (lane == 0) {
+        st->idx = e.idx;
+        st->half_valid = e.half_valid;
+        st->half_cache[0] = e.half_cache[0];
+        st->half_cache[1] = e.half_cache[1];
+    }
+}
+
+// ---------------------------------------------------------------- fixed ---
+// D: CALLS draws per unit; units(per); emit(per, u, l, w) writes the outputs
+// of unit u from the tempered words w[0..CALLS) of lane l.
+template <class D>
+__device__ void op_fixed(Eng& e, uint64_t* carry, int64_t per, const D& d, int lane) {
+    constexpr int C = D::CALLS;
+    constexpr int MAXC = C > 1 ? C - 1 : 1;
+    const int64_t units = D::units(per);
+    if (units == 0) return;
+    int pos = e.idx;
+    int nc = 0;
+    int64_t done = 0;
+    for (;;) {
+        if (pos >= kN) {
+            twist_lds(e.mt, lane);
+            pos = 0;
+            e.twisted = true;
+        }
+        const int avail = nc + (kN - pos);
+        const int64_t can = (int64_t)(avail / C);
+        const int nu = (int)(units - done < can ? units - done : can);
+        for (int t = lane; t < nu * kLanes; t += 64) {
+            const int u = t >> 2, l = t & 3;
+            uint64_t w[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) w[c] = slot_word<kLanes>(e.mt, carry, nc, pos, u * C + c, l);
+            d.emit(per, done + u, l, w);
+        }
+        const int consumed = nu * C;
+        done += nu;
+        if (done == units) {
+            pos = pos + consumed - nc;
+            break;
+        }
+        // Keep the slots of the unit that straddles the block boundary.
+        carry_tail<kLanes, MAXC>(e.mt, carry, nc, pos, consumed, avail, lane);
+        nc = avail - consumed;
+        pos = kN;
+    }
+    __syncthreads();
+    e.idx = pos;
+}
+
+// ------------------------------------------------------------- half state ---
+// Position of a half-draw consumer: block slot `pos` (in half draws) plus an
+// optional carried (already tempered) slot holding the engine's cached half.
+__device__ __forceinline__ void half_begin(const Eng& e, uint64_t* carry, int& pos, int& nc,
+                                          int lane) {
+    pos = 2 * e.idx;
+    nc = 0;
+    if (e.half_valid) {
+        if (lane < 2) carry[lane] = e.half_cache[lane];
+        nc = 1;
+    }
+    __syncthreads();
+}
+
+// Write back the half-draw position: an odd block slot means the draw's upper
+// half is cached. When the position is even the reference still keeps the
+// last split draw's upper half in its cache member; mirror that too.
+__device__ __forceinline__ void half_end(Eng& e, int pos, int nc, bool used) {
+    if (nc > 0) {
+        e.idx = pos / 2;  // the cached half was not consumed
+        return;
+    }
+    if (pos & 1) {
+        e.idx = (pos + 1) / 2;
+        e.half_valid = 1;
+        e.half_cache[0] = temper(e.mt[2 * pos]);
+        e.half_cache[1] = temper(e.mt[2 * pos + 1]);
+    } else {
+        e.idx = pos / 2;
+        e.half_valid = 0;
+        if (used) {
+            e.half_cache[0] = temper(e.mt[2 * (pos - 1)]);
+            e.half_cache[1] = temper(e.mt[2 * (pos - 1) + 1]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- reject ---
+constexpr int kRejectCap = 1 << 22;  // candidates without output before giving up
+
+// Scratch a rejection op needs in LDS.
+template <class P>
+struct RejectLds {
+    uint64_t carry[(P::K - 1) * P::SW];
+    float rest[P::W];
+    int32_t tab[64 / (P::W / 2) + 1];
+};
+
+// P: W lanes per packet, SW words per slot (4 full draw / 2 half draw), K
+// slots per candidate; eval(w, h, v, acc) computes lane 2p+h of a candidate
+// from word p of each of its K slots; store(out, i, v) writes element i.
+//
+// Queue order of CompressMask (arch/AVX/PacketFilter.h:147-184, arch/SSE/
+// PacketFilter.h:118-150) from wave ballots: a fully accepted candidate is
+// emitted as its own packet; accepted lanes of the others form the queue in
+// (candidate, lane) order and queue item f lands in queued packet f/W; the
+// output index of a packet = bypassing candidates before its emitter + queued
+// packets before it.
+template <class P>
+__device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t per, const P& pol,
+                          uint32_t* status, int lane) {
+    constexpr int W = P::W, SW = P::SW, K = P::K;
+    constexpr int IPC = W / 2;     // threads per candidate (two lanes each)
+    constexpr int CPS = 64 / IPC;  // candidates per wave step
+    constexpr int SLOTS = SlotMode<SW>::kSlots;
+    constexpr int MAXC = K - 1;    // straddling slots, or the cached half
+    const int64_t T = per / W;
+    if (T == 0) return;
+    uint64_t* carry = s.carry;
+    int pos, nc = 0;
+    if (SW == 2) {
+        half_begin(e, carry, pos, nc, lane);
+    } else {
+        pos = e.idx;
+    }
+    bool used = false;
+    int64_t full_cnt = 0, fifo = 0, since_emit = 0;
+    int end_v = -1;
+    const int cl = lane / IPC, p = lane % IPC;
+    for (;;) {
+        if (pos >= SLOTS) {
+            twist_lds(e.mt, lane);
+            pos = 0;
+            e.twisted = true;
+            used = true;
+        }
+        const int avail = nc + (SLOTS - pos);
+        const int ncand = avail / K;
+        for (int c0 = 0; c0 < ncand; c0 += CPS) {
+            const int c = c0 + cl;
+            const bool live = c < ncand;
+            float v[2] = {0.f, 0.f};
+            bool a[2] = {false, false};
+            if (live) {
+                uint64_t w[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) w[k] = slot_word<SW>(e.mt, carry, nc, pos, c * K + k, p);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) pol.eval(w, h, v[h], a[h]);
+            }
+            const uint64_t B0 = __ballot(a[0]), B1 = __ballot(a[1]);
+            const uint64_t cmask = (IPC == 4 ? 0xFull : 0x3ull) << (cl * IPC);
+            const bool full = live && ((B0 & B1 & cmask) == cmask);
+            const bool q0 = a[0] && !full, q1 = a[1] && !full;
+            const uint64_t A0 = __ballot(q0), A1 = __ballot(q1);
+            const uint64_t FC = __ballot(full && p == 0);
+            const uint64_t below_c = lanemask_lt(cl * IPC), below = lanemask_lt(lane);
+            const int64_t fb = full_cnt + __popcll(FC & below_c);
+            const int64_t fstart = fifo + __popcll(A0 & below_c) + __popcll(A1 & below_c);
+            const int64_t f0 = fifo + __popcll(A0 & below) + __popcll(A1 & below);
+            const int64_t f1 = f0 + (q0 ? 1 : 0);
+            const int64_t fifo_end = fifo + __popcll(A0) + __popcll(A1);
+            const int64_t m0 = fifo / W, m_end = fifo_end / W;
+            const int acnt = __popcll(A0 & cmask) + __popcll(A1 & cmask);
+            int64_t emit = -1;
+            if (live && (full || (acnt > 0 && (fstart + acnt) / W > fstart / W))) emit = fb + fstart / W;
+            if (live && !full && emit >= 0 && p == 0) s.tab[fstart / W - m0] = (int32_t)emit;
+            const uint64_t LAST = __ballot(emit == T - 1 && p == 0);
+            __syncthreads();
+            if (full && emit < T) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) pol.store(o, emit * W + 2 * p + h, v[h]);
+            }
+            const int pend = (int)(fifo % W);
+            if (m_end > m0 && lane < pend) {
+                const int64_t ex = s.tab[0];
+                if (ex < T) pol.store(o, ex * W + lane, s.rest[lane]);
+            }
+            float park[2] = {0.f, 0.f};
+            int park_at[2] = {-1, -1};
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const bool q = h ? q1 : q0;
+                const int64_t f = h ? f1 : f0;
+                if (!q) continue;
+                const int64_t m = f / W;
+                if (m < m_end) {
+                    const int64_t ex = s.tab[m - m0];
+                    if (ex < T) pol.store(o, ex * W + (f % W), v[h]);
+                } else {
+                    park[h] = v[h];
+                    park_at[h] = (int)(f % W);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (park_at[h] >= 0) s.rest[park_at[h]] = park[h];
+            full_cnt += __popcll(FC);
+            fifo = fifo_end;
+            if (LAST) {
+                const int clast = (int)(__ffsll((long long)LAST) - 1) / IPC;
+                end_v = (c0 + clast + 1) * K;
+                break;
+            }
+            if (__popcll(FC) + (m_end - m0) > 0) {
+                since_emit = 0;
+            } else {
+                since_emit += CPS;
+                if (since_emit > kRejectCap) break;
+            }
+        }
+        if (end_v >= 0) break;
+        if (since_emit > kRejectCap) {
+            if (lane == 0) atomicOr(status, kStatusIterCap);
+            end_v = ncand * K;
+            break;
+        }
+        carry_tail<SW, MAXC>(e.mt, carry, nc, pos, ncand * K, avail, lane);
+        nc = avail - ncand * K;
+        pos = SLOTS;
+    }
+    __syncthreads();
+    if (end_v > nc) used = true;
+    int bpos = pos, rem = nc;
+    if (end_v >= nc) {
+        bpos = pos + end_v - nc;
+        rem = 0;
+    }
+    if (SW == 4) {
+        e.idx = bpos;
+    } else {
+        half_end(e, bpos, rem, used);
+    }
+}
+
+// -------------------------------------------------------------- lockstep ---
+constexpr int kHalfSlots = kWords / 2;  // half draws per block
+constexpr int kLockstepCap = 1 << 20;   // iterations per packet before giving up
+
+// Cursor over the slot sequence of a lockstep op: SW words per slot (4 full
+// draw, 2 half draw), at most MAXC carried slots across a twist.
+template <int SW, int MAXC>
+struct Cursor {
+    static constexpr int kSlots = SlotMode<SW>::kSlots;
+    uint64_t* mt;
+    uint64_t* carry;
+    int pos;
+    int nc;
+    bool used;
+    bool* twisted;
+
+    __device__ int avail() const { return nc + (kSlots - pos); }
+    __device__ uint64_t word(int v, int p) const { return slot_word<SW>(mt, carry, nc, pos, v, p); }
+    // Tempered 32-bit word of float lane j (0..2*SW-1) in virtual slot v.
+    __device__ uint32_t word32(int v, int j) const {
+        const uint64_t w = word(v, j >> 1);
+        return (j & 1) ? hi32(w) : lo32(w);
+    }
+    __device__ void advance(int n) {
+        if (n >= nc) {
+            if (n > nc) used = true;
+            pos += n - nc;
+            nc = 0;
+        } else {
+            // Only part of the carried slots consumed: shift them down.
+            for (int i = 0; i + n < nc; ++i)
+                for (int q = 0; q < SW; ++q) carry[i * SW + q] = carry[(i + n) * SW + q];
+            nc -= n;
+        }
+    }
+    // Make at least `need` (<= MAXC+1) slots available; called by the whole wave.
+    __device__ void ensure(int need, int lane) {
+        const int a = avail();
+        if (a >= need) return;
+        carry_tail<SW, MAXC>(mt, carry, nc, pos, 0, a, lane);
+        nc = a;
+        twist_lds(mt, lane);
+        pos = 0;
+        *twisted = true;
+    }
+};
+using HalfCursor = Cursor<2, 1>;
+
+__device__ __forceinline__ HalfCursor cursor_begin(Eng& e, uint64_t* carry, int lane) {
+    HalfCursor c;
+    c.mt = e.mt;
+    c.carry = carry;
+    c.used = false;
+    c.twisted = &e.twisted;
+    half_begin(e, carry, c.pos, c.nc, lane);
+    return c;
+}
+
+__device__ __forceinline__ void cursor_end(Eng& e, const HalfCursor& c) {
+    __syncthreads();
+    half_end(e, c.pos, c.nc, c.used);
+}
+
+// Knuth loop of PoissonGen::packetOp (Discrete.h:860-873); returns lane j's count.
+__device__ __forceinline__ int32_t knuth_packet(HalfCursor& cur, float thr, int lane,
+                                                uint32_t* status) {
+    const int j = lane & 3;
+    float val = 1.0f;
+    int32_t res = 0;
+    for (int it = 0;; ++it) {
+        cur.ensure(1, lane);
+        const float u = ur_f32(cur.word32(0, j));
+        cur.advance(1);
+        val = val * u;
+        const bool c = thr < val;
+        if (!(__ballot(c) & 0xFull)) break;
+        res += c ? 1 : 0;
+        if (it > kLockstepCap) {
+            if (lane == 0) atomicOr(status, kStatusIterCap);
+            break;
+        }
+    }
+    return res;
+}
+
+// PoissonGen, mean < 12; BinomialGen's Poisson regime with the final flip.
+__device__ void op_knuth(Eng& e, uint64_t* carry, int32_t* o, int64_t per, float thr,
+                         int32_t trials, int flip, uint32_t* status, int lane) {
+    const int64_t T = per / 4;
+    if (T == 0) return;
+    HalfCursor cur = cursor_begin(e, carry, lane);
+    for (int64_t pk = 0; pk < T; ++pk) {
+        const int32_t r = knuth_packet(cur, thr, lane, status);
+        if (lane < 4) o[4 * pk + lane] = flip ? trials - r : r;
+    }
+    cursor_end(e, cur);
+}
+
+// BinomialGen, trials < 25 (Discrete.h:996-1006): up to 16 independent half
+// draws compared at once, counted per lane with a ballot.
+__device__ void op_binomial_small(Eng& e, uint64_t* carry, int32_t* o, int64_t per, int32_t trials,
+                                  float pp, int lane) {
+    const int64_t T = per / 4;
+    if (T == 0) return;
+    const int i = lane >> 2, j = lane & 3;
+    HalfCursor cur = cursor_begin(e, carry, lane);
+    const uint64_t jmask = 0x1111111111111111ull << j;
+    for (int64_t pk = 0; pk < T; ++pk) {
+        int32_t res = trials;
+        int left = trials;
+        while (left > 0) {
+            cur.ensure(1, lane);
+            int take = left < 16 ? left : 16;
+            const int a = cur.avail();
+            take = take < a ? take : a;
+            bool c = false;
+            if (i < take) c = pp <= ur_f32(cur.word32(i, j));
+            res -= __popcll(__ballot(c) & jmask);
+            cur.advance(take);
+            left -= take;
+        }
+        if (lane < 4) o[4 * pk + lane] = res;
+    }
+    cursor_end(e, cur);
+}
+
+// BinomialVGen<int32_t>::packetOp (Discrete.h:1124-1192). Tangent iterations
+// are evaluated 16 at a time (thread 4i+j runs iteration i of lane j); each
+// open lane takes its first accepted candidate and the packet consumes the
+// iterations up to the slowest lane, as the sequential loop does.
+__device__ void op_binomial_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t* o, int64_t per,
+                                const int32_t* trials, const float* prob, uint32_t* status,
+                                int lane) {
+    const int64_t T = per / 4;
+    if (T == 0) return;
+    const int i = lane >> 2, j = lane & 3;
+    HalfCursor cur = cursor_begin(e, carry, lane);
+    const uint64_t jmask = 0x1111111111111111ull << j;
+    for (int64_t pk = 0; pk < T; ++pk) {
+        const int32_t n = trials[4 * pk + j];
+        const float p = prob[4 * pk + j];
+        const float psmall = (p < 1.0f - p) ? p : 1.0f - p;  // pmin(p, 1-p)
+        const float ptr = (float)n;
+        const float pmean = ptr * psmall;
+        const float pg1 = plgamma_approx(ptr + 1.0f);
+        const float psq = sqrtf((2.0f * pmean) * (1.0f - psmall));
+        const float plp = plog(psmall);
+        const float plq = plog(1.0f - psmall);
+        const bool valid = pmean < 1.0f;
+        float valid_res = 0.0f;
+        const uint64_t V = __ballot(valid) & 0xFull;
+        if (V) {
+            const float nem = pexp(-pmean);
+            float val = 1.0f;
+            int32_t res = 0;
+            for (int it = 0;; ++it) {
+                cur.ensure(1, lane);
+                const float u = ur_f32(cur.word32(0, j));
+                cur.advance(1);
+                val = val * u;
+                const bool c = (nem < val) && valid;
+                if (!(__ballot(c) & 0xFull)) break;
+                res += c ? 1 : 0;
+                if (it > kLockstepCap) {
+                    if (lane == 0) atomicOr(status, kStatusIterCap);
+                    break;
+                }
+            }
+            valid_res = (float)res;
+        }
+        if (V != 0xFull) {
+            uint64_t done = V;
+            for (int it = 0;; it += 16) {
+                cur.ensure(2, lane);
+                const int ni_avail = cur.avail() / 2;
+                const int ni = ni_avail < 16 ? ni_avail : 16;
+                float fres = 0.0f;
+                bool fv = false;
+                if (i < ni) {
+                    float s, c;
+                    psincos(ERD_F(kPi) * ur_f32(cur.word32(2 * i, j)), s, c);
+                    const float ys = s / c;
+                    fres = truncf(psq * ys + pmean);
+                    const float pp1 = (ERD_F(1.2) * psq) * (1.0f + ys * ys);
+                    const float pp2 = pexp((((pg1 - plgamma_approx(fres + 1.0f)) -
+                                             plgamma_approx((ptr + 1.0f) - fres)) +
+                                            fres * plp) +
+                                           (ptr - fres) * plq);
+                    const bool c1 = (0.0f <= fres) && (fres <= ptr);
+                    const float u2 = ur_f32(cur.word32(2 * i + 1, j));
+                    fv = c1 && (u2 <= pp1 * pp2);
+                }
+                const uint64_t A = __ballot(fv);
+                int need = 0;
+                bool all = true;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (done & (1ull << q)) continue;
+                    const uint64_t m = A & (0x1111111111111111ull << q);
+                    if (m) {
+                        const int first = (__ffsll((long long)m) - 1) >> 2;
+                        need = need > first + 1 ? need : first + 1;
+                    } else {
+                        all = false;
+                    }
+                }
+                if (!all) need = ni;
+                const uint64_t mine = A & jmask;
+                const bool open = !(done & (1ull << j));
+                if (open && mine && i == ((__ffsll((long long)mine) - 1) >> 2)) res_lds[j] = fres;
+                __syncthreads();
+                if (open && mine) valid_res = res_lds[j];
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (A & (0x1111111111111111ull << q)) done |= 1ull << q;
+                cur.advance(2 * need);
+                if (all) break;
+                if (it > kLockstepCap) {
+                    if (lane == 0) atomicOr(status, kStatusIterCap);
+                    break;
+                }
+            }
+        }
+        const int32_t k = (int32_t)valid_res;
+        if (lane < 4) o[4 * pk + j] = (p == psmall) ? k : n - k;
+    }
+    cursor_end(e, cur);
+}
+
+// ------------------------------------------------- gamma / Poisson VoP ---
+// The reference has no GammaVGen/PoissonVGen. These follow its BinomialVGen
+// design (Discrete.h:1124-1192): per-lane parameters in float, lockstep
+// iterations, each lane keeps its first accepted candidate. Speculative wave
+// evaluation as in op_binomial_vop.
+
+// Gamma, Packet8f: three full draws per iteration (ux, vx, u3); lanes with
+// alpha <= 1 run the GS step (NormalExp.h:536-558), others the tangent step
+// (580-603) on the first two draws. Thread 4i+l runs iteration i for float
+// lanes 2l and 2l+1.
+__device__ void op_gamma_vop(Eng& e, uint64_t* carry, float* res_lds, float* o, int64_t per,
+                             const float* alpha, const float* beta, uint32_t* status, int lane) {
+    const int64_t T = per / 8;
+    if (T == 0) return;
+    const int i = lane >> 2, l = lane & 3;
+    Cursor<4, 2> cur;
+    cur.mt = e.mt;
+    cur.carry = carry;
+    cur.pos = e.idx;
+    cur.nc = 0;
+    cur.used = false;
+    cur.twisted = &e.twisted;
+    for (int64_t pk = 0; pk < T; ++pk) {
+        float al[2], be[2], px[2], inv[2], am1[2], sq[2], res[2] = {0.f, 0.f};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            al[h] = alpha[8 * pk + 2 * l + h];
+            be[h] = beta[8 * pk + 2 * l + h];
+            px[h] = ERD_F(kE) / (al[h] + ERD_F(kE));
+            inv[h] = 1.0f / al[h];
+            am1[h] = al[h] - 1.0f;
+            sq[h] = sqrtf(2.0f * al[h] - 1.0f);
+        }
+        uint32_t done = 0;  // bit j = float lane j has its value
+        for (int it = 0;; it += 16) {
+            cur.ensure(3, lane);
+            const int ni_avail = cur.avail() / 3;
+            const int ni = ni_avail < 16 ? ni_avail : 16;
+            float xv[2] = {0.f, 0.f};
+            bool acc[2] = {false, false};
+            if (i < ni) {
+                const uint64_t w0 = cur.word(3 * i, l), w1 = cur.word(3 * i + 1, l),
+                               w2 = cur.word(3 * i + 2, l);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const float u1 = ur_f32(h ? hi32(w0) : lo32(w0));
+                    const float u2 = ur_f32(h ? hi32(w1) : lo32(w1));
+                    if (al[h] <= 1.0f) {
+                        const float vx = u2 + 1.1920928955078125e-07f / 8;
+                        const float lv = plog_pos(vx);
+                        const float x1 = pexp(inv[h] * lv);
+                        const float q1 = pexp(-x1);
+                        const float x2 = 1.0f - lv;
+                        const float q2 = pexp(plog_pos(x2) * am1[h]);
+                        const bool c = u1 < px[h];
+                        xv[h] = c ? x1 : x2;
+                        acc[h] = ur_f32(h ? hi32(w2) : lo32(w2)) < (c ? q1 : q2);
+                    } else {
+                        float ys, yc;
+                        psincos(ERD_F(kPi) * u1, ys, yc);
+                        const float yx = ys / yc;
+                        xv[h] = sq[h] * yx + am1[h];
+                        const float ub = (yx * yx + 1.0f) * pexp(am1[h] * plog(xv[h] / am1[h]) - yx * sq[h]);
+                        acc[h] = (0.0f < xv[h]) && (u2 <= ub);
+                    }
+                }
+            }
+            const uint64_t A0 = __ballot(acc[0]), A1 = __ballot(acc[1]);
+            int need = 0;
+            bool all = true;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (done & (1u << j)) continue;
+                const uint64_t m = ((j & 1) ? A1 : A0) & (0x1111111111111111ull << (j >> 1));
+                if (m) {
+                    const int first = (__ffsll((long long)m) - 1) >> 2;
+                    need = need > first + 1 ? need : first + 1;
+                } else {
+                    all = false;
+                }
+            }
+            if (!all) need = ni;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int j = 2 * l + h;
+                const uint64_t m = (h ? A1 : A0) & (0x1111111111111111ull << l);
+                if (!(done & (1u << j)) && m && i == ((__ffsll((long long)m) - 1) >> 2))
+                    res_lds[j] = be[h] * xv[h];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint64_t m = ((j & 1) ? A1 : A0) & (0x1111111111111111ull << (j >> 1));
+                if (m) done |= 1u << j;
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) res[h] = res_lds[2 * l + h];
+            __syncthreads();
+            cur.advance(3 * need);
+            if (all) break;
+            if (it > kLockstepCap) {
+                if (lane == 0) atomicOr(status, kStatusIterCap);
+                break;
+            }
+        }
+        if (i == 0) *reinterpret_cast<float2*>(o + 8 * pk + 2 * l) = make_float2(res[0], res[1]);
+    }
+    __syncthreads();
+    e.idx = cur.pos;
+}
+
+// Poisson, Packet4f via half draws: Knuth lockstep for lanes with mean < 12,
+// then tangent lockstep for the others with float constants.
+__device__ void op_poisson_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t* o, int64_t per,
+                               const float* mean, uint32_t* status, int lane) {
+    const int64_t T = per / 4;
+    if (T == 0) return;
+    const int i = lane >> 2, j = lane & 3;
+    HalfCursor cur = cursor_begin(e, carry, lane);
+    const uint64_t jmask = 0x1111111111111111ull << j;
+    for (int64_t pk = 0; pk < T; ++pk) {
+        const float m = mean[4 * pk + j];
+        const bool small = m < 12.0f;
+        const float nem = pexp(-m);
+        const float psq = sqrtf(2.0f * m);
+        const float plm = plog(m);
+        const float pg1 = m * plm - plgamma_approx(m + 1.0f);
+        float res = 0.0f;
+        const uint64_t V = __ballot(small) & 0xFull;
+        if (V) {
+            float val = 1.0f;
+            int32_t cnt = 0;
+            for (int it = 0;; ++it) {
+                cur.ensure(1, lane);
+                const float u = ur_f32(cur.word32(0, j));
+                cur.advance(1);
+                val = val * u;
+                const bool c = (nem < val) && small;
+                if (!(__ballot(c) & 0xFull)) break;
+                cnt += c ? 1 : 0;
+                if (it > kLockstepCap) {
+                    if (lane == 0) atomicOr(status, kStatusIterCap);
+                    break;
+                }
+            }
+            res = (float)cnt;
+        }
+        if (V != 0xFull) {
+            uint64_t done = V;
+            for (int it = 0;; it += 16) {
+                cur.ensure(2, lane);
+                const int ni_avail = cur.avail() / 2;
+                const int ni = ni_avail < 16 ? ni_avail : 16;
+                float fres = 0.0f;
+                bool fv = false;
+                if (i < ni) {
+                    float s, c;
+                    psincos(ERD_F(kPi) * ur_f32(cur.word32(2 * i, j)), s, c);
+                    const float yx = s / c;
+                    fres = truncf(psq * yx + m);
+                    const float p1 = (yx * yx + 1.0f) * ERD_F(0.9);
+                    const float p2 = pexp((fres * plm - plgamma_approx(fres + 1.0f)) - pg1);
+                    const float u2 = ur_f32(cur.word32(2 * i + 1, j));
+                    fv = (0.0f <= fres) && (u2 <= p1 * p2);
+                }
+                const uint64_t A = __ballot(fv);
+                int need = 0;
+                bool all = true;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (done & (1ull << q)) continue;
+                    const uint64_t mq = A & (0x1111111111111111ull << q);
+                    if (mq) {
+                        const int first = (__ffsll((long long)mq) - 1) >> 2;
+                        need = need > first + 1 ? need : first + 1;
+                    } else {
+                        all = false;
+                    }
+                }
+                if (!all) need = ni;
+                const uint64_t mine = A & jmask;
+                const bool open = !(done & (1ull << j));
+                if (open && mine && i == ((__ffsll((long long)mine) - 1) >> 2)) res_lds[j] = fres;
+                __syncthreads();
+                if (open && mine) res = res_lds[j];
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (A & (0x1111111111111111ull << q)) done |= 1ull << q;
+                cur.advance(2 * need);
+                if (all) break;
+                if (it > kLockstepCap) {
+                    if (lane == 0) atomicOr(status, kStatusIterCap);
+                    break;
+                }
+            }
+        }
+        if (lane < 4) o[4 * pk + j] = (int32_t)res;
+    }
+    cursor_end(e, cur);
+}
+
+}  // namespace erd

@@ -1,0 +1,41 @@
+// Segment programs: a list of fills that run one after another on the same
+// engine, in one launch. Used by the multivariate generators (a Wishart or
+// Dirichlet draw is a sequence of normal/gamma fills on one engine) and by the
+// serving loop (a client's queued requests). Plain data, shared by host and
+// device code.
+#pragma once
+#include <stdint.h>
+
+namespace erd {
+
+enum SegKind : int32_t {
+    SEG_U64 = 0,
+    SEG_UNIFORM_F32 = 1,     // d[0]=a, d[1]=b, i0=affine
+    SEG_UNIFORM_F64 = 2,     // d[0]=a, d[1]=b, i0=affine
+    SEG_STD_NORMAL_F32 = 3,
+    SEG_NORMAL_F32 = 4,      // d[0]=mean, d[1]=sd
+    SEG_NORMAL_VOP_F32 = 5,  // in0=mean, in1=sd
+    SEG_EXPONENTIAL_F32 = 6, // d[0]=lambda
+    SEG_CAUCHY_F32 = 7,      // d[0]=a, d[1]=b
+    SEG_STUDENT_T_F32 = 8,   // d[0]=n
+    SEG_GAMMA_F32 = 9,       // d[0]=alpha, d[1]=beta
+    SEG_POISSON_I32 = 10,    // d[0..4] = PoissonGen constants
+    SEG_BINOMIAL_I32 = 11,   // i0=trials, d[0..7] = BinomialGen constants
+    SEG_BINOMIAL_VOP_I32 = 12,  // in0=trials (int32), in1=p (float)
+    SEG_GAMMA_VOP_F32 = 13,  // in0=alpha, in1=beta
+    SEG_POISSON_VOP_I32 = 14,   // in0=mean (float)
+    SEG_DISCARD = 15,        // count = draws to skip
+    SEG_KIND_COUNT
+};
+
+struct Segment {
+    int32_t kind;
+    int32_t i0;        // integer parameter (trials, affine flag, ...)
+    int64_t count;     // outputs (draws for SEG_DISCARD)
+    void* out;         // device output (count elements)
+    const void* in0;   // per-element parameters (VoP kinds)
+    const void* in1;
+    double d[8];       // scalar parameters, already in the form the kernel uses
+};
+
+}  // namespace erd

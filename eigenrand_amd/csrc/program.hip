@@ -1,0 +1,246 @@
+// Segment-program kernel: one workgroup (one wave) per engine group runs that
+// engine's list of fills in order, with the engine resident in LDS for the
+// whole list. Each segment reproduces the corresponding reference fill
+// exactly (same ops as the single-fill kernels), so a program is equivalent
+// to issuing its fills one after another on that engine.
+#include <cmath>
+#include <algorithm>
+#include "erd_device.hpp"
+#include "erd_dists.hpp"
+#include "erd_kernels.hpp"
+#include "erd_ops.hpp"
+#include "erd_segment.hpp"
+
+namespace erd {
+
+union ProgramLds {
+    uint64_t carry_fixed[18 * kLanes];  // GammaIntF32<19>: 18 straddling draws
+    RejectLds<GammaGS> gs;
+    RejectLds<GammaTangent> gt;
+    RejectLds<PoissonTangent> pt;
+    RejectLds<BinomialTangent> bt;
+    struct {
+        uint64_t carry[2 * kLanes];
+        float res[8];
+    } vop;
+};
+
+template <int A>
+__device__ void run_gamma_int(Eng& e, ProgramLds& s, float* out, int64_t n, float beta, int lane) {
+    GammaIntF32<A> d{out, beta};
+    op_fixed(e, s.carry_fixed, n, d, lane);
+}
+
+__device__ void run_gamma(Eng& e, ProgramLds& s, float* out, int64_t n, float alpha, float beta,
+                          uint32_t* status, int lane) {
+    if (alpha < 1.0f) {
+        GammaGS pol{beta, (float)(kE / (alpha + kE)), 1 / alpha, alpha - 1};
+        op_reject(e, s.gs, out, n, pol, status, lane);
+        return;
+    }
+    if (alpha == 1.0f) {
+        ExponentialF32 d{out, 1.0f, beta, 1};
+        op_fixed(e, s.carry_fixed, n, d, lane);
+        return;
+    }
+    const int count = (int)alpha;
+    if ((float)count == alpha && count < 20) {
+        switch (count) {
+            case 2: run_gamma_int<2>(e, s, out, n, beta, lane); return;
+            case 3: run_gamma_int<3>(e, s, out, n, beta, lane); return;
+            case 4: run_gamma_int<4>(e, s, out, n, beta, lane); return;
+            case 5: run_gamma_int<5>(e, s, out, n, beta, lane); return;
+            case 6: run_gamma_int<6>(e, s, out, n, beta, lane); return;
+            case 7: run_gamma_int<7>(e, s, out, n, beta, lane); return;
+            case 8: run_gamma_int<8>(e, s, out, n, beta, lane); return;
+            case 9: run_gamma_int<9>(e, s, out, n, beta, lane); return;
+            case 10: run_gamma_int<10>(e, s, out, n, beta, lane); return;
+            case 11: run_gamma_int<11>(e, s, out, n, beta, lane); return;
+            case 12: run_gamma_int<12>(e, s, out, n, beta, lane); return;
+            case 13: run_gamma_int<13>(e, s, out, n, beta, lane); return;
+            case 14: run_gamma_int<14>(e, s, out, n, beta, lane); return;
+            case 15: run_gamma_int<15>(e, s, out, n, beta, lane); return;
+            case 16: run_gamma_int<16>(e, s, out, n, beta, lane); return;
+            case 17: run_gamma_int<17>(e, s, out, n, beta, lane); return;
+            case 18: run_gamma_int<18>(e, s, out, n, beta, lane); return;
+            default: run_gamma_int<19>(e, s, out, n, beta, lane); return;
+        }
+    }
+    GammaTangent pol{beta, sqrtf(2 * alpha - 1), alpha - 1};
+    op_reject(e, s.gt, out, n, pol, status, lane);
+}
+
+// discard(n) on the resident engine: only the block position and twists.
+__device__ void run_discard(Eng& e, int64_t calls, int lane) {
+    int64_t end = e.idx + calls;
+    if (end > kN) {
+        const int64_t twists = (end - 1) / kN;
+        for (int64_t t = 0; t < twists; ++t) twist_lds(e.mt, lane);
+        e.twisted = true;
+        end -= twists * kN;
+    }
+    e.idx = (int)end;
+}
+
+__device__ void run_segment(Eng& e, ProgramLds& s, const Segment& sg, uint32_t* status, int lane) {
+    const int64_t n = sg.count;
+    switch (sg.kind) {
+        case SEG_U64: {
+            RawU64 d{(uint64_t*)sg.out};
+            op_fixed(e, s.carry_fixed, n, d, lane);
+            break;
+        }
+        case SEG_UNIFORM_F32: {
+            UniformF32 d{(float*)sg.out, (float)sg.d[0], (float)sg.d[1] - (float)sg.d[0], sg.i0};
+            op_fixed(e, s.carry_fixed, n, d, lane);
+            break;
+        }
+        case SEG_UNIFORM_F64: {
+            UniformF64 d{(double*)sg.out, sg.d[0], sg.d[1] - sg.d[0], sg.i0};
+            op_fixed(e, s.carry_fixed, n, d, lane);
+            break;
+        }
+        case SEG_STD_NORMAL_F32: {
+            NormalF32<0> d{(float*)sg.out, 0.f, 1.f, nullptr, nullptr};
+            op_fixed(e, s.carry_fixed, n, d, lane);
+            break;
+        }
+        case SEG_NORMAL_F32: {
+            NormalF32<1> d{(float*)sg.out, (float)sg.d[0], (float)sg.d[1], nullptr, nullptr};
+            op_fixed(e, s.carry_fixed, n, d, lane);
+            break;
+        }
+        case SEG_NORMAL_VOP_F32: {
+            NormalF32<2> d{(float*)sg.out, 0.f, 1.f, (const float*)sg.in0, (const float*)sg.in1};
+            op_fixed(e, s.carry_fixed, n, d, lane);
+            break;
+        }
+        case SEG_EXPONENTIAL_F32: {
+            ExponentialF32 d{(float*)sg.out, (float)sg.d[0], 1.0f, 0};
+            op_fixed(e, s.carry_fixed, n, d, lane);
+            break;
+        }
+        case SEG_CAUCHY_F32: {
+            CauchyF32 d{(float*)sg.out, (float)sg.d[0], (float)sg.d[1]};
+            op_fixed(e, s.carry_fixed, n, d, lane);
+            break;
+        }
+        case SEG_STUDENT_T_F32: {
+            const float nu = (float)sg.d[0];
+            StudentTF32 d{(float*)sg.out, nu, -2 / nu};
+            op_fixed(e, s.carry_fixed, n, d, lane);
+            break;
+        }
+        case SEG_GAMMA_F32:
+            run_gamma(e, s, (float*)sg.out, n, (float)sg.d[0], (float)sg.d[1], status, lane);
+            break;
+        case SEG_POISSON_I32: {
+            // d: mean, ne_mean, sqrt_tmean, log_mean, g1
+            if (sg.d[0] < 12) {
+                op_knuth(e, s.vop.carry, (int32_t*)sg.out, n, (float)sg.d[1], 0, 0, status, lane);
+            } else {
+                PoissonTangent pol{(float)sg.d[2], (float)sg.d[0], (float)sg.d[3], (float)sg.d[4]};
+                op_reject(e, s.pt, (int32_t*)sg.out, n, pol, status, lane);
+            }
+            break;
+        }
+        case SEG_BINOMIAL_I32: {
+            // i0: trials; d: p, small_p, mean, g1, sqrt_v, log_small_p, log_small_q, e^-mean
+            const int32_t trials = sg.i0;
+            const int flip = (sg.d[0] == sg.d[1]) ? 0 : 1;
+            if (trials < 25) {
+                op_binomial_small(e, s.vop.carry, (int32_t*)sg.out, n, trials, (float)sg.d[0], lane);
+            } else if (sg.d[2] < 1.0) {
+                op_knuth(e, s.vop.carry, (int32_t*)sg.out, n, (float)sg.d[7], trials, flip, status, lane);
+            } else {
+                BinomialTangent pol{(float)trials, (float)sg.d[4], (float)sg.d[2], (float)sg.d[5],
+                                    (float)sg.d[6], (float)sg.d[3], trials, flip};
+                op_reject(e, s.bt, (int32_t*)sg.out, n, pol, status, lane);
+            }
+            break;
+        }
+        case SEG_BINOMIAL_VOP_I32:
+            op_binomial_vop(e, s.vop.carry, s.vop.res, (int32_t*)sg.out, n, (const int32_t*)sg.in0,
+                            (const float*)sg.in1, status, lane);
+            break;
+        case SEG_GAMMA_VOP_F32:
+            op_gamma_vop(e, s.vop.carry, s.vop.res, (float*)sg.out, n, (const float*)sg.in0,
+                         (const float*)sg.in1, status, lane);
+            break;
+        case SEG_POISSON_VOP_I32:
+            op_poisson_vop(e, s.vop.carry, s.vop.res, (int32_t*)sg.out, n, (const float*)sg.in0, status,
+                           lane);
+            break;
+        case SEG_DISCARD:
+            run_discard(e, n, lane);
+            break;
+        default:
+            if (lane == 0) atomicOr(status, kStatusBadSegment);
+            break;
+    }
+    __syncthreads();
+}
+
+// Group b runs segments [begin[b], begin[b+1]) on engine engine_of[b].
+__global__ __launch_bounds__(64) void k_program(EngineState* __restrict__ states,
+                                                const int32_t* __restrict__ engine_of,
+                                                const int32_t* __restrict__ begin,
+                                                const Segment* __restrict__ segs,
+                                                uint32_t* __restrict__ status) {
+    __shared__ uint64_t mt[kWords];
+    __shared__ ProgramLds s;
+    const int lane = threadIdx.x;
+    const int b = blockIdx.x;
+    EngineState* st = states + engine_of[b];
+    Eng e;
+    eng_load(e, mt, st, lane);
+    for (int k = begin[b]; k < begin[b + 1]; ++k) run_segment(e, s, segs[k], status, lane);
+    eng_store(e, st, lane);
+}
+
+hipError_t launch_program(EngineState* st, int32_t n_groups, const int32_t* d_engine_of,
+                          const int32_t* d_begin, const Segment* d_segs, uint32_t* status,
+                          hipStream_t s) {
+    if (n_groups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_program, dim3((unsigned)n_groups), dim3(64), 0, s, st, d_engine_of, d_begin,
+                       d_segs, status);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ host: segments ---
+// Constructor constants of the reference generators, computed on the host in
+// double exactly as the reference constructors do.
+
+Segment seg_poisson(int32_t* out, int64_t count, double mean) {
+    Segment s{};
+    s.kind = SEG_POISSON_I32;
+    s.count = count;
+    s.out = out;
+    s.d[0] = mean;                                   // PoissonGen (Discrete.h:807-813)
+    s.d[1] = std::exp(-mean);
+    s.d[2] = std::sqrt(2 * mean);
+    s.d[3] = std::log(mean);
+    s.d[4] = mean * s.d[3] - std::lgamma(mean + 1);
+    return s;
+}
+
+Segment seg_binomial(int32_t* out, int64_t count, int32_t trials, double p) {
+    Segment s{};
+    s.kind = SEG_BINOMIAL_I32;
+    s.count = count;
+    s.out = out;
+    s.i0 = trials;                                   // BinomialGen (Discrete.h:929-941)
+    s.d[0] = p;
+    s.d[1] = std::min(p, 1 - p);
+    s.d[2] = trials * s.d[1];
+    s.d[7] = std::exp(-s.d[2]);                      // PoissonGen{mean}.ne_mean
+    if (!(trials < 25 || s.d[2] < 1.0)) {
+        s.d[3] = std::lgamma(trials + 1);
+        s.d[4] = std::sqrt(2 * s.d[2] * (1 - s.d[1]));
+        s.d[5] = std::log(s.d[1]);
+        s.d[6] = std::log(1 - s.d[1]);
+    }
+    return s;
+}
+
+}  // namespace erd

@@ -123,6 +123,55 @@ int erd_fill_binomial_i32(erd_engines* e, int32_t* out, int64_t per_engine, int3
 int erd_fill_binomial_vop_i32(erd_engines* e, int32_t* out, int64_t per_engine, const int32_t* trials,
                               const float* p, void* stream);
 
+/* Vectorize-over-params gamma and Poisson. The reference has no GammaVGen or
+ * PoissonVGen (its VoP generators stop at BinomialVGen); these follow the
+ * BinomialVGen design (Discrete.h:1124-1192): per-element parameters in float,
+ * lockstep candidate iterations, each lane keeps its first accepted candidate.
+ * Gamma: Packet8f, 3 full draws per iteration, GS step for alpha <= 1, tangent
+ * step otherwise. Poisson: Packet4f via half draws, Knuth lockstep for
+ * mean < 12, tangent lockstep otherwise. Device arrays of n*per_engine. */
+int erd_fill_gamma_vop_f32(erd_engines* e, float* out, int64_t per_engine, const float* alpha,
+                           const float* beta, void* stream);
+int erd_fill_poisson_vop_i32(erd_engines* e, int32_t* out, int64_t per_engine, const float* mean,
+                             void* stream);
+
+/* ------------------------------------------------------ batched requests --- */
+/* Heterogeneous fills in one launch. Requests naming the same engine run on
+ * it in array order; different engines run concurrently. Each request gives
+ * exactly the result of the corresponding erd_fill_* call on a one-engine view
+ * of `engine` issued in that order (the reference equivalent: successive
+ * free-function fills on one engine object). */
+typedef enum erd_dist {
+    ERD_DIST_U64 = 0,           /* count = 4*draws                           */
+    ERD_DIST_UNIFORM_F32 = 1,   /* a=min, b=max                              */
+    ERD_DIST_UNIFORM_F64 = 2,   /* a=min, b=max                              */
+    ERD_DIST_STD_NORMAL_F32 = 3,
+    ERD_DIST_NORMAL_F32 = 4,    /* a=mean, b=stdev                           */
+    ERD_DIST_NORMAL_VOP_F32 = 5,/* in0=mean[], in1=stdev[]                   */
+    ERD_DIST_EXPONENTIAL_F32 = 6, /* a=lambda                                */
+    ERD_DIST_CAUCHY_F32 = 7,    /* a, b                                      */
+    ERD_DIST_STUDENT_T_F32 = 8, /* a=n                                       */
+    ERD_DIST_GAMMA_F32 = 9,     /* a=alpha, b=beta                           */
+    ERD_DIST_POISSON_I32 = 10,  /* a=mean                                    */
+    ERD_DIST_BINOMIAL_I32 = 11, /* a=trials, b=p                             */
+    ERD_DIST_BINOMIAL_VOP_I32 = 12, /* in0=trials[] (int32), in1=p[] (float) */
+    ERD_DIST_GAMMA_VOP_F32 = 13,    /* in0=alpha[], in1=beta[]               */
+    ERD_DIST_POISSON_VOP_I32 = 14,  /* in0=mean[] (float)                    */
+    ERD_DIST_DISCARD = 15           /* count = draws skipped, out unused      */
+} erd_dist;
+
+typedef struct erd_request {
+    int32_t dist;       /* erd_dist                                       */
+    int32_t engine;     /* index into the engine set                      */
+    int64_t count;      /* outputs (a multiple of the packet width)       */
+    void* out;          /* device output                                  */
+    const void* in0;    /* device per-element parameters (VoP kinds)      */
+    const void* in1;
+    double a, b;        /* scalar parameters                              */
+} erd_request;
+
+int erd_run_requests(erd_engines* e, const erd_request* requests, int64_t n_requests, void* stream);
+
 /* -------------------------------------------------------- device memory --- */
 /* Small helpers so callers without another GPU runtime can use the library. */
 int erd_device_alloc(void** ptr, size_t bytes);

@@ -684,3 +684,143 @@ int64_t orc_compress_stream(int W, const float* values, const uint8_t* masks, in
     return -1;
 }
 }
+
+// ----------------------------------------------------------------------------
+// Vectorize-over-params gamma and Poisson. The reference has no GammaVGen or
+// PoissonVGen; these follow the design of its BinomialVGen<int32_t>
+// (Discrete.h:1124-1192): per-lane parameters computed in float, lockstep
+// iterations, each lane keeps its first accepted candidate.
+namespace {
+
+// Gamma: Packet8f (full draws). Every iteration draws three packets (ux, vx,
+// u3 as in the GS branch, NormalExp.h:536-558); lanes with alpha <= 1 use
+// them as GS does, lanes with alpha > 1 run the tangent step (580-603) on the
+// first two. Output beta * x.
+void gammav8(Vmt& e, const float alpha[8], const float beta[8], float out[8]) {
+    float px[8], inv[8], am1[8], sq[8], res[8] = {0};
+    bool done[8];
+    for (int j = 0; j < 8; ++j) {
+        px[j] = F(kE) / (alpha[j] + F(kE));
+        inv[j] = 1.0f / alpha[j];
+        am1[j] = alpha[j] - 1.0f;
+        sq[j] = std::sqrt(2.0f * alpha[j] - 1.0f);
+        done[j] = false;
+    }
+    for (;;) {
+        float u1[8], u2[8], u3[8];
+        ur8(e, u1);
+        ur8(e, u2);
+        ur8(e, u3);
+        bool all = true;
+        for (int j = 0; j < 8; ++j) {
+            float x;
+            bool acc;
+            if (alpha[j] <= 1.0f) {
+                float vx = u2[j] + 1.1920928955078125e-07f / 8;
+                float lv = plog(vx);
+                float x1 = pexp(inv[j] * lv);
+                float q1 = pexp(-x1);
+                float x2 = 1.0f - lv;
+                float q2 = pexp(plog(x2) * am1[j]);
+                bool c = u1[j] < px[j];
+                x = c ? x1 : x2;
+                acc = u3[j] < (c ? q1 : q2);
+            } else {
+                float ys, yc;
+                psincos(kPiF * u1[j], ys, yc);
+                float yx = ys / yc;
+                x = sq[j] * yx + am1[j];
+                float ub = (yx * yx + 1.0f) * pexp(am1[j] * plog(x / am1[j]) - yx * sq[j]);
+                acc = (0.0f < x) && (u2[j] <= ub);
+            }
+            if (acc && !done[j]) { res[j] = beta[j] * x; done[j] = true; }
+            all = all && done[j];
+        }
+        if (all) break;
+    }
+    std::memcpy(out, res, 32);
+}
+
+// Poisson: Packet4f (half draws), per-lane mean m. Lanes with m < 12 run the
+// Knuth product loop in lockstep (one half draw per iteration); then the
+// others run the tangent step with float constants (two half draws per
+// iteration): psq = sqrt(2m), plm = log m, pg1 = m*plm - lgamma_approx(m+1).
+void poissonv4(Vmt& e, const float mean[4], int32_t out[4]) {
+    bool small[4], any_small = false, all_small = true;
+    float nem[4], psq[4], plm[4], pg1[4], res[4] = {0, 0, 0, 0};
+    for (int j = 0; j < 4; ++j) {
+        small[j] = mean[j] < 12.0f;
+        any_small |= small[j];
+        all_small &= small[j];
+        nem[j] = pexp(-mean[j]);
+        psq[j] = std::sqrt(2.0f * mean[j]);
+        plm[j] = plog(mean[j]);
+        pg1[j] = mean[j] * plm[j] - plgamma_approx(mean[j] + 1.0f);
+    }
+    if (any_small) {
+        float val[4] = {1, 1, 1, 1};
+        int32_t cnt[4] = {0, 0, 0, 0};
+        for (;;) {
+            float u[4];
+            ur4(e, u);
+            bool any = false, c[4];
+            for (int j = 0; j < 4; ++j) {
+                val[j] = val[j] * u[j];
+                c[j] = (nem[j] < val[j]) && small[j];
+                any |= c[j];
+            }
+            if (!any) break;
+            for (int j = 0; j < 4; ++j) cnt[j] += c[j] ? 1 : 0;
+        }
+        for (int j = 0; j < 4; ++j) res[j] = (float)cnt[j];
+    }
+    if (!all_small) {
+        bool done[4];
+        for (int j = 0; j < 4; ++j) done[j] = small[j];
+        for (;;) {
+            float u1[4], u2[4], fres[4], p12[4];
+            bool c1[4];
+            ur4(e, u1);
+            for (int j = 0; j < 4; ++j) {
+                float s, c;
+                psincos(kPiF * u1[j], s, c);
+                float yx = s / c;
+                fres[j] = ptruncate(psq[j] * yx + mean[j]);
+                float p1 = (yx * yx + 1.0f) * F(0.9);
+                float p2 = pexp((fres[j] * plm[j] - plgamma_approx(fres[j] + 1.0f)) - pg1[j]);
+                c1[j] = 0.0f <= fres[j];
+                p12[j] = p1 * p2;
+            }
+            ur4(e, u2);
+            bool all = true;
+            for (int j = 0; j < 4; ++j) {
+                bool acc = c1[j] && (u2[j] <= p12[j]);
+                if (acc && !done[j]) { res[j] = fres[j]; done[j] = true; }
+                all = all && done[j];
+            }
+            if (all) break;
+        }
+    }
+    for (int j = 0; j < 4; ++j) out[j] = (int32_t)res[j];
+}
+
+}  // namespace
+
+extern "C" {
+void orc_fill_gamma_vop_f32(void* states, int64_t n_engines, float* out, int64_t per,
+                            const float* alpha, const float* beta, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        int64_t b = g * per;
+        for (int64_t i = 0; i < per; i += 8) gammav8(e[g], alpha + b + i, beta + b + i, out + b + i);
+    });
+}
+void orc_fill_poisson_vop_i32(void* states, int64_t n_engines, int32_t* out, int64_t per,
+                              const float* mean, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        int64_t b = g * per;
+        for (int64_t i = 0; i < per; i += 4) poissonv4(e[g], mean + b + i, out + b + i);
+    });
+}
+}

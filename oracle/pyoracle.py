@@ -196,3 +196,21 @@ def compress_stream(W, values, masks, max_packets):
     f.restype = C.c_int64
     used = f(C.c_int(W), _p(values), _p(masks), I64(n), _p(out), I64(max_packets))
     return out.reshape(max_packets, W), used
+
+
+def fill_gamma_vop_f32(e, alpha, beta, threads=1):
+    alpha = np.ascontiguousarray(alpha, dtype=np.float32)
+    beta = np.ascontiguousarray(beta, dtype=np.float32)
+    per = alpha.size // len(e)
+    out = np.empty(alpha.size, dtype=np.float32)
+    lib().orc_fill_gamma_vop_f32(_p(e.raw), I64(len(e)), _p(out), I64(per), _p(alpha), _p(beta),
+                                 C.c_int(threads))
+    return out
+
+
+def fill_poisson_vop_i32(e, mean, threads=1):
+    mean = np.ascontiguousarray(mean, dtype=np.float32)
+    per = mean.size // len(e)
+    out = np.empty(mean.size, dtype=np.int32)
+    lib().orc_fill_poisson_vop_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), _p(mean), C.c_int(threads))
+    return out
