@@ -27,7 +27,9 @@ __all__ = [
     "Vmt19937_64", "ErdError", "InvalidArgument", "STATE_BYTES",
     "randbits_u64", "uniform_real", "uniform_real_f64", "normal", "normal_vop",
     "exponential", "cauchy", "student_t", "gamma", "chi_squared", "poisson", "binomial",
-    "binomial_vop", "gamma_vop", "poisson_vop", "run_requests",
+    "binomial_vop", "gamma_vop", "poisson_vop", "run_requests", "cholesky_batch",
+    "mvnormal_batch", "wishart_batch", "dirichlet_batch", "MvNormalGen", "WishartGen",
+    "DirichletGen", "alias_table", "discrete", "multinomial_batch", "MultinomialGen",
 ]
 
 STATE_BYTES = 10016  # erd_engine_state_bytes()
@@ -288,3 +290,205 @@ def run_requests(urng, requests, stream=None):
         arr[i].a = float(r.get("a", 0.0))
         arr[i].b = float(r.get("b", 0.0))
     _lib.call("erd_run_requests", urng.handle, C.cast(arr, C.c_void_p), len(requests), _stream_ptr(stream))
+
+
+# ------------------------------------------------------------ multivariate ---
+def _concat_f64(mats):
+    import torch
+    flat = [np.asarray(m, dtype=np.float64).reshape(-1, order="F") for m in mats]
+    return torch.from_numpy(np.concatenate(flat)).cuda()
+
+
+def cholesky_batch(mats, stream=None):
+    """FP64 Cholesky of each matrix on the GPU; returns (list of L, info)."""
+    torch = _torch()
+    dims = np.array([np.asarray(m).shape[0] for m in mats], dtype=np.int32)
+    a = _concat_f64(mats)
+    info = np.zeros(len(mats), dtype=np.int32)
+    rc = _lib.load().erd_cholesky_f64_batch(len(mats), dims.ctypes.data_as(C.c_void_p), _ptr(a),
+                                            info.ctypes.data_as(C.c_void_p), _stream_ptr(stream))
+    torch.cuda.synchronize()
+    out, off = [], 0
+    flat = a.cpu().numpy()
+    for d in dims:
+        out.append(flat[off:off + d * d].reshape(d, d, order="F"))
+        off += d * d
+    return out, info, rc
+
+
+def mvnormal_batch(urng, covs, samples, means=None, return_z=False, stream=None):
+    """Batched MvNormalGen<float>(mean_i, cov_i).generate(engine_i, samples_i).
+
+    Instance i uses engine i of `urng`. Returns a list of (d_i, samples_i)
+    float32 CUDA tensors (and the standard normals Z_i if return_z)."""
+    torch = _torch()
+    n = len(covs)
+    dims = np.array([np.asarray(c).shape[0] for c in covs], dtype=np.int32)
+    samples = np.broadcast_to(np.asarray(samples, dtype=np.int64), (n,)).copy()
+    cov = _concat_f64(covs)
+    mean = None
+    if means is not None:
+        mean = torch.from_numpy(np.concatenate([np.asarray(m, np.float32).ravel() for m in means])).cuda()
+    total = int((dims.astype(np.int64) * samples).sum())
+    out = torch.empty(total, dtype=torch.float32, device="cuda")
+    z = torch.empty(total, dtype=torch.float32, device="cuda") if return_z else None
+    _lib.call("erd_mvnormal_batch", urng.handle, n, dims.ctypes.data_as(C.c_void_p),
+              samples.ctypes.data_as(C.c_void_p), _ptr(cov), _ptr(mean), _ptr(out), _ptr(z),
+              _stream_ptr(stream))
+    xs, zs, off = [], [], 0
+    for d, m in zip(dims, samples):
+        k = int(d) * int(m)
+        xs.append(out[off:off + k].view(int(m), int(d)).t())
+        if return_z:
+            zs.append(z[off:off + k].view(int(m), int(d)).t())
+        off += k
+    return (xs, zs) if return_z else xs
+
+
+def wishart_batch(urng, scales, df, samples, stream=None):
+    """Batched WishartGen<float>(df_i, scale_i).generate(engine_i, samples_i);
+    each result is (d_i, d_i * samples_i) column-major like the reference."""
+    torch = _torch()
+    n = len(scales)
+    dims = np.array([np.asarray(c).shape[0] for c in scales], dtype=np.int32)
+    df = np.broadcast_to(np.asarray(df, dtype=np.int64), (n,)).copy()
+    samples = np.broadcast_to(np.asarray(samples, dtype=np.int64), (n,)).copy()
+    scale = _concat_f64(scales)
+    total = int((dims.astype(np.int64) ** 2 * samples).sum())
+    out = torch.empty(total, dtype=torch.float32, device="cuda")
+    _lib.call("erd_wishart_batch", urng.handle, n, dims.ctypes.data_as(C.c_void_p),
+              df.ctypes.data_as(C.c_void_p), samples.ctypes.data_as(C.c_void_p), _ptr(scale),
+              _ptr(out), _stream_ptr(stream))
+    res, off = [], 0
+    for d, m in zip(dims, samples):
+        k = int(d) * int(d) * int(m)
+        res.append(out[off:off + k].view(int(d) * int(m), int(d)).t())
+        off += k
+    return res
+
+
+def dirichlet_batch(urng, alphas, samples, stream=None):
+    """Batched DirichletGen<float>(alpha_i).generate(engine_i, samples_i);
+    each result is (K_i, samples_i)."""
+    torch = _torch()
+    n = len(alphas)
+    dims = np.array([len(a) for a in alphas], dtype=np.int32)
+    samples = np.broadcast_to(np.asarray(samples, dtype=np.int64), (n,)).copy()
+    alpha = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float32) for a in alphas]))
+    total = int((dims.astype(np.int64) * samples).sum())
+    out = torch.empty(total, dtype=torch.float32, device="cuda")
+    _lib.call("erd_dirichlet_batch", urng.handle, n, dims.ctypes.data_as(C.c_void_p),
+              samples.ctypes.data_as(C.c_void_p), alpha.ctypes.data_as(C.c_void_p), _ptr(out),
+              _stream_ptr(stream))
+    res, off = [], 0
+    for k, m in zip(dims, samples):
+        c = int(k) * int(m)
+        res.append(out[off:off + c].view(int(m), int(k)).t())
+        off += c
+    return res
+
+
+class MvNormalGen:
+    """Rand::MvNormalGen<float>(mean, cov) on the GPU (MvNormal.h:56-116)."""
+
+    def __init__(self, mean, cov):
+        self.mean = np.asarray(mean, dtype=np.float32).ravel()
+        self.cov = np.asarray(cov, dtype=np.float64)
+
+    def dims(self):
+        return self.mean.size
+
+    def generate(self, urng, samples):
+        return mvnormal_batch(urng, [self.cov], [samples], [self.mean])[0]
+
+
+class WishartGen:
+    """Rand::WishartGen<float>(df, scale) on the GPU (MvNormal.h:184-302)."""
+
+    def __init__(self, df, scale):
+        self.df = int(df)
+        self.scale = np.asarray(scale, dtype=np.float64)
+
+    def dims(self):
+        return self.scale.shape[0]
+
+    def generate(self, urng, samples):
+        return wishart_batch(urng, [self.scale], [self.df], [samples])[0]
+
+
+class DirichletGen:
+    """Rand::DirichletGen<float>(alpha) on the GPU (Multinomial.h:159-217)."""
+
+    def __init__(self, alpha):
+        self.alpha = np.asarray(alpha, dtype=np.float32).ravel()
+
+    def dims(self):
+        return self.alpha.size
+
+    def generate(self, urng, samples):
+        return dirichlet_batch(urng, [self.alpha], [samples])[0]
+
+
+def alias_table(weights):
+    """Host table of DiscreteGen<int32_t, int32_t>: ("cdf", cdf) for K < 16,
+    else ("alias", prob, alias) (AliasMethod, Discrete.h:78-192)."""
+    w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64))
+    K = w.size
+    if K < 16:
+        cdf = np.zeros(K, dtype=np.uint32)
+        _lib.call("erd_alias_table", w.ctypes.data_as(C.c_void_p), K, None, None,
+                  cdf.ctypes.data_as(C.c_void_p))
+        return ("cdf", cdf)
+    size = 1 << int(np.ceil(np.log2(K)))
+    prob = np.zeros(size, dtype=np.int32)
+    alias = np.zeros(size, dtype=np.int32)
+    _lib.call("erd_alias_table", w.ctypes.data_as(C.c_void_p), K, prob.ctypes.data_as(C.c_void_p),
+              alias.ctypes.data_as(C.c_void_p), None)
+    return ("alias", prob, alias)
+
+
+def discrete(rows, cols, urng, weights, out=None, stream=None):
+    """Rand::discrete<MatrixXi>(rows, cols, urng, weights) (int32)."""
+    torch = _torch()
+    w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64))
+    per = int(rows) * int(cols)
+    buf = _alloc(urng, per, torch.int32, out)
+    _lib.call("erd_fill_discrete_i32", urng.handle, _ptr(buf), per, w.ctypes.data_as(C.c_void_p), w.size,
+              _stream_ptr(stream))
+    return _shape(buf, urng, rows, cols) if out is None else buf
+
+
+def multinomial_batch(urng, trials, weights, samples, stream=None):
+    """Batched MultinomialGen<int32_t>(trials_i, weights_i).generate(engine_i,
+    samples_i); each result is (K_i, samples_i) int32."""
+    torch = _torch()
+    n = len(weights)
+    dims = np.array([len(w) for w in weights], dtype=np.int64)
+    trials = np.broadcast_to(np.asarray(trials, dtype=np.int32), (n,)).copy()
+    samples = np.broadcast_to(np.asarray(samples, dtype=np.int64), (n,)).copy()
+    w = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64) for x in weights]))
+    total = int((dims * samples).sum())
+    out = torch.empty(total, dtype=torch.int32, device="cuda")
+    _lib.call("erd_multinomial_batch", urng.handle, n, dims.ctypes.data_as(C.c_void_p),
+              trials.ctypes.data_as(C.c_void_p), samples.ctypes.data_as(C.c_void_p),
+              w.ctypes.data_as(C.c_void_p), _ptr(out), _stream_ptr(stream))
+    res, off = [], 0
+    for k, m in zip(dims, samples):
+        c = int(k) * int(m)
+        res.append(out[off:off + c].view(int(m), int(k)).t())
+        off += c
+    return res
+
+
+class MultinomialGen:
+    """Rand::MultinomialGen<int32_t>(trials, weights) on the GPU (Multinomial.h:25-134)."""
+
+    def __init__(self, trials, weights):
+        self.trials = int(trials)
+        self.weights = np.asarray(weights, dtype=np.float64).ravel()
+
+    def dims(self):
+        return self.weights.size
+
+    def generate(self, urng, samples):
+        return multinomial_batch(urng, [self.trials], [self.weights], [samples])[0]

@@ -68,6 +68,13 @@ _SIGS = {
     "erd_fill_gamma_vop_f32": [P, P, I64, P, P, P],
     "erd_fill_poisson_vop_i32": [P, P, I64, P, P],
     "erd_run_requests": [P, P, I64, P],
+    "erd_cholesky_f64_batch": [I64, P, P, P, P],
+    "erd_mvnormal_batch": [P, I64, P, P, P, P, P, P, P],
+    "erd_wishart_batch": [P, I64, P, P, P, P, P, P],
+    "erd_dirichlet_batch": [P, I64, P, P, P, P, P],
+    "erd_alias_table": [P, I64, P, P, P],
+    "erd_fill_discrete_i32": [P, P, I64, P, I64, P],
+    "erd_multinomial_batch": [P, I64, P, P, P, P, P, P],
     "erd_device_alloc": [C.POINTER(P), C.c_size_t],
     "erd_device_free": [P],
     "erd_memcpy_d2h": [P, P, C.c_size_t, P],

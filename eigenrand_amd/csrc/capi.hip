@@ -397,27 +397,18 @@ int make_segment(const erd_request& r, Segment& s) {
     return ERD_OK;
 }
 
-// Groups requests by engine (keeping each engine's order), uploads the
-// program and launches it. Device buffers come from the stream-ordered
-// allocator and are released on the same stream.
-int run_requests(erd_engines* e, const erd_request* reqs, int64_t n, hipStream_t stream) {
-    if (!e || !e->states || n < 0 || (n > 0 && !reqs)) return ERD_ERR_INVALID;
+// Uploads a segment program (segments grouped by engine: consecutive
+// segments with the same engine form one group, run in order) and launches
+// it. Device buffers come from the stream-ordered allocator and are released
+// on the same stream.
+int run_segments(erd_engines* e, const std::vector<Segment>& segs, const std::vector<int32_t>& seg_engine,
+                 hipStream_t stream) {
+    const int64_t n = (int64_t)segs.size();
     if (n == 0) return ERD_OK;
-    std::vector<int64_t> order((size_t)n);
-    for (int64_t i = 0; i < n; ++i) {
-        if (reqs[i].engine < 0 || reqs[i].engine >= e->count) return ERD_ERR_INVALID;
-        order[(size_t)i] = i;
-    }
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int64_t x, int64_t y) { return reqs[x].engine < reqs[y].engine; });
-    std::vector<Segment> segs((size_t)n);
     std::vector<int32_t> engine_of, begin;
     for (int64_t k = 0; k < n; ++k) {
-        const erd_request& r = reqs[order[(size_t)k]];
-        int rc = make_segment(r, segs[(size_t)k]);
-        if (rc) return rc;
-        if (engine_of.empty() || engine_of.back() != r.engine) {
-            engine_of.push_back(r.engine);
+        if (engine_of.empty() || engine_of.back() != seg_engine[(size_t)k]) {
+            engine_of.push_back(seg_engine[(size_t)k]);
             begin.push_back((int32_t)k);
         }
     }
@@ -434,8 +425,8 @@ int run_requests(erd_engines* e, const erd_request* reqs, int64_t n, hipStream_t
     std::memcpy(host.data() + seg_bytes + sizeof(int32_t) * (size_t)groups, begin.data(),
                 sizeof(int32_t) * (size_t)(groups + 1));
     err = hipMemcpyAsync(dbuf, host.data(), host.size(), hipMemcpyHostToDevice, stream);
-    // The copy is staged from pageable memory, so `host` may be released once
-    // it returns; synchronise the stream to make that unconditional.
+    // The copy is staged from pageable memory; synchronise so that `host` may
+    // be released unconditionally when this function returns.
     if (err == hipSuccess) err = hipStreamSynchronize(stream);
     if (err == hipSuccess)
         err = launch_program(e->states, groups, (const int32_t*)(dbuf + seg_bytes),
@@ -444,6 +435,28 @@ int run_requests(erd_engines* e, const erd_request* reqs, int64_t n, hipStream_t
     hipError_t e2 = hipFreeAsync(dbuf, stream);
     if (err == hipSuccess) err = e2;
     return err == hipSuccess ? ERD_OK : ERD_ERR_HIP;
+}
+
+// Groups requests by engine (keeping each engine's order) into a program.
+int run_requests(erd_engines* e, const erd_request* reqs, int64_t n, hipStream_t stream) {
+    if (!e || !e->states || n < 0 || (n > 0 && !reqs)) return ERD_ERR_INVALID;
+    if (n == 0) return ERD_OK;
+    std::vector<int64_t> order((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        if (reqs[i].engine < 0 || reqs[i].engine >= e->count) return ERD_ERR_INVALID;
+        order[(size_t)i] = i;
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t x, int64_t y) { return reqs[x].engine < reqs[y].engine; });
+    std::vector<Segment> segs((size_t)n);
+    std::vector<int32_t> seg_engine((size_t)n);
+    for (int64_t k = 0; k < n; ++k) {
+        const erd_request& r = reqs[order[(size_t)k]];
+        int rc = make_segment(r, segs[(size_t)k]);
+        if (rc) return rc;
+        seg_engine[(size_t)k] = r.engine;
+    }
+    return run_segments(e, segs, seg_engine, stream);
 }
 
 }  // namespace erd

@@ -1,0 +1,55 @@
+// Shared host/device descriptors of the batched multivariate kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace erd {
+
+// One instance of a batched L*Z (+ mean) transform.
+struct MvInstance {
+    int32_t d;           // dimension
+    int32_t pad;
+    int64_t n;           // columns of Z / X
+    int64_t split_off;   // offset of the padded dp x dp bf16 L planes
+    int64_t z_off;       // offset of Z (d x n, column-major floats)
+    int64_t x_off;       // offset of X (d x n, column-major floats)
+    const float* mean;   // d floats, or null
+};
+
+// A work item of the persistent transform: rows [row0, row0 + 16*MAXR) x
+// columns [col0, col0 + NT) of one instance.
+struct MvTile {
+    int32_t inst;
+    int32_t row0;
+    int64_t col0;
+};
+
+hipError_t launch_cholesky_f64(double* a, const int64_t* off, const int32_t* dims, int32_t* info,
+                               int n_inst, hipStream_t s);
+hipError_t launch_split_l(const double* l64, const int64_t* off, const int32_t* dims,
+                          const int64_t* split_off, uint16_t* hi, uint16_t* lo, float* l32, int n_inst,
+                          int max_dp, hipStream_t s);
+// variant: 0 = (1 col block per wave, 256-row tiles), 1 = (2, 128-row), 2 = (4, 64-row),
+// 3 = (1, 64-row). Any dimension works with any variant (rows are tiled).
+hipError_t launch_lz_gemm(int variant, int grid, const MvTile* tiles, int32_t n_tiles, int32_t* counter,
+                          const MvInstance* inst, const uint16_t* lhi, const uint16_t* llo,
+                          const float* z, float* x, hipStream_t s);
+
+struct SyrkTile {
+    int32_t inst;
+    int32_t sample;
+    int32_t ta, tb;
+};
+hipError_t launch_wishart_assemble(const MvInstance* inst, const int64_t* nrm_off, const int64_t* chi_off,
+                                   const float* nrm, const float* chi, float* a, int n_inst,
+                                   hipStream_t s);
+hipError_t launch_syrk_tt(const SyrkTile* tiles, int n_tiles, const MvInstance* inst, const float* t,
+                          float* w, const int64_t* w_off, hipStream_t s);
+hipError_t launch_dirichlet_norm(const int32_t* dims, const int64_t* samples, const int64_t* g_off,
+                                 const float* g, float* out, int n_inst, int64_t max_n, hipStream_t s);
+
+constexpr int kLzVariants = 4;
+inline int lz_cols(int variant) { return variant == 1 ? 128 : variant == 2 ? 256 : 64; }
+inline int lz_rows(int variant) { return variant == 0 ? 256 : variant == 1 ? 128 : 64; }
+
+}  // namespace erd

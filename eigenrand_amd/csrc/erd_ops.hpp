@@ -705,4 +705,245 @@ __device__ void op_poisson_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t*
     cursor_end(e, cur);
 }
 
+// ------------------------------------------------------ discrete draws ---
+// Lookup tables of DiscreteGen<int32_t, int32_t> (Discrete.h:352-485), built
+// on the host: K < 16 uses the uint32 CDF (ncdf = K), otherwise the alias
+// table (prob/alias of 2^bits entries, bitmask = 2^bits - 1).
+struct DiscreteTable {
+    const uint32_t* cdf;
+    const int32_t* prob;
+    const int32_t* alias;
+    int32_t ncdf;      // K if the CDF path is used, else 0
+    uint32_t bitmask;
+};
+
+// packetOp category of one 32-bit word (Discrete.h:458-475): CDF counts
+// thresholds <= the 31-bit value; alias compares the logical word >> 1
+// against prob (signed) and falls back to alias.
+__device__ __forceinline__ int32_t discrete_packet(const DiscreteTable& t, uint32_t word) {
+    if (t.ncdf) {
+        const int32_t rx = (int32_t)(word & 0x7FFFFFFFu);
+        int32_t ret = t.ncdf - 1;
+        for (int i = 0; i + 1 < t.ncdf; ++i) ret -= (rx < (int32_t)t.cdf[i]) ? 1 : 0;
+        return ret;
+    }
+    const int32_t albit = (int32_t)(word & t.bitmask);
+    return ((int32_t)(word >> 1) < t.prob[albit]) ? albit : t.alias[albit];
+}
+
+// operator() category (Discrete.h:428-441): lower_bound over the CDF, or the
+// alias test with an arithmetic shift and an unsigned compare.
+__device__ __forceinline__ int32_t discrete_scalar(const DiscreteTable& t, uint32_t word) {
+    const int32_t rx = (int32_t)word;
+    if (t.ncdf) {
+        const uint32_t r = (uint32_t)(rx & 0x7FFFFFFF);
+        int32_t i = 0;
+        while (i < t.ncdf - 1 && t.cdf[i] < r) ++i;
+        return i;
+    }
+    const uint32_t albit = (uint32_t)rx & t.bitmask;
+    const uint32_t alx = (uint32_t)(rx >> 1);
+    return (alx < (uint32_t)t.prob[albit]) ? (int32_t)albit : t.alias[albit];
+}
+
+// DiscreteGen fill (packet path): one full draw gives 8 categories; the
+// first Packet4i is returned, the second cached for the next packetOp
+// (Discrete.h:443-483), so a unit = 1 draw = 8 outputs (a fill of 4 mod 8
+// ends with an unused cached half).
+struct DiscreteUnits {
+    static constexpr int CALLS = 1;
+    int32_t* out;
+    DiscreteTable t;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return (n + 7) / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        const int32_t a = discrete_packet(t, lo32(w[0])), b = discrete_packet(t, hi32(w[0]));
+        const int64_t i = 8 * u + 2 * l;  // words 2l, 2l+1 of the draw
+        if (i < per) {
+            out[i] = a;
+            out[i + 1] = b;
+        }
+    }
+};
+
+// MultinomialGen::generate, trials < max(30(K-1), 100) (Multinomial.h:95-133):
+// per sample, trials>>4 blocks of 16 draws (4 packetOps each), then the
+// (trials&15)>>2 tail packetOps and trials&3 scalar draws (one full engine
+// draw each, first word). The DiscreteGen cache (valid flag + cached upper
+// half) carries from sample to sample. Counts go to out[s*K + category]
+// (zeroed by the caller) with atomics.
+__device__ void op_multinomial_small(Eng& e, int32_t* out, int64_t n, int32_t T, int32_t K,
+                                     const DiscreteTable& tab, int lane) {
+    int pos = e.idx;
+    bool v = false;          // DiscreteGen cache valid
+    uint64_t cw0 = 0, cw1 = 0;  // cached upper half (u64 lanes 2, 3), tempered
+    const int32_t Q = 4 * (T >> 4) + ((T & 15) >> 2);
+    const int32_t S = T & 3;
+    for (int64_t s = 0; s < n; ++s) {
+        int32_t* col = out + s * K;
+        const int32_t nq = 4 * Q;  // packet draws of this sample
+        int32_t first = 0;
+        if (v && nq > 0) {
+            if (lane < 4) {
+                const uint64_t w = (lane >> 1) ? cw1 : cw0;
+                atomicAdd(col + discrete_packet(tab, (lane & 1) ? hi32(w) : lo32(w)), 1);
+            }
+            first = 4;
+        }
+        // draws [first, nq) come from new full draws, 8 words each
+        const int32_t nw = nq - first;
+        const int32_t calls = (nw + 7) >> 3;
+        int32_t done = 0;
+        while (done < calls) {
+            if (pos >= kN) {
+                twist_lds(e.mt, lane);
+                pos = 0;
+                e.twisted = true;
+            }
+            const int32_t take = (calls - done) < (kN - pos) ? (calls - done) : (kN - pos);
+            for (int32_t q = lane; q < take * 8; q += 64) {
+                const int32_t j = done * 8 + q;  // word index among the new draws
+                if (j < nw) {
+                    const uint64_t w = temper(e.mt[(pos + (q >> 3)) * kLanes + ((q & 7) >> 1)]);
+                    atomicAdd(col + discrete_packet(tab, (q & 1) ? hi32(w) : lo32(w)), 1);
+                }
+            }
+            pos += take;
+            done += take;
+        }
+        if (nq > 0) {
+            const bool v2 = v != ((Q & 1) != 0);
+            if (v2) {
+                cw0 = temper(e.mt[(pos - 1) * kLanes + 2]);
+                cw1 = temper(e.mt[(pos - 1) * kLanes + 3]);
+            }
+            v = v2;
+        }
+        for (int32_t k = 0; k < S; ++k) {
+            if (pos >= kN) {
+                twist_lds(e.mt, lane);
+                pos = 0;
+                e.twisted = true;
+            }
+            if (lane == 0) atomicAdd(col + discrete_scalar(tab, lo32(temper(e.mt[pos * kLanes]))), 1);
+            pos += 1;
+        }
+    }
+    __syncthreads();
+    e.idx = pos;
+}
+
+// BinomialVGen<float>::packetOp at Packet8f (Discrete.h:1251-1330), the
+// per-row sampler of MultinomialGen's large-trials path (Multinomial.h:74-91):
+// trials come from r[] (int32), p is one float for the row, and the row
+// result t goes to out[i*stride] while r[i] -= t. Full draws; thread 8i+j
+// runs iteration i of lane j (8 iterations per step).
+__device__ void op_binomial_row8(Eng& e, uint64_t* carry, float* res_lds, int32_t* out, int64_t stride,
+                                 int32_t* r, int64_t n, float p, uint32_t* status, int lane) {
+    const int64_t T = n / 8;
+    if (T == 0) return;
+    const int i = lane >> 3, j = lane & 7;
+    Cursor<4, 1> cur;
+    cur.mt = e.mt;
+    cur.carry = carry;
+    cur.pos = e.idx;
+    cur.nc = 0;
+    cur.used = false;
+    cur.twisted = &e.twisted;
+    const uint64_t jmask = 0x0101010101010101ull << j;
+    const float psmall = (p < 1.0f - p) ? p : 1.0f - p;
+    const float plp = plog(psmall);
+    const float plq = plog(1.0f - psmall);
+    for (int64_t pk = 0; pk < T; ++pk) {
+        const int32_t nt = r[8 * pk + j];
+        const float ptr = (float)nt;
+        const float pmean = ptr * psmall;
+        const float pg1 = plgamma_approx(ptr + 1.0f);
+        const float psq = sqrtf((2.0f * pmean) * (1.0f - psmall));
+        const bool valid = pmean < 1.0f;
+        float valid_res = 0.0f;
+        const uint64_t V = __ballot(valid) & 0xFFull;
+        if (V) {
+            const float nem = pexp(-pmean);
+            float val = 1.0f, res = 0.0f;
+            for (int it = 0;; ++it) {
+                cur.ensure(1, lane);
+                const float u = ur_f32(cur.word32(0, j));
+                cur.advance(1);
+                val = val * u;
+                const bool c = (nem < val) && valid;
+                if (!(__ballot(c) & 0xFFull)) break;
+                res = res + (c ? 1.0f : 0.0f);
+                if (it > kLockstepCap) {
+                    if (lane == 0) atomicOr(status, kStatusIterCap);
+                    break;
+                }
+            }
+            valid_res = res;
+        }
+        if (V != 0xFFull) {
+            uint64_t done = V;
+            for (int it = 0;; it += 8) {
+                cur.ensure(2, lane);
+                const int ni_avail = cur.avail() / 2;
+                const int ni = ni_avail < 8 ? ni_avail : 8;
+                float fres = 0.0f;
+                bool fv = false;
+                if (i < ni) {
+                    float s, c;
+                    psincos(ERD_F(kPi) * ur_f32(cur.word32(2 * i, j)), s, c);
+                    const float ys = s / c;
+                    fres = truncf(psq * ys + pmean);
+                    const float pp1 = (ERD_F(1.2) * psq) * (1.0f + ys * ys);
+                    const float pp2 = pexp((((pg1 - plgamma_approx(fres + 1.0f)) -
+                                             plgamma_approx((ptr + 1.0f) - fres)) +
+                                            fres * plp) +
+                                           (ptr - fres) * plq);
+                    const bool c1 = (0.0f <= fres) && (fres <= ptr);
+                    const float u2 = ur_f32(cur.word32(2 * i + 1, j));
+                    fv = c1 && (u2 <= pp1 * pp2);
+                }
+                const uint64_t A = __ballot(fv);
+                int need = 0;
+                bool all = true;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    if (done & (1ull << q)) continue;
+                    const uint64_t m = A & (0x0101010101010101ull << q);
+                    if (m) {
+                        const int fi = (__ffsll((long long)m) - 1) >> 3;
+                        need = need > fi + 1 ? need : fi + 1;
+                    } else {
+                        all = false;
+                    }
+                }
+                if (!all) need = ni;
+                const uint64_t mine = A & jmask;
+                const bool open = !(done & (1ull << j));
+                if (open && mine && i == ((__ffsll((long long)mine) - 1) >> 3)) res_lds[j] = fres;
+                __syncthreads();
+                if (open && mine) valid_res = res_lds[j];
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (A & (0x0101010101010101ull << q)) done |= 1ull << q;
+                cur.advance(2 * need);
+                if (all) break;
+                if (it > kLockstepCap) {
+                    if (lane == 0) atomicOr(status, kStatusIterCap);
+                    break;
+                }
+            }
+        }
+        const int32_t k = (int32_t)valid_res;
+        const int32_t t = (p == psmall) ? k : nt - k;
+        if (lane < 8) {
+            out[(8 * pk + j) * stride] = t;
+            r[8 * pk + j] = nt - t;
+        }
+    }
+    __syncthreads();
+    e.idx = cur.pos;
+}
+
 }  // namespace erd

@@ -1,0 +1,376 @@
+// Batched multivariate kernels: FP64 Cholesky, split-BF16 MFMA transform
+// (MvNormalGen's L*z, Wishart's chol*A), Wishart assembly and T*T^T, and the
+// Dirichlet column normalisation.
+//
+// Instances of uneven dimension share one persistent launch: the host builds
+// a tile list ordered by decreasing work and each workgroup pulls tiles from
+// an atomic counter until the list is empty.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "erd_device.hpp"
+#include "erd_mv.hpp"
+
+namespace erd {
+
+// ----------------------------------------------------------- Cholesky ---
+// Right-looking Cholesky in FP64 of each instance's d x d matrix (column
+// major, in place on a scratch copy; the strictly upper part is zeroed).
+// info[i] = 0 on success, j+1 if the j-th pivot is not positive.
+__global__ __launch_bounds__(256) void k_cholesky_f64(double* __restrict__ a,
+                                                      const int64_t* __restrict__ off,
+                                                      const int32_t* __restrict__ dims,
+                                                      int32_t* __restrict__ info) {
+    __shared__ double ljj_s;
+    __shared__ int bad;
+    const int inst = blockIdx.x;
+    const int d = dims[inst];
+    double* A = a + off[inst];
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int j = 0; j < d; ++j) {
+        if (tid == 0) {
+            const double ajj = A[(int64_t)j * d + j];
+            if (!(ajj > 0.0)) {
+                bad = j + 1;
+                ljj_s = 1.0;
+            } else {
+                ljj_s = sqrt(ajj);
+                A[(int64_t)j * d + j] = ljj_s;
+            }
+        }
+        __syncthreads();
+        if (bad) break;
+        const double inv = 1.0 / ljj_s;
+        for (int i = j + 1 + tid; i < d; i += 256) A[(int64_t)j * d + i] *= inv;
+        __syncthreads();
+        // A[i][k] -= L[i][j] * L[k][j] for j < k <= i: wave w takes columns
+        // k = j+1+w, j+5+w, ...; lanes take consecutive rows (coalesced).
+        for (int k = j + 1 + wave; k < d; k += 4) {
+            const double lkj = A[(int64_t)j * d + k];
+            for (int i = k + lane; i < d; i += 64)
+                A[(int64_t)k * d + i] -= A[(int64_t)j * d + i] * lkj;
+        }
+        __syncthreads();
+    }
+    // zero the strictly upper triangle
+    for (int64_t t = tid; t < (int64_t)d * d; t += 256) {
+        const int col = (int)(t / d), row = (int)(t % d);
+        if (row < col) A[t] = 0.0;
+    }
+    if (tid == 0) info[inst] = bad;
+}
+
+// -------------------------------------------------------- split-BF16 ---
+__device__ __forceinline__ uint16_t bf16_rne(float x) {
+    const uint32_t u = __float_as_uint(x);
+    const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// L (FP64, column-major d x d) -> row-major bf16 hi/lo planes of a padded
+// dp x dp tile (dp = d rounded up to 16), plus the float L (column-major)
+// that float consumers use. hi = bf16(l), lo = bf16(l - hi) with l = float(L).
+__global__ void k_split_l(const double* __restrict__ l64, const int64_t* __restrict__ off,
+                          const int32_t* __restrict__ dims, const int64_t* __restrict__ split_off,
+                          uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
+                          float* __restrict__ l32) {
+    const int inst = blockIdx.y;
+    const int d = dims[inst];
+    const int dp = (d + 15) & ~15;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)dp * dp) return;
+    const int r = (int)(t / dp), c = (int)(t % dp);  // row-major position
+    float v = 0.0f;
+    if (r < d && c < d) {
+        v = (float)l64[off[inst] + (int64_t)c * d + r];
+        l32[off[inst] + (int64_t)c * d + r] = v;
+    }
+    const uint16_t h = bf16_rne(v);
+    hi[split_off[inst] + t] = h;
+    lo[split_off[inst] + t] = bf16_rne(v - bf16_to_f(h));
+}
+
+// ------------------------------------------------------- MFMA L * Z ---
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int KC = 32;       // k per chunk (one v_mfma_f32_16x16x32_bf16)
+constexpr int KPAD = 40;     // LDS row pitch in bf16 (80 B: keeps 16 B alignment)
+
+// Output tile = rows [row0, row0 + 16*MAXR) x NT = 64*NCB columns. Wave w
+// owns column blocks w*NCB .. w*NCB+NCB-1 (16 columns each) and every 16-row
+// block of the tile. Per 32-wide k chunk the workgroup
+// stages L[:, k0:k0+32] (hi, lo) and Z[k0:k0+32, cols] (split on the fly)
+// in LDS; each wave issues hi*hi + hi*lo + lo*hi MFMAs, skipping row blocks
+// above the diagonal (L is lower triangular).
+template <int NCB, int MAXR>
+__global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tiles, int32_t n_tiles,
+                                                 int32_t* __restrict__ counter,
+                                                 const MvInstance* __restrict__ inst,
+                                                 const uint16_t* __restrict__ lhi,
+                                                 const uint16_t* __restrict__ llo,
+                                                 const float* __restrict__ z,
+                                                 float* __restrict__ x) {
+    constexpr int NT = 64 * NCB;
+    __shared__ __align__(16) uint16_t la[2][MAXR * 16 * KPAD];
+    __shared__ __align__(16) uint16_t zb[2][NT * KPAD];
+    __shared__ int tile_s;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int l16 = lane & 15, g = lane >> 4;
+    for (;;) {
+        if (tid == 0) tile_s = atomicAdd(counter, 1);
+        __syncthreads();
+        const int t = tile_s;
+        __syncthreads();
+        if (t >= n_tiles) break;
+        const MvTile tl = tiles[t];
+        const MvInstance in = inst[tl.inst];
+        const int d = in.d, dp = (d + 15) & ~15;
+        const int row0 = tl.row0;
+        const int R = (dp - row0) / 16 < MAXR ? (dp - row0) / 16 : MAXR;
+        const int kmax = row0 + 16 * R;  // L[i][k] = 0 for k > i
+        const int64_t n = in.n;
+        const int64_t c0 = tl.col0;
+        f32x4 acc[NCB][MAXR];
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+            for (int rb = 0; rb < MAXR; ++rb) acc[cb][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const uint16_t* Lh = lhi + in.split_off;
+        const uint16_t* Ll = llo + in.split_off;
+        const float* Z = z + in.z_off;
+        for (int k0 = 0; k0 < kmax; k0 += KC) {
+            // stage L[row0 + r, k0:k0+32] (row-major bf16, 16 B per thread-step)
+            for (int q = tid; q < R * 16 * (KC / 8); q += 256) {
+                const int r = q / (KC / 8), kk = (q % (KC / 8)) * 8;
+                uint4 vh = {0, 0, 0, 0}, vl = {0, 0, 0, 0};
+                if (k0 + kk < dp) {
+                    vh = *reinterpret_cast<const uint4*>(Lh + (int64_t)(row0 + r) * dp + k0 + kk);
+                    vl = *reinterpret_cast<const uint4*>(Ll + (int64_t)(row0 + r) * dp + k0 + kk);
+                }
+                *reinterpret_cast<uint4*>(&la[0][r * KPAD + kk]) = vh;
+                *reinterpret_cast<uint4*>(&la[1][r * KPAD + kk]) = vl;
+            }
+            // stage Z[k0:k0+32, c0:c0+NT] split into hi/lo, layout [col][k]
+            for (int q = tid; q < NT * (KC / 8); q += 256) {
+                const int c = q / (KC / 8), kk = (q % (KC / 8)) * 8;
+                const int64_t col = c0 + c;
+                uint16_t h[8], lw[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int k = k0 + kk + e;
+                    const float v = (k < d && col < n) ? Z[col * d + k] : 0.0f;
+                    h[e] = bf16_rne(v);
+                    lw[e] = bf16_rne(v - bf16_to_f(h[e]));
+                }
+                uint4 vh, vl;
+                vh.x = h[0] | ((uint32_t)h[1] << 16); vh.y = h[2] | ((uint32_t)h[3] << 16);
+                vh.z = h[4] | ((uint32_t)h[5] << 16); vh.w = h[6] | ((uint32_t)h[7] << 16);
+                vl.x = lw[0] | ((uint32_t)lw[1] << 16); vl.y = lw[2] | ((uint32_t)lw[3] << 16);
+                vl.z = lw[4] | ((uint32_t)lw[5] << 16); vl.w = lw[6] | ((uint32_t)lw[7] << 16);
+                *reinterpret_cast<uint4*>(&zb[0][c * KPAD + kk]) = vh;
+                *reinterpret_cast<uint4*>(&zb[1][c * KPAD + kk]) = vl;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+                const int col = (wave * NCB + cb) * 16 + l16;
+                const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&zb[0][col * KPAD + 8 * g]);
+                const bf16x8 bl = *reinterpret_cast<const bf16x8*>(&zb[1][col * KPAD + 8 * g]);
+#pragma unroll
+                for (int rb = 0; rb < MAXR; ++rb) {
+                    if (rb < R && row0 + rb * 16 + 15 >= k0) {
+                        const int row = rb * 16 + l16;
+                        const bf16x8 ah = *reinterpret_cast<const bf16x8*>(&la[0][row * KPAD + 8 * g]);
+                        const bf16x8 al = *reinterpret_cast<const bf16x8*>(&la[1][row * KPAD + 8 * g]);
+                        acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[cb][rb], 0, 0, 0);
+                        acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[cb][rb], 0, 0, 0);
+                        acc[cb][rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[cb][rb], 0, 0, 0);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // epilogue: X[i][j] = acc + mean[i]; lane holds rows 4g..4g+3 of column l16
+        float* X = x + in.x_off;
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb) {
+            const int64_t col = c0 + (wave * NCB + cb) * 16 + l16;
+            if (col >= n) continue;
+#pragma unroll
+            for (int rb = 0; rb < MAXR; ++rb) {
+                if (rb >= R) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = row0 + rb * 16 + 4 * g + r;
+                    if (row < d) {
+                        const float mu = in.mean ? in.mean[row] : 0.0f;
+                        X[col * d + row] = acc[cb][rb][r] + mu;
+                    }
+                }
+            }
+        }
+    }
+}
+
+hipError_t launch_lz_gemm(int variant, int grid, const MvTile* tiles, int32_t n_tiles, int32_t* counter,
+                          const MvInstance* inst, const uint16_t* lhi, const uint16_t* llo,
+                          const float* z, float* x, hipStream_t s) {
+    switch (variant) {
+        case 0: hipLaunchKernelGGL((k_lz_gemm<1, 16>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter,
+                                   inst, lhi, llo, z, x); break;
+        case 1: hipLaunchKernelGGL((k_lz_gemm<2, 8>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter,
+                                   inst, lhi, llo, z, x); break;
+        case 2: hipLaunchKernelGGL((k_lz_gemm<4, 4>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter,
+                                   inst, lhi, llo, z, x); break;
+        default: hipLaunchKernelGGL((k_lz_gemm<1, 4>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter,
+                                    inst, lhi, llo, z, x); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_cholesky_f64(double* a, const int64_t* off, const int32_t* dims, int32_t* info,
+                               int n_inst, hipStream_t s) {
+    if (n_inst <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cholesky_f64, dim3(n_inst), dim3(256), 0, s, a, off, dims, info);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_l(const double* l64, const int64_t* off, const int32_t* dims,
+                          const int64_t* split_off, uint16_t* hi, uint16_t* lo, float* l32, int n_inst,
+                          int max_dp, hipStream_t s) {
+    if (n_inst <= 0) return hipSuccess;
+    const int64_t elems = (int64_t)max_dp * max_dp;
+    dim3 grid((unsigned)((elems + 255) / 256), (unsigned)n_inst);
+    hipLaunchKernelGGL(k_split_l, grid, dim3(256), 0, s, l64, off, dims, split_off, hi, lo, l32);
+    return hipGetLastError();
+}
+
+}  // namespace erd
+
+namespace erd {
+
+// --------------------------------------------------------------- Wishart ---
+// rand_mat of WishartGen::generate (MvNormal.h:240-266) for every sample:
+// column i of sample j holds normals in rows i+1..d-1 (taken in order from the
+// instance's normal fill), sqrt(chi^2_i) on the diagonal, zeros above.
+// a: d x (d*n) column-major; nrm: n*d(d-1)/2 normals; chi: d rows of n.
+__global__ void k_wishart_assemble(const MvInstance* __restrict__ inst, const int64_t* __restrict__ nrm_off,
+                                   const int64_t* __restrict__ chi_off, const float* __restrict__ nrm,
+                                   const float* __restrict__ chi, float* __restrict__ a) {
+    const int i_inst = blockIdx.y;
+    const MvInstance in = inst[i_inst];
+    const int d = in.d;
+    const int64_t n = in.n / d;  // columns of a = d * samples
+    const int64_t total = (int64_t)d * d * n;
+    const int64_t per_sample = (int64_t)d * (d - 1) / 2;
+    float* A = a + in.z_off;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t col = t / d;
+        const int row = (int)(t % d);
+        const int64_t j = col / d;
+        const int i = (int)(col % d);
+        float v = 0.0f;
+        if (row == i) {
+            v = sqrtf(chi[chi_off[i_inst] + (int64_t)i * n + j]);
+        } else if (row > i) {
+            // normals before column i of sample j: sum_{c<i} (d-1-c)
+            const int64_t before = (int64_t)i * (d - 1) - (int64_t)i * (i - 1) / 2;
+            v = nrm[nrm_off[i_inst] + j * per_sample + before + (row - i - 1)];
+        }
+        A[t] = v;
+    }
+}
+
+// W_j = T_j T_j^T (MvNormal.h:271-275) in FP32. One workgroup per (instance,
+// sample, 32x32 output tile with tile row >= tile column); T is lower
+// triangular so the k loop stops at the smaller tile's last row.
+
+__global__ __launch_bounds__(256) void k_syrk_tt(const SyrkTile* __restrict__ tiles,
+                                                 const MvInstance* __restrict__ inst,
+                                                 const float* __restrict__ tmat,
+                                                 float* __restrict__ w, const int64_t* __restrict__ w_off) {
+    __shared__ float ta_s[32][33];
+    __shared__ float tb_s[32][33];
+    const SyrkTile tl = tiles[blockIdx.x];
+    const MvInstance in = inst[tl.inst];
+    const int d = in.d;
+    const float* T = tmat + in.x_off + (int64_t)tl.sample * d * d;  // d x d column-major
+    float* W = w + w_off[tl.inst] + (int64_t)tl.sample * d * d;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    const int a0 = tl.ta * 32, b0 = tl.tb * 32;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    const int kend = (b0 + 32 < d ? b0 + 32 : d);  // T[b][k] = 0 for k > b
+    for (int k0 = 0; k0 < kend; k0 += 32) {
+        for (int q = threadIdx.x; q < 32 * 32; q += 256) {
+            const int r = q & 31, kk = q >> 5;
+            const int k = k0 + kk;
+            ta_s[r][kk] = (a0 + r < d && k < d) ? T[(int64_t)k * d + a0 + r] : 0.0f;
+            tb_s[r][kk] = (b0 + r < d && k < d) ? T[(int64_t)k * d + b0 + r] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int kk = 0; kk < 32; ++kk) {
+            const float bv = tb_s[tx][kk];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = fmaf(ta_s[ty * 4 + q][kk], bv, acc[q]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int ra = a0 + ty * 4 + q, rb = b0 + tx;
+        if (ra < d && rb < d) {
+            W[(int64_t)rb * d + ra] = acc[q];
+            W[(int64_t)ra * d + rb] = acc[q];
+        }
+    }
+}
+
+// ------------------------------------------------------------- Dirichlet ---
+// ret.array().rowwise() /= ret.array().colwise().sum() (Multinomial.h:200-201)
+// on the K x n result; g holds row k of gamma draws contiguously (g[k*n+s]).
+__global__ void k_dirichlet_norm(const int32_t* __restrict__ dims, const int64_t* __restrict__ samples,
+                                 const int64_t* __restrict__ g_off, const float* __restrict__ g,
+                                 float* __restrict__ out) {
+    const int i_inst = blockIdx.y;
+    const int K = dims[i_inst];
+    const int64_t n = samples[i_inst];
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const float* G = g + g_off[i_inst];
+    float* O = out + g_off[i_inst];
+    float sum = 0.0f;
+    for (int k = 0; k < K; ++k) sum += G[(int64_t)k * n + s];
+    for (int k = 0; k < K; ++k) O[s * K + k] = G[(int64_t)k * n + s] / sum;
+}
+
+hipError_t launch_wishart_assemble(const MvInstance* inst, const int64_t* nrm_off, const int64_t* chi_off,
+                                   const float* nrm, const float* chi, float* a, int n_inst,
+                                   hipStream_t s) {
+    if (n_inst <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_wishart_assemble, dim3(256, (unsigned)n_inst), dim3(256), 0, s, inst, nrm_off,
+                       chi_off, nrm, chi, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_syrk_tt(const SyrkTile* tiles, int n_tiles, const MvInstance* inst, const float* t,
+                          float* w, const int64_t* w_off, hipStream_t s) {
+    if (n_tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_syrk_tt, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles, inst, t, w, w_off);
+    return hipGetLastError();
+}
+
+hipError_t launch_dirichlet_norm(const int32_t* dims, const int64_t* samples, const int64_t* g_off,
+                                 const float* g, float* out, int n_inst, int64_t max_n, hipStream_t s) {
+    if (n_inst <= 0) return hipSuccess;
+    dim3 grid((unsigned)((max_n + 255) / 256), (unsigned)n_inst);
+    hipLaunchKernelGGL(k_dirichlet_norm, grid, dim3(256), 0, s, dims, samples, g_off, g, out);
+    return hipGetLastError();
+}
+
+}  // namespace erd

@@ -1,0 +1,594 @@
+// Host orchestration of the batched multivariate generators (C ABI):
+// MvNormalGen, WishartGen, DirichletGen over many instances of uneven
+// dimension, each instance driven by its own engine.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+#include "eigenrand_amd.h"
+#include "erd_capi_internal.hpp"
+#include "erd_device.hpp"
+#include "erd_kernels.hpp"
+#include "erd_mv.hpp"
+#include "erd_segment.hpp"
+
+namespace erd {
+
+int run_segments(erd_engines* e, const std::vector<Segment>& segs, const std::vector<int32_t>& engine_of,
+                 hipStream_t stream);
+
+namespace {
+
+inline int hip_rc(hipError_t e) {
+    if (e == hipSuccess) return ERD_OK;
+    return e == hipErrorOutOfMemory ? ERD_ERR_OOM : ERD_ERR_HIP;
+}
+
+// Stream-ordered scratch released when the object goes out of scope.
+struct Scratch {
+    hipStream_t s;
+    std::vector<void*> ptrs;
+    explicit Scratch(hipStream_t st) : s(st) {}
+    ~Scratch() {
+        for (void* p : ptrs) (void)hipFreeAsync(p, s);
+    }
+    template <class T>
+    hipError_t alloc(T** p, size_t count) {
+        *p = nullptr;
+        if (count == 0) count = 1;
+        hipError_t e = hipMallocAsync((void**)p, sizeof(T) * count, s);
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+    // Host vectors may die right after the call, so the copy is completed
+    // before returning.
+    template <class T>
+    hipError_t upload(T** p, const std::vector<T>& v) {
+        hipError_t e = alloc(p, v.size());
+        if (e == hipSuccess && !v.empty())
+            e = hipMemcpyAsync(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        return e;
+    }
+};
+
+// ------------------------------------------------------------ autotuner ---
+// Picks the L*Z tile variant per 64-wide dimension bucket by timing every
+// variant on the first batch that contains the bucket; the choice is cached
+// for the process. All variants give identical results (same per-element k
+// order), so tuning never changes an output. ERD_AUTOTUNE=0 disables it.
+std::mutex g_tune_mu;
+std::map<int, int> g_tuned;
+
+int default_variant(int dp) { return dp <= 64 ? 2 : dp <= 128 ? 1 : 0; }
+
+bool autotune_enabled() {
+    const char* v = std::getenv("ERD_AUTOTUNE");
+    return !(v && v[0] == '0');
+}
+
+int bucket_of(int d) { return ((d + 15) & ~15) <= 64 ? 0 : ((d + 15) & ~15) <= 128 ? 1 : ((d + 15) & ~15) <= 256 ? 2 : 3; }
+
+void build_tiles(const std::vector<MvInstance>& inst, const std::vector<int>& which, int variant,
+                 std::vector<MvTile>& tiles) {
+    const int rows = lz_rows(variant), cols = lz_cols(variant);
+    for (int i : which) {
+        const int dp = (inst[i].d + 15) & ~15;
+        for (int r = 0; r < dp; r += rows)
+            for (int64_t c = 0; c < inst[i].n; c += cols) tiles.push_back(MvTile{i, r, c});
+    }
+    // Largest instances first (longest k loops), so the tail of the launch is short.
+    std::stable_sort(tiles.begin(), tiles.end(), [&](const MvTile& a, const MvTile& b) {
+        return inst[a.inst].d + 0 > inst[b.inst].d + 0;
+    });
+}
+
+int grid_for(int variant, size_t n_tiles) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int per_cu = variant == 0 ? 3 : 4;
+    return (int)std::min<size_t>(n_tiles, (size_t)cus * per_cu);
+}
+
+// X = L*Z (+mean) for every instance: group instances by tuned variant, one
+// persistent launch per group.
+int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, const uint16_t* lhi, const uint16_t* llo,
+           const float* z, float* x, hipStream_t s, Scratch& sc) {
+    std::map<int, std::vector<int>> by_bucket;
+    for (int i = 0; i < (int)inst.size(); ++i) by_bucket[bucket_of(inst[i].d)].push_back(i);
+    std::map<int, std::vector<int>> by_variant;
+    for (auto& kv : by_bucket) {
+        int v;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tuned.find(kv.first);
+            v = it != g_tuned.end() ? it->second : -1;
+        }
+        if (v < 0 && autotune_enabled()) {
+            float best = 1e30f;
+            int best_v = default_variant(kv.first == 0 ? 64 : kv.first == 1 ? 128 : 256);
+            for (int cand = 0; cand < kLzVariants; ++cand) {
+                std::vector<MvTile> tiles;
+                build_tiles(inst, kv.second, cand, tiles);
+                MvTile* d_tiles;
+                int32_t* d_cnt;
+                if (sc.upload(&d_tiles, tiles) != hipSuccess || sc.alloc(&d_cnt, 1) != hipSuccess) return ERD_ERR_HIP;
+                hipEvent_t a, b;
+                hipEventCreate(&a);
+                hipEventCreate(&b);
+                hipMemsetAsync(d_cnt, 0, sizeof(int32_t), s);
+                hipEventRecord(a, s);
+                hipError_t e = launch_lz_gemm(cand, grid_for(cand, tiles.size()), d_tiles, (int32_t)tiles.size(),
+                                              d_cnt, d_inst, lhi, llo, z, x, s);
+                hipEventRecord(b, s);
+                hipEventSynchronize(b);
+                float ms = 0;
+                hipEventElapsedTime(&ms, a, b);
+                hipEventDestroy(a);
+                hipEventDestroy(b);
+                if (e != hipSuccess) return ERD_ERR_HIP;
+                if (ms < best) {
+                    best = ms;
+                    best_v = cand;
+                }
+            }
+            v = best_v;
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tuned[kv.first] = v;
+        }
+        if (v < 0) v = default_variant(kv.first == 0 ? 64 : kv.first == 1 ? 128 : 256);
+        auto& dst = by_variant[v];
+        dst.insert(dst.end(), kv.second.begin(), kv.second.end());
+    }
+    for (auto& kv : by_variant) {
+        std::vector<MvTile> tiles;
+        build_tiles(inst, kv.second, kv.first, tiles);
+        MvTile* d_tiles;
+        int32_t* d_cnt;
+        if (sc.upload(&d_tiles, tiles) != hipSuccess || sc.alloc(&d_cnt, 1) != hipSuccess) return ERD_ERR_HIP;
+        if (hipMemsetAsync(d_cnt, 0, sizeof(int32_t), s) != hipSuccess) return ERD_ERR_HIP;
+        hipError_t e = launch_lz_gemm(kv.first, grid_for(kv.first, tiles.size()), d_tiles, (int32_t)tiles.size(),
+                                      d_cnt, d_inst, lhi, llo, z, x, s);
+        if (e != hipSuccess) return ERD_ERR_HIP;
+    }
+    return ERD_OK;
+}
+
+// FP64 Cholesky of n matrices (device, concatenated column-major, in place),
+// then bf16 hi/lo split and float copy of L. Returns ERD_ERR_NOT_PSD if any
+// pivot is not positive (info filled per instance).
+int factor_and_split(const std::vector<int32_t>& dims, double* d_a, const std::vector<int64_t>& off,
+                     uint16_t** lhi, uint16_t** llo, float** l32, std::vector<int64_t>& split_off,
+                     std::vector<int32_t>* info_out, hipStream_t s, Scratch& sc) {
+    const int n = (int)dims.size();
+    int64_t* d_off;
+    int32_t *d_dims, *d_info;
+    if (sc.upload(&d_off, off) || sc.upload(&d_dims, dims) || sc.alloc(&d_info, n)) return ERD_ERR_HIP;
+    if (launch_cholesky_f64(d_a, d_off, d_dims, d_info, n, s) != hipSuccess) return ERD_ERR_HIP;
+    std::vector<int32_t> info(n);
+    if (hipMemcpyAsync(info.data(), d_info, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return ERD_ERR_HIP;
+    if (info_out) *info_out = info;
+    for (int v : info)
+        if (v) return ERD_ERR_NOT_PSD;
+    if (!lhi) return ERD_OK;
+    split_off.assign(n + 1, 0);
+    int max_dp = 16;
+    int64_t total_l = 0;
+    for (int i = 0; i < n; ++i) {
+        const int dp = (dims[i] + 15) & ~15;
+        max_dp = std::max(max_dp, dp);
+        split_off[i + 1] = split_off[i] + (int64_t)dp * dp;
+        total_l += (int64_t)dims[i] * dims[i];
+    }
+    int64_t* d_soff;
+    if (sc.upload(&d_soff, split_off) || sc.alloc(lhi, split_off[n]) || sc.alloc(llo, split_off[n]) ||
+        sc.alloc(l32, total_l))
+        return ERD_ERR_HIP;
+    if (launch_split_l(d_a, d_off, d_dims, d_soff, *lhi, *llo, *l32, n, max_dp, s) != hipSuccess)
+        return ERD_ERR_HIP;
+    return ERD_OK;
+}
+
+}  // namespace
+}  // namespace erd
+
+using namespace erd;
+
+extern "C" {
+
+int erd_cholesky_f64_batch(int64_t n_inst, const int32_t* dims, double* a, int32_t* info, void* stream) {
+    if (n_inst <= 0 || !dims || !a) return ERD_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<int32_t> vd(dims, dims + n_inst);
+    std::vector<int64_t> off(n_inst + 1, 0);
+    for (int64_t i = 0; i < n_inst; ++i) {
+        if (vd[i] <= 0) return ERD_ERR_INVALID;
+        off[i + 1] = off[i] + (int64_t)vd[i] * vd[i];
+    }
+    Scratch sc(s);
+    std::vector<int32_t> inf;
+    std::vector<int64_t> so;
+    int rc = factor_and_split(vd, a, off, nullptr, nullptr, nullptr, so, &inf, s, sc);
+    if (info) std::memcpy(info, inf.data(), sizeof(int32_t) * (size_t)n_inst);
+    return rc;
+}
+
+int erd_mvnormal_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* samples,
+                       const double* cov, const float* mean, float* out, float* z_out, void* stream) {
+    if (!e || !e->states || n_inst <= 0 || n_inst > e->count || !dims || !samples || !cov || !out)
+        return ERD_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<int32_t> vd(dims, dims + n_inst);
+    std::vector<int64_t> off(n_inst + 1, 0), zoff(n_inst + 1, 0), moff(n_inst + 1, 0);
+    for (int64_t i = 0; i < n_inst; ++i) {
+        if (vd[i] <= 0 || samples[i] < 0 || ((int64_t)vd[i] * samples[i]) % 8 != 0) return ERD_ERR_INVALID;
+        off[i + 1] = off[i] + (int64_t)vd[i] * vd[i];
+        zoff[i + 1] = zoff[i] + (int64_t)vd[i] * samples[i];
+        moff[i + 1] = moff[i] + vd[i];
+    }
+    Scratch sc(s);
+    double* d_a;
+    if (sc.alloc(&d_a, off[n_inst]) != hipSuccess) return ERD_ERR_OOM;
+    if (hipMemcpyAsync(d_a, cov, sizeof(double) * off[n_inst], hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return ERD_ERR_HIP;
+    uint16_t *lhi, *llo;
+    float* l32;
+    std::vector<int64_t> split_off;
+    int rc = factor_and_split(vd, d_a, off, &lhi, &llo, &l32, split_off, nullptr, s, sc);
+    if (rc) return rc;
+    // Z_i: the StdNormalGen fill of d_i x n_i on engine i (MvNormal.h:103-108).
+    float* z = z_out;
+    if (!z && sc.alloc(&z, zoff[n_inst]) != hipSuccess) return ERD_ERR_OOM;
+    std::vector<Segment> segs;
+    std::vector<int32_t> engine_of;
+    for (int64_t i = 0; i < n_inst; ++i) {
+        Segment sg{};
+        sg.kind = SEG_STD_NORMAL_F32;
+        sg.count = (int64_t)vd[i] * samples[i];
+        sg.out = z + zoff[i];
+        segs.push_back(sg);
+        engine_of.push_back((int32_t)i);
+    }
+    rc = run_segments(e, segs, engine_of, s);
+    if (rc) return rc;
+    std::vector<MvInstance> inst(n_inst);
+    for (int64_t i = 0; i < n_inst; ++i)
+        inst[i] = MvInstance{vd[i], 0, samples[i], split_off[i], zoff[i], zoff[i], mean ? mean + moff[i] : nullptr};
+    MvInstance* d_inst;
+    if (sc.upload(&d_inst, inst) != hipSuccess) return ERD_ERR_HIP;
+    return run_lz(inst, d_inst, lhi, llo, z, out, s, sc);
+}
+
+int erd_wishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* df,
+                      const int64_t* samples, const double* scale, float* out, void* stream) {
+    if (!e || !e->states || n_inst <= 0 || n_inst > e->count || !dims || !df || !samples || !scale || !out)
+        return ERD_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<int32_t> vd(dims, dims + n_inst);
+    std::vector<int64_t> off(n_inst + 1, 0), aoff(n_inst + 1, 0), nrm_off(n_inst + 1, 0), chi_off(n_inst + 1, 0);
+    for (int64_t i = 0; i < n_inst; ++i) {
+        const int64_t d = vd[i], n = samples[i];
+        // df > d - 1 (MvNormal.h:205); fills need whole packets
+        if (d <= 0 || n < 0 || df[i] <= d - 1 || n % 8 != 0) return ERD_ERR_INVALID;
+        off[i + 1] = off[i] + d * d;
+        aoff[i + 1] = aoff[i] + d * d * n;
+        nrm_off[i + 1] = nrm_off[i] + n * d * (d - 1) / 2;
+        chi_off[i + 1] = chi_off[i] + d * n;
+    }
+    Scratch sc(s);
+    double* d_a;
+    if (sc.alloc(&d_a, off[n_inst]) != hipSuccess) return ERD_ERR_OOM;
+    if (hipMemcpyAsync(d_a, scale, sizeof(double) * off[n_inst], hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return ERD_ERR_HIP;
+    uint16_t *lhi, *llo;
+    float* l32;
+    std::vector<int64_t> split_off;
+    int rc = factor_and_split(vd, d_a, off, &lhi, &llo, &l32, split_off, nullptr, s, sc);
+    if (rc) return rc;
+    float *nrm, *chi, *amat, *tmat;
+    if (sc.alloc(&nrm, nrm_off[n_inst]) || sc.alloc(&chi, chi_off[n_inst]) || sc.alloc(&amat, aoff[n_inst]) ||
+        sc.alloc(&tmat, aoff[n_inst]))
+        return ERD_ERR_OOM;
+    // Engine program per instance: the normal fill, then chi^2(df - i) for
+    // i = 0..d-1 (ChiSquaredGen = GammaGen(n/2, 2)), as generate() draws them.
+    std::vector<Segment> segs;
+    std::vector<int32_t> engine_of;
+    for (int64_t i = 0; i < n_inst; ++i) {
+        const int64_t d = vd[i], n = samples[i];
+        Segment sg{};
+        sg.kind = SEG_STD_NORMAL_F32;
+        sg.count = n * d * (d - 1) / 2;
+        sg.out = nrm + nrm_off[i];
+        segs.push_back(sg);
+        engine_of.push_back((int32_t)i);
+        for (int64_t k = 0; k < d; ++k) {
+            Segment g{};
+            g.kind = SEG_GAMMA_F32;
+            g.count = n;
+            g.out = chi + chi_off[i] + k * n;
+            const float nu = (float)(df[i] - k);
+            g.d[0] = nu * 0.5f;
+            g.d[1] = 2.0f;
+            segs.push_back(g);
+            engine_of.push_back((int32_t)i);
+        }
+    }
+    rc = run_segments(e, segs, engine_of, s);
+    if (rc) return rc;
+    std::vector<MvInstance> inst(n_inst);
+    for (int64_t i = 0; i < n_inst; ++i)
+        inst[i] = MvInstance{vd[i], 0, (int64_t)vd[i] * samples[i], split_off[i], aoff[i], aoff[i], nullptr};
+    MvInstance* d_inst;
+    int64_t *d_nrm_off, *d_chi_off, *d_aoff;
+    if (sc.upload(&d_inst, inst) || sc.upload(&d_nrm_off, nrm_off) || sc.upload(&d_chi_off, chi_off) ||
+        sc.upload(&d_aoff, aoff))
+        return ERD_ERR_HIP;
+    if (launch_wishart_assemble(d_inst, d_nrm_off, d_chi_off, nrm, chi, amat, (int)n_inst, s) != hipSuccess)
+        return ERD_ERR_HIP;
+    rc = run_lz(inst, d_inst, lhi, llo, amat, tmat, s, sc);
+    if (rc) return rc;
+    std::vector<SyrkTile> tiles;
+    for (int64_t i = 0; i < n_inst; ++i) {
+        const int nb = (vd[i] + 31) / 32;
+        for (int64_t j = 0; j < samples[i]; ++j)
+            for (int a = 0; a < nb; ++a)
+                for (int b = 0; b <= a; ++b) tiles.push_back(SyrkTile{(int32_t)i, (int32_t)j, a, b});
+    }
+    SyrkTile* d_tiles;
+    if (sc.upload(&d_tiles, tiles) != hipSuccess) return ERD_ERR_HIP;
+    if (launch_syrk_tt(d_tiles, (int)tiles.size(), d_inst, tmat, out, d_aoff, s) != hipSuccess) return ERD_ERR_HIP;
+    return ERD_OK;
+}
+
+int erd_dirichlet_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* samples,
+                        const float* alpha, float* out, void* stream) {
+    if (!e || !e->states || n_inst <= 0 || n_inst > e->count || !dims || !samples || !alpha || !out)
+        return ERD_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<int32_t> vd(dims, dims + n_inst);
+    std::vector<int64_t> goff(n_inst + 1, 0), vs(samples, samples + n_inst);
+    int64_t aoff = 0, max_n = 0;
+    for (int64_t i = 0; i < n_inst; ++i) {
+        if (vd[i] <= 0 || vs[i] < 0 || vs[i] % 8 != 0) return ERD_ERR_INVALID;
+        goff[i + 1] = goff[i] + (int64_t)vd[i] * vs[i];
+        max_n = std::max(max_n, vs[i]);
+    }
+    Scratch sc(s);
+    float* g;
+    if (sc.alloc(&g, goff[n_inst]) != hipSuccess) return ERD_ERR_OOM;
+    // One GammaGen(alpha_k) fill of n per category, in order (Multinomial.h:190-199).
+    std::vector<Segment> segs;
+    std::vector<int32_t> engine_of;
+    for (int64_t i = 0; i < n_inst; ++i) {
+        for (int k = 0; k < vd[i]; ++k) {
+            const float a = alpha[aoff + k];
+            if (!(a > 0.0f) || !std::isfinite(a)) return ERD_ERR_INVALID;
+            Segment sg{};
+            sg.kind = SEG_GAMMA_F32;
+            sg.count = vs[i];
+            sg.out = g + goff[i] + (int64_t)k * vs[i];
+            sg.d[0] = a;
+            sg.d[1] = 1.0;
+            segs.push_back(sg);
+            engine_of.push_back((int32_t)i);
+        }
+        aoff += vd[i];
+    }
+    int rc = run_segments(e, segs, engine_of, s);
+    if (rc) return rc;
+    int32_t* d_dims;
+    int64_t *d_samples, *d_goff;
+    if (sc.upload(&d_dims, vd) || sc.upload(&d_samples, vs) || sc.upload(&d_goff, goff)) return ERD_ERR_HIP;
+    if (launch_dirichlet_norm(d_dims, d_samples, d_goff, g, out, (int)n_inst, max_n, s) != hipSuccess)
+        return ERD_ERR_HIP;
+    return ERD_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ discrete / multinomial ---
+namespace erd {
+namespace {
+
+// Host side of DiscreteGen<int32_t, int32_t>'s constructor (Discrete.h:
+// 367-391) and AliasMethod::buildTable (Discrete.h:78-192), in double.
+struct HostTable {
+    std::vector<uint32_t> cdf;
+    std::vector<int32_t> prob, alias;
+    uint32_t bitmask = 0;
+};
+
+int build_table(const double* w, int64_t K, HostTable& t) {
+    if (K <= 0) return ERD_ERR_INVALID;
+    double sum = 0;
+    for (int64_t i = 0; i < K; ++i) sum += w[i];
+    if (!std::isfinite(sum)) return ERD_ERR_NONFINITE;
+    if (K < 16) {
+        std::vector<double> c;
+        double acc = 0;
+        for (int64_t i = 0; i < K; ++i) c.push_back(acc += w[i]);
+        for (double p : c) t.cdf.push_back((uint32_t)(p / c.back() * 0x80000000));
+        return ERD_OK;
+    }
+    const size_t msize = (size_t)K;
+    const size_t nb = (size_t)std::ceil(std::log2((double)msize));
+    const size_t psize = (size_t)1 << nb;
+    t.prob.assign(psize, 0);
+    t.alias.assign(psize, 0);
+    t.bitmask = (uint32_t)(psize - 1);
+    const double mean = sum / psize;
+    std::vector<double> f(psize, 0.0);
+    for (size_t i = 0; i < msize; ++i) f[i] = w[i] / mean;
+    const int32_t imax = 0x7FFFFFFF;
+    size_t over = 0, under = 0, mm;
+    while (over < psize && f[over] < 1) ++over;
+    while (under < psize && f[under] >= 1) ++under;
+    mm = under + 1;
+    while (over < psize && under < psize) {
+        t.prob[under] = (int32_t)(f[under] * (imax + 1.0));
+        t.alias[under] = (int32_t)over;
+        f[over] += f[under] - 1;
+        if (f[over] >= 1 || mm <= over) {
+            for (under = mm; under < psize && f[under] >= 1; ++under) {
+            }
+            mm = under + 1;
+        } else {
+            under = over;
+        }
+        while (over < psize && f[over] < 1) ++over;
+    }
+    for (; over < psize; ++over)
+        if (f[over] >= 1) {
+            t.prob[over] = imax;
+            t.alias[over] = (int32_t)over;
+        }
+    if (under < psize) {
+        t.prob[under] = imax;
+        t.alias[under] = (int32_t)under;
+        for (under = mm; under < msize; ++under)
+            if (f[under] < 1) {
+                t.prob[under] = imax;
+                t.alias[under] = (int32_t)under;
+            }
+    }
+    return ERD_OK;
+}
+
+// Upload a table and fill the segment's table fields.
+int upload_table(const HostTable& t, Segment& sg, Scratch& sc) {
+    if (!t.cdf.empty()) {
+        uint32_t* d;
+        if (sc.upload(&d, t.cdf) != hipSuccess) return ERD_ERR_HIP;
+        sg.in0 = d;
+        sg.in1 = nullptr;
+        sg.d[1] = (double)t.cdf.size();
+        sg.d[2] = 0;
+    } else {
+        int32_t *p, *a;
+        if (sc.upload(&p, t.prob) != hipSuccess || sc.upload(&a, t.alias) != hipSuccess) return ERD_ERR_HIP;
+        sg.in0 = p;
+        sg.in1 = a;
+        sg.d[1] = 0;
+        sg.d[2] = (double)t.bitmask;
+    }
+    return ERD_OK;
+}
+
+}  // namespace
+}  // namespace erd
+
+extern "C" {
+
+int erd_alias_table(const double* weights, int64_t K, int32_t* prob, int32_t* alias, uint32_t* cdf) {
+    if (!weights || K <= 0) return ERD_ERR_INVALID;
+    HostTable t;
+    int rc = build_table(weights, K, t);
+    if (rc) return rc;
+    if (!t.cdf.empty()) {
+        if (cdf) std::memcpy(cdf, t.cdf.data(), 4 * t.cdf.size());
+    } else {
+        if (prob) std::memcpy(prob, t.prob.data(), 4 * t.prob.size());
+        if (alias) std::memcpy(alias, t.alias.data(), 4 * t.alias.size());
+    }
+    return ERD_OK;
+}
+
+int erd_fill_discrete_i32(erd_engines* e, int32_t* out, int64_t per, const double* weights, int64_t K,
+                          void* stream) {
+    if (!e || !e->states || per < 0 || per % 4 != 0 || (per > 0 && !out) || !weights) return ERD_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    HostTable t;
+    int rc = build_table(weights, K, t);
+    if (rc) return rc;
+    Scratch sc(s);
+    std::vector<Segment> segs;
+    std::vector<int32_t> eng;
+    Segment proto{};
+    proto.kind = SEG_DISCRETE_I32;
+    proto.count = per;
+    rc = upload_table(t, proto, sc);
+    if (rc) return rc;
+    for (int64_t g = 0; g < e->count; ++g) {
+        Segment sg = proto;
+        sg.out = out + g * per;
+        segs.push_back(sg);
+        eng.push_back((int32_t)g);
+    }
+    return run_segments(e, segs, eng, s);
+}
+
+int erd_multinomial_batch(erd_engines* e, int64_t n_inst, const int64_t* dims, const int32_t* trials,
+                          const int64_t* samples, const double* weights, int32_t* out, void* stream) {
+    if (!e || !e->states || n_inst <= 0 || n_inst > e->count || !dims || !trials || !samples || !weights ||
+        !out)
+        return ERD_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    Scratch sc(s);
+    std::vector<Segment> segs;
+    std::vector<int32_t> eng;
+    int64_t woff = 0, ooff = 0;
+    for (int64_t i = 0; i < n_inst; ++i) {
+        const int64_t K = dims[i], n = samples[i];
+        const int32_t T = trials[i];
+        if (K <= 0 || n < 0 || T < 0) return ERD_ERR_INVALID;
+        for (int64_t k = 0; k < K; ++k)
+            if (weights[woff + k] < 0) return ERD_ERR_INVALID;  // eigen_assert(probs[i] >= 0)
+        HostTable t;
+        int rc = build_table(weights + woff, K, t);
+        if (rc) return rc;
+        int32_t* o = out + ooff;
+        if (T < std::max<int64_t>(30 * (K - 1), 100)) {
+            if (hipMemsetAsync(o, 0, sizeof(int32_t) * K * n, s) != hipSuccess) return ERD_ERR_HIP;
+            Segment sg{};
+            sg.kind = SEG_MULTINOMIAL_SMALL;
+            sg.count = n;
+            sg.out = o;
+            sg.i0 = T;
+            sg.d[0] = (double)K;
+            rc = upload_table(t, sg, sc);
+            if (rc) return rc;
+            segs.push_back(sg);
+            eng.push_back((int32_t)i);
+        } else {
+            if (n % 8 != 0) return ERD_ERR_INVALID;  // BinomialVGen<float> rows run in Packet8f
+            std::vector<double> probs(weights + woff, weights + woff + K);
+            double sum = 0;
+            for (double v : probs) sum += v;
+            for (double& v : probs) v /= sum;
+            int32_t* r;
+            if (sc.upload(&r, std::vector<int32_t>((size_t)n, T)) != hipSuccess) return ERD_ERR_HIP;
+            double rest_p = 1 - probs[0];
+            for (int64_t row = 0; row + 1 < K; ++row) {
+                Segment sg{};
+                sg.kind = SEG_MULTINOMIAL_ROW;
+                sg.count = n;
+                sg.out = o + row;
+                sg.in0 = r;
+                sg.d[0] = row == 0 ? (float)probs[0] : (float)(probs[row] / rest_p);
+                sg.d[1] = (double)K;
+                if (row > 0) rest_p -= probs[row];
+                segs.push_back(sg);
+                eng.push_back((int32_t)i);
+            }
+            Segment last{};
+            last.kind = SEG_MULTINOMIAL_LAST;
+            last.count = n;
+            last.out = o + (K - 1);
+            last.in0 = r;
+            last.d[1] = (double)K;
+            segs.push_back(last);
+            eng.push_back((int32_t)i);
+        }
+        woff += K;
+        ooff += K * n;
+    }
+    return run_segments(e, segs, eng, s);
+}
+
+}  // extern "C"

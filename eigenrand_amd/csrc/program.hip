@@ -174,6 +174,32 @@ __device__ void run_segment(Eng& e, ProgramLds& s, const Segment& sg, uint32_t* 
         case SEG_DISCARD:
             run_discard(e, n, lane);
             break;
+        case SEG_DISCRETE_I32:
+        case SEG_MULTINOMIAL_SMALL: {
+            DiscreteTable t;
+            t.ncdf = (int32_t)sg.d[1];
+            t.cdf = (const uint32_t*)sg.in0;
+            t.prob = (const int32_t*)sg.in0;
+            t.alias = (const int32_t*)sg.in1;
+            t.bitmask = (uint32_t)sg.d[2];
+            if (sg.kind == SEG_DISCRETE_I32) {
+                DiscreteUnits d{(int32_t*)sg.out, t};
+                op_fixed(e, s.carry_fixed, n, d, lane);
+            } else {
+                op_multinomial_small(e, (int32_t*)sg.out, n, sg.i0, (int32_t)sg.d[0], t, lane);
+            }
+            break;
+        }
+        case SEG_MULTINOMIAL_ROW:
+            op_binomial_row8(e, s.vop.carry, s.vop.res, (int32_t*)sg.out, (int64_t)sg.d[1], (int32_t*)sg.in0, n,
+                             (float)sg.d[0], status, lane);
+            break;
+        case SEG_MULTINOMIAL_LAST: {
+            const int64_t stride = (int64_t)sg.d[1];
+            const int32_t* rr = (const int32_t*)sg.in0;
+            for (int64_t i = lane; i < n; i += 64) ((int32_t*)sg.out)[i * stride] = rr[i];
+            break;
+        }
         default:
             if (lane == 0) atomicOr(status, kStatusBadSegment);
             break;

@@ -172,6 +172,61 @@ typedef struct erd_request {
 
 int erd_run_requests(erd_engines* e, const erd_request* requests, int64_t n_requests, void* stream);
 
+/* ---------------------------------------------------- multivariate batch --- */
+/* Batched FP64 Cholesky (the factorisation MvNormalGen/WishartGen take from
+ * detail::get_lt, MvNormal.h:20-38, done in FP64 on the GPU instead of the
+ * reference's float LLT). `a`: DEVICE, n_inst column-major d_i x d_i matrices
+ * back to back, factored in place (L, upper part zeroed). dims/info: HOST.
+ * Returns ERD_ERR_NOT_PSD if some info[i] != 0 (the reference then falls back
+ * to an eigen-decomposition; this boundary reports it instead). */
+int erd_cholesky_f64_batch(int64_t n_inst, const int32_t* dims, double* a, int32_t* info, void* stream);
+
+/* MvNormalGen<float>(mean_i, cov_i).generate(engine_i, samples_i) for
+ * i < n_inst (MvNormal.h:56-116); instance i uses engine i of `e`.
+ * dims/samples: HOST arrays; cov: DEVICE doubles (column-major d_i x d_i, back
+ * to back); mean: DEVICE floats (d_i each, back to back) or NULL for zero;
+ * out: DEVICE floats, X_i = L_i Z_i + mean_i (d_i x samples_i, column-major,
+ * back to back). Z_i is the engine's StdNormalGen fill of d_i*samples_i
+ * (bit-exact to the reference); optional z_out receives it. L_i comes from
+ * the FP64 Cholesky; L*Z runs as split-BF16 (hi+lo) MFMA with FP32
+ * accumulation. d_i*samples_i must be a multiple of 8. */
+int erd_mvnormal_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* samples,
+                       const double* cov, const float* mean, float* out, float* z_out, void* stream);
+
+/* WishartGen<float>(df_i, scale_i).generate(engine_i, samples_i)
+ * (MvNormal.h:184-302): out_i is d_i x (d_i*samples_i) column-major (sample j
+ * in columns [j*d_i, (j+1)*d_i)). samples_i must be a multiple of 8 and
+ * df_i > d_i - 1. scale: DEVICE doubles as for cov above. */
+int erd_wishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* df,
+                      const int64_t* samples, const double* scale, float* out, void* stream);
+
+/* DirichletGen<float>(alpha_i).generate(engine_i, samples_i)
+ * (Multinomial.h:159-217): out_i is K_i x samples_i column-major. alpha: HOST
+ * floats (K_i each, back to back); samples_i must be a multiple of 8. */
+int erd_dirichlet_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* samples,
+                        const float* alpha, float* out, void* stream);
+
+/* ------------------------------------------------- discrete / multinomial --- */
+/* Host-side table of DiscreteGen<int32_t, int32_t> (Discrete.h:367-391): for
+ * K < 16 the uint32 CDF (K entries) into cdf; otherwise AliasMethod
+ * (Discrete.h:78-192) prob/alias of 2^ceil(log2 K) entries. HOST arrays.
+ * ERD_ERR_NONFINITE if the weights do not sum to a finite value. */
+int erd_alias_table(const double* weights, int64_t K, int32_t* prob, int32_t* alias, uint32_t* cdf);
+
+/* discrete<int32>(…, weights): DiscreteGen<int32_t, int32_t> packet path
+ * (Discrete.h:443-483, AVX2: a Packet8i per draw handed out as two Packet4i).
+ * weights: HOST doubles; per_engine a multiple of 4. */
+int erd_fill_discrete_i32(erd_engines* e, int32_t* out, int64_t per_engine, const double* weights, int64_t K,
+                          void* stream);
+
+/* MultinomialGen<int32_t>(trials_i, weights_i).generate(engine_i, samples_i)
+ * (Multinomial.h:25-134): out_i is K_i x samples_i column-major int32.
+ * dims/trials/samples/weights: HOST; out: DEVICE. trials < max(30(K-1),100)
+ * uses per-sample alias/CDF draws, otherwise K-1 BinomialVGen<float> rows
+ * (samples_i must then be a multiple of 8). */
+int erd_multinomial_batch(erd_engines* e, int64_t n_inst, const int64_t* dims, const int32_t* trials,
+                          const int64_t* samples, const double* weights, int32_t* out, void* stream);
+
 /* -------------------------------------------------------- device memory --- */
 /* Small helpers so callers without another GPU runtime can use the library. */
 int erd_device_alloc(void** ptr, size_t bytes);

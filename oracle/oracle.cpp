@@ -365,11 +365,20 @@ struct Binomial4 {
 // BinomialVGen<int32_t>::packetOp (Discrete.h:1124-1192): per-lane float
 // parameters, lockstep Knuth loop for lanes with mean < 1, then lockstep
 // tangent rejection keeping each lane's first accepted candidate.
-void binomialv4(Vmt& e, const int32_t trials[4], const float p[4], int32_t out[4]) {
-    float psmall[4], ptr[4], pmean[4], pg1[4], psq[4], plp[4], plq[4];
-    bool valid[4];
-    float valid_res[4] = {0, 0, 0, 0};
-    for (int j = 0; j < 4; ++j) {
+template <int W>
+inline void urW(Vmt& e, float* u) {
+    if (W == 4) ur4(e, u); else ur8(e, u);
+}
+
+// W = 4: BinomialVGen<int32_t> at Packet4i/Packet4f (half draws); W = 8:
+// BinomialVGen<float> at Packet8f (full draws, Discrete.h:1251-1330, used by
+// MultinomialGen with int32 trials stored in float lanes).
+template <int W>
+void binomialv(Vmt& e, const int32_t trials[W], const float p[W], int32_t out[W]) {
+    float psmall[W], ptr[W], pmean[W], pg1[W], psq[W], plp[W], plq[W];
+    bool valid[W];
+    float valid_res[W] = {0};
+    for (int j = 0; j < W; ++j) {
         psmall[j] = (p[j] < 1.0f - p[j]) ? p[j] : 1.0f - p[j];   // pmin(p, 1-p)
         ptr[j] = (float)trials[j];
         pmean[j] = ptr[j] * psmall[j];
@@ -379,32 +388,33 @@ void binomialv4(Vmt& e, const int32_t trials[4], const float p[4], int32_t out[4
         plq[j] = plog(1.0f - psmall[j]);
         valid[j] = pmean[j] < 1.0f;
     }
-    bool any_valid = valid[0] || valid[1] || valid[2] || valid[3];
+    bool any_valid = false, all_valid = true;
+    for (int j = 0; j < W; ++j) { any_valid |= valid[j]; all_valid &= valid[j]; }
     if (any_valid) {
-        int32_t res[4] = {0, 0, 0, 0};
-        float val[4] = {1, 1, 1, 1}, nem[4];
-        for (int j = 0; j < 4; ++j) nem[j] = pexp(-pmean[j]);
+        int32_t res[W] = {0};
+        float val[W], nem[W];
+        for (int j = 0; j < W; ++j) val[j] = 1.0f;
+        for (int j = 0; j < W; ++j) nem[j] = pexp(-pmean[j]);
         for (;;) {
-            float u[4];
-            ur4(e, u);
-            bool any = false, c[4];
-            for (int j = 0; j < 4; ++j) {
+            float u[W];
+            urW<W>(e, u);
+            bool any = false, c[W];
+            for (int j = 0; j < W; ++j) {
                 val[j] = val[j] * u[j];
                 c[j] = (nem[j] < val[j]) && valid[j];
                 any |= c[j];
             }
             if (!any) break;
-            for (int j = 0; j < 4; ++j) res[j] += c[j] ? 1 : 0;
+            for (int j = 0; j < W; ++j) res[j] += c[j] ? 1 : 0;
         }
-        for (int j = 0; j < 4; ++j) valid_res[j] = (float)res[j];
+        for (int j = 0; j < W; ++j) valid_res[j] = (float)res[j];
     }
-    bool all_valid = valid[0] && valid[1] && valid[2] && valid[3];
     if (!all_valid) {
         for (;;) {
-            float u1[4], u2[4], fres[4], p12[4];
-            bool c1[4];
-            ur4(e, u1);
-            for (int j = 0; j < 4; ++j) {
+            float u1[W], u2[W], fres[W], p12[W];
+            bool c1[W];
+            urW<W>(e, u1);
+            for (int j = 0; j < W; ++j) {
                 float s, c;
                 psincos(kPiF * u1[j], s, c);
                 float ys = s / c;
@@ -417,9 +427,9 @@ void binomialv4(Vmt& e, const int32_t trials[4], const float p[4], int32_t out[4
                 c1[j] = (0.0f <= fres[j]) && (fres[j] <= ptr[j]);
                 p12[j] = pp1 * pp2;
             }
-            ur4(e, u2);
+            urW<W>(e, u2);
             bool all = true;
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < W; ++j) {
                 bool fv = c1[j] && (u2[j] <= p12[j]);
                 if (fv && !valid[j]) valid_res[j] = fres[j];
                 valid[j] = valid[j] || fv;
@@ -428,7 +438,7 @@ void binomialv4(Vmt& e, const int32_t trials[4], const float p[4], int32_t out[4
             if (all) break;
         }
     }
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < W; ++j) {
         int32_t k = (int32_t)valid_res[j];
         out[j] = (p[j] == psmall[j]) ? k : trials[j] - k;
     }
@@ -647,7 +657,7 @@ void orc_fill_binomial_vop_i32(void* states, int64_t n_engines, int32_t* out, in
     for_engines(n_engines, threads, [&](int64_t g) {
         int64_t base = g * per_engine;
         for (int64_t i = 0; i < per_engine; i += 4)
-            binomialv4(e[g], trials + base + i, p + base + i, out + base + i);
+            binomialv<4>(e[g], trials + base + i, p + base + i, out + base + i);
     });
 }
 
@@ -822,5 +832,210 @@ void orc_fill_poisson_vop_i32(void* states, int64_t n_engines, int32_t* out, int
         int64_t b = g * per;
         for (int64_t i = 0; i < per; i += 4) poissonv4(e[g], mean + b + i, out + b + i);
     });
+}
+}
+
+// ----------------------------------------------------------------------------
+// DiscreteGen<int32_t, int32_t> and MultinomialGen (Discrete.h:23-207,
+// 352-485; MvDists/Multinomial.h:25-134).
+namespace {
+
+// AliasMethod<int32_t, int32_t>::buildTable (Discrete.h:78-192), in double.
+struct AliasTable {
+    std::vector<int32_t> prob, alias;
+    size_t bitsize = 0, bitmask = 0;
+};
+
+bool build_alias(const double* w, size_t msize, AliasTable& t) {
+    double sum = 0;
+    for (size_t i = 0; i < msize; ++i) sum += w[i];
+    if (!std::isfinite(sum)) return false;
+    const size_t nbsize = (size_t)std::ceil(std::log2((double)msize));
+    const size_t psize = (size_t)1 << nbsize;
+    t.prob.assign(psize, 0);
+    t.alias.assign(psize, 0);
+    t.bitsize = nbsize;
+    t.bitmask = ((size_t)(-1)) >> (sizeof(size_t) * 8 - nbsize);
+    sum /= psize;
+    std::vector<double> f(psize, 0.0);
+    for (size_t i = 0; i < msize; ++i) f[i] = w[i] / sum;
+    const int32_t imax = std::numeric_limits<int32_t>::max();
+    size_t over = 0, under = 0, mm;
+    while (over < psize && f[over] < 1) ++over;
+    while (under < psize && f[under] >= 1) ++under;
+    mm = under + 1;
+    while (over < psize && under < psize) {
+        t.prob[under] = (int32_t)(f[under] * (imax + 1.0));
+        t.alias[under] = (int32_t)over;
+        f[over] += f[under] - 1;
+        if (f[over] >= 1 || mm <= over) {
+            for (under = mm; under < psize && f[under] >= 1; ++under);
+            mm = under + 1;
+        } else {
+            under = over;
+        }
+        while (over < psize && f[over] < 1) ++over;
+    }
+    for (; over < psize; ++over) {
+        if (f[over] >= 1) {
+            t.prob[over] = imax;
+            t.alias[over] = (int32_t)over;
+        }
+    }
+    if (under < psize) {
+        t.prob[under] = imax;
+        t.alias[under] = (int32_t)under;
+        for (under = mm; under < msize; ++under) {
+            if (f[under] < 1) {
+                t.prob[under] = imax;
+                t.alias[under] = (int32_t)under;
+            }
+        }
+    }
+    return true;
+}
+
+struct Discrete {
+    std::vector<uint32_t> cdf;  // K < 16
+    AliasTable at;              // K >= 16
+    bool valid = false;         // AVX2: upper Packet4i of the last Packet8i
+    int32_t cache[4];
+
+    bool init(const double* w, size_t K) {
+        if (K < 16) {
+            std::vector<double> c;
+            double acc = 0;
+            for (size_t i = 0; i < K; ++i) c.push_back(acc += w[i]);
+            for (double p : c) cdf.push_back((uint32_t)(p / c.back() * 0x80000000));
+            return true;
+        }
+        return build_alias(w, K, at);
+    }
+    int32_t from_word_packet(uint32_t word) const {
+        if (!cdf.empty()) {
+            const int32_t rx = (int32_t)(word & 0x7FFFFFFFu);
+            int32_t ret = (int32_t)cdf.size() - 1;
+            for (size_t i = 0; i + 1 < cdf.size(); ++i) ret += (rx < (int32_t)cdf[i]) ? -1 : 0;
+            return ret;
+        }
+        const int32_t albit = (int32_t)(word & (uint32_t)at.bitmask);
+        const bool c = (int32_t)(word >> 1) < at.prob[albit];
+        return c ? albit : at.alias[albit];
+    }
+    // packetOp<Packet4i> under AVX2 (Discrete.h:443-483).
+    void packet4(Vmt& e, int32_t out[4]) {
+        if (valid) {
+            valid = false;
+            std::memcpy(out, cache, 16);
+            return;
+        }
+        uint64_t p[4];
+        vmt_draw(e, p);
+        int32_t ret[8];
+        for (int j = 0; j < 8; ++j) ret[j] = from_word_packet((uint32_t)(p[j >> 1] >> (32 * (j & 1))));
+        valid = true;
+        std::memcpy(cache, ret + 4, 16);
+        std::memcpy(out, ret, 16);
+    }
+    // operator() (Discrete.h:428-441): one full draw, first 32-bit word.
+    int32_t scalar(Vmt& e) {
+        uint64_t p[4];
+        vmt_draw(e, p);
+        const int32_t rx = (int32_t)(uint32_t)p[0];
+        if (!cdf.empty()) {
+            const int32_t r = rx & 0x7FFFFFFF;
+            return (int32_t)(std::lower_bound(cdf.begin(), cdf.end() - 1, (uint32_t)r) - cdf.begin());
+        }
+        const size_t albit = (size_t)(int64_t)rx & at.bitmask;
+        const uint32_t alx = (uint32_t)(rx >> 1);
+        if (alx < (uint32_t)at.prob[albit]) return (int32_t)albit;
+        return at.alias[albit];
+    }
+};
+
+// MultinomialGen::generate(urng, samples) (Multinomial.h:60-133); out is
+// K x samples column-major.
+bool multinomial(Vmt& e, int32_t trials, const double* w, int64_t K, int64_t samples, int32_t* out) {
+    std::vector<double> probs(w, w + K);
+    Discrete disc;
+    if (!disc.init(probs.data(), (size_t)K)) return false;
+    double sum = 0;
+    for (double v : probs) sum += v;
+    for (double& v : probs) v /= sum;
+    std::fill(out, out + K * samples, 0);
+    if (trials < std::max<int64_t>(30 * (K - 1), 100)) {
+        for (int64_t s = 0; s < samples; ++s) {
+            int32_t* col = out + s * K;
+            int32_t i = 0;
+            for (; i < (trials & ~15); i += 16) {
+                int32_t buf[16];
+                for (int q = 0; q < 4; ++q) disc.packet4(e, buf + 4 * q);
+                for (int j = 0; j < 16; ++j) col[buf[j]] += 1;
+            }
+            const int r = trials - i;
+            int32_t buf[16];
+            int j = 0;
+            for (; j + 4 <= r; j += 4) disc.packet4(e, buf + j);
+            for (; j < r; ++j) buf[j] = disc.scalar(e);
+            for (j = 0; j < r; ++j) col[buf[j]] += 1;
+        }
+        return true;
+    }
+    std::vector<int32_t> r_trials((size_t)samples, trials), t((size_t)samples);
+    double rest_p = 1 - probs[0];
+    for (int64_t row = 0; row + 1 < K; ++row) {
+        const float pp = row == 0 ? (float)probs[0] : (float)(probs[row] / rest_p);
+        for (int64_t s = 0; s < samples; s += 8) {
+            float pv[8];
+            for (int q = 0; q < 8; ++q) pv[q] = pp;
+            binomialv<8>(e, r_trials.data() + s, pv, t.data() + s);
+        }
+        for (int64_t s = 0; s < samples; ++s) {
+            r_trials[(size_t)s] -= t[(size_t)s];
+            out[s * K + row] = t[(size_t)s];
+        }
+        if (row > 0) rest_p -= probs[row];
+    }
+    for (int64_t s = 0; s < samples; ++s) out[s * K + K - 1] = r_trials[(size_t)s];
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+// Alias table of AliasMethod<int32_t> for weights w (K >= 16); prob/alias
+// must hold 2^ceil(log2 K) entries. Returns the bit size, or -1.
+int orc_alias_table(const double* w, int64_t K, int32_t* prob, int32_t* alias) {
+    AliasTable t;
+    if (!build_alias(w, (size_t)K, t)) return -1;
+    std::memcpy(prob, t.prob.data(), 4 * t.prob.size());
+    std::memcpy(alias, t.alias.data(), 4 * t.alias.size());
+    return (int)t.bitsize;
+}
+
+// Batched MultinomialGen: instance i on engine i; out_i K_i x samples_i.
+int orc_multinomial(void* states, int64_t n_inst, const int32_t* trials, const int64_t* K,
+                    const int64_t* samples, const double* w, int32_t* out) {
+    Vmt* e = (Vmt*)states;
+    int64_t woff = 0, ooff = 0;
+    for (int64_t i = 0; i < n_inst; ++i) {
+        if (!multinomial(e[i], trials[i], w + woff, K[i], samples[i], out + ooff)) return -1;
+        woff += K[i];
+        ooff += K[i] * samples[i];
+    }
+    return 0;
+}
+
+// DiscreteGen<int32_t, int32_t> fill (packet path, AVX2 Packet4i halves of a
+// Packet8i): per engine `per` draws, a multiple of 4.
+int orc_fill_discrete_i32(void* states, int64_t n_engines, int32_t* out, int64_t per, const double* w,
+                          int64_t K) {
+    Vmt* e = (Vmt*)states;
+    for (int64_t g = 0; g < n_engines; ++g) {
+        Discrete d;
+        if (!d.init(w, (size_t)K)) return -1;
+        for (int64_t i = 0; i < per; i += 4) d.packet4(e[g], out + g * per + i);
+    }
+    return 0;
 }
 }

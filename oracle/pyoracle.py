@@ -214,3 +214,38 @@ def fill_poisson_vop_i32(e, mean, threads=1):
     out = np.empty(mean.size, dtype=np.int32)
     lib().orc_fill_poisson_vop_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), _p(mean), C.c_int(threads))
     return out
+
+
+def alias_table(w):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    size = 1 << int(np.ceil(np.log2(w.size)))
+    prob = np.zeros(size, dtype=np.int32)
+    alias = np.zeros(size, dtype=np.int32)
+    bits = lib().orc_alias_table(_p(w), I64(w.size), _p(prob), _p(alias))
+    assert bits >= 0
+    return prob, alias
+
+
+def fill_discrete_i32(e, per, w):
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    out = np.empty(len(e) * per, dtype=np.int32)
+    rc = lib().orc_fill_discrete_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), _p(w), I64(w.size))
+    assert rc == 0
+    return out
+
+
+def multinomial(e, trials, weights, samples):
+    """Batched MultinomialGen on engines 0..n-1 of e; returns list of (K, samples)."""
+    n = len(weights)
+    K = np.array([len(w) for w in weights], dtype=np.int64)
+    trials = np.broadcast_to(np.asarray(trials, dtype=np.int32), (n,)).copy()
+    samples = np.broadcast_to(np.asarray(samples, dtype=np.int64), (n,)).copy()
+    w = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float64) for x in weights]))
+    out = np.empty(int((K * samples).sum()), dtype=np.int32)
+    rc = lib().orc_multinomial(_p(e.raw), I64(n), _p(trials), _p(K), _p(samples), _p(w), _p(out))
+    assert rc == 0
+    res, off = [], 0
+    for k, m in zip(K, samples):
+        res.append(out[off:off + k * m].reshape(m, k).T)
+        off += k * m
+    return res

@@ -1,0 +1,190 @@
+"""Batched multivariate generators on the GPU against the FP64 oracle.
+
+The univariate draws inside each generator are bit-exact to the oracle (the
+standard normals Z of MvNormal are compared directly); the float results of
+the products are bounded against float64 products of the same draws, with
+the bound written in each test; distribution checks follow test_mv.cpp
+(EXPECT_SIMILAR_MATRIX, 10% relative)."""
+import numpy as np
+import pytest
+
+from test_gpu_fixed import assert_same
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_cov(rng, d):
+    a = rng.normal(0, 1 / np.sqrt(d), (d, d))
+    return a @ a.T + d * np.eye(d)  # cov = A A^T + d I  (config 4)
+
+
+def test_cholesky_fp64_matches_lapack(er):
+    rng = np.random.default_rng(1)
+    dims = [1, 4, 17, 64, 100, 256, 300]
+    mats = [rand_cov(rng, d) for d in dims]
+    ls, info, rc = er.cholesky_batch(mats)
+    assert rc == 0 and not info.any()
+    for m, l in zip(mats, ls):
+        ref = np.linalg.cholesky(m)
+        assert np.allclose(l, ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+        assert np.allclose(np.triu(l, 1), 0)
+
+
+def test_cholesky_reports_not_psd(er):
+    m = np.diag([1.0, -1.0, 2.0])
+    _, info, rc = er.cholesky_batch([np.eye(2), m])
+    assert rc == 3 and info[0] == 0 and info[1] == 2
+
+
+# split-BF16 (hi+lo) with FP32 accumulation: hi+lo keeps 16 significant bits
+# of each operand (residual <= 2^-18 relative) and the dropped lo*lo term is
+# <= 2^-18, so per element
+#   |X - X64| <= (2^-16 + d * 2^-24) * sum_k |L_ik||Z_kj| + 2^-22 |mean_i|
+# (the d * 2^-24 term is the FP32 accumulation). Tested bound.
+def lz_bound(d):
+    return 2.0 ** -16 + d * 2.0 ** -24
+
+
+@pytest.mark.parametrize("dims", [[4, 9, 16, 17, 31, 64], [65, 100, 128, 200, 256], [300, 512]])
+def test_mvnormal_batch_against_fp64(er, orc, dims):
+    import orc_mv
+    rng = np.random.default_rng(sum(dims))
+    samples = [64 if d % 2 else 40 for d in dims]
+    covs = [rand_cov(rng, d) for d in dims]
+    means = [rng.uniform(-10, 10, d).astype(np.float32) for d in dims]
+    seeds = [1000 + 4 * i for i in range(len(dims))]
+    urng = er.Vmt19937_64(seeds=seeds)
+    xs, zs = er.mvnormal_batch(urng, covs, samples, means, return_z=True)
+    worst = 0.0
+    for i, d in enumerate(dims):
+        x64, z = orc_mv.mvnormal(orc.Engines(seeds=[seeds[i]]), covs[i], samples[i], means[i])
+        assert_same(zs[i].t().contiguous().cpu().numpy(), z.T)  # the normals are bit-exact
+        x = xs[i].cpu().numpy().astype(np.float64)
+        scale = np.abs(np.linalg.cholesky(covs[i])) @ np.abs(z.astype(np.float64))
+        err = np.abs(x - x64) / (scale + np.abs(means[i])[:, None] * 2.0 ** -6 / d)
+        worst = max(worst, float(err.max()))
+        assert err.max() <= lz_bound(d), (d, err.max())
+    print("max |X-X64| / (|L||Z|) =", worst)
+
+
+def test_mvnormal_distribution_9d(er):
+    """test_mv.cpp:98-122: 9-d normal, sample mean/cov within 10%."""
+    rng = np.random.default_rng(9)
+    d = 9
+    mean = np.arange(d, dtype=np.float32)
+    cov = rand_cov(rng, d)
+    gen = er.MvNormalGen(mean, cov)
+    urng = er.Vmt19937_64(42)
+    x = gen.generate(urng, 8000).cpu().numpy().astype(np.float64)
+    assert np.allclose(x.mean(axis=1), mean, atol=0.1 * np.sqrt(np.diag(cov)).max())
+    assert np.allclose(np.cov(x), cov, rtol=0.1, atol=0.1 * np.abs(cov).max())
+
+
+def test_wishart_batch_against_fp64(er, orc):
+    import orc_mv
+    rng = np.random.default_rng(3)
+    dims = [4, 7, 33, 64]
+    scales = [rand_cov(rng, d) for d in dims]
+    df = [d + 3 for d in dims]
+    samples = [16, 8, 8, 8]
+    seeds = [50, 60, 70, 80]
+    urng = er.Vmt19937_64(seeds=seeds)
+    ws = er.wishart_batch(urng, scales, df, samples)
+    for i, d in enumerate(dims):
+        w64, a = orc_mv.wishart(orc.Engines(seeds=[seeds[i]]), df[i], scales[i], samples[i])
+        w = ws[i].cpu().numpy().astype(np.float64)
+        # bound: |W - W64| <= 2^-12 * (|T| |T|^T) with |T| = |L| |A|
+        t = np.abs(np.linalg.cholesky(scales[i])) @ np.abs(a.astype(np.float64))
+        for j in range(samples[i]):
+            tj = t[:, j * d:(j + 1) * d]
+            bound = 2.0 ** -12 * (tj @ tj.T)
+            assert np.all(np.abs(w[:, j * d:(j + 1) * d] - w64[:, j * d:(j + 1) * d]) <= bound), (d, j)
+            assert np.array_equal(w[:, j * d:(j + 1) * d], w[:, j * d:(j + 1) * d].T)  # symmetric
+
+
+def test_wishart_mean(er):
+    """test_mv.cpp:124-150: E[W] = df * scale within 10%."""
+    rng = np.random.default_rng(4)
+    d, df, n = 5, 8, 4096
+    scale = rand_cov(rng, d)
+    w = er.WishartGen(df, scale).generate(er.Vmt19937_64(7), n).cpu().numpy().astype(np.float64)
+    mean = w.reshape(d, n, d).mean(axis=1)
+    assert np.allclose(mean, df * scale, rtol=0.1, atol=0.1 * df * np.abs(scale).max())
+
+
+def test_dirichlet_batch_against_oracle(er, orc):
+    import orc_mv
+    rng = np.random.default_rng(5)
+    ks = [2, 3, 17, 100]
+    alphas = [np.exp(rng.uniform(np.log(0.01), np.log(10), k)).astype(np.float32) for k in ks]
+    samples = [64, 32, 16, 24]
+    seeds = [5, 6, 7, 8]
+    urng = er.Vmt19937_64(seeds=seeds)
+    res = er.dirichlet_batch(urng, alphas, samples)
+    for i, k in enumerate(ks):
+        r64, g = orc_mv.dirichlet(orc.Engines(seeds=[seeds[i]]), alphas[i], samples[i])
+        r = res[i].cpu().numpy().astype(np.float64)
+        # gamma draws are bit-exact; the float sum over K categories and the
+        # division add at most (K+1) float ulps relative to the float64 result
+        assert np.all(np.abs(r - r64) <= (k + 1) * 2.0 ** -23 * np.abs(r64) + 1e-38), k
+        assert np.allclose(r.sum(axis=0), 1, atol=(k + 1) * 2.0 ** -22)
+
+
+def test_dirichlet_mean(er):
+    """test_mv.cpp:203-222: E[x_k] = alpha_k / sum(alpha) within 10%."""
+    alpha = np.float32([0.5, 1.0, 2.5, 4.0])
+    x = er.DirichletGen(alpha).generate(er.Vmt19937_64(11), 8192).cpu().numpy()
+    assert np.allclose(x.mean(axis=1), alpha / alpha.sum(), rtol=0.1)
+
+
+def test_alias_table_matches_oracle(er, orc):
+    rng = np.random.default_rng(12)
+    for K in (3, 15, 16, 100, 1000, 10000):
+        w = rng.exponential(1.0, K)
+        t = er.alias_table(w)
+        if K < 16:
+            c = np.cumsum(w)
+            np.testing.assert_array_equal(t[1], (c / c[-1] * 0x80000000).astype(np.uint32))
+        else:
+            prob, alias = orc.alias_table(w)
+            np.testing.assert_array_equal(t[1], prob)
+            np.testing.assert_array_equal(t[2], alias)
+
+
+@pytest.mark.parametrize("K", [4, 16, 37, 1000])
+def test_discrete_fill_matches_oracle(er, orc, K):
+    rng = np.random.default_rng(K)
+    w = rng.uniform(0, 1, K)
+    seeds = [1, 2, 3]
+    urng = er.Vmt19937_64(seeds=seeds)
+    ref = orc.Engines(seeds=seeds)
+    for n in (4, 8, 1004, 12):
+        out = er.discrete(n, 1, urng, w)
+        assert_same(out.reshape(len(seeds), n).cpu().numpy(), orc.fill_discrete_i32(ref, n, w))
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+
+
+@pytest.mark.parametrize("cases", [
+    [(20, 4, 16), (99, 10, 24), (7, 3, 8), (100, 1, 8)],           # per-sample discrete draws
+    [(1000, 4, 64), (5000, 100, 16), (3001, 3, 8)],                # BinomialVGen<float> rows
+    [(50, 1000, 8), (1, 20, 16), (250, 12, 40), (2000, 16, 24)],  # mixed
+])
+def test_multinomial_batch_matches_oracle(er, orc, cases):
+    rng = np.random.default_rng(len(cases))
+    trials = [c[0] for c in cases]
+    weights = [rng.uniform(0.05, 1, c[1]) for c in cases]
+    samples = [c[2] for c in cases]
+    seeds = [10 + 4 * i for i in range(len(cases))]
+    urng = er.Vmt19937_64(seeds=seeds)
+    got = er.multinomial_batch(urng, trials, weights, samples)
+    exp = orc.multinomial(orc.Engines(seeds=seeds), trials, weights, samples)
+    for g, x, t in zip(got, exp, trials):
+        g = g.cpu().numpy()
+        np.testing.assert_array_equal(g, x)
+        assert np.all(g.sum(axis=0) == t)
+
+
+def test_multinomial_rejects_nonfinite(er):
+    urng = er.Vmt19937_64(1)
+    with pytest.raises(er.ErdError):
+        er.multinomial_batch(urng, [10], [[1.0, np.inf, 2.0]], [8])
