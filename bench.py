@@ -54,6 +54,72 @@ def cpu_baseline(max_threads=16):
     }
 
 
+def timed(fn, reps=5, warmup=1):
+    """Average ms of fn() over reps, HIP events on the current stream."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def sweep():
+    """Secondary measurements: the univariate sweep (2^28 f32 per distribution,
+    4096 engines), vectorize-over-params (2^24 elements) and the batched
+    multivariate configuration. One JSON object per line on stdout."""
+    import numpy as np
+    import torch
+    import eigenrand_amd as er
+
+    def emit(name, n, ms, bytes_per):
+        print(json.dumps({"sweep": name, "elements": n, "ms": round(ms, 4),
+                          "gsamples_per_s": round(n / ms / 1e6, 3),
+                          "hbm_gbs": round(n * bytes_per / ms / 1e6, 1),
+                          "hbm_frac": round(n * bytes_per / ms / 1e6 / HBM_PEAK_GBS, 4)}), flush=True)
+
+    urng = er.Vmt19937_64(0, n=N_ENGINES, stride=4)
+    out = torch.empty(N_SAMPLES, dtype=torch.float32, device="cuda")
+    outi = torch.empty(N_SAMPLES, dtype=torch.int32, device="cuda")
+    univ = [
+        ("uniformReal f32", lambda: er.uniform_real(PER_ENGINE, 1, urng, out=out)),
+        ("normal(0,1) f32", lambda: er.normal(PER_ENGINE, 1, urng, out=out)),
+        ("exponential(1) f32", lambda: er.exponential(PER_ENGINE, 1, urng, 1.0, out=out)),
+        ("gamma(0.5,1) f32", lambda: er.gamma(PER_ENGINE, 1, urng, 0.5, 1.0, out=out)),
+        ("gamma(1,1) f32", lambda: er.gamma(PER_ENGINE, 1, urng, 1.0, 1.0, out=out)),
+        ("gamma(3.7,1) f32", lambda: er.gamma(PER_ENGINE, 1, urng, 3.7, 1.0, out=out)),
+        ("studentT(5) f32", lambda: er.student_t(PER_ENGINE, 1, urng, 5.0, out=out)),
+        ("cauchy(0,1) f32", lambda: er.cauchy(PER_ENGINE, 1, urng, 0.0, 1.0, out=out)),
+        ("poisson(16) i32", lambda: er.poisson(PER_ENGINE, 1, urng, 16.0, out=outi)),
+        ("binomial(100,0.75) i32", lambda: er.binomial(PER_ENGINE, 1, urng, 100, 0.75, out=outi)),
+    ]
+    for name, fn in univ:
+        emit(name, N_SAMPLES, timed(fn, reps=3), 4)
+
+    # vectorize-over-params, 2^24 elements, parameters from mt19937_64(2024)
+    nv = 1 << 24
+    rng = np.random.Generator(np.random.MT19937(2024))
+    vp = er.Vmt19937_64(0, n=N_ENGINES, stride=4)
+    mu = torch.from_numpy(rng.uniform(-10, 10, nv).astype(np.float32)).cuda()
+    sd = torch.from_numpy(rng.uniform(0.1, 5, nv).astype(np.float32)).cuda()
+    shape = torch.from_numpy(np.exp(rng.uniform(np.log(0.1), np.log(50), nv)).astype(np.float32)).cuda()
+    ones = torch.ones(nv, dtype=torch.float32, device="cuda")
+    bn = torch.from_numpy(rng.integers(1, 1001, nv).astype(np.int32)).cuda()
+    bp = torch.from_numpy(rng.uniform(0, 1, nv).astype(np.float32)).cuda()
+    pm = torch.from_numpy(np.exp(rng.uniform(np.log(0.01), np.log(1e4), nv)).astype(np.float32)).cuda()
+    vo = torch.empty(nv, dtype=torch.float32, device="cuda")
+    vi = torch.empty(nv, dtype=torch.int32, device="cuda")
+    emit("VoP normal f32", nv, timed(lambda: er.normal_vop(vp, mu, sd, out=vo), reps=3), 12)
+    emit("VoP gamma f32", nv, timed(lambda: er.gamma_vop(vp, shape, ones, out=vo), reps=3), 12)
+    emit("VoP binomial i32", nv, timed(lambda: er.binomial_vop(vp, bn, bp, out=vi), reps=3), 12)
+    emit("VoP poisson i32", nv, timed(lambda: er.poisson_vop(vp, pm, out=vi), reps=3), 8)
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "normal_f32_2p28_traffic.json")
     if not os.path.exists(p):
@@ -68,6 +134,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--sweep", action="store_true", help="secondary per-distribution measurements")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the sampling path runs on a single MI355X "
@@ -78,6 +147,17 @@ def main():
     import eigenrand_amd as er
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        src = torch.empty(N_SAMPLES, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * N_SAMPLES, "copy_bytes_written": 4 * N_SAMPLES}))
+        return
+    if args.sweep:
+        sweep()
+        return
     stream = torch.cuda.current_stream()
     urng = er.Vmt19937_64(0, n=N_ENGINES, stride=4)
     out = torch.empty(N_SAMPLES, dtype=torch.float32, device="cuda")
