@@ -1,0 +1,63 @@
+// Internal host-side helpers shared by the C ABI sources (not exported).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <vector>
+#include "erd_capi_internal.hpp"
+#include "erd_mv.hpp"
+#include "erd_segment.hpp"
+
+struct erd_request;
+
+namespace erd {
+
+// Stream-ordered scratch released when the object goes out of scope.
+struct Scratch {
+    hipStream_t s;
+    std::vector<void*> ptrs;
+    explicit Scratch(hipStream_t st) : s(st) {}
+    ~Scratch() {
+        for (void* p : ptrs) (void)hipFreeAsync(p, s);
+    }
+    template <class T>
+    hipError_t alloc(T** p, size_t count) {
+        *p = nullptr;
+        if (count == 0) count = 1;
+        hipError_t e = hipMallocAsync((void**)p, sizeof(T) * count, s);
+        if (e == hipSuccess) ptrs.push_back(*p);
+        return e;
+    }
+    // Host vectors may die right after the call, so the copy is completed
+    // before returning.
+    template <class T>
+    hipError_t upload(T** p, const std::vector<T>& v) {
+        hipError_t e = alloc(p, v.size());
+        if (e == hipSuccess && !v.empty())
+            e = hipMemcpyAsync(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        return e;
+    }
+};
+
+// Host side of DiscreteGen<int32_t, int32_t>'s constructor (Discrete.h:
+// 367-391) and AliasMethod::buildTable (Discrete.h:78-192), in double.
+struct HostTable {
+    std::vector<uint32_t> cdf;
+    std::vector<int32_t> prob, alias;
+    uint32_t bitmask = 0;
+};
+
+// Segment programs (capi.hip).
+int run_segments(erd_engines* e, const std::vector<Segment>& segs, const std::vector<int32_t>& engine_of,
+                 hipStream_t stream);
+int make_segment(const erd_request& r, Segment& s);
+
+// Multivariate helpers (mv_host.hip).
+int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t s, Scratch& sc);
+int factor_and_split(const std::vector<int32_t>& dims, double* d_a, const std::vector<int64_t>& off,
+                     uint16_t** lhi, uint16_t** llo, float** l32, std::vector<int64_t>& split_off,
+                     std::vector<int32_t>* info_out, hipStream_t s, Scratch& sc);
+int build_table(const double* w, int64_t K, HostTable& t);
+int upload_table(const HostTable& t, Segment& sg, Scratch& sc);
+
+}  // namespace erd

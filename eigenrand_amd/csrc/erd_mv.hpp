@@ -7,13 +7,14 @@ namespace erd {
 
 // One instance of a batched L*Z (+ mean) transform.
 struct MvInstance {
-    int32_t d;           // dimension
+    int32_t d;             // dimension
     int32_t pad;
-    int64_t n;           // columns of Z / X
-    int64_t split_off;   // offset of the padded dp x dp bf16 L planes
-    int64_t z_off;       // offset of Z (d x n, column-major floats)
-    int64_t x_off;       // offset of X (d x n, column-major floats)
-    const float* mean;   // d floats, or null
+    int64_t n;             // columns of Z / X
+    const uint16_t* lhi;   // padded dp x dp row-major bf16 planes of L (hi, lo)
+    const uint16_t* llo;
+    const float* z;        // d x n, column-major
+    float* x;              // d x n, column-major
+    const float* mean;     // d floats, or null
 };
 
 // A work item of the persistent transform: rows [row0, row0 + 16*MAXR) x
@@ -32,8 +33,7 @@ hipError_t launch_split_l(const double* l64, const int64_t* off, const int32_t* 
 // variant: 0 = (1 col block per wave, 256-row tiles), 1 = (2, 128-row), 2 = (4, 64-row),
 // 3 = (1, 64-row). Any dimension works with any variant (rows are tiled).
 hipError_t launch_lz_gemm(int variant, int grid, const MvTile* tiles, int32_t n_tiles, int32_t* counter,
-                          const MvInstance* inst, const uint16_t* lhi, const uint16_t* llo,
-                          const float* z, float* x, hipStream_t s);
+                          const MvInstance* inst, hipStream_t s);
 
 struct SyrkTile {
     int32_t inst;
@@ -41,10 +41,9 @@ struct SyrkTile {
     int32_t ta, tb;
 };
 hipError_t launch_wishart_assemble(const MvInstance* inst, const int64_t* nrm_off, const int64_t* chi_off,
-                                   const float* nrm, const float* chi, float* a, int n_inst,
-                                   hipStream_t s);
-hipError_t launch_syrk_tt(const SyrkTile* tiles, int n_tiles, const MvInstance* inst, const float* t,
-                          float* w, const int64_t* w_off, hipStream_t s);
+                                   const float* nrm, const float* chi, int n_inst, hipStream_t s);
+hipError_t launch_syrk_tt(const SyrkTile* tiles, int n_tiles, const MvInstance* inst, float* w,
+                          const int64_t* w_off, hipStream_t s);
 hipError_t launch_dirichlet_norm(const int32_t* dims, const int64_t* samples, const int64_t* g_off,
                                  const float* g, float* out, int n_inst, int64_t max_n, hipStream_t s);
 

@@ -109,11 +109,7 @@ constexpr int KPAD = 40;     // LDS row pitch in bf16 (80 B: keeps 16 B alignmen
 template <int NCB, int MAXR>
 __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tiles, int32_t n_tiles,
                                                  int32_t* __restrict__ counter,
-                                                 const MvInstance* __restrict__ inst,
-                                                 const uint16_t* __restrict__ lhi,
-                                                 const uint16_t* __restrict__ llo,
-                                                 const float* __restrict__ z,
-                                                 float* __restrict__ x) {
+                                                 const MvInstance* __restrict__ inst) {
     constexpr int NT = 64 * NCB;
     __shared__ __align__(16) uint16_t la[2][MAXR * 16 * KPAD];
     __shared__ __align__(16) uint16_t zb[2][NT * KPAD];
@@ -139,9 +135,9 @@ __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tile
         for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
             for (int rb = 0; rb < MAXR; ++rb) acc[cb][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
-        const uint16_t* Lh = lhi + in.split_off;
-        const uint16_t* Ll = llo + in.split_off;
-        const float* Z = z + in.z_off;
+        const uint16_t* Lh = in.lhi;
+        const uint16_t* Ll = in.llo;
+        const float* Z = in.z;
         for (int k0 = 0; k0 < kmax; k0 += KC) {
             // stage L[row0 + r, k0:k0+32] (row-major bf16, 16 B per thread-step)
             for (int q = tid; q < R * 16 * (KC / 8); q += 256) {
@@ -195,7 +191,7 @@ __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tile
             __syncthreads();
         }
         // epilogue: X[i][j] = acc + mean[i]; lane holds rows 4g..4g+3 of column l16
-        float* X = x + in.x_off;
+        float* X = in.x;
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) {
             const int64_t col = c0 + (wave * NCB + cb) * 16 + l16;
@@ -217,17 +213,12 @@ __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tile
 }
 
 hipError_t launch_lz_gemm(int variant, int grid, const MvTile* tiles, int32_t n_tiles, int32_t* counter,
-                          const MvInstance* inst, const uint16_t* lhi, const uint16_t* llo,
-                          const float* z, float* x, hipStream_t s) {
+                          const MvInstance* inst, hipStream_t s) {
     switch (variant) {
-        case 0: hipLaunchKernelGGL((k_lz_gemm<1, 16>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter,
-                                   inst, lhi, llo, z, x); break;
-        case 1: hipLaunchKernelGGL((k_lz_gemm<2, 8>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter,
-                                   inst, lhi, llo, z, x); break;
-        case 2: hipLaunchKernelGGL((k_lz_gemm<4, 4>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter,
-                                   inst, lhi, llo, z, x); break;
-        default: hipLaunchKernelGGL((k_lz_gemm<1, 4>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter,
-                                    inst, lhi, llo, z, x); break;
+        case 0: hipLaunchKernelGGL((k_lz_gemm<1, 16>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter, inst); break;
+        case 1: hipLaunchKernelGGL((k_lz_gemm<2, 8>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter, inst); break;
+        case 2: hipLaunchKernelGGL((k_lz_gemm<4, 4>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter, inst); break;
+        default: hipLaunchKernelGGL((k_lz_gemm<1, 4>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter, inst); break;
     }
     return hipGetLastError();
 }
@@ -260,14 +251,14 @@ namespace erd {
 // a: d x (d*n) column-major; nrm: n*d(d-1)/2 normals; chi: d rows of n.
 __global__ void k_wishart_assemble(const MvInstance* __restrict__ inst, const int64_t* __restrict__ nrm_off,
                                    const int64_t* __restrict__ chi_off, const float* __restrict__ nrm,
-                                   const float* __restrict__ chi, float* __restrict__ a) {
+                                   const float* __restrict__ chi) {
     const int i_inst = blockIdx.y;
     const MvInstance in = inst[i_inst];
     const int d = in.d;
     const int64_t n = in.n / d;  // columns of a = d * samples
     const int64_t total = (int64_t)d * d * n;
     const int64_t per_sample = (int64_t)d * (d - 1) / 2;
-    float* A = a + in.z_off;
+    float* A = const_cast<float*>(in.z);
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (int64_t)gridDim.x * blockDim.x) {
         const int64_t col = t / d;
@@ -292,14 +283,13 @@ __global__ void k_wishart_assemble(const MvInstance* __restrict__ inst, const in
 
 __global__ __launch_bounds__(256) void k_syrk_tt(const SyrkTile* __restrict__ tiles,
                                                  const MvInstance* __restrict__ inst,
-                                                 const float* __restrict__ tmat,
                                                  float* __restrict__ w, const int64_t* __restrict__ w_off) {
     __shared__ float ta_s[32][33];
     __shared__ float tb_s[32][33];
     const SyrkTile tl = tiles[blockIdx.x];
     const MvInstance in = inst[tl.inst];
     const int d = in.d;
-    const float* T = tmat + in.x_off + (int64_t)tl.sample * d * d;  // d x d column-major
+    const float* T = in.x + (int64_t)tl.sample * d * d;  // d x d column-major
     float* W = w + w_off[tl.inst] + (int64_t)tl.sample * d * d;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
     const int a0 = tl.ta * 32, b0 = tl.tb * 32;
@@ -350,18 +340,17 @@ __global__ void k_dirichlet_norm(const int32_t* __restrict__ dims, const int64_t
 }
 
 hipError_t launch_wishart_assemble(const MvInstance* inst, const int64_t* nrm_off, const int64_t* chi_off,
-                                   const float* nrm, const float* chi, float* a, int n_inst,
-                                   hipStream_t s) {
+                                   const float* nrm, const float* chi, int n_inst, hipStream_t s) {
     if (n_inst <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_wishart_assemble, dim3(256, (unsigned)n_inst), dim3(256), 0, s, inst, nrm_off,
-                       chi_off, nrm, chi, a);
+                       chi_off, nrm, chi);
     return hipGetLastError();
 }
 
-hipError_t launch_syrk_tt(const SyrkTile* tiles, int n_tiles, const MvInstance* inst, const float* t,
-                          float* w, const int64_t* w_off, hipStream_t s) {
+hipError_t launch_syrk_tt(const SyrkTile* tiles, int n_tiles, const MvInstance* inst, float* w,
+                          const int64_t* w_off, hipStream_t s) {
     if (n_tiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_syrk_tt, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles, inst, t, w, w_off);
+    hipLaunchKernelGGL(k_syrk_tt, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles, inst, w, w_off);
     return hipGetLastError();
 }
 

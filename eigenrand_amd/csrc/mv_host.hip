@@ -15,46 +15,14 @@
 #include "erd_kernels.hpp"
 #include "erd_mv.hpp"
 #include "erd_segment.hpp"
+#include "erd_internal.hpp"
 
 namespace erd {
-
-int run_segments(erd_engines* e, const std::vector<Segment>& segs, const std::vector<int32_t>& engine_of,
-                 hipStream_t stream);
-
-namespace {
 
 inline int hip_rc(hipError_t e) {
     if (e == hipSuccess) return ERD_OK;
     return e == hipErrorOutOfMemory ? ERD_ERR_OOM : ERD_ERR_HIP;
 }
-
-// Stream-ordered scratch released when the object goes out of scope.
-struct Scratch {
-    hipStream_t s;
-    std::vector<void*> ptrs;
-    explicit Scratch(hipStream_t st) : s(st) {}
-    ~Scratch() {
-        for (void* p : ptrs) (void)hipFreeAsync(p, s);
-    }
-    template <class T>
-    hipError_t alloc(T** p, size_t count) {
-        *p = nullptr;
-        if (count == 0) count = 1;
-        hipError_t e = hipMallocAsync((void**)p, sizeof(T) * count, s);
-        if (e == hipSuccess) ptrs.push_back(*p);
-        return e;
-    }
-    // Host vectors may die right after the call, so the copy is completed
-    // before returning.
-    template <class T>
-    hipError_t upload(T** p, const std::vector<T>& v) {
-        hipError_t e = alloc(p, v.size());
-        if (e == hipSuccess && !v.empty())
-            e = hipMemcpyAsync(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        return e;
-    }
-};
 
 // ------------------------------------------------------------ autotuner ---
 // Picks the L*Z tile variant per 64-wide dimension bucket by timing every
@@ -96,8 +64,7 @@ int grid_for(int variant, size_t n_tiles) {
 
 // X = L*Z (+mean) for every instance: group instances by tuned variant, one
 // persistent launch per group.
-int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, const uint16_t* lhi, const uint16_t* llo,
-           const float* z, float* x, hipStream_t s, Scratch& sc) {
+int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t s, Scratch& sc) {
     std::map<int, std::vector<int>> by_bucket;
     for (int i = 0; i < (int)inst.size(); ++i) by_bucket[bucket_of(inst[i].d)].push_back(i);
     std::map<int, std::vector<int>> by_variant;
@@ -123,7 +90,7 @@ int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, const uint16
                 hipMemsetAsync(d_cnt, 0, sizeof(int32_t), s);
                 hipEventRecord(a, s);
                 hipError_t e = launch_lz_gemm(cand, grid_for(cand, tiles.size()), d_tiles, (int32_t)tiles.size(),
-                                              d_cnt, d_inst, lhi, llo, z, x, s);
+                                              d_cnt, d_inst, s);
                 hipEventRecord(b, s);
                 hipEventSynchronize(b);
                 float ms = 0;
@@ -152,7 +119,7 @@ int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, const uint16
         if (sc.upload(&d_tiles, tiles) != hipSuccess || sc.alloc(&d_cnt, 1) != hipSuccess) return ERD_ERR_HIP;
         if (hipMemsetAsync(d_cnt, 0, sizeof(int32_t), s) != hipSuccess) return ERD_ERR_HIP;
         hipError_t e = launch_lz_gemm(kv.first, grid_for(kv.first, tiles.size()), d_tiles, (int32_t)tiles.size(),
-                                      d_cnt, d_inst, lhi, llo, z, x, s);
+                                      d_cnt, d_inst, s);
         if (e != hipSuccess) return ERD_ERR_HIP;
     }
     return ERD_OK;
@@ -195,7 +162,6 @@ int factor_and_split(const std::vector<int32_t>& dims, double* d_a, const std::v
     return ERD_OK;
 }
 
-}  // namespace
 }  // namespace erd
 
 using namespace erd;
@@ -259,10 +225,11 @@ int erd_mvnormal_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, cons
     if (rc) return rc;
     std::vector<MvInstance> inst(n_inst);
     for (int64_t i = 0; i < n_inst; ++i)
-        inst[i] = MvInstance{vd[i], 0, samples[i], split_off[i], zoff[i], zoff[i], mean ? mean + moff[i] : nullptr};
+        inst[i] = MvInstance{vd[i], 0, samples[i], lhi + split_off[i], llo + split_off[i], z + zoff[i],
+                             out + zoff[i], mean ? mean + moff[i] : nullptr};
     MvInstance* d_inst;
     if (sc.upload(&d_inst, inst) != hipSuccess) return ERD_ERR_HIP;
-    return run_lz(inst, d_inst, lhi, llo, z, out, s, sc);
+    return run_lz(inst, d_inst, s, sc);
 }
 
 int erd_wishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* df,
@@ -323,15 +290,16 @@ int erd_wishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const
     if (rc) return rc;
     std::vector<MvInstance> inst(n_inst);
     for (int64_t i = 0; i < n_inst; ++i)
-        inst[i] = MvInstance{vd[i], 0, (int64_t)vd[i] * samples[i], split_off[i], aoff[i], aoff[i], nullptr};
+        inst[i] = MvInstance{vd[i], 0, (int64_t)vd[i] * samples[i], lhi + split_off[i], llo + split_off[i],
+                             amat + aoff[i], tmat + aoff[i], nullptr};
     MvInstance* d_inst;
     int64_t *d_nrm_off, *d_chi_off, *d_aoff;
     if (sc.upload(&d_inst, inst) || sc.upload(&d_nrm_off, nrm_off) || sc.upload(&d_chi_off, chi_off) ||
         sc.upload(&d_aoff, aoff))
         return ERD_ERR_HIP;
-    if (launch_wishart_assemble(d_inst, d_nrm_off, d_chi_off, nrm, chi, amat, (int)n_inst, s) != hipSuccess)
+    if (launch_wishart_assemble(d_inst, d_nrm_off, d_chi_off, nrm, chi, (int)n_inst, s) != hipSuccess)
         return ERD_ERR_HIP;
-    rc = run_lz(inst, d_inst, lhi, llo, amat, tmat, s, sc);
+    rc = run_lz(inst, d_inst, s, sc);
     if (rc) return rc;
     std::vector<SyrkTile> tiles;
     for (int64_t i = 0; i < n_inst; ++i) {
@@ -342,7 +310,7 @@ int erd_wishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const
     }
     SyrkTile* d_tiles;
     if (sc.upload(&d_tiles, tiles) != hipSuccess) return ERD_ERR_HIP;
-    if (launch_syrk_tt(d_tiles, (int)tiles.size(), d_inst, tmat, out, d_aoff, s) != hipSuccess) return ERD_ERR_HIP;
+    if (launch_syrk_tt(d_tiles, (int)tiles.size(), d_inst, out, d_aoff, s) != hipSuccess) return ERD_ERR_HIP;
     return ERD_OK;
 }
 
@@ -394,15 +362,6 @@ int erd_dirichlet_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, con
 
 // ------------------------------------------------------------ discrete / multinomial ---
 namespace erd {
-namespace {
-
-// Host side of DiscreteGen<int32_t, int32_t>'s constructor (Discrete.h:
-// 367-391) and AliasMethod::buildTable (Discrete.h:78-192), in double.
-struct HostTable {
-    std::vector<uint32_t> cdf;
-    std::vector<int32_t> prob, alias;
-    uint32_t bitmask = 0;
-};
 
 int build_table(const double* w, int64_t K, HostTable& t) {
     if (K <= 0) return ERD_ERR_INVALID;
@@ -480,7 +439,6 @@ int upload_table(const HostTable& t, Segment& sg, Scratch& sc) {
     return ERD_OK;
 }
 
-}  // namespace
 }  // namespace erd
 
 extern "C" {

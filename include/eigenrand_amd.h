@@ -227,6 +227,61 @@ int erd_fill_discrete_i32(erd_engines* e, int32_t* out, int64_t per_engine, cons
 int erd_multinomial_batch(erd_engines* e, int64_t n_inst, const int64_t* dims, const int32_t* trials,
                           const int64_t* samples, const double* weights, int32_t* out, void* stream);
 
+/* ---------------------------------------------------------------- serving --- */
+/* A request loop with continuous batching. Each client (any 64-bit id) owns
+ * engines_per_client Vmt19937_64 engines resident on the device, engine g
+ * seeded splitmix64(client ^ seed_salt) + 4g. Requests queue as they arrive;
+ * a launcher thread packs the oldest admissible ones (client not busy, output
+ * within the HBM budget) into one launch on a free stream while earlier
+ * batches run; a completion thread stamps them done. A request spreads over
+ * e = clamp(count/65536, 1, engines_per_client) of its client's engines, each
+ * filling an equal, packet-aligned block (MvNormal/multinomial: blocks of
+ * samples); its result depends only on the client's request sequence, never
+ * on what else is in the batch. Outputs are allocated by the server and
+ * counted against hbm_budget_bytes until erd_server_release. No counterpart
+ * in the reference (which has no runtime). */
+#define ERD_SERVE_MVNORMAL 100    /* dim=d, count=samples, host_param=cov (d*d col-major), host_mean */
+#define ERD_SERVE_MULTINOMIAL 101 /* dim=K, count=samples, a=trials, host_param=weights (K)        */
+
+typedef struct erd_server erd_server;
+
+typedef struct erd_server_config {
+    int64_t max_clients;
+    int32_t engines_per_client;
+    int32_t max_inflight;        /* batches running at once (one stream each) */
+    int64_t hbm_budget_bytes;    /* admission cap on outputs not yet released */
+    int64_t max_batch_requests;
+    uint64_t seed_salt;
+} erd_server_config;
+
+typedef struct erd_serve_request {
+    uint64_t client;
+    int32_t kind;                /* erd_dist (scalar-parameter kinds) or ERD_SERVE_* */
+    int32_t dim;
+    int64_t count;               /* outputs, or samples for ERD_SERVE_*       */
+    double a, b;                 /* scalar parameters as in erd_request       */
+    const double* host_param;    /* copied at submit                          */
+    const float* host_mean;
+} erd_serve_request;
+
+typedef struct erd_server_stats {
+    int64_t completed, elements, batches;
+    double p50_ms, p99_ms, max_ms, span_s;
+    int64_t budget_in_use;
+} erd_server_stats_t;
+
+int erd_server_create(erd_server** out, const erd_server_config* cfg);
+int erd_server_destroy(erd_server* srv);
+int erd_server_submit(erd_server* srv, const erd_serve_request* req, uint64_t* ticket);
+/* Waits (timeout_s < 0: forever); returns the request's status and its
+ * device output (out_elems elements: e blocks of the padded per-engine size;
+ * for MvNormal/multinomial a dim x (padded samples) column-major matrix). */
+int erd_server_wait(erd_server* srv, uint64_t ticket, double timeout_s, void** out, int64_t* out_elems);
+int erd_server_release(erd_server* srv, uint64_t ticket);
+int erd_server_stats(erd_server* srv, erd_server_stats_t* stats);
+/* Non-owning view of a client's engines (after its first request ran). */
+int erd_server_client_engines(erd_server* srv, uint64_t client, erd_engines** view);
+
 /* -------------------------------------------------------- device memory --- */
 /* Small helpers so callers without another GPU runtime can use the library. */
 int erd_device_alloc(void** ptr, size_t bytes);
