@@ -8,8 +8,11 @@ of the fill over the whole 1 GiB output, which stays resident in HBM.
 
 Prints one JSON line (rank 0) with Gsamples/s, the HBM-write roofline of the
 fill kernel (algorithmic bytes = 4 B written per sample, timed with HIP events
-on the launch stream), and the CPU baseline (the oracle restatement of the
-reference algorithm, timed on this host's cores).
+on the launch stream), the CPU baseline (the oracle restatement of the
+reference algorithm, timed on this host's cores) and the serving leg: a mixed
+request stream (univariate / MvNormal / multinomial, 64 clients) through the
+batched loop and through per-request launches, saturated and under Poisson
+arrivals at 70% of each mode's saturated rate, with p50/p99 latency.
 """
 import argparse
 import json
@@ -120,6 +123,138 @@ def sweep():
     emit("VoP poisson i32", nv, timed(lambda: er.poisson_vop(vp, pm, out=vi), reps=3), 8)
 
 
+def serving_workload(n_req, n_clients, seed=5):
+    """Mixed request stream of the serving configuration: 60% univariate
+    (size logU 1e3..1e8; normal/gamma/binomial/poisson), 30% MvNormal (d
+    U{8..512} fixed per client, cov = A A^T + d I with A ~ N(0, 1/d), samples
+    logU 1..1e5), 10% multinomial (K logU 10..1e4, trials logU 1..1e6,
+    samples logU 1..1e3)."""
+    import numpy as np
+    from eigenrand_amd.serving import MVNORMAL, MULTINOMIAL
+    rng = np.random.default_rng(seed)
+    lu = lambda lo, hi: float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+    covs = {}
+    for c in range(n_clients):
+        d = int(rng.integers(8, 513))
+        a = rng.normal(0, 1 / np.sqrt(d), (d, d))
+        covs[c] = a @ a.T + d * np.eye(d)
+    reqs = []
+    for _ in range(n_req):
+        c = int(rng.integers(n_clients))
+        u = rng.uniform()
+        if u < 0.6:
+            kind = ["std_normal", "gamma", "binomial", "poisson"][int(rng.integers(4))]
+            r = dict(client=c, kind=kind, count=int(lu(1e3, 1e8)))
+            if kind == "gamma":
+                r.update(a=lu(0.1, 50), b=1.0)
+            elif kind == "binomial":
+                r.update(a=int(rng.integers(1, 1001)), b=float(rng.uniform()))
+            elif kind == "poisson":
+                r.update(a=lu(0.01, 1e4))
+            r["elems"] = r["count"]
+        elif u < 0.9:
+            d = covs[c].shape[0]
+            r = dict(client=c, kind=MVNORMAL, count=int(lu(1, 1e5)), cov=covs[c],
+                     mean=np.zeros(d, np.float32))
+            r["elems"] = d * r["count"]
+        else:
+            K = int(lu(10, 1e4))
+            r = dict(client=c, kind=MULTINOMIAL, count=int(lu(1, 1e3)), a=int(lu(1, 1e6)),
+                     weights=rng.uniform(0, 1, K))
+            r["elems"] = K * r["count"]
+        reqs.append(r)
+    return reqs, covs
+
+
+def serving(n_req=300, n_clients=64, engines_per_client=256):
+    """Serving configuration on one GPU: the batched loop (continuous batching
+    over 8 streams) against per-request launches (one request per launch, one
+    at a time) on the same server code, engines and outputs. Returns a dict."""
+    import queue
+    import threading
+    import numpy as np
+    from eigenrand_amd.serving import Server, MVNORMAL
+
+    reqs, covs = serving_workload(n_req, n_clients)
+    total_elems = sum(r["elems"] for r in reqs)
+    modes = {
+        "batched": dict(max_inflight=8, max_batch_requests=256, max_batch_elements=1 << 26),
+        "per_request": dict(max_inflight=1, max_batch_requests=1),
+    }
+
+    def submit(srv, r):
+        return srv.submit(**{k: v for k, v in r.items() if k != "elems"})
+
+    def warm(srv):
+        # seeds every client's engines and caches its covariance factor
+        ts = [submit(srv, dict(client=c, kind=MVNORMAL, count=8, cov=covs[c])) for c in range(n_clients)]
+        for t in ts:
+            srv.wait(t)
+            srv.release(t)
+        srv.reset_stats()
+
+    def closed(srv):
+        t0 = time.perf_counter()
+        ts = [submit(srv, r) for r in reqs]
+        for t in ts:
+            srv.wait(t)
+            srv.release(t)
+        dt = time.perf_counter() - t0
+        st = srv.stats()
+        srv.reset_stats()
+        return dt, st
+
+    def open_loop(srv, rate, seed=17):
+        rng = np.random.default_rng(seed)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+        q = queue.Queue()
+
+        def drain():
+            for _ in range(len(reqs)):
+                t = q.get()
+                srv.wait(t)
+                srv.release(t)
+
+        th = threading.Thread(target=drain)
+        th.start()
+        t0 = time.perf_counter()
+        for r, at in zip(reqs, arrivals):
+            lag = t0 + at - time.perf_counter()
+            if lag > 0:
+                time.sleep(lag)
+            q.put(submit(srv, r))
+        th.join()
+        st = srv.stats()
+        srv.reset_stats()
+        return st
+
+    res = {"requests": n_req, "clients": n_clients, "engines_per_client": engines_per_client,
+           "elements": int(total_elems), "hbm_budget_bytes": 64 << 30}
+    for name, cfg in modes.items():
+        with Server(max_clients=n_clients, engines_per_client=engines_per_client,
+                    hbm_budget_bytes=64 << 30, seed_salt=0x5EED, **cfg) as srv:
+            warm(srv)
+            dt, st = closed(srv)
+            sat_rate = n_req / dt
+            load = 0.7 * sat_rate
+            ol = open_loop(srv, load)
+            res[name] = {
+                "saturated_gsamples_per_s": round(total_elems / dt / 1e9, 3),
+                "saturated_requests_per_s": round(sat_rate, 1),
+                "saturated_batches": st["batches"],
+                "open_loop_rate_per_s": round(load, 1),
+                "open_loop_gsamples_per_s": round(ol["elements"] / ol["span_s"] / 1e9, 3),
+                "p50_ms": round(ol["p50_ms"], 3),
+                "p99_ms": round(ol["p99_ms"], 3),
+                "max_ms": round(ol["max_ms"], 3),
+                "batches": ol["batches"],
+            }
+        print(json.dumps({"serving_mode": name, **res[name]}), file=sys.stderr, flush=True)
+    b, p = res["batched"], res["per_request"]
+    res["throughput_ratio"] = round(b["saturated_gsamples_per_s"] / p["saturated_gsamples_per_s"], 2)
+    return res
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "normal_f32_2p28_traffic.json")
     if not os.path.exists(p):
@@ -135,6 +270,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--sweep", action="store_true", help="secondary per-distribution measurements")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving (p50/p99) leg")
+    ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -216,6 +353,11 @@ def main():
         },
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }
+    if not args.no_serve:
+        srv = serving(n_req=args.serve_requests)
+        line["p50_ms"] = srv["batched"]["p50_ms"]
+        line["p99_ms"] = srv["batched"]["p99_ms"]
+        line["serving"] = srv
     print(json.dumps(line), flush=True)
 
 

@@ -75,6 +75,14 @@ _SIGS = {
     "erd_alias_table": [P, I64, P, P, P],
     "erd_fill_discrete_i32": [P, P, I64, P, I64, P],
     "erd_multinomial_batch": [P, I64, P, P, P, P, P, P],
+    "erd_server_create": [C.POINTER(P), P],
+    "erd_server_destroy": [P],
+    "erd_server_submit": [P, P, C.POINTER(U64)],
+    "erd_server_wait": [P, U64, F64, C.POINTER(P), C.POINTER(I64)],
+    "erd_server_release": [P, U64],
+    "erd_server_stats": [P, P],
+    "erd_server_reset_stats": [P],
+    "erd_server_client_engines": [P, U64, C.POINTER(P)],
     "erd_device_alloc": [C.POINTER(P), C.c_size_t],
     "erd_device_free": [P],
     "erd_memcpy_d2h": [P, P, C.c_size_t, P],

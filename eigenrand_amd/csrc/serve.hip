@@ -80,6 +80,7 @@ struct Req {
     int state = 0;              // 0 queued, 1 running, 2 done, 3 failed
     int status = ERD_OK;
     int slot = -1;
+    bool seed_first = false;    // first request of its client: seeds the engines
 };
 
 // Device factorisation cached per (client, covariance).
@@ -162,15 +163,15 @@ int Server::prepare_and_launch(Batch* b) {
     const int E = cfg.engines_per_client;
     // Seed the engines of clients seen for the first time:
     // engine g of client c = Vmt19937_64(splitmix64(c ^ salt) + 4g).
+    // (The launcher marks at most one request per new client as seed_first.)
     for (Req* q : b->reqs) {
-        if (slot_seeded[q->slot]) continue;
+        if (!q->seed_first) continue;
         std::vector<uint64_t> seeds(E);
-        const uint64_t base = splitmix64(slot_client[q->slot] ^ cfg.seed_salt);
+        const uint64_t base = splitmix64(q->r.client ^ cfg.seed_salt);
         for (int g = 0; g < E; ++g) seeds[g] = base + 4ull * (uint64_t)g;
         uint64_t* d_seeds;
         if (sc.upload(&d_seeds, seeds) != hipSuccess) return ERD_ERR_HIP;
         if (launch_seed(pool->states + (int64_t)q->slot * E, d_seeds, E, s) != hipSuccess) return ERD_ERR_HIP;
-        slot_seeded[q->slot] = 1;
     }
     std::vector<Segment> segs;
     std::vector<int32_t> seg_engine;
@@ -182,7 +183,7 @@ int Server::prepare_and_launch(Batch* b) {
         if (r.kind == ERD_SERVE_MVNORMAL) {
             const int d = r.dim;
             const uint64_t key = hash_params(q->param);
-            auto fk = std::make_pair(slot_client[q->slot], key);
+            auto fk = std::make_pair(r.client, key);
             auto it = factors.find(fk);
             if (it == factors.end()) {
                 double* a;
@@ -333,9 +334,13 @@ void Server::launcher_loop() {
                     // pass stays blocked so that its order is kept.
                     std::vector<char> blocked(slot_client.size() + 1, 0);
                     std::vector<Req*> take;
+                    int64_t take_elems = 0;
                     for (auto it = pending.begin(); it != pending.end();) {
                         Req* q = *it;
                         if ((int64_t)take.size() >= cfg.max_batch_requests) break;
+                        if (!take.empty() && cfg.max_batch_elements > 0 &&
+                            take_elems + q->out_elems > cfg.max_batch_elements)
+                            break;
                         const int sl = q->slot;
                         if (blocked[sl] || slot_busy[sl] ||
                             budget_used + (int64_t)q->out_bytes > cfg.hbm_budget_bytes) {
@@ -344,7 +349,12 @@ void Server::launcher_loop() {
                             continue;
                         }
                         budget_used += (int64_t)q->out_bytes;
+                        take_elems += q->out_elems;
                         q->state = 1;
+                        if (!slot_seeded[sl]) {
+                            slot_seeded[sl] = 1;
+                            q->seed_first = true;
+                        }
                         take.push_back(q);
                         it = pending.erase(it);
                     }
@@ -476,13 +486,13 @@ int erd_server_submit(erd_server* srv, const erd_serve_request* r, uint64_t* tic
             sum += v;
         }
         if (!std::isfinite(sum)) return ERD_ERR_NONFINITE;
+        if (!(sum > 0)) return ERD_ERR_INVALID;
         q->e_req = (int)std::min<int64_t>(E, std::max<int64_t>(1, (int64_t)r->dim * r->count / erd::kSplit));
         q->per = erd::round_up((r->count + q->e_req - 1) / q->e_req, 8);
         q->out_elems = (int64_t)r->dim * q->per * q->e_req;
         q->out_bytes = sizeof(int32_t) * (size_t)q->out_elems;
     } else {
-        if (r->kind < 0 || r->kind > ERD_DIST_POISSON_I32 || r->kind == ERD_DIST_NORMAL_VOP_F32 ||
-            r->kind >= ERD_DIST_BINOMIAL_VOP_I32)
+        if (r->kind < 0 || r->kind > ERD_DIST_BINOMIAL_I32 || r->kind == ERD_DIST_NORMAL_VOP_F32)
             return ERD_ERR_INVALID;  // per-element parameter kinds are not served
         erd_request probe{};
         probe.dist = r->kind;
@@ -546,7 +556,8 @@ int erd_server_release(erd_server* srv, uint64_t ticket) {
         S.reqs.erase(it);
     }
     S.cv.notify_all();
-    return p ? (hipFree(p) == hipSuccess ? ERD_OK : ERD_ERR_HIP) : ERD_OK;
+    // The request is complete, so a stream-ordered free needs no ordering.
+    return p ? (hipFreeAsync(p, nullptr) == hipSuccess ? ERD_OK : ERD_ERR_HIP) : ERD_OK;
 }
 
 int erd_server_stats(erd_server* srv, erd_server_stats_t* st) {
@@ -571,12 +582,28 @@ int erd_server_stats(erd_server* srv, erd_server_stats_t* st) {
     return ERD_OK;
 }
 
+int erd_server_reset_stats(erd_server* srv) {
+    if (!srv) return ERD_ERR_INVALID;
+    Server& S = srv->s;
+    std::lock_guard<std::mutex> lk(S.mu);
+    S.latencies.clear();
+    S.done_requests = S.done_elems = S.batches = 0;
+    S.t_first_submit = S.t_last_done = 0;
+    return ERD_OK;
+}
+
 int erd_server_destroy(erd_server* srv) {
     if (!srv) return ERD_OK;
     Server& S = srv->s;
     {
+        // Requests never launched fail with ERD_ERR_INVALID, so waiters return.
         std::lock_guard<std::mutex> lk(S.mu);
         S.stop = true;
+        for (Req* q : S.pending) {
+            q->state = 3;
+            q->status = ERD_ERR_INVALID;
+        }
+        S.pending.clear();
     }
     S.cv.notify_all();
     S.launcher.join();
@@ -591,7 +618,8 @@ int erd_server_destroy(erd_server* srv) {
     }
     S.completer.join();
     for (auto& kv : S.reqs)
-        if (kv.second->out) hipFree(kv.second->out);
+        if (kv.second->out) hipFreeAsync(kv.second->out, nullptr);
+    hipDeviceSynchronize();
     for (auto& kv : S.factors) {
         hipFree(kv.second.hi);
         hipFree(kv.second.lo);

@@ -1,0 +1,156 @@
+"""Python binding of the serving runtime (erd_server_* in include/eigenrand_amd.h).
+
+A `Server` owns per-client Vmt19937_64 engines resident on the device and a
+launcher/completer pair of native threads that batch queued requests into
+segment-program launches as streams and HBM budget free up. Engine g of client
+c is seeded `splitmix64(c ^ seed_salt) + 4 g` (its four MT19937-64 lanes take
+that value .. +3), so a client's results depend only on its own request
+sequence.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+MVNORMAL = 100
+MULTINOMIAL = 101
+SPLIT = 1 << 16  # outputs per engine before a request spreads over more engines
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("max_clients", C.c_int64), ("engines_per_client", C.c_int32),
+                ("max_inflight", C.c_int32), ("hbm_budget_bytes", C.c_int64),
+                ("max_batch_requests", C.c_int64), ("seed_salt", C.c_uint64),
+                ("max_batch_elements", C.c_int64)]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("client", C.c_uint64), ("kind", C.c_int32), ("dim", C.c_int32),
+                ("count", C.c_int64), ("a", C.c_double), ("b", C.c_double),
+                ("host_param", C.c_void_p), ("host_mean", C.c_void_p)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("elements", C.c_int64), ("batches", C.c_int64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
+                ("span_s", C.c_double), ("budget_in_use", C.c_int64)]
+
+
+def splitmix64(x):
+    m = (1 << 64) - 1
+    x = (x + 0x9E3779B97F4A7C15) & m
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & m
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & m
+    return x ^ (x >> 31)
+
+
+def client_seeds(client, engines, seed_salt=0):
+    """Per-engine seeds of a client's engines."""
+    base = splitmix64((int(client) ^ int(seed_salt)) & ((1 << 64) - 1))
+    return [(base + 4 * g) & ((1 << 64) - 1) for g in range(engines)]
+
+
+def spread(count, engines_per_client, width, dim=1):
+    """(engines, per-engine outputs or columns) a request is spread over."""
+    e = int(min(engines_per_client, max(1, (dim * count) // SPLIT)))
+    per = -(-count // e)
+    per = -(-per // width) * width
+    return e, per
+
+
+_ELEM = {0: (np.uint64, 4), 2: (np.float64, 4), 10: (np.int32, 4), 11: (np.int32, 4)}
+
+
+def kind_layout(kind):
+    """(numpy dtype, packet width) of a scalar-parameter request kind."""
+    if kind in _ELEM:
+        return _ELEM[kind]
+    if kind == MULTINOMIAL:
+        return np.int32, 8
+    return np.float32, 8
+
+
+class Server:
+    def __init__(self, max_clients=64, engines_per_client=256, max_inflight=8,
+                 hbm_budget_bytes=64 << 30, max_batch_requests=256, seed_salt=0,
+                 max_batch_elements=0):
+        self.cfg = ServerConfig(max_clients, engines_per_client, max_inflight, hbm_budget_bytes,
+                                max_batch_requests, seed_salt & ((1 << 64) - 1), max_batch_elements)
+        h = C.c_void_p()
+        _lib.call("erd_server_create", C.byref(h), C.byref(self.cfg))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and _lib._lib is not None:
+            _lib._lib.erd_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, client, kind, count, a=0.0, b=0.0, cov=None, mean=None, weights=None, dim=0):
+        """Queue one request; returns its ticket. `kind` is a DIST name or
+        number (scalar-parameter kinds), MVNORMAL (cov d x d, optional mean)
+        or MULTINOMIAL (weights K, trials a)."""
+        if isinstance(kind, str):
+            kind = _lib.DIST[kind]
+        req = ServeRequest(int(client) & ((1 << 64) - 1), int(kind), int(dim), int(count),
+                           float(a), float(b), None, None)
+        keep = []
+        if kind == MVNORMAL:
+            c = np.asfortranarray(np.asarray(cov, dtype=np.float64))
+            req.dim = c.shape[0]
+            keep.append(c)
+            req.host_param = c.ctypes.data
+            if mean is not None:
+                m = np.ascontiguousarray(np.asarray(mean, dtype=np.float32))
+                keep.append(m)
+                req.host_mean = m.ctypes.data
+        elif kind == MULTINOMIAL:
+            w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64))
+            req.dim = w.size
+            keep.append(w)
+            req.host_param = w.ctypes.data
+        t = C.c_uint64()
+        _lib.call("erd_server_submit", self._h, C.byref(req), C.byref(t))
+        return t.value
+
+    def wait(self, ticket, timeout_s=-1.0):
+        """Blocks until the request is done; returns (device pointer, elements)."""
+        p, n = C.c_void_p(), C.c_int64()
+        _lib.call("erd_server_wait", self._h, C.c_uint64(ticket), float(timeout_s), C.byref(p),
+                  C.byref(n))
+        return p.value, n.value
+
+    def fetch(self, ticket, dtype):
+        """Waits and copies the request's output to a host numpy array."""
+        p, n = self.wait(ticket)
+        out = np.empty(n, dtype=dtype)
+        _lib.call("erd_memcpy_d2h", out.ctypes.data_as(C.c_void_p), C.c_void_p(p), out.nbytes, None)
+        return out
+
+    def release(self, ticket):
+        _lib.call("erd_server_release", self._h, C.c_uint64(ticket))
+
+    def stats(self):
+        st = ServerStats()
+        _lib.call("erd_server_stats", self._h, C.byref(st))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        _lib.call("erd_server_reset_stats", self._h)
+
+    def client_engines(self, client):
+        """The client's engines as a non-owning Vmt19937_64 view."""
+        from . import Vmt19937_64
+        h = C.c_void_p()
+        _lib.call("erd_server_client_engines", self._h, C.c_uint64(int(client) & ((1 << 64) - 1)),
+                  C.byref(h))
+        return Vmt19937_64(_handle=h, _parent=self)

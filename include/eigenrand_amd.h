@@ -252,6 +252,7 @@ typedef struct erd_server_config {
     int64_t hbm_budget_bytes;    /* admission cap on outputs not yet released */
     int64_t max_batch_requests;
     uint64_t seed_salt;
+    int64_t max_batch_elements;  /* cap on one batch's outputs (0: none); a batch takes >= 1 request */
 } erd_server_config;
 
 typedef struct erd_serve_request {
@@ -279,6 +280,8 @@ int erd_server_submit(erd_server* srv, const erd_serve_request* req, uint64_t* t
 int erd_server_wait(erd_server* srv, uint64_t ticket, double timeout_s, void** out, int64_t* out_elems);
 int erd_server_release(erd_server* srv, uint64_t ticket);
 int erd_server_stats(erd_server* srv, erd_server_stats_t* stats);
+/* Clears the latency record and counters (e.g. after a warm-up pass). */
+int erd_server_reset_stats(erd_server* srv);
 /* Non-owning view of a client's engines (after its first request ran). */
 int erd_server_client_engines(erd_server* srv, uint64_t client, erd_engines** view);
 
