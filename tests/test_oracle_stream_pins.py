@@ -1,0 +1,247 @@
+"""Pins of the oracle's samplers against the raw engine stream.
+
+Each test draws the engine's raw u64 stream (itself pinned to std::mt19937_64)
+and recomputes the samples independently of the oracle's C++ code. The
+recomputation is written from the reference's packetOp text in Python: which
+draws feed which output slot, the accept tests, and the CompressMask queue.
+
+- Fixed-consumption samplers (normal, exponential, Cauchy, Student-t) are
+  checked in float64 within a tolerance.
+- Integer and rejection samplers are checked exactly, with float32 numpy
+  arithmetic for the control flow. For the transcendental bits they use the
+  oracle's elementwise plog/pexp/psincos/plgamma, which test_oracle_math pins
+  to libm.
+
+A dropped term, a swapped draw or a wrong queue order fails here.
+"""
+import math
+
+import numpy as np
+import pytest
+
+f32 = np.float32
+EPS8 = f32(2.0 ** -26)  # nonzero_uniform_real: U + FLT_EPSILON / 8
+
+
+def full_words(orc, e, draws):
+    """(n_engines, draws, 8) u32 words: float j of a draw = word j."""
+    raw = orc.fill_u64(e.copy(), draws)
+    return raw.view(np.uint32).reshape(len(e), draws, 8)
+
+
+def half_words(orc, e, halves):
+    """(n_engines, halves, 4) u32 words of the half() sequence."""
+    out = []
+    for g in range(len(e)):
+        raw = orc.fill_half_u64(e.subset(g, 1).copy(), halves)
+        out.append(raw.view(np.uint32).reshape(halves, 4))
+    return np.stack(out)
+
+
+def ur(w):
+    return ((w & 0x7FFFFF).astype(np.float64) / 2.0 ** 23).astype(f32)
+
+
+def test_normal_is_box_muller_of_the_stream(orc):
+    e = orc.Engines(n=3, seed0=77)
+    per = 1024
+    got = orc.fill_normal_f32(e.copy(), per).reshape(3, per // 16, 2, 8)
+    w = full_words(orc, e, per // 8).reshape(3, per // 16, 2, 8)
+    u1 = 1.0 - ur(w[:, :, 0, :]).astype(np.float64)   # call 2m -> 1 - U
+    u2 = ur(w[:, :, 1, :]).astype(np.float64)         # call 2m+1 -> theta
+    r = np.sqrt(-2 * np.log(u1))
+    want_sin = r * np.sin(2 * math.pi * u2)
+    want_cos = r * np.cos(2 * math.pi * u2)
+    tol = 4e-6 * (1 + np.abs(want_sin))
+    assert np.all(np.abs(got[:, :, 0, :] - want_sin) <= tol)
+    assert np.all(np.abs(got[:, :, 1, :] - want_cos) <= 4e-6 * (1 + np.abs(want_cos)))
+
+
+def test_exponential_cauchy_studentt_of_the_stream(orc):
+    e = orc.Engines(n=2, seed0=5)
+    per = 512
+    w = full_words(orc, e, 2 * per // 8)
+    u = ur(w[:, :per // 8, :]).astype(np.float64).reshape(2, per)
+    ex = orc.fill_exponential_f32(e.copy(), per, 2.0).reshape(2, per)
+    np.testing.assert_allclose(ex, -np.log(1 - u) / 2.0, rtol=3e-6, atol=1e-7)
+    ca = orc.fill_cauchy_f32(e.copy(), per, 1.0, 3.0).reshape(2, per)
+    # tan is ill-conditioned near the poles: take the float32 argument
+    x = (f32(math.pi) * (ur(w[:, :per // 8, :]).reshape(2, per) - f32(0.5))).astype(np.float64)
+    want = 1.0 + 3.0 * np.tan(x)
+    np.testing.assert_allclose(ca, want, rtol=2e-5, atol=2e-5)
+    # Student-t: calls 2k and 2k+1 per packet, u1 = 1 - U
+    n = 5.0
+    st = orc.fill_studentt_f32(e.copy(), per, n).reshape(2, per // 8, 8)
+    ww = w.reshape(2, per // 8, 2, 8)
+    a = 1 - ur(ww[:, :, 0, :]).astype(np.float64)
+    b = ur(ww[:, :, 1, :]).astype(np.float64)
+    want = np.sqrt(n * (np.exp(np.log(a) * (-2 / n)) - 1)) * np.sin(2 * math.pi * b)
+    np.testing.assert_allclose(st, want, rtol=3e-5, atol=3e-5)
+
+
+@pytest.mark.parametrize("alpha", [1.0, 2.0, 7.0])
+def test_gamma_integer_alpha_of_the_stream(orc, alpha):
+    e = orc.Engines(n=2, seed0=9)
+    per, beta = 256, f32(1.5)
+    k = int(alpha)
+    got = orc.fill_gamma_f32(e.copy(), per, alpha, float(beta)).reshape(2, per // 8, 8)
+    w = full_words(orc, e, k * per // 8).reshape(2, per // 8, k, 8)
+    if k == 1:  # beta * ExponentialGen: -(plog(1 - U) / 1)
+        want = beta * -(orc.plog(f32(1) - ur(w[:, :, 0, :])) / f32(1))
+    else:       # -(beta * plog(prod (U + eps/8))), multiplied in draw order
+        y = ur(w[:, :, 0, :]) + EPS8
+        for c in range(1, k):
+            y = y * (ur(w[:, :, c, :]) + EPS8)
+        want = -(beta * orc.plog(y))
+    np.testing.assert_array_equal(got, want.astype(f32))
+
+
+def _queue(W):
+    """CompressMask<W> queue (arch/AVX/PacketFilter.h:156-177): a fully
+    accepted packet bypasses; otherwise accepted lanes append in lane order
+    and W of them are emitted once available."""
+    rest = []
+
+    def push(cands, acc):
+        nonlocal rest
+        if all(acc):
+            return list(cands)
+        rest = rest + [c for c, a in zip(cands, acc) if a]
+        if len(rest) >= W:
+            out, rest = rest[:W], rest[W:]
+            return out
+        return None
+    return push
+
+
+@pytest.mark.parametrize("alpha", [0.3, 0.8])
+def test_gamma_gs_rejection_of_the_stream(orc, alpha):
+    e = orc.Engines(n=2, seed0=31)
+    per, beta = 64, f32(2.0)
+    got = orc.fill_gamma_f32(e.copy(), per, alpha, float(beta)).reshape(2, per)
+    a = f32(alpha)
+    px = f32(math.e / (float(a) + math.e))
+    w = full_words(orc, e, 600)
+    for g in range(2):
+        push, out, d = _queue(8), [], 0
+        while len(out) < per:
+            ux, vx, u3 = ur(w[g, d]), ur(w[g, d + 1]) + EPS8, ur(w[g, d + 2])
+            d += 3
+            lv = orc.plog(vx)
+            x1 = orc.pexp((f32(1) / a) * lv)
+            q1 = orc.pexp(-x1)
+            x2 = f32(1) - orc.plog(vx)
+            q2 = orc.pexp(orc.plog(x2) * (a - f32(1)))
+            c = ux < px
+            xx, qx = np.where(c, x1, x2), np.where(c, q1, q2)
+            r = push(list(beta * xx), list(u3 < qx))
+            if r is not None:
+                out += r
+        np.testing.assert_array_equal(got[g], np.array(out[:per], dtype=f32))
+
+
+def test_gamma_tangent_rejection_of_the_stream(orc):
+    e = orc.Engines(n=2, seed0=44)
+    per, alpha, beta = 64, f32(3.7), f32(1.0)
+    got = orc.fill_gamma_f32(e.copy(), per, float(alpha), float(beta)).reshape(2, per)
+    sq = np.sqrt(f32(2) * alpha - f32(1))
+    a1 = alpha - f32(1)
+    w = full_words(orc, e, 400)
+    for g in range(2):
+        push, out, d = _queue(8), [], 0
+        while len(out) < per:
+            ys, yc = orc.psincos(f32(math.pi) * ur(w[g, d]))
+            yx = ys / yc
+            xx = sq * yx + a1
+            ux = ur(w[g, d + 1])
+            d += 2
+            ub = (yx * yx + f32(1)) * orc.pexp(a1 * orc.plog(xx / a1) - yx * sq)
+            acc = (f32(0) < xx) & (ux <= ub)
+            r = push(list(beta * xx), list(acc))
+            if r is not None:
+                out += r
+        np.testing.assert_array_equal(got[g], np.array(out[:per], dtype=f32))
+
+
+@pytest.mark.parametrize("mean", [0.05, 3.0, 11.5])
+def test_poisson_knuth_of_the_stream(orc, mean):
+    e = orc.Engines(n=2, seed0=101)
+    per = 128
+    got = orc.fill_poisson_i32(e.copy(), per, mean).reshape(2, per // 4, 4)
+    h = half_words(orc, e, 4000)
+    thr = f32(math.exp(-mean))
+    for g in range(2):
+        s = 0
+        for k in range(per // 4):
+            val = np.ones(4, f32)
+            res = np.zeros(4, np.int32)
+            while True:
+                val = val * ur(h[g, s])
+                s += 1
+                c = thr < val
+                if not c.any():
+                    break
+                res += c
+            np.testing.assert_array_equal(got[g, k], res)
+
+
+def test_poisson_tangent_rejection_of_the_stream(orc):
+    e = orc.Engines(n=2, seed0=7)
+    per, mean = 64, 16.0
+    got = orc.fill_poisson_i32(e.copy(), per, mean).reshape(2, per)
+    psq, pm = f32(math.sqrt(2 * mean)), f32(mean)
+    plm, pg1 = f32(math.log(mean)), f32(mean * math.log(mean) - math.lgamma(mean + 1))
+    h = half_words(orc, e, 400)
+    for g in range(2):
+        push, out, s = _queue(4), [], 0
+        while len(out) < per:
+            ps, pc = orc.psincos(f32(math.pi) * ur(h[g, s]))
+            yx = ps / pc
+            fres = np.trunc(psq * yx + pm).astype(f32)
+            p1 = (yx * yx + f32(1)) * f32(0.9)
+            p2 = orc.pexp(fres * plm - orc.plgamma(fres + f32(1)) - pg1)
+            acc = (f32(0) <= fres) & (ur(h[g, s + 1]) <= p1 * p2)
+            s += 2
+            r = push(list(fres), list(acc))
+            if r is not None:
+                out += r
+        np.testing.assert_array_equal(got[g], np.array(out[:per], dtype=f32).astype(np.int32))
+
+
+@pytest.mark.parametrize("n,p", [(7, 0.3), (24, 0.81)])
+def test_binomial_small_n_of_the_stream(orc, n, p):
+    e = orc.Engines(n=2, seed0=3)
+    per = 64
+    got = orc.fill_binomial_i32(e.copy(), per, n, p).reshape(2, per // 4, 4)
+    h = half_words(orc, e, n * per // 4).reshape(2, per // 4, n, 4)
+    # res = n + sum_i (p <= U_i ? -1 : 0), one half draw per trial
+    want = n - (f32(p) <= ur(h)).sum(axis=2)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_binomial_tangent_rejection_of_the_stream(orc):
+    e = orc.Engines(n=2, seed0=12)
+    per, n, p = 64, 100, 0.75
+    got = orc.fill_binomial_i32(e.copy(), per, n, p).reshape(2, per)
+    sp = min(p, 1 - p)
+    mean = n * sp
+    ptr, pm = f32(n), f32(mean)
+    psv = f32(math.sqrt(2 * mean * (1 - sp)))
+    plp, plq, pg1 = f32(math.log(sp)), f32(math.log(1 - sp)), f32(math.lgamma(n + 1))
+    h = half_words(orc, e, 600)
+    for g in range(2):
+        push, out, s = _queue(4), [], 0
+        while len(out) < per:
+            ps, pc = orc.psincos(f32(math.pi) * ur(h[g, s]))
+            ys = ps / pc
+            fres = np.trunc(psv * ys + pm).astype(f32)
+            p1 = (f32(1.2) * psv) * (f32(1) + ys * ys)
+            p2 = orc.pexp(((pg1 - orc.plgamma(fres + f32(1))) - orc.plgamma((ptr + f32(1)) - fres))
+                          + fres * plp + (ptr - fres) * plq)
+            acc = (f32(0) <= fres) & (fres <= ptr) & (ur(h[g, s + 1]) <= p1 * p2)
+            s += 2
+            r = push(list(fres), list(acc))
+            if r is not None:
+                out += r
+        res = np.array(out[:per], dtype=f32).astype(np.int32)
+        np.testing.assert_array_equal(got[g], n - res)  # p != small_p: n - k
