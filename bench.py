@@ -100,6 +100,12 @@ def sweep():
         ("cauchy(0,1) f32", lambda: er.cauchy(PER_ENGINE, 1, urng, 0.0, 1.0, out=out)),
         ("poisson(16) i32", lambda: er.poisson(PER_ENGINE, 1, urng, 16.0, out=outi)),
         ("binomial(100,0.75) i32", lambda: er.binomial(PER_ENGINE, 1, urng, 100, 0.75, out=outi)),
+        # the other regimes of the integer samplers
+        ("poisson(0.05) i32 [Knuth]", lambda: er.poisson(PER_ENGINE, 1, urng, 0.05, out=outi)),
+        ("poisson(3) i32 [Knuth]", lambda: er.poisson(PER_ENGINE, 1, urng, 3.0, out=outi)),
+        ("poisson(10) i32 [Knuth]", lambda: er.poisson(PER_ENGINE, 1, urng, 10.0, out=outi)),
+        ("binomial(10,0.3) i32 [n<25]", lambda: er.binomial(PER_ENGINE, 1, urng, 10, 0.3, out=outi)),
+        ("binomial(1000,0.0005) i32 [Knuth]", lambda: er.binomial(PER_ENGINE, 1, urng, 1000, 0.0005, out=outi)),
     ]
     for name, fn in univ:
         emit(name, N_SAMPLES, timed(fn, reps=3), 4)
@@ -166,7 +172,7 @@ def serving_workload(n_req, n_clients, seed=5):
     return reqs, covs
 
 
-def serving(n_req=300, n_clients=64, engines_per_client=256):
+def serving(n_req=300, n_clients=64, engines_per_client=1024):
     """Serving configuration on one GPU: the batched loop (continuous batching
     over 8 streams) against per-request launches (one request per launch, one
     at a time) on the same server code, engines and outputs. Returns a dict."""
@@ -186,8 +192,11 @@ def serving(n_req=300, n_clients=64, engines_per_client=256):
         return srv.submit(**{k: v for k, v in r.items() if k != "elems"})
 
     def warm(srv):
-        # seeds every client's engines and caches its covariance factor
+        # seeds every client's engines, caches its covariance factor, and runs
+        # the workload once so that the device memory pool holds the output
+        # sizes (steady state)
         ts = [submit(srv, dict(client=c, kind=MVNORMAL, count=8, cov=covs[c])) for c in range(n_clients)]
+        ts += [submit(srv, r) for r in reqs]
         for t in ts:
             srv.wait(t)
             srv.release(t)

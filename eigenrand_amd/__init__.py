@@ -24,7 +24,8 @@ from . import _lib
 from ._lib import ErdError, InvalidArgument
 
 __all__ = [
-    "Vmt19937_64", "ErdError", "InvalidArgument", "STATE_BYTES",
+    "Vmt19937_64", "P8_mt19937_64", "ErdError", "InvalidArgument", "STATE_BYTES",
+    "p8_randbits_u64", "p8_randbits_u32", "p8_uniform_real",
     "randbits_u64", "uniform_real", "uniform_real_f64", "normal", "normal_vop",
     "exponential", "cauchy", "student_t", "gamma", "chi_squared", "poisson", "binomial",
     "binomial_vop", "gamma_vop", "poisson_vop", "run_requests", "cholesky_batch",
@@ -121,6 +122,54 @@ class Vmt19937_64:
         v = C.c_uint32()
         _lib.call("erd_engines_status", self._h, C.byref(v), _stream_ptr(stream))
         return int(v.value)
+
+
+class P8_mt19937_64(Vmt19937_64):
+    """`n` P8_mt19937_64 adaptors (PacketRandomEngine.h:429-517, 600-609).
+
+    Adaptor k = the Vmt19937_64 pair (seed_k, seed_k + 4), so its eight
+    64-bit lanes are std::mt19937_64(seed_k .. seed_k + 7) on any ISA. The
+    handle holds 2n Vmt engines (len() counts those); `adaptors` is n.
+    """
+
+    def __init__(self, seed=Vmt19937_64.default_seed, n=1, stride=8, seeds=None, stream=None):
+        if seeds is None:
+            seeds = [int(seed) + k * int(stride) for k in range(int(n))]
+        arr = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
+        h = C.c_void_p()
+        _lib.call("erd_engines_create_p8", C.byref(h), arr.ctypes.data_as(C.c_void_p), int(arr.size),
+                  _stream_ptr(stream))
+        super().__init__(_handle=h)
+
+    @property
+    def adaptors(self):
+        return len(self) // 2
+
+
+def _p8_fill(name, count, urng, dtype, out, stream):
+    torch = _torch()
+    n = len(urng) // 2
+    if out is None:
+        out = torch.empty(n * int(count), dtype=dtype, device="cuda")
+    elif out.dtype != dtype or not out.is_cuda or not out.is_contiguous() or out.numel() != n * count:
+        raise InvalidArgument("output tensor", 1)
+    _lib.call(name, urng.handle, _ptr(out), int(count), _stream_ptr(stream))
+    return out.view(n, int(count))
+
+
+def p8_randbits_u64(count, urng, out=None, stream=None):
+    """`count` calls of P8_mt19937_64::operator() per adaptor; (n, count) uint64."""
+    return _p8_fill("erd_p8_fill_u64", count, urng, _torch().uint64, out, stream)
+
+
+def p8_randbits_u32(count, urng, out=None, stream=None):
+    """`count` calls of P8_mt19937_64_32::operator() per adaptor; (n, count) uint32."""
+    return _p8_fill("erd_p8_fill_u32", count, urng, _torch().uint32, out, stream)
+
+
+def p8_uniform_real(count, urng, out=None, stream=None):
+    """`count` calls of the adaptor's uniform_real() per adaptor; (n, count) float32."""
+    return _p8_fill("erd_p8_fill_uniform_f32", count, urng, _torch().float32, out, stream)
 
 
 def _alloc(urng, per_engine, dtype, out):

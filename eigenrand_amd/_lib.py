@@ -52,6 +52,10 @@ _SIGS = {
     "erd_engines_discard": [P, I64, P],
     "erd_engines_status": [P, C.POINTER(C.c_uint32), P],
     "erd_fill_u64": [P, P, I64, P],
+    "erd_engines_create_p8": [C.POINTER(P), P, I64, P],
+    "erd_p8_fill_u64": [P, P, I64, P],
+    "erd_p8_fill_u32": [P, P, I64, P],
+    "erd_p8_fill_uniform_f32": [P, P, I64, P],
     "erd_fill_uniform_f32": [P, P, I64, F32, F32, P],
     "erd_fill_uniform_f64": [P, P, I64, F64, F64, P],
     "erd_fill_std_normal_f32": [P, P, I64, P],

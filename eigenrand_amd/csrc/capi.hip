@@ -168,6 +168,35 @@ int erd_fill_u64(erd_engines* e, uint64_t* out, int64_t per, void* stream) {
     return hip_status(erd::launch_fill_u64(e->states, e->count, out, per, S(stream)));
 }
 
+int erd_engines_create_p8(erd_engines** out, const uint64_t* seeds, int64_t n_p8, void* stream) {
+    if (!out || !seeds || n_p8 <= 0) return ERD_ERR_INVALID;
+    std::vector<uint64_t> vs((size_t)(2 * n_p8));
+    for (int64_t k = 0; k < n_p8; ++k) {
+        vs[2 * k] = seeds[k];           // rngs[0]{seed}
+        vs[2 * k + 1] = seeds[k] + 4u;  // rngs[1]{seed + 1 * u64_stride}, u64_stride = 4
+    }
+    return erd_engines_create(out, vs.data(), 2 * n_p8, stream);
+}
+
+static int p8_fill(erd_engines* e, void* out, int64_t per, int64_t width, int mode, void* stream) {
+    int rc = check_fill(e, out, per, width);
+    if (rc) return rc;
+    if (e->count % 2 != 0) return ERD_ERR_INVALID;  // adaptors are engine pairs
+    return hip_status(erd::launch_fill_p8(e->states, e->count, out, per / 2, mode, S(stream)));
+}
+
+int erd_p8_fill_u64(erd_engines* e, uint64_t* out, int64_t per, void* stream) {
+    return p8_fill(e, out, per, 8, 0, stream);
+}
+
+int erd_p8_fill_u32(erd_engines* e, uint32_t* out, int64_t per, void* stream) {
+    return p8_fill(e, out, per, 16, 1, stream);
+}
+
+int erd_p8_fill_uniform_f32(erd_engines* e, float* out, int64_t per, void* stream) {
+    return p8_fill(e, out, per, 16, 2, stream);
+}
+
 int erd_fill_uniform_f32(erd_engines* e, float* out, int64_t per, float a, float b, void* stream) {
     int rc = check_fill(e, out, per, 8);
     if (rc) return rc;

@@ -51,6 +51,39 @@ struct UniformF64 {  // arch/AVX/RandUtils.h:212-225: 4 doubles per call
     }
 };
 
+// P8_mt19937_64 / P8_mt19937_64_32 (ParallelRandomEngineAdaptor<UInt,
+// Vmt19937_64, 8>, PacketRandomEngine.h:429-517): one adaptor = the Vmt pair
+// (seed, seed+4); a refill writes draw k of Vmt i to buffer words
+// [8i, 8i+8) in u64 (MODE 0), at [16k + 8i, 16k + 8i + 8) as 32-bit words
+// low half first (MODE 1), or as bit_to_ur_float of those words
+// (uniform_real(), MODE 2). Engine g of the set is Vmt (g & 1) of adaptor g/2.
+template <int MODE>
+struct P8Fill {
+    static constexpr int CALLS = 1;
+    void* out;
+    int64_t per_v;  // outputs per Vmt engine (half of the adaptor's block)
+    int half;
+    __device__ void rebase(int64_t off) {
+        const int64_t g = off / per_v;
+        half = (int)(g & 1);
+        const int64_t base = (g >> 1) * 2 * per_v;
+        if (MODE == 0) out = (uint64_t*)out + base;
+        else out = (uint32_t*)out + base;
+    }
+    __device__ static int64_t units(int64_t n) { return MODE == 0 ? n / 4 : n / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        if (MODE == 0) {
+            ((uint64_t*)out)[8 * u + 4 * half + l] = w[0];
+        } else if (MODE == 1) {
+            *reinterpret_cast<uint2*>((uint32_t*)out + 16 * u + 8 * half + 2 * l) =
+                make_uint2(lo32(w[0]), hi32(w[0]));
+        } else {
+            *reinterpret_cast<float2*>((float*)out + 16 * u + 8 * half + 2 * l) =
+                make_float2(ur_f32(lo32(w[0])), ur_f32(hi32(w[0])));
+        }
+    }
+};
+
 // Box-Muller pair of StdNormalGen::packetOp (NormalExp.h:60-82): calls A, B
 // give u1 = 1 - U(A), u2 = U(B); the first packet is r*sin, the cached second
 // packet r*cos. A fill of 8 mod 16 outputs ends with an unused cos packet.

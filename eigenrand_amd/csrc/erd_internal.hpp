@@ -54,6 +54,13 @@ int make_segment(const erd_request& r, Segment& s);
 
 // Multivariate helpers (mv_host.hip).
 int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t s, Scratch& sc);
+// One persistent L*Z launch: variant, grid and its tile queue.
+struct LzLaunch {
+    int variant = 0;
+    int grid = 0;
+    std::vector<MvTile> tiles;
+};
+int plan_lz(const std::vector<MvInstance>& inst, std::vector<LzLaunch>& plan);
 int factor_and_split(const std::vector<int32_t>& dims, double* d_a, const std::vector<int64_t>& off,
                      uint16_t** lhi, uint16_t** llo, float** l32, std::vector<int64_t>& split_off,
                      std::vector<int32_t>* info_out, hipStream_t s, Scratch& sc);

@@ -26,6 +26,10 @@ hipError_t launch_discard(EngineState* st, int64_t n, int64_t calls, hipStream_t
 
 // fill_fixed.hip
 hipError_t launch_fill_u64(EngineState* st, int64_t n, uint64_t* out, int64_t per, hipStream_t s);
+// P8 adaptor fills: n Vmt engines (n/2 adaptors), per_v outputs per Vmt
+// engine; mode 0 u64, 1 u32, 2 uniform_real() floats.
+hipError_t launch_fill_p8(EngineState* st, int64_t n, void* out, int64_t per_v, int mode,
+                          hipStream_t s);
 hipError_t launch_fill_uniform_f32(EngineState* st, int64_t n, float* out, int64_t per, float a,
                                    float b, int affine, hipStream_t s);
 hipError_t launch_fill_uniform_f64(EngineState* st, int64_t n, double* out, int64_t per, double a,

@@ -358,15 +358,97 @@ __device__ __forceinline__ int32_t knuth_packet(HalfCursor& cur, float thr, int 
     return res;
 }
 
+// op_knuth's wide path: every thread evaluates the packet that would start at
+// one of 64 consecutive slots, for up to 32 iterations, so a window stages the
+// uniforms of 95 slots in LDS (16-byte aligned, one float4 per slot).
+constexpr int kKnuthStarts = 64;
+constexpr int kKnuthIters = 32;
+constexpr int kKnuthSlots = kKnuthStarts + kKnuthIters - 1;
+constexpr int kKnuthStage = kKnuthSlots * 4;
+
 // PoissonGen, mean < 12; BinomialGen's Poisson regime with the final flip.
-__device__ void op_knuth(Eng& e, uint64_t* carry, int32_t* o, int64_t per, float thr,
+//
+// A packet starting at slot P consumes 1 + max_j(stop_j) half draws, where
+// stop_j is the first iteration whose running product of lane j is <= thr;
+// the product never increases, so lane j's count is stop_j.
+//
+// Wide path: the wave stages the uniforms of slots P..P+94 in LDS (or up to
+// the end of the block), and thread t multiplies the 4 lanes' uniforms from
+// slot P+t on, in the loop's order, for up to 32 iterations inside the
+// window. That is the packet that would start at P+t. The
+// packets actually produced form a chain: start 0, then each next start is
+// the previous start plus its consumption. A scalar walk marks the chain
+// while starts lie in the window and each packet completes within its 32
+// iterations, and the marked threads store their packets at their rank in
+// the chain. A packet needing more iterations falls back to the
+// one-slot-at-a-time loop. Results and consumption are those of the
+// sequential loop.
+__device__ void op_knuth(Eng& e, uint64_t* carry, float* ustage, int32_t* o, int64_t per, float thr,
                          int32_t trials, int flip, uint32_t* status, int lane) {
     const int64_t T = per / 4;
     if (T == 0) return;
     HalfCursor cur = cursor_begin(e, carry, lane);
-    for (int64_t pk = 0; pk < T; ++pk) {
-        const int32_t r = knuth_packet(cur, thr, lane, status);
-        if (lane < 4) o[4 * pk + lane] = flip ? trials - r : r;
+    int64_t pk = 0;
+    while (pk < T) {
+        bool slow = true;
+        if (cur.nc == 0 && cur.avail() > 0) {
+            // The window is the next S block slots (fewer at the end of the
+            // block: packets that would need slots past it are finished by
+            // the one-slot loop, which twists at the same point the
+            // reference does). u64 word w holds floats 2w, 2w+1 (slot w/2).
+            const int S = min(cur.avail(), kKnuthSlots);
+            for (int w = lane; w < 2 * S; w += 64) {
+                const uint64_t x = temper(cur.mt[2 * cur.pos + w]);
+                *reinterpret_cast<float2*>(ustage + 2 * w) = make_float2(ur_f32(lo32(x)), ur_f32(hi32(x)));
+            }
+            __syncthreads();
+            // Iterations thread `lane` may take without leaving the window.
+            const int iters = max(0, min(kKnuthIters, S - lane));
+            float v0 = 1.0f, v1 = 1.0f, v2 = 1.0f, v3 = 1.0f;
+            int s0 = kKnuthIters, s1 = kKnuthIters, s2 = kKnuthIters, s3 = kKnuthIters;
+            for (int s = 0; s < iters; ++s) {
+                const float4 u = *reinterpret_cast<const float4*>(ustage + 4 * (lane + s));
+                v0 = v0 * u.x;
+                v1 = v1 * u.y;
+                v2 = v2 * u.z;
+                v3 = v3 * u.w;
+                if (s0 == kKnuthIters && !(thr < v0)) s0 = s;
+                if (s1 == kKnuthIters && !(thr < v1)) s1 = s;
+                if (s2 == kKnuthIters && !(thr < v2)) s2 = s;
+                if (s3 == kKnuthIters && !(thr < v3)) s3 = s;
+                if (s0 < kKnuthIters && s1 < kKnuthIters && s2 < kKnuthIters && s3 < kKnuthIters) break;
+            }
+            const int m = max(max(s0, s1), max(s2, s3));
+            const int cons = m < iters ? m + 1 : 0;  // 0: runs past the window
+            uint64_t chain = 0;
+            int at = 0;
+            int64_t n = 0;
+            const int64_t room = T - pk;
+            slow = false;
+            while (at < min(kKnuthStarts, S) && n < room) {
+                const int c = __builtin_amdgcn_readlane(cons, at);
+                if (c == 0) {
+                    slow = true;
+                    break;
+                }
+                chain |= 1ull << at;
+                ++n;
+                at += c;
+            }
+            if ((chain >> lane) & 1ull) {
+                const int64_t q = pk + __popcll(chain & lanemask_lt(lane));
+                *reinterpret_cast<int4*>(o + 4 * q) =
+                    flip ? make_int4(trials - s0, trials - s1, trials - s2, trials - s3) : make_int4(s0, s1, s2, s3);
+            }
+            pk += n;
+            __syncthreads();  // ustage is rewritten by the next window
+            cur.advance(at);
+        }
+        if (slow && pk < T) {
+            const int32_t r = knuth_packet(cur, thr, lane, status);
+            if (lane < 4) o[4 * pk + lane] = flip ? trials - r : r;
+            ++pk;
+        }
     }
     cursor_end(e, cur);
 }
@@ -839,7 +921,7 @@ __device__ void op_multinomial_small(Eng& e, int32_t* out, int64_t n, int32_t T,
 // result t goes to out[i*stride] while r[i] -= t. Full draws; thread 8i+j
 // runs iteration i of lane j (8 iterations per step).
 __device__ void op_binomial_row8(Eng& e, uint64_t* carry, float* res_lds, int32_t* out, int64_t stride,
-                                 int32_t* r, int64_t n, float p, uint32_t* status, int lane) {
+                                 int32_t* r, int32_t t0, int64_t n, float p, uint32_t* status, int lane) {
     const int64_t T = n / 8;
     if (T == 0) return;
     const int i = lane >> 3, j = lane & 7;
@@ -855,7 +937,7 @@ __device__ void op_binomial_row8(Eng& e, uint64_t* carry, float* res_lds, int32_
     const float plp = plog(psmall);
     const float plq = plog(1.0f - psmall);
     for (int64_t pk = 0; pk < T; ++pk) {
-        const int32_t nt = r[8 * pk + j];
+        const int32_t nt = t0 >= 0 ? t0 : r[8 * pk + j];  // t0: first row, every sample has t0 trials
         const float ptr = (float)nt;
         const float pmean = ptr * psmall;
         const float pg1 = plgamma_approx(ptr + 1.0f);

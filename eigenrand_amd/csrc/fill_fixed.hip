@@ -46,6 +46,15 @@ hipError_t launch_fill_u64(EngineState* st, int64_t n, uint64_t* out, int64_t pe
     return launch_fixed(st, n, per, d, s);
 }
 
+hipError_t launch_fill_p8(EngineState* st, int64_t n, void* out, int64_t per_v, int mode,
+                          hipStream_t s) {
+    switch (mode) {
+        case 0: return launch_fixed(st, n, per_v, P8Fill<0>{out, per_v, 0}, s);
+        case 1: return launch_fixed(st, n, per_v, P8Fill<1>{out, per_v, 0}, s);
+        default: return launch_fixed(st, n, per_v, P8Fill<2>{out, per_v, 0}, s);
+    }
+}
+
 hipError_t launch_fill_uniform_f32(EngineState* st, int64_t n, float* out, int64_t per, float a,
                                    float b, int affine, hipStream_t s) {
     UniformF32 d{out, a, b - a, affine};

@@ -30,12 +30,13 @@ __global__ __launch_bounds__(64) void k_poisson_knuth(EngineState* __restrict__ 
                                                       uint32_t* __restrict__ status) {
     __shared__ uint64_t mt[kWords];
     __shared__ uint64_t carry[2];
+    __shared__ __align__(16) float ustage[kKnuthStage];
     const int lane = threadIdx.x;
     const int64_t g = blockIdx.x;
     if (per / 4 == 0) return;
     Eng e;
     eng_load(e, mt, states + g, lane);
-    op_knuth(e, carry, out + g * per, per, thr, trials, flip, status, lane);
+    op_knuth(e, carry, ustage, out + g * per, per, thr, trials, flip, status, lane);
     eng_store(e, states + g, lane);
 }
 

@@ -57,9 +57,41 @@ void build_tiles(const std::vector<MvInstance>& inst, const std::vector<int>& wh
 
 int grid_for(int variant, size_t n_tiles) {
     int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
     const int per_cu = variant == 0 ? 3 : 4;
     return (int)std::min<size_t>(n_tiles, (size_t)cus * per_cu);
+}
+
+// Tile lists of run_lz without launching, for callers that stage their own
+// uploads. Returns ERD_ERR_UNSUPPORTED while a dimension bucket is still to
+// be tuned (run_lz then tunes it).
+int plan_lz(const std::vector<MvInstance>& inst, std::vector<LzLaunch>& plan) {
+    std::map<int, std::vector<int>> by_variant;
+    for (int i = 0; i < (int)inst.size(); ++i) {
+        const int bkt = bucket_of(inst[i].d);
+        int v = -1;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tuned.find(bkt);
+            if (it != g_tuned.end()) v = it->second;
+        }
+        if (v < 0) {
+            if (autotune_enabled()) return ERD_ERR_UNSUPPORTED;
+            v = default_variant(bkt == 0 ? 64 : bkt == 1 ? 128 : 256);
+        }
+        by_variant[v].push_back(i);
+    }
+    plan.clear();
+    for (auto& kv : by_variant) {
+        LzLaunch l;
+        l.variant = kv.first;
+        build_tiles(inst, kv.second, kv.first, l.tiles);
+        l.grid = grid_for(kv.first, l.tiles.size());
+        plan.push_back(std::move(l));
+    }
+    return ERD_OK;
 }
 
 // X = L*Z (+mean) for every instance: group instances by tuned variant, one
@@ -84,19 +116,20 @@ int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t 
                 MvTile* d_tiles;
                 int32_t* d_cnt;
                 if (sc.upload(&d_tiles, tiles) != hipSuccess || sc.alloc(&d_cnt, 1) != hipSuccess) return ERD_ERR_HIP;
-                hipEvent_t a, b;
-                hipEventCreate(&a);
-                hipEventCreate(&b);
-                hipMemsetAsync(d_cnt, 0, sizeof(int32_t), s);
-                hipEventRecord(a, s);
-                hipError_t e = launch_lz_gemm(cand, grid_for(cand, tiles.size()), d_tiles, (int32_t)tiles.size(),
-                                              d_cnt, d_inst, s);
-                hipEventRecord(b, s);
-                hipEventSynchronize(b);
+                hipEvent_t a = nullptr, b = nullptr;
                 float ms = 0;
-                hipEventElapsedTime(&ms, a, b);
-                hipEventDestroy(a);
-                hipEventDestroy(b);
+                hipError_t e = hipEventCreate(&a);
+                if (e == hipSuccess) e = hipEventCreate(&b);
+                if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, sizeof(int32_t), s);
+                if (e == hipSuccess) e = hipEventRecord(a, s);
+                if (e == hipSuccess)
+                    e = launch_lz_gemm(cand, grid_for(cand, tiles.size()), d_tiles, (int32_t)tiles.size(), d_cnt,
+                                       d_inst, s);
+                if (e == hipSuccess) e = hipEventRecord(b, s);
+                if (e == hipSuccess) e = hipEventSynchronize(b);
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+                if (a) (void)hipEventDestroy(a);
+                if (b) (void)hipEventDestroy(b);
                 if (e != hipSuccess) return ERD_ERR_HIP;
                 if (ms < best) {
                     best = ms;
@@ -519,8 +552,8 @@ int erd_multinomial_batch(erd_engines* e, int64_t n_inst, const int64_t* dims, c
             double sum = 0;
             for (double v : probs) sum += v;
             for (double& v : probs) v /= sum;
-            int32_t* r;
-            if (sc.upload(&r, std::vector<int32_t>((size_t)n, T)) != hipSuccess) return ERD_ERR_HIP;
+            int32_t* r;  // remaining trials per sample, written by the first row
+            if (sc.alloc(&r, (size_t)n) != hipSuccess) return ERD_ERR_OOM;
             double rest_p = 1 - probs[0];
             for (int64_t row = 0; row + 1 < K; ++row) {
                 Segment sg{};
@@ -528,6 +561,7 @@ int erd_multinomial_batch(erd_engines* e, int64_t n_inst, const int64_t* dims, c
                 sg.count = n;
                 sg.out = o + row;
                 sg.in0 = r;
+                sg.i0 = row == 0 ? T : -1;
                 sg.d[0] = row == 0 ? (float)probs[0] : (float)(probs[row] / rest_p);
                 sg.d[1] = (double)K;
                 if (row > 0) rest_p -= probs[row];
@@ -539,6 +573,7 @@ int erd_multinomial_batch(erd_engines* e, int64_t n_inst, const int64_t* dims, c
             last.count = n;
             last.out = o + (K - 1);
             last.in0 = r;
+            last.i0 = K == 1 ? T : -1;  // a single category takes every trial
             last.d[1] = (double)K;
             segs.push_back(last);
             eng.push_back((int32_t)i);

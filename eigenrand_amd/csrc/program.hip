@@ -23,6 +23,10 @@ union ProgramLds {
         uint64_t carry[2 * kLanes];
         float res[8];
     } vop;
+    struct {
+        uint64_t carry[2 * kLanes];
+        alignas(16) float u[kKnuthStage];
+    } knuth;
 };
 
 template <int A>
@@ -137,7 +141,7 @@ __device__ void run_segment(Eng& e, ProgramLds& s, const Segment& sg, uint32_t* 
         case SEG_POISSON_I32: {
             // d: mean, ne_mean, sqrt_tmean, log_mean, g1
             if (sg.d[0] < 12) {
-                op_knuth(e, s.vop.carry, (int32_t*)sg.out, n, (float)sg.d[1], 0, 0, status, lane);
+                op_knuth(e, s.knuth.carry, s.knuth.u, (int32_t*)sg.out, n, (float)sg.d[1], 0, 0, status, lane);
             } else {
                 PoissonTangent pol{(float)sg.d[2], (float)sg.d[0], (float)sg.d[3], (float)sg.d[4]};
                 op_reject(e, s.pt, (int32_t*)sg.out, n, pol, status, lane);
@@ -151,7 +155,7 @@ __device__ void run_segment(Eng& e, ProgramLds& s, const Segment& sg, uint32_t* 
             if (trials < 25) {
                 op_binomial_small(e, s.vop.carry, (int32_t*)sg.out, n, trials, (float)sg.d[0], lane);
             } else if (sg.d[2] < 1.0) {
-                op_knuth(e, s.vop.carry, (int32_t*)sg.out, n, (float)sg.d[7], trials, flip, status, lane);
+                op_knuth(e, s.knuth.carry, s.knuth.u, (int32_t*)sg.out, n, (float)sg.d[7], trials, flip, status, lane);
             } else {
                 BinomialTangent pol{(float)trials, (float)sg.d[4], (float)sg.d[2], (float)sg.d[5],
                                     (float)sg.d[6], (float)sg.d[3], trials, flip};
@@ -191,13 +195,13 @@ __device__ void run_segment(Eng& e, ProgramLds& s, const Segment& sg, uint32_t* 
             break;
         }
         case SEG_MULTINOMIAL_ROW:
-            op_binomial_row8(e, s.vop.carry, s.vop.res, (int32_t*)sg.out, (int64_t)sg.d[1], (int32_t*)sg.in0, n,
-                             (float)sg.d[0], status, lane);
+            op_binomial_row8(e, s.vop.carry, s.vop.res, (int32_t*)sg.out, (int64_t)sg.d[1], (int32_t*)sg.in0, sg.i0,
+                             n, (float)sg.d[0], status, lane);
             break;
         case SEG_MULTINOMIAL_LAST: {
             const int64_t stride = (int64_t)sg.d[1];
             const int32_t* rr = (const int32_t*)sg.in0;
-            for (int64_t i = lane; i < n; i += 64) ((int32_t*)sg.out)[i * stride] = rr[i];
+            for (int64_t i = lane; i < n; i += 64) ((int32_t*)sg.out)[i * stride] = sg.i0 >= 0 ? sg.i0 : rr[i];
             break;
         }
         default:

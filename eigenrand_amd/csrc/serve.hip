@@ -9,10 +9,19 @@
 // earlier batches are still running on other streams. A completion thread
 // waits on each batch's event, stamps its requests done and frees the slot.
 //
-// A request's result depends only on its client's engine state and on the
-// request itself: requests of one client run in submission order on that
-// client's engines and never share them with another client, so a result is
-// the same whether the request runs alone or in a mix.
+// Engines: a request is spread over e consecutive engines of its client
+// (about 2^16 normal-equivalents of work per engine, so heavier samplers
+// spread wider), taken from a per-client cursor that rotates through the
+// client's E engines in submission order. A request may start once no batch
+// in flight holds any of its engines; requests on disjoint engines of one
+// client therefore run concurrently, and requests sharing an engine run in
+// submission order (in one batch, in segment order; across batches, the later
+// one waits for the earlier batch to finish).
+//
+// A request's result depends only on its client's request sequence: which
+// engines it uses and their states are fixed by the earlier requests of the
+// same client, and clients never share engines. A result is the same whether
+// the request runs alone or in a mix.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -36,7 +45,7 @@
 namespace erd {
 namespace {
 
-constexpr int64_t kSplit = 1 << 16;  // outputs per engine before a request spreads further
+constexpr double kEngineWork = 65536.0;  // normal-equivalent samples per engine
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -66,12 +75,47 @@ size_t elem_bytes(int kind) {
     }
 }
 
+// Outputs (samples for MvNormal/multinomial) one engine takes for a request,
+// from the relative cost of each sampler (measured full-chip rates: normal =
+// 1, rejection gamma ~5, tangent Poisson ~7, binomial ~12, Knuth ~ mean + 2).
+int64_t chunk_for(const erd_serve_request& r) {
+    double cost = 1.0;
+    switch (r.kind) {
+        case ERD_DIST_STUDENT_T_F32: cost = 2.0; break;
+        case ERD_DIST_GAMMA_F32: {
+            const double a = r.a;
+            if (a == 1.0) cost = 1.0;
+            else if (a == std::floor(a) && a < 20) cost = a / 2;
+            else cost = 5.0;
+            break;
+        }
+        case ERD_DIST_POISSON_I32: cost = r.a < 12 ? 1.0 + r.a / 2 : 8.0; break;
+        case ERD_DIST_BINOMIAL_I32: {
+            const double n = r.a, m = n * std::min(r.b, 1 - r.b);
+            cost = n < 25 ? 1.0 + n / 8 : (m < 1 ? 1.5 : 12.0);
+            break;
+        }
+        case ERD_SERVE_MVNORMAL: cost = (double)r.dim; break;
+        case ERD_SERVE_MULTINOMIAL: {
+            const double K = r.dim, T = r.a;
+            cost = T < std::max(30.0 * (K - 1), 100.0) ? 1.0 + T / 8 + K / 8 : 12.0 * K;
+            break;
+        }
+        default: break;
+    }
+    return std::max<int64_t>(1, (int64_t)(kEngineWork / std::max(cost, 1e-3)));
+}
+
+struct Req;
+void spread(const erd_serve_request& r, int E, int width, Req* q);
+
 struct Req {
     uint64_t ticket = 0;
     erd_serve_request r{};
     std::vector<double> param;  // MvNormal cov / multinomial weights
     std::vector<float> mean;
     int e_req = 1;              // engines the request is spread over
+    int eng0 = 0;               // first of them (client-local, wraps modulo E)
     int64_t per = 0;            // outputs (or columns) per engine
     int64_t out_elems = 0;
     size_t out_bytes = 0;
@@ -97,7 +141,16 @@ struct Batch {
     std::vector<Req*> reqs;
     std::unique_ptr<Scratch> scratch;
     int status = ERD_OK;
+    uint32_t serial = 0;
 };
+
+// e = ceil(count / chunk) engines (at most E), each taking an equal,
+// packet-aligned share.
+void spread(const erd_serve_request& r, int E, int width, Req* q) {
+    const int64_t chunk = chunk_for(r);
+    q->e_req = (int)std::min<int64_t>(E, (r.count + chunk - 1) / chunk);
+    q->per = round_up((r.count + q->e_req - 1) / q->e_req, width);
+}
 
 }  // namespace
 
@@ -110,9 +163,14 @@ struct Server {
     std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
     std::unordered_map<uint64_t, int> slot_of;  // client -> engine slot
     std::vector<uint64_t> slot_client;
-    std::vector<char> slot_busy, slot_seeded;
+    std::vector<char> slot_seeded;
+    std::vector<int> slot_cursor;                // next engine of the client
+    std::vector<uint32_t> owner;                 // pool engine -> batch holding it (0: none)
+    uint32_t next_serial = 1;
     std::vector<hipStream_t> streams;
     std::vector<char> stream_free;
+    std::vector<void*> pin;        // per stream: pinned staging buffer
+    std::vector<size_t> pin_cap;
     std::deque<Batch*> inflight;
     std::map<std::pair<uint64_t, uint64_t>, Factor> factors;
     int64_t budget_used = 0;
@@ -130,12 +188,28 @@ struct Server {
         if ((int64_t)slot_client.size() >= cfg.max_clients) return -1;
         const int s = (int)slot_client.size();
         slot_client.push_back(client);
-        slot_busy.push_back(0);
         slot_seeded.push_back(0);
+        slot_cursor.push_back(0);
         slot_of[client] = s;
         return s;
     }
 
+    // Pool index of engine g of request q.
+    int32_t engine_of(const Req* q, int g) const {
+        const int E = cfg.engines_per_client;
+        return q->slot * E + (q->eng0 + g) % E;
+    }
+    // Engines a request holds while its batch runs: all of the client's when
+    // it seeds them, else its own e_req.
+    template <class F>
+    void for_held(const Req* q, F f) const {
+        const int E = cfg.engines_per_client;
+        if (q->seed_first) {
+            for (int g = 0; g < E; ++g) f(q->slot * E + g);
+        } else {
+            for (int g = 0; g < q->e_req; ++g) f(engine_of(q, g));
+        }
+    }
     int prepare_and_launch(Batch* b);
     void launcher_loop();
     void completer_loop();
@@ -155,23 +229,55 @@ uint64_t hash_params(const std::vector<double>& v) {
 
 }  // namespace
 
-// Builds and launches one batch (called without the server lock).
+// Host data of one batch (seeds, discrete tables, means, the segment program
+// and the L*Z tile queues), packed into one pinned buffer and uploaded with a
+// single copy. Pointers into it are patched once the device block exists.
+struct Stage {
+    std::vector<char> host;
+    // 256-byte aligned space for `bytes`, zero-filled.
+    size_t reserve(size_t bytes) {
+        const size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + bytes);
+        return off;
+    }
+    size_t put(const void* p, size_t bytes) {
+        const size_t off = reserve(bytes);
+        if (bytes) std::memcpy(host.data() + off, p, bytes);
+        return off;
+    }
+    template <class T>
+    size_t put(const std::vector<T>& v) {
+        return v.empty() ? reserve(0) : put(v.data(), sizeof(T) * v.size());
+    }
+};
+
+// A staged pointer to patch: segment `seg`'s in0/in1 (field 0/1) or MvNormal
+// instance `seg`'s mean (field 2) = device block + off.
+struct Reloc {
+    size_t seg;
+    int field;
+    size_t off;
+};
+
+// Builds and launches one batch (called without the server lock). Only the
+// first use of a covariance synchronises (its FP64 factorisation must be
+// checked); everything else is enqueued on the batch's stream.
 int Server::prepare_and_launch(Batch* b) {
     hipStream_t s = b->stream;
     b->scratch.reset(new Scratch(s));
     Scratch& sc = *b->scratch;
     const int E = cfg.engines_per_client;
-    // Seed the engines of clients seen for the first time:
+    Stage st;
+    std::vector<Reloc> relocs;
+    // Engines of clients seen for the first time:
     // engine g of client c = Vmt19937_64(splitmix64(c ^ salt) + 4g).
-    // (The launcher marks at most one request per new client as seed_first.)
+    std::vector<std::pair<int, size_t>> seed_jobs;  // (slot, staged seeds)
     for (Req* q : b->reqs) {
         if (!q->seed_first) continue;
         std::vector<uint64_t> seeds(E);
         const uint64_t base = splitmix64(q->r.client ^ cfg.seed_salt);
         for (int g = 0; g < E; ++g) seeds[g] = base + 4ull * (uint64_t)g;
-        uint64_t* d_seeds;
-        if (sc.upload(&d_seeds, seeds) != hipSuccess) return ERD_ERR_HIP;
-        if (launch_seed(pool->states + (int64_t)q->slot * E, d_seeds, E, s) != hipSuccess) return ERD_ERR_HIP;
+        seed_jobs.emplace_back(q->slot, st.put(seeds));
     }
     std::vector<Segment> segs;
     std::vector<int32_t> seg_engine;
@@ -179,7 +285,6 @@ int Server::prepare_and_launch(Batch* b) {
     for (Req* q : b->reqs) {
         const erd_serve_request& r = q->r;
         if (hipMallocAsync(&q->out, q->out_bytes, s) != hipSuccess) return ERD_ERR_OOM;
-        const int32_t e0 = q->slot * E;
         if (r.kind == ERD_SERVE_MVNORMAL) {
             const int d = r.dim;
             const uint64_t key = hash_params(q->param);
@@ -201,11 +306,17 @@ int Server::prepare_and_launch(Batch* b) {
                 Factor f;
                 f.d = d;
                 const int dp = (d + 15) & ~15;
-                if (hipMalloc(&f.hi, sizeof(uint16_t) * dp * dp) != hipSuccess ||
-                    hipMalloc(&f.lo, sizeof(uint16_t) * dp * dp) != hipSuccess)
+                if (hipMalloc(&f.hi, sizeof(uint16_t) * dp * dp) != hipSuccess) return ERD_ERR_OOM;
+                if (hipMalloc(&f.lo, sizeof(uint16_t) * dp * dp) != hipSuccess) {
+                    (void)hipFree(f.hi);
                     return ERD_ERR_OOM;
-                hipMemcpyAsync(f.hi, hi, sizeof(uint16_t) * dp * dp, hipMemcpyDeviceToDevice, s);
-                hipMemcpyAsync(f.lo, lo, sizeof(uint16_t) * dp * dp, hipMemcpyDeviceToDevice, s);
+                }
+                if (hipMemcpyAsync(f.hi, hi, sizeof(uint16_t) * dp * dp, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                    hipMemcpyAsync(f.lo, lo, sizeof(uint16_t) * dp * dp, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+                    (void)hipFree(f.hi);
+                    (void)hipFree(f.lo);
+                    return ERD_ERR_HIP;
+                }
                 it = factors.emplace(fk, f).first;
             }
             float* z;
@@ -216,74 +327,83 @@ int Server::prepare_and_launch(Batch* b) {
                 sg.count = (int64_t)d * q->per;
                 sg.out = z + (int64_t)g * d * q->per;
                 segs.push_back(sg);
-                seg_engine.push_back(e0 + g);
+                seg_engine.push_back(engine_of(q, g));
             }
-            float* mean = nullptr;
-            if (!q->mean.empty()) {
-                if (sc.upload(&mean, q->mean) != hipSuccess) return ERD_ERR_HIP;
-            }
+            if (!q->mean.empty()) relocs.push_back({mv.size(), 2, st.put(q->mean)});
             mv.push_back(MvInstance{d, 0, q->per * q->e_req, it->second.hi, it->second.lo, z, (float*)q->out,
-                                    mean});
+                                    nullptr});
         } else if (r.kind == ERD_SERVE_MULTINOMIAL) {
             const int64_t K = r.dim;
             const int32_t T = (int32_t)r.a;
-            HostTable t;
-            int rc = build_table(q->param.data(), K, t);
-            if (rc) return rc;
             int32_t* o = (int32_t*)q->out;
             const bool small = T < std::max<int64_t>(30 * (K - 1), 100);
-            Segment proto{};
             if (small) {
+                HostTable t;
+                int rc = build_table(q->param.data(), K, t);
+                if (rc) return rc;
                 if (hipMemsetAsync(o, 0, q->out_bytes, s) != hipSuccess) return ERD_ERR_HIP;
+                Segment proto{};
                 proto.kind = SEG_MULTINOMIAL_SMALL;
                 proto.i0 = T;
                 proto.d[0] = (double)K;
-                rc = upload_table(t, proto, sc);
-                if (rc) return rc;
+                size_t in0, in1 = 0;
+                if (!t.cdf.empty()) {
+                    in0 = st.put(t.cdf);
+                    proto.d[1] = (double)t.cdf.size();
+                } else {
+                    in0 = st.put(t.prob);
+                    in1 = st.put(t.alias);
+                    proto.d[2] = (double)t.bitmask;
+                }
+                for (int g = 0; g < q->e_req; ++g) {
+                    Segment sg = proto;
+                    sg.count = q->per;
+                    sg.out = o + (int64_t)g * K * q->per;
+                    relocs.push_back({segs.size(), 0, in0});
+                    if (t.cdf.empty()) relocs.push_back({segs.size(), 1, in1});
+                    segs.push_back(sg);
+                    seg_engine.push_back(engine_of(q, g));
+                }
+                continue;
             }
             std::vector<double> probs = q->param;
             double sum = 0;
             for (double v : probs) sum += v;
             for (double& v : probs) v /= sum;
+            int32_t* rr;  // remaining trials per sample, written by the first row
+            if (sc.alloc(&rr, (size_t)q->per * q->e_req) != hipSuccess) return ERD_ERR_OOM;
             for (int g = 0; g < q->e_req; ++g) {
                 int32_t* og = o + (int64_t)g * K * q->per;
-                if (small) {
-                    Segment sg = proto;
-                    sg.count = q->per;
-                    sg.out = og;
-                    segs.push_back(sg);
-                    seg_engine.push_back(e0 + g);
-                    continue;
-                }
-                int32_t* rr;
-                if (sc.upload(&rr, std::vector<int32_t>((size_t)q->per, T)) != hipSuccess) return ERD_ERR_HIP;
+                int32_t* rg = rr + (int64_t)g * q->per;
                 double rest_p = 1 - probs[0];
                 for (int64_t row = 0; row + 1 < K; ++row) {
                     Segment sg{};
                     sg.kind = SEG_MULTINOMIAL_ROW;
+                    sg.i0 = row == 0 ? T : -1;
                     sg.count = q->per;
                     sg.out = og + row;
-                    sg.in0 = rr;
+                    sg.in0 = rg;
                     sg.d[0] = row == 0 ? (float)probs[0] : (float)(probs[row] / rest_p);
                     sg.d[1] = (double)K;
                     if (row > 0) rest_p -= probs[row];
                     segs.push_back(sg);
-                    seg_engine.push_back(e0 + g);
+                    seg_engine.push_back(engine_of(q, g));
                 }
                 Segment last{};
                 last.kind = SEG_MULTINOMIAL_LAST;
+                last.i0 = K == 1 ? T : -1;
                 last.count = q->per;
                 last.out = og + (K - 1);
-                last.in0 = rr;
+                last.in0 = rg;
                 last.d[1] = (double)K;
                 segs.push_back(last);
-                seg_engine.push_back(e0 + g);
+                seg_engine.push_back(engine_of(q, g));
             }
         } else {
             for (int g = 0; g < q->e_req; ++g) {
                 erd_request er{};
                 er.dist = r.kind;
-                er.engine = e0 + g;
+                er.engine = engine_of(q, g);
                 er.count = q->per;
                 er.out = (char*)q->out + (size_t)g * q->per * elem_bytes(r.kind);
                 er.a = r.a;
@@ -296,24 +416,80 @@ int Server::prepare_and_launch(Batch* b) {
             }
         }
     }
-    // Group the segments by engine, keeping each engine's order.
+    // L*Z plan (tuned variants); an untuned dimension bucket falls back to
+    // run_lz, which tunes it with synchronised trial launches.
+    std::vector<LzLaunch> plan;
+    const bool staged_lz = mv.empty() || plan_lz(mv, plan) == ERD_OK;
+    // Layout of the rest of the block: program, instances, tile queues.
     std::vector<size_t> order(segs.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(),
                      [&](size_t x, size_t y) { return seg_engine[x] < seg_engine[y]; });
-    std::vector<Segment> segs2;
-    std::vector<int32_t> eng2;
-    for (size_t i : order) {
-        segs2.push_back(segs[i]);
-        eng2.push_back(seg_engine[i]);
+    std::vector<int32_t> engine_list, begin;
+    for (size_t k = 0; k < order.size(); ++k) {
+        const int32_t eg = seg_engine[order[k]];
+        if (engine_list.empty() || engine_list.back() != eg) {
+            engine_list.push_back(eg);
+            begin.push_back((int32_t)k);
+        }
     }
-    int rc = run_segments(pool, segs2, eng2, s);
-    if (rc) return rc;
+    begin.push_back((int32_t)order.size());
+    const size_t off_segs = st.reserve(sizeof(Segment) * segs.size());
+    const size_t off_eng = st.put(engine_list);
+    const size_t off_begin = st.put(begin);
+    const size_t off_mv = st.reserve(sizeof(MvInstance) * mv.size());
+    std::vector<size_t> off_tiles, off_cnt;
+    if (staged_lz) {
+        const int32_t zero = 0;
+        for (const LzLaunch& l : plan) {
+            off_tiles.push_back(st.put(l.tiles));
+            off_cnt.push_back(st.put(&zero, sizeof(zero)));
+        }
+    }
+    char* dev;
+    if (sc.alloc(&dev, st.host.size()) != hipSuccess) return ERD_ERR_OOM;
+    for (const Reloc& rl : relocs) {
+        void* ptr = dev + rl.off;
+        if (rl.field == 0) segs[rl.seg].in0 = ptr;
+        else if (rl.field == 1) segs[rl.seg].in1 = ptr;
+        else mv[rl.seg].mean = (const float*)ptr;
+    }
+    Segment* sorted = reinterpret_cast<Segment*>(st.host.data() + off_segs);
+    for (size_t k = 0; k < order.size(); ++k) sorted[k] = segs[order[k]];
+    if (!mv.empty()) std::memcpy(st.host.data() + off_mv, mv.data(), sizeof(MvInstance) * mv.size());
+    // Pinned copy of the block (this stream slot's buffer; the slot is idle).
+    const int si = b->stream_idx;
+    if (pin_cap[si] < st.host.size()) {
+        if (pin[si]) (void)hipHostFree(pin[si]);
+        pin[si] = nullptr;
+        pin_cap[si] = 0;
+        const size_t cap = std::max(st.host.size(), (size_t)1 << 20) * 2;
+        if (hipHostMalloc(&pin[si], cap, hipHostMallocDefault) != hipSuccess) return ERD_ERR_OOM;
+        pin_cap[si] = cap;
+    }
+    std::memcpy(pin[si], st.host.data(), st.host.size());
+    if (hipMemcpyAsync(dev, pin[si], st.host.size(), hipMemcpyHostToDevice, s) != hipSuccess) return ERD_ERR_HIP;
+    for (auto& j : seed_jobs)
+        if (launch_seed(pool->states + (int64_t)j.first * E, (const uint64_t*)(dev + j.second), E, s) !=
+            hipSuccess)
+            return ERD_ERR_HIP;
+    if (!segs.empty() &&
+        launch_program(pool->states, (int32_t)engine_list.size(), (const int32_t*)(dev + off_eng),
+                       (const int32_t*)(dev + off_begin), (const Segment*)(dev + off_segs), pool->status,
+                       s) != hipSuccess)
+        return ERD_ERR_HIP;
     if (!mv.empty()) {
-        MvInstance* d_inst;
-        if (sc.upload(&d_inst, mv) != hipSuccess) return ERD_ERR_HIP;
-        rc = run_lz(mv, d_inst, s, sc);
-        if (rc) return rc;
+        MvInstance* d_inst = (MvInstance*)(dev + off_mv);
+        if (staged_lz) {
+            for (size_t k = 0; k < plan.size(); ++k)
+                if (launch_lz_gemm(plan[k].variant, plan[k].grid, (const MvTile*)(dev + off_tiles[k]),
+                                   (int32_t)plan[k].tiles.size(), (int32_t*)(dev + off_cnt[k]), d_inst,
+                                   s) != hipSuccess)
+                    return ERD_ERR_HIP;
+        } else {
+            const int rc = run_lz(mv, d_inst, s, sc);
+            if (rc) return rc;
+        }
     }
     if (hipEventRecord(b->ev, s) != hipSuccess) return ERD_ERR_HIP;
     return ERD_OK;
@@ -335,6 +511,7 @@ void Server::launcher_loop() {
                     std::vector<char> blocked(slot_client.size() + 1, 0);
                     std::vector<Req*> take;
                     int64_t take_elems = 0;
+                    const uint32_t serial = next_serial;
                     for (auto it = pending.begin(); it != pending.end();) {
                         Req* q = *it;
                         if ((int64_t)take.size() >= cfg.max_batch_requests) break;
@@ -342,25 +519,29 @@ void Server::launcher_loop() {
                             take_elems + q->out_elems > cfg.max_batch_elements)
                             break;
                         const int sl = q->slot;
-                        if (blocked[sl] || slot_busy[sl] ||
-                            budget_used + (int64_t)q->out_bytes > cfg.hbm_budget_bytes) {
-                            blocked[sl] = 1;
+                        bool ok = !blocked[sl] && budget_used + (int64_t)q->out_bytes <= cfg.hbm_budget_bytes;
+                        if (ok) {
+                            q->seed_first = !slot_seeded[sl];
+                            for_held(q, [&](int32_t g) { ok = ok && (owner[g] == 0 || owner[g] == serial); });
+                        }
+                        if (!ok) {
+                            q->seed_first = false;
+                            blocked[sl] = 1;  // keeps the client's later requests behind it
                             ++it;
                             continue;
                         }
+                        slot_seeded[sl] = 1;
+                        for_held(q, [&](int32_t g) { owner[g] = serial; });
                         budget_used += (int64_t)q->out_bytes;
                         take_elems += q->out_elems;
                         q->state = 1;
-                        if (!slot_seeded[sl]) {
-                            slot_seeded[sl] = 1;
-                            q->seed_first = true;
-                        }
                         take.push_back(q);
                         it = pending.erase(it);
                     }
                     if (!take.empty()) {
-                        for (Req* q : take) slot_busy[q->slot] = 1;
+                        ++next_serial;
                         b = new Batch();
+                        b->serial = serial;
                         b->reqs = take;
                         b->stream_idx = free_idx;
                         b->stream = streams[free_idx];
@@ -371,8 +552,8 @@ void Server::launcher_loop() {
                 cv.wait(lk);
             }
         }
-        hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
-        b->status = prepare_and_launch(b);
+        b->status = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming) == hipSuccess ? prepare_and_launch(b)
+                                                                                      : ERD_ERR_HIP;
         {
             std::lock_guard<std::mutex> lk(mu);
             inflight.push_back(b);
@@ -394,11 +575,11 @@ void Server::completer_loop() {
         if (b->status == ERD_OK) {
             if (hipEventSynchronize(b->ev) != hipSuccess) b->status = ERD_ERR_HIP;
         } else {
-            hipStreamSynchronize(b->stream);
+            (void)hipStreamSynchronize(b->stream);
         }
         const double t = now_s();
         b->scratch.reset();  // stream-ordered frees
-        hipEventDestroy(b->ev);
+        if (b->ev) (void)hipEventDestroy(b->ev);
         {
             std::lock_guard<std::mutex> lk(mu);
             inflight.pop_front();
@@ -406,7 +587,9 @@ void Server::completer_loop() {
                 q->t_done = t;
                 q->state = b->status == ERD_OK ? 2 : 3;
                 q->status = b->status;
-                slot_busy[q->slot] = 0;
+                for_held(q, [&](int32_t g) {
+                    if (owner[g] == b->serial) owner[g] = 0;
+                });
                 if (b->status == ERD_OK) {
                     latencies.push_back(t - q->t_submit);
                     ++done_requests;
@@ -442,6 +625,16 @@ int erd_server_create(erd_server** out, const erd_server_config* cfg) {
     if (!srv) return ERD_ERR_OOM;
     Server& S = srv->s;
     S.cfg = *cfg;
+    S.owner.assign((size_t)(cfg->max_clients * cfg->engines_per_client), 0);
+    // Outputs and scratch come from the device's default stream-ordered pool;
+    // keep freed blocks in it instead of returning them to the driver at
+    // every synchronisation, so that steady-state requests never map memory.
+    int dev = 0;
+    hipMemPool_t mp;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&mp, dev) == hipSuccess) {
+        uint64_t keep = ~0ull;
+        (void)hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     int rc = erd_engines_create_range(&S.pool, 0, 4, cfg->max_clients * cfg->engines_per_client, nullptr);
     if (rc) {
         delete srv;
@@ -456,6 +649,8 @@ int erd_server_create(erd_server** out, const erd_server_config* cfg) {
         }
         S.streams.push_back(st);
         S.stream_free.push_back(1);
+        S.pin.push_back(nullptr);
+        S.pin_cap.push_back(0);
     }
     S.launcher = std::thread([&S] { S.launcher_loop(); });
     S.completer = std::thread([&S] { S.completer_loop(); });
@@ -473,8 +668,7 @@ int erd_server_submit(erd_server* srv, const erd_serve_request* r, uint64_t* tic
         if (r->dim <= 0 || !r->host_param) return ERD_ERR_INVALID;
         q->param.assign(r->host_param, r->host_param + (size_t)r->dim * r->dim);
         if (r->host_mean) q->mean.assign(r->host_mean, r->host_mean + r->dim);
-        q->e_req = (int)std::min<int64_t>(E, std::max<int64_t>(1, (int64_t)r->dim * r->count / erd::kSplit));
-        q->per = erd::round_up((r->count + q->e_req - 1) / q->e_req, 8);
+        erd::spread(*r, E, 8, q.get());
         q->out_elems = (int64_t)r->dim * q->per * q->e_req;
         q->out_bytes = sizeof(float) * (size_t)q->out_elems;
     } else if (r->kind == ERD_SERVE_MULTINOMIAL) {
@@ -487,8 +681,7 @@ int erd_server_submit(erd_server* srv, const erd_serve_request* r, uint64_t* tic
         }
         if (!std::isfinite(sum)) return ERD_ERR_NONFINITE;
         if (!(sum > 0)) return ERD_ERR_INVALID;
-        q->e_req = (int)std::min<int64_t>(E, std::max<int64_t>(1, (int64_t)r->dim * r->count / erd::kSplit));
-        q->per = erd::round_up((r->count + q->e_req - 1) / q->e_req, 8);
+        erd::spread(*r, E, 8, q.get());
         q->out_elems = (int64_t)r->dim * q->per * q->e_req;
         q->out_bytes = sizeof(int32_t) * (size_t)q->out_elems;
     } else {
@@ -503,8 +696,7 @@ int erd_server_submit(erd_server* srv, const erd_serve_request* r, uint64_t* tic
         erd::Segment tmp;
         int rc = erd::make_segment(probe, tmp);
         if (rc) return rc;
-        q->e_req = (int)std::min<int64_t>(E, std::max<int64_t>(1, r->count / erd::kSplit));
-        q->per = erd::round_up((r->count + q->e_req - 1) / q->e_req, erd::packet_width(r->kind));
+        erd::spread(*r, E, erd::packet_width(r->kind), q.get());
         q->out_elems = q->per * q->e_req;
         q->out_bytes = erd::elem_bytes(r->kind) * (size_t)q->out_elems;
     }
@@ -514,6 +706,8 @@ int erd_server_submit(erd_server* srv, const erd_serve_request* r, uint64_t* tic
         const int sl = S.slot_for(r->client);
         if (sl < 0) return ERD_ERR_UNSUPPORTED;  // engine slots exhausted
         q->slot = sl;
+        q->eng0 = S.slot_cursor[sl];
+        S.slot_cursor[sl] = (q->eng0 + q->e_req) % E;
         q->ticket = S.next_ticket++;
         q->t_submit = erd::now_s();
         if (S.t_first_submit == 0) S.t_first_submit = q->t_submit;
@@ -618,13 +812,15 @@ int erd_server_destroy(erd_server* srv) {
     }
     S.completer.join();
     for (auto& kv : S.reqs)
-        if (kv.second->out) hipFreeAsync(kv.second->out, nullptr);
-    hipDeviceSynchronize();
+        if (kv.second->out) (void)hipFreeAsync(kv.second->out, nullptr);
+    (void)hipDeviceSynchronize();
     for (auto& kv : S.factors) {
-        hipFree(kv.second.hi);
-        hipFree(kv.second.lo);
+        (void)hipFree(kv.second.hi);
+        (void)hipFree(kv.second.lo);
     }
-    for (hipStream_t st : S.streams) hipStreamDestroy(st);
+    for (hipStream_t st : S.streams) (void)hipStreamDestroy(st);
+    for (void* p : S.pin)
+        if (p) (void)hipHostFree(p);
     erd_engines_destroy(S.pool);
     delete srv;
     return ERD_OK;

@@ -4,8 +4,10 @@ A `Server` owns per-client Vmt19937_64 engines resident on the device and a
 launcher/completer pair of native threads that batch queued requests into
 segment-program launches as streams and HBM budget free up. Engine g of client
 c is seeded `splitmix64(c ^ seed_salt) + 4 g` (its four MT19937-64 lanes take
-that value .. +3), so a client's results depend only on its own request
-sequence.
+that value .. +3). A request uses `spread(...)[0]` consecutive engines of its
+client starting at a per-client cursor that advances (modulo the client's
+engine count) in submission order, so a client's results depend only on its
+own request sequence.
 """
 import ctypes as C
 
@@ -15,7 +17,7 @@ from . import _lib
 
 MVNORMAL = 100
 MULTINOMIAL = 101
-SPLIT = 1 << 16  # outputs per engine before a request spreads over more engines
+ENGINE_WORK = 65536.0  # normal-equivalent samples per engine
 
 
 class ServerConfig(C.Structure):
@@ -51,9 +53,37 @@ def client_seeds(client, engines, seed_salt=0):
     return [(base + 4 * g) & ((1 << 64) - 1) for g in range(engines)]
 
 
-def spread(count, engines_per_client, width, dim=1):
-    """(engines, per-engine outputs or columns) a request is spread over."""
-    e = int(min(engines_per_client, max(1, (dim * count) // SPLIT)))
+def chunk_for(kind, a=0.0, b=0.0, dim=0):
+    """Outputs (samples for MvNormal/multinomial) one engine takes; mirrors
+    chunk_for in serve.hip."""
+    if isinstance(kind, str):
+        kind = _lib.DIST[kind]
+    cost = 1.0
+    if kind == 8:        # Student-t
+        cost = 2.0
+    elif kind == 9:      # gamma
+        if a == 1.0:
+            cost = 1.0
+        elif a == float(int(a)) and a < 20:
+            cost = a / 2
+        else:
+            cost = 5.0
+    elif kind == 10:     # Poisson
+        cost = 1.0 + a / 2 if a < 12 else 8.0
+    elif kind == 11:     # binomial
+        m = a * min(b, 1 - b)
+        cost = 1.0 + a / 8 if a < 25 else (1.5 if m < 1 else 12.0)
+    elif kind == MVNORMAL:
+        cost = float(dim)
+    elif kind == MULTINOMIAL:
+        cost = 1.0 + a / 8 + dim / 8 if a < max(30.0 * (dim - 1), 100.0) else 12.0 * dim
+    return max(1, int(ENGINE_WORK / max(cost, 1e-3)))
+
+
+def spread(kind, count, engines_per_client, width, a=0.0, b=0.0, dim=0):
+    """(engines, per-engine outputs or samples) a request is spread over."""
+    chunk = chunk_for(kind, a, b, dim)
+    e = int(min(engines_per_client, -(-count // chunk)))
     per = -(-count // e)
     per = -(-per // width) * width
     return e, per

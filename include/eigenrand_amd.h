@@ -80,6 +80,25 @@ int erd_engines_status(erd_engines* e, uint32_t* status_out, void* stream);
  * of draw k; per_engine = 4 * draws. */
 int erd_fill_u64(erd_engines* e, uint64_t* out, int64_t per_engine, void* stream);
 
+/* ------------------------------------------------------- P8 adaptor --- */
+/* P8_mt19937_64 = ParallelRandomEngineAdaptor<uint64_t, Vmt19937_64, 8>
+ * (PacketRandomEngine.h:429-517, 600-609): adaptor k = the Vmt pair
+ * (seeds[k], seeds[k]+4), so its 8 u64 lanes are mt19937_64(seed .. seed+7)
+ * on any ISA. A set of n adaptors is an erd_engines set of 2n Vmt engines
+ * (adaptor k = engines 2k, 2k+1); snapshot/copy/discard work on it as usual.
+ * The adaptor's output buffers are empty between calls because every fill
+ * is a whole number of refills. */
+int erd_engines_create_p8(erd_engines** out, const uint64_t* seeds, int64_t n_p8, void* stream);
+/* operator() of P8_mt19937_64: out[8k + j] = mt19937_64(seed + j)[k];
+ * per_p8 is a multiple of 8. */
+int erd_p8_fill_u64(erd_engines* e, uint64_t* out, int64_t per_p8, void* stream);
+/* operator() of P8_mt19937_64_32: 16 uint32 per refill, Vmt i's 4 lanes as
+ * low/high word pairs at [8i, 8i+8); per_p8 is a multiple of 16. */
+int erd_p8_fill_u32(erd_engines* e, uint32_t* out, int64_t per_p8, void* stream);
+/* uniform_real() of the adaptor: bit_to_ur_float of the same 16 words per
+ * refill; per_p8 is a multiple of 16. */
+int erd_p8_fill_uniform_f32(erd_engines* e, float* out, int64_t per_p8, void* stream);
+
 /* ---------------------------------------------------------- univariate --- */
 /* uniformReal<float>: StdUniformRealGen (Basic.h:584-590); with (a,b) it is
  * UniformRealGen (Basic.h:626-634): u*(b-a)+a. Identical values for (0,1). */
@@ -229,17 +248,20 @@ int erd_multinomial_batch(erd_engines* e, int64_t n_inst, const int64_t* dims, c
 
 /* ---------------------------------------------------------------- serving --- */
 /* A request loop with continuous batching. Each client (any 64-bit id) owns
- * engines_per_client Vmt19937_64 engines resident on the device, engine g
- * seeded splitmix64(client ^ seed_salt) + 4g. Requests queue as they arrive;
- * a launcher thread packs the oldest admissible ones (client not busy, output
- * within the HBM budget) into one launch on a free stream while earlier
- * batches run; a completion thread stamps them done. A request spreads over
- * e = clamp(count/65536, 1, engines_per_client) of its client's engines, each
- * filling an equal, packet-aligned block (MvNormal/multinomial: blocks of
- * samples); its result depends only on the client's request sequence, never
- * on what else is in the batch. Outputs are allocated by the server and
- * counted against hbm_budget_bytes until erd_server_release. No counterpart
- * in the reference (which has no runtime). */
+ * E = engines_per_client Vmt19937_64 engines resident on the device, engine g
+ * seeded splitmix64(client ^ seed_salt) + 4g. A request spreads over
+ * e = min(E, ceil(count / chunk)) consecutive engines of its client, starting
+ * at a per-client cursor that advances by e (mod E) in submission order; chunk
+ * gives each engine about 2^16 normal-equivalents of work (heavier samplers
+ * spread wider), and each engine fills an equal, packet-aligned block
+ * (MvNormal/multinomial: blocks of samples). Requests queue as they arrive; a
+ * launcher thread packs the oldest admissible ones (none of their engines held
+ * by a batch in flight, output within the HBM budget) into one launch on a
+ * free stream while earlier batches run; a completion thread stamps them done.
+ * A result depends only on the client's request sequence, never on what else
+ * is in the batch. Outputs are allocated by the server and counted against
+ * hbm_budget_bytes until erd_server_release. No counterpart in the reference
+ * (which has no runtime). */
 #define ERD_SERVE_MVNORMAL 100    /* dim=d, count=samples, host_param=cov (d*d col-major), host_mean */
 #define ERD_SERVE_MULTINOMIAL 101 /* dim=K, count=samples, a=trials, host_param=weights (K)        */
 

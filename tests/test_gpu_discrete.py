@@ -78,3 +78,19 @@ def test_binomial_vop_edge_parameters(er, orc):
     ref = orc.Engines(seeds=[3])
     out = er.binomial_vop(urng, torch.from_numpy(trials).cuda(), torch.from_numpy(p).cuda())
     assert_same(out.cpu().numpy(), orc.fill_binomial_vop_i32(ref, trials, p))
+
+
+@pytest.mark.parametrize("mean", [0.01, 2.5, 7.0, 11.9])
+def test_poisson_knuth_long_fills(er, orc, mean):
+    """Long Knuth-regime fills: many twists, windows that chain several
+    packets and packets that run past a window."""
+    urng = er.Vmt19937_64(seeds=SEEDS)
+    ref = orc.Engines(seeds=SEEDS)
+    for n in (16384, 36):
+        out = er.poisson(n, 1, urng, mean)
+        assert_same(flat(out), orc.fill_poisson_i32(ref, n, mean, threads=8))
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+    # BinomialGen's Poisson regime (n >= 25, n*min(p,1-p) < 1) with the flip
+    out = er.binomial(4096, 1, urng, 400, 0.998)
+    assert_same(flat(out), orc.fill_binomial_i32(ref, 4096, 400, 0.998, threads=8))
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)

@@ -172,3 +172,34 @@ def test_full_size_normal_sampled_parity_and_moments(er, orc):
     assert abs(v - 1) < 5 * (2 / n) ** 0.5
     assert abs(k - 3) < 5 * (24 / n) ** 0.5
     assert torch.isfinite(out).all()
+
+
+def test_p8_adaptor_streams(er, orc, mt_golden):
+    """P8_mt19937_64: out[8k + j] = mt19937_64(seed + j)[k]; the u32 and
+    uniform_real() fills continue the same refill sequence."""
+    urng = er.P8_mt19937_64(seeds=[0, 1000])
+    assert urng.adaptors == 2 and len(urng) == 4
+    d = er.p8_randbits_u64(8 * 1000, urng).cpu().numpy()
+    a0 = d[0].reshape(1000, 8)
+    for j in range(8):
+        np.testing.assert_array_equal(a0[:, j], mt_golden["streams"][j])
+    ref = orc.Engines(seeds=[1000, 1004])
+    np.testing.assert_array_equal(d[1], orc.fill_p8_u64(ref, 1000))
+    u32 = er.p8_randbits_u32(16 * 64, urng).cpu().numpy()
+    uf = er.p8_uniform_real(16 * 64, urng).cpu().numpy()
+    w = orc.fill_p8_u64(ref, 128).view(np.uint32)  # 16 words per refill
+    np.testing.assert_array_equal(u32[1], w[:16 * 64])
+    np.testing.assert_array_equal(uf[1], ((w[16 * 64:] & 0x7FFFFF) / 2.0 ** 23).astype(np.float32))
+    snap = urng.snapshot().reshape(4, -1)
+    np.testing.assert_array_equal(snap[2:].ravel(), ref.raw)
+
+
+def test_p8_rejects_bad_sizes(er):
+    urng = er.P8_mt19937_64(5)
+    with pytest.raises(er.InvalidArgument):
+        er.p8_randbits_u64(12, urng)
+    with pytest.raises(er.InvalidArgument):
+        er.p8_uniform_real(8, urng)
+    odd = er.Vmt19937_64(1, n=3)
+    with pytest.raises(er.InvalidArgument):
+        er.p8_randbits_u64(8, odd)

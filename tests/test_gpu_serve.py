@@ -28,35 +28,53 @@ def _spd(d, rng):
     return a @ a.T + d * np.eye(d)
 
 
+class Client:
+    """Replays a client's requests on plain engine sets seeded like the
+    server's, following its engine cursor."""
+
+    def __init__(self, client, E, salt=0x5EED):
+        self.eng = _direct_engines(client, E, salt)
+        self.E = E
+        self.cursor = 0
+
+    def parts(self, e):
+        """Contiguous views covering engines cursor .. cursor+e-1 (mod E)."""
+        c, E = self.cursor, self.E
+        self.cursor = (c + e) % E
+        if c + e <= E:
+            return [self.eng.view(c, e)]
+        return [self.eng.view(c, E - c), self.eng.view(0, c + e - E)]
+
+
 def test_serve_univariate_equals_direct_fills():
     import eigenrand_amd as er
     from eigenrand_amd.serving import spread
     E = 8
+    reqs = [("std_normal", 200_000, 0.0, 0.0, 8, torch.float32, np.float32,
+             lambda v, p, o: er.normal(p, 1, v, out=o)),
+            ("gamma", 70_001, 3.7, 2.0, 8, torch.float32, np.float32,
+             lambda v, p, o: er.gamma(p, 1, v, 3.7, 2.0, out=o)),
+            ("poisson", 1000, 16.0, 0.0, 4, torch.int32, np.int32,
+             lambda v, p, o: er.poisson(p, 1, v, 16.0, out=o)),
+            ("poisson", 100_000, 3.5, 0.0, 4, torch.int32, np.int32,
+             lambda v, p, o: er.poisson(p, 1, v, 3.5, out=o)),
+            ("binomial", 300_000, 100, 0.75, 4, torch.int32, np.int32,
+             lambda v, p, o: er.binomial(p, 1, v, 100, 0.75, out=o))]
     with _srv() as s:
-        t1 = s.submit(7, "std_normal", 200_000)
-        t2 = s.submit(7, "gamma", 70_001, a=3.7, b=2.0)
-        t3 = s.submit(7, "poisson", 1000, a=16.0)
-        t4 = s.submit(7, "binomial", 300_000, a=100, b=0.75)
-        g1 = s.fetch(t1, np.float32)
-        g2 = s.fetch(t2, np.float32)
-        g3 = s.fetch(t3, np.int32)
-        g4 = s.fetch(t4, np.int32)
-    eng = _direct_engines(7, E)
-    e1, p1 = spread(200_000, E, 8)
-    e2, p2 = spread(70_001, E, 8)
-    e3, p3 = spread(1000, E, 4)
-    e4, p4 = spread(300_000, E, 4)
-    assert (e1, e2, e3, e4) == (3, 1, 1, 4)
-    o1 = torch.empty(e1 * p1, dtype=torch.float32, device="cuda")
-    er.normal(p1, 1, eng.view(0, e1), out=o1)
-    o2 = torch.empty(e2 * p2, dtype=torch.float32, device="cuda")
-    er.gamma(p2, 1, eng.view(0, e2), 3.7, 2.0, out=o2)
-    o3 = torch.empty(e3 * p3, dtype=torch.int32, device="cuda")
-    er.poisson(p3, 1, eng.view(0, e3), 16.0, out=o3)
-    o4 = torch.empty(e4 * p4, dtype=torch.int32, device="cuda")
-    er.binomial(p4, 1, eng.view(0, e4), 100, 0.75, out=o4)
-    for g, o in ((g1, o1), (g2, o2), (g3, o3), (g4, o4)):
-        np.testing.assert_array_equal(g, o.cpu().numpy())
+        tickets = [s.submit(7, k, n, a=a, b=b) for k, n, a, b, *_ in reqs]
+        got = [s.fetch(t, r[6]) for t, r in zip(tickets, reqs)]
+    cl = Client(7, E)
+    engines_used = []
+    for (kind, n, a, b, width, tdt, ndt, fill), g in zip(reqs, got):
+        e, p = spread(kind, n, E, width, a, b)
+        engines_used.append(e)
+        outs = []
+        for v in cl.parts(e):
+            o = torch.empty(len(v) * p, dtype=tdt, device="cuda")
+            fill(v, p, o)
+            outs.append(o.cpu().numpy())
+        np.testing.assert_array_equal(g, np.concatenate(outs), err_msg=kind)
+    assert engines_used[1] > 1 and engines_used[-1] == E  # wrapped and overlapping ranges
 
 
 def test_serve_mvnormal_and_multinomial_equal_batch_calls():
@@ -73,18 +91,27 @@ def test_serve_mvnormal_and_multinomial_equal_batch_calls():
         tm = s.submit(3, MVNORMAL, 9000, cov=cov, mean=mean)
         ts = s.submit(4, MULTINOMIAL, 1003, a=50, weights=w_small)
         tr = s.submit(5, MULTINOMIAL, 20000, a=1000, weights=w_rows)
+        tr2 = s.submit(5, MULTINOMIAL, 64, a=1000, weights=w_rows)
         gm = s.fetch(tm, np.float32)
         gs = s.fetch(ts, np.int32)
         gr = s.fetch(tr, np.int32)
-    em, pm = spread(9000, E, 8, d)
-    xs = er.mvnormal_batch(_direct_engines(3, E).view(0, em), [cov] * em, [pm] * em, means=[mean] * em)
-    want = np.concatenate([x.t().contiguous().cpu().numpy().ravel() for x in xs])
-    np.testing.assert_array_equal(gm, want)
-    for got, client, w, T, n in ((gs, 4, w_small, 50, 1003), (gr, 5, w_rows, 1000, 20000)):
-        e, p = spread(n, E, 8, len(w))
-        ms = er.multinomial_batch(_direct_engines(client, E).view(0, e), [T] * e, [w] * e, [p] * e)
-        want = np.concatenate([m.t().contiguous().cpu().numpy().ravel() for m in ms])
-        np.testing.assert_array_equal(got, want)
+        gr2 = s.fetch(tr2, np.int32)
+    em, pm = spread(MVNORMAL, 9000, E, 8, dim=d)
+    assert em > 1
+    want = []
+    for v in Client(3, E).parts(em):
+        xs = er.mvnormal_batch(v, [cov] * len(v), [pm] * len(v), means=[mean] * len(v))
+        want += [x.t().contiguous().cpu().numpy().ravel() for x in xs]
+    np.testing.assert_array_equal(gm, np.concatenate(want))
+    c4, c5 = Client(4, E), Client(5, E)
+    for got, cl, w, T, n in ((gs, c4, w_small, 50, 1003), (gr, c5, w_rows, 1000, 20000),
+                             (gr2, c5, w_rows, 1000, 64)):
+        e, p = spread(MULTINOMIAL, n, E, 8, a=T, dim=len(w))
+        want = []
+        for v in cl.parts(e):
+            ms = er.multinomial_batch(v, [T] * len(v), [w] * len(v), [p] * len(v))
+            want += [m.t().contiguous().cpu().numpy().ravel() for m in ms]
+        np.testing.assert_array_equal(got, np.concatenate(want))
         assert (got.reshape(-1, len(w)).sum(axis=1) == T).all()
 
 
