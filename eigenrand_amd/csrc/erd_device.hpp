@@ -47,19 +47,43 @@ __device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t as_u(float f) { return __float_as_uint(f); }
 
 // ----------------------------------------------------------- MT19937-64 ---
+// a ^ (b & c) in one v_bitop3_b32 (truth-table index 4a + 2b + c).
+__device__ __forceinline__ uint32_t xor_and(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x78);
+}
+// Bits 31..0 of ({hi, lo} >> n), n < 32: one v_alignbit_b32.
+__device__ __forceinline__ uint32_t funnel_lo(uint32_t hi, uint32_t lo, uint32_t n) {
+    return __builtin_amdgcn_alignbit(hi, lo, n);
+}
+
 __device__ __forceinline__ uint64_t temper(uint64_t y) {
-    // PacketRandomEngine.h:303-308
-    y ^= (y >> 29) & 0x5555555555555555ull;
-    y ^= (y << 17) & 0x71d67fffeda60000ull;
-    y ^= (y << 37) & 0xfff7eee000000000ull;
-    y ^= y >> 43;
-    return y;
+    // PacketRandomEngine.h:303-308, on 32-bit halves:
+    //   y ^= (y >> 29) & 0x5555555555555555
+    //   y ^= (y << 17) & 0x71d67fffeda60000
+    //   y ^= (y << 37) & 0xfff7eee000000000
+    //   y ^= y >> 43
+    uint32_t lo = (uint32_t)y, hi = (uint32_t)(y >> 32);
+    const uint32_t a_lo = funnel_lo(hi, lo, 29), a_hi = hi >> 29;
+    lo = xor_and(lo, a_lo, 0x55555555u);
+    hi = xor_and(hi, a_hi, 0x55555555u);
+    const uint32_t b_lo = lo << 17, b_hi = funnel_lo(hi, lo, 15);
+    lo = xor_and(lo, b_lo, 0xeda60000u);
+    hi = xor_and(hi, b_hi, 0x71d67fffu);
+    hi = xor_and(hi, lo << 5, 0xfff7eee0u);
+    lo ^= hi >> 11;
+    return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ __forceinline__ uint64_t twist_word(uint64_t cur, uint64_t next, uint64_t far) {
-    uint64_t y = (cur & kUM) | (next & kLM);
-    uint64_t mag = (uint64_t)(-(int64_t)(y & 1)) & kA;
-    return far ^ (y >> 1) ^ mag;
+    // y = (cur & UM) | (next & LM); far ^ (y >> 1) ^ (y odd ? A : 0), with
+    // UM = upper 33 bits, on 32-bit halves.
+    const uint32_t y_hi = (uint32_t)(cur >> 32);
+    const uint32_t y_lo = __builtin_amdgcn_bitop3_b32((uint32_t)cur, (uint32_t)next, 0x7fffffffu, 0xd8);  // c ? b : a
+    const uint32_t sh_lo = funnel_lo(y_hi, y_lo, 1), sh_hi = y_hi >> 1;
+    const uint32_t odd = 0u - (y_lo & 1u);
+    const uint32_t r_lo = (uint32_t)far ^ sh_lo ^ (odd & (uint32_t)kA);
+    const uint32_t r_hi = (uint32_t)(far >> 32) ^ sh_hi ^ (odd & (uint32_t)(kA >> 32));
+    return ((uint64_t)r_hi << 32) | r_lo;
 }
 
 // In-place twist of the 4 lanes held in LDS (packet-major, so consecutive
@@ -97,6 +121,28 @@ __device__ __forceinline__ void store_state(EngineState* st, const uint64_t* lds
 
 // ------------------------------------------------------ bits -> uniforms ---
 // bit_to_ur_float: MorePacketMath.h:205-215.
+// Correctly rounded sqrt for x = +-0 or x >= 2^-96 (normal, no rescaling
+// needed): v_sqrt_f32 is within 1 ulp; the neighbour whose residual shows it
+// is the rounded root replaces it (the same correction the compiler applies
+// for -fhip-fp32-correctly-rounded-divide-sqrt, without its denormal scaling
+// and zero/infinity checks, which these inputs never need).
+__device__ __forceinline__ float sqrt_cr_normal(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = __int_as_float(__float_as_int(s) - 1);
+    const float up = __int_as_float(__float_as_int(s) + 1);
+    const float r_dn = __builtin_fmaf(-dn, s, x);
+    const float r_up = __builtin_fmaf(-up, s, x);
+    float r = (r_dn <= 0.0f) ? dn : s;
+    r = (r_up > 0.0f) ? up : r;
+    return r;
+}
+
+// 1 - bit_to_ur_float(w): (f - 1) is exact for f in [1, 2), and so is
+// 1 - (f - 1) = 2 - f.
+__device__ __forceinline__ float one_minus_ur_f32(uint32_t w) {
+    return 2.0f - as_f((w & 0x7FFFFFu) | 0x3F800000u);
+}
+
 __device__ __forceinline__ float ur_f32(uint32_t w) {
     return as_f((w & 0x7FFFFFu) | 0x3F800000u) - 1.0f;
 }

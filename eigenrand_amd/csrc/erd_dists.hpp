@@ -103,9 +103,10 @@ struct NormalF32 {
         float sn[2], cs[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const float u1 = 1.0f - ur_f32(h ? hi32(w[0]) : lo32(w[0]));
+            const float u1 = one_minus_ur_f32(h ? hi32(w[0]) : lo32(w[0]));
             const float u2 = ur_f32(h ? hi32(w[1]) : lo32(w[1]));
-            const float radius = sqrtf(-2.0f * plog_pos(u1));
+            // -2 ln(u1) for u1 in [2^-23, 1] is +-0 or >= 2^-22
+            const float radius = sqrt_cr_normal(-2.0f * plog_pos(u1));
             float s, c;
             psincos(ERD_F(2 * kPi) * u2, s, c);
             sn[h] = radius * s;

@@ -65,11 +65,26 @@ __device__ void op_fixed(Eng& e, uint64_t* carry, int64_t per, const D& d, int l
         const int avail = nc + (kN - pos);
         const int64_t can = (int64_t)(avail / C);
         const int nu = (int)(units - done < can ? units - done : can);
-        for (int t = lane; t < nu * kLanes; t += 64) {
+        int t = lane;
+        if (nc > 0) {
+            // Only unit 0 can hold carried words (nc < C); the first pass of
+            // the wave takes the general path.
+            if (t < nu * kLanes) {
+                const int u = t >> 2, l = t & 3;
+                uint64_t w[C];
+#pragma unroll
+                for (int c = 0; c < C; ++c) w[c] = slot_word<kLanes>(e.mt, carry, nc, pos, u * C + c, l);
+                d.emit(per, done + u, l, w);
+            }
+            t += 64;
+        }
+        // Units past the carried words read the block directly: word c of
+        // unit u is block draw pos - nc + u*C + c.
+        for (; t < nu * kLanes; t += 64) {
             const int u = t >> 2, l = t & 3;
             uint64_t w[C];
 #pragma unroll
-            for (int c = 0; c < C; ++c) w[c] = slot_word<kLanes>(e.mt, carry, nc, pos, u * C + c, l);
+            for (int c = 0; c < C; ++c) w[c] = temper(e.mt[(pos - nc + u * C + c) * kLanes + l]);
             d.emit(per, done + u, l, w);
         }
         const int consumed = nu * C;

@@ -76,8 +76,10 @@ size_t elem_bytes(int kind) {
 }
 
 // Outputs (samples for MvNormal/multinomial) one engine takes for a request,
-// from the relative cost of each sampler (measured full-chip rates: normal =
-// 1, rejection gamma ~5, tangent Poisson ~7, binomial ~12, Knuth ~ mean + 2).
+// from the relative cost of each sampler: full-chip rates of the fills, as a
+// multiple of the normal fill's time per sample (gamma: GS 5, tangent 6;
+// Poisson: Knuth 10 + 6 * mean, tangent 7; binomial: n < 25 1 + 1.2 n,
+// Knuth 13, tangent 12).
 int64_t chunk_for(const erd_serve_request& r) {
     double cost = 1.0;
     switch (r.kind) {
@@ -86,13 +88,13 @@ int64_t chunk_for(const erd_serve_request& r) {
             const double a = r.a;
             if (a == 1.0) cost = 1.0;
             else if (a == std::floor(a) && a < 20) cost = a / 2;
-            else cost = 5.0;
+            else cost = a < 1 ? 5.0 : 6.0;
             break;
         }
-        case ERD_DIST_POISSON_I32: cost = r.a < 12 ? 1.0 + r.a / 2 : 8.0; break;
+        case ERD_DIST_POISSON_I32: cost = r.a < 12 ? 10.0 + 6.0 * r.a : 7.0; break;
         case ERD_DIST_BINOMIAL_I32: {
             const double n = r.a, m = n * std::min(r.b, 1 - r.b);
-            cost = n < 25 ? 1.0 + n / 8 : (m < 1 ? 1.5 : 12.0);
+            cost = n < 25 ? 1.0 + 1.2 * n : (m < 1 ? 13.0 : 12.0);
             break;
         }
         case ERD_SERVE_MVNORMAL: cost = (double)r.dim; break;
@@ -147,7 +149,7 @@ struct Batch {
 // e = ceil(count / chunk) engines (at most E), each taking an equal,
 // packet-aligned share.
 void spread(const erd_serve_request& r, int E, int width, Req* q) {
-    const int64_t chunk = chunk_for(r);
+    const int64_t chunk = std::max<int64_t>(chunk_for(r), width);  // at least one packet
     q->e_req = (int)std::min<int64_t>(E, (r.count + chunk - 1) / chunk);
     q->per = round_up((r.count + q->e_req - 1) / q->e_req, width);
 }

@@ -67,12 +67,12 @@ def chunk_for(kind, a=0.0, b=0.0, dim=0):
         elif a == float(int(a)) and a < 20:
             cost = a / 2
         else:
-            cost = 5.0
+            cost = 5.0 if a < 1 else 6.0
     elif kind == 10:     # Poisson
-        cost = 1.0 + a / 2 if a < 12 else 8.0
+        cost = 10.0 + 6.0 * a if a < 12 else 7.0
     elif kind == 11:     # binomial
         m = a * min(b, 1 - b)
-        cost = 1.0 + a / 8 if a < 25 else (1.5 if m < 1 else 12.0)
+        cost = 1.0 + 1.2 * a if a < 25 else (13.0 if m < 1 else 12.0)
     elif kind == MVNORMAL:
         cost = float(dim)
     elif kind == MULTINOMIAL:
@@ -82,7 +82,7 @@ def chunk_for(kind, a=0.0, b=0.0, dim=0):
 
 def spread(kind, count, engines_per_client, width, a=0.0, b=0.0, dim=0):
     """(engines, per-engine outputs or samples) a request is spread over."""
-    chunk = chunk_for(kind, a, b, dim)
+    chunk = max(chunk_for(kind, a, b, dim), width)
     e = int(min(engines_per_client, -(-count // chunk)))
     per = -(-count // e)
     per = -(-per // width) * width
