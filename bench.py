@@ -127,6 +127,9 @@ def sweep():
     emit("VoP gamma f32", nv, timed(lambda: er.gamma_vop(vp, shape, ones, out=vo), reps=3), 12)
     emit("VoP binomial i32", nv, timed(lambda: er.binomial_vop(vp, bn, bp, out=vi), reps=3), 12)
     emit("VoP poisson i32", nv, timed(lambda: er.poisson_vop(vp, pm, out=vi), reps=3), 8)
+    del out, outi, mu, sd, shape, ones, bn, bp, pm, vo, vi
+    torch.cuda.empty_cache()
+    mv_sweep(lambda d: print(json.dumps(d), flush=True))
 
 
 def serving_workload(n_req, n_clients, seed=5):
@@ -264,6 +267,71 @@ def serving(n_req=300, n_clients=64, engines_per_client=1024):
     return res
 
 
+def mv_sweep(emit_line):
+    """Batched multivariate configuration: 4096 MvNormal (d ~ U{4..256},
+    cov = A A^T + d I with A ~ N(0, 1/d), 1024 samples), 512 Wishart
+    (df = d + 3, 64 draws, d ~ U{4..256}) and 2048 Dirichlet (K ~ U{2..1000},
+    alpha ~ logU(0.01, 10), 1024 samples each; the sample count is this
+    build's choice). Instance i runs on engine i; parameters from
+    numpy default_rng(4). Factorisation (the generators' constructors) is
+    timed once, generation over repeated launches."""
+    import numpy as np
+    import torch
+    import eigenrand_amd as er
+    rng = np.random.default_rng(4)
+
+    def covs_for(dims):
+        out = []
+        for d in dims:
+            a = rng.normal(0, 1 / np.sqrt(d), (d, d))
+            out.append(a @ a.T + d * np.eye(d))
+        return out
+
+    def once(fn):
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        r = fn()
+        b.record()
+        torch.cuda.synchronize()
+        return r, a.elapsed_time(b)
+
+    dims = rng.integers(4, 257, 4096)
+    covs = covs_for(dims)
+    gen, ms_f = once(lambda: er.MvNormalBatch(covs))
+    urng = er.Vmt19937_64(0, n=4096, stride=4)
+    n = 1024
+    out = torch.empty(int(dims.sum()) * n, dtype=torch.float32, device="cuda")
+    ms = timed(lambda: gen.generate(urng, n, out=out, flat=True), reps=3)
+    elems = int(dims.sum()) * n
+    read = int((dims.astype(np.int64) ** 2 * 8 + dims * 8).sum())
+    emit_line({"sweep": "MvNormal batch generate", "instances": 4096, "elements": elems, "ms": round(ms, 4),
+               "gsamples_per_s": round(elems / ms / 1e6, 3),
+               "hbm_gbs": round((4 * elems + read) / ms / 1e6, 1),
+               "hbm_frac": round((4 * elems + read) / ms / 1e6 / HBM_PEAK_GBS, 4),
+               "factorise_ms": round(ms_f, 3)})
+    del gen, out, covs
+    wd = rng.integers(4, 257, 512)
+    scales = covs_for(wd)
+    wgen, ms_f = once(lambda: er.WishartBatch(scales, wd + 3))
+    wrng = er.Vmt19937_64(1 << 20, n=512, stride=4)
+    wout = torch.empty(int((wd.astype(np.int64) ** 2).sum()) * 64, dtype=torch.float32, device="cuda")
+    ms = timed(lambda: wgen.generate(wrng, 64, out=wout, flat=True), reps=3)
+    elems = wout.numel()
+    emit_line({"sweep": "Wishart batch generate", "instances": 512, "elements": elems, "ms": round(ms, 4),
+               "gsamples_per_s": round(elems / ms / 1e6, 3), "hbm_gbs": round(4 * elems / ms / 1e6, 1),
+               "hbm_frac": round(4 * elems / ms / 1e6 / HBM_PEAK_GBS, 4), "factorise_ms": round(ms_f, 3)})
+    del wgen, wout, scales
+    K = rng.integers(2, 1001, 2048)
+    alphas = [np.exp(rng.uniform(np.log(0.01), np.log(10), k)).astype(np.float32) for k in K]
+    drng = er.Vmt19937_64(1 << 21, n=2048, stride=4)
+    ms = timed(lambda: er.dirichlet_batch(drng, alphas, 1024), reps=3)
+    elems = int(K.sum()) * 1024
+    emit_line({"sweep": "Dirichlet batch", "instances": 2048, "elements": elems, "ms": round(ms, 4),
+               "gsamples_per_s": round(elems / ms / 1e6, 3), "hbm_gbs": round(4 * elems / ms / 1e6, 1),
+               "hbm_frac": round(4 * elems / ms / 1e6 / HBM_PEAK_GBS, 4)})
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "normal_f32_2p28_traffic.json")
     if not os.path.exists(p):
@@ -279,6 +347,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--sweep", action="store_true", help="secondary per-distribution measurements")
+    ap.add_argument("--mv", action="store_true", help="only the batched multivariate measurements")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (p50/p99) leg")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -303,6 +372,9 @@ def main():
         return
     if args.sweep:
         sweep()
+        return
+    if args.mv:
+        mv_sweep(lambda d: print(json.dumps(d), flush=True))
         return
     stream = torch.cuda.current_stream()
     urng = er.Vmt19937_64(0, n=N_ENGINES, stride=4)

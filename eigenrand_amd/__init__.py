@@ -30,6 +30,7 @@ __all__ = [
     "exponential", "cauchy", "student_t", "gamma", "chi_squared", "poisson", "binomial",
     "binomial_vop", "gamma_vop", "poisson_vop", "run_requests", "cholesky_batch",
     "mvnormal_batch", "wishart_batch", "dirichlet_batch", "MvNormalGen", "WishartGen",
+    "MvFactors", "MvNormalBatch", "WishartBatch",
     "DirichletGen", "alias_table", "discrete", "multinomial_batch", "MultinomialGen",
 ]
 
@@ -437,32 +438,128 @@ def dirichlet_batch(urng, alphas, samples, stream=None):
     return res
 
 
+class MvFactors:
+    """FP64 Cholesky factors of a batch of covariances (or Wishart scales),
+    computed once on the device like the generators' constructors
+    (MvNormal.h:20-38, 86-98, 196-215) and reused by every generate()."""
+
+    def __init__(self, mats, stream=None):
+        n = len(mats)
+        self.dims = np.array([np.asarray(c).shape[0] for c in mats], dtype=np.int32)
+        a = _concat_f64(mats)
+        self.info = np.zeros(n, dtype=np.int32)
+        h = C.c_void_p()
+        _lib.call("erd_mv_factors_create", C.byref(h), n, self.dims.ctypes.data_as(C.c_void_p), _ptr(a),
+                  self.info.ctypes.data_as(C.c_void_p), _stream_ptr(stream))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self):
+        return int(self.dims.size)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.erd_mv_factors_destroy(h)
+            self._h = None
+
+
+class MvNormalBatch:
+    """Many MvNormalGen<float>(mean_i, cov_i) at once; generate() runs
+    instance i on engine i of `urng`, reusing the factors."""
+
+    def __init__(self, covs, means=None, stream=None):
+        self.factors = MvFactors(covs, stream)
+        self.mean = None
+        if means is not None:
+            self.mean = _torch().from_numpy(
+                np.concatenate([np.asarray(m, np.float32).ravel() for m in means])).cuda()
+
+    def generate(self, urng, samples, return_z=False, out=None, stream=None, flat=False):
+        """Per-instance (d_i, samples_i) views, or with flat=True the flat
+        output buffer (instances back to back, each column-major)."""
+        torch = _torch()
+        dims = self.factors.dims
+        n = dims.size
+        samples = np.broadcast_to(np.asarray(samples, dtype=np.int64), (n,)).copy()
+        total = int((dims.astype(np.int64) * samples).sum())
+        if out is None:
+            out = torch.empty(total, dtype=torch.float32, device="cuda")
+        z = torch.empty(total, dtype=torch.float32, device="cuda") if return_z else None
+        _lib.call("erd_mvnormal_generate", urng.handle, self.factors.handle, samples.ctypes.data_as(C.c_void_p),
+                  _ptr(self.mean), _ptr(out), _ptr(z), _stream_ptr(stream))
+        if flat:
+            return (out, z) if return_z else out
+        xs, zs, off = [], [], 0
+        for d, m in zip(dims, samples):
+            k = int(d) * int(m)
+            xs.append(out[off:off + k].view(int(m), int(d)).t())
+            if return_z:
+                zs.append(z[off:off + k].view(int(m), int(d)).t())
+            off += k
+        return (xs, zs) if return_z else xs
+
+
+class WishartBatch:
+    """Many WishartGen<float>(df_i, scale_i) at once (factors reused)."""
+
+    def __init__(self, scales, df, stream=None):
+        self.factors = MvFactors(scales, stream)
+        self.df = np.broadcast_to(np.asarray(df, dtype=np.int64), (len(scales),)).copy()
+
+    def generate(self, urng, samples, out=None, stream=None, flat=False):
+        torch = _torch()
+        dims = self.factors.dims
+        n = dims.size
+        samples = np.broadcast_to(np.asarray(samples, dtype=np.int64), (n,)).copy()
+        total = int((dims.astype(np.int64) ** 2 * samples).sum())
+        if out is None:
+            out = torch.empty(total, dtype=torch.float32, device="cuda")
+        _lib.call("erd_wishart_generate", urng.handle, self.factors.handle, self.df.ctypes.data_as(C.c_void_p),
+                  samples.ctypes.data_as(C.c_void_p), _ptr(out), _stream_ptr(stream))
+        if flat:
+            return out
+        res, off = [], 0
+        for d, m in zip(dims, samples):
+            k = int(d) * int(d) * int(m)
+            res.append(out[off:off + k].view(int(d) * int(m), int(d)).t())
+            off += k
+        return res
+
+
 class MvNormalGen:
-    """Rand::MvNormalGen<float>(mean, cov) on the GPU (MvNormal.h:56-116)."""
+    """Rand::MvNormalGen<float>(mean, cov) on the GPU (MvNormal.h:56-116);
+    the covariance is factorised once, at construction."""
 
     def __init__(self, mean, cov):
         self.mean = np.asarray(mean, dtype=np.float32).ravel()
         self.cov = np.asarray(cov, dtype=np.float64)
+        self._batch = MvNormalBatch([self.cov], [self.mean])
 
     def dims(self):
         return self.mean.size
 
     def generate(self, urng, samples):
-        return mvnormal_batch(urng, [self.cov], [samples], [self.mean])[0]
+        return self._batch.generate(urng, [samples])[0]
 
 
 class WishartGen:
-    """Rand::WishartGen<float>(df, scale) on the GPU (MvNormal.h:184-302)."""
+    """Rand::WishartGen<float>(df, scale) on the GPU (MvNormal.h:184-302);
+    the scale is factorised once, at construction."""
 
     def __init__(self, df, scale):
         self.df = int(df)
         self.scale = np.asarray(scale, dtype=np.float64)
+        self._batch = WishartBatch([self.scale], [self.df])
 
     def dims(self):
         return self.scale.shape[0]
 
     def generate(self, urng, samples):
-        return wishart_batch(urng, [self.scale], [self.df], [samples])[0]
+        return self._batch.generate(urng, [samples])[0]
 
 
 class DirichletGen:

@@ -75,6 +75,11 @@ _SIGS = {
     "erd_cholesky_f64_batch": [I64, P, P, P, P],
     "erd_mvnormal_batch": [P, I64, P, P, P, P, P, P, P],
     "erd_wishart_batch": [P, I64, P, P, P, P, P, P],
+    "erd_mv_factors_create": [C.POINTER(P), I64, P, P, P, P],
+    "erd_mv_factors_destroy": [P],
+    "erd_mv_factors_count": [P],
+    "erd_mvnormal_generate": [P, P, P, P, P, P, P],
+    "erd_wishart_generate": [P, P, P, P, P, P],
     "erd_dirichlet_batch": [P, I64, P, P, P, P, P],
     "erd_alias_table": [P, I64, P, P, P],
     "erd_fill_discrete_i32": [P, P, I64, P, I64, P],
@@ -97,6 +102,7 @@ _RESTYPES = {
     "erd_status_string": C.c_char_p,
     "erd_engine_state_bytes": C.c_size_t,
     "erd_engines_count": I64,
+    "erd_mv_factors_count": I64,
     "erd_engines_device_ptr": P,
 }
 

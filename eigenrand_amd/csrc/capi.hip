@@ -9,6 +9,8 @@
 #include "erd_device.hpp"
 #include "erd_kernels.hpp"
 #include "erd_capi_internal.hpp"
+#include "erd_internal.hpp"
+#include "erd_segment.hpp"
 
 using erd::EngineState;
 
@@ -338,7 +340,6 @@ int erd_stream_synchronize(void* stream) { return hip_status(hipStreamSynchroniz
 }  // extern "C"
 
 // ---------------------------------------------------------------- VoP + batched requests ---
-#include "erd_segment.hpp"
 
 extern "C" {
 
@@ -453,10 +454,7 @@ int run_segments(erd_engines* e, const std::vector<Segment>& segs, const std::ve
     std::memcpy(host.data() + seg_bytes, engine_of.data(), sizeof(int32_t) * (size_t)groups);
     std::memcpy(host.data() + seg_bytes + sizeof(int32_t) * (size_t)groups, begin.data(),
                 sizeof(int32_t) * (size_t)(groups + 1));
-    err = hipMemcpyAsync(dbuf, host.data(), host.size(), hipMemcpyHostToDevice, stream);
-    // The copy is staged from pageable memory; synchronise so that `host` may
-    // be released unconditionally when this function returns.
-    if (err == hipSuccess) err = hipStreamSynchronize(stream);
+    err = upload_async(dbuf, host.data(), host.size(), stream);
     if (err == hipSuccess)
         err = launch_program(e->states, groups, (const int32_t*)(dbuf + seg_bytes),
                              (const int32_t*)(dbuf + seg_bytes) + groups, (const Segment*)dbuf,

@@ -1,6 +1,7 @@
 // Definition of the opaque erd_engines handle (shared by the C ABI sources).
 #pragma once
 #include <stdint.h>
+#include <vector>
 #include "erd_device.hpp"
 
 struct erd_engines {
@@ -8,4 +9,14 @@ struct erd_engines {
     uint32_t* status = nullptr;          // device status word (kernel error bits)
     int64_t count = 0;
     bool owns = false;                   // views share the parent's memory
+};
+
+// Factors of a batch of MvNormalGen / WishartGen instances: the FP64 Cholesky
+// of each covariance, kept as bf16 hi/lo planes (row-major, padded to a
+// multiple of 16) for the L*Z transform.
+struct erd_mv_factors {
+    std::vector<int32_t> dims;
+    std::vector<int64_t> split_off;  // plane offset of instance i (elements)
+    uint16_t* lhi = nullptr;         // device, owned
+    uint16_t* llo = nullptr;
 };

@@ -11,6 +11,11 @@ struct erd_request;
 
 namespace erd {
 
+// Enqueues a host-to-device copy of `bytes` on `s` through pinned staging
+// (staging.hip): `src` may be released as soon as this returns, and the host
+// does not wait for the stream.
+hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t s);
+
 // Stream-ordered scratch released when the object goes out of scope.
 struct Scratch {
     hipStream_t s;
@@ -27,14 +32,19 @@ struct Scratch {
         if (e == hipSuccess) ptrs.push_back(*p);
         return e;
     }
-    // Host vectors may die right after the call, so the copy is completed
-    // before returning.
+    // Keep an allocation past this scope (the caller frees it later).
+    void keep(const void* p) {
+        for (size_t i = 0; i < ptrs.size(); ++i)
+            if (ptrs[i] == p) {
+                ptrs.erase(ptrs.begin() + (long)i);
+                return;
+            }
+    }
+    // Device copy of a host vector (which may be released on return).
     template <class T>
     hipError_t upload(T** p, const std::vector<T>& v) {
         hipError_t e = alloc(p, v.size());
-        if (e == hipSuccess && !v.empty())
-            e = hipMemcpyAsync(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e == hipSuccess && !v.empty()) e = upload_async(*p, v.data(), sizeof(T) * v.size(), s);
         return e;
     }
 };

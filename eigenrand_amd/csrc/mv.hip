@@ -323,20 +323,40 @@ __global__ __launch_bounds__(256) void k_syrk_tt(const SyrkTile* __restrict__ ti
 
 // ------------------------------------------------------------- Dirichlet ---
 // ret.array().rowwise() /= ret.array().colwise().sum() (Multinomial.h:200-201)
-// on the K x n result; g holds row k of gamma draws contiguously (g[k*n+s]).
-__global__ void k_dirichlet_norm(const int32_t* __restrict__ dims, const int64_t* __restrict__ samples,
-                                 const int64_t* __restrict__ g_off, const float* __restrict__ g,
-                                 float* __restrict__ out) {
+// on the K x n result; g holds row k of gamma draws contiguously (g[k*n+s]),
+// out is K x n column-major (out[s*K+k]). One wave per 64 samples: each lane
+// sums its sample's column in category order (coalesced over samples), then
+// 64x64 tiles are divided and transposed through LDS so that the stores run
+// along categories.
+constexpr int kDirTile = 64;
+
+__global__ __launch_bounds__(64) void k_dirichlet_norm(const int32_t* __restrict__ dims,
+                                                       const int64_t* __restrict__ samples,
+                                                       const int64_t* __restrict__ g_off,
+                                                       const float* __restrict__ g, float* __restrict__ out) {
+    __shared__ float tile[kDirTile][kDirTile + 1];
     const int i_inst = blockIdx.y;
     const int K = dims[i_inst];
     const int64_t n = samples[i_inst];
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
+    const int64_t s0 = (int64_t)blockIdx.x * kDirTile;
+    if (s0 >= n) return;
+    const int lane = threadIdx.x;
     const float* G = g + g_off[i_inst];
     float* O = out + g_off[i_inst];
+    const int64_t s = s0 + lane;
+    const int ns = (int)min<int64_t>(kDirTile, n - s0);
     float sum = 0.0f;
-    for (int k = 0; k < K; ++k) sum += G[(int64_t)k * n + s];
-    for (int k = 0; k < K; ++k) O[s * K + k] = G[(int64_t)k * n + s] / sum;
+    if (lane < ns)
+        for (int k = 0; k < K; ++k) sum += G[(int64_t)k * n + s];
+    for (int k0 = 0; k0 < K; k0 += kDirTile) {
+        const int nk = min(kDirTile, K - k0);
+        if (lane < ns)
+            for (int i = 0; i < nk; ++i) tile[i][lane] = G[(int64_t)(k0 + i) * n + s] / sum;
+        __syncthreads();
+        if (lane < nk)
+            for (int j = 0; j < ns; ++j) O[(s0 + j) * K + k0 + lane] = tile[lane][j];
+        __syncthreads();
+    }
 }
 
 hipError_t launch_wishart_assemble(const MvInstance* inst, const int64_t* nrm_off, const int64_t* chi_off,
@@ -357,8 +377,8 @@ hipError_t launch_syrk_tt(const SyrkTile* tiles, int n_tiles, const MvInstance* 
 hipError_t launch_dirichlet_norm(const int32_t* dims, const int64_t* samples, const int64_t* g_off,
                                  const float* g, float* out, int n_inst, int64_t max_n, hipStream_t s) {
     if (n_inst <= 0) return hipSuccess;
-    dim3 grid((unsigned)((max_n + 255) / 256), (unsigned)n_inst);
-    hipLaunchKernelGGL(k_dirichlet_norm, grid, dim3(256), 0, s, dims, samples, g_off, g, out);
+    dim3 grid((unsigned)((max_n + kDirTile - 1) / kDirTile), (unsigned)n_inst);
+    hipLaunchKernelGGL(k_dirichlet_norm, grid, dim3(kDirTile), 0, s, dims, samples, g_off, g, out);
     return hipGetLastError();
 }
 

@@ -7,6 +7,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <new>
 #include <mutex>
 #include <vector>
 #include "eigenrand_amd.h"
@@ -44,15 +46,20 @@ int bucket_of(int d) { return ((d + 15) & ~15) <= 64 ? 0 : ((d + 15) & ~15) <= 1
 void build_tiles(const std::vector<MvInstance>& inst, const std::vector<int>& which, int variant,
                  std::vector<MvTile>& tiles) {
     const int rows = lz_rows(variant), cols = lz_cols(variant);
-    for (int i : which) {
+    // Largest instances first (longest k loops), so the tail of the launch is short.
+    std::vector<int> order(which);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst[a].d > inst[b].d; });
+    size_t total = tiles.size();
+    for (int i : order) {
+        const int dp = (inst[i].d + 15) & ~15;
+        total += (size_t)((dp + rows - 1) / rows) * (size_t)((inst[i].n + cols - 1) / cols);
+    }
+    tiles.reserve(total);
+    for (int i : order) {
         const int dp = (inst[i].d + 15) & ~15;
         for (int r = 0; r < dp; r += rows)
             for (int64_t c = 0; c < inst[i].n; c += cols) tiles.push_back(MvTile{i, r, c});
     }
-    // Largest instances first (longest k loops), so the tail of the launch is short.
-    std::stable_sort(tiles.begin(), tiles.end(), [&](const MvTile& a, const MvTile& b) {
-        return inst[a.inst].d + 0 > inst[b.inst].d + 0;
-    });
 }
 
 int grid_for(int variant, size_t n_tiles) {
@@ -195,6 +202,16 @@ int factor_and_split(const std::vector<int32_t>& dims, double* d_a, const std::v
     return ERD_OK;
 }
 
+// Stream-ordered release of MvNormal/Wishart factor planes after the work
+// on `s`.
+int release_factors(erd_mv_factors* f, hipStream_t s) {
+    hipError_t e = hipSuccess;
+    if (f->lhi) e = hipFreeAsync(f->lhi, s);
+    if (f->llo && e == hipSuccess) e = hipFreeAsync(f->llo, s);
+    delete f;
+    return e == hipSuccess ? ERD_OK : ERD_ERR_HIP;
+}
+
 }  // namespace erd
 
 using namespace erd;
@@ -218,29 +235,62 @@ int erd_cholesky_f64_batch(int64_t n_inst, const int32_t* dims, double* a, int32
     return rc;
 }
 
-int erd_mvnormal_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* samples,
-                       const double* cov, const float* mean, float* out, float* z_out, void* stream) {
-    if (!e || !e->states || n_inst <= 0 || n_inst > e->count || !dims || !samples || !cov || !out)
-        return ERD_ERR_INVALID;
+int erd_mv_factors_create(erd_mv_factors** out, int64_t n_inst, const int32_t* dims, const double* a,
+                          int32_t* info, void* stream) {
+    if (!out || n_inst <= 0 || !dims || !a) return ERD_ERR_INVALID;
+    *out = nullptr;
     hipStream_t s = (hipStream_t)stream;
-    std::vector<int32_t> vd(dims, dims + n_inst);
-    std::vector<int64_t> off(n_inst + 1, 0), zoff(n_inst + 1, 0), moff(n_inst + 1, 0);
+    std::unique_ptr<erd_mv_factors> f(new (std::nothrow) erd_mv_factors());
+    if (!f) return ERD_ERR_OOM;
+    f->dims.assign(dims, dims + n_inst);
+    std::vector<int64_t> off(n_inst + 1, 0);
     for (int64_t i = 0; i < n_inst; ++i) {
-        if (vd[i] <= 0 || samples[i] < 0 || ((int64_t)vd[i] * samples[i]) % 8 != 0) return ERD_ERR_INVALID;
-        off[i + 1] = off[i] + (int64_t)vd[i] * vd[i];
-        zoff[i + 1] = zoff[i] + (int64_t)vd[i] * samples[i];
-        moff[i + 1] = moff[i] + vd[i];
+        if (f->dims[i] <= 0) return ERD_ERR_INVALID;
+        off[i + 1] = off[i] + (int64_t)f->dims[i] * f->dims[i];
     }
     Scratch sc(s);
     double* d_a;
     if (sc.alloc(&d_a, off[n_inst]) != hipSuccess) return ERD_ERR_OOM;
-    if (hipMemcpyAsync(d_a, cov, sizeof(double) * off[n_inst], hipMemcpyDeviceToDevice, s) != hipSuccess)
+    if (hipMemcpyAsync(d_a, a, sizeof(double) * off[n_inst], hipMemcpyDeviceToDevice, s) != hipSuccess)
         return ERD_ERR_HIP;
-    uint16_t *lhi, *llo;
     float* l32;
-    std::vector<int64_t> split_off;
-    int rc = factor_and_split(vd, d_a, off, &lhi, &llo, &l32, split_off, nullptr, s, sc);
+    std::vector<int32_t> inf;
+    int rc = factor_and_split(f->dims, d_a, off, &f->lhi, &f->llo, &l32, f->split_off, &inf, s, sc);
+    if (info && !inf.empty()) std::memcpy(info, inf.data(), sizeof(int32_t) * (size_t)n_inst);
     if (rc) return rc;
+    sc.keep(f->lhi);
+    sc.keep(f->llo);
+    *out = f.release();
+    return ERD_OK;
+}
+
+int erd_mv_factors_destroy(erd_mv_factors* f) {
+    if (!f) return ERD_OK;
+    // The planes may still be read by work on any stream: wait for the device.
+    if (hipDeviceSynchronize() != hipSuccess) {
+        delete f;
+        return ERD_ERR_HIP;
+    }
+    const int rc = release_factors(f, nullptr);
+    return rc ? rc : (hipStreamSynchronize(nullptr) == hipSuccess ? ERD_OK : ERD_ERR_HIP);
+}
+
+int64_t erd_mv_factors_count(const erd_mv_factors* f) { return f ? (int64_t)f->dims.size() : 0; }
+
+int erd_mvnormal_generate(erd_engines* e, const erd_mv_factors* f, const int64_t* samples, const float* mean,
+                          float* out, float* z_out, void* stream) {
+    if (!e || !e->states || !f || !samples || !out) return ERD_ERR_INVALID;
+    const int64_t n_inst = (int64_t)f->dims.size();
+    if (n_inst > e->count) return ERD_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    const std::vector<int32_t>& vd = f->dims;
+    std::vector<int64_t> zoff(n_inst + 1, 0), moff(n_inst + 1, 0);
+    for (int64_t i = 0; i < n_inst; ++i) {
+        if (samples[i] < 0 || ((int64_t)vd[i] * samples[i]) % 8 != 0) return ERD_ERR_INVALID;
+        zoff[i + 1] = zoff[i] + (int64_t)vd[i] * samples[i];
+        moff[i + 1] = moff[i] + vd[i];
+    }
+    Scratch sc(s);
     // Z_i: the StdNormalGen fill of d_i x n_i on engine i (MvNormal.h:103-108).
     float* z = z_out;
     if (!z && sc.alloc(&z, zoff[n_inst]) != hipSuccess) return ERD_ERR_OOM;
@@ -254,43 +304,46 @@ int erd_mvnormal_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, cons
         segs.push_back(sg);
         engine_of.push_back((int32_t)i);
     }
-    rc = run_segments(e, segs, engine_of, s);
+    int rc = run_segments(e, segs, engine_of, s);
     if (rc) return rc;
     std::vector<MvInstance> inst(n_inst);
     for (int64_t i = 0; i < n_inst; ++i)
-        inst[i] = MvInstance{vd[i], 0, samples[i], lhi + split_off[i], llo + split_off[i], z + zoff[i],
-                             out + zoff[i], mean ? mean + moff[i] : nullptr};
+        inst[i] = MvInstance{vd[i], 0, samples[i], f->lhi + f->split_off[i], f->llo + f->split_off[i],
+                             z + zoff[i], out + zoff[i], mean ? mean + moff[i] : nullptr};
     MvInstance* d_inst;
     if (sc.upload(&d_inst, inst) != hipSuccess) return ERD_ERR_HIP;
     return run_lz(inst, d_inst, s, sc);
 }
 
-int erd_wishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* df,
-                      const int64_t* samples, const double* scale, float* out, void* stream) {
-    if (!e || !e->states || n_inst <= 0 || n_inst > e->count || !dims || !df || !samples || !scale || !out)
+int erd_mvnormal_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* samples,
+                       const double* cov, const float* mean, float* out, float* z_out, void* stream) {
+    if (!e || !e->states || n_inst <= 0 || n_inst > e->count || !dims || !samples || !cov || !out)
         return ERD_ERR_INVALID;
+    erd_mv_factors* f;
+    int rc = erd_mv_factors_create(&f, n_inst, dims, cov, nullptr, stream);
+    if (rc) return rc;
+    rc = erd_mvnormal_generate(e, f, samples, mean, out, z_out, stream);
+    const int rc2 = release_factors(f, (hipStream_t)stream);
+    return rc ? rc : rc2;
+}
+
+int erd_wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64_t* df, const int64_t* samples,
+                         float* out, void* stream) {
+    if (!e || !e->states || !f || !df || !samples || !out) return ERD_ERR_INVALID;
+    const int64_t n_inst = (int64_t)f->dims.size();
+    if (n_inst > e->count) return ERD_ERR_INVALID;
     hipStream_t s = (hipStream_t)stream;
-    std::vector<int32_t> vd(dims, dims + n_inst);
-    std::vector<int64_t> off(n_inst + 1, 0), aoff(n_inst + 1, 0), nrm_off(n_inst + 1, 0), chi_off(n_inst + 1, 0);
+    const std::vector<int32_t>& vd = f->dims;
+    std::vector<int64_t> aoff(n_inst + 1, 0), nrm_off(n_inst + 1, 0), chi_off(n_inst + 1, 0);
     for (int64_t i = 0; i < n_inst; ++i) {
         const int64_t d = vd[i], n = samples[i];
         // df > d - 1 (MvNormal.h:205); fills need whole packets
-        if (d <= 0 || n < 0 || df[i] <= d - 1 || n % 8 != 0) return ERD_ERR_INVALID;
-        off[i + 1] = off[i] + d * d;
+        if (n < 0 || df[i] <= d - 1 || n % 8 != 0) return ERD_ERR_INVALID;
         aoff[i + 1] = aoff[i] + d * d * n;
         nrm_off[i + 1] = nrm_off[i] + n * d * (d - 1) / 2;
         chi_off[i + 1] = chi_off[i] + d * n;
     }
     Scratch sc(s);
-    double* d_a;
-    if (sc.alloc(&d_a, off[n_inst]) != hipSuccess) return ERD_ERR_OOM;
-    if (hipMemcpyAsync(d_a, scale, sizeof(double) * off[n_inst], hipMemcpyDeviceToDevice, s) != hipSuccess)
-        return ERD_ERR_HIP;
-    uint16_t *lhi, *llo;
-    float* l32;
-    std::vector<int64_t> split_off;
-    int rc = factor_and_split(vd, d_a, off, &lhi, &llo, &l32, split_off, nullptr, s, sc);
-    if (rc) return rc;
     float *nrm, *chi, *amat, *tmat;
     if (sc.alloc(&nrm, nrm_off[n_inst]) || sc.alloc(&chi, chi_off[n_inst]) || sc.alloc(&amat, aoff[n_inst]) ||
         sc.alloc(&tmat, aoff[n_inst]))
@@ -319,12 +372,12 @@ int erd_wishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const
             engine_of.push_back((int32_t)i);
         }
     }
-    rc = run_segments(e, segs, engine_of, s);
+    int rc = run_segments(e, segs, engine_of, s);
     if (rc) return rc;
     std::vector<MvInstance> inst(n_inst);
     for (int64_t i = 0; i < n_inst; ++i)
-        inst[i] = MvInstance{vd[i], 0, (int64_t)vd[i] * samples[i], lhi + split_off[i], llo + split_off[i],
-                             amat + aoff[i], tmat + aoff[i], nullptr};
+        inst[i] = MvInstance{vd[i], 0, (int64_t)vd[i] * samples[i], f->lhi + f->split_off[i],
+                             f->llo + f->split_off[i], amat + aoff[i], tmat + aoff[i], nullptr};
     MvInstance* d_inst;
     int64_t *d_nrm_off, *d_chi_off, *d_aoff;
     if (sc.upload(&d_inst, inst) || sc.upload(&d_nrm_off, nrm_off) || sc.upload(&d_chi_off, chi_off) ||
@@ -345,6 +398,20 @@ int erd_wishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const
     if (sc.upload(&d_tiles, tiles) != hipSuccess) return ERD_ERR_HIP;
     if (launch_syrk_tt(d_tiles, (int)tiles.size(), d_inst, out, d_aoff, s) != hipSuccess) return ERD_ERR_HIP;
     return ERD_OK;
+}
+
+int erd_wishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* df,
+                      const int64_t* samples, const double* scale, float* out, void* stream) {
+    if (!e || !e->states || n_inst <= 0 || n_inst > e->count || !dims || !df || !samples || !scale || !out)
+        return ERD_ERR_INVALID;
+    for (int64_t i = 0; i < n_inst; ++i)
+        if (dims[i] <= 0 || samples[i] < 0 || df[i] <= dims[i] - 1 || samples[i] % 8 != 0) return ERD_ERR_INVALID;
+    erd_mv_factors* f;
+    int rc = erd_mv_factors_create(&f, n_inst, dims, scale, nullptr, stream);
+    if (rc) return rc;
+    rc = erd_wishart_generate(e, f, df, samples, out, stream);
+    const int rc2 = release_factors(f, (hipStream_t)stream);
+    return rc ? rc : rc2;
 }
 
 int erd_dirichlet_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* samples,

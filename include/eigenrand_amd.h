@@ -212,6 +212,28 @@ int erd_cholesky_f64_batch(int64_t n_inst, const int32_t* dims, double* a, int32
 int erd_mvnormal_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* samples,
                        const double* cov, const float* mean, float* out, float* z_out, void* stream);
 
+/* Generator construction and generation as separate steps, as the reference
+ * does (MvNormalGen's and WishartGen's constructors factorise once,
+ * MvNormal.h:20-38, 86-98, 196-215; generate() reuses the factor).
+ * erd_mv_factors_create factorises n_inst covariances (or Wishart scales;
+ * DEVICE doubles as above, dims a HOST array) in FP64 on the device and keeps
+ * the factors; info (HOST, optional, n_inst) receives each instance's
+ * Cholesky status (0 = ok, k > 0: pivot k not positive), and a non-positive
+ * pivot returns ERD_ERR_NOT_PSD. Generation uses instance i on engine i. */
+typedef struct erd_mv_factors erd_mv_factors;
+int erd_mv_factors_create(erd_mv_factors** out, int64_t n_inst, const int32_t* dims, const double* a,
+                          int32_t* info, void* stream);
+int erd_mv_factors_destroy(erd_mv_factors* f);
+int64_t erd_mv_factors_count(const erd_mv_factors* f);
+/* MvNormalGen::generate for every instance of f (arguments as in
+ * erd_mvnormal_batch). */
+int erd_mvnormal_generate(erd_engines* e, const erd_mv_factors* f, const int64_t* samples, const float* mean,
+                          float* out, float* z_out, void* stream);
+/* WishartGen::generate for every instance of f (arguments as in
+ * erd_wishart_batch). */
+int erd_wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64_t* df, const int64_t* samples,
+                         float* out, void* stream);
+
 /* WishartGen<float>(df_i, scale_i).generate(engine_i, samples_i)
  * (MvNormal.h:184-302): out_i is d_i x (d_i*samples_i) column-major (sample j
  * in columns [j*d_i, (j+1)*d_i)). samples_i must be a multiple of 8 and

@@ -189,3 +189,33 @@ def test_multinomial_rejects_nonfinite(er):
     urng = er.Vmt19937_64(1)
     with pytest.raises(er.ErdError):
         er.multinomial_batch(urng, [10], [[1.0, np.inf, 2.0]], [8])
+
+
+def test_factors_once_generate_many(er):
+    """MvNormalBatch / WishartBatch factorise at construction; each generate()
+    equals the one-shot batch call on the same engine states."""
+    rng = np.random.default_rng(77)
+    dims = [5, 24, 130]
+    covs = [rand_cov(rng, d) for d in dims]
+    means = [rng.normal(size=d).astype(np.float32) for d in dims]
+    seeds = [7, 11, 13]
+    gen = er.MvNormalBatch(covs, means)
+    a = er.Vmt19937_64(seeds=seeds)
+    b = er.Vmt19937_64(seeds=seeds)
+    for n in (16, 40):
+        xs = gen.generate(a, n)
+        ys = er.mvnormal_batch(b, covs, n, means)
+        for x, y in zip(xs, ys):
+            assert_same(x.cpu().numpy(), y.cpu().numpy())
+    np.testing.assert_array_equal(a.snapshot(), b.snapshot())
+    wgen = er.WishartBatch(covs[:2], [d + 3 for d in dims[:2]])
+    w1 = wgen.generate(a, 8)
+    w2 = er.wishart_batch(b, covs[:2], [d + 3 for d in dims[:2]], 8)
+    for x, y in zip(w1, w2):
+        assert_same(x.cpu().numpy(), y.cpu().numpy())
+
+
+def test_factors_report_not_psd(er):
+    bad = -np.eye(6)
+    with pytest.raises(er.ErdError):
+        er.MvFactors([np.eye(3), bad])
