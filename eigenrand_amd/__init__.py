@@ -24,7 +24,7 @@ from . import _lib
 from ._lib import ErdError, InvalidArgument
 
 __all__ = [
-    "Vmt19937_64", "P8_mt19937_64", "ErdError", "InvalidArgument", "STATE_BYTES",
+    "Vmt19937_64", "P8_mt19937_64", "Generator", "ErdError", "InvalidArgument", "STATE_BYTES",
     "p8_randbits_u64", "p8_randbits_u32", "p8_uniform_real",
     "randbits_u64", "uniform_real", "uniform_real_f64", "normal", "normal_vop",
     "exponential", "cauchy", "student_t", "gamma", "chi_squared", "poisson", "binomial",
@@ -171,6 +171,42 @@ def p8_randbits_u32(count, urng, out=None, stream=None):
 def p8_uniform_real(count, urng, out=None, stream=None):
     """`count` calls of the adaptor's uniform_real() per adaptor; (n, count) float32."""
     return _p8_fill("erd_p8_fill_uniform_f32", count, urng, _torch().float32, out, stream)
+
+
+class Generator:
+    """A reused generator object on `n_engines` engines: the GPU form of
+    `auto gen = Rand::makeXxxGen(a, b)` followed by several
+    gen.generate(per_engine, 1, engine_g) calls. It keeps the normal
+    generator's cached cos packet and the rejection samplers' CompressMask
+    leftovers between calls (Basic.h:268-342), so consecutive fills equal one
+    larger fill. dist: "uniform", "std_normal", "normal", "exponential",
+    "cauchy", "student_t", "gamma", "poisson" or "binomial"."""
+
+    def __init__(self, dist, a=0.0, b=0.0, n_engines=1, stream=None):
+        self.dist = dist
+        kind = _lib.DIST[dist]
+        h = C.c_void_p()
+        _lib.call("erd_generator_create", C.byref(h), kind, float(a), float(b), int(n_engines),
+                  _stream_ptr(stream))
+        self._h = h
+        self.n = int(n_engines)
+
+    def generate(self, per_engine, urng, out=None, stream=None):
+        """Flat output: engine g's block is out[g*per_engine:(g+1)*per_engine]."""
+        torch = _torch()
+        dtype = torch.int32 if self.dist in ("poisson", "binomial") else torch.float32
+        buf = _alloc(urng, int(per_engine), dtype, out)
+        _lib.call("erd_generator_fill", self._h, urng.handle, _ptr(buf), int(per_engine), _stream_ptr(stream))
+        return buf
+
+    def reset(self, stream=None):
+        _lib.call("erd_generator_reset", self._h, _stream_ptr(stream))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.erd_generator_destroy(h)
+            self._h = None
 
 
 def _alloc(urng, per_engine, dtype, out):

@@ -71,6 +71,10 @@ _SIGS = {
     "erd_fill_binomial_vop_i32": [P, P, I64, P, P, P],
     "erd_fill_gamma_vop_f32": [P, P, I64, P, P, P],
     "erd_fill_poisson_vop_i32": [P, P, I64, P, P],
+    "erd_generator_create": [C.POINTER(P), C.c_int32, F64, F64, I64, P],
+    "erd_generator_destroy": [P],
+    "erd_generator_reset": [P, P],
+    "erd_generator_fill": [P, P, P, I64, P],
     "erd_run_requests": [P, P, I64, P],
     "erd_cholesky_f64_batch": [I64, P, P, P, P],
     "erd_mvnormal_batch": [P, I64, P, P, P, P, P, P, P],

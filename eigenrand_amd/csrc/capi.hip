@@ -251,17 +251,25 @@ int erd_fill_studentt_f32(erd_engines* e, float* out, int64_t per, float n, void
     return hip_status(erd::launch_fill_studentt_f32(e->states, e->count, out, per, n, S(stream)));
 }
 
-int erd_fill_gamma_f32(erd_engines* e, float* out, int64_t per, float alpha, float beta, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// The fills of GammaGen, PoissonGen and BinomialGen; gs (optional) holds the
+// CompressMask queues of a reused generator object (rejection regimes only:
+// the other regimes have no state).
+int fill_gamma(erd_engines* e, float* out, int64_t per, float alpha, float beta, erd::GenState* gs,
+               void* stream) {
     int rc = check_fill(e, out, per, 8);
     if (rc) return rc;
     if (!(alpha > 0.0f) || !std::isfinite(alpha)) return ERD_ERR_INVALID;
     if (alpha == 1.0f || ((float)(int)alpha == alpha && alpha < 20.0f))
         return hip_status(erd::launch_fill_gamma_fixed_f32(e->states, e->count, out, per, alpha, beta, S(stream)));
     return hip_status(erd::launch_fill_gamma_reject_f32(e->states, e->count, out, per, alpha, beta,
-                                                        e->status, S(stream)));
+                                                        e->status, S(stream), gs));
 }
 
-int erd_fill_poisson_i32(erd_engines* e, int32_t* out, int64_t per, double mean, void* stream) {
+int fill_poisson(erd_engines* e, int32_t* out, int64_t per, double mean, erd::GenState* gs, void* stream) {
     int rc = check_fill(e, out, per, 4);
     if (rc) return rc;
     if (!(mean >= 0.0) || !std::isfinite(mean)) return ERD_ERR_INVALID;
@@ -277,11 +285,11 @@ int erd_fill_poisson_i32(erd_engines* e, int32_t* out, int64_t per, double mean,
     pp.log_mean = std::log(mean);
     pp.g1 = mean * pp.log_mean - std::lgamma(mean + 1);
     return hip_status(erd::launch_fill_poisson_reject_i32(e->states, e->count, out, per, pp, e->status,
-                                                          S(stream)));
+                                                          S(stream), gs));
 }
 
-int erd_fill_binomial_i32(erd_engines* e, int32_t* out, int64_t per, int32_t trials, double p,
-                          void* stream) {
+int fill_binomial(erd_engines* e, int32_t* out, int64_t per, int32_t trials, double p, erd::GenState* gs,
+                  void* stream) {
     int rc = check_fill(e, out, per, 4);
     if (rc) return rc;
     if (trials < 0 || !(p >= 0.0 && p <= 1.0)) return ERD_ERR_INVALID;
@@ -304,7 +312,99 @@ int erd_fill_binomial_i32(erd_engines* e, int32_t* out, int64_t per, int32_t tri
     bp.log_small_p = std::log(bp.small_p);
     bp.log_small_q = std::log(1 - bp.small_p);
     return hip_status(erd::launch_fill_binomial_reject_i32(e->states, e->count, out, per, bp, e->status,
-                                                           S(stream)));
+                                                           S(stream), gs));
+}
+
+}  // namespace
+
+extern "C" {
+
+int erd_fill_gamma_f32(erd_engines* e, float* out, int64_t per, float alpha, float beta, void* stream) {
+    return fill_gamma(e, out, per, alpha, beta, nullptr, stream);
+}
+
+int erd_fill_poisson_i32(erd_engines* e, int32_t* out, int64_t per, double mean, void* stream) {
+    return fill_poisson(e, out, per, mean, nullptr, stream);
+}
+
+int erd_fill_binomial_i32(erd_engines* e, int32_t* out, int64_t per, int32_t trials, double p,
+                          void* stream) {
+    return fill_binomial(e, out, per, trials, p, nullptr, stream);
+}
+
+// ------------------------------------------------------ generator objects ---
+struct erd_generator {
+    int32_t dist;
+    double a, b;
+    int64_t n;
+    erd::GenState* state;  // device, one per engine
+};
+
+int erd_generator_create(erd_generator** out, int32_t dist, double a, double b, int64_t n_engines,
+                         void* stream) {
+    if (!out || n_engines <= 0) return ERD_ERR_INVALID;
+    *out = nullptr;
+    switch (dist) {
+        case ERD_DIST_UNIFORM_F32: case ERD_DIST_STD_NORMAL_F32: case ERD_DIST_NORMAL_F32:
+        case ERD_DIST_EXPONENTIAL_F32: case ERD_DIST_CAUCHY_F32: case ERD_DIST_STUDENT_T_F32:
+        case ERD_DIST_GAMMA_F32: case ERD_DIST_POISSON_I32: case ERD_DIST_BINOMIAL_I32: break;
+        default: return ERD_ERR_INVALID;
+    }
+    erd_generator* g = new (std::nothrow) erd_generator{dist, a, b, n_engines, nullptr};
+    if (!g) return ERD_ERR_OOM;
+    hipError_t err = hipMalloc(&g->state, sizeof(erd::GenState) * (size_t)n_engines);
+    if (err == hipSuccess)
+        err = hipMemsetAsync(g->state, 0, sizeof(erd::GenState) * (size_t)n_engines, S(stream));
+    if (err == hipSuccess) err = hipStreamSynchronize(S(stream));
+    if (err != hipSuccess) {
+        if (g->state) (void)hipFree(g->state);
+        delete g;
+        return hip_status(err);
+    }
+    *out = g;
+    return ERD_OK;
+}
+
+int erd_generator_destroy(erd_generator* g) {
+    if (!g) return ERD_OK;
+    hipError_t err = hipDeviceSynchronize();
+    if (g->state) (void)hipFree(g->state);
+    delete g;
+    return hip_status(err);
+}
+
+int erd_generator_reset(erd_generator* g, void* stream) {
+    if (!g) return ERD_ERR_INVALID;
+    return hip_status(hipMemsetAsync(g->state, 0, sizeof(erd::GenState) * (size_t)g->n, S(stream)));
+}
+
+int erd_generator_fill(erd_generator* g, erd_engines* e, void* out, int64_t per, void* stream) {
+    if (!g || !e || e->count != g->n) return ERD_ERR_INVALID;
+    switch (g->dist) {
+        case ERD_DIST_STD_NORMAL_F32:
+        case ERD_DIST_NORMAL_F32: {
+            int rc = check_fill(e, out, per, 8);
+            if (rc) return rc;
+            const int mode = g->dist == ERD_DIST_NORMAL_F32 ? 1 : 0;
+            return hip_status(erd::launch_fill_normal_gen_f32(e->states, e->count, (float*)out, per, mode,
+                                                              (float)g->a, (float)g->b, nullptr, nullptr,
+                                                              g->state, S(stream)));
+        }
+        case ERD_DIST_GAMMA_F32:
+            return fill_gamma(e, (float*)out, per, (float)g->a, (float)g->b, g->state, stream);
+        case ERD_DIST_POISSON_I32:
+            return fill_poisson(e, (int32_t*)out, per, g->a, g->state, stream);
+        case ERD_DIST_BINOMIAL_I32: {
+            const int32_t trials = (int32_t)g->a;
+            if ((double)trials != g->a) return ERD_ERR_INVALID;
+            return fill_binomial(e, (int32_t*)out, per, trials, g->b, g->state, stream);
+        }
+        case ERD_DIST_UNIFORM_F32: return erd_fill_uniform_f32(e, (float*)out, per, (float)g->a, (float)g->b, stream);
+        case ERD_DIST_EXPONENTIAL_F32: return erd_fill_exponential_f32(e, (float*)out, per, (float)g->a, stream);
+        case ERD_DIST_CAUCHY_F32: return erd_fill_cauchy_f32(e, (float*)out, per, (float)g->a, (float)g->b, stream);
+        case ERD_DIST_STUDENT_T_F32: return erd_fill_studentt_f32(e, (float*)out, per, (float)g->a, stream);
+        default: return ERD_ERR_INVALID;
+    }
 }
 
 int erd_fill_binomial_vop_i32(erd_engines* e, int32_t* out, int64_t per, const int32_t* trials,

@@ -39,6 +39,16 @@ struct EngineState {
 };
 static_assert(sizeof(EngineState) == 10016, "engine record layout");
 
+// Cached state of a reused generator object on one engine (Basic.h:268-342):
+// StdNormalGen's pending cos packet (count 8) or a rejection sampler's
+// CompressMask leftovers (count < packet width). count = 0: nothing cached.
+struct GenState {
+    int32_t count;
+    int32_t reserved;
+    float v[8];
+};
+static_assert(sizeof(GenState) == 40, "generator state layout");
+
 // Error bits a kernel can raise (OR-ed into a device status word).
 constexpr uint32_t kStatusIterCap = 1u;
 constexpr uint32_t kStatusBadSegment = 2u;

@@ -94,6 +94,7 @@ struct NormalF32 {
     float mean, sd;
     const float* vmean;
     const float* vsd;
+    GenState* gs;  // reused generator: receives the cos packet of a trailing half unit
     __device__ void rebase(int64_t off) {
         out += off;
         if (MODE == 2) { vmean += off; vsd += off; }
@@ -114,6 +115,10 @@ struct NormalF32 {
         }
         const int64_t base = 16 * u + 2 * l;
         const bool full = 16 * u + 16 <= per;
+        if (!full && gs) {  // the generator keeps the raw cos packet for its next call
+            gs->v[2 * l] = cs[0];
+            gs->v[2 * l + 1] = cs[1];
+        }
         if (MODE == 1) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) { sn[h] = sn[h] * sd + mean; cs[h] = cs[h] * sd + mean; }

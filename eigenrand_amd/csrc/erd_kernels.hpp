@@ -8,6 +8,7 @@ namespace erd {
 
 struct EngineState;
 struct Segment;
+struct GenState;
 
 // PoissonGen constructor constants, computed on the host in double with libm
 // (Discrete.h:807-813).
@@ -47,14 +48,22 @@ hipError_t launch_fill_studentt_f32(EngineState* st, int64_t n, float* out, int6
 hipError_t launch_fill_gamma_fixed_f32(EngineState* st, int64_t n, float* out, int64_t per,
                                        float alpha, float beta, hipStream_t s);
 
-// fill_reject.hip (CompressMask rejection samplers)
+// fill_reject.hip (CompressMask rejection samplers); gs: optional per-engine
+// queues of a reused generator object.
 hipError_t launch_fill_gamma_reject_f32(EngineState* st, int64_t n, float* out, int64_t per,
-                                        float alpha, float beta, uint32_t* status, hipStream_t s);
+                                        float alpha, float beta, uint32_t* status, hipStream_t s,
+                                        GenState* gs = nullptr);
 hipError_t launch_fill_poisson_reject_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
-                                          const PoissonParams& pp, uint32_t* status, hipStream_t s);
+                                          const PoissonParams& pp, uint32_t* status, hipStream_t s,
+                                          GenState* gs = nullptr);
 hipError_t launch_fill_binomial_reject_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
-                                           const BinomialParams& bp, uint32_t* status,
-                                           hipStream_t s);
+                                           const BinomialParams& bp, uint32_t* status, hipStream_t s,
+                                           GenState* gs = nullptr);
+// Reused StdNormalGen / NormalGen (mode 0 / 1) or NormalVGen (mode 2):
+// starts with the cached cos packet of gs and leaves a trailing one in it.
+hipError_t launch_fill_normal_gen_f32(EngineState* st, int64_t n, float* out, int64_t per, int mode,
+                                      float mean, float sd, const float* vmean, const float* vsd,
+                                      GenState* gs, hipStream_t s);
 
 // fill_lockstep.hip (packet-lockstep samplers on half draws)
 hipError_t launch_fill_poisson_knuth_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,

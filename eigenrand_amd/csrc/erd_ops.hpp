@@ -160,9 +160,12 @@ struct RejectLds {
 // (candidate, lane) order and queue item f lands in queued packet f/W; the
 // output index of a packet = bypassing candidates before its emitter + queued
 // packets before it.
+// gs (optional): a reused generator's queue, taken as the initial rest queue
+// and replaced by the leftovers at the end (NULL: a fresh generator whose
+// leftovers are dropped, as the reference's free functions do).
 template <class P>
 __device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t per, const P& pol,
-                          uint32_t* status, int lane) {
+                          uint32_t* status, int lane, GenState* gs = nullptr) {
     constexpr int W = P::W, SW = P::SW, K = P::K;
     constexpr int IPC = W / 2;     // threads per candidate (two lanes each)
     constexpr int CPS = 64 / IPC;  // candidates per wave step
@@ -179,6 +182,12 @@ __device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t p
     }
     bool used = false;
     int64_t full_cnt = 0, fifo = 0, since_emit = 0;
+    if (gs) {
+        const int r = gs->count;  // < W queued values, oldest first
+        if (lane < r) s.rest[lane] = gs->v[lane];
+        fifo = r;
+        __syncthreads();
+    }
     int end_v = -1;
     const int cl = lane / IPC, p = lane % IPC;
     for (;;) {
@@ -202,24 +211,40 @@ __device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t p
 #pragma unroll
                 for (int h = 0; h < 2; ++h) pol.eval(w, h, v[h], a[h]);
             }
-            const uint64_t B0 = __ballot(a[0]), B1 = __ballot(a[1]);
+            // Queue bookkeeping over the candidates that count (`lv`).
             const uint64_t cmask = (IPC == 4 ? 0xFull : 0x3ull) << (cl * IPC);
-            const bool full = live && ((B0 & B1 & cmask) == cmask);
-            const bool q0 = a[0] && !full, q1 = a[1] && !full;
-            const uint64_t A0 = __ballot(q0), A1 = __ballot(q1);
-            const uint64_t FC = __ballot(full && p == 0);
             const uint64_t below_c = lanemask_lt(cl * IPC), below = lanemask_lt(lane);
-            const int64_t fb = full_cnt + __popcll(FC & below_c);
-            const int64_t fstart = fifo + __popcll(A0 & below_c) + __popcll(A1 & below_c);
-            const int64_t f0 = fifo + __popcll(A0 & below) + __popcll(A1 & below);
-            const int64_t f1 = f0 + (q0 ? 1 : 0);
-            const int64_t fifo_end = fifo + __popcll(A0) + __popcll(A1);
-            const int64_t m0 = fifo / W, m_end = fifo_end / W;
-            const int acnt = __popcll(A0 & cmask) + __popcll(A1 & cmask);
-            int64_t emit = -1;
-            if (live && (full || (acnt > 0 && (fstart + acnt) / W > fstart / W))) emit = fb + fstart / W;
-            if (live && !full && emit >= 0 && p == 0) s.tab[fstart / W - m0] = (int32_t)emit;
+            bool full, q0, q1;
+            uint64_t A0, A1, FC;
+            int64_t fb, fstart, f0, f1, fifo_end, m_end, emit;
+            const int64_t m0 = fifo / W;
+            auto plan = [&](bool lv) {
+                const uint64_t B0 = __ballot(lv && a[0]), B1 = __ballot(lv && a[1]);
+                full = lv && ((B0 & B1 & cmask) == cmask);
+                q0 = lv && a[0] && !full;
+                q1 = lv && a[1] && !full;
+                A0 = __ballot(q0);
+                A1 = __ballot(q1);
+                FC = __ballot(full && p == 0);
+                fb = full_cnt + __popcll(FC & below_c);
+                fstart = fifo + __popcll(A0 & below_c) + __popcll(A1 & below_c);
+                f0 = fifo + __popcll(A0 & below) + __popcll(A1 & below);
+                f1 = f0 + (q0 ? 1 : 0);
+                fifo_end = fifo + __popcll(A0) + __popcll(A1);
+                m_end = fifo_end / W;
+                const int acnt = __popcll(A0 & cmask) + __popcll(A1 & cmask);
+                emit = -1;
+                if (lv && (full || (acnt > 0 && (fstart + acnt) / W > fstart / W))) emit = fb + fstart / W;
+            };
+            plan(live);
             const uint64_t LAST = __ballot(emit == T - 1 && p == 0);
+            if (LAST) {
+                // Candidates after the one emitting the last packet are never
+                // drawn: drop them so that the queue ends as the reference's.
+                const int clast = (int)(__ffsll((long long)LAST) - 1) / IPC;
+                plan(live && cl <= clast);
+            }
+            if (!full && emit >= 0 && p == 0) s.tab[fstart / W - m0] = (int32_t)emit;
             __syncthreads();
             if (full && emit < T) {
 #pragma unroll
@@ -275,6 +300,11 @@ __device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t p
         pos = SLOTS;
     }
     __syncthreads();
+    if (gs) {  // leftovers: the fifo % W values of the unfinished queued packet
+        const int r = (int)(fifo % W);
+        if (lane < r) gs->v[lane] = s.rest[lane];
+        if (lane == 0) gs->count = r;
+    }
     if (end_v > nc) used = true;
     int bpos = pos, rem = nc;
     if (end_v >= nc) {

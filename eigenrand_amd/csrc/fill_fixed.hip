@@ -32,6 +32,39 @@ __global__ __launch_bounds__(64) void k_fill_fixed(EngineState* __restrict__ sta
     eng_store(e, states + g, lane);
 }
 
+// Reused normal generator: the cached cos packet (count 8) is emitted first,
+// in the call's affine form; a trailing half unit leaves its cos packet in gs.
+template <int MODE>
+__global__ __launch_bounds__(64) void k_fill_normal_gen(EngineState* __restrict__ states, int64_t per,
+                                                        NormalF32<MODE> d, GenState* __restrict__ gs_all) {
+    __shared__ uint64_t mt[kWords];
+    __shared__ uint64_t carry[kLanes];
+    const int lane = threadIdx.x;
+    const int64_t g = blockIdx.x;
+    if (per < 8) return;
+    GenState* gs = gs_all + g;
+    Eng e;
+    eng_load(e, mt, states + g, lane);
+    d.rebase(g * per);
+    int64_t n = per;
+    if (gs->count == 8) {
+        if (lane < 8) {
+            const float z = gs->v[lane];
+            float x = z;
+            if (MODE == 1) x = z * d.sd + d.mean;
+            if (MODE == 2) x = z * d.vsd[lane] + d.vmean[lane];
+            d.out[lane] = x;
+        }
+        d.rebase(8);
+        n -= 8;
+    }
+    __syncthreads();
+    d.gs = gs;
+    op_fixed(e, carry, n, d, lane);
+    if (lane == 0) gs->count = (n % 16 == 8) ? 8 : 0;
+    eng_store(e, states + g, lane);
+}
+
 // ------------------------------------------------------------- launchers ---
 template <class D>
 static hipError_t launch_fixed(EngineState* states, int64_t n_engines, int64_t per, const D& d,
@@ -81,6 +114,23 @@ hipError_t launch_fill_normal_vop_f32(EngineState* st, int64_t n, float* out, in
                                       const float* mean, const float* sd, hipStream_t s) {
     NormalF32<2> d{out, 0.f, 1.f, mean, sd};
     return launch_fixed(st, n, per, d, s);
+}
+
+hipError_t launch_fill_normal_gen_f32(EngineState* st, int64_t n, float* out, int64_t per, int mode,
+                                      float mean, float sd, const float* vmean, const float* vsd,
+                                      GenState* gs, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (mode == 0) {
+        hipLaunchKernelGGL(k_fill_normal_gen<0>, dim3((unsigned)n), dim3(64), 0, s, st, per,
+                           NormalF32<0>{out, mean, sd, nullptr, nullptr, nullptr}, gs);
+    } else if (mode == 1) {
+        hipLaunchKernelGGL(k_fill_normal_gen<1>, dim3((unsigned)n), dim3(64), 0, s, st, per,
+                           NormalF32<1>{out, mean, sd, nullptr, nullptr, nullptr}, gs);
+    } else {
+        hipLaunchKernelGGL(k_fill_normal_gen<2>, dim3((unsigned)n), dim3(64), 0, s, st, per,
+                           NormalF32<2>{out, 0.f, 1.f, vmean, vsd, nullptr}, gs);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_exponential_f32(EngineState* st, int64_t n, float* out, int64_t per,

@@ -191,6 +191,25 @@ typedef struct erd_request {
 
 int erd_run_requests(erd_engines* e, const erd_request* requests, int64_t n_requests, void* stream);
 
+/* ---------------------------------------------------- generator objects --- */
+/* A reused generator object on n engines: `auto gen = Rand::makeXxxGen(a, b);`
+ * followed by several gen.generate(per_engine, 1, engine_g) calls. The
+ * object keeps, per engine, what the reference's generator keeps between
+ * calls (Basic.h:268-342): StdNormalGen/NormalGen's cached cos packet
+ * (NormalExp.h:64-68, 80) and the CompressMask leftovers of GammaGen,
+ * PoissonGen and BinomialGen in their rejection regimes (NormalExp.h:440,
+ * Discrete.h:793). Consecutive fills therefore equal one larger fill, while
+ * the plain erd_fill_* calls behave like the free functions (a fresh
+ * generator per call). dist: an erd_dist scalar kind with a, b as in
+ * erd_request (uniform, normal, exponential, Cauchy, Student-t, gamma,
+ * Poisson, binomial). Fills must use an engine set of n_engines engines. */
+typedef struct erd_generator erd_generator;
+int erd_generator_create(erd_generator** out, int32_t dist, double a, double b, int64_t n_engines,
+                         void* stream);
+int erd_generator_destroy(erd_generator* g);
+int erd_generator_reset(erd_generator* g, void* stream);  /* empties the caches */
+int erd_generator_fill(erd_generator* g, erd_engines* e, void* out, int64_t per_engine, void* stream);
+
 /* ---------------------------------------------------- multivariate batch --- */
 /* Batched FP64 Cholesky (the factorisation MvNormalGen/WishartGen take from
  * detail::get_lt, MvNormal.h:20-38, done in FP64 on the GPU instead of the

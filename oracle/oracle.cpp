@@ -661,6 +661,59 @@ void orc_fill_binomial_vop_i32(void* states, int64_t n_engines, int32_t* out, in
     });
 }
 
+// Reused generator objects (`auto gen = Rand::makeXxxGen(...)` followed by
+// several gen.generate() calls): per-engine generators that keep the normal
+// generator's cached cos packet and the rejection samplers' CompressMask
+// leftovers between fills (EigenRand/Dists/Basic.h:268-342). dist numbers are
+// erd_dist's: 3 std normal, 4 normal(a, b), 9 gamma(a, b), 10 Poisson(a),
+// 11 binomial(a, b); other kinds have no state and use the plain fills.
+struct GenObjects {
+    int dist;
+    double a, b;
+    std::vector<StdNormal8> normal;
+    std::vector<Gamma8> gamma;
+    std::vector<Poisson4> poisson;
+    std::vector<Binomial4> binomial;
+};
+
+void* orc_gen_create(int dist, double a, double b, int64_t n) {
+    GenObjects* h = new GenObjects{dist, a, b, {}, {}, {}, {}};
+    for (int64_t g = 0; g < n; ++g) {
+        if (dist == 3 || dist == 4) h->normal.emplace_back();
+        if (dist == 9) h->gamma.emplace_back((float)a, (float)b);
+        if (dist == 10) h->poisson.emplace_back(a);
+        if (dist == 11) h->binomial.emplace_back((int32_t)a, b);
+    }
+    return h;
+}
+
+void orc_gen_destroy(void* h) { delete (GenObjects*)h; }
+
+// gen.generate(per_engine, 1, engine_g) for every engine g.
+void orc_gen_fill(void* handle, void* states, int64_t n_engines, void* out, int64_t per_engine, int threads) {
+    GenObjects* h = (GenObjects*)handle;
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        if (h->dist == 3 || h->dist == 4) {
+            float* o = (float*)out + g * per_engine;
+            for (int64_t i = 0; i < per_engine; i += 8) {
+                h->normal[g].op(e[g], o + i);
+                if (h->dist == 4)
+                    for (int j = 0; j < 8; ++j) o[i + j] = o[i + j] * (float)h->b + (float)h->a;
+            }
+        } else if (h->dist == 9) {
+            float* o = (float*)out + g * per_engine;
+            for (int64_t i = 0; i < per_engine; i += 8) h->gamma[g].op(e[g], o + i);
+        } else if (h->dist == 10) {
+            int32_t* o = (int32_t*)out + g * per_engine;
+            for (int64_t i = 0; i < per_engine; i += 4) h->poisson[g].op(e[g], o + i);
+        } else if (h->dist == 11) {
+            int32_t* o = (int32_t*)out + g * per_engine;
+            for (int64_t i = 0; i < per_engine; i += 4) h->binomial[g].op(e[g], o + i);
+        }
+    });
+}
+
 // Elementwise access to the packet math, for pinning it against libm.
 void orc_plog(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = plog(x[i]); }
 void orc_pexp(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = pexp(x[i]); }

@@ -249,3 +249,26 @@ def multinomial(e, trials, weights, samples):
         res.append(out[off:off + k * m].reshape(m, k).T)
         off += k * m
     return res
+
+
+class Generators:
+    """Reused generator objects on n engines (oracle side of erd_generator):
+    dist 3 std normal, 4 normal(a, b), 9 gamma(a, b), 10 Poisson(a),
+    11 binomial(a, b)."""
+
+    def __init__(self, dist, a, b, n):
+        f = lib().orc_gen_create
+        f.restype = C.c_void_p
+        self.h = C.c_void_p(f(C.c_int(dist), C.c_double(a), C.c_double(b), I64(n)))
+        self.dist = dist
+
+    def fill(self, e, per, threads=1):
+        dt = np.int32 if self.dist in (10, 11) else np.float32
+        out = np.empty(len(e) * per, dtype=dt)
+        lib().orc_gen_fill(self.h, _p(e.raw), I64(len(e)), _p(out), I64(per), C.c_int(threads))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.orc_gen_destroy(self.h)
+            self.h = None

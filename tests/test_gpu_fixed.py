@@ -203,3 +203,28 @@ def test_p8_rejects_bad_sizes(er):
     odd = er.Vmt19937_64(1, n=3)
     with pytest.raises(er.InvalidArgument):
         er.p8_randbits_u64(8, odd)
+
+
+GEN_CASES = [
+    ("std_normal", 3, 0.0, 0.0, [8, 24, 40, 8, 16]),
+    ("normal", 4, 2.0, 0.5, [24, 8, 16, 8]),
+    ("gamma", 9, 0.4, 1.5, [8, 16, 40, 8, 1000]),
+    ("gamma", 9, 3.7, 1.0, [16, 8, 24, 2000]),
+    ("poisson", 10, 30.0, 0.0, [4, 12, 20, 4, 2500]),
+    ("poisson", 10, 2.5, 0.0, [4, 8, 12]),
+    ("binomial", 11, 100, 0.3, [4, 12, 8, 1500]),
+]
+
+
+@pytest.mark.parametrize("name,dist,a,b,sizes", GEN_CASES)
+def test_generator_objects_match_oracle(er, orc, name, dist, a, b, sizes):
+    seeds = [3, 77, 2 ** 40 + 5, 16380]
+    urng = er.Vmt19937_64(seeds=seeds)
+    ref = orc.Engines(seeds=seeds)
+    gen = er.Generator(name, a, b, len(seeds))
+    rgen = orc.Generators(dist, a, b, len(seeds))
+    for n in sizes:
+        got = gen.generate(n, urng).cpu().numpy()
+        assert_same(got, rgen.fill(ref, n))
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+    assert urng.status() == 0
