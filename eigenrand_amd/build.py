@@ -48,6 +48,22 @@ def build_library(verbose=False):
     return LIB
 
 
+CXX = os.environ.get("CXX", "g++")
+SHIM_EXAMPLE = os.path.join(ROOT, "tests", "cpp", "shim_example")
+
+
+def build_shim_example(verbose=False):
+    """The C++ front end's example program, linked against the library."""
+    src = SHIM_EXAMPLE + ".cpp"
+    cmd = [CXX, "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), src,
+           "-L", HERE, "-lerd_amd", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../eigenrand_amd",
+           "-o", SHIM_EXAMPLE]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return SHIM_EXAMPLE
+
+
 def build_oracle(verbose=False):
     cmd = ["make", "-s", "-C", os.path.join(ROOT, "oracle")]
     if verbose:
@@ -58,4 +74,5 @@ def build_oracle(verbose=False):
 if __name__ == "__main__":
     build_oracle(verbose=True)
     build_library(verbose=True)
+    build_shim_example(verbose=True)
     print("built", LIB)

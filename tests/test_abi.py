@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 from conftest import ROOT
 
 HEADER = os.path.join(ROOT, "include", "eigenrand_amd.h")
@@ -36,3 +38,21 @@ def test_python_binding_covers_header():
 def test_state_bytes_without_gpu():
     from eigenrand_amd import _lib, STATE_BYTES
     assert _lib.load().erd_engine_state_bytes() == STATE_BYTES
+
+
+def test_cpp_front_end_compiles():
+    """include/eigenrand_amd.hpp builds warning-free with the host compiler."""
+    import shutil
+    import subprocess
+    import tempfile
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("no host C++ compiler")
+    src = "#include \"eigenrand_amd.hpp\"\nint main() { return 0; }\n"
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "t.cpp")
+        with open(path, "w") as f:
+            f.write(src)
+        r = subprocess.run([cxx, "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                            "-I", os.path.join(ROOT, "include"), path], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
