@@ -28,6 +28,7 @@ __all__ = [
     "p8_randbits_u64", "p8_randbits_u32", "p8_uniform_real",
     "randbits_u64", "uniform_real", "uniform_real_f64", "normal", "normal_vop",
     "exponential", "cauchy", "student_t", "gamma", "chi_squared", "poisson", "binomial",
+    "lognormal", "weibull", "extreme_value",
     "binomial_vop", "gamma_vop", "poisson_vop", "run_requests", "cholesky_batch",
     "mvnormal_batch", "wishart_batch", "dirichlet_batch", "MvNormalGen", "WishartGen",
     "MvFactors", "MvNormalBatch", "WishartBatch",
@@ -284,6 +285,23 @@ def exponential(rows, cols, urng, lmbda=1.0, out=None, stream=None):
     """Rand::exponential<MatrixXf>(rows, cols, urng, lambda)."""
     return _fill("erd_fill_exponential_f32", urng, rows, cols, _torch().float32, out, stream,
                  float(lmbda))
+
+
+def lognormal(rows, cols, urng, mean=0.0, stdev=1.0, out=None, stream=None):
+    """Rand::lognormal<MatrixXf>(rows, cols, urng, mean, stdev) (LognormalGen)."""
+    return _fill("erd_fill_lognormal_f32", urng, rows, cols, _torch().float32, out, stream,
+                 float(mean), float(stdev))
+
+
+def weibull(rows, cols, urng, a=1.0, b=1.0, out=None, stream=None):
+    """Rand::weibull<MatrixXf>(rows, cols, urng, a, b) (WeibullGen)."""
+    return _fill("erd_fill_weibull_f32", urng, rows, cols, _torch().float32, out, stream, float(a), float(b))
+
+
+def extreme_value(rows, cols, urng, a=0.0, b=1.0, out=None, stream=None):
+    """Rand::extremeValue<MatrixXf>(rows, cols, urng, a, b) (ExtremeValueGen)."""
+    return _fill("erd_fill_extreme_value_f32", urng, rows, cols, _torch().float32, out, stream,
+                 float(a), float(b))
 
 
 def cauchy(rows, cols, urng, a=0.0, b=1.0, out=None, stream=None):

@@ -63,6 +63,9 @@ _SIGS = {
     "erd_fill_normal_vop_f32": [P, P, I64, P, P, P],
     "erd_fill_exponential_f32": [P, P, I64, F32, P],
     "erd_fill_cauchy_f32": [P, P, I64, F32, F32, P],
+    "erd_fill_lognormal_f32": [P, P, I64, F32, F32, P],
+    "erd_fill_weibull_f32": [P, P, I64, F32, F32, P],
+    "erd_fill_extreme_value_f32": [P, P, I64, F32, F32, P],
     "erd_fill_studentt_f32": [P, P, I64, F32, P],
     "erd_fill_gamma_f32": [P, P, I64, F32, F32, P],
     "erd_fill_chisquared_f32": [P, P, I64, F32, P],
@@ -91,6 +94,7 @@ _SIGS = {
     "erd_server_create": [C.POINTER(P), P],
     "erd_server_destroy": [P],
     "erd_server_submit": [P, P, C.POINTER(U64)],
+    "erd_server_register_mvnormal": [P, U64, C.c_int32, P, P, C.POINTER(U64)],
     "erd_server_wait": [P, U64, F64, C.POINTER(P), C.POINTER(I64)],
     "erd_server_release": [P, U64],
     "erd_server_stats": [P, P],
@@ -154,5 +158,6 @@ class Request(C.Structure):
 DIST = {
     "u64": 0, "uniform": 1, "uniform_f64": 2, "std_normal": 3, "normal": 4, "normal_vop": 5,
     "exponential": 6, "cauchy": 7, "student_t": 8, "gamma": 9, "poisson": 10, "binomial": 11,
-    "binomial_vop": 12, "gamma_vop": 13, "poisson_vop": 14, "discard": 15,
+    "binomial_vop": 12, "gamma_vop": 13, "poisson_vop": 14, "discard": 15, "lognormal": 16,
+    "weibull": 17, "extreme_value": 18,
 }

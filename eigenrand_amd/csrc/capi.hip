@@ -239,6 +239,24 @@ int erd_fill_exponential_f32(erd_engines* e, float* out, int64_t per, float lamb
     return hip_status(erd::launch_fill_exponential_f32(e->states, e->count, out, per, lambda, S(stream)));
 }
 
+int erd_fill_lognormal_f32(erd_engines* e, float* out, int64_t per, float mean, float sd, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_lognormal_f32(e->states, e->count, out, per, mean, sd, S(stream)));
+}
+
+int erd_fill_weibull_f32(erd_engines* e, float* out, int64_t per, float a, float b, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_weibull_f32(e->states, e->count, out, per, a, b, S(stream)));
+}
+
+int erd_fill_extreme_value_f32(erd_engines* e, float* out, int64_t per, float a, float b, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_extreme_value_f32(e->states, e->count, out, per, a, b, S(stream)));
+}
+
 int erd_fill_cauchy_f32(erd_engines* e, float* out, int64_t per, float a, float b, void* stream) {
     int rc = check_fill(e, out, per, 8);
     if (rc) return rc;
@@ -347,7 +365,8 @@ int erd_generator_create(erd_generator** out, int32_t dist, double a, double b, 
     switch (dist) {
         case ERD_DIST_UNIFORM_F32: case ERD_DIST_STD_NORMAL_F32: case ERD_DIST_NORMAL_F32:
         case ERD_DIST_EXPONENTIAL_F32: case ERD_DIST_CAUCHY_F32: case ERD_DIST_STUDENT_T_F32:
-        case ERD_DIST_GAMMA_F32: case ERD_DIST_POISSON_I32: case ERD_DIST_BINOMIAL_I32: break;
+        case ERD_DIST_GAMMA_F32: case ERD_DIST_POISSON_I32: case ERD_DIST_BINOMIAL_I32:
+        case ERD_DIST_LOGNORMAL_F32: case ERD_DIST_WEIBULL_F32: case ERD_DIST_EXTREME_VALUE_F32: break;
         default: return ERD_ERR_INVALID;
     }
     erd_generator* g = new (std::nothrow) erd_generator{dist, a, b, n_engines, nullptr};
@@ -382,10 +401,11 @@ int erd_generator_fill(erd_generator* g, erd_engines* e, void* out, int64_t per,
     if (!g || !e || e->count != g->n) return ERD_ERR_INVALID;
     switch (g->dist) {
         case ERD_DIST_STD_NORMAL_F32:
-        case ERD_DIST_NORMAL_F32: {
+        case ERD_DIST_NORMAL_F32:
+        case ERD_DIST_LOGNORMAL_F32: {
             int rc = check_fill(e, out, per, 8);
             if (rc) return rc;
-            const int mode = g->dist == ERD_DIST_NORMAL_F32 ? 1 : 0;
+            const int mode = g->dist == ERD_DIST_NORMAL_F32 ? 1 : g->dist == ERD_DIST_LOGNORMAL_F32 ? 3 : 0;
             return hip_status(erd::launch_fill_normal_gen_f32(e->states, e->count, (float*)out, per, mode,
                                                               (float)g->a, (float)g->b, nullptr, nullptr,
                                                               g->state, S(stream)));
@@ -403,6 +423,9 @@ int erd_generator_fill(erd_generator* g, erd_engines* e, void* out, int64_t per,
         case ERD_DIST_EXPONENTIAL_F32: return erd_fill_exponential_f32(e, (float*)out, per, (float)g->a, stream);
         case ERD_DIST_CAUCHY_F32: return erd_fill_cauchy_f32(e, (float*)out, per, (float)g->a, (float)g->b, stream);
         case ERD_DIST_STUDENT_T_F32: return erd_fill_studentt_f32(e, (float*)out, per, (float)g->a, stream);
+        case ERD_DIST_WEIBULL_F32: return erd_fill_weibull_f32(e, (float*)out, per, (float)g->a, (float)g->b, stream);
+        case ERD_DIST_EXTREME_VALUE_F32:
+            return erd_fill_extreme_value_f32(e, (float*)out, per, (float)g->a, (float)g->b, stream);
         default: return ERD_ERR_INVALID;
     }
 }
@@ -520,6 +543,18 @@ int make_segment(const erd_request& r, Segment& s) {
             if (r.count > 0 && (!r.in0 || !r.in1)) return ERD_ERR_INVALID;
             break;
         case ERD_DIST_DISCARD: width = 1; break;
+        case ERD_DIST_LOGNORMAL_F32:
+            s.kind = SEG_LOGNORMAL_F32;
+            s.d[0] = (float)r.a; s.d[1] = (float)r.b;
+            break;
+        case ERD_DIST_WEIBULL_F32:
+            s.kind = SEG_WEIBULL_F32;
+            s.d[0] = 1 / (float)r.a; s.d[1] = (float)r.b;
+            break;
+        case ERD_DIST_EXTREME_VALUE_F32:
+            s.kind = SEG_EXTREME_VALUE_F32;
+            s.d[0] = (float)r.a; s.d[1] = (float)r.b;
+            break;
         default: return ERD_ERR_INVALID;
     }
     if (r.count % width != 0) return ERD_ERR_INVALID;

@@ -87,7 +87,8 @@ struct P8Fill {
 // Box-Muller pair of StdNormalGen::packetOp (NormalExp.h:60-82): calls A, B
 // give u1 = 1 - U(A), u2 = U(B); the first packet is r*sin, the cached second
 // packet r*cos. A fill of 8 mod 16 outputs ends with an unused cos packet.
-template <int MODE>  // 0 = StdNormalGen, 1 = NormalGen(mean, sd), 2 = NormalVGen
+template <int MODE>  // 0 = StdNormalGen, 1 = NormalGen(mean, sd), 2 = NormalVGen,
+                     // 3 = LognormalGen(mean, sd) = pexp of NormalGen (NormalExp.h:171-210)
 struct NormalF32 {
     static constexpr int CALLS = 2;
     float* out;
@@ -122,6 +123,12 @@ struct NormalF32 {
         if (MODE == 1) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) { sn[h] = sn[h] * sd + mean; cs[h] = cs[h] * sd + mean; }
+        } else if (MODE == 3) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                sn[h] = pexp(sn[h] * sd + mean);
+                cs[h] = pexp(cs[h] * sd + mean);
+            }
         } else if (MODE == 2) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -131,6 +138,40 @@ struct NormalF32 {
         }
         *reinterpret_cast<float2*>(out + base) = make_float2(sn[0], sn[1]);
         if (full) *reinterpret_cast<float2*>(out + base + 8) = make_float2(cs[0], cs[1]);
+    }
+};
+
+struct WeibullF32 {  // NormalExp.h:649-655: b * exp(log(-log(1 - U)) * (1/a))
+    static constexpr int CALLS = 1;
+    float* out;
+    float inv_a, b;  // 1/a rounded in float, as the reference's pset1(1 / a)
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        float v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float x = 1.0f - ur_f32(h ? hi32(w[0]) : lo32(w[0]));
+            v[h] = pexp(plog(-plog(x)) * inv_a) * b;
+        }
+        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
+    }
+};
+
+struct ExtremeValueF32 {  // NormalExp.h:725-731: a - log(-log(U + eps/8)) * b
+    static constexpr int CALLS = 1;
+    float* out;
+    float a, b;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        float v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float x = nzur_f32(h ? hi32(w[0]) : lo32(w[0]));
+            v[h] = a - plog(-plog(x)) * b;
+        }
+        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
     }
 };
 

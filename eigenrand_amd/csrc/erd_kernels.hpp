@@ -45,6 +45,12 @@ hipError_t launch_fill_cauchy_f32(EngineState* st, int64_t n, float* out, int64_
                                   float b, hipStream_t s);
 hipError_t launch_fill_studentt_f32(EngineState* st, int64_t n, float* out, int64_t per, float nu,
                                     hipStream_t s);
+hipError_t launch_fill_lognormal_f32(EngineState* st, int64_t n, float* out, int64_t per, float mean,
+                                     float sd, hipStream_t s);
+hipError_t launch_fill_weibull_f32(EngineState* st, int64_t n, float* out, int64_t per, float a, float b,
+                                   hipStream_t s);
+hipError_t launch_fill_extreme_value_f32(EngineState* st, int64_t n, float* out, int64_t per, float a,
+                                         float b, hipStream_t s);
 hipError_t launch_fill_gamma_fixed_f32(EngineState* st, int64_t n, float* out, int64_t per,
                                        float alpha, float beta, hipStream_t s);
 
@@ -59,7 +65,8 @@ hipError_t launch_fill_poisson_reject_i32(EngineState* st, int64_t n, int32_t* o
 hipError_t launch_fill_binomial_reject_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
                                            const BinomialParams& bp, uint32_t* status, hipStream_t s,
                                            GenState* gs = nullptr);
-// Reused StdNormalGen / NormalGen (mode 0 / 1) or NormalVGen (mode 2):
+// Reused StdNormalGen / NormalGen (mode 0 / 1), NormalVGen (mode 2) or
+// LognormalGen (mode 3):
 // starts with the cached cos packet of gs and leaves a trailing one in it.
 hipError_t launch_fill_normal_gen_f32(EngineState* st, int64_t n, float* out, int64_t per, int mode,
                                       float mean, float sd, const float* vmean, const float* vsd,

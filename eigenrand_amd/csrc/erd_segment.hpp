@@ -30,6 +30,9 @@ enum SegKind : int32_t {
     SEG_MULTINOMIAL_ROW = 18,    // count=samples, in0=r (int32, updated), d[0]=p, d[1]=stride,
                                  // i0 >= 0: trials of every sample (first row; r is not read)
     SEG_MULTINOMIAL_LAST = 19,   // out[s*stride] = r[s] (i0 >= 0: i0); d[1]=stride
+    SEG_LOGNORMAL_F32 = 20,      // d[0]=mean, d[1]=sd
+    SEG_WEIBULL_F32 = 21,        // d[0]=1/a (float), d[1]=b
+    SEG_EXTREME_VALUE_F32 = 22,  // d[0]=a, d[1]=b
     SEG_KIND_COUNT
 };
 

@@ -52,6 +52,7 @@ __global__ __launch_bounds__(64) void k_fill_normal_gen(EngineState* __restrict_
             const float z = gs->v[lane];
             float x = z;
             if (MODE == 1) x = z * d.sd + d.mean;
+            if (MODE == 3) x = pexp(z * d.sd + d.mean);
             if (MODE == 2) x = z * d.vsd[lane] + d.vmean[lane];
             d.out[lane] = x;
         }
@@ -126,11 +127,32 @@ hipError_t launch_fill_normal_gen_f32(EngineState* st, int64_t n, float* out, in
     } else if (mode == 1) {
         hipLaunchKernelGGL(k_fill_normal_gen<1>, dim3((unsigned)n), dim3(64), 0, s, st, per,
                            NormalF32<1>{out, mean, sd, nullptr, nullptr, nullptr}, gs);
+    } else if (mode == 3) {
+        hipLaunchKernelGGL(k_fill_normal_gen<3>, dim3((unsigned)n), dim3(64), 0, s, st, per,
+                           NormalF32<3>{out, mean, sd, nullptr, nullptr, nullptr}, gs);
     } else {
         hipLaunchKernelGGL(k_fill_normal_gen<2>, dim3((unsigned)n), dim3(64), 0, s, st, per,
                            NormalF32<2>{out, 0.f, 1.f, vmean, vsd, nullptr}, gs);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_fill_lognormal_f32(EngineState* st, int64_t n, float* out, int64_t per, float mean,
+                                     float sd, hipStream_t s) {
+    NormalF32<3> d{out, mean, sd, nullptr, nullptr, nullptr};
+    return launch_fixed(st, n, per, d, s);
+}
+
+hipError_t launch_fill_weibull_f32(EngineState* st, int64_t n, float* out, int64_t per, float a, float b,
+                                   hipStream_t s) {
+    WeibullF32 d{out, 1 / a, b};
+    return launch_fixed(st, n, per, d, s);
+}
+
+hipError_t launch_fill_extreme_value_f32(EngineState* st, int64_t n, float* out, int64_t per, float a,
+                                         float b, hipStream_t s) {
+    ExtremeValueF32 d{out, a, b};
+    return launch_fixed(st, n, per, d, s);
 }
 
 hipError_t launch_fill_exponential_f32(EngineState* st, int64_t n, float* out, int64_t per,

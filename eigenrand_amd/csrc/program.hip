@@ -194,6 +194,21 @@ __device__ void run_segment(Eng& e, ProgramLds& s, const Segment& sg, uint32_t* 
             }
             break;
         }
+        case SEG_LOGNORMAL_F32: {
+            NormalF32<3> d{(float*)sg.out, (float)sg.d[0], (float)sg.d[1], nullptr, nullptr, nullptr};
+            op_fixed(e, s.carry_fixed, n, d, lane);
+            break;
+        }
+        case SEG_WEIBULL_F32: {
+            WeibullF32 d{(float*)sg.out, (float)sg.d[0], (float)sg.d[1]};
+            op_fixed(e, s.carry_fixed, n, d, lane);
+            break;
+        }
+        case SEG_EXTREME_VALUE_F32: {
+            ExtremeValueF32 d{(float*)sg.out, (float)sg.d[0], (float)sg.d[1]};
+            op_fixed(e, s.carry_fixed, n, d, lane);
+            break;
+        }
         case SEG_MULTINOMIAL_ROW:
             op_binomial_row8(e, s.vop.carry, s.vop.res, (int32_t*)sg.out, (int64_t)sg.d[1], (int32_t*)sg.in0, sg.i0,
                              n, (float)sg.d[0], status, lane);

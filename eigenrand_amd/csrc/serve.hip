@@ -108,14 +108,28 @@ int64_t chunk_for(const erd_serve_request& r) {
     return std::max<int64_t>(1, (int64_t)(kEngineWork / std::max(cost, 1e-3)));
 }
 
+// An MvNormal model of a client: its covariance (and mean), factorised in
+// FP64 on its first request and kept as bf16 hi/lo planes. Registered models
+// are named by id; a request that passes the matrix itself uses the model of
+// the same (client, contents).
+struct Model {
+    uint64_t client = 0;
+    int d = 0;
+    std::vector<double> cov;
+    std::vector<float> mean;
+    uint16_t* hi = nullptr;  // device planes, set by the launcher thread
+    uint16_t* lo = nullptr;
+};
+
 struct Req;
 void spread(const erd_serve_request& r, int E, int width, Req* q);
 
 struct Req {
     uint64_t ticket = 0;
     erd_serve_request r{};
-    std::vector<double> param;  // MvNormal cov / multinomial weights
-    std::vector<float> mean;
+    std::vector<double> param;  // multinomial weights
+    std::vector<float> mean;    // MvNormal mean given with the request
+    Model* model = nullptr;     // MvNormal model
     int e_req = 1;              // engines the request is spread over
     int eng0 = 0;               // first of them (client-local, wraps modulo E)
     int64_t per = 0;            // outputs (or columns) per engine
@@ -127,13 +141,6 @@ struct Req {
     int status = ERD_OK;
     int slot = -1;
     bool seed_first = false;    // first request of its client: seeds the engines
-};
-
-// Device factorisation cached per (client, covariance).
-struct Factor {
-    int d = 0;
-    uint16_t* hi = nullptr;
-    uint16_t* lo = nullptr;
 };
 
 struct Batch {
@@ -174,7 +181,10 @@ struct Server {
     std::vector<void*> pin;        // per stream: pinned staging buffer
     std::vector<size_t> pin_cap;
     std::deque<Batch*> inflight;
-    std::map<std::pair<uint64_t, uint64_t>, Factor> factors;
+    std::unordered_map<uint64_t, std::unique_ptr<Model>> models;     // id -> model
+    std::map<std::pair<uint64_t, uint64_t>, uint64_t> model_by_hash;  // (client, contents) -> id
+    uint64_t next_model = 1;
+    double prep_s = 0;  // launcher time spent building and launching batches
     int64_t budget_used = 0;
     uint64_t next_ticket = 1;
     bool stop = false;
@@ -288,14 +298,12 @@ int Server::prepare_and_launch(Batch* b) {
         const erd_serve_request& r = q->r;
         if (hipMallocAsync(&q->out, q->out_bytes, s) != hipSuccess) return ERD_ERR_OOM;
         if (r.kind == ERD_SERVE_MVNORMAL) {
-            const int d = r.dim;
-            const uint64_t key = hash_params(q->param);
-            auto fk = std::make_pair(r.client, key);
-            auto it = factors.find(fk);
-            if (it == factors.end()) {
+            Model& m = *q->model;
+            const int d = m.d;
+            if (!m.hi) {  // first request of the model: FP64 factorisation
                 double* a;
                 if (sc.alloc(&a, (size_t)d * d) != hipSuccess) return ERD_ERR_OOM;
-                if (hipMemcpyAsync(a, q->param.data(), sizeof(double) * d * d, hipMemcpyHostToDevice, s) !=
+                if (hipMemcpyAsync(a, m.cov.data(), sizeof(double) * d * d, hipMemcpyHostToDevice, s) !=
                         hipSuccess ||
                     hipStreamSynchronize(s) != hipSuccess)
                     return ERD_ERR_HIP;
@@ -304,22 +312,21 @@ int Server::prepare_and_launch(Batch* b) {
                 std::vector<int64_t> so;
                 int rc = factor_and_split({d}, a, {0, (int64_t)d * d}, &hi, &lo, &l32, so, nullptr, s, sc);
                 if (rc) return rc;
-                // keep the bf16 planes for later requests of this client
-                Factor f;
-                f.d = d;
                 const int dp = (d + 15) & ~15;
-                if (hipMalloc(&f.hi, sizeof(uint16_t) * dp * dp) != hipSuccess) return ERD_ERR_OOM;
-                if (hipMalloc(&f.lo, sizeof(uint16_t) * dp * dp) != hipSuccess) {
-                    (void)hipFree(f.hi);
+                uint16_t *fhi = nullptr, *flo = nullptr;
+                if (hipMalloc(&fhi, sizeof(uint16_t) * dp * dp) != hipSuccess) return ERD_ERR_OOM;
+                if (hipMalloc(&flo, sizeof(uint16_t) * dp * dp) != hipSuccess) {
+                    (void)hipFree(fhi);
                     return ERD_ERR_OOM;
                 }
-                if (hipMemcpyAsync(f.hi, hi, sizeof(uint16_t) * dp * dp, hipMemcpyDeviceToDevice, s) != hipSuccess ||
-                    hipMemcpyAsync(f.lo, lo, sizeof(uint16_t) * dp * dp, hipMemcpyDeviceToDevice, s) != hipSuccess) {
-                    (void)hipFree(f.hi);
-                    (void)hipFree(f.lo);
+                if (hipMemcpyAsync(fhi, hi, sizeof(uint16_t) * dp * dp, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                    hipMemcpyAsync(flo, lo, sizeof(uint16_t) * dp * dp, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+                    (void)hipFree(fhi);
+                    (void)hipFree(flo);
                     return ERD_ERR_HIP;
                 }
-                it = factors.emplace(fk, f).first;
+                m.hi = fhi;
+                m.lo = flo;
             }
             float* z;
             if (sc.alloc(&z, (size_t)q->out_elems) != hipSuccess) return ERD_ERR_OOM;
@@ -331,9 +338,9 @@ int Server::prepare_and_launch(Batch* b) {
                 segs.push_back(sg);
                 seg_engine.push_back(engine_of(q, g));
             }
-            if (!q->mean.empty()) relocs.push_back({mv.size(), 2, st.put(q->mean)});
-            mv.push_back(MvInstance{d, 0, q->per * q->e_req, it->second.hi, it->second.lo, z, (float*)q->out,
-                                    nullptr});
+            const std::vector<float>& mean = q->mean.empty() ? m.mean : q->mean;
+            if (!mean.empty()) relocs.push_back({mv.size(), 2, st.put(mean)});
+            mv.push_back(MvInstance{d, 0, q->per * q->e_req, m.hi, m.lo, z, (float*)q->out, nullptr});
         } else if (r.kind == ERD_SERVE_MULTINOMIAL) {
             const int64_t K = r.dim;
             const int32_t T = (int32_t)r.a;
@@ -554,12 +561,15 @@ void Server::launcher_loop() {
                 cv.wait(lk);
             }
         }
+        const double t_prep = now_s();
         b->status = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming) == hipSuccess ? prepare_and_launch(b)
                                                                                       : ERD_ERR_HIP;
+        const double dt_prep = now_s() - t_prep;
         {
             std::lock_guard<std::mutex> lk(mu);
             inflight.push_back(b);
             ++batches;
+            prep_s += dt_prep;
         }
         cv.notify_all();
     }
@@ -660,6 +670,35 @@ int erd_server_create(erd_server** out, const erd_server_config* cfg) {
     return ERD_OK;
 }
 
+int erd_server_register_mvnormal(erd_server* srv, uint64_t client, int32_t d, const double* cov,
+                                 const float* mean, uint64_t* model) {
+    if (!srv || !cov || !model || d <= 0) return ERD_ERR_INVALID;
+    Server& S = srv->s;
+    std::unique_ptr<erd::Model> m(new erd::Model());
+    m->client = client;
+    m->d = d;
+    m->cov.assign(cov, cov + (size_t)d * d);
+    for (double v : m->cov)
+        if (!std::isfinite(v)) return ERD_ERR_NONFINITE;
+    if (mean) m->mean.assign(mean, mean + d);
+    // A matrix passed without a mean is looked up by contents, so that
+    // repeated requests with the same covariance share one factorisation.
+    const uint64_t key = mean ? 0 : erd::hash_params(m->cov) ^ (uint64_t)d;
+    std::lock_guard<std::mutex> lk(S.mu);
+    if (!mean) {
+        auto it = S.model_by_hash.find(std::make_pair(client, key));
+        if (it != S.model_by_hash.end()) {
+            *model = it->second;
+            return ERD_OK;
+        }
+    }
+    const uint64_t id = S.next_model++;
+    S.models[id] = std::move(m);
+    if (!mean) S.model_by_hash[std::make_pair(client, key)] = id;
+    *model = id;
+    return ERD_OK;
+}
+
 int erd_server_submit(erd_server* srv, const erd_serve_request* r, uint64_t* ticket) {
     if (!srv || !r || !ticket || r->count <= 0) return ERD_ERR_INVALID;
     Server& S = srv->s;
@@ -667,11 +706,26 @@ int erd_server_submit(erd_server* srv, const erd_serve_request* r, uint64_t* tic
     q->r = *r;
     const int E = S.cfg.engines_per_client;
     if (r->kind == ERD_SERVE_MVNORMAL) {
-        if (r->dim <= 0 || !r->host_param) return ERD_ERR_INVALID;
-        q->param.assign(r->host_param, r->host_param + (size_t)r->dim * r->dim);
-        if (r->host_mean) q->mean.assign(r->host_mean, r->host_mean + r->dim);
-        erd::spread(*r, E, 8, q.get());
-        q->out_elems = (int64_t)r->dim * q->per * q->e_req;
+        int d = r->dim;
+        if (r->model) {
+            std::lock_guard<std::mutex> lk(S.mu);
+            auto it = S.models.find(r->model);
+            if (it == S.models.end() || it->second->client != r->client) return ERD_ERR_INVALID;
+            q->model = it->second.get();
+            if (d == 0) d = q->model->d;
+            if (d != q->model->d) return ERD_ERR_INVALID;
+        } else {
+            if (d <= 0 || !r->host_param) return ERD_ERR_INVALID;
+            uint64_t id = 0;
+            int rc = erd_server_register_mvnormal(srv, r->client, d, r->host_param, nullptr, &id);
+            if (rc) return rc;
+            std::lock_guard<std::mutex> lk(S.mu);
+            q->model = S.models[id].get();
+        }
+        q->r.dim = d;
+        if (r->host_mean) q->mean.assign(r->host_mean, r->host_mean + d);
+        erd::spread(q->r, E, 8, q.get());
+        q->out_elems = (int64_t)d * q->per * q->e_req;
         q->out_bytes = sizeof(float) * (size_t)q->out_elems;
     } else if (r->kind == ERD_SERVE_MULTINOMIAL) {
         if (r->dim <= 0 || !r->host_param || r->a < 0) return ERD_ERR_INVALID;
@@ -687,8 +741,10 @@ int erd_server_submit(erd_server* srv, const erd_serve_request* r, uint64_t* tic
         q->out_elems = (int64_t)r->dim * q->per * q->e_req;
         q->out_bytes = sizeof(int32_t) * (size_t)q->out_elems;
     } else {
-        if (r->kind < 0 || r->kind > ERD_DIST_BINOMIAL_I32 || r->kind == ERD_DIST_NORMAL_VOP_F32)
-            return ERD_ERR_INVALID;  // per-element parameter kinds are not served
+        const bool scalar = (r->kind >= 0 && r->kind <= ERD_DIST_BINOMIAL_I32 && r->kind != ERD_DIST_NORMAL_VOP_F32) ||
+                            r->kind == ERD_DIST_LOGNORMAL_F32 || r->kind == ERD_DIST_WEIBULL_F32 ||
+                            r->kind == ERD_DIST_EXTREME_VALUE_F32;
+        if (!scalar) return ERD_ERR_INVALID;  // per-element parameter kinds are not served
         erd_request probe{};
         probe.dist = r->kind;
         probe.count = erd::packet_width(r->kind);
@@ -775,6 +831,7 @@ int erd_server_stats(erd_server* srv, erd_server_stats_t* st) {
     st->max_ms = l.empty() ? 0.0 : l.back() * 1e3;
     st->span_s = S.t_last_done > S.t_first_submit ? S.t_last_done - S.t_first_submit : 0.0;
     st->budget_in_use = S.budget_used;
+    st->prep_s = S.prep_s;
     return ERD_OK;
 }
 
@@ -785,6 +842,7 @@ int erd_server_reset_stats(erd_server* srv) {
     S.latencies.clear();
     S.done_requests = S.done_elems = S.batches = 0;
     S.t_first_submit = S.t_last_done = 0;
+    S.prep_s = 0;
     return ERD_OK;
 }
 
@@ -816,9 +874,9 @@ int erd_server_destroy(erd_server* srv) {
     for (auto& kv : S.reqs)
         if (kv.second->out) (void)hipFreeAsync(kv.second->out, nullptr);
     (void)hipDeviceSynchronize();
-    for (auto& kv : S.factors) {
-        (void)hipFree(kv.second.hi);
-        (void)hipFree(kv.second.lo);
+    for (auto& kv : S.models) {
+        if (kv.second->hi) (void)hipFree(kv.second->hi);
+        if (kv.second->lo) (void)hipFree(kv.second->lo);
     }
     for (hipStream_t st : S.streams) (void)hipStreamDestroy(st);
     for (void* p : S.pin)

@@ -30,13 +30,13 @@ class ServerConfig(C.Structure):
 class ServeRequest(C.Structure):
     _fields_ = [("client", C.c_uint64), ("kind", C.c_int32), ("dim", C.c_int32),
                 ("count", C.c_int64), ("a", C.c_double), ("b", C.c_double),
-                ("host_param", C.c_void_p), ("host_mean", C.c_void_p)]
+                ("host_param", C.c_void_p), ("host_mean", C.c_void_p), ("model", C.c_uint64)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("completed", C.c_int64), ("elements", C.c_int64), ("batches", C.c_int64),
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
-                ("span_s", C.c_double), ("budget_in_use", C.c_int64)]
+                ("span_s", C.c_double), ("budget_in_use", C.c_int64), ("prep_s", C.c_double)]
 
 
 def splitmix64(x):
@@ -125,16 +125,31 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    def submit(self, client, kind, count, a=0.0, b=0.0, cov=None, mean=None, weights=None, dim=0):
+    def register_mvnormal(self, client, cov, mean=None):
+        """Registers a client's MvNormal model; returns its id for submit(model=...)."""
+        c = np.asfortranarray(np.asarray(cov, dtype=np.float64))
+        m = None if mean is None else np.ascontiguousarray(np.asarray(mean, dtype=np.float32))
+        mid = C.c_uint64()
+        _lib.call("erd_server_register_mvnormal", self._h, C.c_uint64(int(client) & ((1 << 64) - 1)),
+                  int(c.shape[0]), c.ctypes.data_as(C.c_void_p),
+                  None if m is None else m.ctypes.data_as(C.c_void_p), C.byref(mid))
+        return mid.value
+
+    def submit(self, client, kind, count, a=0.0, b=0.0, cov=None, mean=None, weights=None, dim=0, model=0):
         """Queue one request; returns its ticket. `kind` is a DIST name or
-        number (scalar-parameter kinds), MVNORMAL (cov d x d, optional mean)
-        or MULTINOMIAL (weights K, trials a)."""
+        number (scalar-parameter kinds), MVNORMAL (a registered `model`, or
+        cov d x d, optional mean) or MULTINOMIAL (weights K, trials a)."""
         if isinstance(kind, str):
             kind = _lib.DIST[kind]
         req = ServeRequest(int(client) & ((1 << 64) - 1), int(kind), int(dim), int(count),
-                           float(a), float(b), None, None)
+                           float(a), float(b), None, None, int(model))
         keep = []
-        if kind == MVNORMAL:
+        if kind == MVNORMAL and model:
+            if mean is not None:
+                m = np.ascontiguousarray(np.asarray(mean, dtype=np.float32))
+                keep.append(m)
+                req.host_mean = m.ctypes.data
+        elif kind == MVNORMAL:
             c = np.asfortranarray(np.asarray(cov, dtype=np.float64))
             req.dim = c.shape[0]
             keep.append(c)

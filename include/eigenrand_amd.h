@@ -116,6 +116,15 @@ int erd_fill_normal_f32(erd_engines* e, float* out, int64_t per_engine, float me
  * free function 1294-1335). mean/stdev are device arrays of n*per_engine. */
 int erd_fill_normal_vop_f32(erd_engines* e, float* out, int64_t per_engine, const float* mean,
                             const float* stdev, void* stream);
+/* lognormal<float>(…, mean, stdev): LognormalGen (NormalExp.h:171-210), the
+ * exp of NormalGen's packets (same cos-packet caching). */
+int erd_fill_lognormal_f32(erd_engines* e, float* out, int64_t per_engine, float mean, float stdev,
+                           void* stream);
+/* weibull<float>(…, a, b): WeibullGen (NormalExp.h:614-656). */
+int erd_fill_weibull_f32(erd_engines* e, float* out, int64_t per_engine, float a, float b, void* stream);
+/* extremeValue<float>(…, a, b): ExtremeValueGen (NormalExp.h:690-732). */
+int erd_fill_extreme_value_f32(erd_engines* e, float* out, int64_t per_engine, float a, float b,
+                               void* stream);
 /* exponential<float>(…, lambda): ExponentialGen (NormalExp.h:356-399). */
 int erd_fill_exponential_f32(erd_engines* e, float* out, int64_t per_engine, float lambda,
                              void* stream);
@@ -176,7 +185,10 @@ typedef enum erd_dist {
     ERD_DIST_BINOMIAL_VOP_I32 = 12, /* in0=trials[] (int32), in1=p[] (float) */
     ERD_DIST_GAMMA_VOP_F32 = 13,    /* in0=alpha[], in1=beta[]               */
     ERD_DIST_POISSON_VOP_I32 = 14,  /* in0=mean[] (float)                    */
-    ERD_DIST_DISCARD = 15           /* count = draws skipped, out unused      */
+    ERD_DIST_DISCARD = 15,          /* count = draws skipped, out unused      */
+    ERD_DIST_LOGNORMAL_F32 = 16,    /* a=mean, b=stdev of the log             */
+    ERD_DIST_WEIBULL_F32 = 17,      /* a=shape, b=scale                       */
+    ERD_DIST_EXTREME_VALUE_F32 = 18 /* a=location, b=scale                    */
 } erd_dist;
 
 typedef struct erd_request {
@@ -201,8 +213,9 @@ int erd_run_requests(erd_engines* e, const erd_request* requests, int64_t n_requ
  * Discrete.h:793). Consecutive fills therefore equal one larger fill, while
  * the plain erd_fill_* calls behave like the free functions (a fresh
  * generator per call). dist: an erd_dist scalar kind with a, b as in
- * erd_request (uniform, normal, exponential, Cauchy, Student-t, gamma,
- * Poisson, binomial). Fills must use an engine set of n_engines engines. */
+ * erd_request (uniform, normal, lognormal, exponential, Cauchy, Student-t,
+ * Weibull, extreme value, gamma, Poisson, binomial). Fills must use an
+ * engine set of n_engines engines. */
 typedef struct erd_generator erd_generator;
 int erd_generator_create(erd_generator** out, int32_t dist, double a, double b, int64_t n_engines,
                          void* stream);
@@ -321,21 +334,29 @@ typedef struct erd_server_config {
 typedef struct erd_serve_request {
     uint64_t client;
     int32_t kind;                /* erd_dist (scalar-parameter kinds) or ERD_SERVE_* */
-    int32_t dim;
+    int32_t dim;                 /* MvNormal: d (0 with a model: the model's)  */
     int64_t count;               /* outputs, or samples for ERD_SERVE_*       */
     double a, b;                 /* scalar parameters as in erd_request       */
     const double* host_param;    /* copied at submit                          */
-    const float* host_mean;
+    const float* host_mean;      /* MvNormal: overrides the model's mean      */
+    uint64_t model;              /* MvNormal: a registered model, or 0 to pass
+                                    the covariance in host_param               */
 } erd_serve_request;
 
 typedef struct erd_server_stats {
     int64_t completed, elements, batches;
     double p50_ms, p99_ms, max_ms, span_s;
     int64_t budget_in_use;
+    double prep_s;               /* launcher time spent building and launching batches */
 } erd_server_stats_t;
 
 int erd_server_create(erd_server** out, const erd_server_config* cfg);
 int erd_server_destroy(erd_server* srv);
+/* Registers a client's MvNormal model (HOST covariance d x d column-major,
+ * optional HOST mean); requests name it by *model. Its FP64 factorisation runs
+ * once, on its first request (MvNormalGen's constructor, MvNormal.h:86-98). */
+int erd_server_register_mvnormal(erd_server* srv, uint64_t client, int32_t d, const double* cov,
+                                 const float* mean, uint64_t* model);
 int erd_server_submit(erd_server* srv, const erd_serve_request* req, uint64_t* ticket);
 /* Waits (timeout_s < 0: forever); returns the request's status and its
  * device output (out_elems elements: e blocks of the padded per-engine size;

@@ -202,6 +202,24 @@ inline DeviceMatrix<float> normal(int64_t rows, int64_t cols, Vmt19937_64& urng,
         return erd_fill_normal_f32(urng.handle(), o, n, mean, stdev, stream);
     }, "erd_fill_normal_f32");
 }
+inline DeviceMatrix<float> lognormal(int64_t rows, int64_t cols, Vmt19937_64& urng, float mean = 0.f,
+                                     float stdev = 1.f, void* stream = nullptr) {
+    return detail::fill<float>(rows, cols, urng.size(), [&](float* o, int64_t n) {
+        return erd_fill_lognormal_f32(urng.handle(), o, n, mean, stdev, stream);
+    }, "erd_fill_lognormal_f32");
+}
+inline DeviceMatrix<float> weibull(int64_t rows, int64_t cols, Vmt19937_64& urng, float a = 1.f, float b = 1.f,
+                                   void* stream = nullptr) {
+    return detail::fill<float>(rows, cols, urng.size(), [&](float* o, int64_t n) {
+        return erd_fill_weibull_f32(urng.handle(), o, n, a, b, stream);
+    }, "erd_fill_weibull_f32");
+}
+inline DeviceMatrix<float> extremeValue(int64_t rows, int64_t cols, Vmt19937_64& urng, float a = 0.f,
+                                        float b = 1.f, void* stream = nullptr) {
+    return detail::fill<float>(rows, cols, urng.size(), [&](float* o, int64_t n) {
+        return erd_fill_extreme_value_f32(urng.handle(), o, n, a, b, stream);
+    }, "erd_fill_extreme_value_f32");
+}
 inline DeviceMatrix<float> exponential(int64_t rows, int64_t cols, Vmt19937_64& urng, float lambda = 1.f,
                                        void* stream = nullptr) {
     return detail::fill<float>(rows, cols, urng.size(), [&](float* o, int64_t n) {
@@ -289,6 +307,9 @@ struct StdNormalGen : GenBase<ERD_DIST_STD_NORMAL_F32, float> {
 };
 struct NormalGen : GenBase<ERD_DIST_NORMAL_F32, float> {
     NormalGen(float mean, float stdev, int64_t engines = 1) : GenBase(mean, stdev, engines) {}
+};
+struct LognormalGen : GenBase<ERD_DIST_LOGNORMAL_F32, float> {
+    LognormalGen(float mean, float stdev, int64_t engines = 1) : GenBase(mean, stdev, engines) {}
 };
 struct GammaGen : GenBase<ERD_DIST_GAMMA_F32, float> {
     GammaGen(float alpha, float beta, int64_t engines = 1) : GenBase(alpha, beta, engines) {}

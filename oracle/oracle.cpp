@@ -579,6 +579,49 @@ void orc_fill_exponential_f32(void* states, int64_t n_engines, float* out, int64
 }
 
 // CauchyGen (NormalExp.h:843-855): a + b*tan(pi*(u-0.5)) via psincos.
+// LognormalGen (NormalExp.h:171-210): pexp of NormalGen's packets.
+void orc_fill_lognormal_f32(void* states, int64_t n_engines, float* out, int64_t per_engine,
+                            float mean, float sd, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        float* o = out + g * per_engine;
+        StdNormal8 gen;
+        for (int64_t i = 0; i < per_engine; i += 8) {
+            gen.op(e[g], o + i);
+            for (int j = 0; j < 8; ++j) o[i + j] = pexp(o[i + j] * sd + mean);
+        }
+    });
+}
+
+// WeibullGen (NormalExp.h:649-655): b * exp(log(-log(1 - U)) * (1/a)).
+void orc_fill_weibull_f32(void* states, int64_t n_engines, float* out, int64_t per_engine,
+                          float a, float b, int threads) {
+    Vmt* e = (Vmt*)states;
+    const float inv_a = 1 / a;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        float* o = out + g * per_engine;
+        float u[8];
+        for (int64_t i = 0; i < per_engine; i += 8) {
+            ur8(e[g], u);
+            for (int j = 0; j < 8; ++j) o[i + j] = pexp(plog(-plog(1.0f - u[j])) * inv_a) * b;
+        }
+    });
+}
+
+// ExtremeValueGen (NormalExp.h:725-731): a - log(-log(U + eps/8)) * b.
+void orc_fill_extreme_value_f32(void* states, int64_t n_engines, float* out, int64_t per_engine,
+                                float a, float b, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        float* o = out + g * per_engine;
+        float u[8];
+        for (int64_t i = 0; i < per_engine; i += 8) {
+            nzur8(e[g], u);
+            for (int j = 0; j < 8; ++j) o[i + j] = a - plog(-plog(u[j])) * b;
+        }
+    });
+}
+
 void orc_fill_cauchy_f32(void* states, int64_t n_engines, float* out, int64_t per_engine,
                          float a, float b, int threads) {
     Vmt* e = (Vmt*)states;
@@ -666,7 +709,8 @@ void orc_fill_binomial_vop_i32(void* states, int64_t n_engines, int32_t* out, in
 // generator's cached cos packet and the rejection samplers' CompressMask
 // leftovers between fills (EigenRand/Dists/Basic.h:268-342). dist numbers are
 // erd_dist's: 3 std normal, 4 normal(a, b), 9 gamma(a, b), 10 Poisson(a),
-// 11 binomial(a, b); other kinds have no state and use the plain fills.
+// 11 binomial(a, b), 16 lognormal(a, b); other kinds have no state and use
+// the plain fills.
 struct GenObjects {
     int dist;
     double a, b;
@@ -679,7 +723,7 @@ struct GenObjects {
 void* orc_gen_create(int dist, double a, double b, int64_t n) {
     GenObjects* h = new GenObjects{dist, a, b, {}, {}, {}, {}};
     for (int64_t g = 0; g < n; ++g) {
-        if (dist == 3 || dist == 4) h->normal.emplace_back();
+        if (dist == 3 || dist == 4 || dist == 16) h->normal.emplace_back();
         if (dist == 9) h->gamma.emplace_back((float)a, (float)b);
         if (dist == 10) h->poisson.emplace_back(a);
         if (dist == 11) h->binomial.emplace_back((int32_t)a, b);
@@ -694,12 +738,14 @@ void orc_gen_fill(void* handle, void* states, int64_t n_engines, void* out, int6
     GenObjects* h = (GenObjects*)handle;
     Vmt* e = (Vmt*)states;
     for_engines(n_engines, threads, [&](int64_t g) {
-        if (h->dist == 3 || h->dist == 4) {
+        if (h->dist == 3 || h->dist == 4 || h->dist == 16) {
             float* o = (float*)out + g * per_engine;
             for (int64_t i = 0; i < per_engine; i += 8) {
                 h->normal[g].op(e[g], o + i);
                 if (h->dist == 4)
                     for (int j = 0; j < 8; ++j) o[i + j] = o[i + j] * (float)h->b + (float)h->a;
+                if (h->dist == 16)
+                    for (int j = 0; j < 8; ++j) o[i + j] = pexp(o[i + j] * (float)h->b + (float)h->a);
             }
         } else if (h->dist == 9) {
             float* o = (float*)out + g * per_engine;

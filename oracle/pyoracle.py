@@ -121,6 +121,18 @@ def fill_exponential_f32(e, per, lmbda=1.0, threads=1):
     return _f32(e, per, "orc_fill_exponential_f32", C.c_float(lmbda), threads=threads)
 
 
+def fill_lognormal_f32(e, per, mean=0.0, sd=1.0, threads=1):
+    return _f32(e, per, "orc_fill_lognormal_f32", C.c_float(mean), C.c_float(sd), threads=threads)
+
+
+def fill_weibull_f32(e, per, a=1.0, b=1.0, threads=1):
+    return _f32(e, per, "orc_fill_weibull_f32", C.c_float(a), C.c_float(b), threads=threads)
+
+
+def fill_extreme_value_f32(e, per, a=0.0, b=1.0, threads=1):
+    return _f32(e, per, "orc_fill_extreme_value_f32", C.c_float(a), C.c_float(b), threads=threads)
+
+
 def fill_cauchy_f32(e, per, a=0.0, b=1.0, threads=1):
     return _f32(e, per, "orc_fill_cauchy_f32", C.c_float(a), C.c_float(b), threads=threads)
 
@@ -254,7 +266,7 @@ def multinomial(e, trials, weights, samples):
 class Generators:
     """Reused generator objects on n engines (oracle side of erd_generator):
     dist 3 std normal, 4 normal(a, b), 9 gamma(a, b), 10 Poisson(a),
-    11 binomial(a, b)."""
+    11 binomial(a, b), 16 lognormal(a, b)."""
 
     def __init__(self, dist, a, b, n):
         f = lib().orc_gen_create

@@ -81,6 +81,11 @@ CASES = [
     ("gamma_3", lambda er, n, g: er.gamma(n, 1, g, 3.0, 1.5), lambda o, e, n: o.fill_gamma_f32(e, n, 3.0, 1.5)),
     ("gamma_19", lambda er, n, g: er.gamma(n, 1, g, 19.0, 1.0), lambda o, e, n: o.fill_gamma_f32(e, n, 19.0, 1.0)),
     ("chi_squared_4", lambda er, n, g: er.chi_squared(n, 1, g, 4.0), lambda o, e, n: o.fill_gamma_f32(e, n, 2.0, 2.0)),
+    ("lognormal", lambda er, n, g: er.lognormal(n, 1, g, 0.5, 0.75),
+     lambda o, e, n: o.fill_lognormal_f32(e, n, 0.5, 0.75)),
+    ("weibull", lambda er, n, g: er.weibull(n, 1, g, 1.7, 2.5), lambda o, e, n: o.fill_weibull_f32(e, n, 1.7, 2.5)),
+    ("extreme_value", lambda er, n, g: er.extreme_value(n, 1, g, 1.0, 3.0),
+     lambda o, e, n: o.fill_extreme_value_f32(e, n, 1.0, 3.0)),
 ]
 
 
@@ -207,6 +212,7 @@ def test_p8_rejects_bad_sizes(er):
 
 GEN_CASES = [
     ("std_normal", 3, 0.0, 0.0, [8, 24, 40, 8, 16]),
+    ("lognormal", 16, 0.2, 0.7, [8, 8, 24]),
     ("normal", 4, 2.0, 0.5, [24, 8, 16, 8]),
     ("gamma", 9, 0.4, 1.5, [8, 16, 40, 8, 1000]),
     ("gamma", 9, 3.7, 1.0, [16, 8, 24, 2000]),

@@ -197,3 +197,20 @@ def test_serve_rejects_bad_requests():
         s.wait(s.submit(1, "std_normal", 64))
         with pytest.raises(ErdError):
             s.submit(2, "std_normal", 64)             # engine slots exhausted
+
+
+def test_serve_registered_model_equals_inline_covariance():
+    from eigenrand_amd.serving import MVNORMAL
+    rng = np.random.default_rng(5)
+    cov = _spd(40, rng)
+    mean = rng.normal(size=40).astype(np.float32)
+    with _srv() as s:
+        t = s.submit(2, MVNORMAL, 3000, cov=cov, mean=mean)
+        inline = s.fetch(t, np.float32)
+    with _srv() as s:
+        mid = s.register_mvnormal(2, cov, mean)
+        t = s.submit(2, MVNORMAL, 3000, model=mid)
+        reg = s.fetch(t, np.float32)
+        with pytest.raises(Exception):
+            s.submit(3, MVNORMAL, 10, model=mid)  # another client's model
+    np.testing.assert_array_equal(inline, reg)

@@ -8,6 +8,7 @@ import pytest
 CASES = [
     (3, 0.0, 0.0, 8, [8, 24, 40, 8]),          # std normal: odd packet counts leave a cos cached
     (4, 2.0, 0.5, 8, [24, 8, 16]),             # normal(mean, sd)
+    (16, 0.2, 0.7, 8, [8, 8, 24]),             # lognormal: NormalGen's cache
     (9, 0.4, 1.5, 8, [8, 16, 40, 8]),          # gamma GS (alpha < 1)
     (9, 3.7, 1.0, 8, [16, 8, 24]),             # gamma tangent
     (9, 3.0, 2.0, 8, [8, 16]),                 # integer alpha: no state
@@ -23,6 +24,8 @@ def plain(orc, e, dist, a, b, n):
         return orc.fill_normal_f32(e, n)
     if dist == 4:
         return orc.fill_normal_f32(e, n, a, b)
+    if dist == 16:
+        return orc.fill_lognormal_f32(e, n, a, b)
     if dist == 9:
         return orc.fill_gamma_f32(e, n, a, b)
     if dist == 10:

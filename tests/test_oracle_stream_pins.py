@@ -245,3 +245,19 @@ def test_binomial_tangent_rejection_of_the_stream(orc):
                 out += r
         res = np.array(out[:per], dtype=f32).astype(np.int32)
         np.testing.assert_array_equal(got[g], n - res)  # p != small_p: n - k
+
+
+def test_lognormal_weibull_extreme_value_of_the_stream(orc):
+    e = orc.Engines(n=2, seed0=21)
+    per = 512
+    w = full_words(orc, e, per // 8)
+    u = ur(w).astype(np.float64).reshape(2, per)
+    wb = orc.fill_weibull_f32(e.copy(), per, 1.7, 2.5).reshape(2, per)
+    np.testing.assert_allclose(wb, 2.5 * (-np.log(1 - u)) ** (1 / 1.7), rtol=2e-5, atol=1e-6)
+    ev = orc.fill_extreme_value_f32(e.copy(), per, 1.0, 3.0).reshape(2, per)
+    unz = u + 2.0 ** -26
+    np.testing.assert_allclose(ev, 1.0 - 3.0 * np.log(-np.log(unz)), rtol=2e-5, atol=2e-5)
+    # lognormal = exp of the NormalGen fill, packet for packet
+    ln = orc.fill_lognormal_f32(e.copy(), per, 0.5, 0.75)
+    nz = orc.fill_normal_f32(e.copy(), per, 0.5, 0.75)
+    np.testing.assert_allclose(ln, np.exp(nz.astype(np.float64)), rtol=3e-7 * 4)
