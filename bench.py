@@ -358,6 +358,7 @@ def main():
     ap.add_argument("--mv", action="store_true", help="only the batched multivariate measurements")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (p50/p99) leg")
     ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-only", action="store_true", help="only the serving leg (one JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -383,6 +384,9 @@ def main():
         return
     if args.mv:
         mv_sweep(lambda d: print(json.dumps(d), flush=True))
+        return
+    if args.serve_only:
+        print(json.dumps({"serving": serving(n_req=args.serve_requests)}), flush=True)
         return
     stream = torch.cuda.current_stream()
     urng = er.Vmt19937_64(0, n=N_ENGINES, stride=4)

@@ -480,6 +480,15 @@ void orc_fill_u64(void* states, int64_t n_engines, uint64_t* out, int64_t calls,
     });
 }
 
+// MersenneTwister::discard (PacketRandomEngine.h:317-323): `calls` draws.
+void orc_discard(void* states, int64_t n_engines, int64_t calls, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        uint64_t w[4];
+        for (int64_t k = 0; k < calls; ++k) vmt_draw(e[g], w);
+    });
+}
+
 // P8_mt19937_64 / ParallelRandomEngineAdaptor<uint64_t, Vmt19937_64, 8>
 // (PacketRandomEngine.h:429-517, 600-609): two engines seeded s and s+4,
 // buffer = [engine0 call, engine1 call]; out[8k + 4i + l].

@@ -68,6 +68,10 @@ def fill_u64(e, draws, threads=1):
     return out
 
 
+def discard(e, calls, threads=1):
+    lib().orc_discard(_p(e.raw), I64(len(e)), I64(calls), C.c_int(threads))
+
+
 def fill_half_u64(e, halves):
     out = np.empty(len(e) * halves * 2, dtype=np.uint64)
     lib().orc_fill_half_u64(_p(e.raw), I64(len(e)), _p(out), I64(halves))

@@ -234,3 +234,32 @@ def test_generator_objects_match_oracle(er, orc, name, dist, a, b, sizes):
         assert_same(got, rgen.fill(ref, n))
     np.testing.assert_array_equal(urng.snapshot(), ref.raw)
     assert urng.status() == 0
+
+
+def test_offsets_past_2_31_elements(er, orc):
+    """Two engines of 2^30 + 8 floats: engine 1's block ends past element
+    2^31, so its tail checks 64-bit output indexing."""
+    import torch
+    per = (1 << 30) + 8
+    urng = er.Vmt19937_64(seeds=[3, 11])
+    out = torch.empty(2 * per, dtype=torch.float32, device="cuda")
+    er.uniform_real(per, 1, urng, out=out)
+    tail = out[-64:].cpu().numpy()
+    ref = orc.Engines(seeds=[11])
+    orc.discard(ref, per // 8 - 8, threads=1)
+    assert_same(tail, orc.fill_uniform_f32(ref, 64))
+    np.testing.assert_array_equal(urng.snapshot()[10016:], ref.raw)
+    del out
+
+
+def test_many_engines_and_empty_fills(er, orc):
+    """2^20 engines in one launch; zero-size fills leave engines untouched."""
+    n = 1 << 20
+    urng = er.Vmt19937_64(7, n=n, stride=4)
+    before = urng.view(n - 2, 2).snapshot()
+    er.normal(0, 1, urng)
+    er.poisson(0, 1, urng, 30.0)
+    np.testing.assert_array_equal(urng.view(n - 2, 2).snapshot(), before)
+    x = er.normal(16, 1, urng).reshape(n, 16)
+    ref = orc.Engines(seeds=[7 + 4 * (n - 1)])
+    assert_same(x[n - 1].cpu().numpy(), orc.fill_normal_f32(ref, 16))

@@ -80,3 +80,10 @@ def test_uniform_f64_closed_form(orc, mt_golden):
         w = mt_golden["streams"][lane][:64]
         exp = (w & np.uint64((1 << 52) - 1)).astype(np.float64) / 2.0 ** 52
         np.testing.assert_array_equal(u[:, lane], exp)
+
+
+def test_discard_equals_drawing(orc):
+    e1, e2 = orc.Engines(seeds=[9, 10]), orc.Engines(seeds=[9, 10])
+    orc.discard(e1, 1234)
+    orc.fill_u64(e2, 1234)
+    np.testing.assert_array_equal(e1.raw, e2.raw)
