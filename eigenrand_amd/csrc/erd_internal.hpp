@@ -76,5 +76,8 @@ int factor_and_split(const std::vector<int32_t>& dims, double* d_a, const std::v
                      std::vector<int32_t>* info_out, hipStream_t s, Scratch& sc);
 int build_table(const double* w, int64_t K, HostTable& t);
 int upload_table(const HostTable& t, Segment& sg, Scratch& sc);
+// MultinomialGen's conditional-binomial rows (Multinomial.h:74-91): the float
+// probability of rows 0..K-2, from weights normalised in double.
+std::vector<float> multinomial_row_p(const double* w, int64_t K);
 
 }  // namespace erd

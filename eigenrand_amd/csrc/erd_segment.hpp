@@ -27,12 +27,12 @@ enum SegKind : int32_t {
     SEG_DISCARD = 15,        // count = draws to skip
     SEG_DISCRETE_I32 = 16,   // in0=cdf|prob, in1=alias, d[1]=ncdf, d[2]=bitmask
     SEG_MULTINOMIAL_SMALL = 17,  // count=samples, i0=trials, d[0]=K, tables as SEG_DISCRETE_I32
-    SEG_MULTINOMIAL_ROW = 18,    // count=samples, in0=r (int32, updated), d[0]=p, d[1]=stride,
-                                 // i0 >= 0: trials of every sample (first row; r is not read)
-    SEG_MULTINOMIAL_LAST = 19,   // out[s*stride] = r[s] (i0 >= 0: i0); d[1]=stride
-    SEG_LOGNORMAL_F32 = 20,      // d[0]=mean, d[1]=sd
-    SEG_WEIBULL_F32 = 21,        // d[0]=1/a (float), d[1]=b
-    SEG_EXTREME_VALUE_F32 = 22,  // d[0]=a, d[1]=b
+    SEG_MULTINOMIAL_ROWS = 18,   // conditional-binomial rows of one instance: count=samples,
+                                 // i0=trials, in0=r (int32 scratch, count), in1=row p (float,
+                                 // K-1), d[1]=K; out is the K x count column-major block
+    SEG_LOGNORMAL_F32 = 19,      // d[0]=mean, d[1]=sd
+    SEG_WEIBULL_F32 = 20,        // d[0]=1/a (float), d[1]=b
+    SEG_EXTREME_VALUE_F32 = 21,  // d[0]=a, d[1]=b
     SEG_KIND_COUNT
 };
 

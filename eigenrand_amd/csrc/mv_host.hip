@@ -539,6 +539,20 @@ int upload_table(const HostTable& t, Segment& sg, Scratch& sc) {
     return ERD_OK;
 }
 
+std::vector<float> multinomial_row_p(const double* w, int64_t K) {
+    std::vector<double> probs(w, w + K);
+    double sum = 0;
+    for (double v : probs) sum += v;
+    for (double& v : probs) v /= sum;
+    std::vector<float> p;
+    double rest_p = 1 - probs[0];
+    for (int64_t row = 0; row + 1 < K; ++row) {
+        p.push_back(row == 0 ? (float)probs[0] : (float)(probs[row] / rest_p));
+        if (row > 0) rest_p -= probs[row];
+    }
+    return p;
+}
+
 }  // namespace erd
 
 extern "C" {
@@ -615,34 +629,19 @@ int erd_multinomial_batch(erd_engines* e, int64_t n_inst, const int64_t* dims, c
             eng.push_back((int32_t)i);
         } else {
             if (n % 8 != 0) return ERD_ERR_INVALID;  // BinomialVGen<float> rows run in Packet8f
-            std::vector<double> probs(weights + woff, weights + woff + K);
-            double sum = 0;
-            for (double v : probs) sum += v;
-            for (double& v : probs) v /= sum;
             int32_t* r;  // remaining trials per sample, written by the first row
             if (sc.alloc(&r, (size_t)n) != hipSuccess) return ERD_ERR_OOM;
-            double rest_p = 1 - probs[0];
-            for (int64_t row = 0; row + 1 < K; ++row) {
-                Segment sg{};
-                sg.kind = SEG_MULTINOMIAL_ROW;
-                sg.count = n;
-                sg.out = o + row;
-                sg.in0 = r;
-                sg.i0 = row == 0 ? T : -1;
-                sg.d[0] = row == 0 ? (float)probs[0] : (float)(probs[row] / rest_p);
-                sg.d[1] = (double)K;
-                if (row > 0) rest_p -= probs[row];
-                segs.push_back(sg);
-                eng.push_back((int32_t)i);
-            }
-            Segment last{};
-            last.kind = SEG_MULTINOMIAL_LAST;
-            last.count = n;
-            last.out = o + (K - 1);
-            last.in0 = r;
-            last.i0 = K == 1 ? T : -1;  // a single category takes every trial
-            last.d[1] = (double)K;
-            segs.push_back(last);
+            float* prow;
+            if (sc.upload(&prow, multinomial_row_p(weights + woff, K)) != hipSuccess) return ERD_ERR_HIP;
+            Segment sg{};
+            sg.kind = SEG_MULTINOMIAL_ROWS;
+            sg.count = n;
+            sg.out = o;
+            sg.in0 = r;
+            sg.in1 = prow;
+            sg.i0 = T;
+            sg.d[1] = (double)K;
+            segs.push_back(sg);
             eng.push_back((int32_t)i);
         }
         woff += K;

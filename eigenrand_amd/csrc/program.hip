@@ -209,14 +209,19 @@ __device__ void run_segment(Eng& e, ProgramLds& s, const Segment& sg, uint32_t* 
             op_fixed(e, s.carry_fixed, n, d, lane);
             break;
         }
-        case SEG_MULTINOMIAL_ROW:
-            op_binomial_row8(e, s.vop.carry, s.vop.res, (int32_t*)sg.out, (int64_t)sg.d[1], (int32_t*)sg.in0, sg.i0,
-                             n, (float)sg.d[0], status, lane);
-            break;
-        case SEG_MULTINOMIAL_LAST: {
-            const int64_t stride = (int64_t)sg.d[1];
-            const int32_t* rr = (const int32_t*)sg.in0;
-            for (int64_t i = lane; i < n; i += 64) ((int32_t*)sg.out)[i * stride] = sg.i0 >= 0 ? sg.i0 : rr[i];
+        case SEG_MULTINOMIAL_ROWS: {
+            // MultinomialGen's large-trials path (Multinomial.h:74-91) for the
+            // samples of one engine: row i is BinomialVGen<float>(r, p_i)
+            // over the samples (row 0 with every sample's trials), r -= row;
+            // the last category takes what remains.
+            const int64_t K = (int64_t)sg.d[1];
+            const float* prow = (const float*)sg.in1;
+            int32_t* rr = (int32_t*)sg.in0;
+            int32_t* o = (int32_t*)sg.out;
+            for (int64_t row = 0; row + 1 < K; ++row)
+                op_binomial_row8(e, s.vop.carry, s.vop.res, o + row, K, rr, row == 0 ? sg.i0 : -1, n, prow[row],
+                                 status, lane);
+            for (int64_t i = lane; i < n; i += 64) o[i * K + (K - 1)] = K == 1 ? sg.i0 : rr[i];
             break;
         }
         default:

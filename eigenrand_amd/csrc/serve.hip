@@ -7,7 +7,8 @@
 // the HBM budget) and turns them into one segment program launch (+ the
 // batched L*Z transform for MvNormal requests) on a free stream, while
 // earlier batches are still running on other streams. A completion thread
-// waits on each batch's event, stamps its requests done and frees the slot.
+// polls the in-flight batches' events, retires each batch as soon as it has
+// finished (in any order), stamps its requests done and frees its stream.
 //
 // Engines: a request is spread over e consecutive engines of its client
 // (about 2^16 normal-equivalents of work per engine, so heavier samplers
@@ -375,37 +376,19 @@ int Server::prepare_and_launch(Batch* b) {
                 }
                 continue;
             }
-            std::vector<double> probs = q->param;
-            double sum = 0;
-            for (double v : probs) sum += v;
-            for (double& v : probs) v /= sum;
             int32_t* rr;  // remaining trials per sample, written by the first row
             if (sc.alloc(&rr, (size_t)q->per * q->e_req) != hipSuccess) return ERD_ERR_OOM;
+            const size_t in1 = st.put(multinomial_row_p(q->param.data(), K));
             for (int g = 0; g < q->e_req; ++g) {
-                int32_t* og = o + (int64_t)g * K * q->per;
-                int32_t* rg = rr + (int64_t)g * q->per;
-                double rest_p = 1 - probs[0];
-                for (int64_t row = 0; row + 1 < K; ++row) {
-                    Segment sg{};
-                    sg.kind = SEG_MULTINOMIAL_ROW;
-                    sg.i0 = row == 0 ? T : -1;
-                    sg.count = q->per;
-                    sg.out = og + row;
-                    sg.in0 = rg;
-                    sg.d[0] = row == 0 ? (float)probs[0] : (float)(probs[row] / rest_p);
-                    sg.d[1] = (double)K;
-                    if (row > 0) rest_p -= probs[row];
-                    segs.push_back(sg);
-                    seg_engine.push_back(engine_of(q, g));
-                }
-                Segment last{};
-                last.kind = SEG_MULTINOMIAL_LAST;
-                last.i0 = K == 1 ? T : -1;
-                last.count = q->per;
-                last.out = og + (K - 1);
-                last.in0 = rg;
-                last.d[1] = (double)K;
-                segs.push_back(last);
+                Segment sg{};
+                sg.kind = SEG_MULTINOMIAL_ROWS;
+                sg.i0 = T;
+                sg.count = q->per;
+                sg.out = o + (int64_t)g * K * q->per;
+                sg.in0 = rr + (int64_t)g * q->per;
+                sg.d[1] = (double)K;
+                relocs.push_back({segs.size(), 1, in1});
+                segs.push_back(sg);
                 seg_engine.push_back(engine_of(q, g));
             }
         } else {
@@ -575,26 +558,46 @@ void Server::launcher_loop() {
     }
 }
 
+// Batches on different streams finish in any order: each is retired as soon
+// as its event has completed, so a long batch holds only its own stream and
+// engines, not the retirement of the batches launched after it.
 void Server::completer_loop() {
+    std::vector<Batch*> snap;
     for (;;) {
         Batch* b = nullptr;
         {
             std::unique_lock<std::mutex> lk(mu);
             cv.wait(lk, [&] { return stop || !inflight.empty(); });
             if (inflight.empty()) return;
-            b = inflight.front();
+            snap.assign(inflight.begin(), inflight.end());
         }
-        if (b->status == ERD_OK) {
-            if (hipEventSynchronize(b->ev) != hipSuccess) b->status = ERD_ERR_HIP;
-        } else {
-            (void)hipStreamSynchronize(b->stream);
+        for (Batch* c : snap) {
+            if (c->status != ERD_OK) {
+                (void)hipStreamSynchronize(c->stream);
+                b = c;
+                break;
+            }
+            const hipError_t q = hipEventQuery(c->ev);
+            if (q == hipSuccess) {
+                b = c;
+                break;
+            }
+            if (q != hipErrorNotReady) {
+                c->status = ERD_ERR_HIP;
+                b = c;
+                break;
+            }
+        }
+        if (!b) {
+            std::this_thread::sleep_for(std::chrono::microseconds(10));
+            continue;
         }
         const double t = now_s();
         b->scratch.reset();  // stream-ordered frees
         if (b->ev) (void)hipEventDestroy(b->ev);
         {
             std::lock_guard<std::mutex> lk(mu);
-            inflight.pop_front();
+            inflight.erase(std::find(inflight.begin(), inflight.end(), b));
             for (Req* q : b->reqs) {
                 q->t_done = t;
                 q->state = b->status == ERD_OK ? 2 : 3;

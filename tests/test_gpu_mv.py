@@ -169,6 +169,7 @@ def test_discrete_fill_matches_oracle(er, orc, K):
     [(20, 4, 16), (99, 10, 24), (7, 3, 8), (100, 1, 8)],           # per-sample discrete draws
     [(1000, 4, 64), (5000, 100, 16), (3001, 3, 8)],                # BinomialVGen<float> rows
     [(50, 1000, 8), (1, 20, 16), (250, 12, 40), (2000, 16, 24)],  # mixed
+    [(60000, 1500, 16), (5000, 2, 8)],                              # long row chains
 ])
 def test_multinomial_batch_matches_oracle(er, orc, cases):
     rng = np.random.default_rng(len(cases))
