@@ -411,6 +411,216 @@ constexpr int kKnuthIters = 32;
 constexpr int kKnuthSlots = kKnuthStarts + kKnuthIters - 1;
 constexpr int kKnuthStage = kKnuthSlots * 4;
 
+// op_knuth's prefix path (larger means, where a packet spans many slots):
+// a window of kLpSlots slots holds, per lane, prefix sums of a fixed-point
+// log2 of the uniforms, so the stopping iteration of a packet starting at any
+// slot is one comparison per candidate iteration (one per thread).
+//
+// Exactness. The loop's product p_i = fl(p_{i-1} u_i) satisfies
+// |log2 p_i - sum log2 u| <= (i+1) 2^-24 / ln 2 (one rounding per product;
+// every product compared is a normal number above e^-12 or is the first one
+// at or below the threshold). The prefix difference D_i approximates
+// sum log2 u with at most (i+1) (e_log + 2^-20) error: e_log is the error of
+// v_log_f32 on the values k 2^-23 (at most 1.9e-6, one ulp, measured over all
+// 2^23 of them) and 2^-20 the rounding to kLpFrac fraction bits; the
+// threshold's own log2 adds the same once. For i < 64 the total stays under
+// 2^-12 = 128 fixed-point units, so an iteration whose D_i lies more than
+// kLpMargin = 256 units below
+// (above) log2(thr) has p_i <= thr (p_i > thr). Only if an iteration before
+// the first certainly-stopped one lies within the margin is the packet's lane
+// recomputed with the exact product loop (about 1 lane in 1500).
+constexpr int kLpSlots = 128;
+constexpr int kLpFrac = 19;
+constexpr int32_t kLpMargin = 256;
+// Thresholds e^-mean below this (mean above about 2.3, where a packet spans
+// about seven slots or more) take the prefix path.
+constexpr float kKnuthPrefixThr = 0.1f;
+
+// Tempered word q of window slot v: the carried half (v < nc), block slot
+// pos + v - nc, or past the block end a slot of the next block, computed
+// read-only from the untwisted block (its first kM words depend on this
+// block only); the block is twisted in place once a packet uses such a slot.
+__device__ __forceinline__ uint64_t lp_word(const uint64_t* mt, const uint64_t* carry, int pos, int nc, int v,
+                                            int q) {
+    if (v < nc) return carry[q];
+    const int b = pos + v - nc;
+    if (b < kHalfSlots) return temper(mt[2 * b + q]);
+    const int w = 2 * (b - kHalfSlots) + q;
+    return temper(twist_word(mt[w], mt[w + kLanes], mt[w + kM * kLanes]));
+}
+
+__device__ __forceinline__ float lp_u(const uint64_t* mt, const uint64_t* carry, int pos, int nc, int v, int j) {
+    const uint64_t x = lp_word(mt, carry, pos, nc, v, j >> 1);
+    return ur_f32((j & 1) ? hi32(x) : lo32(x));
+}
+
+// Fixed-point log2 (kLpFrac fraction bits); u = 0 maps to -32, below every
+// threshold's log2 (the threshold is at least e^-12).
+__device__ __forceinline__ uint32_t lp_fix(float u) {
+    const float l = fmaxf(__builtin_amdgcn_logf(u), -32.0f);
+    return (uint32_t)__float2int_rn(l * (float)(1 << kLpFrac));
+}
+
+// Inclusive scan over the wave, modulo 2^32: Hillis-Steele within rows of 16
+// lanes (DPP row shifts, zero past the row start), then the row totals.
+template <int N>
+__device__ __forceinline__ uint32_t dpp_row_shr(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x110 + N, 0xf, 0xf, true);
+}
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
+    x += dpp_row_shr<1>(x);
+    x += dpp_row_shr<2>(x);
+    x += dpp_row_shr<4>(x);
+    x += dpp_row_shr<8>(x);
+    const uint32_t r0 = __builtin_amdgcn_readlane(x, 15);
+    const uint32_t r1 = r0 + __builtin_amdgcn_readlane(x, 31);
+    const uint32_t r2 = r1 + __builtin_amdgcn_readlane(x, 47);
+    return x + (lane < 16 ? 0u : lane < 32 ? r0 : lane < 48 ? r1 : r2);
+}
+
+// The window's prefix sums stay in registers: thread t holds
+// r0 = P[t + 1] and r1 = P[t + 65] per lane, where P[k] is the sum of lp_fix
+// over window slots < k (P[0] = 0). Sums wrap modulo 2^32; differences over
+// at most 64 slots are exact.
+struct LpWindow {
+    uint32_t r0[4], r1[4];
+};
+
+__device__ __forceinline__ void lp_stage(const uint64_t* mt, const uint64_t* carry, int pos, int nc, LpWindow& w,
+                                         int lane) {
+    uint32_t a0[4], a1[4];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const uint64_t x0 = lp_word(mt, carry, pos, nc, lane, q);
+        const uint64_t x1 = lp_word(mt, carry, pos, nc, lane + 64, q);
+        a0[2 * q] = lp_fix(ur_f32(lo32(x0)));
+        a0[2 * q + 1] = lp_fix(ur_f32(hi32(x0)));
+        a1[2 * q] = lp_fix(ur_f32(lo32(x1)));
+        a1[2 * q + 1] = lp_fix(ur_f32(hi32(x1)));
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        w.r0[j] = wave_incl_scan(a0[j], lane);
+        w.r1[j] = __builtin_amdgcn_readlane(w.r0[j], 63) + wave_incl_scan(a1[j], lane);
+    }
+}
+
+// P[s + 1 + lane] for window slot s (0 <= s <= 128): the value lives in lane
+// (s + lane) mod 64, in r0 if that lane is at or after s, else in r1.
+__device__ __forceinline__ uint32_t lp_fetch(const LpWindow& w, int j, int s, int lane) {
+    const uint32_t v = lane < s ? w.r1[j] : w.r0[j];
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(((s + lane) & 63) << 2, (int)v);
+}
+
+// Exact stopping iteration of lane j for a packet starting at window slot s:
+// the reference's product loop, at most `cap` iterations (wave-uniform).
+__device__ __attribute__((noinline)) int lp_exact(const uint64_t* mt, const uint64_t* carry, int pos, int nc,
+                                                   int s, int j, float thr, int cap) {
+    float val = 1.0f;
+    int k = 0;
+    for (; k < cap; ++k) {
+        val = val * lp_u(mt, carry, pos, nc, s + k, j);
+        if (!(thr < val)) break;
+    }
+    return __builtin_amdgcn_readfirstlane(k);
+}
+
+// Move the cursor past `n` window slots; a position past the block end means
+// a packet used slots of the next block, so the block is twisted now.
+__device__ __forceinline__ void lp_advance(HalfCursor& cur, int n, int lane) {
+    cur.advance(n);
+    if (cur.pos > kHalfSlots) {
+        __syncthreads();
+        twist_lds(cur.mt, lane);
+        cur.pos -= kHalfSlots;
+        *cur.twisted = true;
+    }
+}
+
+// First index (0..63) at which lane j of the packet is certainly stopped, and
+// whether an index up to it lies within the margin (then the exact loop decides).
+struct LpLane {
+    uint64_t stop, amb;
+};
+__device__ __forceinline__ LpLane lp_lane(uint32_t pt, uint32_t c, bool in) {
+    // Past the window: a value that is neither stopped nor within the margin.
+    const uint32_t d = in ? pt - c : 0x7fffffffu;
+    LpLane r;
+    r.stop = __builtin_amdgcn_ballot_w64((int32_t)d < -kLpMargin);
+    r.amb = __builtin_amdgcn_ballot_w64(d + (uint32_t)kLpMargin <= (uint32_t)(2 * kLpMargin));
+    return r;
+}
+
+// Prefix path of op_knuth: per window, one step per packet. Thread t tests
+// iteration t of the packet starting at window slot s for all 4 lanes
+// against the threshold (one ballot per lane gives each lane's first
+// certainly-stopped iteration); the packet consumes up to its slowest lane
+// and the next starts after it. A packet whose stop lies past the window
+// restarts the window at its first slot.
+__device__ void op_knuth_lp(Eng& e, uint64_t* carry, int32_t* o, int64_t T, float thr, int32_t trials, int flip,
+                            uint32_t* status, int lane) {
+    HalfCursor cur = cursor_begin(e, carry, lane);
+    const uint32_t tfix = (uint32_t)__builtin_amdgcn_readfirstlane(
+        __float2int_rn(__builtin_amdgcn_logf(thr) * (float)(1 << kLpFrac)));
+    const int j = lane & 3;
+    int64_t flushed = 0;  // packets written to o
+    int nb = 0;           // packets held in obuf (lane 4k+j: packet flushed+k, lane j)
+    int32_t obuf = 0;
+    LpWindow w;
+    while (flushed + nb < T) {
+        lp_stage(cur.mt, carry, cur.pos, cur.nc, w, lane);
+        int s = 0;
+        uint32_t c0 = tfix, c1 = tfix, c2 = tfix, c3 = tfix;  // P[s] + log2(thr), per lane
+        bool serial = false;
+        while (flushed + nb < T) {
+            const int room = kLpSlots - s;
+            const bool in = lane < room;
+            const uint32_t p0 = lp_fetch(w, 0, s, lane), p1 = lp_fetch(w, 1, s, lane);
+            const uint32_t p2 = lp_fetch(w, 2, s, lane), p3 = lp_fetch(w, 3, s, lane);
+            const LpLane l0 = lp_lane(p0, c0, in), l1 = lp_lane(p1, c1, in);
+            const LpLane l2 = lp_lane(p2, c2, in), l3 = lp_lane(p3, c3, in);
+            if (!l0.stop || !l1.stop || !l2.stop || !l3.stop) {
+                serial = room >= 64;  // longer than 64 iterations: exact one-slot loop
+                break;
+            }
+            int st0 = (int)__builtin_ctzll(l0.stop), st1 = (int)__builtin_ctzll(l1.stop);
+            int st2 = (int)__builtin_ctzll(l2.stop), st3 = (int)__builtin_ctzll(l3.stop);
+            if (l0.amb & ((2ull << st0) - 1))
+                st0 = __builtin_amdgcn_readfirstlane(lp_exact(cur.mt, carry, cur.pos, cur.nc, s, 0, thr, st0));
+            if (l1.amb & ((2ull << st1) - 1))
+                st1 = __builtin_amdgcn_readfirstlane(lp_exact(cur.mt, carry, cur.pos, cur.nc, s, 1, thr, st1));
+            if (l2.amb & ((2ull << st2) - 1))
+                st2 = __builtin_amdgcn_readfirstlane(lp_exact(cur.mt, carry, cur.pos, cur.nc, s, 2, thr, st2));
+            if (l3.amb & ((2ull << st3) - 1))
+                st3 = __builtin_amdgcn_readfirstlane(lp_exact(cur.mt, carry, cur.pos, cur.nc, s, 3, thr, st3));
+            const int last = max(max(st0, st1), max(st2, st3));  // consumption - 1
+            const int mine = j == 0 ? st0 : j == 1 ? st1 : j == 2 ? st2 : st3;
+            obuf = (lane >> 2) == nb ? mine : obuf;
+            if (++nb == 16) {
+                o[4 * flushed + lane] = flip ? trials - obuf : obuf;
+                flushed += 16;
+                nb = 0;
+            }
+            c0 = __builtin_amdgcn_readlane(p0, last) + tfix;
+            c1 = __builtin_amdgcn_readlane(p1, last) + tfix;
+            c2 = __builtin_amdgcn_readlane(p2, last) + tfix;
+            c3 = __builtin_amdgcn_readlane(p3, last) + tfix;
+            s += last + 1;
+        }
+        lp_advance(cur, s, lane);
+        if (serial) {
+            if (lane < 4 * nb) o[4 * flushed + lane] = flip ? trials - obuf : obuf;
+            flushed += nb;
+            nb = 0;
+            const int32_t r = knuth_packet(cur, thr, lane, status);
+            if (lane < 4) o[4 * flushed + lane] = flip ? trials - r : r;
+            ++flushed;
+        }
+    }
+    if (lane < 4 * nb) o[4 * flushed + lane] = flip ? trials - obuf : obuf;
+    cursor_end(e, cur);
+}
+
 // PoissonGen, mean < 12; BinomialGen's Poisson regime with the final flip.
 //
 // A packet starting at slot P consumes 1 + max_j(stop_j) half draws, where
@@ -427,11 +637,16 @@ constexpr int kKnuthStage = kKnuthSlots * 4;
 // iterations, and the marked threads store their packets at their rank in
 // the chain. A packet needing more iterations falls back to the
 // one-slot-at-a-time loop. Results and consumption are those of the
-// sequential loop.
+// sequential loop. Larger means (thr < kKnuthPrefixThr), where most of the 64
+// speculative starts are wasted, take op_knuth_lp.
 __device__ void op_knuth(Eng& e, uint64_t* carry, float* ustage, int32_t* o, int64_t per, float thr,
                          int32_t trials, int flip, uint32_t* status, int lane) {
     const int64_t T = per / 4;
     if (T == 0) return;
+    if (thr < kKnuthPrefixThr) {
+        op_knuth_lp(e, carry, o, T, thr, trials, flip, status, lane);
+        return;
+    }
     HalfCursor cur = cursor_begin(e, carry, lane);
     int64_t pk = 0;
     while (pk < T) {

@@ -40,6 +40,23 @@ __global__ __launch_bounds__(64) void k_poisson_knuth(EngineState* __restrict__ 
     eng_store(e, states + g, lane);
 }
 
+// The same for thresholds below kKnuthPrefixThr: the prefix path keeps its
+// window in registers, so the workgroup needs only the engine in LDS.
+__global__ __launch_bounds__(64) void k_poisson_knuth_lp(EngineState* __restrict__ states,
+                                                         int32_t* __restrict__ out, int64_t per,
+                                                         float thr, int32_t trials, int flip,
+                                                         uint32_t* __restrict__ status) {
+    __shared__ uint64_t mt[kWords];
+    __shared__ uint64_t carry[2];
+    const int lane = threadIdx.x;
+    const int64_t g = blockIdx.x;
+    if (per / 4 == 0) return;
+    Eng e;
+    eng_load(e, mt, states + g, lane);
+    op_knuth_lp(e, carry, out + g * per, per / 4, thr, trials, flip, status, lane);
+    eng_store(e, states + g, lane);
+}
+
 // BinomialGen, trials < 25.
 __global__ __launch_bounds__(64) void k_binomial_small(EngineState* __restrict__ states,
                                                        int32_t* __restrict__ out, int64_t per,
@@ -129,8 +146,8 @@ hipError_t launch_fill_poisson_knuth_i32(EngineState* st, int64_t n, int32_t* ou
                                          hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const float thr = (float)std::exp(-mean);
-    hipLaunchKernelGGL(k_poisson_knuth, dim3((unsigned)n), dim3(64), 0, s, st, out, per, thr, trials,
-                       flip, status);
+    hipLaunchKernelGGL(thr < kKnuthPrefixThr ? k_poisson_knuth_lp : k_poisson_knuth, dim3((unsigned)n), dim3(64), 0,
+                       s, st, out, per, thr, trials, flip, status);
     return hipGetLastError();
 }
 

@@ -94,3 +94,16 @@ def test_poisson_knuth_long_fills(er, orc, mean):
     out = er.binomial(4096, 1, urng, 400, 0.998)
     assert_same(flat(out), orc.fill_binomial_i32(ref, 4096, 400, 0.998, threads=8))
     np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+
+
+@pytest.mark.parametrize("mean", [2.2, 2.4, 5.0, 9.99])
+def test_poisson_knuth_prefix_many_windows(er, orc, mean):
+    """Means on both sides of the prefix-path switch (e^-mean = 0.1), over
+    enough packets that lanes within the fixed-point margin (recomputed with
+    the exact loop) and windows crossing twists occur many times."""
+    urng = er.Vmt19937_64(seeds=SEEDS)
+    ref = orc.Engines(seeds=SEEDS)
+    for n in (1 << 18, 4, 8):
+        out = er.poisson(n, 1, urng, mean)
+        assert_same(flat(out), orc.fill_poisson_i32(ref, n, mean, threads=8))
+        np.testing.assert_array_equal(urng.snapshot(), ref.raw)
