@@ -179,7 +179,7 @@ def serving_workload(n_req, n_clients, seed=5):
     return reqs, covs
 
 
-def serving(n_req=300, n_clients=64, engines_per_client=1024):
+def serving(n_req=300, n_clients=64, engines_per_client=4096, engine_work=0):
     """Serving configuration on one GPU: the batched loop (continuous batching
     over 8 streams) against per-request launches (one request per launch, one
     at a time) on the same server code, engines and outputs. Returns a dict."""
@@ -218,12 +218,14 @@ def serving(n_req=300, n_clients=64, engines_per_client=1024):
 
     def closed(srv):
         t0 = time.perf_counter()
+        m0 = time.monotonic_ns()
         ts = [submit(srv, r) for r in reqs]
         for t in ts:
             srv.wait(t)
             srv.release(t)
         dt = time.perf_counter() - t0
         st = srv.stats()
+        st["window_ns"] = [m0, time.monotonic_ns()]
         srv.reset_stats()
         return dt, st
 
@@ -252,10 +254,11 @@ def serving(n_req=300, n_clients=64, engines_per_client=1024):
         return st
 
     res = {"requests": n_req, "clients": n_clients, "engines_per_client": engines_per_client,
+           "engine_work": engine_work or 65536,
            "elements": int(total_elems), "hbm_budget_bytes": 64 << 30}
     for name, cfg in modes.items():
         with Server(max_clients=n_clients, engines_per_client=engines_per_client,
-                    hbm_budget_bytes=64 << 30, seed_salt=0x5EED, **cfg) as srv:
+                    hbm_budget_bytes=64 << 30, seed_salt=0x5EED, engine_work=engine_work, **cfg) as srv:
             warm(srv)
             dt, st = closed(srv)
             sat_rate = n_req / dt
@@ -265,6 +268,7 @@ def serving(n_req=300, n_clients=64, engines_per_client=1024):
                 "saturated_gsamples_per_s": round(total_elems / dt / 1e9, 3),
                 "saturated_requests_per_s": round(sat_rate, 1),
                 "saturated_batches": st["batches"],
+                "saturated_launcher_busy_frac": round(st["prep_s"] / dt, 3),
                 "open_loop_rate_per_s": round(load, 1),
                 "open_loop_gsamples_per_s": round(ol["elements"] / ol["span_s"] / 1e9, 3),
                 "p50_ms": round(ol["p50_ms"], 3),
@@ -273,7 +277,8 @@ def serving(n_req=300, n_clients=64, engines_per_client=1024):
                 "batches": ol["batches"],
                 "launcher_busy_frac": round(ol["prep_s"] / ol["span_s"], 3) if ol["span_s"] else None,
             }
-        print(json.dumps({"serving_mode": name, **res[name]}), file=sys.stderr, flush=True)
+        print(json.dumps({"serving_mode": name, **res[name], "closed_loop_window_ns": st["window_ns"]}),
+              file=sys.stderr, flush=True)
     b, p = res["batched"], res["per_request"]
     res["throughput_ratio"] = round(b["saturated_gsamples_per_s"] / p["saturated_gsamples_per_s"], 2)
     return res

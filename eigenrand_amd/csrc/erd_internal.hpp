@@ -16,19 +16,41 @@ namespace erd {
 // does not wait for the stream.
 hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t s);
 
-// Stream-ordered scratch released when the object goes out of scope.
+// A source of device blocks that are recycled by their owner rather than
+// freed through the driver (the serving loop's block pool).
+struct Recycler {
+    virtual void* get(size_t bytes) = 0;
+    virtual void put(void* p) = 0;
+
+protected:
+    ~Recycler() = default;
+};
+
+// Stream-ordered scratch released when the object goes out of scope. With a
+// recycler, blocks come from it and go back to it on destruction, which the
+// owner does only once the stream's work on them has completed.
 struct Scratch {
     hipStream_t s;
     std::vector<void*> ptrs;
-    explicit Scratch(hipStream_t st) : s(st) {}
+    Recycler* rec = nullptr;
+    explicit Scratch(hipStream_t st, Recycler* r = nullptr) : s(st), rec(r) {}
     ~Scratch() {
-        for (void* p : ptrs) (void)hipFreeAsync(p, s);
+        for (void* p : ptrs) {
+            if (rec) rec->put(p);
+            else (void)hipFreeAsync(p, s);
+        }
     }
     template <class T>
     hipError_t alloc(T** p, size_t count) {
         *p = nullptr;
         if (count == 0) count = 1;
-        hipError_t e = hipMallocAsync((void**)p, sizeof(T) * count, s);
+        hipError_t e;
+        if (rec) {
+            *p = (T*)rec->get(sizeof(T) * count);
+            e = *p ? hipSuccess : hipErrorOutOfMemory;
+        } else {
+            e = hipMallocAsync((void**)p, sizeof(T) * count, s);
+        }
         if (e == hipSuccess) ptrs.push_back(*p);
         return e;
     }

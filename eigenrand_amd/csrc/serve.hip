@@ -46,7 +46,7 @@
 namespace erd {
 namespace {
 
-constexpr double kEngineWork = 65536.0;  // normal-equivalent samples per engine
+constexpr double kEngineWork = 65536.0;  // default normal-equivalent samples per engine
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -81,7 +81,7 @@ size_t elem_bytes(int kind) {
 // multiple of the normal fill's time per sample (gamma: GS 5, tangent 6;
 // Poisson: Knuth 10 + 6 * mean, tangent 7; binomial: n < 25 1 + 1.2 n,
 // Knuth 13, tangent 12).
-int64_t chunk_for(const erd_serve_request& r) {
+int64_t chunk_for(const erd_serve_request& r, double work) {
     double cost = 1.0;
     switch (r.kind) {
         case ERD_DIST_STUDENT_T_F32: cost = 2.0; break;
@@ -106,7 +106,7 @@ int64_t chunk_for(const erd_serve_request& r) {
         }
         default: break;
     }
-    return std::max<int64_t>(1, (int64_t)(kEngineWork / std::max(cost, 1e-3)));
+    return std::max<int64_t>(1, (int64_t)(work / std::max(cost, 1e-3)));
 }
 
 // An MvNormal model of a client: its covariance (and mean), factorised in
@@ -123,7 +123,7 @@ struct Model {
 };
 
 struct Req;
-void spread(const erd_serve_request& r, int E, int width, Req* q);
+void spread(const erd_serve_request& r, int E, int width, double work, Req* q);
 
 struct Req {
     uint64_t ticket = 0;
@@ -156,17 +156,108 @@ struct Batch {
 
 // e = ceil(count / chunk) engines (at most E), each taking an equal,
 // packet-aligned share.
-void spread(const erd_serve_request& r, int E, int width, Req* q) {
-    const int64_t chunk = std::max<int64_t>(chunk_for(r), width);  // at least one packet
+void spread(const erd_serve_request& r, int E, int width, double work, Req* q) {
+    const int64_t chunk = std::max<int64_t>(chunk_for(r, work), width);  // at least one packet
     q->e_req = (int)std::min<int64_t>(E, (r.count + chunk - 1) / chunk);
     q->per = round_up((r.count + q->e_req - 1) / q->e_req, width);
 }
+
+// Device blocks for request outputs and batch scratch, recycled by the server
+// rather than allocated and freed through the driver for every request (those
+// calls serialise the launcher, the completer and the callers' releases on the
+// runtime's locks). Sizes are rounded up to 2^k * {4,5,6,7}/4 (at most 25%
+// slack). A block returns to its class's free list once its last user is
+// done: an output at release(), scratch when its batch has completed. Freed
+// blocks beyond `max_cached` bytes go back to the driver.
+class BlockPool final : public Recycler {
+public:
+    explicit BlockPool(size_t max_cached) : max_cached_(max_cached) {}
+    ~BlockPool() { drop_cached(); }
+
+    void* get(size_t bytes) override {
+        size_t size;
+        const int c = size_class(bytes, &size);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            if ((size_t)c < free_.size() && !free_[c].empty()) {
+                void* p = free_[c].back();
+                free_[c].pop_back();
+                cached_ -= size;
+                return p;
+            }
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, size) != hipSuccess) {
+            drop_cached();  // fragmentation: give the cached blocks back and retry once
+            if (hipMalloc(&p, size) != hipSuccess) return nullptr;
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        class_of_[p] = c;
+        return p;
+    }
+
+    void put(void* p) override {
+        if (!p) return;
+        std::unique_lock<std::mutex> lk(mu_);
+        auto it = class_of_.find(p);
+        if (it == class_of_.end()) return;
+        const int c = it->second;
+        const size_t size = class_size(c);
+        if (cached_ + size > max_cached_) {
+            class_of_.erase(it);
+            lk.unlock();
+            (void)hipFree(p);
+            return;
+        }
+        if ((size_t)c >= free_.size()) free_.resize(c + 1);
+        free_[c].push_back(p);
+        cached_ += size;
+    }
+
+private:
+    static size_t class_size(int c) { return ((size_t)4 + (c & 3)) << (c >> 2) >> 2; }
+    static int size_class(size_t bytes, size_t* size) {
+        bytes = std::max<size_t>(bytes, 256);
+        int k = 63 - __builtin_clzll(bytes);  // 2^k <= bytes < 2^(k+1)
+        for (int m = 0; m < 4; ++m) {
+            const int c = 4 * k + m;
+            if (class_size(c) >= bytes) {
+                *size = class_size(c);
+                return c;
+            }
+        }
+        *size = class_size(4 * (k + 1));
+        return 4 * (k + 1);
+    }
+    void drop_cached() {
+        std::vector<void*> drop;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (auto& f : free_) {
+                for (void* p : f) {
+                    class_of_.erase(p);
+                    drop.push_back(p);
+                }
+                f.clear();
+            }
+            cached_ = 0;
+        }
+        for (void* p : drop) (void)hipFree(p);
+    }
+
+    std::mutex mu_;
+    std::unordered_map<void*, int> class_of_;  // every block handed out or cached
+    std::vector<std::vector<void*>> free_;     // cached blocks per class
+    size_t cached_ = 0;
+    size_t max_cached_;
+};
 
 }  // namespace
 
 struct Server {
     erd_server_config cfg{};
     erd_engines* pool = nullptr;
+    std::unique_ptr<BlockPool> blocks;  // outputs and batch scratch
     std::mutex mu;
     std::condition_variable cv;
     std::deque<Req*> pending;
@@ -277,7 +368,7 @@ struct Reloc {
 // checked); everything else is enqueued on the batch's stream.
 int Server::prepare_and_launch(Batch* b) {
     hipStream_t s = b->stream;
-    b->scratch.reset(new Scratch(s));
+    b->scratch.reset(new Scratch(s, blocks.get()));
     Scratch& sc = *b->scratch;
     const int E = cfg.engines_per_client;
     Stage st;
@@ -297,7 +388,7 @@ int Server::prepare_and_launch(Batch* b) {
     std::vector<MvInstance> mv;
     for (Req* q : b->reqs) {
         const erd_serve_request& r = q->r;
-        if (hipMallocAsync(&q->out, q->out_bytes, s) != hipSuccess) return ERD_ERR_OOM;
+        if (!(q->out = blocks->get(q->out_bytes))) return ERD_ERR_OOM;
         if (r.kind == ERD_SERVE_MVNORMAL) {
             Model& m = *q->model;
             const int d = m.d;
@@ -641,15 +732,9 @@ int erd_server_create(erd_server** out, const erd_server_config* cfg) {
     Server& S = srv->s;
     S.cfg = *cfg;
     S.owner.assign((size_t)(cfg->max_clients * cfg->engines_per_client), 0);
-    // Outputs and scratch come from the device's default stream-ordered pool;
-    // keep freed blocks in it instead of returning them to the driver at
-    // every synchronisation, so that steady-state requests never map memory.
-    int dev = 0;
-    hipMemPool_t mp;
-    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&mp, dev) == hipSuccess) {
-        uint64_t keep = ~0ull;
-        (void)hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &keep);
-    }
+    // Outputs and scratch are recycled blocks; up to the HBM budget of freed
+    // blocks stays cached, so steady-state requests allocate nothing.
+    S.blocks.reset(new erd::BlockPool((size_t)std::max<int64_t>(cfg->hbm_budget_bytes, 0)));
     int rc = erd_engines_create_range(&S.pool, 0, 4, cfg->max_clients * cfg->engines_per_client, nullptr);
     if (rc) {
         delete srv;
@@ -708,6 +793,7 @@ int erd_server_submit(erd_server* srv, const erd_serve_request* r, uint64_t* tic
     std::unique_ptr<Req> q(new Req());
     q->r = *r;
     const int E = S.cfg.engines_per_client;
+    const double work = S.cfg.engine_work > 0 ? (double)S.cfg.engine_work : erd::kEngineWork;
     if (r->kind == ERD_SERVE_MVNORMAL) {
         int d = r->dim;
         if (r->model) {
@@ -727,7 +813,7 @@ int erd_server_submit(erd_server* srv, const erd_serve_request* r, uint64_t* tic
         }
         q->r.dim = d;
         if (r->host_mean) q->mean.assign(r->host_mean, r->host_mean + d);
-        erd::spread(q->r, E, 8, q.get());
+        erd::spread(q->r, E, 8, work, q.get());
         q->out_elems = (int64_t)d * q->per * q->e_req;
         q->out_bytes = sizeof(float) * (size_t)q->out_elems;
     } else if (r->kind == ERD_SERVE_MULTINOMIAL) {
@@ -740,7 +826,7 @@ int erd_server_submit(erd_server* srv, const erd_serve_request* r, uint64_t* tic
         }
         if (!std::isfinite(sum)) return ERD_ERR_NONFINITE;
         if (!(sum > 0)) return ERD_ERR_INVALID;
-        erd::spread(*r, E, 8, q.get());
+        erd::spread(*r, E, 8, work, q.get());
         q->out_elems = (int64_t)r->dim * q->per * q->e_req;
         q->out_bytes = sizeof(int32_t) * (size_t)q->out_elems;
     } else {
@@ -757,7 +843,7 @@ int erd_server_submit(erd_server* srv, const erd_serve_request* r, uint64_t* tic
         erd::Segment tmp;
         int rc = erd::make_segment(probe, tmp);
         if (rc) return rc;
-        erd::spread(*r, E, erd::packet_width(r->kind), q.get());
+        erd::spread(*r, E, erd::packet_width(r->kind), work, q.get());
         q->out_elems = q->per * q->e_req;
         q->out_bytes = erd::elem_bytes(r->kind) * (size_t)q->out_elems;
     }
@@ -811,8 +897,9 @@ int erd_server_release(erd_server* srv, uint64_t ticket) {
         S.reqs.erase(it);
     }
     S.cv.notify_all();
-    // The request is complete, so a stream-ordered free needs no ordering.
-    return p ? (hipFreeAsync(p, nullptr) == hipSuccess ? ERD_OK : ERD_ERR_HIP) : ERD_OK;
+    // The request is complete and the caller is done with its output.
+    S.blocks->put(p);
+    return ERD_OK;
 }
 
 int erd_server_stats(erd_server* srv, erd_server_stats_t* st) {
@@ -875,8 +962,9 @@ int erd_server_destroy(erd_server* srv) {
     }
     S.completer.join();
     for (auto& kv : S.reqs)
-        if (kv.second->out) (void)hipFreeAsync(kv.second->out, nullptr);
+        if (kv.second->out) S.blocks->put(kv.second->out);
     (void)hipDeviceSynchronize();
+    S.blocks.reset();
     for (auto& kv : S.models) {
         if (kv.second->hi) (void)hipFree(kv.second->hi);
         if (kv.second->lo) (void)hipFree(kv.second->lo);

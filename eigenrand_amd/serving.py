@@ -24,7 +24,7 @@ class ServerConfig(C.Structure):
     _fields_ = [("max_clients", C.c_int64), ("engines_per_client", C.c_int32),
                 ("max_inflight", C.c_int32), ("hbm_budget_bytes", C.c_int64),
                 ("max_batch_requests", C.c_int64), ("seed_salt", C.c_uint64),
-                ("max_batch_elements", C.c_int64)]
+                ("max_batch_elements", C.c_int64), ("engine_work", C.c_int64)]
 
 
 class ServeRequest(C.Structure):
@@ -53,9 +53,9 @@ def client_seeds(client, engines, seed_salt=0):
     return [(base + 4 * g) & ((1 << 64) - 1) for g in range(engines)]
 
 
-def chunk_for(kind, a=0.0, b=0.0, dim=0):
+def chunk_for(kind, a=0.0, b=0.0, dim=0, engine_work=0):
     """Outputs (samples for MvNormal/multinomial) one engine takes; mirrors
-    chunk_for in serve.hip."""
+    chunk_for in serve.hip (engine_work 0: the default)."""
     if isinstance(kind, str):
         kind = _lib.DIST[kind]
     cost = 1.0
@@ -77,12 +77,13 @@ def chunk_for(kind, a=0.0, b=0.0, dim=0):
         cost = float(dim)
     elif kind == MULTINOMIAL:
         cost = 1.0 + a / 8 + dim / 8 if a < max(30.0 * (dim - 1), 100.0) else 12.0 * dim
-    return max(1, int(ENGINE_WORK / max(cost, 1e-3)))
+    work = float(engine_work) if engine_work > 0 else ENGINE_WORK
+    return max(1, int(work / max(cost, 1e-3)))
 
 
-def spread(kind, count, engines_per_client, width, a=0.0, b=0.0, dim=0):
+def spread(kind, count, engines_per_client, width, a=0.0, b=0.0, dim=0, engine_work=0):
     """(engines, per-engine outputs or samples) a request is spread over."""
-    chunk = max(chunk_for(kind, a, b, dim), width)
+    chunk = max(chunk_for(kind, a, b, dim, engine_work), width)
     e = int(min(engines_per_client, -(-count // chunk)))
     per = -(-count // e)
     per = -(-per // width) * width
@@ -104,9 +105,10 @@ def kind_layout(kind):
 class Server:
     def __init__(self, max_clients=64, engines_per_client=256, max_inflight=8,
                  hbm_budget_bytes=64 << 30, max_batch_requests=256, seed_salt=0,
-                 max_batch_elements=0):
+                 max_batch_elements=0, engine_work=0):
         self.cfg = ServerConfig(max_clients, engines_per_client, max_inflight, hbm_budget_bytes,
-                                max_batch_requests, seed_salt & ((1 << 64) - 1), max_batch_elements)
+                                max_batch_requests, seed_salt & ((1 << 64) - 1), max_batch_elements,
+                                engine_work)
         h = C.c_void_p()
         _lib.call("erd_server_create", C.byref(h), C.byref(self.cfg))
         self._h = h

@@ -329,6 +329,8 @@ typedef struct erd_server_config {
     int64_t max_batch_requests;
     uint64_t seed_salt;
     int64_t max_batch_elements;  /* cap on one batch's outputs (0: none); a batch takes >= 1 request */
+    int64_t engine_work;         /* normal-equivalent samples per engine when a request is spread
+                                    over its client's engines (0: 65536) */
 } erd_server_config;
 
 typedef struct erd_serve_request {
