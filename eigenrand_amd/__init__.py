@@ -29,6 +29,8 @@ __all__ = [
     "randbits_u64", "uniform_real", "uniform_real_f64", "normal", "normal_vop",
     "exponential", "cauchy", "student_t", "gamma", "chi_squared", "poisson", "binomial",
     "lognormal", "weibull", "extreme_value",
+    "bernoulli", "balanced", "geometric", "randbits_i32", "uniform_real_vop", "balanced_vop", "bernoulli_vop",
+    "exponential_vop", "cauchy_vop", "lognormal_vop", "student_t_vop", "weibull_vop", "extreme_value_vop",
     "binomial_vop", "gamma_vop", "poisson_vop", "run_requests", "cholesky_batch",
     "mvnormal_batch", "wishart_batch", "dirichlet_batch", "MvNormalGen", "WishartGen",
     "MvFactors", "MvNormalBatch", "WishartBatch",
@@ -335,6 +337,90 @@ def binomial(rows, cols, urng, trials=1, p=0.5, out=None, stream=None):
     """Rand::binomial<MatrixXi>(rows, cols, urng, trials, p) (int32)."""
     return _fill("erd_fill_binomial_i32", urng, rows, cols, _torch().int32, out, stream, int(trials),
                  float(p))
+
+
+def bernoulli(rows, cols, urng, p=0.5, out=None, stream=None, dtype=None):
+    """Rand::bernoulli<MatrixXf>(rows, cols, urng, p) (BernoulliGen): float32
+    1/0 from full draws; dtype=torch.int32 gives the int32 (Packet4i, half
+    draw) form."""
+    torch = _torch()
+    if dtype is torch.int32:
+        return _fill("erd_fill_bernoulli_i32", urng, rows, cols, torch.int32, out, stream, float(p))
+    return _fill("erd_fill_bernoulli_f32", urng, rows, cols, torch.float32, out, stream, float(p))
+
+
+def balanced(rows, cols, urng, a=-1.0, b=1.0, out=None, stream=None):
+    """Rand::balanced<MatrixXf>(rows, cols, urng[, a, b]) (BalancedGen /
+    Balanced2Gen): reals in [a, b]."""
+    return _fill("erd_fill_balanced_f32", urng, rows, cols, _torch().float32, out, stream, float(a), float(b))
+
+
+def geometric(rows, cols, urng, p=0.5, out=None, stream=None):
+    """Rand::geometric<MatrixXi>(rows, cols, urng, p) (GeometricGen, int32)."""
+    return _fill("erd_fill_geometric_i32", urng, rows, cols, _torch().int32, out, stream, float(p))
+
+
+def randbits_i32(rows, cols, urng, out=None, stream=None):
+    """Rand::randBits<MatrixXi>(rows, cols, urng) (RandbitsGen<int32_t>)."""
+    return _fill("erd_fill_randbits_i32", urng, rows, cols, _torch().int32, out, stream)
+
+
+def _vop(name, urng, params, out, stream):
+    """Vectorize-over-params fill: params are float32 CUDA tensors of
+    n*per_engine elements (engine g uses its block)."""
+    torch = _torch()
+    params = [p.contiguous() for p in params]
+    if any(p.numel() != params[0].numel() for p in params) or params[0].numel() % len(urng):
+        raise InvalidArgument(name, 1)
+    per = params[0].numel() // len(urng)
+    buf = _alloc(urng, per, torch.float32, out)
+    _lib.call(name, urng.handle, _ptr(buf), per, *[_ptr(p) for p in params], _stream_ptr(stream))
+    return buf
+
+
+def uniform_real_vop(urng, a, b, out=None, stream=None):
+    """Rand::uniformReal(urng, a, b) with per-element bounds (UniformRealVGen)."""
+    return _vop("erd_fill_uniform_vop_f32", urng, [a, b], out, stream)
+
+
+def balanced_vop(urng, a, b, out=None, stream=None):
+    """Rand::balanced(urng, a, b) with per-element bounds (BalancedVGen)."""
+    return _vop("erd_fill_balanced_vop_f32", urng, [a, b], out, stream)
+
+
+def bernoulli_vop(urng, p, out=None, stream=None):
+    """Rand::bernoulli(urng, p_array) (BernoulliVGen<float>)."""
+    return _vop("erd_fill_bernoulli_vop_f32", urng, [p], out, stream)
+
+
+def exponential_vop(urng, lmbda, out=None, stream=None):
+    """Rand::exponential(urng, lambda_array) (ExponentialVGen)."""
+    return _vop("erd_fill_exponential_vop_f32", urng, [lmbda], out, stream)
+
+
+def cauchy_vop(urng, a, b, out=None, stream=None):
+    """Rand::cauchy(urng, a, b) with per-element parameters (CauchyVGen)."""
+    return _vop("erd_fill_cauchy_vop_f32", urng, [a, b], out, stream)
+
+
+def lognormal_vop(urng, mean, stdev, out=None, stream=None):
+    """Rand::lognormal(urng, mean, stdev) with per-element parameters (LognormalVGen)."""
+    return _vop("erd_fill_lognormal_vop_f32", urng, [mean, stdev], out, stream)
+
+
+def student_t_vop(urng, n, out=None, stream=None):
+    """Rand::studentT(urng, n_array) (StudentTVGen)."""
+    return _vop("erd_fill_studentt_vop_f32", urng, [n], out, stream)
+
+
+def weibull_vop(urng, a, b, out=None, stream=None):
+    """Rand::weibull(urng, a, b) with per-element parameters (WeibullVGen)."""
+    return _vop("erd_fill_weibull_vop_f32", urng, [a, b], out, stream)
+
+
+def extreme_value_vop(urng, a, b, out=None, stream=None):
+    """Rand::extremeValue(urng, a, b) with per-element parameters (ExtremeValueVGen)."""
+    return _vop("erd_fill_extreme_value_vop_f32", urng, [a, b], out, stream)
 
 
 def binomial_vop(urng, trials, p, out=None, stream=None):

@@ -74,6 +74,20 @@ _SIGS = {
     "erd_fill_binomial_vop_i32": [P, P, I64, P, P, P],
     "erd_fill_gamma_vop_f32": [P, P, I64, P, P, P],
     "erd_fill_poisson_vop_i32": [P, P, I64, P, P],
+    "erd_fill_bernoulli_f32": [P, P, I64, F64, P],
+    "erd_fill_bernoulli_vop_f32": [P, P, I64, P, P],
+    "erd_fill_balanced_f32": [P, P, I64, F32, F32, P],
+    "erd_fill_balanced_vop_f32": [P, P, I64, P, P, P],
+    "erd_fill_uniform_vop_f32": [P, P, I64, P, P, P],
+    "erd_fill_exponential_vop_f32": [P, P, I64, P, P],
+    "erd_fill_cauchy_vop_f32": [P, P, I64, P, P, P],
+    "erd_fill_lognormal_vop_f32": [P, P, I64, P, P, P],
+    "erd_fill_studentt_vop_f32": [P, P, I64, P, P],
+    "erd_fill_weibull_vop_f32": [P, P, I64, P, P, P],
+    "erd_fill_extreme_value_vop_f32": [P, P, I64, P, P, P],
+    "erd_fill_bernoulli_i32": [P, P, I64, F64, P],
+    "erd_fill_geometric_i32": [P, P, I64, F64, P],
+    "erd_fill_randbits_i32": [P, P, I64, P],
     "erd_generator_create": [C.POINTER(P), C.c_int32, F64, F64, I64, P],
     "erd_generator_destroy": [P],
     "erd_generator_reset": [P, P],

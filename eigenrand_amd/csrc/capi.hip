@@ -484,6 +484,101 @@ int erd_fill_poisson_vop_i32(erd_engines* e, int32_t* out, int64_t per, const fl
                                                        S(stream)));
 }
 
+// ------------------------------------------------ raw-bit / VoP float fills ---
+int erd_fill_bernoulli_f32(erd_engines* e, float* out, int64_t per, double p, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    if (!(p >= 0 && p <= 1)) return ERD_ERR_INVALID;  // eigen_assert(0 <= _p && _p <= 1)
+    const int32_t pi = (int32_t)(uint32_t)(p * 2147483648.0);
+    return hip_status(erd::launch_fill_bernoulli_f32(e->states, e->count, out, per, pi, nullptr, S(stream)));
+}
+
+int erd_fill_bernoulli_vop_f32(erd_engines* e, float* out, int64_t per, const float* p, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    if (per > 0 && !p) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_fill_bernoulli_f32(e->states, e->count, out, per, 0, p, S(stream)));
+}
+
+int erd_fill_balanced_f32(erd_engines* e, float* out, int64_t per, float a, float b, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_balanced_f32(e->states, e->count, out, per, a, b, nullptr, nullptr,
+                                                    S(stream)));
+}
+
+int erd_fill_balanced_vop_f32(erd_engines* e, float* out, int64_t per, const float* a, const float* b,
+                              void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    if (per > 0 && (!a || !b)) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_fill_balanced_f32(e->states, e->count, out, per, 0.f, 0.f, a, b, S(stream)));
+}
+
+static int vop_fill(erd_engines* e, int kind, float* out, int64_t per, const float* a, const float* b,
+                    bool two, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    if (per > 0 && (!a || (two && !b))) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_fill_vop_f32(e->states, e->count, kind, out, per, a, b, S(stream)));
+}
+
+int erd_fill_uniform_vop_f32(erd_engines* e, float* out, int64_t per, const float* a, const float* b,
+                             void* stream) {
+    return vop_fill(e, erd::VOP_UNIFORM, out, per, a, b, true, stream);
+}
+
+int erd_fill_exponential_vop_f32(erd_engines* e, float* out, int64_t per, const float* lambda, void* stream) {
+    return vop_fill(e, erd::VOP_EXPONENTIAL, out, per, lambda, nullptr, false, stream);
+}
+
+int erd_fill_cauchy_vop_f32(erd_engines* e, float* out, int64_t per, const float* a, const float* b,
+                            void* stream) {
+    return vop_fill(e, erd::VOP_CAUCHY, out, per, a, b, true, stream);
+}
+
+int erd_fill_lognormal_vop_f32(erd_engines* e, float* out, int64_t per, const float* mean, const float* sd,
+                               void* stream) {
+    return vop_fill(e, erd::VOP_LOGNORMAL, out, per, mean, sd, true, stream);
+}
+
+int erd_fill_studentt_vop_f32(erd_engines* e, float* out, int64_t per, const float* n, void* stream) {
+    return vop_fill(e, erd::VOP_STUDENT_T, out, per, n, nullptr, false, stream);
+}
+
+int erd_fill_weibull_vop_f32(erd_engines* e, float* out, int64_t per, const float* a, const float* b,
+                             void* stream) {
+    return vop_fill(e, erd::VOP_WEIBULL, out, per, a, b, true, stream);
+}
+
+int erd_fill_extreme_value_vop_f32(erd_engines* e, float* out, int64_t per, const float* a, const float* b,
+                                   void* stream) {
+    return vop_fill(e, erd::VOP_EXTREME_VALUE, out, per, a, b, true, stream);
+}
+
+// ------------------------------------------------------ half-draw int32 fills ---
+int erd_fill_bernoulli_i32(erd_engines* e, int32_t* out, int64_t per, double p, void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    if (!(p >= 0 && p <= 1)) return ERD_ERR_INVALID;
+    const int32_t pi = (int32_t)(uint32_t)(p * 2147483648.0);
+    return hip_status(erd::launch_fill_bernoulli_i32(e->states, e->count, out, per, pi, S(stream)));
+}
+
+int erd_fill_geometric_i32(erd_engines* e, int32_t* out, int64_t per, double p, void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    if (!(p >= 0 && p <= 1)) return ERD_ERR_INVALID;
+    const float rlog_q = (float)(1 / std::log(1 - p));  // GeometricGen's constructor, in double
+    return hip_status(erd::launch_fill_geometric_i32(e->states, e->count, out, per, rlog_q, S(stream)));
+}
+
+int erd_fill_randbits_i32(erd_engines* e, int32_t* out, int64_t per, void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_randbits_i32(e->states, e->count, out, per, S(stream)));
+}
+
 }  // extern "C"
 
 namespace erd {

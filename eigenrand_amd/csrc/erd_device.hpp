@@ -160,6 +160,17 @@ __device__ __forceinline__ float ur_f32(uint32_t w) {
 __device__ __forceinline__ float nzur_f32(uint32_t w) {
     return ur_f32(w) + 1.1920928955078125e-07f / 8;
 }
+// RandUtils::zero_to_one at Packet8f (arch/AVX/RandUtils.h:200-204):
+// cvtepi32_ps(w & 0x7FFFFFFF) / float(0x7FFFFFFF); the divisor rounds to 2^31,
+// so the quotient is the converted integer scaled exactly.
+__device__ __forceinline__ float zero_to_one_f32(uint32_t w) {
+    return (float)(int32_t)(w & 0x7FFFFFFFu) / 2147483648.0f;
+}
+// x86 cvttps2dq: truncation, with NaN and out-of-range inputs giving INT32_MIN.
+__device__ __forceinline__ int32_t cvtt_x86(float f) {
+    if (!(f < 2147483648.0f) || !(f >= -2147483648.0f)) return (int32_t)0x80000000u;
+    return (int32_t)f;
+}
 // bit_to_ur_double: MorePacketMath.h:217-227.
 __device__ __forceinline__ double ur_f64(uint64_t w) {
     return __longlong_as_double((long long)((w & 0xFFFFFFFFFFFFFull) | (1023ull << 52))) - 1.0;

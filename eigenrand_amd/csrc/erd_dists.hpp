@@ -88,7 +88,8 @@ struct P8Fill {
 // give u1 = 1 - U(A), u2 = U(B); the first packet is r*sin, the cached second
 // packet r*cos. A fill of 8 mod 16 outputs ends with an unused cos packet.
 template <int MODE>  // 0 = StdNormalGen, 1 = NormalGen(mean, sd), 2 = NormalVGen,
-                     // 3 = LognormalGen(mean, sd) = pexp of NormalGen (NormalExp.h:171-210)
+                     // 3 = LognormalGen(mean, sd) = pexp of NormalGen (NormalExp.h:171-210),
+                     // 4 = LognormalVGen = pexp of NormalVGen (NormalExp.h:229-233)
 struct NormalF32 {
     static constexpr int CALLS = 2;
     float* out;
@@ -98,7 +99,7 @@ struct NormalF32 {
     GenState* gs;  // reused generator: receives the cos packet of a trailing half unit
     __device__ void rebase(int64_t off) {
         out += off;
-        if (MODE == 2) { vmean += off; vsd += off; }
+        if (MODE == 2 || MODE == 4) { vmean += off; vsd += off; }
     }
     __device__ static int64_t units(int64_t n) { return (n + 15) / 16; }
     __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
@@ -129,11 +130,15 @@ struct NormalF32 {
                 sn[h] = pexp(sn[h] * sd + mean);
                 cs[h] = pexp(cs[h] * sd + mean);
             }
-        } else if (MODE == 2) {
+        } else if (MODE == 2 || MODE == 4) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 sn[h] = sn[h] * vsd[base + h] + vmean[base + h];
                 if (full) cs[h] = cs[h] * vsd[base + 8 + h] + vmean[base + 8 + h];
+                if (MODE == 4) {
+                    sn[h] = pexp(sn[h]);
+                    if (full) cs[h] = pexp(cs[h]);
+                }
             }
         }
         *reinterpret_cast<float2*>(out + base) = make_float2(sn[0], sn[1]);
@@ -230,6 +235,178 @@ struct StudentTF32 {  // NormalExp.h:285-301 (sine from the reference's psincos)
             v[h] = radius * s;
         }
         *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
+    }
+};
+
+// ------------------------------------------- raw-bit and VoP float units ---
+// One full draw per 8 outputs; float j of the draw comes from 32-bit word j.
+// VoP units read per-element parameters laid out like the outputs.
+
+// BernoulliGen<float>::packetOp (Basic.h:693-707): (w & 0x7FFFFFFF) < p with
+// p = uint32(p * 2^31) compared as signed int32 lanes (so p = 1 gives 0, as the
+// packet path does); BernoulliVGen<float> (Basic.h:722-733) converts
+// p * 2^31 with cvttps2dq per element (726-733).
+template <int VOP>
+struct BernoulliF32 {
+    static constexpr int CALLS = 1;
+    float* out;
+    int32_t p;
+    const float* vp;
+    __device__ void rebase(int64_t off) {
+        out += off;
+        if (VOP) vp += off;
+    }
+    __device__ static int64_t units(int64_t n) { return n / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        const int64_t i = 8 * u + 2 * l;
+        float v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int32_t r = (int32_t)((h ? hi32(w[0]) : lo32(w[0])) & 0x7FFFFFFFu);
+            const int32_t t = VOP ? cvtt_x86(vp[i + h] * 2147483648.0f) : p;
+            v[h] = r < t ? 1.0f : 0.0f;
+        }
+        *reinterpret_cast<float2*>(out + i) = make_float2(v[0], v[1]);
+    }
+};
+
+// Balanced2Gen (Basic.h:437-470, RandUtils.h:42-45): zero_to_one * slope +
+// bias, slope = b - a and bias = a in float; BalancedGen's z*2 - 1 is the
+// same arithmetic for (a, b) = (-1, 1). VOP: BalancedVGen (Basic.h:487-493),
+// z * (b - a) + a per element.
+template <int VOP>
+struct BalancedF32 {
+    static constexpr int CALLS = 1;
+    float* out;
+    float slope, bias;
+    const float* va;
+    const float* vb;
+    __device__ void rebase(int64_t off) {
+        out += off;
+        if (VOP) { va += off; vb += off; }
+    }
+    __device__ static int64_t units(int64_t n) { return n / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        const int64_t i = 8 * u + 2 * l;
+        float v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float z = zero_to_one_f32(h ? hi32(w[0]) : lo32(w[0]));
+            v[h] = VOP ? z * (vb[i + h] - va[i + h]) + va[i + h] : z * slope + bias;
+        }
+        *reinterpret_cast<float2*>(out + i) = make_float2(v[0], v[1]);
+    }
+};
+
+// Vectorize-over-params units of the single-draw float generators:
+//   0 UniformRealVGen   (Basic.h:653-660)      U * (b - a) + a
+//   1 ExponentialVGen   (NormalExp.h:418-424)  -(log(1 - U) / a)
+//   2 CauchyVGen        (NormalExp.h:875-884)  a + b * tan(pi * (U - 0.5))
+//   3 WeibullVGen       (NormalExp.h:675-681)  exp(log(-log(1 - U)) * (1 / a)) * b
+//   4 ExtremeValueVGen  (NormalExp.h:752-760)  a - log(-log(U + eps/8)) * b
+template <int KIND>
+struct VopF32 {
+    static constexpr int CALLS = 1;
+    float* out;
+    const float* va;
+    const float* vb;
+    __device__ void rebase(int64_t off) {
+        out += off;
+        va += off;
+        if (KIND != 1) vb += off;
+    }
+    __device__ static int64_t units(int64_t n) { return n / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        const int64_t i = 8 * u + 2 * l;
+        float v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t x = h ? hi32(w[0]) : lo32(w[0]);
+            const float a = va[i + h];
+            if (KIND == 0) {
+                v[h] = ur_f32(x) * (vb[i + h] - a) + a;
+            } else if (KIND == 1) {
+                v[h] = -(plog_pos(1.0f - ur_f32(x)) / a);
+            } else if (KIND == 2) {
+                float sn, cs;
+                psincos(ERD_F(kPi) * (ur_f32(x) - 0.5f), sn, cs);
+                v[h] = a + vb[i + h] * (sn / cs);
+            } else if (KIND == 3) {
+                v[h] = pexp(plog(-plog(1.0f - ur_f32(x))) * (1.0f / a)) * vb[i + h];
+            } else {
+                v[h] = a - plog(-plog(nzur_f32(x))) * vb[i + h];
+            }
+        }
+        *reinterpret_cast<float2*>(out + i) = make_float2(v[0], v[1]);
+    }
+};
+
+// StudentTVGen (NormalExp.h:332-345): StudentTGen with per-element n and -2/n.
+struct StudentTVopF32 {
+    static constexpr int CALLS = 2;
+    float* out;
+    const float* vn;
+    __device__ void rebase(int64_t off) {
+        out += off;
+        vn += off;
+    }
+    __device__ static int64_t units(int64_t n_) { return n_ / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        const int64_t i = 8 * u + 2 * l;
+        float v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float n = vn[i + h];
+            const float u1 = 1.0f - ur_f32(h ? hi32(w[0]) : lo32(w[0]));
+            const float u2 = ur_f32(h ? hi32(w[1]) : lo32(w[1]));
+            const float radius = sqrtf(n * (pexp(plog(u1) * (-2.0f / n)) - 1.0f));
+            float s, c;
+            psincos(ERD_F(2 * kPi) * u2, s, c);
+            v[h] = radius * s;
+        }
+        *reinterpret_cast<float2*>(out + i) = make_float2(v[0], v[1]);
+    }
+};
+
+// ------------------------------------------------------ half-draw int32 ---
+// Packet4i consumers fed by half() (one half draw per 4 outputs, lane j =
+// 32-bit word j of the half); emit(i, x) writes outputs i..i+3 from the
+// half's tempered words x[0], x[1].
+
+// BernoulliGen<int32_t>::packetOp (Basic.h:693-707) at Packet4i.
+struct BernoulliI32 {
+    int32_t* out;
+    int32_t p;
+    __device__ void emit(int64_t i, const uint64_t* x) const {
+        int4 r;
+        r.x = (int32_t)(lo32(x[0]) & 0x7FFFFFFFu) < p ? 1 : 0;
+        r.y = (int32_t)(hi32(x[0]) & 0x7FFFFFFFu) < p ? 1 : 0;
+        r.z = (int32_t)(lo32(x[1]) & 0x7FFFFFFFu) < p ? 1 : 0;
+        r.w = (int32_t)(hi32(x[1]) & 0x7FFFFFFFu) < p ? 1 : 0;
+        *reinterpret_cast<int4*>(out + i) = r;
+    }
+};
+
+// GeometricGen::packetOp (Discrete.h:1372-1380): cvtt(trunc(log(1 - U) *
+// rlog_q)) with U from the half draw and rlog_q = 1 / log(1 - p) in float.
+struct GeometricI32 {
+    int32_t* out;
+    float rlog_q;
+    __device__ int32_t one(uint32_t w) const {
+        return cvtt_x86(truncf(plog(1.0f - ur_f32(w)) * rlog_q));
+    }
+    __device__ void emit(int64_t i, const uint64_t* x) const {
+        *reinterpret_cast<int4*>(out + i) =
+            make_int4(one(lo32(x[0])), one(hi32(x[0])), one(lo32(x[1])), one(hi32(x[1])));
+    }
+};
+
+// RandbitsGen<int32_t>::packetOp (Basic.h:394-400) at Packet4i: the half's bits.
+struct RandbitsI32 {
+    int32_t* out;
+    __device__ void emit(int64_t i, const uint64_t* x) const {
+        *reinterpret_cast<int4*>(out + i) =
+            make_int4((int32_t)lo32(x[0]), (int32_t)hi32(x[0]), (int32_t)lo32(x[1]), (int32_t)hi32(x[1]));
     }
 };
 

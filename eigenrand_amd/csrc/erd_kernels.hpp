@@ -53,6 +53,30 @@ hipError_t launch_fill_extreme_value_f32(EngineState* st, int64_t n, float* out,
                                          float b, hipStream_t s);
 hipError_t launch_fill_gamma_fixed_f32(EngineState* st, int64_t n, float* out, int64_t per,
                                        float alpha, float beta, hipStream_t s);
+// BernoulliGen<float> (p = uint32(p * 2^31) as int32) or BernoulliVGen (vp != NULL).
+hipError_t launch_fill_bernoulli_f32(EngineState* st, int64_t n, float* out, int64_t per, int32_t p,
+                                     const float* vp, hipStream_t s);
+// Balanced2Gen(a, b), or BalancedVGen with per-element a, b (va != NULL).
+hipError_t launch_fill_balanced_f32(EngineState* st, int64_t n, float* out, int64_t per, float a, float b,
+                                    const float* va, const float* vb, hipStream_t s);
+// Vectorize-over-params float fills with one or two per-element parameters.
+enum VopKind : int {
+    VOP_UNIFORM = 0,        // a, b
+    VOP_EXPONENTIAL = 1,    // lambda
+    VOP_CAUCHY = 2,         // a, b
+    VOP_WEIBULL = 3,        // a, b
+    VOP_EXTREME_VALUE = 4,  // a, b
+    VOP_STUDENT_T = 5,      // n
+    VOP_LOGNORMAL = 6,      // mean, sd
+};
+hipError_t launch_fill_vop_f32(EngineState* st, int64_t n, int kind, float* out, int64_t per, const float* va,
+                               const float* vb, hipStream_t s);
+// Half-draw int32 fills (Packet4i through half()).
+hipError_t launch_fill_bernoulli_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, int32_t p,
+                                     hipStream_t s);
+hipError_t launch_fill_geometric_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, float rlog_q,
+                                     hipStream_t s);
+hipError_t launch_fill_randbits_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, hipStream_t s);
 
 // fill_reject.hip (CompressMask rejection samplers); gs: optional per-engine
 // queues of a reused generator object.

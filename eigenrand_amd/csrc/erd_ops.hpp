@@ -381,6 +381,28 @@ __device__ __forceinline__ void cursor_end(Eng& e, const HalfCursor& c) {
     half_end(e, c.pos, c.nc, c.used);
 }
 
+// Fixed half-draw consumers (one half() per Packet4i): packet k of the fill
+// takes virtual slot k (the engine's cached half first, if any), 64 packets
+// per wave step; the last odd half leaves the draw's upper half cached.
+template <class D>
+__device__ void op_half_fixed(Eng& e, uint64_t* carry, int64_t T, const D& d, int lane) {
+    if (T == 0) return;
+    HalfCursor cur = cursor_begin(e, carry, lane);
+    int64_t done = 0;
+    while (done < T) {
+        cur.ensure(1, lane);
+        const int64_t take = min((int64_t)cur.avail(), T - done);
+        for (int v = lane; v < take; v += 64) {
+            const uint64_t x[2] = {cur.word(v, 0), cur.word(v, 1)};
+            d.emit(4 * (done + v), x);
+        }
+        __syncthreads();  // carried slots are read before ensure() rewrites them
+        cur.advance((int)take);
+        done += take;
+    }
+    cursor_end(e, cur);
+}
+
 // Knuth loop of PoissonGen::packetOp (Discrete.h:860-873); returns lane j's count.
 __device__ __forceinline__ int32_t knuth_packet(HalfCursor& cur, float thr, int lane,
                                                 uint32_t* status) {

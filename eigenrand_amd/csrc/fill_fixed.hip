@@ -209,4 +209,77 @@ hipError_t launch_fill_gamma_fixed_f32(EngineState* st, int64_t n, float* out, i
     }
 }
 
+// ------------------------------------------- raw-bit and VoP float fills ---
+hipError_t launch_fill_bernoulli_f32(EngineState* st, int64_t n, float* out, int64_t per, int32_t p,
+                                     const float* vp, hipStream_t s) {
+    if (vp) {
+        BernoulliF32<1> d{out, 0, vp};
+        return launch_fixed(st, n, per, d, s);
+    }
+    BernoulliF32<0> d{out, p, nullptr};
+    return launch_fixed(st, n, per, d, s);
+}
+
+hipError_t launch_fill_balanced_f32(EngineState* st, int64_t n, float* out, int64_t per, float a, float b,
+                                    const float* va, const float* vb, hipStream_t s) {
+    if (va) {
+        BalancedF32<1> d{out, 0.f, 0.f, va, vb};
+        return launch_fixed(st, n, per, d, s);
+    }
+    BalancedF32<0> d{out, b - a, a, nullptr, nullptr};
+    return launch_fixed(st, n, per, d, s);
+}
+
+hipError_t launch_fill_vop_f32(EngineState* st, int64_t n, int kind, float* out, int64_t per, const float* va,
+                               const float* vb, hipStream_t s) {
+    switch (kind) {
+        case VOP_UNIFORM: return launch_fixed(st, n, per, VopF32<0>{out, va, vb}, s);
+        case VOP_EXPONENTIAL: return launch_fixed(st, n, per, VopF32<1>{out, va, nullptr}, s);
+        case VOP_CAUCHY: return launch_fixed(st, n, per, VopF32<2>{out, va, vb}, s);
+        case VOP_WEIBULL: return launch_fixed(st, n, per, VopF32<3>{out, va, vb}, s);
+        case VOP_EXTREME_VALUE: return launch_fixed(st, n, per, VopF32<4>{out, va, vb}, s);
+        case VOP_STUDENT_T: return launch_fixed(st, n, per, StudentTVopF32{out, va}, s);
+        case VOP_LOGNORMAL:
+            return launch_fixed(st, n, per, NormalF32<4>{out, 0.f, 1.f, va, vb, nullptr}, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// ----------------------------------------------------- half-draw int32 fills ---
+template <class D>
+__global__ __launch_bounds__(64) void k_fill_half(EngineState* __restrict__ states, int32_t* __restrict__ out,
+                                                  int64_t per, D d) {
+    __shared__ uint64_t mt[kWords];
+    __shared__ uint64_t carry[2];
+    const int lane = threadIdx.x;
+    const int64_t g = blockIdx.x;
+    if (per / 4 == 0) return;
+    Eng e;
+    eng_load(e, mt, states + g, lane);
+    d.out = out + g * per;
+    op_half_fixed(e, carry, per / 4, d, lane);
+    eng_store(e, states + g, lane);
+}
+
+template <class D>
+static hipError_t launch_half(EngineState* st, int64_t n, int32_t* out, int64_t per, const D& d, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_half<D>, dim3((unsigned)n), dim3(64), 0, s, st, out, per, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_bernoulli_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, int32_t p,
+                                     hipStream_t s) {
+    return launch_half(st, n, out, per, BernoulliI32{nullptr, p}, s);
+}
+
+hipError_t launch_fill_geometric_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, float rlog_q,
+                                     hipStream_t s) {
+    return launch_half(st, n, out, per, GeometricI32{nullptr, rlog_q}, s);
+}
+
+hipError_t launch_fill_randbits_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, hipStream_t s) {
+    return launch_half(st, n, out, per, RandbitsI32{nullptr}, s);
+}
+
 }  // namespace erd

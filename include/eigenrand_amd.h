@@ -139,7 +139,54 @@ int erd_fill_gamma_f32(erd_engines* e, float* out, int64_t per_engine, float alp
 /* chiSquared<float>(…, n) = GammaGen(n*0.5, 2) (NormalExp.h:767-802). */
 int erd_fill_chisquared_f32(erd_engines* e, float* out, int64_t per_engine, float n, void* stream);
 
+/* bernoulli<float>(…, p): BernoulliGen (Basic.h:670-708) at Packet8f, one draw
+ * per 8 outputs of 1.0f/0.0f; 0 <= p <= 1. The packet path compares 31 random
+ * bits with uint32(p * 2^31) as signed int32 lanes, so p = 1 gives zeros. */
+int erd_fill_bernoulli_f32(erd_engines* e, float* out, int64_t per_engine, double p, void* stream);
+/* bernoulli(urng, p_array): BernoulliVGen<float> (Basic.h:711-734). */
+int erd_fill_bernoulli_vop_f32(erd_engines* e, float* out, int64_t per_engine, const float* p,
+                               void* stream);
+/* balanced<float>(…) = (a, b) = (-1, 1): BalancedGen (Basic.h:408-430);
+ * balanced<float>(…, a, b): Balanced2Gen (Basic.h:437-470), zero_to_one *
+ * (b - a) + a with zero_to_one = (w & 0x7FFFFFFF) / 0x7FFFFFFF in float. */
+int erd_fill_balanced_f32(erd_engines* e, float* out, int64_t per_engine, float a, float b, void* stream);
+/* balanced(urng, a_array, b_array): BalancedVGen (Basic.h:472-495). */
+int erd_fill_balanced_vop_f32(erd_engines* e, float* out, int64_t per_engine, const float* a,
+                              const float* b, void* stream);
+/* Vectorize-over-params forms of the single-draw generators; parameters are
+ * device arrays of n*per_engine elements laid out like the output.
+ *   uniformReal(urng, a, b):      UniformRealVGen  (Basic.h:638-661)
+ *   exponential(urng, lambda):    ExponentialVGen  (NormalExp.h:402-425)
+ *   cauchy(urng, a, b):           CauchyVGen       (NormalExp.h:859-885)
+ *   lognormal(urng, mean, stdev): LognormalVGen    (NormalExp.h:213-234), pexp of NormalVGen
+ *   studentT(urng, n):            StudentTVGen     (NormalExp.h:305-346)
+ *   weibull(urng, a, b):          WeibullVGen      (NormalExp.h:659-682)
+ *   extremeValue(urng, a, b):     ExtremeValueVGen (NormalExp.h:736-761) */
+int erd_fill_uniform_vop_f32(erd_engines* e, float* out, int64_t per_engine, const float* a, const float* b,
+                             void* stream);
+int erd_fill_exponential_vop_f32(erd_engines* e, float* out, int64_t per_engine, const float* lambda,
+                                 void* stream);
+int erd_fill_cauchy_vop_f32(erd_engines* e, float* out, int64_t per_engine, const float* a, const float* b,
+                            void* stream);
+int erd_fill_lognormal_vop_f32(erd_engines* e, float* out, int64_t per_engine, const float* mean,
+                               const float* stdev, void* stream);
+int erd_fill_studentt_vop_f32(erd_engines* e, float* out, int64_t per_engine, const float* n, void* stream);
+int erd_fill_weibull_vop_f32(erd_engines* e, float* out, int64_t per_engine, const float* a, const float* b,
+                             void* stream);
+int erd_fill_extreme_value_vop_f32(erd_engines* e, float* out, int64_t per_engine, const float* a,
+                                   const float* b, void* stream);
+
 /* ------------------------------------------------------ integer (int32) --- */
+/* bernoulli<int32>(…, p): BernoulliGen (Basic.h:670-708) at Packet4i, one
+ * half() draw per 4 outputs of 1/0; 0 <= p <= 1. */
+int erd_fill_bernoulli_i32(erd_engines* e, int32_t* out, int64_t per_engine, double p, void* stream);
+/* geometric<int32>(…, p): GeometricGen (Discrete.h:1339-1381), trunc(log(1 -
+ * U) * (1 / log(1 - p))) from a half() draw per 4 outputs, converted as
+ * cvttps2dq does (out of range: INT32_MIN); 0 <= p <= 1. */
+int erd_fill_geometric_i32(erd_engines* e, int32_t* out, int64_t per_engine, double p, void* stream);
+/* randBits<int32>(…): RandbitsGen (Basic.h:378-401) at Packet4i, the bits of
+ * one half() draw per 4 outputs. */
+int erd_fill_randbits_i32(erd_engines* e, int32_t* out, int64_t per_engine, void* stream);
 /* poisson<int32>(…, mean): PoissonGen (Discrete.h:788-904). mean >= 0. */
 int erd_fill_poisson_i32(erd_engines* e, int32_t* out, int64_t per_engine, double mean, void* stream);
 /* binomial<int32>(…, trials, p): BinomialGen (Discrete.h:911-1053). */

@@ -631,6 +631,126 @@ void orc_fill_extreme_value_f32(void* states, int64_t n_engines, float* out, int
     });
 }
 
+// BernoulliGen<float>::packetOp (Basic.h:693-707) at Packet8f: 31 bits of
+// each 32-bit word against p = uint32(p * 2^31), compared as signed int32
+// lanes (pcmplt on Packet8i). vp != NULL: BernoulliVGen<float> (Basic.h:
+// 726-733), p per element = cvttps(p_i * 2^31).
+void orc_fill_bernoulli_f32(void* states, int64_t n_engines, float* out, int64_t per_engine, double p,
+                            const float* vp, int threads) {
+    Vmt* e = (Vmt*)states;
+    const int32_t pi = (int32_t)(uint32_t)(p * 0x80000000);
+    for_engines(n_engines, threads, [&](int64_t g) {
+        float* o = out + g * per_engine;
+        uint64_t w[4];
+        for (int64_t i = 0; i < per_engine; i += 8) {
+            vmt_draw(e[g], w);
+            for (int j = 0; j < 8; ++j) {
+                const int32_t r = (int32_t)((uint32_t)(w[j >> 1] >> (32 * (j & 1))) & 0x7FFFFFFFu);
+                const int32_t t = vp ? cvttps(vp[g * per_engine + i + j] * (float)0x80000000) : pi;
+                o[i + j] = r < t ? 1.0f : 0.0f;
+            }
+        }
+    });
+}
+
+// Balanced2Gen (Basic.h:437-470) and RandUtils::balanced (RandUtils.h:42-45):
+// zero_to_one * slope + bias with slope = b - a, bias = a (float); va != NULL:
+// BalancedVGen (Basic.h:487-493), zero_to_one * (b_i - a_i) + a_i.
+void orc_fill_balanced_f32(void* states, int64_t n_engines, float* out, int64_t per_engine, float a, float b,
+                           const float* va, const float* vb, int threads) {
+    Vmt* e = (Vmt*)states;
+    const float slope = b - a, bias = a;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        float* o = out + g * per_engine;
+        uint64_t w[4];
+        for (int64_t i = 0; i < per_engine; i += 8) {
+            vmt_draw(e[g], w);
+            for (int j = 0; j < 8; ++j) {
+                const float z = zero_to_one_float((uint32_t)(w[j >> 1] >> (32 * (j & 1))));
+                if (va) {
+                    const int64_t k = g * per_engine + i + j;
+                    o[i + j] = z * (vb[k] - va[k]) + va[k];
+                } else {
+                    o[i + j] = z * slope + bias;
+                }
+            }
+        }
+    });
+}
+
+// Vectorize-over-params single-draw generators (kind numbers of the device
+// library's VopKind): 0 UniformRealVGen (Basic.h:653-660), 1 ExponentialVGen
+// (NormalExp.h:418-424), 2 CauchyVGen (NormalExp.h:875-884), 3 WeibullVGen
+// (NormalExp.h:675-681), 4 ExtremeValueVGen (NormalExp.h:752-760), 5
+// StudentTVGen (NormalExp.h:332-345, two draws per packet), 6 LognormalVGen
+// (NormalExp.h:229-233) = pexp of NormalVGen.
+void orc_fill_vop_f32(void* states, int64_t n_engines, int kind, float* out, int64_t per_engine,
+                      const float* va, const float* vb, int threads) {
+    Vmt* e = (Vmt*)states;
+    if (kind == 6) {
+        orc_fill_normal_vop_f32(states, n_engines, out, per_engine, va, vb, threads);
+        for (int64_t i = 0; i < n_engines * per_engine; ++i) out[i] = pexp(out[i]);
+        return;
+    }
+    for_engines(n_engines, threads, [&](int64_t g) {
+        float* o = out + g * per_engine;
+        float u[8], u2[8];
+        for (int64_t i = 0; i < per_engine; i += 8) {
+            if (kind == 4) nzur8(e[g], u);
+            else ur8(e[g], u);
+            if (kind == 5) ur8(e[g], u2);
+            for (int j = 0; j < 8; ++j) {
+                const int64_t k = g * per_engine + i + j;
+                const float a = va[k];
+                float v;
+                if (kind == 0) {
+                    v = u[j] * (vb[k] - a) + a;
+                } else if (kind == 1) {
+                    v = -(plog(1.0f - u[j]) / a);
+                } else if (kind == 2) {
+                    float sn, cs;
+                    psincos(kPiF * (u[j] - 0.5f), sn, cs);
+                    v = a + vb[k] * (sn / cs);
+                } else if (kind == 3) {
+                    v = pexp(plog(-plog(1.0f - u[j])) * (1.0f / a)) * vb[k];
+                } else if (kind == 4) {
+                    v = a - plog(-plog(u[j])) * vb[k];
+                } else {
+                    const float u1 = 1.0f - u[j];
+                    const float radius = std::sqrt(a * (pexp(plog(u1) * (-2.0f / a)) - 1.0f));
+                    float sn, cs;
+                    psincos((float)(2 * kPi) * u2[j], sn, cs);
+                    v = radius * sn;
+                }
+                o[i + j] = v;
+            }
+        }
+    });
+}
+
+// Packet4i fills through half(): BernoulliGen<int32_t> (Basic.h:693-707)
+// mode 0, GeometricGen (Discrete.h:1372-1380) mode 1 with rlog_q = 1/log(1-p)
+// in double then float, RandbitsGen<int32_t> (Basic.h:394-400) mode 2.
+void orc_fill_half_i32(void* states, int64_t n_engines, int mode, int32_t* out, int64_t per_engine, double p,
+                       int threads) {
+    Vmt* e = (Vmt*)states;
+    const int32_t pi = (int32_t)(uint32_t)(p * 0x80000000);
+    const float rlog_q = (float)(1 / std::log(1 - p));
+    for_engines(n_engines, threads, [&](int64_t g) {
+        int32_t* o = out + g * per_engine;
+        uint64_t h[2];
+        for (int64_t i = 0; i < per_engine; i += 4) {
+            vmt_half(e[g], h);
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t w = (uint32_t)(h[j >> 1] >> (32 * (j & 1)));
+                if (mode == 0) o[i + j] = (int32_t)(w & 0x7FFFFFFFu) < pi ? 1 : 0;
+                else if (mode == 1) o[i + j] = cvttps(ptruncate(plog(1.0f - bit_to_ur_float(w)) * rlog_q));
+                else o[i + j] = (int32_t)w;
+            }
+        }
+    });
+}
+
 void orc_fill_cauchy_f32(void* states, int64_t n_engines, float* out, int64_t per_engine,
                          float a, float b, int threads) {
     Vmt* e = (Vmt*)states;

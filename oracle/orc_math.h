@@ -179,4 +179,17 @@ static inline float plgamma_approx(float x) {
 // ptruncate: round toward zero (arch/AVX/MorePacketMath.h:651-655).
 static inline float ptruncate(float a) { return std::trunc(a); }
 
+// pcast<Packet4f/8f, Packet4i/8i> = cvttps2dq: truncation; NaN and values
+// outside the int32 range give 0x80000000 (the "integer indefinite" value).
+static inline int32_t cvttps(float f) {
+    if (!(f < 2147483648.0f) || !(f >= -2147483648.0f)) return INT32_MIN;
+    return (int32_t)f;
+}
+
+// RandUtils<Packet8f>::zero_to_one (arch/AVX/RandUtils.h:200-204):
+// cvtepi32_ps(w & 0x7FFFFFFF) / pset1<Packet8f>(0x7FFFFFFF).
+static inline float zero_to_one_float(uint32_t w) {
+    return (float)(int32_t)(w & 0x7FFFFFFFu) / (float)0x7FFFFFFF;
+}
+
 }  // namespace orc

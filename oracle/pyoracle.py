@@ -149,6 +149,60 @@ def fill_gamma_f32(e, per, alpha=1.0, beta=1.0, threads=1):
     return _f32(e, per, "orc_fill_gamma_f32", C.c_float(alpha), C.c_float(beta), threads=threads)
 
 
+def fill_bernoulli_f32(e, per, p=0.5, threads=1):
+    return _f32(e, per, "orc_fill_bernoulli_f32", C.c_double(p), None, threads=threads)
+
+
+def fill_bernoulli_vop_f32(e, p, threads=1):
+    p = np.ascontiguousarray(p, dtype=np.float32)
+    return _f32(e, p.size // len(e), "orc_fill_bernoulli_f32", C.c_double(0.0), _p(p), threads=threads)
+
+
+def fill_balanced_f32(e, per, a=-1.0, b=1.0, threads=1):
+    return _f32(e, per, "orc_fill_balanced_f32", C.c_float(a), C.c_float(b), None, None, threads=threads)
+
+
+def fill_balanced_vop_f32(e, a, b, threads=1):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    return _f32(e, a.size // len(e), "orc_fill_balanced_f32", C.c_float(0), C.c_float(0), _p(a), _p(b),
+                threads=threads)
+
+
+# kind numbers of orc_fill_vop_f32
+VOP_KINDS = {"uniform": 0, "exponential": 1, "cauchy": 2, "weibull": 3, "extreme_value": 4,
+             "studentt": 5, "lognormal": 6}
+
+
+def fill_vop_f32(e, kind, a, b=None, threads=1):
+    """Vectorize-over-params float fills: a (and b) hold one value per output."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    bb = None if b is None else np.ascontiguousarray(b, dtype=np.float32)
+    out = np.empty(a.size, dtype=np.float32)
+    lib().orc_fill_vop_f32(_p(e.raw), I64(len(e)), C.c_int(VOP_KINDS[kind]), _p(out), I64(a.size // len(e)),
+                           _p(a), None if bb is None else _p(bb), C.c_int(threads))
+    return out
+
+
+def _half_i32(e, per, mode, p, threads):
+    out = np.empty(len(e) * per, dtype=np.int32)
+    lib().orc_fill_half_i32(_p(e.raw), I64(len(e)), C.c_int(mode), _p(out), I64(per), C.c_double(p),
+                            C.c_int(threads))
+    return out
+
+
+def fill_bernoulli_i32(e, per, p=0.5, threads=1):
+    return _half_i32(e, per, 0, p, threads)
+
+
+def fill_geometric_i32(e, per, p=0.5, threads=1):
+    return _half_i32(e, per, 1, p, threads)
+
+
+def fill_randbits_i32(e, per, threads=1):
+    return _half_i32(e, per, 2, 0.5, threads)
+
+
 def fill_poisson_i32(e, per, mean, threads=1):
     out = np.empty(len(e) * per, dtype=np.int32)
     lib().orc_fill_poisson_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), C.c_double(mean),

@@ -86,6 +86,11 @@ CASES = [
     ("weibull", lambda er, n, g: er.weibull(n, 1, g, 1.7, 2.5), lambda o, e, n: o.fill_weibull_f32(e, n, 1.7, 2.5)),
     ("extreme_value", lambda er, n, g: er.extreme_value(n, 1, g, 1.0, 3.0),
      lambda o, e, n: o.fill_extreme_value_f32(e, n, 1.0, 3.0)),
+    ("bernoulli", lambda er, n, g: er.bernoulli(n, 1, g, 0.3), lambda o, e, n: o.fill_bernoulli_f32(e, n, 0.3)),
+    ("bernoulli_1", lambda er, n, g: er.bernoulli(n, 1, g, 1.0), lambda o, e, n: o.fill_bernoulli_f32(e, n, 1.0)),
+    ("balanced", lambda er, n, g: er.balanced(n, 1, g), lambda o, e, n: o.fill_balanced_f32(e, n)),
+    ("balanced_ab", lambda er, n, g: er.balanced(n, 1, g, -3.0, 5.5),
+     lambda o, e, n: o.fill_balanced_f32(e, n, -3.0, 5.5)),
 ]
 
 
@@ -263,3 +268,70 @@ def test_many_engines_and_empty_fills(er, orc):
     x = er.normal(16, 1, urng).reshape(n, 16)
     ref = orc.Engines(seeds=[7 + 4 * (n - 1)])
     assert_same(x[n - 1].cpu().numpy(), orc.fill_normal_f32(ref, 16))
+
+
+HALF_CASES = [
+    ("bernoulli_i32", lambda er, n, g: er.bernoulli(n, 1, g, 0.3, dtype=__import__("torch").int32),
+     lambda o, e, n: o.fill_bernoulli_i32(e, n, 0.3)),
+    ("geometric", lambda er, n, g: er.geometric(n, 1, g, 0.3), lambda o, e, n: o.fill_geometric_i32(e, n, 0.3)),
+    ("geometric_tiny_p", lambda er, n, g: er.geometric(n, 1, g, 1e-9),
+     lambda o, e, n: o.fill_geometric_i32(e, n, 1e-9)),
+    ("randbits_i32", lambda er, n, g: er.randbits_i32(n, 1, g), lambda o, e, n: o.fill_randbits_i32(e, n)),
+]
+
+
+@pytest.mark.parametrize("name,gpu,cpu", HALF_CASES, ids=[c[0] for c in HALF_CASES])
+def test_half_draw_fill_matches_oracle(er, orc, name, gpu, cpu):
+    """Packet4i fills through half(): odd packet counts leave the upper half
+    cached for the next half-draw fill, and a full-draw fill in between leaves
+    it in place; engine records must match after every step."""
+    seeds = [3, 1000, 2 ** 40 + 5, 16380, 77]
+    urng = er.Vmt19937_64(seeds=seeds)
+    ref = orc.Engines(seeds=seeds)
+    for n in (4, 12, 312 * 8 + 4, 2500, 4):
+        assert_same(flat(gpu(er, n, urng)), cpu(orc, ref, n))
+        np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+        assert_same(flat(er.uniform_real(8, 1, urng)), orc.fill_uniform_f32(ref, 8))
+
+
+VOP_CASES = [
+    ("uniform", "uniform_real_vop", 2), ("exponential", "exponential_vop", 1), ("cauchy", "cauchy_vop", 2),
+    ("weibull", "weibull_vop", 2), ("extreme_value", "extreme_value_vop", 2), ("studentt", "student_t_vop", 1),
+    ("lognormal", "lognormal_vop", 2),
+]
+
+
+@pytest.mark.parametrize("kind,fn,nparams", VOP_CASES, ids=[c[0] for c in VOP_CASES])
+def test_vop_fill_matches_oracle(er, orc, kind, fn, nparams):
+    import torch
+    seeds = [11, 12, 13]
+    per = 3000 * 8 + 8  # lognormal: ends on an unused cos half packet
+    rng = np.random.default_rng(len(kind))
+    a = rng.uniform(0.3, 4, 3 * per).astype(np.float32)
+    b = rng.uniform(0.3, 4, 3 * per).astype(np.float32)
+    urng = er.Vmt19937_64(seeds=seeds)
+    ref = orc.Engines(seeds=seeds)
+    er.randbits_u64(3, urng)
+    orc.fill_u64(ref, 3)
+    args = [torch.from_numpy(a).cuda()] + ([torch.from_numpy(b).cuda()] if nparams == 2 else [])
+    out = getattr(er, fn)(urng, *args)
+    assert_same(out.cpu().numpy(), orc.fill_vop_f32(ref, kind, a, b if nparams == 2 else None))
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+
+
+def test_balanced_and_bernoulli_vop_match_oracle(er, orc):
+    import torch
+    seeds = [21, 22]
+    per = 4096
+    rng = np.random.default_rng(3)
+    a = rng.uniform(-5, 5, 2 * per).astype(np.float32)
+    b = (a + rng.uniform(0.1, 5, 2 * per)).astype(np.float32)
+    p = rng.uniform(0, 1, 2 * per).astype(np.float32)
+    p[:8] = [0, 1, 0.5, 1e-9, 1 - 1e-7, 0.25, 1, 0]
+    urng = er.Vmt19937_64(seeds=seeds)
+    ref = orc.Engines(seeds=seeds)
+    got = er.balanced_vop(urng, torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda())
+    assert_same(got.cpu().numpy(), orc.fill_balanced_vop_f32(ref, a, b))
+    got = er.bernoulli_vop(urng, torch.from_numpy(p).cuda())
+    assert_same(got.cpu().numpy(), orc.fill_bernoulli_vop_f32(ref, p))
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)

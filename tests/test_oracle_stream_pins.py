@@ -261,3 +261,102 @@ def test_lognormal_weibull_extreme_value_of_the_stream(orc):
     ln = orc.fill_lognormal_f32(e.copy(), per, 0.5, 0.75)
     nz = orc.fill_normal_f32(e.copy(), per, 0.5, 0.75)
     np.testing.assert_allclose(ln, np.exp(nz.astype(np.float64)), rtol=3e-7 * 4)
+
+
+def test_bernoulli_balanced_randbits_of_the_stream(orc):
+    """Raw-bit generators against the pinned u64 stream: 31 bits against
+    uint32(p * 2^31) as signed int32 (so p = 1 gives zeros on the packet path),
+    zero_to_one = int(w & 0x7FFFFFFF) / 2^31 in float, and the half() words."""
+    e = orc.Engines(n=2, seed0=5)
+    per = 4096
+    w = full_words(orc, e, per // 8).reshape(2, per)
+    r31 = (w & 0x7FFFFFFF).astype(np.int64)
+    for p in (0.3, 0.5, 1.0, 0.0):
+        got = orc.fill_bernoulli_f32(e.copy(), per, p).reshape(2, per)
+        t = np.int64(np.uint32(p * 2.0 ** 31).astype(np.int32))
+        np.testing.assert_array_equal(got, (r31 < t).astype(f32))
+    assert not orc.fill_bernoulli_f32(e.copy(), per, 1.0).any()
+    assert abs(orc.fill_bernoulli_f32(e.copy(), 1 << 16, 0.3).mean() - 0.3) < 0.01
+    z = (r31.astype(f32) / f32(2.0 ** 31)).astype(f32)
+    np.testing.assert_array_equal(orc.fill_balanced_f32(e.copy(), per).reshape(2, per), z * f32(2) - f32(1))
+    ab = orc.fill_balanced_f32(e.copy(), per, -3.0, 5.5).reshape(2, per)
+    np.testing.assert_array_equal(ab, z * (f32(5.5) - f32(-3.0)) + f32(-3.0))
+    big = orc.fill_balanced_f32(e.copy(), 1 << 16, -3.0, 5.5)
+    assert big.min() >= -3.0 and big.max() <= 5.5
+    assert abs(big.mean() - 1.25) < 0.05 and abs(big.var() - 8.5 ** 2 / 12) < 0.1
+    # int32 forms through half()
+    h = half_words(orc, e, per // 4).reshape(2, per)
+    np.testing.assert_array_equal(orc.fill_randbits_i32(e.copy(), per).reshape(2, per), h.view(np.int32))
+    bi = orc.fill_bernoulli_i32(e.copy(), per, 0.3).reshape(2, per)
+    np.testing.assert_array_equal(bi, ((h & 0x7FFFFFFF).astype(np.int64) < np.int64(int(0.3 * 2 ** 31))).astype(np.int32))
+
+
+def test_geometric_of_the_stream(orc):
+    """trunc(log(1 - U) / log(1 - p)) from half() words: equal to a float64
+    evaluation except where the quotient lies within float rounding of an
+    integer; the distribution is Geometric(p) on {0, 1, ...}."""
+    e = orc.Engines(n=2, seed0=9)
+    per = 1 << 15
+    p = 0.3
+    got = orc.fill_geometric_i32(e.copy(), per, p).reshape(2, per)
+    h = half_words(orc, e, per // 4).reshape(2, per)
+    u = ur(h).astype(np.float64)
+    ref = np.trunc(np.log(1 - u) * np.float64(np.float32(1 / np.log(1 - p)))).astype(np.int64)
+    diff = got - ref
+    assert np.abs(diff).max() <= 1 and (diff != 0).mean() < 1e-3
+    x = got.ravel()
+    assert abs(x.mean() - (1 - p) / p) < 0.05
+    for k in range(4):
+        assert abs((x == k).mean() - (1 - p) ** k * p) < 0.01
+    # p = 0: 1/log(1) = inf -> every product is -inf or NaN -> INT32_MIN (cvttps)
+    assert (orc.fill_geometric_i32(e.copy(), 64, 0.0) == np.iinfo(np.int32).min).all()
+
+
+@pytest.mark.parametrize("kind,scalar", [
+    ("uniform", lambda o, e, n, a, b: o.fill_uniform_f32(e, n, a, b)),
+    ("exponential", lambda o, e, n, a, b: o.fill_exponential_f32(e, n, a)),
+    ("cauchy", lambda o, e, n, a, b: o.fill_cauchy_f32(e, n, a, b)),
+    ("weibull", lambda o, e, n, a, b: o.fill_weibull_f32(e, n, a, b)),
+    ("extreme_value", lambda o, e, n, a, b: o.fill_extreme_value_f32(e, n, a, b)),
+    ("studentt", lambda o, e, n, a, b: o.fill_studentt_f32(e, n, a)),
+    ("lognormal", lambda o, e, n, a, b: o.fill_lognormal_f32(e, n, a, b)),
+])
+def test_vop_with_constant_parameters_is_the_scalar_fill(orc, kind, scalar):
+    """A vectorize-over-params fill whose parameters are all equal is the
+    scalar generator's fill (the per-element formulas reduce to it), for two
+    parameter sets; with varying parameters, element i depends on its own
+    parameters only (checked by changing one block)."""
+    e = orc.Engines(n=3, seed0=33)
+    per = 1024
+    for a, b in ((1.7, 2.5), (0.6, 4.0)):
+        va = np.full(3 * per, a, np.float32)
+        vb = np.full(3 * per, b, np.float32)
+        np.testing.assert_array_equal(orc.fill_vop_f32(e.copy(), kind, va, vb),
+                                      scalar(orc, e.copy(), per, a, b))
+    rng = np.random.default_rng(1)
+    va = rng.uniform(0.5, 3, 3 * per).astype(np.float32)
+    vb = rng.uniform(0.5, 3, 3 * per).astype(np.float32)
+    x = orc.fill_vop_f32(e.copy(), kind, va, vb)
+    va2 = va.copy()
+    va2[per:2 * per] += 1
+    y = orc.fill_vop_f32(e.copy(), kind, va2, vb)
+    np.testing.assert_array_equal(x[:per], y[:per])
+    np.testing.assert_array_equal(x[2 * per:], y[2 * per:])
+
+
+def test_uniform_and_balanced_vop_are_affine_maps_of_the_stream(orc):
+    e = orc.Engines(n=2, seed0=3)
+    per = 2048
+    rng = np.random.default_rng(2)
+    a = rng.uniform(-5, 5, 2 * per).astype(np.float32)
+    b = (a + rng.uniform(0.1, 5, 2 * per)).astype(np.float32)
+    w = full_words(orc, e, per // 8).reshape(2 * per)
+    u = ur(w)
+    np.testing.assert_array_equal(orc.fill_vop_f32(e.copy(), "uniform", a, b), u * (b - a) + a)
+    z = ((w & 0x7FFFFFFF).astype(np.int64).astype(f32) / f32(2.0 ** 31)).astype(f32)
+    np.testing.assert_array_equal(orc.fill_balanced_vop_f32(e.copy(), a, b), z * (b - a) + a)
+    p = rng.uniform(0, 1, 2 * per).astype(np.float32)
+    t = (p * f32(2.0 ** 31)).astype(np.float64)
+    thr = np.where(t >= 2.0 ** 31, -(2 ** 31), np.trunc(t)).astype(np.int64)
+    np.testing.assert_array_equal(orc.fill_bernoulli_vop_f32(e.copy(), p),
+                                  ((w & 0x7FFFFFFF).astype(np.int64) < thr).astype(f32))
