@@ -29,7 +29,7 @@ __all__ = [
     "randbits_u64", "uniform_real", "uniform_real_f64", "normal", "normal_vop",
     "exponential", "cauchy", "student_t", "gamma", "chi_squared", "poisson", "binomial",
     "lognormal", "weibull", "extreme_value",
-    "bernoulli", "balanced", "geometric", "randbits_i32", "uniform_real_vop", "balanced_vop", "bernoulli_vop",
+    "bernoulli", "balanced", "geometric", "randbits_i32", "uniform_int", "uniform_real_vop", "balanced_vop", "bernoulli_vop",
     "exponential_vop", "cauchy_vop", "lognormal_vop", "student_t_vop", "weibull_vop", "extreme_value_vop",
     "binomial_vop", "gamma_vop", "poisson_vop", "run_requests", "cholesky_batch",
     "mvnormal_batch", "wishart_batch", "dirichlet_batch", "MvNormalGen", "WishartGen",
@@ -358,6 +358,11 @@ def balanced(rows, cols, urng, a=-1.0, b=1.0, out=None, stream=None):
 def geometric(rows, cols, urng, p=0.5, out=None, stream=None):
     """Rand::geometric<MatrixXi>(rows, cols, urng, p) (GeometricGen, int32)."""
     return _fill("erd_fill_geometric_i32", urng, rows, cols, _torch().int32, out, stream, float(p))
+
+
+def uniform_int(rows, cols, urng, min=0, max=0, out=None, stream=None):
+    """Rand::uniformInt<MatrixXi>(rows, cols, urng, min, max) (UniformIntGen, int32)."""
+    return _fill("erd_fill_uniform_int_i32", urng, rows, cols, _torch().int32, out, stream, int(min), int(max))
 
 
 def randbits_i32(rows, cols, urng, out=None, stream=None):

@@ -573,6 +573,16 @@ int erd_fill_geometric_i32(erd_engines* e, int32_t* out, int64_t per, double p, 
     return hip_status(erd::launch_fill_geometric_i32(e->states, e->count, out, per, rlog_q, S(stream)));
 }
 
+int erd_fill_uniform_int_i32(erd_engines* e, int32_t* out, int64_t per, int32_t min, int32_t max, void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    const int64_t pdiff = (int64_t)max - (int64_t)min;
+    if (pdiff < 0) return ERD_ERR_INVALID;
+    if (pdiff > INT32_MAX) return ERD_ERR_UNSUPPORTED;  // the reference's int32 max - min overflows
+    return hip_status(
+        erd::launch_fill_uniform_int_i32(e->states, e->count, out, per, min, (uint32_t)pdiff, S(stream)));
+}
+
 int erd_fill_randbits_i32(erd_engines* e, int32_t* out, int64_t per, void* stream) {
     int rc = check_fill(e, out, per, 4);
     if (rc) return rc;

@@ -401,6 +401,19 @@ struct GeometricI32 {
     }
 };
 
+// UniformIntGen<int32_t>::packetOp when pdiff + 1 is a power of two
+// (Discrete.h:303-306): (rx & bitmask) + min from one half draw.
+struct UniformIntPow2I32 {
+    int32_t* out;
+    uint32_t mask;
+    int32_t pmin;
+    __device__ int32_t one(uint32_t w) const { return (int32_t)((w & mask) + (uint32_t)pmin); }
+    __device__ void emit(int64_t i, const uint64_t* x) const {
+        *reinterpret_cast<int4*>(out + i) =
+            make_int4(one(lo32(x[0])), one(hi32(x[0])), one(lo32(x[1])), one(hi32(x[1])));
+    }
+};
+
 // RandbitsGen<int32_t>::packetOp (Basic.h:394-400) at Packet4i: the half's bits.
 struct RandbitsI32 {
     int32_t* out;

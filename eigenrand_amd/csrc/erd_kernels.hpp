@@ -77,6 +77,9 @@ hipError_t launch_fill_bernoulli_i32(EngineState* st, int64_t n, int32_t* out, i
 hipError_t launch_fill_geometric_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, float rlog_q,
                                      hipStream_t s);
 hipError_t launch_fill_randbits_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, hipStream_t s);
+// UniformIntGen<int32_t>(lo, lo + pdiff), pdiff < 2^31.
+hipError_t launch_fill_uniform_int_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, int32_t lo,
+                                       uint32_t pdiff, hipStream_t s);
 
 // fill_reject.hip (CompressMask rejection samplers); gs: optional per-engine
 // queues of a reused generator object.

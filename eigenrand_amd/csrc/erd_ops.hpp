@@ -403,6 +403,145 @@ __device__ void op_half_fixed(Eng& e, uint64_t* carry, int64_t T, const D& d, in
     cursor_end(e, cur);
 }
 
+// UniformIntGen<int32_t>::packetOp, rejection branch (Discrete.h:291-335):
+// a packet draws a half, tests its candidates (rx & mask, then rx shifted
+// right by `bits` while bits fit) through CompressMask<16> (W = 4) and, once a
+// packet is emitted, drops the rest of that draw; the next packet starts on a
+// new half. So every half draw ("slot") emits at most one packet, and the
+// process is a chain over slots whose state is the queue count r (0..3).
+//
+// A wave takes 64 slots at once: thread t evaluates its slot's candidates and
+// the slot's transition for each incoming r (emission, candidate, values
+// appended, outgoing r); a prefix composition of the transitions over the
+// wave gives every slot its actual incoming r. Queued values live in an LDS
+// ring at consecutive positions, so the queue at any point is the last r
+// values appended and a queue emission outputs the 4 values starting at the
+// oldest queued one. The fill stops at the slot emitting its last packet.
+struct UniformIntParams {
+    uint32_t mask;  // bitmask as 32 bits
+    int32_t plen;   // pdiff + 1 as int32 (compared signed, as pcmplt does)
+    int32_t pmin;
+    int bits;       // bitsize
+    int cands;      // candidates per half draw
+};
+constexpr int kUiRing = 512;  // >= 64 slots x 6 appended values + 3 queued
+
+__device__ void op_uniform_int(Eng& e, uint64_t* carry, int32_t* ring, int32_t* o, int64_t T,
+                               const UniformIntParams& u, int lane) {
+    if (T == 0) return;
+    HalfCursor cur = cursor_begin(e, carry, lane);
+    int64_t pk = 0;
+    int r = 0;          // queued values (wave-uniform)
+    uint32_t head = 0;  // ring position after the newest queued value
+    while (pk < T) {
+        cur.ensure(1, lane);
+        const int take = min(cur.avail(), 64);
+        const bool live = lane < take;
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        if (live) {
+            const uint64_t x0 = cur.word(lane, 0), x1 = cur.word(lane, 1);
+            w[0] = lo32(x0); w[1] = hi32(x0); w[2] = lo32(x1); w[3] = hi32(x1);
+        }
+        uint64_t accm = 0;  // acceptance bits: 4 per candidate
+        for (int k = 0; k < u.cands; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int32_t c = (int32_t)((w[j] >> (k * u.bits)) & u.mask);
+                if (c < u.plen) accm |= 1ull << (4 * k + j);
+            }
+        // transition per incoming count: out count (2 bits), kind (0 none,
+        // 1 bypass, 2 queue), emitting candidate, values appended
+        uint32_t fmap = 0, kinds = 0, kem = 0, apps = 0;
+#pragma unroll
+        for (int r0 = 0; r0 < 4; ++r0) {
+            int rr = r0, a = 0, kind = 0, ke = 0;
+            if (live) {
+                for (int k = 0; k < u.cands; ++k) {
+                    const uint32_t m = (uint32_t)(accm >> (4 * k)) & 0xFu;
+                    if (m == 0xFu) { kind = 1; ke = k; break; }
+                    const int n = __popc(m);
+                    a += n;
+                    rr += n;
+                    if (rr >= 4) { kind = 2; ke = k; rr -= 4; break; }
+                }
+            }
+            fmap |= (uint32_t)rr << (2 * r0);
+            kinds |= (uint32_t)kind << (2 * r0);
+            kem |= (uint32_t)ke << (5 * r0);
+            apps |= (uint32_t)a << (3 * r0);
+        }
+        // inclusive prefix composition of the maps over the wave (slot order)
+        uint32_t pre = fmap;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t before = (uint32_t)__shfl_up((int)pre, d);
+            if (lane >= d) {
+                uint32_t h = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) h |= ((pre >> (2 * ((before >> (2 * q)) & 3u))) & 3u) << (2 * q);
+                pre = h;
+            }
+        }
+        uint32_t excl = (uint32_t)__shfl_up((int)pre, 1);
+        if (lane == 0) excl = 0xE4u;  // identity: 0,1,2,3
+        const int rin = (int)((excl >> (2 * r)) & 3u);
+        const int kind = (int)((kinds >> (2 * rin)) & 3u);
+        const int ke = (int)((kem >> (5 * rin)) & 31u);
+        const int a = (int)((apps >> (3 * rin)) & 7u);
+        const int rout = (int)((fmap >> (2 * rin)) & 3u);
+        // packets: slot t emits packet pk + (emitting slots before t); the
+        // fill ends at the slot that emits packet T - 1
+        const uint64_t E = __ballot(kind != 0);
+        const int64_t before = __popcll(E & lanemask_lt(lane));
+        const int64_t need = T - pk;
+        int used = take;
+        if ((int64_t)__popcll(E) >= need) {
+            uint64_t m = E;
+            for (int64_t q = 1; q < need; ++q) m &= m - 1;
+            used = (int)__ffsll((long long)m);  // slots up to the emitting one
+        }
+        const bool mine = lane < used;
+        // ring positions of appended values: exclusive prefix over the wave
+        int incl = mine ? a : 0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d);
+            if (lane >= d) incl += v;
+        }
+        const uint32_t A = head + (uint32_t)(incl - (mine ? a : 0));
+        if (mine) {
+            uint32_t pos = A;
+            const int last = kind == 1 ? ke - 1 : (kind == 2 ? ke : u.cands - 1);
+            for (int k = 0; k <= last; ++k) {
+                const uint32_t m = (uint32_t)(accm >> (4 * k)) & 0xFu;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (m & (1u << j)) ring[pos++ % kUiRing] = (int32_t)((w[j] >> (k * u.bits)) & u.mask);
+            }
+        }
+        __syncthreads();
+        if (mine && kind != 0) {
+            int4 v;
+            if (kind == 1) {
+                v = make_int4((int32_t)((w[0] >> (ke * u.bits)) & u.mask), (int32_t)((w[1] >> (ke * u.bits)) & u.mask),
+                              (int32_t)((w[2] >> (ke * u.bits)) & u.mask), (int32_t)((w[3] >> (ke * u.bits)) & u.mask));
+            } else {
+                const uint32_t q = A - (uint32_t)rin;
+                v = make_int4(ring[q % kUiRing], ring[(q + 1) % kUiRing], ring[(q + 2) % kUiRing],
+                              ring[(q + 3) % kUiRing]);
+            }
+            v.x += u.pmin; v.y += u.pmin; v.z += u.pmin; v.w += u.pmin;
+            *reinterpret_cast<int4*>(o + 4 * (pk + before)) = v;
+        }
+        r = __builtin_amdgcn_readlane(rout, used - 1);
+        head += (uint32_t)__builtin_amdgcn_readlane(incl, used - 1);
+        pk += __popcll(E & (used == 64 ? ~0ull : ((1ull << used) - 1)));
+        __syncthreads();  // ring and carried slots are rewritten next
+        cur.advance(used);
+    }
+    cursor_end(e, cur);
+}
+
 // Knuth loop of PoissonGen::packetOp (Discrete.h:860-873); returns lane j's count.
 __device__ __forceinline__ int32_t knuth_packet(HalfCursor& cur, float thr, int lane,
                                                 uint32_t* status) {

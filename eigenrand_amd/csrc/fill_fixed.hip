@@ -9,6 +9,8 @@
 // (unit, lane) items, so every MT word is read and tempered exactly once.
 // Units that straddle a block boundary keep their first words in a small LDS
 // carry area across the twist (op_fixed in erd_ops.hpp).
+#include <algorithm>
+#include <cmath>
 #include "erd_device.hpp"
 #include "erd_dists.hpp"
 #include "erd_kernels.hpp"
@@ -280,6 +282,48 @@ hipError_t launch_fill_geometric_i32(EngineState* st, int64_t n, int32_t* out, i
 
 hipError_t launch_fill_randbits_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, hipStream_t s) {
     return launch_half(st, n, out, per, RandbitsI32{nullptr}, s);
+}
+
+// UniformIntGen<int32_t>: a constant range writes min without drawing; a
+// power-of-two range masks one half draw per packet; otherwise the
+// CompressMask chain of op_uniform_int.
+__global__ __launch_bounds__(64) void k_fill_uniform_int(EngineState* __restrict__ states, int32_t* __restrict__ out,
+                                                         int64_t per, UniformIntParams u) {
+    __shared__ uint64_t mt[kWords];
+    __shared__ uint64_t carry[2];
+    __shared__ int32_t ring[kUiRing];
+    const int lane = threadIdx.x;
+    const int64_t g = blockIdx.x;
+    if (per / 4 == 0) return;
+    Eng e;
+    eng_load(e, mt, states + g, lane);
+    op_uniform_int(e, carry, ring, out + g * per, per / 4, u, lane);
+    eng_store(e, states + g, lane);
+}
+
+__global__ void k_fill_const_i32(int32_t* __restrict__ out, int64_t n, int32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = v;
+}
+
+hipError_t launch_fill_uniform_int_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, int32_t lo,
+                                       uint32_t pdiff, hipStream_t s) {
+    if (n <= 0 || per <= 0) return hipSuccess;
+    if (pdiff == 0) {
+        const int64_t total = n * per;
+        const int64_t blocks = std::min<int64_t>((total + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_fill_const_i32, dim3((unsigned)blocks), dim3(256), 0, s, out, total, lo);
+        return hipGetLastError();
+    }
+    // UniformIntGen's constructor (Discrete.h:229-246) for pdiff < 2^31
+    const int bits = (int)std::ceil(std::log2((double)pdiff + 1.0));
+    const uint32_t mask = (uint32_t)((1ull << bits) - 1);
+    if (pdiff == mask) return launch_half(st, n, out, per, UniformIntPow2I32{nullptr, mask, lo}, s);
+    int cands = 1;
+    for (int bitcnt = bits; bitcnt + bits < 32; bitcnt += bits) ++cands;
+    const UniformIntParams u{mask, (int32_t)(pdiff + 1), lo, bits, cands};
+    hipLaunchKernelGGL(k_fill_uniform_int, dim3((unsigned)n), dim3(64), 0, s, st, out, per, u);
+    return hipGetLastError();
 }
 
 }  // namespace erd

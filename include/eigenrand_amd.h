@@ -184,6 +184,13 @@ int erd_fill_bernoulli_i32(erd_engines* e, int32_t* out, int64_t per_engine, dou
  * U) * (1 / log(1 - p))) from a half() draw per 4 outputs, converted as
  * cvttps2dq does (out of range: INT32_MIN); 0 <= p <= 1. */
 int erd_fill_geometric_i32(erd_engines* e, int32_t* out, int64_t per_engine, double p, void* stream);
+/* uniformInt<int32>(…, min, max): UniformIntGen (Discrete.h:215-335) at
+ * Packet4i: candidates of `bitsize` bits from half() draws through
+ * CompressMask<16>, the rest of a draw dropped once a packet is emitted.
+ * Requires 0 <= max - min < 2^31 (ERD_ERR_INVALID if max < min,
+ * ERD_ERR_UNSUPPORTED beyond, where the reference's int32 max - min overflows). */
+int erd_fill_uniform_int_i32(erd_engines* e, int32_t* out, int64_t per_engine, int32_t min, int32_t max,
+                             void* stream);
 /* randBits<int32>(…): RandbitsGen (Basic.h:378-401) at Packet4i, the bits of
  * one half() draw per 4 outputs. */
 int erd_fill_randbits_i32(erd_engines* e, int32_t* out, int64_t per_engine, void* stream);

@@ -277,6 +277,103 @@ inline DeviceMatrix<int32_t> binomial(Vmt19937_64& urng, const DeviceMatrix<int3
     }, "erd_fill_binomial_vop_i32");
 }
 
+// Raw-bit and small integer generators.
+inline DeviceMatrix<int32_t> randBitsInt(int64_t rows, int64_t cols, Vmt19937_64& urng, void* stream = nullptr) {
+    return detail::fill<int32_t>(rows, cols, urng.size(), [&](int32_t* o, int64_t n) {
+        return erd_fill_randbits_i32(urng.handle(), o, n, stream);
+    }, "erd_fill_randbits_i32");
+}
+inline DeviceMatrix<int32_t> uniformInt(int64_t rows, int64_t cols, Vmt19937_64& urng, int32_t min, int32_t max,
+                                        void* stream = nullptr) {
+    return detail::fill<int32_t>(rows, cols, urng.size(), [&](int32_t* o, int64_t n) {
+        return erd_fill_uniform_int_i32(urng.handle(), o, n, min, max, stream);
+    }, "erd_fill_uniform_int_i32");
+}
+inline DeviceMatrix<float> bernoulli(int64_t rows, int64_t cols, Vmt19937_64& urng, double p = 0.5,
+                                     void* stream = nullptr) {
+    return detail::fill<float>(rows, cols, urng.size(), [&](float* o, int64_t n) {
+        return erd_fill_bernoulli_f32(urng.handle(), o, n, p, stream);
+    }, "erd_fill_bernoulli_f32");
+}
+inline DeviceMatrix<int32_t> bernoulliInt(int64_t rows, int64_t cols, Vmt19937_64& urng, double p = 0.5,
+                                          void* stream = nullptr) {
+    return detail::fill<int32_t>(rows, cols, urng.size(), [&](int32_t* o, int64_t n) {
+        return erd_fill_bernoulli_i32(urng.handle(), o, n, p, stream);
+    }, "erd_fill_bernoulli_i32");
+}
+inline DeviceMatrix<float> balanced(int64_t rows, int64_t cols, Vmt19937_64& urng, float a = -1.f, float b = 1.f,
+                                    void* stream = nullptr) {
+    return detail::fill<float>(rows, cols, urng.size(), [&](float* o, int64_t n) {
+        return erd_fill_balanced_f32(urng.handle(), o, n, a, b, stream);
+    }, "erd_fill_balanced_f32");
+}
+inline DeviceMatrix<int32_t> geometric(int64_t rows, int64_t cols, Vmt19937_64& urng, double p = 0.5,
+                                       void* stream = nullptr) {
+    return detail::fill<int32_t>(rows, cols, urng.size(), [&](int32_t* o, int64_t n) {
+        return erd_fill_geometric_i32(urng.handle(), o, n, p, stream);
+    }, "erd_fill_geometric_i32");
+}
+
+// Vectorize-over-params overloads: parameters are device matrices shaped
+// like the result (engine g takes block g).
+namespace detail {
+template <class F>
+DeviceMatrix<float> vop(const DeviceMatrix<float>& like, Vmt19937_64& urng, F&& f, const char* what) {
+    return fill<float>(like.rows, like.cols, urng.size(), [&](float* o, int64_t n) { return f(o, n); }, what);
+}
+}  // namespace detail
+inline DeviceMatrix<float> uniformReal(Vmt19937_64& urng, const DeviceMatrix<float>& a, const DeviceMatrix<float>& b,
+                                       void* stream = nullptr) {
+    return detail::vop(a, urng, [&](float* o, int64_t n) {
+        return erd_fill_uniform_vop_f32(urng.handle(), o, n, a.values.data(), b.values.data(), stream);
+    }, "erd_fill_uniform_vop_f32");
+}
+inline DeviceMatrix<float> balanced(Vmt19937_64& urng, const DeviceMatrix<float>& a, const DeviceMatrix<float>& b,
+                                    void* stream = nullptr) {
+    return detail::vop(a, urng, [&](float* o, int64_t n) {
+        return erd_fill_balanced_vop_f32(urng.handle(), o, n, a.values.data(), b.values.data(), stream);
+    }, "erd_fill_balanced_vop_f32");
+}
+inline DeviceMatrix<float> bernoulli(Vmt19937_64& urng, const DeviceMatrix<float>& p, void* stream = nullptr) {
+    return detail::vop(p, urng, [&](float* o, int64_t n) {
+        return erd_fill_bernoulli_vop_f32(urng.handle(), o, n, p.values.data(), stream);
+    }, "erd_fill_bernoulli_vop_f32");
+}
+inline DeviceMatrix<float> exponential(Vmt19937_64& urng, const DeviceMatrix<float>& lambda, void* stream = nullptr) {
+    return detail::vop(lambda, urng, [&](float* o, int64_t n) {
+        return erd_fill_exponential_vop_f32(urng.handle(), o, n, lambda.values.data(), stream);
+    }, "erd_fill_exponential_vop_f32");
+}
+inline DeviceMatrix<float> cauchy(Vmt19937_64& urng, const DeviceMatrix<float>& a, const DeviceMatrix<float>& b,
+                                  void* stream = nullptr) {
+    return detail::vop(a, urng, [&](float* o, int64_t n) {
+        return erd_fill_cauchy_vop_f32(urng.handle(), o, n, a.values.data(), b.values.data(), stream);
+    }, "erd_fill_cauchy_vop_f32");
+}
+inline DeviceMatrix<float> lognormal(Vmt19937_64& urng, const DeviceMatrix<float>& mean,
+                                     const DeviceMatrix<float>& stdev, void* stream = nullptr) {
+    return detail::vop(mean, urng, [&](float* o, int64_t n) {
+        return erd_fill_lognormal_vop_f32(urng.handle(), o, n, mean.values.data(), stdev.values.data(), stream);
+    }, "erd_fill_lognormal_vop_f32");
+}
+inline DeviceMatrix<float> studentT(Vmt19937_64& urng, const DeviceMatrix<float>& n_, void* stream = nullptr) {
+    return detail::vop(n_, urng, [&](float* o, int64_t n) {
+        return erd_fill_studentt_vop_f32(urng.handle(), o, n, n_.values.data(), stream);
+    }, "erd_fill_studentt_vop_f32");
+}
+inline DeviceMatrix<float> weibull(Vmt19937_64& urng, const DeviceMatrix<float>& a, const DeviceMatrix<float>& b,
+                                   void* stream = nullptr) {
+    return detail::vop(a, urng, [&](float* o, int64_t n) {
+        return erd_fill_weibull_vop_f32(urng.handle(), o, n, a.values.data(), b.values.data(), stream);
+    }, "erd_fill_weibull_vop_f32");
+}
+inline DeviceMatrix<float> extremeValue(Vmt19937_64& urng, const DeviceMatrix<float>& a,
+                                        const DeviceMatrix<float>& b, void* stream = nullptr) {
+    return detail::vop(a, urng, [&](float* o, int64_t n) {
+        return erd_fill_extreme_value_vop_f32(urng.handle(), o, n, a.values.data(), b.values.data(), stream);
+    }, "erd_fill_extreme_value_vop_f32");
+}
+
 // --------------------------------------------------- reusable generator objects
 // Rand::XxxGen objects: consecutive generate() calls keep the caches the
 // reference keeps (normal cos packet, CompressMask leftovers).

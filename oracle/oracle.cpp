@@ -128,12 +128,12 @@ inline void ur4(Vmt& e, float u[4]) {
 // arch/SSE/PacketFilter.h:118-150 (W=4): a fully accepted packet is emitted as
 // is and leaves the queue untouched; otherwise accepted lanes are appended in
 // lane order and the first W queued values are emitted once W are available.
-template <int W>
-bool compress_append(float value[W], const bool mask[W], float rest[W], int& rest_cnt) {
+template <int W, class T = float>
+bool compress_append(T value[W], const bool mask[W], T rest[W], int& rest_cnt) {
     int cnt = 0;
     for (int i = 0; i < W; ++i) cnt += mask[i];
     if (cnt == W) return true;
-    float q[2 * W];
+    T q[2 * W];
     int n = 0;
     for (int i = 0; i < rest_cnt; ++i) q[n++] = rest[i];
     for (int i = 0; i < W; ++i) if (mask[i]) q[n++] = value[i];
@@ -746,6 +746,62 @@ void orc_fill_half_i32(void* states, int64_t n_engines, int mode, int32_t* out, 
                 if (mode == 0) o[i + j] = (int32_t)(w & 0x7FFFFFFFu) < pi ? 1 : 0;
                 else if (mode == 1) o[i + j] = cvttps(ptruncate(plog(1.0f - bit_to_ur_float(w)) * rlog_q));
                 else o[i + j] = (int32_t)w;
+            }
+        }
+    });
+}
+
+// UniformIntGen<int32_t> (Discrete.h:215-335) at Packet4i; the constructor
+// (229-246) for ranges with 0 <= max - min < 2^31: bitsize = ceil(log2(pdiff
+// + 1)), bitmask = 2^bitsize - 1. pdiff = 0: the constant min, no draws;
+// pdiff = bitmask: (rx & bitmask) + min per half draw; otherwise candidates
+// rx & bitmask, rx >>= bitsize while bitcnt + bitsize < 32, through
+// CompressMask<16> with a signed compare against pdiff + 1; the rest of a
+// half draw is dropped once a packet is emitted. A fresh generator per fill.
+void orc_fill_uniform_int_i32(void* states, int64_t n_engines, int32_t* out, int64_t per_engine, int32_t lo,
+                              int32_t hi, int threads) {
+    Vmt* e = (Vmt*)states;
+    const uint64_t pdiff = (uint64_t)((int64_t)hi - (int64_t)lo);
+    const uint64_t bitsize = pdiff == 0 ? 0 : (uint64_t)std::ceil(std::log2((double)(pdiff + 1)));
+    const uint64_t bitmask = bitsize == 0 ? 0 : (~0ull >> (64 - bitsize));
+    for_engines(n_engines, threads, [&](int64_t g) {
+        int32_t* o = out + g * per_engine;
+        int32_t rest[4];
+        int rest_cnt = 0;
+        for (int64_t i = 0; i < per_engine; i += 4) {
+            if (pdiff == 0) {
+                for (int j = 0; j < 4; ++j) o[i + j] = lo;
+                continue;
+            }
+            uint64_t h[2];
+            vmt_half(e[g], h);
+            uint32_t rx[4];
+            for (int j = 0; j < 4; ++j) rx[j] = (uint32_t)(h[j >> 1] >> (32 * (j & 1)));
+            if (pdiff == bitmask) {
+                for (int j = 0; j < 4; ++j) o[i + j] = (int32_t)((rx[j] & (uint32_t)bitmask) + (uint32_t)lo);
+                continue;
+            }
+            const int32_t plen = (int32_t)(uint32_t)(pdiff + 1);
+            uint64_t bitcnt = bitsize;
+            for (;;) {
+                int32_t cands[4];
+                bool m[4];
+                for (int j = 0; j < 4; ++j) {
+                    cands[j] = (int32_t)(rx[j] & (uint32_t)bitmask);
+                    m[j] = cands[j] < plen;
+                }
+                if (compress_append<4, int32_t>(cands, m, rest, rest_cnt)) {
+                    for (int j = 0; j < 4; ++j) o[i + j] = (int32_t)((uint32_t)cands[j] + (uint32_t)lo);
+                    break;
+                }
+                if (bitcnt + bitsize < 32) {
+                    for (int j = 0; j < 4; ++j) rx[j] >>= bitsize;
+                    bitcnt += bitsize;
+                } else {
+                    vmt_half(e[g], h);
+                    for (int j = 0; j < 4; ++j) rx[j] = (uint32_t)(h[j >> 1] >> (32 * (j & 1)));
+                    bitcnt = bitsize;
+                }
             }
         }
     });

@@ -203,6 +203,13 @@ def fill_randbits_i32(e, per, threads=1):
     return _half_i32(e, per, 2, 0.5, threads)
 
 
+def fill_uniform_int_i32(e, per, lo, hi, threads=1):
+    out = np.empty(len(e) * per, dtype=np.int32)
+    lib().orc_fill_uniform_int_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), C.c_int32(lo), C.c_int32(hi),
+                                   C.c_int(threads))
+    return out
+
+
 def fill_poisson_i32(e, per, mean, threads=1):
     out = np.empty(len(e) * per, dtype=np.int32)
     lib().orc_fill_poisson_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), C.c_double(mean),

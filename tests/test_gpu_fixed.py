@@ -335,3 +335,25 @@ def test_balanced_and_bernoulli_vop_match_oracle(er, orc):
     got = er.bernoulli_vop(urng, torch.from_numpy(p).cuda())
     assert_same(got.cpu().numpy(), orc.fill_bernoulli_vop_f32(ref, p))
     np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+
+
+@pytest.mark.parametrize("lo,hi", [(0, 9), (-3, 4), (0, 2), (10, 1000), (0, 7), (7, 7), (-100, 2 ** 30),
+                                   (0, 2 ** 31 - 1), (0, 2 ** 20 + 1), (-2 ** 31, -2 ** 31 + 5)])
+def test_uniform_int_matches_oracle(er, orc, lo, hi):
+    """UniformIntGen: constant, power-of-two and CompressMask ranges; odd
+    packet counts leave cached halves, and a full-draw fill in between."""
+    seeds = [3, 1000, 2 ** 40 + 5, 16380, 77]
+    urng = er.Vmt19937_64(seeds=seeds)
+    ref = orc.Engines(seeds=seeds)
+    for n in (4, 12, 312 * 8 + 4, 20000, 4):
+        assert_same(flat(er.uniform_int(n, 1, urng, lo, hi)), orc.fill_uniform_int_i32(ref, n, lo, hi))
+        np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+        assert_same(flat(er.uniform_real(8, 1, urng)), orc.fill_uniform_f32(ref, 8))
+
+
+def test_uniform_int_rejects_bad_ranges(er):
+    urng = er.Vmt19937_64(1)
+    with pytest.raises(er.ErdError):
+        er.uniform_int(8, 1, urng, 5, 4)
+    with pytest.raises(er.ErdError):
+        er.uniform_int(8, 1, urng, -2 ** 31, 2 ** 31 - 1)

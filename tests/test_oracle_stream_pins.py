@@ -360,3 +360,61 @@ def test_uniform_and_balanced_vop_are_affine_maps_of_the_stream(orc):
     thr = np.where(t >= 2.0 ** 31, -(2 ** 31), np.trunc(t)).astype(np.int64)
     np.testing.assert_array_equal(orc.fill_bernoulli_vop_f32(e.copy(), p),
                                   ((w & 0x7FFFFFFF).astype(np.int64) < thr).astype(f32))
+
+
+def _uniform_int_reference(words, lo, hi, n_packets):
+    """UniformIntGen<int32_t>::packetOp written out on the half() word stream
+    (4 words per draw) with an explicit CompressMask<16> FIFO."""
+    pdiff = hi - lo
+    bits = int(np.ceil(np.log2(pdiff + 1)))
+    mask = (1 << bits) - 1
+    it = iter(words.reshape(-1, 4))
+    out, rest = [], []
+    for _ in range(n_packets):
+        rx = [int(x) for x in next(it)]
+        if pdiff == mask:
+            out += [((x & mask) + lo) for x in rx]
+            continue
+        bitcnt = bits
+        while True:
+            cands = [x & mask for x in rx]
+            acc = [c < pdiff + 1 for c in cands]
+            if all(acc):
+                out += [c + lo for c in cands]
+                break
+            rest += [c for c, a in zip(cands, acc) if a]
+            if len(rest) >= 4:
+                out += [c + lo for c in rest[:4]]
+                rest = rest[4:]
+                break
+            if bitcnt + bits < 32:
+                rx = [x >> bits for x in rx]
+                bitcnt += bits
+            else:
+                rx = [int(x) for x in next(it)]
+                bitcnt = bits
+    return np.array(out, dtype=np.int64)
+
+
+@pytest.mark.parametrize("lo,hi", [(0, 9), (-3, 4), (0, 2), (10, 1000), (0, 7), (5, 5), (-100, 2 ** 30),
+                                   (0, 2 ** 31 - 1)])
+def test_uniform_int_of_the_stream(orc, lo, hi):
+    e = orc.Engines(n=1, seed0=17)
+    per = 2048
+    got = orc.fill_uniform_int_i32(e.copy(), per, lo, hi).astype(np.int64)
+    if hi == lo:
+        assert (got == lo).all()
+        f = e.copy()
+        orc.fill_uniform_int_i32(f, per, lo, hi)
+        np.testing.assert_array_equal(f.raw, e.raw)  # a constant range draws nothing
+        return
+    words = half_words(orc, e, 8 * per)[0]
+    np.testing.assert_array_equal(got, _uniform_int_reference(words, lo, hi, per // 4))
+    big = orc.fill_uniform_int_i32(e.copy(), 1 << 18, lo, hi).astype(np.int64)
+    assert big.min() >= lo and big.max() <= hi
+    if hi - lo < 64:  # every value with probability 1/(hi-lo+1)
+        counts = np.bincount(big - lo, minlength=hi - lo + 1)
+        exp = big.size / (hi - lo + 1)
+        assert ((counts - exp) ** 2 / exp).sum() < 3 * (hi - lo) + 30
+    else:
+        assert abs(big.mean() - (lo + hi) / 2) < 0.01 * (hi - lo)
