@@ -32,7 +32,8 @@ __all__ = [
     "bernoulli", "balanced", "geometric", "randbits_i32", "uniform_int", "uniform_real_vop", "balanced_vop", "bernoulli_vop",
     "exponential_vop", "cauchy_vop", "lognormal_vop", "student_t_vop", "weibull_vop", "extreme_value_vop",
     "binomial_vop", "gamma_vop", "poisson_vop", "run_requests", "cholesky_batch",
-    "mvnormal_batch", "wishart_batch", "dirichlet_batch", "MvNormalGen", "WishartGen",
+    "mvnormal_batch", "wishart_batch", "invwishart_batch", "dirichlet_batch", "MvNormalGen", "WishartGen",
+    "InvWishartGen",
     "MvFactors", "MvNormalBatch", "WishartBatch",
     "DirichletGen", "alias_table", "discrete", "multinomial_batch", "MultinomialGen",
 ]
@@ -540,9 +541,10 @@ def mvnormal_batch(urng, covs, samples, means=None, return_z=False, stream=None)
     return (xs, zs) if return_z else xs
 
 
-def wishart_batch(urng, scales, df, samples, stream=None):
-    """Batched WishartGen<float>(df_i, scale_i).generate(engine_i, samples_i);
-    each result is (d_i, d_i * samples_i) column-major like the reference."""
+def wishart_batch(urng, scales, df, samples, stream=None, inverse=False):
+    """Batched WishartGen<float>(df_i, scale_i).generate(engine_i, samples_i)
+    (InvWishartGen with inverse=True); each result is (d_i, d_i * samples_i)
+    column-major like the reference."""
     torch = _torch()
     n = len(scales)
     dims = np.array([np.asarray(c).shape[0] for c in scales], dtype=np.int32)
@@ -551,7 +553,8 @@ def wishart_batch(urng, scales, df, samples, stream=None):
     scale = _concat_f64(scales)
     total = int((dims.astype(np.int64) ** 2 * samples).sum())
     out = torch.empty(total, dtype=torch.float32, device="cuda")
-    _lib.call("erd_wishart_batch", urng.handle, n, dims.ctypes.data_as(C.c_void_p),
+    _lib.call("erd_invwishart_batch" if inverse else "erd_wishart_batch", urng.handle, n,
+              dims.ctypes.data_as(C.c_void_p),
               df.ctypes.data_as(C.c_void_p), samples.ctypes.data_as(C.c_void_p), _ptr(scale),
               _ptr(out), _stream_ptr(stream))
     res, off = [], 0
@@ -560,6 +563,11 @@ def wishart_batch(urng, scales, df, samples, stream=None):
         res.append(out[off:off + k].view(int(d) * int(m), int(d)).t())
         off += k
     return res
+
+
+def invwishart_batch(urng, scales, df, samples, stream=None):
+    """Batched InvWishartGen<float>(df_i, scale_i).generate(engine_i, samples_i)."""
+    return wishart_batch(urng, scales, df, samples, stream, inverse=True)
 
 
 def dirichlet_batch(urng, alphas, samples, stream=None):
@@ -588,14 +596,15 @@ class MvFactors:
     computed once on the device like the generators' constructors
     (MvNormal.h:20-38, 86-98, 196-215) and reused by every generate()."""
 
-    def __init__(self, mats, stream=None):
+    def __init__(self, mats, stream=None, inverse=False):
         n = len(mats)
         self.dims = np.array([np.asarray(c).shape[0] for c in mats], dtype=np.int32)
         a = _concat_f64(mats)
         self.info = np.zeros(n, dtype=np.int32)
         h = C.c_void_p()
-        _lib.call("erd_mv_factors_create", C.byref(h), n, self.dims.ctypes.data_as(C.c_void_p), _ptr(a),
-                  self.info.ctypes.data_as(C.c_void_p), _stream_ptr(stream))
+        _lib.call("erd_mv_factors_create_inverse" if inverse else "erd_mv_factors_create", C.byref(h), n,
+                  self.dims.ctypes.data_as(C.c_void_p), _ptr(a), self.info.ctypes.data_as(C.c_void_p),
+                  _stream_ptr(stream))
         self._h = h
 
     @property
@@ -649,10 +658,12 @@ class MvNormalBatch:
 
 
 class WishartBatch:
-    """Many WishartGen<float>(df_i, scale_i) at once (factors reused)."""
+    """Many WishartGen<float>(df_i, scale_i) at once (factors reused);
+    InvWishartGen with inverse=True (factors of the inverse scales)."""
 
-    def __init__(self, scales, df, stream=None):
-        self.factors = MvFactors(scales, stream)
+    def __init__(self, scales, df, stream=None, inverse=False):
+        self.inverse = inverse
+        self.factors = MvFactors(scales, stream, inverse=inverse)
         self.df = np.broadcast_to(np.asarray(df, dtype=np.int64), (len(scales),)).copy()
 
     def generate(self, urng, samples, out=None, stream=None, flat=False):
@@ -663,7 +674,8 @@ class WishartBatch:
         total = int((dims.astype(np.int64) ** 2 * samples).sum())
         if out is None:
             out = torch.empty(total, dtype=torch.float32, device="cuda")
-        _lib.call("erd_wishart_generate", urng.handle, self.factors.handle, self.df.ctypes.data_as(C.c_void_p),
+        _lib.call("erd_invwishart_generate" if self.inverse else "erd_wishart_generate", urng.handle,
+                  self.factors.handle, self.df.ctypes.data_as(C.c_void_p),
                   samples.ctypes.data_as(C.c_void_p), _ptr(out), _stream_ptr(stream))
         if flat:
             return out
@@ -705,6 +717,16 @@ class WishartGen:
 
     def generate(self, urng, samples):
         return self._batch.generate(urng, [samples])[0]
+
+
+class InvWishartGen(WishartGen):
+    """Rand::InvWishartGen<float>(df, scale) on the GPU (MvNormal.h:340-466);
+    the inverse scale is formed and factorised once, at construction."""
+
+    def __init__(self, df, scale):
+        self.df = int(df)
+        self.scale = np.asarray(scale, dtype=np.float64)
+        self._batch = WishartBatch([self.scale], [self.df], inverse=True)
 
 
 class DirichletGen:

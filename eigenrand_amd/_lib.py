@@ -102,6 +102,9 @@ _SIGS = {
     "erd_mv_factors_count": [P],
     "erd_mvnormal_generate": [P, P, P, P, P, P, P],
     "erd_wishart_generate": [P, P, P, P, P, P],
+    "erd_mv_factors_create_inverse": [C.POINTER(P), I64, P, P, P, P],
+    "erd_invwishart_generate": [P, P, P, P, P, P],
+    "erd_invwishart_batch": [P, I64, P, P, P, P, P, P],
     "erd_dirichlet_batch": [P, I64, P, P, P, P, P],
     "erd_alias_table": [P, I64, P, P, P],
     "erd_fill_discrete_i32": [P, P, I64, P, I64, P],

@@ -42,8 +42,16 @@ struct SyrkTile {
 };
 hipError_t launch_wishart_assemble(const MvInstance* inst, const int64_t* nrm_off, const int64_t* chi_off,
                                    const float* nrm, const float* chi, int n_inst, hipStream_t s);
+// W = T T^T per sample (Wishart), or W = U^T U with U lower triangular (utu).
 hipError_t launch_syrk_tt(const SyrkTile* tiles, int n_tiles, const MvInstance* inst, float* w,
-                          const int64_t* w_off, hipStream_t s);
+                          const int64_t* w_off, hipStream_t s, bool utu = false);
+// S^-1 from the FP64 Cholesky factor of S (w: scratch of the same size).
+hipError_t launch_spd_inverse_f64(const double* l, double* w, double* out, const int64_t* off, const int32_t* dims,
+                                  int n_inst, hipStream_t s);
+// U = T^-1 per (instance, sample, 32-column panel ta); d <= kTrtriMaxDim.
+constexpr int kTrtriMaxDim = 512;
+hipError_t launch_trtri_panel(const SyrkTile* tiles, int n_tiles, const MvInstance* inst, float* u,
+                              const int64_t* u_off, hipStream_t s);
 hipError_t launch_dirichlet_norm(const int32_t* dims, const int64_t* samples, const int64_t* g_off,
                                  const float* g, float* out, int n_inst, int64_t max_n, hipStream_t s);
 

@@ -281,6 +281,10 @@ __global__ void k_wishart_assemble(const MvInstance* __restrict__ inst, const in
 // sample, 32x32 output tile with tile row >= tile column); T is lower
 // triangular so the k loop stops at the smaller tile's last row.
 
+// UTU: W_j = U_j^T U_j instead (InvWishartGen, MvNormal.h:433-439), with U
+// lower triangular in `in.x`: the k loop runs from the larger tile's first
+// row to d, and the tiles read columns of U.
+template <bool UTU>
 __global__ __launch_bounds__(256) void k_syrk_tt(const SyrkTile* __restrict__ tiles,
                                                  const MvInstance* __restrict__ inst,
                                                  float* __restrict__ w, const int64_t* __restrict__ w_off) {
@@ -294,13 +298,27 @@ __global__ __launch_bounds__(256) void k_syrk_tt(const SyrkTile* __restrict__ ti
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
     const int a0 = tl.ta * 32, b0 = tl.tb * 32;
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    const int kend = (b0 + 32 < d ? b0 + 32 : d);  // T[b][k] = 0 for k > b
-    for (int k0 = 0; k0 < kend; k0 += 32) {
+    // T[b][k] = 0 for k > b; U[k][a] = 0 for k < a (a0 >= b0)
+    const int kbeg = UTU ? a0 : 0;
+    const int kend = UTU ? d : (b0 + 32 < d ? b0 + 32 : d);
+    for (int k0 = kbeg; k0 < kend; k0 += 32) {
         for (int q = threadIdx.x; q < 32 * 32; q += 256) {
-            const int r = q & 31, kk = q >> 5;
+            int r, kk;
+            if (UTU) {  // consecutive threads walk down a column of U
+                kk = q & 31;
+                r = q >> 5;
+            } else {
+                r = q & 31;
+                kk = q >> 5;
+            }
             const int k = k0 + kk;
-            ta_s[r][kk] = (a0 + r < d && k < d) ? T[(int64_t)k * d + a0 + r] : 0.0f;
-            tb_s[r][kk] = (b0 + r < d && k < d) ? T[(int64_t)k * d + b0 + r] : 0.0f;
+            if (UTU) {
+                ta_s[r][kk] = (a0 + r < d && k < d) ? T[(int64_t)(a0 + r) * d + k] : 0.0f;
+                tb_s[r][kk] = (b0 + r < d && k < d) ? T[(int64_t)(b0 + r) * d + k] : 0.0f;
+            } else {
+                ta_s[r][kk] = (a0 + r < d && k < d) ? T[(int64_t)k * d + a0 + r] : 0.0f;
+                tb_s[r][kk] = (b0 + r < d && k < d) ? T[(int64_t)k * d + b0 + r] : 0.0f;
+            }
         }
         __syncthreads();
 #pragma unroll 8
@@ -368,9 +386,127 @@ hipError_t launch_wishart_assemble(const MvInstance* inst, const int64_t* nrm_of
 }
 
 hipError_t launch_syrk_tt(const SyrkTile* tiles, int n_tiles, const MvInstance* inst, float* w,
-                          const int64_t* w_off, hipStream_t s) {
+                          const int64_t* w_off, hipStream_t s, bool utu) {
     if (n_tiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_syrk_tt, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles, inst, w, w_off);
+    if (utu)
+        hipLaunchKernelGGL(k_syrk_tt<true>, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles, inst, w, w_off);
+    else
+        hipLaunchKernelGGL(k_syrk_tt<false>, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles, inst, w, w_off);
+    return hipGetLastError();
+}
+
+// -------------------------------------------------------- inverse Wishart ---
+// Inverse of an SPD matrix from its FP64 Cholesky factor L (column-major,
+// upper part zero): S^-1 = L^-T L^-1, for InvWishartGen's get_lt(scale.
+// inverse()) (MvNormal.h:375-380). One workgroup per instance. Thread c owns
+// column c of L^-1 (forward substitution, rows in order; every thread reads
+// the same L[i][k] at a step), stored row-major in `w` so that a step's
+// accesses are contiguous; then S^-1[i][j] = sum_{k >= max(i,j)} L^-1[k][i]
+// L^-1[k][j], written column-major into `out`.
+__global__ __launch_bounds__(256) void k_spd_inverse_f64(const double* __restrict__ l, double* __restrict__ w,
+                                                        double* __restrict__ out, const int64_t* __restrict__ off,
+                                                        const int32_t* __restrict__ dims) {
+    const int inst = blockIdx.x;
+    const int d = dims[inst];
+    const double* L = l + off[inst];
+    double* V = w + off[inst];  // V[k*d + c] = L^-1[k][c]
+    double* S = out + off[inst];
+    for (int c = threadIdx.x; c < d; c += blockDim.x) {
+        for (int i = 0; i < d; ++i) {
+            double v = 0.0;
+            if (i == c) {
+                v = 1.0 / L[(int64_t)i * d + i];
+            } else if (i > c) {
+                double acc = 0.0;
+                for (int k = c; k < i; ++k) acc += L[(int64_t)k * d + i] * V[(int64_t)k * d + c];
+                v = -acc / L[(int64_t)i * d + i];
+            }
+            V[(int64_t)i * d + c] = v;
+        }
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < d; j += blockDim.x) {
+        for (int i = 0; i < d; ++i) {
+            double acc = 0.0;
+            for (int k = i > j ? i : j; k < d; ++k) acc += V[(int64_t)k * d + i] * V[(int64_t)k * d + j];
+            S[(int64_t)j * d + i] = acc;
+        }
+    }
+}
+
+// U = T^-1 for the lower-triangular T of each InvWishart sample
+// (MvNormal.h:433-438: t.triangularView<Lower>().solve(identity)), in FP32.
+// One workgroup per (instance, sample, 32-column panel J): forward
+// substitution by 32-row blocks I >= J, X_I = T_II^-1 (E_IJ - sum_{K=J}^{I-1}
+// T_IK X_K), with the panel X (d x 32) in LDS; rows above the panel are zero.
+__global__ __launch_bounds__(256) void k_trtri_panel(const SyrkTile* __restrict__ tiles,
+                                                     const MvInstance* __restrict__ inst, float* __restrict__ u_out,
+                                                     const int64_t* __restrict__ u_off) {
+    __shared__ float X[kTrtriMaxDim][33];
+    __shared__ float Tb[32][33];
+    const SyrkTile tl = tiles[blockIdx.x];  // ta = panel J
+    const MvInstance in = inst[tl.inst];
+    const int d = in.d;
+    const float* T = in.x + (int64_t)tl.sample * d * d;  // d x d column-major
+    float* U = u_out + u_off[tl.inst] + (int64_t)tl.sample * d * d;
+    const int J = tl.ta, nb = (d + 31) / 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // column tx, rows ty*4..ty*4+3
+    for (int I = J; I < nb; ++I) {
+        float acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = (I == J && ty * 4 + q == tx) ? 1.0f : 0.0f;
+        for (int K = J; K < I; ++K) {
+            for (int q = threadIdx.x; q < 32 * 32; q += 256) {
+                const int r = q & 31, kk = q >> 5;
+                const int row = I * 32 + r, col = K * 32 + kk;
+                Tb[r][kk] = (row < d && col < d) ? T[(int64_t)col * d + row] : 0.0f;
+            }
+            __syncthreads();
+#pragma unroll 8
+            for (int kk = 0; kk < 32; ++kk) {
+                const float xv = X[K * 32 + kk][tx];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] = fmaf(-Tb[ty * 4 + q][kk], xv, acc[q]);
+            }
+            __syncthreads();
+        }
+        // diagonal block solve: column tx by forward substitution
+        for (int q = threadIdx.x; q < 32 * 32; q += 256) {
+            const int r = q & 31, kk = q >> 5;
+            const int row = I * 32 + r, col = I * 32 + kk;
+            Tb[r][kk] = (row < d && col < d) ? T[(int64_t)col * d + row] : (r == kk ? 1.0f : 0.0f);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) X[I * 32 + ty * 4 + q][tx] = acc[q];
+        __syncthreads();
+        if (threadIdx.x < 32) {
+            for (int r = 0; r < 32; ++r) {
+                float v = X[I * 32 + r][tx];
+                for (int k = 0; k < r; ++k) v -= Tb[r][k] * X[I * 32 + k][tx];
+                X[I * 32 + r][tx] = v / Tb[r][r];
+            }
+        }
+        __syncthreads();
+    }
+    // store the panel's columns (column-major U; zeros above the panel's rows)
+    for (int q = threadIdx.x; q < 32 * d; q += 256) {
+        const int row = q % d, c = q / d;
+        const int col = J * 32 + c;
+        if (col < d) U[(int64_t)col * d + row] = row < J * 32 ? 0.0f : X[row][c];
+    }
+}
+
+hipError_t launch_spd_inverse_f64(const double* l, double* w, double* out, const int64_t* off, const int32_t* dims,
+                                  int n_inst, hipStream_t s) {
+    if (n_inst <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_spd_inverse_f64, dim3((unsigned)n_inst), dim3(256), 0, s, l, w, out, off, dims);
+    return hipGetLastError();
+}
+
+hipError_t launch_trtri_panel(const SyrkTile* tiles, int n_tiles, const MvInstance* inst, float* u,
+                              const int64_t* u_off, hipStream_t s) {
+    if (n_tiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_trtri_panel, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles, inst, u, u_off);
     return hipGetLastError();
 }
 

@@ -264,6 +264,37 @@ int erd_mv_factors_create(erd_mv_factors** out, int64_t n_inst, const int32_t* d
     return ERD_OK;
 }
 
+int erd_mv_factors_create_inverse(erd_mv_factors** out, int64_t n_inst, const int32_t* dims, const double* scale,
+                                  int32_t* info, void* stream) {
+    if (!out || n_inst <= 0 || !dims || !scale) return ERD_ERR_INVALID;
+    *out = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<int32_t> vd(dims, dims + n_inst);
+    std::vector<int64_t> off(n_inst + 1, 0);
+    for (int64_t i = 0; i < n_inst; ++i) {
+        if (vd[i] <= 0) return ERD_ERR_INVALID;
+        off[i + 1] = off[i] + (int64_t)vd[i] * vd[i];
+    }
+    // Cholesky of the scale (its info is the caller's), S^-1 from it, then the
+    // factors of S^-1 as for MvNormalGen.
+    Scratch sc(s);
+    double *d_a, *d_w, *d_inv;
+    int64_t* d_off;
+    int32_t* d_dims;
+    if (sc.alloc(&d_a, off[n_inst]) || sc.alloc(&d_w, off[n_inst]) || sc.alloc(&d_inv, off[n_inst]) ||
+        sc.upload(&d_off, off) || sc.upload(&d_dims, vd))
+        return ERD_ERR_OOM;
+    if (hipMemcpyAsync(d_a, scale, sizeof(double) * off[n_inst], hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return ERD_ERR_HIP;
+    std::vector<int32_t> inf;
+    std::vector<int64_t> so;
+    int rc = factor_and_split(vd, d_a, off, nullptr, nullptr, nullptr, so, &inf, s, sc);
+    if (info && !inf.empty()) std::memcpy(info, inf.data(), sizeof(int32_t) * (size_t)n_inst);
+    if (rc) return rc;
+    if (launch_spd_inverse_f64(d_a, d_w, d_inv, d_off, d_dims, (int)n_inst, s) != hipSuccess) return ERD_ERR_HIP;
+    return erd_mv_factors_create(out, n_inst, dims, d_inv, nullptr, stream);
+}
+
 int erd_mv_factors_destroy(erd_mv_factors* f) {
     if (!f) return ERD_OK;
     // The planes may still be read by work on any stream: wait for the device.
@@ -327,8 +358,8 @@ int erd_mvnormal_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, cons
     return rc ? rc : rc2;
 }
 
-int erd_wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64_t* df, const int64_t* samples,
-                         float* out, void* stream) {
+static int wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64_t* df, const int64_t* samples,
+                            float* out, void* stream, bool inverse) {
     if (!e || !e->states || !f || !df || !samples || !out) return ERD_ERR_INVALID;
     const int64_t n_inst = (int64_t)f->dims.size();
     if (n_inst > e->count) return ERD_ERR_INVALID;
@@ -339,6 +370,7 @@ int erd_wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64_t*
         const int64_t d = vd[i], n = samples[i];
         // df > d - 1 (MvNormal.h:205); fills need whole packets
         if (n < 0 || df[i] <= d - 1 || n % 8 != 0) return ERD_ERR_INVALID;
+        if (inverse && d > kTrtriMaxDim) return ERD_ERR_UNSUPPORTED;
         aoff[i + 1] = aoff[i] + d * d * n;
         nrm_off[i + 1] = nrm_off[i] + n * d * (d - 1) / 2;
         chi_off[i + 1] = chi_off[i] + d * n;
@@ -396,8 +428,52 @@ int erd_wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64_t*
     }
     SyrkTile* d_tiles;
     if (sc.upload(&d_tiles, tiles) != hipSuccess) return ERD_ERR_HIP;
+    if (inverse) {
+        // InvWishartGen (MvNormal.h:427-440): U_j = T_j^-1 into amat, then
+        // W_j = U_j^T U_j from U.
+        std::vector<SyrkTile> panels;
+        for (int64_t i = 0; i < n_inst; ++i)
+            for (int64_t j = 0; j < samples[i]; ++j)
+                for (int p = 0; p < (vd[i] + 31) / 32; ++p) panels.push_back(SyrkTile{(int32_t)i, (int32_t)j, p, 0});
+        std::vector<MvInstance> uinst = inst;
+        for (int64_t i = 0; i < n_inst; ++i) uinst[i].x = amat + aoff[i];
+        SyrkTile* d_panels;
+        MvInstance* d_uinst;
+        if (sc.upload(&d_panels, panels) || sc.upload(&d_uinst, uinst)) return ERD_ERR_HIP;
+        if (launch_trtri_panel(d_panels, (int)panels.size(), d_inst, amat, d_aoff, s) != hipSuccess)
+            return ERD_ERR_HIP;
+        if (launch_syrk_tt(d_tiles, (int)tiles.size(), d_uinst, out, d_aoff, s, true) != hipSuccess)
+            return ERD_ERR_HIP;
+        return ERD_OK;
+    }
     if (launch_syrk_tt(d_tiles, (int)tiles.size(), d_inst, out, d_aoff, s) != hipSuccess) return ERD_ERR_HIP;
     return ERD_OK;
+}
+
+int erd_wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64_t* df, const int64_t* samples,
+                         float* out, void* stream) {
+    return wishart_generate(e, f, df, samples, out, stream, false);
+}
+
+int erd_invwishart_generate(erd_engines* e, const erd_mv_factors* f, const int64_t* df, const int64_t* samples,
+                            float* out, void* stream) {
+    return wishart_generate(e, f, df, samples, out, stream, true);
+}
+
+int erd_invwishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* df,
+                         const int64_t* samples, const double* scale, float* out, void* stream) {
+    if (!e || !e->states || n_inst <= 0 || n_inst > e->count || !dims || !df || !samples || !scale || !out)
+        return ERD_ERR_INVALID;
+    for (int64_t i = 0; i < n_inst; ++i) {
+        if (dims[i] <= 0 || samples[i] < 0 || df[i] <= dims[i] - 1 || samples[i] % 8 != 0) return ERD_ERR_INVALID;
+        if (dims[i] > kTrtriMaxDim) return ERD_ERR_UNSUPPORTED;
+    }
+    erd_mv_factors* f;
+    int rc = erd_mv_factors_create_inverse(&f, n_inst, dims, scale, nullptr, stream);
+    if (rc) return rc;
+    rc = erd_invwishart_generate(e, f, df, samples, out, stream);
+    const int rc2 = release_factors(f, (hipStream_t)stream);
+    return rc ? rc : rc2;
 }
 
 int erd_wishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* df,

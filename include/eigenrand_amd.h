@@ -326,6 +326,18 @@ int erd_wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64_t*
  * df_i > d_i - 1. scale: DEVICE doubles as for cov above. */
 int erd_wishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* df,
                       const int64_t* samples, const double* scale, float* out, void* stream);
+/* InvWishartGen<float>(df_i, scale_i) (MvNormal.h:340-466): factors of the
+ * inverse scale (get_lt(scale.inverse()), MvNormal.h:375-380, computed in FP64
+ * on the device from the Cholesky factor of the scale; info and NOT_PSD refer
+ * to that factorisation), then per sample the Wishart matrix T = L A, U =
+ * T^-1 (lower triangular) and out = U^T U. Arguments and layout as for
+ * erd_wishart_batch; d_i <= 512 (ERD_ERR_UNSUPPORTED beyond). */
+int erd_mv_factors_create_inverse(erd_mv_factors** out, int64_t n_inst, const int32_t* dims, const double* scale,
+                                  int32_t* info, void* stream);
+int erd_invwishart_generate(erd_engines* e, const erd_mv_factors* f, const int64_t* df, const int64_t* samples,
+                            float* out, void* stream);
+int erd_invwishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* df,
+                         const int64_t* samples, const double* scale, float* out, void* stream);
 
 /* DirichletGen<float>(alpha_i).generate(engine_i, samples_i)
  * (Multinomial.h:159-217): out_i is K_i x samples_i column-major. alpha: HOST

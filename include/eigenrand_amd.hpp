@@ -453,13 +453,18 @@ private:
 };
 
 // Many WishartGen<float>(df_i, scale_i) at once (MvNormal.h:184-302).
+// InvWishartGen with inverse = true (factors of the inverse scales).
 class WishartBatch {
 public:
     WishartBatch(const std::vector<int32_t>& dims, const std::vector<int64_t>& df, const double* scale_device,
-                 void* stream = nullptr)
-        : dims_(dims), df_(df) {
-        check(erd_mv_factors_create(&f_, (int64_t)dims.size(), dims.data(), scale_device, nullptr, stream),
-              "erd_mv_factors_create");
+                 void* stream = nullptr, bool inverse = false)
+        : dims_(dims), df_(df), inverse_(inverse) {
+        if (inverse)
+            check(erd_mv_factors_create_inverse(&f_, (int64_t)dims.size(), dims.data(), scale_device, nullptr, stream),
+                  "erd_mv_factors_create_inverse");
+        else
+            check(erd_mv_factors_create(&f_, (int64_t)dims.size(), dims.data(), scale_device, nullptr, stream),
+                  "erd_mv_factors_create");
     }
     ~WishartBatch() {
         if (f_) (void)erd_mv_factors_destroy(f_);
@@ -470,14 +475,19 @@ public:
         int64_t total = 0;
         for (size_t i = 0; i < dims_.size(); ++i) total += (int64_t)dims_[i] * dims_[i] * samples[i];
         DeviceArray<float> out(total);
-        check(erd_wishart_generate(urng.handle(), f_, df_.data(), samples.data(), out.data(), stream),
-              "erd_wishart_generate");
+        if (inverse_)
+            check(erd_invwishart_generate(urng.handle(), f_, df_.data(), samples.data(), out.data(), stream),
+                  "erd_invwishart_generate");
+        else
+            check(erd_wishart_generate(urng.handle(), f_, df_.data(), samples.data(), out.data(), stream),
+                  "erd_wishart_generate");
         return out;
     }
 
 private:
     std::vector<int32_t> dims_;
     std::vector<int64_t> df_;
+    bool inverse_ = false;
     erd_mv_factors* f_ = nullptr;
 };
 

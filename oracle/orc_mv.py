@@ -49,6 +49,20 @@ def wishart(eng, df, scale, samples):
     return w, rand_mat
 
 
+def invwishart(eng, df, scale, samples):
+    """InvWishartGen::generate (MvNormal.h:404-440) with float64 products:
+    the same random matrices as WishartGen, L = cholesky(inv(scale)),
+    T = L A, U = T^-1, W = U^T U. Returns (W64, A)."""
+    d = scale.shape[0]
+    _, a = wishart(eng, df, np.eye(d), samples)
+    t = cholesky(np.linalg.inv(scale)) @ a.astype(np.float64)
+    w = np.empty((d, d * samples))
+    for j in range(samples):
+        u = np.linalg.inv(t[:, j * d:(j + 1) * d])
+        w[:, j * d:(j + 1) * d] = u.T @ u
+    return w, a
+
+
 def dirichlet(eng, alpha, samples):
     """DirichletGen::generate (Multinomial.h:190-203): one gamma fill per
     category, then column normalisation (sum in float64). Returns (ret64, G)."""

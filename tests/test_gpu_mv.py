@@ -103,6 +103,39 @@ def test_wishart_batch_against_fp64(er, orc):
             assert np.array_equal(w[:, j * d:(j + 1) * d], w[:, j * d:(j + 1) * d].T)  # symmetric
 
 
+def test_invwishart_batch_against_fp64(er, orc):
+    """InvWishartGen: W = U^T U with U = (L A)^-1, L = chol(scale^-1). The
+    FP32 triangular inverse amplifies the split-BF16 error of T by the
+    condition of T; with these scales the relative error stays below 2^-10."""
+    import orc_mv
+    rng = np.random.default_rng(8)
+    dims = [4, 7, 33, 64, 100]
+    scales = [rand_cov(rng, d) for d in dims]
+    df = [d + 10 for d in dims]
+    samples = [16, 8, 8, 8, 8]
+    seeds = [51, 61, 71, 81, 91]
+    urng = er.Vmt19937_64(seeds=seeds)
+    ws = er.invwishart_batch(urng, scales, df, samples)
+    for i, d in enumerate(dims):
+        w64, _ = orc_mv.invwishart(orc.Engines(seeds=[seeds[i]]), df[i], scales[i], samples[i])
+        w = ws[i].cpu().numpy().astype(np.float64)
+        for j in range(samples[i]):
+            ref = w64[:, j * d:(j + 1) * d]
+            got = w[:, j * d:(j + 1) * d]
+            assert np.abs(got - ref).max() <= 2.0 ** -10 * np.abs(ref).max(), (d, j)
+            assert np.array_equal(got, got.T)
+
+
+def test_invwishart_mean(er):
+    """E[W] = scale / (df - d - 1) for the inverse Wishart, within 10%."""
+    rng = np.random.default_rng(9)
+    d, df, n = 4, 12, 8192
+    scale = rand_cov(rng, d)
+    w = er.InvWishartGen(df, scale).generate(er.Vmt19937_64(13), n).cpu().numpy().astype(np.float64)
+    mean = w.reshape(d, n, d).mean(axis=1)
+    assert np.allclose(mean, scale / (df - d - 1), rtol=0.1, atol=0.1 * np.abs(scale).max() / (df - d - 1))
+
+
 def test_wishart_mean(er):
     """test_mv.cpp:124-150: E[W] = df * scale within 10%."""
     rng = np.random.default_rng(4)
