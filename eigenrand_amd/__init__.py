@@ -760,14 +760,15 @@ def alias_table(weights):
     return ("alias", prob, alias)
 
 
-def discrete(rows, cols, urng, weights, out=None, stream=None):
-    """Rand::discrete<MatrixXi>(rows, cols, urng, weights) (int32)."""
+def discrete(rows, cols, urng, weights, out=None, stream=None, precision="int32"):
+    """Rand::discrete<MatrixXi>(rows, cols, urng, weights) (int32 precision);
+    precision="float" is Rand::discreteF (DiscreteGen<int32_t, float>)."""
     torch = _torch()
     w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64))
     per = int(rows) * int(cols)
     buf = _alloc(urng, per, torch.int32, out)
-    _lib.call("erd_fill_discrete_i32", urng.handle, _ptr(buf), per, w.ctypes.data_as(C.c_void_p), w.size,
-              _stream_ptr(stream))
+    name = {"int32": "erd_fill_discrete_i32", "float": "erd_fill_discrete_f_i32"}[precision]
+    _lib.call(name, urng.handle, _ptr(buf), per, w.ctypes.data_as(C.c_void_p), w.size, _stream_ptr(stream))
     return _shape(buf, urng, rows, cols) if out is None else buf
 
 

@@ -375,6 +375,7 @@ struct StudentTVopF32 {
 
 // BernoulliGen<int32_t>::packetOp (Basic.h:693-707) at Packet4i.
 struct BernoulliI32 {
+    static constexpr int SLOTS = 1;
     int32_t* out;
     int32_t p;
     __device__ void emit(int64_t i, const uint64_t* x) const {
@@ -390,6 +391,7 @@ struct BernoulliI32 {
 // GeometricGen::packetOp (Discrete.h:1372-1380): cvtt(trunc(log(1 - U) *
 // rlog_q)) with U from the half draw and rlog_q = 1 / log(1 - p) in float.
 struct GeometricI32 {
+    static constexpr int SLOTS = 1;
     int32_t* out;
     float rlog_q;
     __device__ int32_t one(uint32_t w) const {
@@ -404,6 +406,7 @@ struct GeometricI32 {
 // UniformIntGen<int32_t>::packetOp when pdiff + 1 is a power of two
 // (Discrete.h:303-306): (rx & bitmask) + min from one half draw.
 struct UniformIntPow2I32 {
+    static constexpr int SLOTS = 1;
     int32_t* out;
     uint32_t mask;
     int32_t pmin;
@@ -414,8 +417,38 @@ struct UniformIntPow2I32 {
     }
 };
 
+// DiscreteGen<int32_t, float>::packetOp (Discrete.h:579-601) at Packet4i.
+// K < 16 (SLOTS 1): K - #{c in cdf : U < c} with U from one half draw.
+// Otherwise (SLOTS 2): idx = bits & bitmask from one half draw, U from the
+// next; idx if U < prob[idx] (float table), else alias[idx].
+template <int SLOTS_>
+struct DiscreteFI32 {
+    static constexpr int SLOTS = SLOTS_;
+    int32_t* out;
+    const float* cdf;  // or prob
+    const int32_t* alias;
+    int32_t k;         // cdf size
+    uint32_t bitmask;
+    __device__ int32_t one(const uint64_t* x, int j) const {
+        const uint32_t w0 = (j & 1) ? hi32(x[j >> 1]) : lo32(x[j >> 1]);
+        if (SLOTS == 1) {
+            const float u = ur_f32(w0);
+            int32_t r = k;
+            for (int i = 0; i < k; ++i) r -= u < cdf[i] ? 1 : 0;
+            return r;
+        }
+        const uint32_t idx = w0 & bitmask;
+        const float u = ur_f32((j & 1) ? hi32(x[2 + (j >> 1)]) : lo32(x[2 + (j >> 1)]));
+        return u < cdf[idx] ? (int32_t)idx : alias[idx];
+    }
+    __device__ void emit(int64_t i, const uint64_t* x) const {
+        *reinterpret_cast<int4*>(out + i) = make_int4(one(x, 0), one(x, 1), one(x, 2), one(x, 3));
+    }
+};
+
 // RandbitsGen<int32_t>::packetOp (Basic.h:394-400) at Packet4i: the half's bits.
 struct RandbitsI32 {
+    static constexpr int SLOTS = 1;
     int32_t* out;
     __device__ void emit(int64_t i, const uint64_t* x) const {
         *reinterpret_cast<int4*>(out + i) =

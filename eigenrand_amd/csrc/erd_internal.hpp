@@ -77,6 +77,8 @@ struct HostTable {
     std::vector<uint32_t> cdf;
     std::vector<int32_t> prob, alias;
     uint32_t bitmask = 0;
+    // DiscreteGen<int32_t, float> (Discrete.h:492-533): the same tables in float
+    std::vector<float> fcdf, fprob;
 };
 
 // Segment programs (capi.hip).

@@ -77,6 +77,9 @@ hipError_t launch_fill_bernoulli_i32(EngineState* st, int64_t n, int32_t* out, i
 hipError_t launch_fill_geometric_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, float rlog_q,
                                      hipStream_t s);
 hipError_t launch_fill_randbits_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, hipStream_t s);
+// DiscreteGen<int32_t, float>: cdf (k > 0, alias NULL) or float prob + alias tables.
+hipError_t launch_fill_discrete_f_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, const float* tab,
+                                      const int32_t* alias, int32_t k, uint32_t bitmask, hipStream_t s);
 // UniformIntGen<int32_t>(lo, lo + pdiff), pdiff < 2^31.
 hipError_t launch_fill_uniform_int_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, int32_t lo,
                                        uint32_t pdiff, hipStream_t s);

@@ -381,23 +381,31 @@ __device__ __forceinline__ void cursor_end(Eng& e, const HalfCursor& c) {
     half_end(e, c.pos, c.nc, c.used);
 }
 
-// Fixed half-draw consumers (one half() per Packet4i): packet k of the fill
-// takes virtual slot k (the engine's cached half first, if any), 64 packets
-// per wave step; the last odd half leaves the draw's upper half cached.
+// Fixed half-draw consumers (D::SLOTS half() draws per Packet4i): packet k
+// of the fill takes virtual slots [k*SLOTS, (k+1)*SLOTS) (the engine's cached
+// half first, if any), 64 packets per wave step; a last odd half leaves the
+// draw's upper half cached. emit(i, x) gets the tempered words x[2*s + q]
+// (slot s, word q).
 template <class D>
 __device__ void op_half_fixed(Eng& e, uint64_t* carry, int64_t T, const D& d, int lane) {
+    constexpr int S = D::SLOTS;
     if (T == 0) return;
     HalfCursor cur = cursor_begin(e, carry, lane);
     int64_t done = 0;
     while (done < T) {
-        cur.ensure(1, lane);
-        const int64_t take = min((int64_t)cur.avail(), T - done);
+        cur.ensure(S, lane);
+        const int64_t take = min((int64_t)(cur.avail() / S), T - done);
         for (int v = lane; v < take; v += 64) {
-            const uint64_t x[2] = {cur.word(v, 0), cur.word(v, 1)};
+            uint64_t x[2 * S];
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                x[2 * k] = cur.word(v * S + k, 0);
+                x[2 * k + 1] = cur.word(v * S + k, 1);
+            }
             d.emit(4 * (done + v), x);
         }
         __syncthreads();  // carried slots are read before ensure() rewrites them
-        cur.advance((int)take);
+        cur.advance((int)take * S);
         done += take;
     }
     cursor_end(e, cur);

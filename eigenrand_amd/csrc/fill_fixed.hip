@@ -280,6 +280,12 @@ hipError_t launch_fill_geometric_i32(EngineState* st, int64_t n, int32_t* out, i
     return launch_half(st, n, out, per, GeometricI32{nullptr, rlog_q}, s);
 }
 
+hipError_t launch_fill_discrete_f_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, const float* tab,
+                                      const int32_t* alias, int32_t k, uint32_t bitmask, hipStream_t s) {
+    if (!alias) return launch_half(st, n, out, per, DiscreteFI32<1>{nullptr, tab, nullptr, k, 0u}, s);
+    return launch_half(st, n, out, per, DiscreteFI32<2>{nullptr, tab, alias, 0, bitmask}, s);
+}
+
 hipError_t launch_fill_randbits_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, hipStream_t s) {
     return launch_half(st, n, out, per, RandbitsI32{nullptr}, s);
 }

@@ -548,13 +548,17 @@ int build_table(const double* w, int64_t K, HostTable& t) {
         std::vector<double> c;
         double acc = 0;
         for (int64_t i = 0; i < K; ++i) c.push_back(acc += w[i]);
-        for (double p : c) t.cdf.push_back((uint32_t)(p / c.back() * 0x80000000));
+        for (double p : c) {
+            t.cdf.push_back((uint32_t)(p / c.back() * 0x80000000));
+            t.fcdf.push_back((float)(p / c.back()));
+        }
         return ERD_OK;
     }
     const size_t msize = (size_t)K;
     const size_t nb = (size_t)std::ceil(std::log2((double)msize));
     const size_t psize = (size_t)1 << nb;
     t.prob.assign(psize, 0);
+    t.fprob.assign(psize, 0.0f);
     t.alias.assign(psize, 0);
     t.bitmask = (uint32_t)(psize - 1);
     const double mean = sum / psize;
@@ -567,6 +571,7 @@ int build_table(const double* w, int64_t K, HostTable& t) {
     mm = under + 1;
     while (over < psize && under < psize) {
         t.prob[under] = (int32_t)(f[under] * (imax + 1.0));
+        t.fprob[under] = (float)f[under];
         t.alias[under] = (int32_t)over;
         f[over] += f[under] - 1;
         if (f[over] >= 1 || mm <= over) {
@@ -581,14 +586,17 @@ int build_table(const double* w, int64_t K, HostTable& t) {
     for (; over < psize; ++over)
         if (f[over] >= 1) {
             t.prob[over] = imax;
+            t.fprob[over] = 1.0f;
             t.alias[over] = (int32_t)over;
         }
     if (under < psize) {
         t.prob[under] = imax;
+        t.fprob[under] = 1.0f;
         t.alias[under] = (int32_t)under;
         for (under = mm; under < msize; ++under)
             if (f[under] < 1) {
                 t.prob[under] = imax;
+                t.fprob[under] = 1.0f;
                 t.alias[under] = (int32_t)under;
             }
     }
@@ -669,6 +677,27 @@ int erd_fill_discrete_i32(erd_engines* e, int32_t* out, int64_t per, const doubl
         eng.push_back((int32_t)g);
     }
     return run_segments(e, segs, eng, s);
+}
+
+int erd_fill_discrete_f_i32(erd_engines* e, int32_t* out, int64_t per, const double* weights, int64_t K,
+                            void* stream) {
+    if (!e || !e->states || per < 0 || per % 4 != 0 || (per > 0 && !out) || !weights) return ERD_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    HostTable t;
+    int rc = build_table(weights, K, t);
+    if (rc) return rc;
+    Scratch sc(s);
+    float* tab;
+    int32_t* alias = nullptr;
+    if (!t.fcdf.empty()) {
+        if (sc.upload(&tab, t.fcdf) != hipSuccess) return ERD_ERR_HIP;
+    } else if (sc.upload(&tab, t.fprob) != hipSuccess || sc.upload(&alias, t.alias) != hipSuccess) {
+        return ERD_ERR_HIP;
+    }
+    return launch_fill_discrete_f_i32(e->states, e->count, out, per, tab, alias, (int32_t)t.fcdf.size(), t.bitmask,
+                                      s) == hipSuccess
+               ? ERD_OK
+               : ERD_ERR_HIP;
 }
 
 int erd_multinomial_batch(erd_engines* e, int64_t n_inst, const int64_t* dims, const int32_t* trials,

@@ -357,6 +357,12 @@ int erd_alias_table(const double* weights, int64_t K, int32_t* prob, int32_t* al
  * weights: HOST doubles; per_engine a multiple of 4. */
 int erd_fill_discrete_i32(erd_engines* e, int32_t* out, int64_t per_engine, const double* weights, int64_t K,
                           void* stream);
+/* discreteF<int32>(…, weights): DiscreteGen<int32_t, float> (Discrete.h:
+ * 486-603) at Packet4i: K < 16 counts a half draw's uniform against the float
+ * CDF; otherwise the alias table in float, index bits from one half draw and
+ * the uniform from the next. weights: HOST doubles (K). */
+int erd_fill_discrete_f_i32(erd_engines* e, int32_t* out, int64_t per_engine, const double* weights, int64_t K,
+                            void* stream);
 
 /* MultinomialGen<int32_t>(trials_i, weights_i).generate(engine_i, samples_i)
  * (Multinomial.h:25-134): out_i is K_i x samples_i column-major int32.

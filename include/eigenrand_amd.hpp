@@ -289,6 +289,13 @@ inline DeviceMatrix<int32_t> uniformInt(int64_t rows, int64_t cols, Vmt19937_64&
         return erd_fill_uniform_int_i32(urng.handle(), o, n, min, max, stream);
     }, "erd_fill_uniform_int_i32");
 }
+// discreteF<int32>: DiscreteGen<int32_t, float>; weights on the host.
+inline DeviceMatrix<int32_t> discreteF(int64_t rows, int64_t cols, Vmt19937_64& urng, const std::vector<double>& w,
+                                       void* stream = nullptr) {
+    return detail::fill<int32_t>(rows, cols, urng.size(), [&](int32_t* o, int64_t n) {
+        return erd_fill_discrete_f_i32(urng.handle(), o, n, w.data(), (int64_t)w.size(), stream);
+    }, "erd_fill_discrete_f_i32");
+}
 inline DeviceMatrix<float> bernoulli(int64_t rows, int64_t cols, Vmt19937_64& urng, double p = 0.5,
                                      void* stream = nullptr) {
     return detail::fill<float>(rows, cols, urng.size(), [&](float* o, int64_t n) {

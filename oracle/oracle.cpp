@@ -807,6 +807,39 @@ void orc_fill_uniform_int_i32(void* states, int64_t n_engines, int32_t* out, int
     });
 }
 
+// DiscreteGen<int32_t, float> (Discrete.h:486-603) at Packet4i: K < 16: a
+// float CDF (cdf[i] = float(c_i / c_K)) counted against a half draw's
+// uniform; otherwise AliasMethod<float, int32_t> (Discrete.h:23-207 with the
+// float branch of buildTable), index bits from one half draw, the uniform
+// from the next. Tables come from orc_alias_table_f32.
+void orc_fill_discrete_f_i32(void* states, int64_t n_engines, int32_t* out, int64_t per_engine, const float* cdf,
+                             int32_t k, const float* prob, const int32_t* alias, uint32_t bitmask) {
+    Vmt* e = (Vmt*)states;
+    for (int64_t g = 0; g < n_engines; ++g) {
+        int32_t* o = out + g * per_engine;
+        for (int64_t i = 0; i < per_engine; i += 4) {
+            uint64_t h[2];
+            vmt_half(e[g], h);
+            if (cdf) {
+                for (int j = 0; j < 4; ++j) {
+                    const float u = bit_to_ur_float((uint32_t)(h[j >> 1] >> (32 * (j & 1))));
+                    int32_t r = k;
+                    for (int c = 0; c < k; ++c) r -= u < cdf[c] ? 1 : 0;
+                    o[i + j] = r;
+                }
+            } else {
+                uint32_t idx[4];
+                for (int j = 0; j < 4; ++j) idx[j] = (uint32_t)(h[j >> 1] >> (32 * (j & 1))) & bitmask;
+                vmt_half(e[g], h);
+                for (int j = 0; j < 4; ++j) {
+                    const float u = bit_to_ur_float((uint32_t)(h[j >> 1] >> (32 * (j & 1))));
+                    o[i + j] = u < prob[idx[j]] ? (int32_t)idx[j] : alias[idx[j]];
+                }
+            }
+        }
+    }
+}
+
 void orc_fill_cauchy_f32(void* states, int64_t n_engines, float* out, int64_t per_engine,
                          float a, float b, int threads) {
     Vmt* e = (Vmt*)states;
@@ -1127,6 +1160,7 @@ namespace {
 // AliasMethod<int32_t, int32_t>::buildTable (Discrete.h:78-192), in double.
 struct AliasTable {
     std::vector<int32_t> prob, alias;
+    std::vector<float> fprob;  // AliasMethod<float, int32_t>: (float)f, or 1
     size_t bitsize = 0, bitmask = 0;
 };
 
@@ -1137,6 +1171,7 @@ bool build_alias(const double* w, size_t msize, AliasTable& t) {
     const size_t nbsize = (size_t)std::ceil(std::log2((double)msize));
     const size_t psize = (size_t)1 << nbsize;
     t.prob.assign(psize, 0);
+    t.fprob.assign(psize, 0.0f);
     t.alias.assign(psize, 0);
     t.bitsize = nbsize;
     t.bitmask = ((size_t)(-1)) >> (sizeof(size_t) * 8 - nbsize);
@@ -1150,6 +1185,7 @@ bool build_alias(const double* w, size_t msize, AliasTable& t) {
     mm = under + 1;
     while (over < psize && under < psize) {
         t.prob[under] = (int32_t)(f[under] * (imax + 1.0));
+        t.fprob[under] = (float)f[under];
         t.alias[under] = (int32_t)over;
         f[over] += f[under] - 1;
         if (f[over] >= 1 || mm <= over) {
@@ -1163,15 +1199,18 @@ bool build_alias(const double* w, size_t msize, AliasTable& t) {
     for (; over < psize; ++over) {
         if (f[over] >= 1) {
             t.prob[over] = imax;
+            t.fprob[over] = 1.0f;
             t.alias[over] = (int32_t)over;
         }
     }
     if (under < psize) {
         t.prob[under] = imax;
+        t.fprob[under] = 1.0f;
         t.alias[under] = (int32_t)under;
         for (under = mm; under < msize; ++under) {
             if (f[under] < 1) {
                 t.prob[under] = imax;
+                t.fprob[under] = 1.0f;
                 t.alias[under] = (int32_t)under;
             }
         }
@@ -1293,6 +1332,15 @@ int orc_alias_table(const double* w, int64_t K, int32_t* prob, int32_t* alias) {
     AliasTable t;
     if (!build_alias(w, (size_t)K, t)) return -1;
     std::memcpy(prob, t.prob.data(), 4 * t.prob.size());
+    std::memcpy(alias, t.alias.data(), 4 * t.alias.size());
+    return (int)t.bitsize;
+}
+
+// AliasMethod<float, int32_t> (float branch of buildTable, Discrete.h:119-183).
+int orc_alias_table_f32(const double* w, int64_t K, float* prob, int32_t* alias) {
+    AliasTable t;
+    if (!build_alias(w, (size_t)K, t)) return -1;
+    std::memcpy(prob, t.fprob.data(), 4 * t.fprob.size());
     std::memcpy(alias, t.alias.data(), 4 * t.alias.size());
     return (int)t.bitsize;
 }

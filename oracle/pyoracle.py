@@ -303,6 +303,27 @@ def alias_table(w):
     return prob, alias
 
 
+def fill_discrete_f_i32(e, per, w):
+    """DiscreteGen<int32_t, float>: float CDF for K < 16 (cdf_i = float(c_i /
+    c_K), Discrete.h:513-525), else the float alias table."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    out = np.empty(len(e) * per, dtype=np.int32)
+    if w.size < 16:
+        c = np.cumsum(w)
+        cdf = np.ascontiguousarray((c / c[-1]).astype(np.float32))
+        lib().orc_fill_discrete_f_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), _p(cdf), C.c_int32(w.size),
+                                      None, None, C.c_uint32(0))
+        return out
+    size = 1 << int(np.ceil(np.log2(w.size)))
+    prob = np.zeros(size, dtype=np.float32)
+    alias = np.zeros(size, dtype=np.int32)
+    bits = lib().orc_alias_table_f32(_p(w), I64(w.size), _p(prob), _p(alias))
+    assert bits >= 0
+    lib().orc_fill_discrete_f_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), None, C.c_int32(0), _p(prob),
+                                  _p(alias), C.c_uint32(size - 1))
+    return out
+
+
 def fill_discrete_i32(e, per, w):
     w = np.ascontiguousarray(w, dtype=np.float64)
     out = np.empty(len(e) * per, dtype=np.int32)

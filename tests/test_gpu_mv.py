@@ -198,6 +198,21 @@ def test_discrete_fill_matches_oracle(er, orc, K):
     np.testing.assert_array_equal(urng.snapshot(), ref.raw)
 
 
+@pytest.mark.parametrize("K", [3, 15, 16, 37, 1000, 10000])
+def test_discrete_f_fill_matches_oracle(er, orc, K):
+    """discreteF (float tables): one half draw per packet for K < 16, two
+    otherwise; odd packet counts leave cached halves across fills."""
+    rng = np.random.default_rng(K + 1)
+    w = rng.uniform(0, 1, K)
+    seeds = [1, 2, 3]
+    urng = er.Vmt19937_64(seeds=seeds)
+    ref = orc.Engines(seeds=seeds)
+    for n in (4, 12, 1004, 8, 4):
+        out = er.discrete(n, 1, urng, w, precision="float")
+        assert_same(out.reshape(len(seeds), n).cpu().numpy(), orc.fill_discrete_f_i32(ref, n, w))
+        np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+
+
 @pytest.mark.parametrize("cases", [
     [(20, 4, 16), (99, 10, 24), (7, 3, 8), (100, 1, 8)],           # per-sample discrete draws
     [(1000, 4, 64), (5000, 100, 16), (3001, 3, 8)],                # BinomialVGen<float> rows

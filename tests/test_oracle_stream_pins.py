@@ -418,3 +418,28 @@ def test_uniform_int_of_the_stream(orc, lo, hi):
         assert ((counts - exp) ** 2 / exp).sum() < 3 * (hi - lo) + 30
     else:
         assert abs(big.mean() - (lo + hi) / 2) < 0.01 * (hi - lo)
+
+
+@pytest.mark.parametrize("K", [3, 15, 16, 37, 1000])
+def test_discrete_f_of_the_stream(orc, K):
+    """discreteF: K < 16 counts a half draw's uniform against the float CDF
+    (the category is the first c_i above U); K >= 16 takes 2^b-masked bits
+    from one half draw and U from the next against the float alias table;
+    frequencies match the weights."""
+    rng = np.random.default_rng(K)
+    w = rng.uniform(0.1, 1, K)
+    e = orc.Engines(n=1, seed0=23)
+    per = 4096
+    got = orc.fill_discrete_f_i32(e.copy(), per, w)
+    h = half_words(orc, e, 2 * per // 4)[0]
+    if K < 16:
+        c = np.cumsum(w)
+        cdf = (c / c[-1]).astype(f32)
+        u = ur(h[:per // 4]).reshape(-1)
+        exp = np.array([K - int((x < cdf).sum()) for x in u])
+        np.testing.assert_array_equal(got, exp)
+    else:
+        assert got.min() >= 0 and got.max() < K
+    big = orc.fill_discrete_f_i32(e.copy(), 1 << 18, w)
+    freq = np.bincount(big, minlength=K) / big.size
+    assert np.abs(freq - w / w.sum()).max() < 6 * np.sqrt(w.max() / w.sum() / big.size) + 1e-3
