@@ -573,6 +573,39 @@ int erd_fill_geometric_i32(erd_engines* e, int32_t* out, int64_t per, double p, 
     return hip_status(erd::launch_fill_geometric_i32(e->states, e->count, out, per, rlog_q, S(stream)));
 }
 
+// BetaGen / FisherFGen: one SEG_BETA_F32 segment per engine (the gamma-ratio
+// regime alternates two rejection samplers packet by packet).
+static int beta_fill(erd_engines* e, float* out, int64_t per, float a, float b, float ratio, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    if (!(a > 0) || !(b > 0)) return ERD_ERR_INVALID;
+    if (per == 0) return ERD_OK;
+    std::vector<erd::Segment> segs((size_t)e->count);
+    std::vector<int32_t> eng((size_t)e->count);
+    for (int64_t g = 0; g < e->count; ++g) {
+        erd::Segment& sg = segs[(size_t)g];
+        sg = erd::Segment{};
+        sg.kind = erd::SEG_BETA_F32;
+        sg.count = per;
+        sg.out = out + g * per;
+        sg.d[0] = a;
+        sg.d[1] = b;
+        sg.d[2] = ratio;
+        eng[(size_t)g] = (int32_t)g;
+    }
+    return erd::run_segments(e, segs, eng, S(stream));
+}
+
+int erd_fill_beta_f32(erd_engines* e, float* out, int64_t per, float a, float b, void* stream) {
+    return beta_fill(e, out, per, a, b, 0.0f, stream);
+}
+
+int erd_fill_fisher_f_f32(erd_engines* e, float* out, int64_t per, float m, float n, void* stream) {
+    const float a = m * 0.5f, b = n * 0.5f;  // FisherFGen: beta{m * 0.5, n * 0.5}
+    if (!(a > 0) || !(b > 0)) return ERD_ERR_INVALID;
+    return beta_fill(e, out, per, a, b, b / a, stream);
+}
+
 int erd_fill_uniform_int_i32(erd_engines* e, int32_t* out, int64_t per, int32_t min, int32_t max, void* stream) {
     int rc = check_fill(e, out, per, 4);
     if (rc) return rc;

@@ -524,6 +524,24 @@ struct GammaTangent {  // NormalExp.h:580-603
     __device__ void store(Out* o, int64_t i, float v) const { o[i] = v; }
 };
 
+// BetaGen::packetOp with a < 1 and b < 1 (NormalExp.h:950-968): p1 =
+// exp(log U1 * (1/a)), p2 = exp(log U2 * (1/b)), candidate p1 / (p1 + p2),
+// accepted if p1 + p2 <= 1. ratio != 0: FisherFGen (NormalExp.h:1017-1022)
+// stores ratio * x / (1 - x) with ratio = b/a.
+struct BetaJohnk {
+    static constexpr int W = 8, SW = 4, K = 2;
+    using Out = float;
+    float inv_a, inv_b, ratio;
+    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc) const {
+        const float p1 = pexp(plog(ur_f32(h ? hi32(w[0]) : lo32(w[0]))) * inv_a);
+        const float p2 = pexp(plog(ur_f32(h ? hi32(w[1]) : lo32(w[1]))) * inv_b);
+        const float x = p1 + p2;
+        v = p1 / x;
+        acc = x <= 1.0f;
+    }
+    __device__ void store(Out* o, int64_t i, float v) const { o[i] = ratio != 0.0f ? (ratio * v) / (1.0f - v) : v; }
+};
+
 struct PoissonTangent {  // Discrete.h:875-902 (Packet4f via half())
     static constexpr int W = 4, SW = 2, K = 2;
     using Out = int32_t;

@@ -33,6 +33,7 @@ enum SegKind : int32_t {
     SEG_LOGNORMAL_F32 = 19,      // d[0]=mean, d[1]=sd
     SEG_WEIBULL_F32 = 20,        // d[0]=1/a (float), d[1]=b
     SEG_EXTREME_VALUE_F32 = 21,  // d[0]=a, d[1]=b
+    SEG_BETA_F32 = 22,           // d[0]=a, d[1]=b (float), d[2]=FisherF ratio b/a (0: beta)
     SEG_KIND_COUNT
 };
 

@@ -17,6 +17,7 @@ union ProgramLds {
     uint64_t carry_fixed[18 * kLanes];  // GammaIntF32<19>: 18 straddling draws
     RejectLds<GammaGS> gs;
     RejectLds<GammaTangent> gt;
+    RejectLds<BetaJohnk> gt_beta;
     RejectLds<PoissonTangent> pt;
     RejectLds<BinomialTangent> bt;
     struct {
@@ -35,11 +36,13 @@ __device__ void run_gamma_int(Eng& e, ProgramLds& s, float* out, int64_t n, floa
     op_fixed(e, s.carry_fixed, n, d, lane);
 }
 
+// gs: the generator's CompressMask queue carried across calls (BetaGen's two
+// GammaGen members), or NULL for a fresh generator.
 __device__ void run_gamma(Eng& e, ProgramLds& s, float* out, int64_t n, float alpha, float beta,
-                          uint32_t* status, int lane) {
+                          uint32_t* status, int lane, GenState* gs = nullptr) {
     if (alpha < 1.0f) {
         GammaGS pol{beta, (float)(kE / (alpha + kE)), 1 / alpha, alpha - 1};
-        op_reject(e, s.gs, out, n, pol, status, lane);
+        op_reject(e, s.gs, out, n, pol, status, lane, gs);
         return;
     }
     if (alpha == 1.0f) {
@@ -71,7 +74,42 @@ __device__ void run_gamma(Eng& e, ProgramLds& s, float* out, int64_t n, float al
         }
     }
     GammaTangent pol{beta, sqrtf(2 * alpha - 1), alpha - 1};
-    op_reject(e, s.gt, out, n, pol, status, lane);
+    op_reject(e, s.gt, out, n, pol, status, lane, gs);
+}
+
+// Per-segment state that must outlive the ops' LDS union: BetaGen's two
+// gamma queues and the packet pair being combined.
+struct BetaLds {
+    GenState q[2];
+    float x[2][8];
+};
+
+// BetaGen::packetOp (NormalExp.h:950-984) and FisherFGen (1017-1022):
+// a < 1 and b < 1 is the Johnk rejection through CompressMask; otherwise each
+// packet is gd1.packetOp (GammaGen(a)) then gd2.packetOp (GammaGen(b)), each
+// with its own queue, combined as p1 / (p1 + p2). Generators are fresh per
+// fill: the queues start empty and their leftovers are dropped.
+__device__ void run_beta(Eng& e, ProgramLds& s, BetaLds& bl, float* out, int64_t n, float a, float b, float ratio,
+                         uint32_t* status, int lane) {
+    if (a < 1.0f && b < 1.0f) {
+        BetaJohnk pol{1 / a, 1 / b, ratio};
+        op_reject(e, s.gt_beta, out, n, pol, status, lane);
+        return;
+    }
+    if (lane < 2) bl.q[lane].count = 0;
+    __syncthreads();
+    for (int64_t pk = 0; pk < n / 8; ++pk) {
+        run_gamma(e, s, bl.x[0], 8, a, 1.0f, status, lane, &bl.q[0]);
+        __syncthreads();
+        run_gamma(e, s, bl.x[1], 8, b, 1.0f, status, lane, &bl.q[1]);
+        __syncthreads();
+        if (lane < 8) {
+            const float p1 = bl.x[0][lane], p2 = bl.x[1][lane];
+            const float x = p1 / (p1 + p2);
+            out[8 * pk + lane] = ratio != 0.0f ? (ratio * x) / (1.0f - x) : x;
+        }
+        __syncthreads();
+    }
 }
 
 // discard(n) on the resident engine: only the block position and twists.
@@ -86,7 +124,7 @@ __device__ void run_discard(Eng& e, int64_t calls, int lane) {
     e.idx = (int)end;
 }
 
-__device__ void run_segment(Eng& e, ProgramLds& s, const Segment& sg, uint32_t* status, int lane) {
+__device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& sg, uint32_t* status, int lane) {
     const int64_t n = sg.count;
     switch (sg.kind) {
         case SEG_U64: {
@@ -209,6 +247,9 @@ __device__ void run_segment(Eng& e, ProgramLds& s, const Segment& sg, uint32_t* 
             op_fixed(e, s.carry_fixed, n, d, lane);
             break;
         }
+        case SEG_BETA_F32:
+            run_beta(e, s, bl, (float*)sg.out, n, (float)sg.d[0], (float)sg.d[1], (float)sg.d[2], status, lane);
+            break;
         case SEG_MULTINOMIAL_ROWS: {
             // MultinomialGen's large-trials path (Multinomial.h:74-91) for the
             // samples of one engine: row i is BinomialVGen<float>(r, p_i)
@@ -239,12 +280,13 @@ __global__ __launch_bounds__(64) void k_program(EngineState* __restrict__ states
                                                 uint32_t* __restrict__ status) {
     __shared__ uint64_t mt[kWords];
     __shared__ ProgramLds s;
+    __shared__ BetaLds bl;
     const int lane = threadIdx.x;
     const int b = blockIdx.x;
     EngineState* st = states + engine_of[b];
     Eng e;
     eng_load(e, mt, st, lane);
-    for (int k = begin[b]; k < begin[b + 1]; ++k) run_segment(e, s, segs[k], status, lane);
+    for (int k = begin[b]; k < begin[b + 1]; ++k) run_segment(e, s, bl, segs[k], status, lane);
     eng_store(e, st, lane);
 }
 

@@ -176,6 +176,15 @@ int erd_fill_weibull_vop_f32(erd_engines* e, float* out, int64_t per_engine, con
 int erd_fill_extreme_value_vop_f32(erd_engines* e, float* out, int64_t per_engine, const float* a,
                                    const float* b, void* stream);
 
+/* beta<float>(…, a, b): BetaGen (NormalExp.h:896-985). a < 1 and b < 1:
+ * Johnk's rejection through CompressMask (2 draws per candidate); otherwise
+ * per packet GammaGen(a) then GammaGen(b) (each with its own rejection queue),
+ * p1 / (p1 + p2). a, b > 0. */
+int erd_fill_beta_f32(erd_engines* e, float* out, int64_t per_engine, float a, float b, void* stream);
+/* fisherF<float>(…, m, n): FisherFGen (NormalExp.h:987-1023) = BetaGen(m/2,
+ * n/2) mapped by (b/a) x / (1 - x). m, n > 0. */
+int erd_fill_fisher_f_f32(erd_engines* e, float* out, int64_t per_engine, float m, float n, void* stream);
+
 /* ------------------------------------------------------ integer (int32) --- */
 /* bernoulli<int32>(…, p): BernoulliGen (Basic.h:670-708) at Packet4i, one
  * half() draw per 4 outputs of 1/0; 0 <= p <= 1. */

@@ -314,6 +314,18 @@ inline DeviceMatrix<float> balanced(int64_t rows, int64_t cols, Vmt19937_64& urn
         return erd_fill_balanced_f32(urng.handle(), o, n, a, b, stream);
     }, "erd_fill_balanced_f32");
 }
+inline DeviceMatrix<float> beta(int64_t rows, int64_t cols, Vmt19937_64& urng, float a = 1.f, float b = 1.f,
+                                void* stream = nullptr) {
+    return detail::fill<float>(rows, cols, urng.size(), [&](float* o, int64_t n) {
+        return erd_fill_beta_f32(urng.handle(), o, n, a, b, stream);
+    }, "erd_fill_beta_f32");
+}
+inline DeviceMatrix<float> fisherF(int64_t rows, int64_t cols, Vmt19937_64& urng, float m = 1.f, float n_ = 1.f,
+                                   void* stream = nullptr) {
+    return detail::fill<float>(rows, cols, urng.size(), [&](float* o, int64_t n) {
+        return erd_fill_fisher_f_f32(urng.handle(), o, n, m, n_, stream);
+    }, "erd_fill_fisher_f_f32");
+}
 inline DeviceMatrix<int32_t> geometric(int64_t rows, int64_t cols, Vmt19937_64& urng, double p = 0.5,
                                        void* stream = nullptr) {
     return detail::fill<int32_t>(rows, cols, urng.size(), [&](int32_t* o, int64_t n) {

@@ -892,6 +892,50 @@ void orc_fill_gamma_f32(void* states, int64_t n_engines, float* out, int64_t per
     });
 }
 
+// BetaGen (NormalExp.h:896-985) at Packet8f; ratio != 0: FisherFGen
+// (NormalExp.h:987-1023) with ratio = b/a, x -> ratio * x / (1 - x).
+// a < 1 and b < 1: Johnk through CompressMask<32>; otherwise GammaGen(a) and
+// GammaGen(b) (beta 1), one packet each in turn, p1 / (p1 + p2).
+void orc_fill_beta_f32(void* states, int64_t n_engines, float* out, int64_t per_engine, float a, float b,
+                       float ratio, int threads) {
+    Vmt* e = (Vmt*)states;
+    auto fin = [&](float x) { return ratio != 0.0f ? (ratio * x) / (1.0f - x) : x; };
+    for_engines(n_engines, threads, [&](int64_t g) {
+        float* o = out + g * per_engine;
+        if (a < 1 && b < 1) {
+            float rest[8];
+            int rest_cnt = 0;
+            for (int64_t i = 0; i < per_engine; i += 8) {
+                for (;;) {
+                    float u1[8], u2[8], cands[8];
+                    bool acc[8];
+                    ur8(e[g], u1);
+                    ur8(e[g], u2);
+                    for (int j = 0; j < 8; ++j) {
+                        const float p1 = pexp(plog(u1[j]) * (1 / a));
+                        const float p2 = pexp(plog(u2[j]) * (1 / b));
+                        const float x = p1 + p2;
+                        cands[j] = p1 / x;
+                        acc[j] = x <= 1.0f;
+                    }
+                    if (compress_append<8>(cands, acc, rest, rest_cnt)) {
+                        for (int j = 0; j < 8; ++j) o[i + j] = fin(cands[j]);
+                        break;
+                    }
+                }
+            }
+            return;
+        }
+        Gamma8 g1(a, 1.0f), g2(b, 1.0f);
+        for (int64_t i = 0; i < per_engine; i += 8) {
+            float p1[8], p2[8];
+            g1.op(e[g], p1);
+            g2.op(e[g], p2);
+            for (int j = 0; j < 8; ++j) o[i + j] = fin(p1[j] / (p1[j] + p2[j]));
+        }
+    });
+}
+
 void orc_fill_poisson_i32(void* states, int64_t n_engines, int32_t* out, int64_t per_engine,
                           double mean, int threads) {
     Vmt* e = (Vmt*)states;

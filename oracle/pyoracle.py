@@ -210,6 +210,16 @@ def fill_uniform_int_i32(e, per, lo, hi, threads=1):
     return out
 
 
+def fill_beta_f32(e, per, a=1.0, b=1.0, threads=1):
+    return _f32(e, per, "orc_fill_beta_f32", C.c_float(a), C.c_float(b), C.c_float(0.0), threads=threads)
+
+
+def fill_fisher_f_f32(e, per, m=1.0, n=1.0, threads=1):
+    a, b = np.float32(m) * np.float32(0.5), np.float32(n) * np.float32(0.5)
+    return _f32(e, per, "orc_fill_beta_f32", C.c_float(a), C.c_float(b), C.c_float(np.float32(b / a)),
+                threads=threads)
+
+
 def fill_poisson_i32(e, per, mean, threads=1):
     out = np.empty(len(e) * per, dtype=np.int32)
     lib().orc_fill_poisson_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), C.c_double(mean),

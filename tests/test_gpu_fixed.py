@@ -357,3 +357,21 @@ def test_uniform_int_rejects_bad_ranges(er):
         er.uniform_int(8, 1, urng, 5, 4)
     with pytest.raises(er.ErdError):
         er.uniform_int(8, 1, urng, -2 ** 31, 2 ** 31 - 1)
+
+
+@pytest.mark.parametrize("a,b,fisher", [(0.4, 0.7, False), (2.0, 3.0, False), (0.5, 4.5, False), (1.0, 1.0, False),
+                                        (3.3, 0.2, False), (7.0, 2.5, True), (0.6, 1.2, True)])
+def test_beta_fisher_f_match_oracle(er, orc, a, b, fisher):
+    """BetaGen in both regimes (Johnk; GammaGen pairs in GS, integer,
+    exponential and tangent regimes) and FisherFGen."""
+    seeds = [3, 1000, 2 ** 40 + 5]
+    urng = er.Vmt19937_64(seeds=seeds)
+    ref = orc.Engines(seeds=seeds)
+    for n in (8, 16, 2504, 8):
+        if fisher:
+            got, exp = er.fisher_f(n, 1, urng, a, b), orc.fill_fisher_f_f32(ref, n, a, b)
+        else:
+            got, exp = er.beta(n, 1, urng, a, b), orc.fill_beta_f32(ref, n, a, b)
+        assert_same(flat(got), exp)
+        np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+    assert urng.status() == 0

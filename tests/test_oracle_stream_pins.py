@@ -443,3 +443,53 @@ def test_discrete_f_of_the_stream(orc, K):
     big = orc.fill_discrete_f_i32(e.copy(), 1 << 18, w)
     freq = np.bincount(big, minlength=K) / big.size
     assert np.abs(freq - w / w.sum()).max() < 6 * np.sqrt(w.max() / w.sum() / big.size) + 1e-3
+
+
+@pytest.mark.parametrize("a,b", [(0.4, 0.7), (0.5, 0.5), (2.0, 3.0), (0.5, 4.5), (1.0, 1.0), (3.3, 0.2)])
+def test_beta_and_fisher_f_of_the_stream(orc, a, b):
+    """BetaGen: Johnk (a, b < 1) accepted candidates p1/(p1+p2) are the
+    stream's own draws (checked by replaying the queue), and the gamma-ratio
+    regime equals GammaGen(a) and GammaGen(b) packets drawn in turn; moments
+    match Beta(a, b); FisherF is its (b/a) x/(1-x) map."""
+    e = orc.Engines(n=1, seed0=41)
+    per = 1024
+    got = orc.fill_beta_f32(e.copy(), per, a, b)
+    if a < 1 and b < 1:
+        w = full_words(orc, e, 8 * per // 8)[0]  # plenty of draws
+        u = ur(w)
+        outs, rest, k = [], [], 0
+        while len(outs) < per:
+            u1, u2 = u[k], u[k + 1]
+            k += 2
+            p1 = orc.pexp(orc.plog(u1) * f32(1 / f32(a)))
+            p2 = orc.pexp(orc.plog(u2) * f32(1 / f32(b)))
+            x = (p1 + p2).astype(f32)
+            c = (p1 / x).astype(f32)
+            acc = x <= 1
+            if acc.all():
+                outs += list(c)
+                continue
+            rest += list(c[acc])
+            if len(rest) >= 8:
+                outs += rest[:8]
+                rest = rest[8:]
+        np.testing.assert_array_equal(got, np.array(outs[:per], dtype=f32))
+    else:
+        # one GammaGen(a) packet, then one GammaGen(b) packet, per output packet
+        ref, h = [], e.copy()
+        g1 = orc.Generators(9, a, 1.0, 1)
+        g2 = orc.Generators(9, b, 1.0, 1)
+        for _ in range(per // 8):
+            p1 = g1.fill(h, 8)
+            p2 = g2.fill(h, 8)
+            ref += list((p1 / (p1 + p2)).astype(f32))
+        np.testing.assert_array_equal(got, np.array(ref, dtype=f32))
+    big = orc.fill_beta_f32(e.copy(), 1 << 16, a, b).astype(np.float64)
+    mean = a / (a + b)
+    var = a * b / ((a + b) ** 2 * (a + b + 1))
+    assert abs(big.mean() - mean) < 0.01 and abs(big.var() - var) < 0.01
+    fz = orc.fill_fisher_f_f32(e.copy(), per, 2 * a, 2 * b)
+    bz = orc.fill_beta_f32(e.copy(), per, a, b)
+    r = f32(b) / f32(a)
+    with np.errstate(divide="ignore"):  # x = 1 maps to +inf on both sides
+        np.testing.assert_array_equal(fz, (r * bz) / (f32(1) - bz))
