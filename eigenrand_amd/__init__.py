@@ -29,7 +29,7 @@ __all__ = [
     "randbits_u64", "uniform_real", "uniform_real_f64", "normal", "normal_vop",
     "exponential", "cauchy", "student_t", "gamma", "chi_squared", "poisson", "binomial",
     "lognormal", "weibull", "extreme_value",
-    "bernoulli", "balanced", "geometric", "randbits_i32", "uniform_int", "beta", "fisher_f", "uniform_real_vop", "balanced_vop", "bernoulli_vop",
+    "bernoulli", "balanced", "geometric", "randbits_i32", "uniform_int", "beta", "fisher_f", "negative_binomial", "uniform_real_vop", "balanced_vop", "bernoulli_vop",
     "exponential_vop", "cauchy_vop", "lognormal_vop", "student_t_vop", "weibull_vop", "extreme_value_vop",
     "binomial_vop", "gamma_vop", "poisson_vop", "run_requests", "cholesky_batch",
     "mvnormal_batch", "wishart_batch", "invwishart_batch", "dirichlet_batch", "MvNormalGen", "WishartGen",
@@ -364,6 +364,12 @@ def beta(rows, cols, urng, a=1.0, b=1.0, out=None, stream=None):
 def fisher_f(rows, cols, urng, m=1.0, n=1.0, out=None, stream=None):
     """Rand::fisherF<MatrixXf>(rows, cols, urng, m, n) (FisherFGen)."""
     return _fill("erd_fill_fisher_f_f32", urng, rows, cols, _torch().float32, out, stream, float(m), float(n))
+
+
+def negative_binomial(rows, cols, urng, trials=1, p=0.5, out=None, stream=None):
+    """Rand::negativeBinomial<MatrixXi>(rows, cols, urng, trials, p) (int32)."""
+    return _fill("erd_fill_negative_binomial_i32", urng, rows, cols, _torch().int32, out, stream, int(trials),
+                 float(p))
 
 
 def geometric(rows, cols, urng, p=0.5, out=None, stream=None):

@@ -91,6 +91,7 @@ _SIGS = {
     "erd_fill_uniform_int_i32": [P, P, I64, C.c_int32, C.c_int32, P],
     "erd_fill_discrete_f_i32": [P, P, I64, P, I64, P],
     "erd_fill_beta_f32": [P, P, I64, F32, F32, P],
+    "erd_fill_negative_binomial_i32": [P, P, I64, C.c_int32, F64, P],
     "erd_fill_fisher_f_f32": [P, P, I64, F32, F32, P],
     "erd_generator_create": [C.POINTER(P), C.c_int32, F64, F64, I64, P],
     "erd_generator_destroy": [P],

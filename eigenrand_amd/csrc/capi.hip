@@ -596,6 +596,27 @@ static int beta_fill(erd_engines* e, float* out, int64_t per, float a, float b, 
     return erd::run_segments(e, segs, eng, S(stream));
 }
 
+int erd_fill_negative_binomial_i32(erd_engines* e, int32_t* out, int64_t per, int32_t trials, double p,
+                                   void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    if (trials <= 0 || !(p > 0 && p <= 1)) return ERD_ERR_INVALID;
+    if (per == 0) return ERD_OK;
+    std::vector<erd::Segment> segs((size_t)e->count);
+    std::vector<int32_t> eng((size_t)e->count);
+    for (int64_t g = 0; g < e->count; ++g) {
+        erd::Segment& sg = segs[(size_t)g];
+        sg = erd::Segment{};
+        sg.kind = erd::SEG_NEGATIVE_BINOMIAL_I32;
+        sg.count = per;
+        sg.out = out + g * per;
+        sg.d[0] = (float)trials;             // NegativeBinomialGen: gamma{(float)trials,
+        sg.d[1] = (float)((1 - p) / p);      //   (float)((1 - p) / p)}
+        eng[(size_t)g] = (int32_t)g;
+    }
+    return erd::run_segments(e, segs, eng, S(stream));
+}
+
 int erd_fill_beta_f32(erd_engines* e, float* out, int64_t per, float a, float b, void* stream) {
     return beta_fill(e, out, per, a, b, 0.0f, stream);
 }

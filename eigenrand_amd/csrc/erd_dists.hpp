@@ -484,11 +484,14 @@ struct GammaIntF32 {
 // K: slots per candidate; eval(w, h, v, acc): lane 2p+h of a candidate from
 // word p of each of its K slots; store(out, i, v): write output element i.
 
-struct GammaGS {  // NormalExp.h:536-558
-    static constexpr int W = 8, SW = 4, K = 3;
+// W = 8: Packet8f on full draws; W = 4: Packet4f on half draws (the
+// GammaGen<float> inside NegativeBinomialGen).
+template <int W_>
+struct GammaGSW {  // NormalExp.h:536-558
+    static constexpr int W = W_, SW = W_ / 2, K = 3;
     using Out = float;
     float beta, px, inv_alpha, am1;
-    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc) const {
+    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc, int) const {
         const float ux = ur_f32(h ? hi32(w[0]) : lo32(w[0]));
         const float vx = nzur_f32(h ? hi32(w[1]) : lo32(w[1]));
         const float lv = plog_pos(vx);
@@ -506,11 +509,12 @@ struct GammaGS {  // NormalExp.h:536-558
     __device__ void store(Out* o, int64_t i, float v) const { o[i] = v; }
 };
 
-struct GammaTangent {  // NormalExp.h:580-603
-    static constexpr int W = 8, SW = 4, K = 2;
+template <int W_>
+struct GammaTangentW {  // NormalExp.h:580-603
+    static constexpr int W = W_, SW = W_ / 2, K = 2;
     using Out = float;
     float beta, sq, am1;
-    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc) const {
+    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc, int) const {
         float ys, yc;
         psincos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), ys, yc);
         const float yx = ys / yc;
@@ -524,6 +528,9 @@ struct GammaTangent {  // NormalExp.h:580-603
     __device__ void store(Out* o, int64_t i, float v) const { o[i] = v; }
 };
 
+using GammaGS = GammaGSW<8>;
+using GammaTangent = GammaTangentW<8>;
+
 // BetaGen::packetOp with a < 1 and b < 1 (NormalExp.h:950-968): p1 =
 // exp(log U1 * (1/a)), p2 = exp(log U2 * (1/b)), candidate p1 / (p1 + p2),
 // accepted if p1 + p2 <= 1. ratio != 0: FisherFGen (NormalExp.h:1017-1022)
@@ -532,7 +539,7 @@ struct BetaJohnk {
     static constexpr int W = 8, SW = 4, K = 2;
     using Out = float;
     float inv_a, inv_b, ratio;
-    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc) const {
+    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc, int) const {
         const float p1 = pexp(plog(ur_f32(h ? hi32(w[0]) : lo32(w[0]))) * inv_a);
         const float p2 = pexp(plog(ur_f32(h ? hi32(w[1]) : lo32(w[1]))) * inv_b);
         const float x = p1 + p2;
@@ -546,7 +553,7 @@ struct PoissonTangent {  // Discrete.h:875-902 (Packet4f via half())
     static constexpr int W = 4, SW = 2, K = 2;
     using Out = int32_t;
     float psq, pmean, plm, pg1;
-    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc) const {
+    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc, int) const {
         float s, c;
         psincos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), s, c);
         const float yx = s / c;
@@ -561,13 +568,38 @@ struct PoissonTangent {  // Discrete.h:875-902 (Packet4f via half())
     __device__ void store(Out* o, int64_t i, float v) const { o[i] = (int32_t)v; }
 };
 
+// NegativeBinomialGen's tangent stage (GammaPoisson.h:69-96): the Poisson
+// tangent candidate with per-lane constants (sq = sqrt(2 mean), lm = log
+// mean, g1 = mean lm - lgamma_approx(mean + 1)), stored with cvttps.
+struct NegBinTangent {
+    static constexpr int W = 4, SW = 2, K = 2;
+    using Out = int32_t;
+    const float* mean;  // 4 lanes each
+    const float* sq;
+    const float* lm;
+    const float* g1;
+    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc, int ln) const {
+        float s, c;
+        psincos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), s, c);
+        const float yx = s / c;
+        const float fres = truncf(sq[ln] * yx + mean[ln]);
+        const float p1 = (yx * yx + 1.0f) * ERD_F(0.9);
+        const float p2 = pexp((fres * lm[ln] - plgamma_approx(fres + 1.0f)) - g1[ln]);
+        const bool c1 = 0.0f <= fres;
+        const float u2 = ur_f32(h ? hi32(w[1]) : lo32(w[1]));
+        acc = c1 && (u2 <= p1 * p2);
+        v = fres;
+    }
+    __device__ void store(Out* o, int64_t i, float v) const { o[i] = cvtt_x86(v); }
+};
+
 struct BinomialTangent {  // Discrete.h:1011-1051
     static constexpr int W = 4, SW = 2, K = 2;
     using Out = int32_t;
     float ptrials, psqrt_v, pmean, plp, plq, pg1;
     int32_t trials;
     int flip;  // p != small_p: result is trials - k
-    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc) const {
+    __device__ void eval(const uint64_t* w, int h, float& v, bool& acc, int) const {
         float s, c;
         psincos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), s, c);
         const float ys = s / c;

@@ -209,7 +209,7 @@ __device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t p
 #pragma unroll
                 for (int k = 0; k < K; ++k) w[k] = slot_word<SW>(e.mt, carry, nc, pos, c * K + k, p);
 #pragma unroll
-                for (int h = 0; h < 2; ++h) pol.eval(w, h, v[h], a[h]);
+                for (int h = 0; h < 2; ++h) pol.eval(w, h, v[h], a[h], 2 * p + h);
             }
             // Queue bookkeeping over the candidates that count (`lv`).
             const uint64_t cmask = (IPC == 4 ? 0xFull : 0x3ull) << (cl * IPC);

@@ -34,6 +34,7 @@ enum SegKind : int32_t {
     SEG_WEIBULL_F32 = 20,        // d[0]=1/a (float), d[1]=b
     SEG_EXTREME_VALUE_F32 = 21,  // d[0]=a, d[1]=b
     SEG_BETA_F32 = 22,           // d[0]=a, d[1]=b (float), d[2]=FisherF ratio b/a (0: beta)
+    SEG_NEGATIVE_BINOMIAL_I32 = 23,  // d[0]=trials (float), d[1]=(1-p)/p (float)
     SEG_KIND_COUNT
 };
 

@@ -18,6 +18,9 @@ union ProgramLds {
     RejectLds<GammaGS> gs;
     RejectLds<GammaTangent> gt;
     RejectLds<BetaJohnk> gt_beta;
+    RejectLds<GammaGSW<4>> gs4;
+    RejectLds<GammaTangentW<4>> gt4;
+    RejectLds<NegBinTangent> nbt;
     RejectLds<PoissonTangent> pt;
     RejectLds<BinomialTangent> bt;
     struct {
@@ -78,10 +81,12 @@ __device__ void run_gamma(Eng& e, ProgramLds& s, float* out, int64_t n, float al
 }
 
 // Per-segment state that must outlive the ops' LDS union: BetaGen's two
-// gamma queues and the packet pair being combined.
+// gamma queues and the packet pair being combined; NegativeBinomialGen's
+// queue, its packet of means and per-lane tangent constants.
 struct BetaLds {
     GenState q[2];
     float x[2][8];
+    float c[3][4];
 };
 
 // BetaGen::packetOp (NormalExp.h:950-984) and FisherFGen (1017-1022):
@@ -122,6 +127,102 @@ __device__ void run_discard(Eng& e, int64_t calls, int lane) {
         end -= twists * kN;
     }
     e.idx = (int)end;
+}
+
+// GammaGen<float> at Packet4f with integer alpha < 20 or alpha == 1
+// (NormalExp.h:561-578): alpha (or one) half draws per packet.
+__device__ void gamma_fixed_w4(Eng& e, uint64_t* carry, float* dst4, float alpha, float beta, int lane) {
+    HalfCursor cur = cursor_begin(e, carry, lane);
+    const int j = lane & 3;
+    float x;
+    if (alpha == 1.0f) {
+        cur.ensure(1, lane);
+        const float u = ur_f32(cur.word32(0, j));
+        cur.advance(1);
+        x = beta * -(plog(1.0f - u) / 1.0f);
+    } else {
+        float yx = 1.0f;
+        for (int c = 0; c < (int)alpha; ++c) {
+            cur.ensure(1, lane);
+            const float u = nzur_f32(cur.word32(0, j));
+            cur.advance(1);
+            yx = c == 0 ? u : yx * u;
+        }
+        x = -(beta * plog(yx));
+    }
+    if (lane < 4) dst4[lane] = x;
+    cursor_end(e, cur);
+}
+
+// Knuth's product loop with a threshold per lane (NegativeBinomialGen,
+// GammaPoisson.h:57-67); returns lane j's count.
+__device__ int32_t knuth_packet_lanes(Eng& e, uint64_t* carry, const float* thr4, uint32_t* status, int lane) {
+    HalfCursor cur = cursor_begin(e, carry, lane);
+    const int j = lane & 3;
+    const float thr = thr4[j];
+    float val = 1.0f;
+    int32_t res = 0;
+    for (int it = 0;; ++it) {
+        cur.ensure(1, lane);
+        val = val * ur_f32(cur.word32(0, j));
+        cur.advance(1);
+        const bool c = thr < val;
+        if (!(__ballot(c) & 0xFull)) break;
+        res += c ? 1 : 0;
+        if (it > kLockstepCap) {
+            if (lane == 0) atomicOr(status, kStatusIterCap);
+            break;
+        }
+    }
+    cursor_end(e, cur);
+    return res;
+}
+
+// NegativeBinomialGen<int32_t>::packetOp (GammaPoisson.h:52-97) at Packet4i:
+// a packet of means from GammaGen<float>(trials, (1 - p) / p) at Packet4f;
+// if no lane's mean exceeds 12, the per-lane Knuth loop; otherwise tangent
+// rejection whose candidates share the gamma's CompressMask queue. A fresh
+// generator per fill.
+__device__ void run_negbin(Eng& e, ProgramLds& s, BetaLds& bl, int32_t* out, int64_t n, float alpha, float beta,
+                           uint32_t* status, int lane) {
+    GenState* q = &bl.q[0];
+    if (lane == 0) q->count = 0;
+    __syncthreads();
+    const int count = (int)alpha;
+    for (int64_t pk = 0; pk < n / 4; ++pk) {
+        float* mean = bl.x[0];
+        if (alpha < 1.0f) {
+            GammaGSW<4> pol{beta, (float)(kE / (alpha + kE)), 1 / alpha, alpha - 1};
+            op_reject(e, s.gs4, mean, 4, pol, status, lane, q);
+        } else if (alpha == 1.0f || ((float)count == alpha && count < 20)) {
+            gamma_fixed_w4(e, s.vop.carry, mean, alpha, beta, lane);
+        } else {
+            GammaTangentW<4> pol{beta, sqrtf(2 * alpha - 1), alpha - 1};
+            op_reject(e, s.gt4, mean, 4, pol, status, lane, q);
+        }
+        __syncthreads();
+        bool big = false;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) big |= 12.0f < mean[j];
+        if (!big) {
+            if (lane < 4) bl.x[1][lane] = pexp(-mean[lane]);
+            __syncthreads();
+            const int32_t r = knuth_packet_lanes(e, s.vop.carry, bl.x[1], status, lane);
+            if (lane < 4) out[4 * pk + lane] = r;
+        } else {
+            if (lane < 4) {
+                const float m = mean[lane];
+                const float lm = plog(m);
+                bl.c[0][lane] = sqrtf(2.0f * m);
+                bl.c[1][lane] = lm;
+                bl.c[2][lane] = m * lm - plgamma_approx(m + 1.0f);
+            }
+            __syncthreads();
+            NegBinTangent pol{mean, bl.c[0], bl.c[1], bl.c[2]};
+            op_reject(e, s.nbt, out + 4 * pk, 4, pol, status, lane, q);
+        }
+        __syncthreads();
+    }
 }
 
 __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& sg, uint32_t* status, int lane) {
@@ -247,6 +348,9 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
             op_fixed(e, s.carry_fixed, n, d, lane);
             break;
         }
+        case SEG_NEGATIVE_BINOMIAL_I32:
+            run_negbin(e, s, bl, (int32_t*)sg.out, n, (float)sg.d[0], (float)sg.d[1], status, lane);
+            break;
         case SEG_BETA_F32:
             run_beta(e, s, bl, (float*)sg.out, n, (float)sg.d[0], (float)sg.d[1], (float)sg.d[2], status, lane);
             break;

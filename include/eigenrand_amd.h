@@ -193,6 +193,13 @@ int erd_fill_bernoulli_i32(erd_engines* e, int32_t* out, int64_t per_engine, dou
  * U) * (1 / log(1 - p))) from a half() draw per 4 outputs, converted as
  * cvttps2dq does (out of range: INT32_MIN); 0 <= p <= 1. */
 int erd_fill_geometric_i32(erd_engines* e, int32_t* out, int64_t per_engine, double p, void* stream);
+/* negativeBinomial<int32>(…, trials, p): NegativeBinomialGen
+ * (GammaPoisson.h:29-100) at Packet4i: per packet a GammaGen<float>(trials,
+ * (1-p)/p) packet on half draws, then Poisson counts with those means (Knuth
+ * if no mean exceeds 12, else tangent rejection sharing the gamma's
+ * CompressMask queue, as the reference does). trials > 0, 0 < p <= 1. */
+int erd_fill_negative_binomial_i32(erd_engines* e, int32_t* out, int64_t per_engine, int32_t trials, double p,
+                                   void* stream);
 /* uniformInt<int32>(…, min, max): UniformIntGen (Discrete.h:215-335) at
  * Packet4i: candidates of `bitsize` bits from half() draws through
  * CompressMask<16>, the rest of a draw dropped once a packet is emitted.

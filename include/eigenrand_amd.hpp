@@ -326,6 +326,12 @@ inline DeviceMatrix<float> fisherF(int64_t rows, int64_t cols, Vmt19937_64& urng
         return erd_fill_fisher_f_f32(urng.handle(), o, n, m, n_, stream);
     }, "erd_fill_fisher_f_f32");
 }
+inline DeviceMatrix<int32_t> negativeBinomial(int64_t rows, int64_t cols, Vmt19937_64& urng, int32_t trials = 1,
+                                              double p = 0.5, void* stream = nullptr) {
+    return detail::fill<int32_t>(rows, cols, urng.size(), [&](int32_t* o, int64_t n) {
+        return erd_fill_negative_binomial_i32(urng.handle(), o, n, trials, p, stream);
+    }, "erd_fill_negative_binomial_i32");
+}
 inline DeviceMatrix<int32_t> geometric(int64_t rows, int64_t cols, Vmt19937_64& urng, double p = 0.5,
                                        void* stream = nullptr) {
     return detail::fill<int32_t>(rows, cols, urng.size(), [&](int32_t* o, int64_t n) {

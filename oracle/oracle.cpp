@@ -175,23 +175,40 @@ struct StdNormal8 {
 };
 
 // GammaGen::packetOp (NormalExp.h:527-605); ctor constants 453-458.
-struct Gamma8 {
+template <int W>
+inline void urW(Vmt& e, float* u);
+
+// GammaGen<float>::packetOp at W lanes: Packet8f (W = 8, full draws) or
+// Packet4f (W = 4, half draws through half(), as NegativeBinomialGen uses it);
+// the CompressMask queue is `rest`.
+template <int W>
+inline void nzurW(Vmt& e, float* u) {
+    if (W == 4) {
+        ur4(e, u);
+        for (int j = 0; j < 4; ++j) u[j] = u[j] + 1.1920928955078125e-07f / 8;
+    } else {
+        nzur8(e, u);
+    }
+}
+
+template <int W>
+struct GammaPk {
     float alpha, beta, px, sq;
-    float rest[8];
+    float rest[W];
     int rest_cnt = 0;
-    Gamma8(float a, float b) : alpha(a), beta(b) {
+    GammaPk(float a, float b) : alpha(a), beta(b) {
         px = (float)(kE / (alpha + kE));
         sq = std::sqrt(2 * alpha - 1);
     }
-    void op(Vmt& e, float out[8]) {
+    void op(Vmt& e, float out[W]) {
         if (alpha < 1) {
             for (;;) {
-                float ux[8], vx[8], u3[8], cands[8];
-                bool acc[8];
-                ur8(e, ux);
-                nzur8(e, vx);
-                float xx[8], qx[8];
-                for (int j = 0; j < 8; ++j) {
+                float ux[W], vx[W], u3[W], cands[W];
+                bool acc[W];
+                urW<W>(e, ux);
+                nzurW<W>(e, vx);
+                float xx[W], qx[W];
+                for (int j = 0; j < W; ++j) {
                     float lv = plog(vx[j]);
                     float x1 = pexp((1 / alpha) * lv);
                     float q1 = pexp(-x1);
@@ -201,53 +218,54 @@ struct Gamma8 {
                     xx[j] = c ? x1 : x2;
                     qx[j] = c ? q1 : q2;
                 }
-                ur8(e, u3);
-                for (int j = 0; j < 8; ++j) { cands[j] = beta * xx[j]; acc[j] = u3[j] < qx[j]; }
-                if (compress_append<8>(cands, acc, rest, rest_cnt)) { std::memcpy(out, cands, 32); return; }
+                urW<W>(e, u3);
+                for (int j = 0; j < W; ++j) { cands[j] = beta * xx[j]; acc[j] = u3[j] < qx[j]; }
+                if (compress_append<W>(cands, acc, rest, rest_cnt)) { std::memcpy(out, cands, 4 * W); return; }
             }
         }
         if (alpha == 1) {
-            float u[8];
-            ur8(e, u);
-            for (int j = 0; j < 8; ++j) out[j] = beta * -(plog(1.0f - u[j]) / 1.0f);
+            float u[W];
+            urW<W>(e, u);
+            for (int j = 0; j < W; ++j) out[j] = beta * -(plog(1.0f - u[j]) / 1.0f);
             return;
         }
         int count;
         if ((count = (int)alpha) == alpha && count < 20) {
-            float yx[8], u[8];
-            nzur8(e, yx);
+            float yx[W], u[W];
+            nzurW<W>(e, yx);
             while (--count) {
-                nzur8(e, u);
-                for (int j = 0; j < 8; ++j) yx[j] = yx[j] * u[j];
+                nzurW<W>(e, u);
+                for (int j = 0; j < W; ++j) yx[j] = yx[j] * u[j];
             }
-            for (int j = 0; j < 8; ++j) out[j] = -(beta * plog(yx[j]));
+            for (int j = 0; j < W; ++j) out[j] = -(beta * plog(yx[j]));
             return;
         }
         for (;;) {
             const float alpha_1 = alpha - 1;
-            float u1[8], u2[8], cands[8];
-            bool acc[8];
-            ur8(e, u1);
-            float yx[8], xx[8];
-            bool c[8];
-            for (int j = 0; j < 8; ++j) {
+            float u1[W], u2[W], cands[W];
+            bool acc[W];
+            urW<W>(e, u1);
+            float yx[W], xx[W];
+            bool c[W];
+            for (int j = 0; j < W; ++j) {
                 float ys, yc;
                 psincos(kPiF * u1[j], ys, yc);
                 yx[j] = ys / yc;
                 xx[j] = sq * yx[j] + alpha_1;
                 c[j] = 0.0f < xx[j];
             }
-            ur8(e, u2);
-            for (int j = 0; j < 8; ++j) {
+            urW<W>(e, u2);
+            for (int j = 0; j < W; ++j) {
                 float ub = (yx[j] * yx[j] + 1.0f) *
                            pexp(alpha_1 * plog(xx[j] / alpha_1) - yx[j] * sq);
                 acc[j] = c[j] && (u2[j] <= ub);
                 cands[j] = beta * xx[j];
             }
-            if (compress_append<8>(cands, acc, rest, rest_cnt)) { std::memcpy(out, cands, 32); return; }
+            if (compress_append<W>(cands, acc, rest, rest_cnt)) { std::memcpy(out, cands, 4 * W); return; }
         }
     }
 };
+using Gamma8 = GammaPk<8>;
 
 // PoissonGen (EigenRand/Dists/Discrete.h:788-904) at Packet4i/Packet4f.
 struct Poisson4 {
@@ -932,6 +950,74 @@ void orc_fill_beta_f32(void* states, int64_t n_engines, float* out, int64_t per_
             g1.op(e[g], p1);
             g2.op(e[g], p2);
             for (int j = 0; j < 8; ++j) o[i + j] = fin(p1[j] / (p1[j] + p2[j]));
+        }
+    });
+}
+
+// NegativeBinomialGen<int32_t>::packetOp (GammaPoisson.h:29-100) at
+// Packet4i: mean = GammaGen<float>(trials, (1 - p) / p) at Packet4f (half
+// draws, queue Q); if no lane's mean exceeds 12, Knuth's product loop with
+// per-lane thresholds e^-mean; otherwise tangent rejection with per-lane
+// constants whose candidates go through the gamma's own queue Q, emitted
+// values cast with cvttps. A fresh generator per engine fill.
+void orc_fill_negative_binomial_i32(void* states, int64_t n_engines, int32_t* out, int64_t per_engine,
+                                    int32_t trials, double p, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        int32_t* o = out + g * per_engine;
+        GammaPk<4> gam((float)trials, (float)((1 - p) / p));
+        for (int64_t i = 0; i < per_engine; i += 4) {
+            float mean[4];
+            gam.op(e[g], mean);
+            bool any_big = false;
+            for (int j = 0; j < 4; ++j) any_big |= 12.0f < mean[j];
+            if (!any_big) {
+                float val[4] = {1.0f, 1.0f, 1.0f, 1.0f}, ne[4];
+                int32_t res[4] = {0, 0, 0, 0};
+                for (int j = 0; j < 4; ++j) ne[j] = pexp(-mean[j]);
+                for (;;) {
+                    float u[4];
+                    ur4(e[g], u);
+                    bool anyc = false;
+                    for (int j = 0; j < 4; ++j) {
+                        val[j] = val[j] * u[j];
+                        const bool c = ne[j] < val[j];
+                        anyc |= c;
+                        res[j] += c ? 1 : 0;
+                    }
+                    if (!anyc) break;
+                }
+                for (int j = 0; j < 4; ++j) o[i + j] = res[j];
+                continue;
+            }
+            float sq[4], lm[4], g1[4];
+            for (int j = 0; j < 4; ++j) {
+                sq[j] = std::sqrt(2.0f * mean[j]);
+                lm[j] = plog(mean[j]);
+                g1[j] = mean[j] * lm[j] - plgamma_approx(mean[j] + 1.0f);
+            }
+            for (;;) {
+                float u[4], u2[4], cands[4];
+                bool acc[4];
+                ur4(e[g], u);
+                float yx[4];
+                for (int j = 0; j < 4; ++j) {
+                    float sn, cs;
+                    psincos(kPiF * u[j], sn, cs);
+                    yx[j] = sn / cs;
+                    cands[j] = ptruncate(sq[j] * yx[j] + mean[j]);
+                }
+                ur4(e[g], u2);
+                for (int j = 0; j < 4; ++j) {
+                    const float p1 = (yx[j] * yx[j] + 1.0f) * 0.9f;
+                    const float p2 = pexp((cands[j] * lm[j] - plgamma_approx(cands[j] + 1.0f)) - g1[j]);
+                    acc[j] = (0.0f <= cands[j]) && (u2[j] <= p1 * p2);
+                }
+                if (compress_append<4>(cands, acc, gam.rest, gam.rest_cnt)) {
+                    for (int j = 0; j < 4; ++j) o[i + j] = cvttps(cands[j]);
+                    break;
+                }
+            }
         }
     });
 }

@@ -220,6 +220,13 @@ def fill_fisher_f_f32(e, per, m=1.0, n=1.0, threads=1):
                 threads=threads)
 
 
+def fill_negative_binomial_i32(e, per, trials, p, threads=1):
+    out = np.empty(len(e) * per, dtype=np.int32)
+    lib().orc_fill_negative_binomial_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), C.c_int32(trials),
+                                         C.c_double(p), C.c_int(threads))
+    return out
+
+
 def fill_poisson_i32(e, per, mean, threads=1):
     out = np.empty(len(e) * per, dtype=np.int32)
     lib().orc_fill_poisson_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), C.c_double(mean),

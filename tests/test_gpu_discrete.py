@@ -107,3 +107,18 @@ def test_poisson_knuth_prefix_many_windows(er, orc, mean):
         out = er.poisson(n, 1, urng, mean)
         assert_same(flat(out), orc.fill_poisson_i32(ref, n, mean, threads=8))
         np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+
+
+@pytest.mark.parametrize("trials,p", [(1, 0.5), (3, 0.7), (10, 0.5), (25, 0.3), (50, 0.05), (19, 0.9)])
+def test_negative_binomial_matches_oracle(er, orc, trials, p):
+    """NegativeBinomialGen: gamma means on half draws (exponential, integer
+    and tangent regimes), then Knuth or tangent Poisson sharing the gamma's
+    queue; odd packet counts leave cached halves between fills."""
+    urng = er.Vmt19937_64(seeds=SEEDS)
+    ref = orc.Engines(seeds=SEEDS)
+    for n in (4, 12, 1004, 4):
+        assert_same(flat(er.negative_binomial(n, 1, urng, trials, p)),
+                    orc.fill_negative_binomial_i32(ref, n, trials, p))
+        np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+        assert_same(flat(er.uniform_real(8, 1, urng)), orc.fill_uniform_f32(ref, 8))
+    assert urng.status() == 0

@@ -493,3 +493,16 @@ def test_beta_and_fisher_f_of_the_stream(orc, a, b):
     r = f32(b) / f32(a)
     with np.errstate(divide="ignore"):  # x = 1 maps to +inf on both sides
         np.testing.assert_array_equal(fz, (r * bz) / (f32(1) - bz))
+
+
+@pytest.mark.parametrize("trials,p", [(1, 0.5), (3, 0.7), (10, 0.5), (25, 0.3), (50, 0.05), (19, 0.9)])
+def test_negative_binomial_moments(orc, trials, p):
+    """NegativeBinomialGen = Poisson(Gamma(trials, (1-p)/p)): mean
+    trials (1-p)/p and variance trials (1-p)/p^2 within sampling error."""
+    e = orc.Engines(n=4, seed0=3)
+    x = orc.fill_negative_binomial_i32(e, 1 << 15, trials, p).astype(np.float64)
+    m = trials * (1 - p) / p
+    v = trials * (1 - p) / p ** 2
+    assert x.min() >= 0
+    assert abs(x.mean() - m) < 5 * np.sqrt(v / x.size) + 1e-3
+    assert abs(x.var() / v - 1) < 0.05
