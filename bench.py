@@ -110,6 +110,21 @@ def sweep():
         ("poisson(10) i32 [Knuth, prefix]", lambda: er.poisson(PER_ENGINE, 1, urng, 10.0, out=outi)),
         ("binomial(10,0.3) i32 [n<25]", lambda: er.binomial(PER_ENGINE, 1, urng, 10, 0.3, out=outi)),
         ("binomial(1000,0.0005) i32 [Knuth]", lambda: er.binomial(PER_ENGINE, 1, urng, 1000, 0.0005, out=outi)),
+        # the other generators of the same machinery
+        ("bernoulli(0.3) f32", lambda: er.bernoulli(PER_ENGINE, 1, urng, 0.3, out=out)),
+        ("balanced(-1,1) f32", lambda: er.balanced(PER_ENGINE, 1, urng, out=out)),
+        ("uniformInt(0,9) i32", lambda: er.uniform_int(PER_ENGINE, 1, urng, 0, 9, out=outi)),
+        ("uniformInt(0,1023) i32", lambda: er.uniform_int(PER_ENGINE, 1, urng, 0, 1023, out=outi)),
+        ("geometric(0.3) i32", lambda: er.geometric(PER_ENGINE, 1, urng, 0.3, out=outi)),
+        ("discreteF(K=100) i32", lambda: er.discrete(PER_ENGINE, 1, urng, np.arange(1, 101.0), out=outi,
+                                                    precision="float")),
+        ("discrete(K=100) i32", lambda: er.discrete(PER_ENGINE, 1, urng, np.arange(1, 101.0), out=outi)),
+        ("lognormal(0,1) f32", lambda: er.lognormal(PER_ENGINE, 1, urng, 0.0, 1.0, out=out)),
+        ("weibull(1.7,2.5) f32", lambda: er.weibull(PER_ENGINE, 1, urng, 1.7, 2.5, out=out)),
+        ("beta(0.5,0.7) f32 [Johnk]", lambda: er.beta(PER_ENGINE, 1, urng, 0.5, 0.7, out=out)),
+        ("beta(2,3) f32 [gamma pair]", lambda: er.beta(PER_ENGINE, 1, urng, 2.0, 3.0, out=out)),
+        ("fisherF(5,8) f32", lambda: er.fisher_f(PER_ENGINE, 1, urng, 5.0, 8.0, out=out)),
+        ("negativeBinomial(10,0.5) i32", lambda: er.negative_binomial(PER_ENGINE, 1, urng, 10, 0.5, out=outi)),
     ]
     for name, fn in univ:
         emit(name, N_SAMPLES, timed(fn, reps=3), 4)
@@ -330,15 +345,17 @@ def mv_sweep(emit_line):
     del gen, out, covs
     wd = rng.integers(4, 257, 512)
     scales = covs_for(wd)
-    wgen, ms_f = once(lambda: er.WishartBatch(scales, wd + 3))
     wrng = er.Vmt19937_64(1 << 20, n=512, stride=4)
     wout = torch.empty(int((wd.astype(np.int64) ** 2).sum()) * 64, dtype=torch.float32, device="cuda")
-    ms = timed(lambda: wgen.generate(wrng, 64, out=wout, flat=True), reps=3)
-    elems = wout.numel()
-    emit_line({"sweep": "Wishart batch generate", "instances": 512, "elements": elems, "ms": round(ms, 4),
-               "gsamples_per_s": round(elems / ms / 1e6, 3), "hbm_gbs": round(4 * elems / ms / 1e6, 1),
-               "hbm_frac": round(4 * elems / ms / 1e6 / HBM_PEAK_GBS, 4), "factorise_ms": round(ms_f, 3)})
-    del wgen, wout, scales
+    for name, inverse in (("Wishart", False), ("InvWishart", True)):
+        wgen, ms_f = once(lambda: er.WishartBatch(scales, wd + 3, inverse=inverse))
+        ms = timed(lambda: wgen.generate(wrng, 64, out=wout, flat=True), reps=3)
+        elems = wout.numel()
+        emit_line({"sweep": f"{name} batch generate", "instances": 512, "elements": elems, "ms": round(ms, 4),
+                   "gsamples_per_s": round(elems / ms / 1e6, 3), "hbm_gbs": round(4 * elems / ms / 1e6, 1),
+                   "hbm_frac": round(4 * elems / ms / 1e6 / HBM_PEAK_GBS, 4), "factorise_ms": round(ms_f, 3)})
+        del wgen
+    del wout, scales
     K = rng.integers(2, 1001, 2048)
     alphas = [np.exp(rng.uniform(np.log(0.01), np.log(10), k)).astype(np.float32) for k in K]
     drng = er.Vmt19937_64(1 << 21, n=2048, stride=4)
