@@ -128,6 +128,18 @@ def sweep():
     ]
     for name, fn in univ:
         emit(name, N_SAMPLES, timed(fn, reps=3), 4)
+    del out, outi
+    # f64 forms: 2^28 doubles (2 GiB written) from the same engines
+    outd = torch.empty(N_SAMPLES, dtype=torch.float64, device="cuda")
+    univ64 = [
+        ("uniformReal f64", lambda: er.uniform_real_f64(PER_ENGINE, 1, urng, out=outd)),
+        ("normal(0,1) f64", lambda: er.normal_f64(PER_ENGINE, 1, urng, out=outd)),
+        ("exponential(1) f64", lambda: er.exponential_f64(PER_ENGINE, 1, urng, 1.0, out=outd)),
+        ("cauchy(0,1) f64", lambda: er.cauchy_f64(PER_ENGINE, 1, urng, 0.0, 1.0, out=outd)),
+    ]
+    for name, fn in univ64:
+        emit(name, N_SAMPLES, timed(fn, reps=3), 8)
+    del outd
 
     # vectorize-over-params, 2^24 elements, parameters from mt19937_64(2024)
     nv = 1 << 24
@@ -146,7 +158,7 @@ def sweep():
     emit("VoP gamma f32", nv, timed(lambda: er.gamma_vop(vp, shape, ones, out=vo), reps=3), 12)
     emit("VoP binomial i32", nv, timed(lambda: er.binomial_vop(vp, bn, bp, out=vi), reps=3), 12)
     emit("VoP poisson i32", nv, timed(lambda: er.poisson_vop(vp, pm, out=vi), reps=3), 8)
-    del out, outi, mu, sd, shape, ones, bn, bp, pm, vo, vi
+    del mu, sd, shape, ones, bn, bp, pm, vo, vi
     torch.cuda.empty_cache()
     mv_sweep(lambda d: print(json.dumps(d), flush=True))
 

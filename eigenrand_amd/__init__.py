@@ -26,7 +26,8 @@ from ._lib import ErdError, InvalidArgument
 __all__ = [
     "Vmt19937_64", "P8_mt19937_64", "Generator", "ErdError", "InvalidArgument", "STATE_BYTES",
     "p8_randbits_u64", "p8_randbits_u32", "p8_uniform_real",
-    "randbits_u64", "uniform_real", "uniform_real_f64", "normal", "normal_vop",
+    "randbits_u64", "uniform_real", "uniform_real_f64", "normal", "normal_f64", "normal_vop",
+    "exponential_f64", "cauchy_f64",
     "exponential", "cauchy", "student_t", "gamma", "chi_squared", "poisson", "binomial",
     "lognormal", "weibull", "extreme_value",
     "bernoulli", "balanced", "geometric", "randbits_i32", "uniform_int", "beta", "fisher_f", "negative_binomial", "uniform_real_vop", "balanced_vop", "bernoulli_vop",
@@ -92,8 +93,9 @@ class Vmt19937_64:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and _lib._lib is not None:
-            _lib._lib.erd_engines_destroy(h)
+        lib = getattr(_lib, "_lib", None) if _lib is not None else None  # None at interpreter exit
+        if h is not None and lib is not None:
+            lib.erd_engines_destroy(h)
             self._h = None
 
     def view(self, first, n):
@@ -208,8 +210,9 @@ class Generator:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and _lib._lib is not None:
-            _lib._lib.erd_generator_destroy(h)
+        lib = getattr(_lib, "_lib", None) if _lib is not None else None  # None at interpreter exit
+        if h is not None and lib is not None:
+            lib.erd_generator_destroy(h)
             self._h = None
 
 
@@ -269,6 +272,16 @@ def normal(rows, cols, urng, mean=None, stdev=None, out=None, stream=None):
                  float(0.0 if mean is None else mean), float(1.0 if stdev is None else stdev))
 
 
+def normal_f64(rows, cols, urng, mean=None, stdev=None, out=None, stream=None):
+    """Rand::normal<MatrixXd>(rows, cols, urng[, mean, stdev]): StdNormalGen /
+    NormalGen at Packet4d (four doubles per engine draw)."""
+    torch = _torch()
+    if mean is None and stdev is None:
+        return _fill("erd_fill_std_normal_f64", urng, rows, cols, torch.float64, out, stream)
+    return _fill("erd_fill_normal_f64", urng, rows, cols, torch.float64, out, stream,
+                 float(0.0 if mean is None else mean), float(1.0 if stdev is None else stdev))
+
+
 def normal_vop(urng, mean, stdev, out=None, stream=None):
     """Rand::normal(urng, mean_array, stdev_array) (NormalVGen); mean/stdev are
     CUDA float32 tensors of n*per_engine elements (engine g uses its block)."""
@@ -287,6 +300,12 @@ def normal_vop(urng, mean, stdev, out=None, stream=None):
 def exponential(rows, cols, urng, lmbda=1.0, out=None, stream=None):
     """Rand::exponential<MatrixXf>(rows, cols, urng, lambda)."""
     return _fill("erd_fill_exponential_f32", urng, rows, cols, _torch().float32, out, stream,
+                 float(lmbda))
+
+
+def exponential_f64(rows, cols, urng, lmbda=1.0, out=None, stream=None):
+    """Rand::exponential<MatrixXd>(rows, cols, urng, lambda)."""
+    return _fill("erd_fill_exponential_f64", urng, rows, cols, _torch().float64, out, stream,
                  float(lmbda))
 
 
@@ -310,6 +329,12 @@ def extreme_value(rows, cols, urng, a=0.0, b=1.0, out=None, stream=None):
 def cauchy(rows, cols, urng, a=0.0, b=1.0, out=None, stream=None):
     """Rand::cauchy<MatrixXf>(rows, cols, urng, a, b)."""
     return _fill("erd_fill_cauchy_f32", urng, rows, cols, _torch().float32, out, stream,
+                 float(a), float(b))
+
+
+def cauchy_f64(rows, cols, urng, a=0.0, b=1.0, out=None, stream=None):
+    """Rand::cauchy<MatrixXd>(rows, cols, urng, a, b)."""
+    return _fill("erd_fill_cauchy_f64", urng, rows, cols, _torch().float64, out, stream,
                  float(a), float(b))
 
 
@@ -632,8 +657,9 @@ class MvFactors:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and _lib._lib is not None:
-            _lib._lib.erd_mv_factors_destroy(h)
+        lib = getattr(_lib, "_lib", None) if _lib is not None else None  # None at interpreter exit
+        if h is not None and lib is not None:
+            lib.erd_mv_factors_destroy(h)
             self._h = None
 
 

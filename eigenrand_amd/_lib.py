@@ -59,6 +59,10 @@ _SIGS = {
     "erd_fill_uniform_f32": [P, P, I64, F32, F32, P],
     "erd_fill_uniform_f64": [P, P, I64, F64, F64, P],
     "erd_fill_std_normal_f32": [P, P, I64, P],
+    "erd_fill_std_normal_f64": [P, P, I64, P],
+    "erd_fill_normal_f64": [P, P, I64, F64, F64, P],
+    "erd_fill_exponential_f64": [P, P, I64, F64, P],
+    "erd_fill_cauchy_f64": [P, P, I64, F64, F64, P],
     "erd_fill_normal_f32": [P, P, I64, F32, F32, P],
     "erd_fill_normal_vop_f32": [P, P, I64, P, P, P],
     "erd_fill_exponential_f32": [P, P, I64, F32, P],

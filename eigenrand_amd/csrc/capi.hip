@@ -213,6 +213,30 @@ int erd_fill_uniform_f64(erd_engines* e, double* out, int64_t per, double a, dou
     return hip_status(erd::launch_fill_uniform_f64(e->states, e->count, out, per, a, b, affine, S(stream)));
 }
 
+int erd_fill_std_normal_f64(erd_engines* e, double* out, int64_t per, void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_normal_f64(e->states, e->count, out, per, 0.0, 1.0, 0, S(stream)));
+}
+
+int erd_fill_normal_f64(erd_engines* e, double* out, int64_t per, double mean, double sd, void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_normal_f64(e->states, e->count, out, per, mean, sd, 1, S(stream)));
+}
+
+int erd_fill_exponential_f64(erd_engines* e, double* out, int64_t per, double lambda, void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_exponential_f64(e->states, e->count, out, per, lambda, S(stream)));
+}
+
+int erd_fill_cauchy_f64(erd_engines* e, double* out, int64_t per, double a, double b, void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_cauchy_f64(e->states, e->count, out, per, a, b, S(stream)));
+}
+
 int erd_fill_std_normal_f32(erd_engines* e, float* out, int64_t per, void* stream) {
     int rc = check_fill(e, out, per, 8);
     if (rc) return rc;

@@ -309,6 +309,86 @@ __device__ __forceinline__ void psincos(float xin, float& s, float& c) {
     c = as_f(as_u(rc) ^ sign_cos);
 }
 
+// ------------------------------------------------------------ double math ---
+__device__ __forceinline__ double as_d(uint64_t u) { return __longlong_as_double((long long)u); }
+__device__ __forceinline__ uint64_t as_u64(double d) { return (uint64_t)__double_as_longlong(d); }
+
+// ln(x) for finite x >= 2^-1022: the reference's Packet4d plog
+// (arch/AVX/MorePacketMath.h:781-858), i.e. the single-precision cephes
+// polynomial evaluated in double with ln2 split in q1/q2, unfused. The
+// exponent is converted exactly (uint64_to_double, 696-699) and the
+// zero/negative/NaN masks are constant on this domain.
+__device__ __forceinline__ double plog_pos_f64(double x) {
+    const uint64_t b = as_u64(x);
+    double e = (double)(b >> 52) - 1022.0;
+    x = as_d((b & ~0x7ff0000000000000ull) | 0x3FE0000000000000ull);
+    const bool mask = x < 0.707106781186547524;
+    double tmp = mask ? x : 0.0;
+    x = x - 1.0;
+    e = e - (mask ? 1.0 : 0.0);
+    x = x + tmp;
+    const double x2 = x * x;
+    const double x3 = x2 * x;
+    double y = 7.0376836292E-2 * x + -1.1514610310E-1;
+    double y1 = -1.2420140846E-1 * x + 1.4249322787E-1;
+    double y2 = 2.0000714765E-1 * x + -2.4999993993E-1;
+    y = y * x + 1.1676998740E-1;
+    y1 = y1 * x + -1.6668057665E-1;
+    y2 = y2 * x + 3.3333331174E-1;
+    y = y * x3 + y1;
+    y = y * x3 + y2;
+    y = y * x3;
+    y1 = e * -2.12194440e-4;
+    tmp = x2 * 0.5;
+    y = y + y1;
+    x = x - tmp;
+    y2 = e * 0.693359375;
+    x = x + y;
+    return x + y2;
+}
+
+// sincos for doubles: the reference's cephes scheme at double packets
+// (MorePacketMath.h:396-491), single-precision coefficients evaluated in
+// double. The quadrant j = floor(|x| 4/pi) goes through the reference's
+// double_to_int64 (arch/AVX/MorePacketMath.h:731-737: floor, then the
+// 1.5*2^52 bias), which is exact for |x| 4/pi < 2^51.
+__device__ __forceinline__ void psincos_f64(double xin, double& s, double& c) {
+    uint64_t sign_sin = as_u64(xin) & 0x8000000000000000ull;
+    double x = as_d(as_u64(xin) & 0x7FFFFFFFFFFFFFFFull);
+    double y = x * 1.27323954473516;
+    int64_t j = (int64_t)floor(y);
+    j = (j + 1) & ~(int64_t)1;
+    y = (double)j;
+    const uint64_t swap_sin = (uint64_t)(j & 4) << 61;
+    const bool poly_mask = (j & 2) == 0;
+    x = x + y * -0.78515625;
+    x = x + y * -2.4187564849853515625e-4;
+    x = x + y * -3.77489497744594108e-8;
+    const uint64_t sign_cos = (uint64_t)(4 & ~(j - 2)) << 61;
+    sign_sin ^= swap_sin;
+    const double z = x * x;
+    double yc = 2.443315711809948E-005 * z;
+    yc = yc + -1.388731625493765E-003;
+    yc = yc * z;
+    yc = yc + 4.166664568298827E-002;
+    yc = yc * z;
+    yc = yc * z;
+    yc = yc - z * 0.5;
+    yc = yc + 1.0;
+    double ys = -1.9515295891E-4 * z;
+    ys = ys + 8.3321608736E-3;
+    ys = ys * z;
+    ys = ys + -1.6666654611E-1;
+    ys = ys * z;
+    ys = ys * x;
+    ys = ys + x;
+    // masked select plus the masked +0, as in the float form
+    const double rs = (poly_mask ? ys : yc) + 0.0;
+    const double rc = (poly_mask ? yc : ys) + 0.0;
+    s = as_d(as_u64(rs) ^ sign_sin);
+    c = as_d(as_u64(rc) ^ sign_cos);
+}
+
 // lgamma approximation: MorePacketMath.h:158-170.
 __device__ __forceinline__ float plgamma_approx(float x) {
     const float x_3 = x + 3.0f;

@@ -51,6 +51,57 @@ struct UniformF64 {  // arch/AVX/RandUtils.h:212-225: 4 doubles per call
     }
 };
 
+// StdNormalGen / NormalGen at Packet4d (NormalExp.h:60-82, 127-135): draws
+// 2m and 2m+1 give u1 and u2 of four Box-Muller pairs; out[8m + l] = r sin and
+// out[8m + 4 + l] = r cos (the packet the generator caches for its next call).
+// MODE 0 standard, 1 = z * sd + mean.
+template <int MODE>
+struct NormalF64 {
+    static constexpr int CALLS = 2;
+    double* out;
+    double mean, sd;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return (n + 7) / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        const double u1 = 1.0 - ur_f64(w[0]);  // in [2^-52, 1], exact
+        const double u2 = ur_f64(w[1]);
+        const double radius = sqrt(-2.0 * plog_pos_f64(u1));
+        double s, c;
+        psincos_f64((2.0 * kPi) * u2, s, c);
+        double sn = radius * s, cs = radius * c;
+        if (MODE == 1) {
+            sn = sn * sd + mean;
+            cs = cs * sd + mean;
+        }
+        out[8 * u + l] = sn;
+        if (8 * u + 8 <= per) out[8 * u + 4 + l] = cs;
+    }
+};
+
+struct ExponentialF64 {  // NormalExp.h:391-398 at Packet4d
+    static constexpr int CALLS = 1;
+    double* out;
+    double lambda;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 4; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        out[4 * u + l] = -(plog_pos_f64(1.0 - ur_f64(w[0])) / lambda);
+    }
+};
+
+struct CauchyF64 {  // NormalExp.h:843-855 at Packet4d
+    static constexpr int CALLS = 1;
+    double* out;
+    double a, b;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 4; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        double s, c;
+        psincos_f64(kPi * (ur_f64(w[0]) - 0.5), s, c);
+        out[4 * u + l] = a + b * (s / c);
+    }
+};
+
 // P8_mt19937_64 / P8_mt19937_64_32 (ParallelRandomEngineAdaptor<UInt,
 // Vmt19937_64, 8>, PacketRandomEngine.h:429-517): one adaptor = the Vmt pair
 // (seed, seed+4); a refill writes draw k of Vmt i to buffer words

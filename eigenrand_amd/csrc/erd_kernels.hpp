@@ -37,6 +37,12 @@ hipError_t launch_fill_uniform_f64(EngineState* st, int64_t n, double* out, int6
                                    double b, int affine, hipStream_t s);
 hipError_t launch_fill_normal_f32(EngineState* st, int64_t n, float* out, int64_t per, float mean,
                                   float sd, int affine, hipStream_t s);
+hipError_t launch_fill_normal_f64(EngineState* st, int64_t n, double* out, int64_t per, double mean,
+                                  double sd, int affine, hipStream_t s);
+hipError_t launch_fill_exponential_f64(EngineState* st, int64_t n, double* out, int64_t per,
+                                       double lambda, hipStream_t s);
+hipError_t launch_fill_cauchy_f64(EngineState* st, int64_t n, double* out, int64_t per, double a,
+                                  double b, hipStream_t s);
 hipError_t launch_fill_normal_vop_f32(EngineState* st, int64_t n, float* out, int64_t per,
                                       const float* mean, const float* sd, hipStream_t s);
 hipError_t launch_fill_exponential_f32(EngineState* st, int64_t n, float* out, int64_t per,

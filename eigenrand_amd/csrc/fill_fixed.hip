@@ -103,6 +103,22 @@ hipError_t launch_fill_uniform_f64(EngineState* st, int64_t n, double* out, int6
     return launch_fixed(st, n, per, d, s);
 }
 
+hipError_t launch_fill_normal_f64(EngineState* st, int64_t n, double* out, int64_t per, double mean,
+                                  double sd, int affine, hipStream_t s) {
+    if (affine) return launch_fixed(st, n, per, NormalF64<1>{out, mean, sd}, s);
+    return launch_fixed(st, n, per, NormalF64<0>{out, mean, sd}, s);
+}
+
+hipError_t launch_fill_exponential_f64(EngineState* st, int64_t n, double* out, int64_t per,
+                                       double lambda, hipStream_t s) {
+    return launch_fixed(st, n, per, ExponentialF64{out, lambda}, s);
+}
+
+hipError_t launch_fill_cauchy_f64(EngineState* st, int64_t n, double* out, int64_t per, double a,
+                                  double b, hipStream_t s) {
+    return launch_fixed(st, n, per, CauchyF64{out, a, b}, s);
+}
+
 hipError_t launch_fill_normal_f32(EngineState* st, int64_t n, float* out, int64_t per, float mean,
                                   float sd, int affine, hipStream_t s) {
     if (affine) {

@@ -114,8 +114,9 @@ class Server:
         self._h = h
 
     def close(self):
-        if getattr(self, "_h", None) is not None and _lib._lib is not None:
-            _lib._lib.erd_server_destroy(self._h)
+        lib = getattr(_lib, "_lib", None) if _lib is not None else None  # None at interpreter exit
+        if getattr(self, "_h", None) is not None and lib is not None:
+            lib.erd_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

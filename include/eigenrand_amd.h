@@ -107,6 +107,23 @@ int erd_fill_uniform_f32(erd_engines* e, float* out, int64_t per_engine, float a
 /* uniformReal<double> (arch/AVX/RandUtils.h:212-225): 4 doubles per draw. */
 int erd_fill_uniform_f64(erd_engines* e, double* out, int64_t per_engine, double a, double b,
                          void* stream);
+/* f64 forms at Packet4d (4 doubles per engine draw, per_engine a multiple of
+ * 4). Box-Muller, the log and the sincos follow the reference's double packet
+ * code: plog<Packet4d> (arch/AVX/MorePacketMath.h:781-858) and the double
+ * psincos (MorePacketMath.h:396-491). normal<double>: StdNormalGen
+ * (NormalExp.h:60-82), pairs of draws give [r sin x4, r cos x4]; with
+ * (mean, stdev) it is NormalGen (NormalExp.h:127-135). A per_engine that is 4
+ * mod 8 ends with a sine packet whose cosine packet is dropped, as a fresh
+ * generator per call does. */
+int erd_fill_std_normal_f64(erd_engines* e, double* out, int64_t per_engine, void* stream);
+int erd_fill_normal_f64(erd_engines* e, double* out, int64_t per_engine, double mean, double stdev,
+                        void* stream);
+/* exponential<double>(…, lambda): ExponentialGen (NormalExp.h:391-398). */
+int erd_fill_exponential_f64(erd_engines* e, double* out, int64_t per_engine, double lambda,
+                             void* stream);
+/* cauchy<double>(…, a, b): CauchyGen (NormalExp.h:843-855). */
+int erd_fill_cauchy_f64(erd_engines* e, double* out, int64_t per_engine, double a, double b,
+                        void* stream);
 /* normal<float>(rows, cols, urng): StdNormalGen (NormalExp.h:60-82). */
 int erd_fill_std_normal_f32(erd_engines* e, float* out, int64_t per_engine, void* stream);
 /* normal<float>(rows, cols, urng, mean, stdev): NormalGen (NormalExp.h:127-135). */

@@ -202,6 +202,31 @@ inline DeviceMatrix<float> normal(int64_t rows, int64_t cols, Vmt19937_64& urng,
         return erd_fill_normal_f32(urng.handle(), o, n, mean, stdev, stream);
     }, "erd_fill_normal_f32");
 }
+// Rand::normal<MatrixXd> / exponential<MatrixXd> / cauchy<MatrixXd>: the
+// Packet4d forms (four doubles per engine draw).
+inline DeviceMatrix<double> normalDouble(int64_t rows, int64_t cols, Vmt19937_64& urng, void* stream = nullptr) {
+    return detail::fill<double>(rows, cols, urng.size(), [&](double* o, int64_t n) {
+        return erd_fill_std_normal_f64(urng.handle(), o, n, stream);
+    }, "erd_fill_std_normal_f64");
+}
+inline DeviceMatrix<double> normalDouble(int64_t rows, int64_t cols, Vmt19937_64& urng, double mean, double stdev,
+                                         void* stream = nullptr) {
+    return detail::fill<double>(rows, cols, urng.size(), [&](double* o, int64_t n) {
+        return erd_fill_normal_f64(urng.handle(), o, n, mean, stdev, stream);
+    }, "erd_fill_normal_f64");
+}
+inline DeviceMatrix<double> exponentialDouble(int64_t rows, int64_t cols, Vmt19937_64& urng, double lambda = 1.0,
+                                              void* stream = nullptr) {
+    return detail::fill<double>(rows, cols, urng.size(), [&](double* o, int64_t n) {
+        return erd_fill_exponential_f64(urng.handle(), o, n, lambda, stream);
+    }, "erd_fill_exponential_f64");
+}
+inline DeviceMatrix<double> cauchyDouble(int64_t rows, int64_t cols, Vmt19937_64& urng, double a = 0.0,
+                                         double b = 1.0, void* stream = nullptr) {
+    return detail::fill<double>(rows, cols, urng.size(), [&](double* o, int64_t n) {
+        return erd_fill_cauchy_f64(urng.handle(), o, n, a, b, stream);
+    }, "erd_fill_cauchy_f64");
+}
 inline DeviceMatrix<float> lognormal(int64_t rows, int64_t cols, Vmt19937_64& urng, float mean = 0.f,
                                      float stdev = 1.f, void* stream = nullptr) {
     return detail::fill<float>(rows, cols, urng.size(), [&](float* o, int64_t n) {

@@ -560,6 +560,74 @@ void orc_fill_uniform_f64(void* states, int64_t n_engines, double* out, int64_t 
     });
 }
 
+// normal<double>: StdNormalGen at Packet4d (NormalExp.h:60-82), NormalGen
+// z*sd+mean when affine=1 (127-135). Draws 2m, 2m+1 -> four pairs: the sine
+// packet is returned, the cosine packet cached for the next packetOp; a fresh
+// generator per engine block drops a trailing cosine packet.
+void orc_fill_normal_f64(void* states, int64_t n_engines, double* out, int64_t per_engine,
+                         double mean, double sd, int affine, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        double* o = out + g * per_engine;
+        double cache[4];
+        bool valid = false;
+        uint64_t p1[4], p2[4];
+        for (int64_t i = 0; i < per_engine; i += 4) {
+            if (valid) {
+                for (int j = 0; j < 4; ++j) o[i + j] = cache[j];
+                valid = false;
+            } else {
+                vmt_draw(e[g], p1);
+                vmt_draw(e[g], p2);
+                for (int j = 0; j < 4; ++j) {
+                    const double u1 = 1.0 - bit_to_ur_double(p1[j]);
+                    const double u2 = bit_to_ur_double(p2[j]);
+                    const double radius = std::sqrt(-2.0 * plog_d(u1));
+                    const double theta = (2 * kPi) * u2;
+                    double sn, cs;
+                    psincos_d(theta, sn, cs);
+                    cache[j] = radius * cs;
+                    o[i + j] = radius * sn;
+                }
+                valid = true;
+            }
+            if (affine) for (int j = 0; j < 4; ++j) o[i + j] = o[i + j] * sd + mean;
+        }
+    });
+}
+
+// ExponentialGen<double> (NormalExp.h:391-398): -(log(1-u)/lambda).
+void orc_fill_exponential_f64(void* states, int64_t n_engines, double* out, int64_t per_engine,
+                              double lambda, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        double* o = out + g * per_engine;
+        uint64_t p[4];
+        for (int64_t i = 0; i < per_engine; i += 4) {
+            vmt_draw(e[g], p);
+            for (int j = 0; j < 4; ++j) o[i + j] = -(plog_d(1.0 - bit_to_ur_double(p[j])) / lambda);
+        }
+    });
+}
+
+// CauchyGen<double> (NormalExp.h:843-855): a + b*(sin/cos)(pi*(u-0.5)).
+void orc_fill_cauchy_f64(void* states, int64_t n_engines, double* out, int64_t per_engine,
+                         double a, double b, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        double* o = out + g * per_engine;
+        uint64_t p[4];
+        for (int64_t i = 0; i < per_engine; i += 4) {
+            vmt_draw(e[g], p);
+            for (int j = 0; j < 4; ++j) {
+                double sn, cs;
+                psincos_d(kPi * (bit_to_ur_double(p[j]) - 0.5), sn, cs);
+                o[i + j] = a + b * (sn / cs);
+            }
+        }
+    });
+}
+
 // normal: StdNormalGen when affine=0, NormalGen z*sd+mean (NormalExp.h:
 // 127-135) when affine=1. A fresh generator per engine block.
 void orc_fill_normal_f32(void* states, int64_t n_engines, float* out, int64_t per_engine,
@@ -1116,6 +1184,10 @@ void orc_psincos(const float* x, float* s, float* c, int64_t n) {
 }
 void orc_plgamma(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = plgamma_approx(x[i]); }
 void orc_bit_to_ur_float(const uint32_t* w, float* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = bit_to_ur_float(w[i]); }
+void orc_plog_f64(const double* x, double* y, int64_t n) { for (int64_t i = 0; i < n; ++i) y[i] = plog_d(x[i]); }
+void orc_psincos_f64(const double* x, double* s, double* c, int64_t n) {
+    for (int64_t i = 0; i < n; ++i) psincos_d(x[i], s[i], c[i]);
+}
 
 }  // extern "C"
 

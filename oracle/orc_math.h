@@ -165,6 +165,118 @@ static inline void psincos(float xin, float& s, float& c) {
     c = as_f(as_u(xmm2) ^ sign_bit_cos);
 }
 
+static inline uint64_t as_u64(double d) { uint64_t u; std::memcpy(&u, &d, 8); return u; }
+
+// The reference's 64-bit lane conversions (arch/AVX/MorePacketMath.h:696-737),
+// written with their bias constants: uint64_to_double ORs in the bits of 2^52
+// and subtracts 2^52; int64_to_double adds the bits of 1.5*2^52 as an integer
+// and subtracts 1.5*2^52; double_to_int64 floors, adds 1.5*2^52 and
+// subtracts the bias bits as integers.
+static inline double uint64_to_double(uint64_t x) {
+    return as_d(x | as_u64(4503599627370496.0)) - 4503599627370496.0;
+}
+static inline double int64_to_double(int64_t x) {
+    return as_d((uint64_t)x + as_u64(6755399441055744.0)) - 6755399441055744.0;
+}
+static inline int64_t double_to_int64(double x) {
+    x = std::floor(x) + 6755399441055744.0;
+    return (int64_t)(as_u64(x) - as_u64(6755399441055744.0));
+}
+
+// plog<Packet4d>, the reference's double log (arch/AVX/MorePacketMath.h:
+// 781-858): single-precision cephes coefficients evaluated in double with ln2
+// split in q1/q2, unfused; 0 -> -inf, negative or NaN -> NaN (all bits set).
+static inline double plog_d(double x0) {
+    double x = x0;
+    const bool invalid = !(x >= 0.0);             // _CMP_NGE_UQ
+    const bool iszero = (x == 0.0);
+    const double min_norm = as_d(0x10000000000000ull);
+    x = (x > min_norm) ? x : min_norm;            // _mm256_max_pd(x, min_norm)
+    const double emm0 = uint64_to_double(as_u64(x) >> 52);
+    double e = emm0 - 1022.0;
+    x = as_d((as_u64(x) & ~0x7ff0000000000000ull) | as_u64(0.5));
+    const bool mask = x < 0.707106781186547524;
+    double tmp = mask ? x : 0.0;
+    x = x - 1.0;
+    e = e - (mask ? 1.0 : 0.0);
+    x = x + tmp;
+    const double x2 = x * x;
+    const double x3 = x2 * x;
+    double y, y1, y2;
+    y = 7.0376836292E-2 * x + -1.1514610310E-1;
+    y1 = -1.2420140846E-1 * x + +1.4249322787E-1;
+    y2 = +2.0000714765E-1 * x + -2.4999993993E-1;
+    y = y * x + 1.1676998740E-1;
+    y1 = y1 * x + -1.6668057665E-1;
+    y2 = y2 * x + +3.3333331174E-1;
+    y = y * x3 + y1;
+    y = y * x3 + y2;
+    y = y * x3;
+    y1 = e * -2.12194440e-4;
+    tmp = x2 * 0.5;
+    y = y + y1;
+    x = x - tmp;
+    y2 = e * 0.693359375;
+    x = x + y;
+    x = x + y2;
+    if (iszero) return as_d(0xfff0000000000000ull);
+    if (invalid) return as_d(as_u64(x) | 0xFFFFFFFFFFFFFFFFull);
+    return x;
+}
+
+// sincos at double packets: EigenRand/MorePacketMath.h:396-491 (the float
+// scheme's steps with 64-bit lanes and the conversions above).
+static inline void psincos_d(double xin, double& s, double& c) {
+    double x = xin;
+    uint64_t sign_bit_sin = as_u64(x) & 0x8000000000000000ull;
+    x = as_d(as_u64(x) & 0x7FFFFFFFFFFFFFFFull);
+    double y = x * 1.27323954473516;
+    int64_t emm2 = double_to_int64(y);
+    emm2 = emm2 + 1;
+    emm2 = emm2 & ~1ll;
+    y = int64_to_double(emm2);
+    int64_t emm4 = emm2;
+    const uint64_t swap_sign_bit_sin = (uint64_t)(emm2 & 4) << 61;
+    const bool poly_mask = (emm2 & 2) == 0;
+    const double xmm1 = y * -0.78515625;
+    const double xmm2 = y * -2.4187564849853515625e-4;
+    const double xmm3 = y * -3.77489497744594108e-8;
+    x = x + xmm1;
+    x = x + xmm2;
+    x = x + xmm3;
+    emm4 = emm4 - 2;
+    emm4 = 4 & ~emm4;
+    const uint64_t sign_bit_cos = (uint64_t)emm4 << 61;
+    sign_bit_sin ^= swap_sign_bit_sin;
+    const double z = x * x;
+    double yc = 2.443315711809948E-005;
+    yc = yc * z;
+    yc = yc + -1.388731625493765E-003;
+    yc = yc * z;
+    yc = yc + 4.166664568298827E-002;
+    yc = yc * z;
+    yc = yc * z;
+    const double tmp = z * 0.5;
+    yc = yc - tmp;
+    yc = yc + 1.0;
+    double ys = -1.9515295891E-4;
+    ys = ys * z;
+    ys = ys + 8.3321608736E-3;
+    ys = ys * z;
+    ys = ys + -1.6666654611E-1;
+    ys = ys * z;
+    ys = ys * x;
+    ys = ys + x;
+    const double ysin2 = poly_mask ? ys : 0.0;
+    const double ysin1 = poly_mask ? 0.0 : yc;
+    ys = ys - ysin2;
+    yc = yc - ysin1;
+    const double r1 = ysin1 + ysin2;
+    const double r2 = yc + ys;
+    s = as_d(as_u64(r1) ^ sign_bit_sin);
+    c = as_d(as_u64(r2) ^ sign_bit_cos);
+}
+
 // plgamma_approx: EigenRand/MorePacketMath.h:158-170 (float constants).
 static inline float plgamma_approx(float x) {
     const float x_3 = x + 3.0f;

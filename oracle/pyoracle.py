@@ -105,6 +105,26 @@ def fill_uniform_f64(e, per, a=0.0, b=1.0, threads=1):
     return out
 
 
+def _f64(e, per, name, *args, threads=1):
+    out = np.empty(len(e) * per, dtype=np.float64)
+    getattr(lib(), name)(_p(e.raw), I64(len(e)), _p(out), I64(per), *args, C.c_int(threads))
+    return out
+
+
+def fill_normal_f64(e, per, mean=None, sd=None, threads=1):
+    affine = int(mean is not None or sd is not None)
+    return _f64(e, per, "orc_fill_normal_f64", C.c_double(0.0 if mean is None else mean),
+                C.c_double(1.0 if sd is None else sd), C.c_int(affine), threads=threads)
+
+
+def fill_exponential_f64(e, per, lmbda=1.0, threads=1):
+    return _f64(e, per, "orc_fill_exponential_f64", C.c_double(lmbda), threads=threads)
+
+
+def fill_cauchy_f64(e, per, a=0.0, b=1.0, threads=1):
+    return _f64(e, per, "orc_fill_cauchy_f64", C.c_double(a), C.c_double(b), threads=threads)
+
+
 def fill_normal_f32(e, per, mean=None, sd=None, threads=1):
     affine = int(mean is not None or sd is not None)
     return _f32(e, per, "orc_fill_normal_f32", C.c_float(0.0 if mean is None else mean),
@@ -251,8 +271,8 @@ def fill_binomial_vop_i32(e, trials, p, threads=1):
     return out
 
 
-def _elementwise(name, x, nout=1):
-    x = np.ascontiguousarray(x, dtype=np.float32)
+def _elementwise(name, x, nout=1, dtype=np.float32):
+    x = np.ascontiguousarray(x, dtype=dtype)
     outs = [np.empty_like(x) for _ in range(nout)]
     getattr(lib(), name)(_p(x), *[_p(o) for o in outs], I64(x.size))
     return outs[0] if nout == 1 else tuple(outs)
@@ -268,6 +288,14 @@ def pexp(x):
 
 def psincos(x):
     return _elementwise("orc_psincos", x, 2)
+
+
+def plog_f64(x):
+    return _elementwise("orc_plog_f64", x, dtype=np.float64)
+
+
+def psincos_f64(x):
+    return _elementwise("orc_psincos_f64", x, 2, dtype=np.float64)
 
 
 def plgamma(x):

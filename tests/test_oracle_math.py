@@ -80,3 +80,50 @@ def test_bit_to_ur_float_closed_form(orc):
     got = orc.bit_to_ur_float(w)
     exp = ((w & 0x7FFFFF).astype(np.float64) / 2 ** 23).astype(np.float32)
     np.testing.assert_array_equal(got, exp)
+
+
+# ----------------------------------------------------------------- doubles ---
+# The reference's double log (arch/AVX/MorePacketMath.h:781-858) and double
+# sincos (MorePacketMath.h:396-491) evaluate the single-precision cephes
+# polynomials in double: about 1e-9 absolute from libm, far from both a float
+# result (1e-7) and a broken term (>1e-6).
+
+def test_plog_f64_bound(orc):
+    rng = np.random.default_rng(11)
+    # Box-Muller / exponential input 1-U = k*2^-52 in (0,1], and a wide range
+    k = rng.integers(1, 1 << 52, 300000, dtype=np.int64)
+    x = np.concatenate([k / 2.0 ** 52, np.exp(rng.uniform(-700, 700, 100000)),
+                        2.0 ** -np.arange(0, 53), [1.0 - 2.0 ** -52, 2.0 ** -1022]])
+    y = orc.plog_f64(x)
+    ref = np.log(x)
+    err = np.abs(y - ref) / np.maximum(1.0, np.abs(ref))
+    assert err.max() < 2e-9, err.max()
+    # values within 1e-9 but not identical to libm: the float polynomial
+    assert err.max() > 1e-12
+
+
+def test_plog_f64_special_values(orc):
+    y = orc.plog_f64(np.array([0.0, -0.0, -1.0, np.nan, 1.0, 0.5]))
+    assert y[0] == -np.inf and y[1] == -np.inf
+    assert np.isnan(y[2]) and np.isnan(y[3])
+    assert y[4] == 0.0
+    # ln 2 = q2 + q1 with q1 = -2.12194440e-4 and q2 = 0.693359375
+    assert y[5] == -(0.693359375 + -2.12194440e-4)
+
+
+def test_psincos_f64_bound(orc):
+    rng = np.random.default_rng(12)
+    u = rng.integers(0, 1 << 52, 300000, dtype=np.int64) / 2.0 ** 52
+    x = np.concatenate([2 * math.pi * u, math.pi * (u - 0.5), rng.uniform(-50, 50, 100000)])
+    s, c = orc.psincos_f64(x)
+    assert np.abs(s - np.sin(x)).max() < 4e-9
+    assert np.abs(c - np.cos(x)).max() < 4e-9
+    # odd/even symmetry holds bit for bit: the sign is handled on |x|
+    s2, c2 = orc.psincos_f64(-x)
+    assert np.array_equal(s2, -s) and np.array_equal(c2, c)
+
+
+def test_psincos_f64_exact_points(orc):
+    s, c = orc.psincos_f64(np.array([0.0, -0.0]))
+    assert s[0] == 0.0 and c[0] == 1.0 and c[1] == 1.0
+    assert np.signbit(s[1])

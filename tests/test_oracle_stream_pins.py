@@ -79,6 +79,48 @@ def test_exponential_cauchy_studentt_of_the_stream(orc):
     np.testing.assert_allclose(st, want, rtol=3e-5, atol=3e-5)
 
 
+def urd(w64):
+    """bit_to_ur_double of u64 words (52 mantissa bits)."""
+    return (w64 & np.uint64((1 << 52) - 1)).astype(np.float64) / 2.0 ** 52
+
+
+@pytest.mark.parametrize("per", [1024, 1020])
+def test_normal_f64_is_box_muller_of_the_stream(orc, per):
+    # Packet4d: calls 2m, 2m+1 give u1, u2; outputs [r sin x4, r cos x4]; a
+    # block of 4 mod 8 ends with a sine packet and consumes its two calls.
+    e = orc.Engines(n=3, seed0=91)
+    units = (per + 7) // 8
+    got = orc.fill_normal_f64(e.copy(), per).reshape(3, per)
+    raw = orc.fill_u64(e.copy(), 2 * units).reshape(3, units, 2, 4)
+    u1 = 1.0 - urd(raw[:, :, 0, :])
+    u2 = urd(raw[:, :, 1, :])
+    r = np.sqrt(-2 * np.log(u1))
+    want = np.stack([r * np.sin(2 * math.pi * u2), r * np.cos(2 * math.pi * u2)], 2)
+    want = want.reshape(3, units * 8)[:, :per]
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-8 * (1 + np.abs(want)).max())
+    # the engines advance by whole units
+    e2 = e.copy()
+    orc.fill_normal_f64(e2, per)
+    e3 = e.copy()
+    orc.fill_u64(e3, 2 * units)
+    assert np.array_equal(orc.fill_u64(e2, 4), orc.fill_u64(e3, 4))
+    # NormalGen: z * sd + mean, unfused, from the same z
+    aff = orc.fill_normal_f64(e.copy(), per, 2.5, 0.25).reshape(3, per)
+    assert np.array_equal(aff, got * 0.25 + 2.5)
+
+
+def test_exponential_cauchy_f64_of_the_stream(orc):
+    e = orc.Engines(n=2, seed0=17)
+    per = 512
+    u = urd(orc.fill_u64(e.copy(), per // 4)).reshape(2, per)
+    ex = orc.fill_exponential_f64(e.copy(), per, 2.0).reshape(2, per)
+    np.testing.assert_allclose(ex, -np.log(1 - u) / 2.0, rtol=0, atol=2e-9)
+    assert np.all(np.signbit(ex[u == 0]))  # pnegate of +0
+    ca = orc.fill_cauchy_f64(e.copy(), per, 1.0, 3.0).reshape(2, per)
+    want = 1.0 + 3.0 * np.tan(math.pi * (u - 0.5))
+    np.testing.assert_allclose(ca, want, rtol=1e-7, atol=1e-7)
+
+
 @pytest.mark.parametrize("alpha", [1.0, 2.0, 7.0])
 def test_gamma_integer_alpha_of_the_stream(orc, alpha):
     e = orc.Engines(n=2, seed0=9)
