@@ -89,6 +89,10 @@ def sweep():
     urng = er.Vmt19937_64(0, n=N_ENGINES, stride=4)
     out = torch.empty(N_SAMPLES, dtype=torch.float32, device="cuda")
     outi = torch.empty(N_SAMPLES, dtype=torch.int32, device="cuda")
+    # bring the clocks to their steady state before the first row
+    for _ in range(150):
+        er.normal(PER_ENGINE, 1, urng, out=out)
+    torch.cuda.synchronize()
     univ = [
         ("uniformReal f32", lambda: er.uniform_real(PER_ENGINE, 1, urng, out=out)),
         ("normal(0,1) f32", lambda: er.normal(PER_ENGINE, 1, urng, out=out)),
@@ -389,8 +393,10 @@ def load_traffic():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # The GPU's clocks take ~30 ms of continuous work (~75 launches) to settle;
+    # the default warm-up covers that, and 200 timed launches take ~75 ms.
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--sweep", action="store_true", help="secondary per-distribution measurements")
     ap.add_argument("--mv", action="store_true", help="only the batched multivariate measurements")

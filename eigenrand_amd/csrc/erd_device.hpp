@@ -86,13 +86,14 @@ __device__ __forceinline__ uint64_t temper(uint64_t y) {
 
 __device__ __forceinline__ uint64_t twist_word(uint64_t cur, uint64_t next, uint64_t far) {
     // y = (cur & UM) | (next & LM); far ^ (y >> 1) ^ (y odd ? A : 0), with
-    // UM = upper 33 bits, on 32-bit halves.
+    // UM = upper 33 bits, on 32-bit halves. y's parity is next's (bit 0 is in
+    // LM); (y >> 1) ^ (odd & A) is one bitop3 per half.
     const uint32_t y_hi = (uint32_t)(cur >> 32);
     const uint32_t y_lo = __builtin_amdgcn_bitop3_b32((uint32_t)cur, (uint32_t)next, 0x7fffffffu, 0xd8);  // c ? b : a
     const uint32_t sh_lo = funnel_lo(y_hi, y_lo, 1), sh_hi = y_hi >> 1;
-    const uint32_t odd = 0u - (y_lo & 1u);
-    const uint32_t r_lo = (uint32_t)far ^ sh_lo ^ (odd & (uint32_t)kA);
-    const uint32_t r_hi = (uint32_t)(far >> 32) ^ sh_hi ^ (odd & (uint32_t)(kA >> 32));
+    const uint32_t odd = 0u - ((uint32_t)next & 1u);
+    const uint32_t r_lo = (uint32_t)far ^ xor_and(sh_lo, odd, (uint32_t)kA);
+    const uint32_t r_hi = (uint32_t)(far >> 32) ^ xor_and(sh_hi, odd, (uint32_t)(kA >> 32));
     return ((uint64_t)r_hi << 32) | r_lo;
 }
 
@@ -102,12 +103,42 @@ __device__ __forceinline__ uint64_t twist_word(uint64_t cur, uint64_t next, uint
 // phase-1 result at i-156; phase 3 (i = 311) reads new words 0 and 155.
 // Within a phase a word i is read (as i+1) before it is rewritten because the
 // store depends on the load and later loop trips only touch higher i.
+// Twist of words base + lane + 64k, k in [K0, K0 + NT), of a phase with
+// `total` words; `far` is the offset of each word's x[i+m] partner. All loads
+// of the batch are issued before any of its stores, and the stores before any
+// later load: the memory clobbers pin that order, which the other lanes rely
+// on (a word is read as a neighbour's `next` before its own lane rewrites it).
+template <int K0, int NT>
+__device__ __forceinline__ void twist_batch(uint64_t* mt, int base, int total, int far, int lane) {
+    uint64_t c[NT], n[NT], f[NT];
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+        const int r = lane + 64 * (K0 + i);
+        if (64 * (K0 + i) + 63 < total || r < total) {  // full trips need no test
+            c[i] = mt[base + r];
+            n[i] = mt[base + r + kLanes];
+            f[i] = mt[base + r + far];
+        }
+    }
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+        const int r = lane + 64 * (K0 + i);
+        if (64 * (K0 + i) + 63 < total || r < total) mt[base + r] = twist_word(c[i], n[i], f[i]);
+    }
+    asm volatile("" ::: "memory");
+}
+
+// The two large phases run in batches of five 64-word trips, so each batch's
+// LDS loads are in flight together instead of one trip at a time.
 __device__ __forceinline__ void twist_lds(uint64_t* mt, int lane) {
-    for (int t = lane; t < kM * kLanes; t += 64)
-        mt[t] = twist_word(mt[t], mt[t + kLanes], mt[t + kM * kLanes]);
+    constexpr int P1 = kM * kLanes;              // 624 words, trips 0..9
+    constexpr int P2 = (kN - 1) * kLanes - P1;   // 620 words, trips 0..9
+    twist_batch<0, 5>(mt, 0, P1, P1, lane);
+    twist_batch<5, 5>(mt, 0, P1, P1, lane);
     __syncthreads();
-    for (int t = kM * kLanes + lane; t < (kN - 1) * kLanes; t += 64)
-        mt[t] = twist_word(mt[t], mt[t + kLanes], mt[t - kM * kLanes]);
+    twist_batch<0, 5>(mt, P1, P2, -P1, lane);
+    twist_batch<5, 5>(mt, P1, P2, -P1, lane);
     __syncthreads();
     if (lane < kLanes) {
         int t = (kN - 1) * kLanes + lane;
@@ -218,15 +249,16 @@ __device__ __forceinline__ float plog(float x0) {
 }
 
 // plog restricted to inputs in [2^-126, +inf) finite: the special-case masks
-// of plog are constant there, so the result is identical and cheaper.
+// of plog are constant there, so the result is identical and cheaper. For such
+// x, frexp gives the reference's mantissa with the exponent set to -1 and the
+// exponent emm0 + 1, so e is formed from an integer in one conversion.
 __device__ __forceinline__ float plog_pos(float x) {
-    int32_t emm0 = (int32_t)(as_u(x) >> 23) - 0x7f;
-    x = as_f((as_u(x) & 0x807FFFFFu) | 0x3F000000u);
-    float e = (float)emm0 + 1.0f;
+    const int32_t k = __builtin_amdgcn_frexp_expf(x);
+    x = __builtin_amdgcn_frexp_mantf(x);
     const bool mask = x < 0.707106781186547524f;
+    const float e = (float)(mask ? k - 1 : k);
     float tmp = mask ? x : 0.0f;
     x = x - 1.0f;
-    e = e - (mask ? 1.0f : 0.0f);
     x = x + tmp;
     const float x2 = x * x;
     const float x3 = x2 * x;
@@ -307,6 +339,43 @@ __device__ __forceinline__ void psincos(float xin, float& s, float& c) {
     const float rc = (poly_mask ? yc : ys) + 0.0f;
     s = as_f(as_u(rs) ^ sign_sin);
     c = as_f(as_u(rc) ^ sign_cos);
+}
+
+// psincos for x >= +0 (the callers pass pi*U or 2*pi*U with U in [0, 1)).
+// The input's sign bit is 0, and the reduced argument never becomes -0 (each
+// reduction step adds a nonpositive term to a value that is not -0, and an
+// exact cancellation rounds to +0), so neither polynomial can yield -0 and the
+// reference's masked "+0" addition is an identity. Both are left out; the
+// results equal psincos's bit for bit on this domain.
+__device__ __forceinline__ void psincos_pos(float x, float& s, float& c) {
+    float y = x * ERD_F(1.27323954473516);
+    int32_t j = (int32_t)y;
+    j = (j + 1) & ~1;
+    y = (float)j;
+    const uint32_t sign_sin = (uint32_t)(j & 4) << 29;
+    const bool poly_mask = (j & 2) == 0;
+    x = x + y * ERD_F(-0.78515625);
+    x = x + y * ERD_F(-2.4187564849853515625e-4);
+    x = x + y * ERD_F(-3.77489497744594108e-8);
+    const uint32_t sign_cos = (uint32_t)(4 & ~(j - 2)) << 29;
+    const float z = x * x;
+    float yc = ERD_F(2.443315711809948E-005) * z;
+    yc = yc + ERD_F(-1.388731625493765E-003);
+    yc = yc * z;
+    yc = yc + ERD_F(4.166664568298827E-002);
+    yc = yc * z;
+    yc = yc * z;
+    yc = yc - z * ERD_F(0.5);
+    yc = yc + 1.0f;
+    float ys = ERD_F(-1.9515295891E-4) * z;
+    ys = ys + ERD_F(8.3321608736E-3);
+    ys = ys * z;
+    ys = ys + ERD_F(-1.6666654611E-1);
+    ys = ys * z;
+    ys = ys * x;
+    ys = ys + x;
+    s = as_f(as_u(poly_mask ? ys : yc) ^ sign_sin);
+    c = as_f(as_u(poly_mask ? yc : ys) ^ sign_cos);
 }
 
 // ------------------------------------------------------------ double math ---

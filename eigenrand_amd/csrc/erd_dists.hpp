@@ -162,7 +162,7 @@ struct NormalF32 {
             // -2 ln(u1) for u1 in [2^-23, 1] is +-0 or >= 2^-22
             const float radius = sqrt_cr_normal(-2.0f * plog_pos(u1));
             float s, c;
-            psincos(ERD_F(2 * kPi) * u2, s, c);
+            psincos_pos(ERD_F(2 * kPi) * u2, s, c);
             sn[h] = radius * s;
             cs[h] = radius * c;
         }
@@ -282,7 +282,7 @@ struct StudentTF32 {  // NormalExp.h:285-301 (sine from the reference's psincos)
             const float u2 = ur_f32(h ? hi32(w[1]) : lo32(w[1]));
             const float radius = sqrtf(n * (pexp(plog_pos(u1) * k) - 1.0f));
             float s, c;
-            psincos(ERD_F(2 * kPi) * u2, s, c);
+            psincos_pos(ERD_F(2 * kPi) * u2, s, c);
             v[h] = radius * s;
         }
         *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
@@ -412,7 +412,7 @@ struct StudentTVopF32 {
             const float u2 = ur_f32(h ? hi32(w[1]) : lo32(w[1]));
             const float radius = sqrtf(n * (pexp(plog(u1) * (-2.0f / n)) - 1.0f));
             float s, c;
-            psincos(ERD_F(2 * kPi) * u2, s, c);
+            psincos_pos(ERD_F(2 * kPi) * u2, s, c);
             v[h] = radius * s;
         }
         *reinterpret_cast<float2*>(out + i) = make_float2(v[0], v[1]);
@@ -567,7 +567,7 @@ struct GammaTangentW {  // NormalExp.h:580-603
     float beta, sq, am1;
     __device__ void eval(const uint64_t* w, int h, float& v, bool& acc, int) const {
         float ys, yc;
-        psincos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), ys, yc);
+        psincos_pos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), ys, yc);
         const float yx = ys / yc;
         const float xx = sq * yx + am1;
         const bool c = 0.0f < xx;
@@ -606,7 +606,7 @@ struct PoissonTangent {  // Discrete.h:875-902 (Packet4f via half())
     float psq, pmean, plm, pg1;
     __device__ void eval(const uint64_t* w, int h, float& v, bool& acc, int) const {
         float s, c;
-        psincos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), s, c);
+        psincos_pos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), s, c);
         const float yx = s / c;
         const float fres = truncf(psq * yx + pmean);
         const float p1 = (yx * yx + 1.0f) * ERD_F(0.9);
@@ -631,7 +631,7 @@ struct NegBinTangent {
     const float* g1;
     __device__ void eval(const uint64_t* w, int h, float& v, bool& acc, int ln) const {
         float s, c;
-        psincos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), s, c);
+        psincos_pos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), s, c);
         const float yx = s / c;
         const float fres = truncf(sq[ln] * yx + mean[ln]);
         const float p1 = (yx * yx + 1.0f) * ERD_F(0.9);
@@ -652,7 +652,7 @@ struct BinomialTangent {  // Discrete.h:1011-1051
     int flip;  // p != small_p: result is trials - k
     __device__ void eval(const uint64_t* w, int h, float& v, bool& acc, int) const {
         float s, c;
-        psincos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), s, c);
+        psincos_pos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), s, c);
         const float ys = s / c;
         const float fres = truncf(psqrt_v * ys + pmean);
         const float p1 = (ERD_F(1.2) * psqrt_v) * (1.0f + ys * ys);

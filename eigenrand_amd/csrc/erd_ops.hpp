@@ -964,7 +964,7 @@ __device__ void op_binomial_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t
                 bool fv = false;
                 if (i < ni) {
                     float s, c;
-                    psincos(ERD_F(kPi) * ur_f32(cur.word32(2 * i, j)), s, c);
+                    psincos_pos(ERD_F(kPi) * ur_f32(cur.word32(2 * i, j)), s, c);
                     const float ys = s / c;
                     fres = truncf(psq * ys + pmean);
                     const float pp1 = (ERD_F(1.2) * psq) * (1.0f + ys * ys);
@@ -1073,7 +1073,7 @@ __device__ void op_gamma_vop(Eng& e, uint64_t* carry, float* res_lds, float* o, 
                         acc[h] = ur_f32(h ? hi32(w2) : lo32(w2)) < (c ? q1 : q2);
                     } else {
                         float ys, yc;
-                        psincos(ERD_F(kPi) * u1, ys, yc);
+                        psincos_pos(ERD_F(kPi) * u1, ys, yc);
                         const float yx = ys / yc;
                         xv[h] = sq[h] * yx + am1[h];
                         const float ub = (yx * yx + 1.0f) * pexp(am1[h] * plog(xv[h] / am1[h]) - yx * sq[h]);
@@ -1171,7 +1171,7 @@ __device__ void op_poisson_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t*
                 bool fv = false;
                 if (i < ni) {
                     float s, c;
-                    psincos(ERD_F(kPi) * ur_f32(cur.word32(2 * i, j)), s, c);
+                    psincos_pos(ERD_F(kPi) * ur_f32(cur.word32(2 * i, j)), s, c);
                     const float yx = s / c;
                     fres = truncf(psq * yx + m);
                     const float p1 = (yx * yx + 1.0f) * ERD_F(0.9);
@@ -1402,7 +1402,7 @@ __device__ void op_binomial_row8(Eng& e, uint64_t* carry, float* res_lds, int32_
                 bool fv = false;
                 if (i < ni) {
                     float s, c;
-                    psincos(ERD_F(kPi) * ur_f32(cur.word32(2 * i, j)), s, c);
+                    psincos_pos(ERD_F(kPi) * ur_f32(cur.word32(2 * i, j)), s, c);
                     const float ys = s / c;
                     fres = truncf(psq * ys + pmean);
                     const float pp1 = (ERD_F(1.2) * psq) * (1.0f + ys * ys);
