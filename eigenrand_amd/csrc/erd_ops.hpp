@@ -922,21 +922,36 @@ __device__ void op_binomial_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t
     const int i = lane >> 2, j = lane & 3;
     HalfCursor cur = cursor_begin(e, carry, lane);
     const uint64_t jmask = 0x1111111111111111ull << j;
+    int32_t g_n = 0;
+    float g_p = 0.f, g_pg1 = 0.f, g_psq = 0.f, g_plp = 0.f, g_plq = 0.f, g_nem = 0.f;
     for (int64_t pk = 0; pk < T; ++pk) {
-        const int32_t n = trials[4 * pk + j];
-        const float p = prob[4 * pk + j];
+        if ((pk & 15) == 0) {
+            // constants of packets pk..pk+15: thread t holds packet pk + t/4, lane t%4
+            const bool in = pk + i < T;
+            g_n = in ? trials[4 * pk + lane] : 1;
+            g_p = in ? prob[4 * pk + lane] : 0.5f;
+            const float gs = (g_p < 1.0f - g_p) ? g_p : 1.0f - g_p;
+            const float gtr = (float)g_n;
+            const float gmean = gtr * gs;
+            g_pg1 = plgamma_approx(gtr + 1.0f);
+            g_psq = sqrtf((2.0f * gmean) * (1.0f - gs));
+            g_plp = plog(gs);
+            g_plq = plog(1.0f - gs);
+            g_nem = pexp(-gmean);
+        }
+        const int src = 4 * (int)(pk & 15) + j;
+        const int32_t n = __shfl(g_n, src);
+        const float p = __shfl(g_p, src);
         const float psmall = (p < 1.0f - p) ? p : 1.0f - p;  // pmin(p, 1-p)
         const float ptr = (float)n;
         const float pmean = ptr * psmall;
-        const float pg1 = plgamma_approx(ptr + 1.0f);
-        const float psq = sqrtf((2.0f * pmean) * (1.0f - psmall));
-        const float plp = plog(psmall);
-        const float plq = plog(1.0f - psmall);
+        const float pg1 = __shfl(g_pg1, src), psq = __shfl(g_psq, src), plp = __shfl(g_plp, src),
+                    plq = __shfl(g_plq, src);
         const bool valid = pmean < 1.0f;
         float valid_res = 0.0f;
         const uint64_t V = __ballot(valid) & 0xFull;
         if (V) {
-            const float nem = pexp(-pmean);
+            const float nem = __shfl(g_nem, src);
             float val = 1.0f;
             int32_t res = 0;
             for (int it = 0;; ++it) {
@@ -1022,13 +1037,17 @@ __device__ void op_binomial_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t
 
 // Gamma, Packet8f: three full draws per iteration (ux, vx, u3); lanes with
 // alpha <= 1 run the GS step (NormalExp.h:536-558), others the tangent step
-// (580-603) on the first two draws. Thread 4i+l runs iteration i for float
-// lanes 2l and 2l+1.
-__device__ void op_gamma_vop(Eng& e, uint64_t* carry, float* res_lds, float* o, int64_t per,
-                             const float* alpha, const float* beta, uint32_t* status, int lane) {
+// (580-603) on the first two draws. Thread 8i+j evaluates iteration i of float
+// lane j (u64 lane j/2, half j%2), eight iterations per wave step: one
+// candidate per thread, and a lane's first accepted value reaches the thread
+// that stores it through one bpermute. The per-lane constants are computed for
+// eight packets at once (one packet-lane per thread) and fetched by bpermute.
+__device__ void op_gamma_vop(Eng& e, uint64_t* carry, float* o, int64_t per, const float* alpha,
+                             const float* beta, uint32_t* status, int lane) {
+    constexpr uint64_t kLaneBits = 0x0101010101010101ull;  // threads of float lane 0
     const int64_t T = per / 8;
     if (T == 0) return;
-    const int i = lane >> 2, l = lane & 3;
+    const int i = lane >> 3, j = lane & 7, l = j >> 1, h = j & 1;
     Cursor<4, 2> cur;
     cur.mt = e.mt;
     cur.carry = carry;
@@ -1036,82 +1055,76 @@ __device__ void op_gamma_vop(Eng& e, uint64_t* carry, float* res_lds, float* o, 
     cur.nc = 0;
     cur.used = false;
     cur.twisted = &e.twisted;
+    float g_al = 0.f, g_be = 0.f, g_px = 0.f, g_inv = 0.f, g_sq = 0.f;
     for (int64_t pk = 0; pk < T; ++pk) {
-        float al[2], be[2], px[2], inv[2], am1[2], sq[2], res[2] = {0.f, 0.f};
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            al[h] = alpha[8 * pk + 2 * l + h];
-            be[h] = beta[8 * pk + 2 * l + h];
-            px[h] = ERD_F(kE) / (al[h] + ERD_F(kE));
-            inv[h] = 1.0f / al[h];
-            am1[h] = al[h] - 1.0f;
-            sq[h] = sqrtf(2.0f * al[h] - 1.0f);
+        if ((pk & 7) == 0) {
+            // constants of packets pk..pk+7: thread t holds packet pk + t/8, lane t%8
+            const bool in = pk + i < T;
+            g_al = in ? alpha[8 * pk + lane] : 1.0f;
+            g_be = in ? beta[8 * pk + lane] : 1.0f;
+            g_px = ERD_F(kE) / (g_al + ERD_F(kE));
+            g_inv = 1.0f / g_al;
+            g_sq = sqrtf(2.0f * g_al - 1.0f);
         }
-        uint32_t done = 0;  // bit j = float lane j has its value
-        for (int it = 0;; it += 16) {
+        const int src = 8 * (int)(pk & 7) + j;
+        const float al = __shfl(g_al, src), be = __shfl(g_be, src), px = __shfl(g_px, src),
+                    inv = __shfl(g_inv, src), sq = __shfl(g_sq, src);
+        const float am1 = al - 1.0f;
+        const bool gs = al <= 1.0f;
+        float res = 0.0f;
+        uint32_t done = 0;  // bit q = float lane q has its value
+        for (int it = 0;; it += 8) {
             cur.ensure(3, lane);
             const int ni_avail = cur.avail() / 3;
-            const int ni = ni_avail < 16 ? ni_avail : 16;
-            float xv[2] = {0.f, 0.f};
-            bool acc[2] = {false, false};
+            const int ni = ni_avail < 8 ? ni_avail : 8;
+            float xv = 0.f;
+            bool acc = false;
             if (i < ni) {
                 const uint64_t w0 = cur.word(3 * i, l), w1 = cur.word(3 * i + 1, l),
                                w2 = cur.word(3 * i + 2, l);
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const float u1 = ur_f32(h ? hi32(w0) : lo32(w0));
-                    const float u2 = ur_f32(h ? hi32(w1) : lo32(w1));
-                    if (al[h] <= 1.0f) {
-                        const float vx = u2 + 1.1920928955078125e-07f / 8;
-                        const float lv = plog_pos(vx);
-                        const float x1 = pexp(inv[h] * lv);
-                        const float q1 = pexp(-x1);
-                        const float x2 = 1.0f - lv;
-                        const float q2 = pexp(plog_pos(x2) * am1[h]);
-                        const bool c = u1 < px[h];
-                        xv[h] = c ? x1 : x2;
-                        acc[h] = ur_f32(h ? hi32(w2) : lo32(w2)) < (c ? q1 : q2);
-                    } else {
-                        float ys, yc;
-                        psincos_pos(ERD_F(kPi) * u1, ys, yc);
-                        const float yx = ys / yc;
-                        xv[h] = sq[h] * yx + am1[h];
-                        const float ub = (yx * yx + 1.0f) * pexp(am1[h] * plog(xv[h] / am1[h]) - yx * sq[h]);
-                        acc[h] = (0.0f < xv[h]) && (u2 <= ub);
-                    }
+                const float u1 = ur_f32(h ? hi32(w0) : lo32(w0));
+                const float u2 = ur_f32(h ? hi32(w1) : lo32(w1));
+                if (gs) {
+                    const float vx = u2 + 1.1920928955078125e-07f / 8;
+                    const float lv = plog_pos(vx);
+                    const float x1 = pexp(inv * lv);
+                    const float q1 = pexp(-x1);
+                    const float x2 = 1.0f - lv;
+                    const float q2 = pexp(plog_pos(x2) * am1);
+                    const bool c = u1 < px;
+                    xv = c ? x1 : x2;
+                    acc = ur_f32(h ? hi32(w2) : lo32(w2)) < (c ? q1 : q2);
+                } else {
+                    float ys, yc;
+                    psincos_pos(ERD_F(kPi) * u1, ys, yc);
+                    const float yx = ys / yc;
+                    xv = sq * yx + am1;
+                    const float ub = (yx * yx + 1.0f) * pexp(am1 * plog(xv / am1) - yx * sq);
+                    acc = (0.0f < xv) && (u2 <= ub);
                 }
             }
-            const uint64_t A0 = __ballot(acc[0]), A1 = __ballot(acc[1]);
+            const uint64_t A = __ballot(acc);
             int need = 0;
             bool all = true;
+            uint32_t newly = 0;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                if (done & (1u << j)) continue;
-                const uint64_t m = ((j & 1) ? A1 : A0) & (0x1111111111111111ull << (j >> 1));
+            for (int q = 0; q < 8; ++q) {
+                if (done & (1u << q)) continue;
+                const uint64_t m = A & (kLaneBits << q);
                 if (m) {
-                    const int first = (__ffsll((long long)m) - 1) >> 2;
+                    const int first = (__ffsll((long long)m) - 1) >> 3;
                     need = need > first + 1 ? need : first + 1;
+                    newly |= 1u << q;
                 } else {
                     all = false;
                 }
             }
             if (!all) need = ni;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int j = 2 * l + h;
-                const uint64_t m = (h ? A1 : A0) & (0x1111111111111111ull << l);
-                if (!(done & (1u << j)) && m && i == ((__ffsll((long long)m) - 1) >> 2))
-                    res_lds[j] = be[h] * xv[h];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const uint64_t m = ((j & 1) ? A1 : A0) & (0x1111111111111111ull << (j >> 1));
-                if (m) done |= 1u << j;
-            }
-#pragma unroll
-            for (int h = 0; h < 2; ++h) res[h] = res_lds[2 * l + h];
-            __syncthreads();
+            const uint64_t mj = A & (kLaneBits << j);
+            const int src = mj ? (int)(__ffsll((long long)mj) - 1) : lane;
+            const float got = __shfl(be * xv, src);
+            if (newly & (1u << j)) res = got;
+            done |= newly;
             cur.advance(3 * need);
             if (all) break;
             if (it > kLockstepCap) {
@@ -1119,7 +1132,7 @@ __device__ void op_gamma_vop(Eng& e, uint64_t* carry, float* res_lds, float* o, 
                 break;
             }
         }
-        if (i == 0) *reinterpret_cast<float2*>(o + 8 * pk + 2 * l) = make_float2(res[0], res[1]);
+        if (i == 0) o[8 * pk + j] = res;
     }
     __syncthreads();
     e.idx = cur.pos;
@@ -1127,6 +1140,17 @@ __device__ void op_gamma_vop(Eng& e, uint64_t* carry, float* res_lds, float* o, 
 
 // Poisson, Packet4f via half draws: Knuth lockstep for lanes with mean < 12,
 // then tangent lockstep for the others with float constants.
+//
+// The Knuth loop multiplies each small lane's running product by its uniform
+// once per draw and stops at the first draw where no small lane's product
+// still exceeds e^-mean; a lane's count is the number of draws before its own
+// product fell to e^-mean. The products never increase (u < 1), so a lane's
+// count is its own first stopping draw, and the packet consumes the largest
+// of them plus one. Each lane's product chain therefore runs over a batch of
+// up to 4 preloaded draws in its own thread, with one ballot per batch (most
+// chains end within a batch: small means are mostly well below 1). The
+// per-lane constants are computed for 16 packets at once (one packet-lane per
+// thread) and fetched by bpermute; op_binomial_vop does the same.
 __device__ void op_poisson_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t* o, int64_t per,
                                const float* mean, uint32_t* status, int lane) {
     const int64_t T = per / 4;
@@ -1134,32 +1158,58 @@ __device__ void op_poisson_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t*
     const int i = lane >> 2, j = lane & 3;
     HalfCursor cur = cursor_begin(e, carry, lane);
     const uint64_t jmask = 0x1111111111111111ull << j;
+    float g_m = 0.f, g_nem = 0.f, g_psq = 0.f, g_plm = 0.f, g_pg1 = 0.f;
     for (int64_t pk = 0; pk < T; ++pk) {
-        const float m = mean[4 * pk + j];
+        if ((pk & 15) == 0) {
+            // constants of packets pk..pk+15: thread t holds packet pk + t/4, lane t%4
+            g_m = pk + i < T ? mean[4 * pk + lane] : 1.0f;
+            g_nem = pexp(-g_m);
+            g_psq = sqrtf(2.0f * g_m);
+            g_plm = plog(g_m);
+            g_pg1 = g_m * g_plm - plgamma_approx(g_m + 1.0f);
+        }
+        const int src = 4 * (int)(pk & 15) + j;
+        const float m = __shfl(g_m, src), nem = __shfl(g_nem, src), psq = __shfl(g_psq, src),
+                    plm = __shfl(g_plm, src), pg1 = __shfl(g_pg1, src);
         const bool small = m < 12.0f;
-        const float nem = pexp(-m);
-        const float psq = sqrtf(2.0f * m);
-        const float plm = plog(m);
-        const float pg1 = m * plm - plgamma_approx(m + 1.0f);
         float res = 0.0f;
         const uint64_t V = __ballot(small) & 0xFull;
         if (V) {
             float val = 1.0f;
-            int32_t cnt = 0;
-            for (int it = 0;; ++it) {
+            int32_t first = small ? -1 : 0;  // draws before this lane stopped
+            int32_t base = 0;                // draws consumed before the batch
+            for (;;) {
                 cur.ensure(1, lane);
-                const float u = ur_f32(cur.word32(0, j));
-                cur.advance(1);
-                val = val * u;
-                const bool c = (nem < val) && small;
-                if (!(__ballot(c) & 0xFull)) break;
-                cnt += c ? 1 : 0;
-                if (it > kLockstepCap) {
+                const int B = cur.avail() < 4 ? cur.avail() : 4;
+                uint32_t wv[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) wv[t] = t < B ? cur.word32(t, j) : 0u;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    if (first < 0 && t < B) {
+                        val = val * ur_f32(wv[t]);
+                        if (!(nem < val)) first = base + t;
+                    }
+                }
+                if (!(__ballot(first < 0) & 0xFull)) {
+                    int32_t last = __builtin_amdgcn_readlane(first, 0);
+#pragma unroll
+                    for (int q = 1; q < 4; ++q) {
+                        const int32_t f = __builtin_amdgcn_readlane(first, q);
+                        last = last > f ? last : f;
+                    }
+                    cur.advance(last + 1 - base);
+                    break;
+                }
+                cur.advance(B);
+                base += B;
+                if (base > kLockstepCap) {
                     if (lane == 0) atomicOr(status, kStatusIterCap);
+                    first = first < 0 ? base : first;
                     break;
                 }
             }
-            res = (float)cnt;
+            res = (float)first;
         }
         if (V != 0xFull) {
             uint64_t done = V;

@@ -99,13 +99,12 @@ __global__ __launch_bounds__(64) void k_gamma_vop(EngineState* __restrict__ stat
                                                   uint32_t* __restrict__ status) {
     __shared__ uint64_t mt[kWords];
     __shared__ uint64_t carry[2 * kLanes];
-    __shared__ float res_lds[8];
     const int lane = threadIdx.x;
     const int64_t g = blockIdx.x;
     if (per / 8 == 0) return;
     Eng e;
     eng_load(e, mt, states + g, lane);
-    op_gamma_vop(e, carry, res_lds, out + g * per, per, alpha + g * per, beta + g * per, status, lane);
+    op_gamma_vop(e, carry, out + g * per, per, alpha + g * per, beta + g * per, status, lane);
     eng_store(e, states + g, lane);
 }
 

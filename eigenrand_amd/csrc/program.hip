@@ -307,7 +307,7 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
                             (const float*)sg.in1, status, lane);
             break;
         case SEG_GAMMA_VOP_F32:
-            op_gamma_vop(e, s.vop.carry, s.vop.res, (float*)sg.out, n, (const float*)sg.in0,
+            op_gamma_vop(e, s.vop.carry, (float*)sg.out, n, (const float*)sg.in0,
                          (const float*)sg.in1, status, lane);
             break;
         case SEG_POISSON_VOP_I32:

@@ -40,6 +40,24 @@ def test_poisson_vop_matches_oracle(er, orc):
         np.testing.assert_array_equal(urng.snapshot(), ref.raw)
 
 
+def test_poisson_vop_long_knuth_chains(er, orc):
+    """Means in [5, 12): every lane runs the Knuth product chain, often over
+    more than one 16-draw batch and across twists; edge means 0.01, 11.99, 12."""
+    import torch
+    rng = np.random.default_rng(81)
+    per = 4096
+    n = len(SEEDS) * per
+    mean = rng.uniform(5.0, 12.0, n).astype(np.float32)
+    mean[::53] = np.float32(11.99)
+    mean[::71] = np.float32(12.0)
+    mean[::89] = np.float32(0.01)
+    urng = er.Vmt19937_64(seeds=SEEDS)
+    ref = orc.Engines(seeds=SEEDS)
+    out = er.poisson_vop(urng, torch.from_numpy(mean).cuda())
+    assert_same(out.cpu().numpy(), orc.fill_poisson_vop_i32(ref, mean))
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+
+
 def test_requests_equal_sequential_fills(er, orc):
     """A mixed batch on 3 engines equals the same fills issued one by one."""
     import torch
