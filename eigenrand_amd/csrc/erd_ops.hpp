@@ -211,13 +211,20 @@ __device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t p
 #pragma unroll
                 for (int h = 0; h < 2; ++h) pol.eval(w, h, v[h], a[h], 2 * p + h);
             }
-            // Queue bookkeeping over the candidates that count (`lv`).
-            const uint64_t cmask = (IPC == 4 ? 0xFull : 0x3ull) << (cl * IPC);
-            const uint64_t below_c = lanemask_lt(cl * IPC), below = lanemask_lt(lane);
+            // Queue bookkeeping over the candidates that count (`lv`), in
+            // 32-bit offsets from the step's bases: queue item fifo + x is item
+            // r0 + x of queued packet m0 + (r0 + x) / W (r0 = fifo % W), and
+            // packet index full_cnt + m0 + e is local packet e.
+            const int cb = cl * IPC;  // the candidate's first thread
+            const uint64_t cmask = (IPC == 4 ? 0xFull : 0x3ull) << cb;
+            const int64_t m0 = fifo / W;
+            const int r0 = (int)(fifo - m0 * W);
+            const int64_t pbase = full_cnt + m0;
+            const int64_t prem = T - pbase;  // local packets < prem exist
+            const int lim = prem < 0x40000000 ? (int)prem : 0x40000000;
             bool full, q0, q1;
             uint64_t A0, A1, FC;
-            int64_t fb, fstart, f0, f1, fifo_end, m_end, emit;
-            const int64_t m0 = fifo / W;
+            int fstart, f0, tot, emit;
             auto plan = [&](bool lv) {
                 const uint64_t B0 = __ballot(lv && a[0]), B1 = __ballot(lv && a[1]);
                 full = lv && ((B0 & B1 & cmask) == cmask);
@@ -226,32 +233,39 @@ __device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t p
                 A0 = __ballot(q0);
                 A1 = __ballot(q1);
                 FC = __ballot(full && p == 0);
-                fb = full_cnt + __popcll(FC & below_c);
-                fstart = fifo + __popcll(A0 & below_c) + __popcll(A1 & below_c);
-                f0 = fifo + __popcll(A0 & below) + __popcll(A1 & below);
-                f1 = f0 + (q0 ? 1 : 0);
-                fifo_end = fifo + __popcll(A0) + __popcll(A1);
-                m_end = fifo_end / W;
-                const int acnt = __popcll(A0 & cmask) + __popcll(A1 & cmask);
+                // set bits of the ballots in threads below this one; the
+                // candidate's first thread has the counts before the candidate
+                // (quad_perm broadcast of its lane)
+                f0 = r0 + (int)mbcnt64(A1, mbcnt64(A0, 0));
+                const int fbt = (int)mbcnt64(FC, 0);
+                constexpr int kFirst = IPC == 4 ? 0x00 : 0xA0;  // quad_perm 0,0,0,0 / 0,0,2,2
+                fstart = __builtin_amdgcn_mov_dpp(f0, kFirst, 0xF, 0xF, false);
+                const int fb = __builtin_amdgcn_mov_dpp(fbt, kFirst, 0xF, 0xF, false);
+                tot = __popcll(A0) + __popcll(A1);
+                int acnt = (q0 ? 1 : 0) + (q1 ? 1 : 0);  // summed over the candidate's threads
+                acnt += __builtin_amdgcn_mov_dpp(acnt, 0xB1, 0xF, 0xF, false);       // quad_perm 1,0,3,2
+                if (IPC == 4) acnt += __builtin_amdgcn_mov_dpp(acnt, 0x4E, 0xF, 0xF, false);  // 2,3,0,1
                 emit = -1;
-                if (lv && (full || (acnt > 0 && (fstart + acnt) / W > fstart / W))) emit = fb + fstart / W;
+                if (lv && (full || (acnt > 0 && (unsigned)(fstart + acnt) / W > (unsigned)fstart / W)))
+                    emit = fb + (int)((unsigned)fstart / W);
             };
             plan(live);
-            const uint64_t LAST = __ballot(emit == T - 1 && p == 0);
+            const uint64_t LAST = __ballot(emit == lim - 1 && p == 0);
             if (LAST) {
                 // Candidates after the one emitting the last packet are never
                 // drawn: drop them so that the queue ends as the reference's.
                 const int clast = (int)(__ffsll((long long)LAST) - 1) / IPC;
                 plan(live && cl <= clast);
             }
-            if (!full && emit >= 0 && p == 0) s.tab[fstart / W - m0] = (int32_t)emit;
+            const int m_end = (r0 + tot) / W;  // queued packets completed in this step
+            if (!full && emit >= 0 && p == 0) s.tab[(unsigned)fstart / W] = (int32_t)(pbase + emit);
             __syncthreads();
-            if (full && emit < T) {
+            if (full && emit < lim) {
+                const int64_t at = (pbase + emit) * W + 2 * p;
 #pragma unroll
-                for (int h = 0; h < 2; ++h) pol.store(o, emit * W + 2 * p + h, v[h]);
+                for (int h = 0; h < 2; ++h) pol.store(o, at + h, v[h]);
             }
-            const int pend = (int)(fifo % W);
-            if (m_end > m0 && lane < pend) {
+            if (m_end > 0 && lane < r0) {
                 const int64_t ex = s.tab[0];
                 if (ex < T) pol.store(o, ex * W + lane, s.rest[lane]);
             }
@@ -260,11 +274,11 @@ __device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t p
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const bool q = h ? q1 : q0;
-                const int64_t f = h ? f1 : f0;
+                const unsigned f = (unsigned)(f0 + (h ? (q0 ? 1 : 0) : 0));
                 if (!q) continue;
-                const int64_t m = f / W;
+                const int m = (int)(f / W);
                 if (m < m_end) {
-                    const int64_t ex = s.tab[m - m0];
+                    const int64_t ex = s.tab[m];
                     if (ex < T) pol.store(o, ex * W + (f % W), v[h]);
                 } else {
                     park[h] = v[h];
@@ -276,13 +290,13 @@ __device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t p
             for (int h = 0; h < 2; ++h)
                 if (park_at[h] >= 0) s.rest[park_at[h]] = park[h];
             full_cnt += __popcll(FC);
-            fifo = fifo_end;
+            fifo += tot;
             if (LAST) {
                 const int clast = (int)(__ffsll((long long)LAST) - 1) / IPC;
                 end_v = (c0 + clast + 1) * K;
                 break;
             }
-            if (__popcll(FC) + (m_end - m0) > 0) {
+            if (__popcll(FC) + m_end > 0) {
                 since_emit = 0;
             } else {
                 since_emit += CPS;

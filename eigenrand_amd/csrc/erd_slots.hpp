@@ -56,5 +56,10 @@ __device__ __forceinline__ void carry_tail(const uint64_t* mt, uint64_t* carry, 
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
+// acc + the number of set bits of m in lanes below the calling thread
+// (v_mbcnt_lo/hi).
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t m, uint32_t acc) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, acc));
+}
 
 }  // namespace erd
