@@ -69,6 +69,9 @@ int erd_engines_create(erd_engines** out, const uint64_t* seeds, int64_t n, void
         err = hipMemcpyAsync(d_seeds, seeds, sizeof(uint64_t) * (size_t)n, hipMemcpyHostToDevice, S(stream));
     if (err == hipSuccess) err = erd::launch_seed(e->states, d_seeds, n, S(stream));
     if (err == hipSuccess) err = hipStreamSynchronize(S(stream));
+    // the tangent samplers' plgamma table of this device, built here (outside
+    // any stream capture) rather than on the first fill
+    if (err == hipSuccess && !erd::lgamma_table(S(stream))) err = hipErrorOutOfMemory;
     if (d_seeds) (void)hipFree(d_seeds);
     if (err != hipSuccess) {
         if (e->states) (void)hipFree(e->states);

@@ -469,6 +469,17 @@ __device__ __forceinline__ float plgamma_approx(float x) {
     return ret;
 }
 
+// plgamma_approx at integer-valued arguments. The library keeps, per device,
+// a table of plgamma_approx(k) for k = 1..kLgTab computed by plgamma_approx
+// itself (so the values are identical); other arguments are computed. The
+// tangent samplers evaluate it only at fres + 1 and trials + 1 - fres, which
+// are integer-valued (fres = trunc(...)) or not finite.
+constexpr int kLgTab = 1 << 20;
+__device__ __forceinline__ float plgamma_int(const float* lg, float x) {
+    if (x >= 1.0f && x <= (float)kLgTab) return lg[(int)x - 1];
+    return plgamma_approx(x);
+}
+
 __device__ __forceinline__ uint32_t lo32(uint64_t w) { return (uint32_t)w; }
 __device__ __forceinline__ uint32_t hi32(uint64_t w) { return (uint32_t)(w >> 32); }
 

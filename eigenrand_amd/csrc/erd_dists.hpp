@@ -600,18 +600,23 @@ struct BetaJohnk {
     __device__ void store(Out* o, int64_t i, float v) const { o[i] = ratio != 0.0f ? (ratio * v) / (1.0f - v) : v; }
 };
 
+// The tangent policies take plgamma_approx of integer arguments from the
+// table `lg` (plgamma_int) and skip it where the candidate fails its range
+// test: the accept test is false there whatever p2 is.
 struct PoissonTangent {  // Discrete.h:875-902 (Packet4f via half())
     static constexpr int W = 4, SW = 2, K = 2;
     using Out = int32_t;
     float psq, pmean, plm, pg1;
+    const float* lg;
     __device__ void eval(const uint64_t* w, int h, float& v, bool& acc, int) const {
         float s, c;
         psincos_pos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), s, c);
         const float yx = s / c;
         const float fres = truncf(psq * yx + pmean);
         const float p1 = (yx * yx + 1.0f) * ERD_F(0.9);
-        const float p2 = pexp((fres * plm - plgamma_approx(fres + 1.0f)) - pg1);
         const bool c1 = 0.0f <= fres;
+        const float lgf = c1 ? plgamma_int(lg, fres + 1.0f) : 0.0f;
+        const float p2 = pexp((fres * plm - lgf) - pg1);
         const float u2 = ur_f32(h ? hi32(w[1]) : lo32(w[1]));
         acc = c1 && (u2 <= p1 * p2);
         v = fres;
@@ -629,14 +634,16 @@ struct NegBinTangent {
     const float* sq;
     const float* lm;
     const float* g1;
+    const float* lg;
     __device__ void eval(const uint64_t* w, int h, float& v, bool& acc, int ln) const {
         float s, c;
         psincos_pos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), s, c);
         const float yx = s / c;
         const float fres = truncf(sq[ln] * yx + mean[ln]);
         const float p1 = (yx * yx + 1.0f) * ERD_F(0.9);
-        const float p2 = pexp((fres * lm[ln] - plgamma_approx(fres + 1.0f)) - g1[ln]);
         const bool c1 = 0.0f <= fres;
+        const float lgf = c1 ? plgamma_int(lg, fres + 1.0f) : 0.0f;
+        const float p2 = pexp((fres * lm[ln] - lgf) - g1[ln]);
         const float u2 = ur_f32(h ? hi32(w[1]) : lo32(w[1]));
         acc = c1 && (u2 <= p1 * p2);
         v = fres;
@@ -650,17 +657,20 @@ struct BinomialTangent {  // Discrete.h:1011-1051
     float ptrials, psqrt_v, pmean, plp, plq, pg1;
     int32_t trials;
     int flip;  // p != small_p: result is trials - k
+    const float* lg;
     __device__ void eval(const uint64_t* w, int h, float& v, bool& acc, int) const {
         float s, c;
         psincos_pos(ERD_F(kPi) * ur_f32(h ? hi32(w[0]) : lo32(w[0])), s, c);
         const float ys = s / c;
         const float fres = truncf(psqrt_v * ys + pmean);
         const float p1 = (ERD_F(1.2) * psqrt_v) * (1.0f + ys * ys);
-        const float p2 = pexp((((pg1 - plgamma_approx(fres + 1.0f)) -
-                                plgamma_approx((ptrials + 1.0f) - fres)) +
-                               fres * plp) +
-                              (ptrials - fres) * plq);
         const bool c1 = (0.0f <= fres) && (fres <= ptrials);
+        float lg1 = 0.0f, lg2 = 0.0f;
+        if (c1) {
+            lg1 = plgamma_int(lg, fres + 1.0f);
+            lg2 = plgamma_int(lg, (ptrials + 1.0f) - fres);
+        }
+        const float p2 = pexp((((pg1 - lg1) - lg2) + fres * plp) + (ptrials - fres) * plq);
         const float u2 = ur_f32(h ? hi32(w[1]) : lo32(w[1]));
         acc = c1 && (u2 <= p1 * p2);
         v = fres;

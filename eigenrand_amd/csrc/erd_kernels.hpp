@@ -125,6 +125,10 @@ hipError_t launch_fill_poisson_vop_i32(EngineState* st, int64_t n, int32_t* out,
                                        const float* mean, uint32_t* status, hipStream_t s);
 
 // program.hip (segment programs)
+// Device table of plgamma_approx(k), k = 1..kLgTab, for the current device:
+// built on first use (one kernel on `s`, then a synchronisation of `s`) and
+// kept until the process ends. NULL if it cannot be allocated.
+const float* lgamma_table(hipStream_t s);
 hipError_t launch_program(EngineState* st, int32_t n_groups, const int32_t* d_engine_of,
                           const int32_t* d_begin, const Segment* d_segs, uint32_t* status,
                           hipStream_t s);

@@ -930,7 +930,7 @@ __device__ void op_binomial_small(Eng& e, uint64_t* carry, int32_t* o, int64_t p
 // iterations up to the slowest lane, as the sequential loop does.
 __device__ void op_binomial_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t* o, int64_t per,
                                 const int32_t* trials, const float* prob, uint32_t* status,
-                                int lane) {
+                                const float* lg, int lane) {
     const int64_t T = per / 4;
     if (T == 0) return;
     const int i = lane >> 2, j = lane & 3;
@@ -997,11 +997,13 @@ __device__ void op_binomial_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t
                     const float ys = s / c;
                     fres = truncf(psq * ys + pmean);
                     const float pp1 = (ERD_F(1.2) * psq) * (1.0f + ys * ys);
-                    const float pp2 = pexp((((pg1 - plgamma_approx(fres + 1.0f)) -
-                                             plgamma_approx((ptr + 1.0f) - fres)) +
-                                            fres * plp) +
-                                           (ptr - fres) * plq);
                     const bool c1 = (0.0f <= fres) && (fres <= ptr);
+                    float lg1 = 0.0f, lg2 = 0.0f;  // unused when the range test fails
+                    if (c1) {
+                        lg1 = plgamma_int(lg, fres + 1.0f);
+                        lg2 = plgamma_int(lg, (ptr + 1.0f) - fres);
+                    }
+                    const float pp2 = pexp((((pg1 - lg1) - lg2) + fres * plp) + (ptr - fres) * plq);
                     const float u2 = ur_f32(cur.word32(2 * i + 1, j));
                     fv = c1 && (u2 <= pp1 * pp2);
                 }
@@ -1166,7 +1168,7 @@ __device__ void op_gamma_vop(Eng& e, uint64_t* carry, float* o, int64_t per, con
 // per-lane constants are computed for 16 packets at once (one packet-lane per
 // thread) and fetched by bpermute; op_binomial_vop does the same.
 __device__ void op_poisson_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t* o, int64_t per,
-                               const float* mean, uint32_t* status, int lane) {
+                               const float* mean, uint32_t* status, const float* lg, int lane) {
     const int64_t T = per / 4;
     if (T == 0) return;
     const int i = lane >> 2, j = lane & 3;
@@ -1239,9 +1241,11 @@ __device__ void op_poisson_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t*
                     const float yx = s / c;
                     fres = truncf(psq * yx + m);
                     const float p1 = (yx * yx + 1.0f) * ERD_F(0.9);
-                    const float p2 = pexp((fres * plm - plgamma_approx(fres + 1.0f)) - pg1);
+                    const bool c1 = 0.0f <= fres;
+                    const float lgf = c1 ? plgamma_int(lg, fres + 1.0f) : 0.0f;  // unused if !c1
+                    const float p2 = pexp((fres * plm - lgf) - pg1);
                     const float u2 = ur_f32(cur.word32(2 * i + 1, j));
-                    fv = (0.0f <= fres) && (u2 <= p1 * p2);
+                    fv = c1 && (u2 <= p1 * p2);
                 }
                 const uint64_t A = __ballot(fv);
                 int need = 0;
@@ -1414,7 +1418,8 @@ __device__ void op_multinomial_small(Eng& e, int32_t* out, int64_t n, int32_t T,
 // result t goes to out[i*stride] while r[i] -= t. Full draws; thread 8i+j
 // runs iteration i of lane j (8 iterations per step).
 __device__ void op_binomial_row8(Eng& e, uint64_t* carry, float* res_lds, int32_t* out, int64_t stride,
-                                 int32_t* r, int32_t t0, int64_t n, float p, uint32_t* status, int lane) {
+                                 int32_t* r, int32_t t0, int64_t n, float p, uint32_t* status,
+                                 const float* lg, int lane) {
     const int64_t T = n / 8;
     if (T == 0) return;
     const int i = lane >> 3, j = lane & 7;
@@ -1470,11 +1475,13 @@ __device__ void op_binomial_row8(Eng& e, uint64_t* carry, float* res_lds, int32_
                     const float ys = s / c;
                     fres = truncf(psq * ys + pmean);
                     const float pp1 = (ERD_F(1.2) * psq) * (1.0f + ys * ys);
-                    const float pp2 = pexp((((pg1 - plgamma_approx(fres + 1.0f)) -
-                                             plgamma_approx((ptr + 1.0f) - fres)) +
-                                            fres * plp) +
-                                           (ptr - fres) * plq);
                     const bool c1 = (0.0f <= fres) && (fres <= ptr);
+                    float lg1 = 0.0f, lg2 = 0.0f;  // unused when the range test fails
+                    if (c1) {
+                        lg1 = plgamma_int(lg, fres + 1.0f);
+                        lg2 = plgamma_int(lg, (ptr + 1.0f) - fres);
+                    }
+                    const float pp2 = pexp((((pg1 - lg1) - lg2) + fres * plp) + (ptr - fres) * plq);
                     const float u2 = ur_f32(cur.word32(2 * i + 1, j));
                     fv = c1 && (u2 <= pp1 * pp2);
                 }

@@ -77,7 +77,8 @@ __global__ __launch_bounds__(64) void k_binomial_vop(EngineState* __restrict__ s
                                                      int32_t* __restrict__ out, int64_t per,
                                                      const int32_t* __restrict__ trials,
                                                      const float* __restrict__ prob,
-                                                     uint32_t* __restrict__ status) {
+                                                     uint32_t* __restrict__ status,
+                                                     const float* __restrict__ lg) {
     __shared__ uint64_t mt[kWords];
     __shared__ uint64_t carry[2];
     __shared__ float res_lds[4];
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(64) void k_binomial_vop(EngineState* __restrict__ s
     if (per / 4 == 0) return;
     Eng e;
     eng_load(e, mt, states + g, lane);
-    op_binomial_vop(e, carry, res_lds, out + g * per, per, trials + g * per, prob + g * per, status,
+    op_binomial_vop(e, carry, res_lds, out + g * per, per, trials + g * per, prob + g * per, status, lg,
                     lane);
     eng_store(e, states + g, lane);
 }
@@ -111,7 +112,8 @@ __global__ __launch_bounds__(64) void k_gamma_vop(EngineState* __restrict__ stat
 __global__ __launch_bounds__(64) void k_poisson_vop(EngineState* __restrict__ states,
                                                     int32_t* __restrict__ out, int64_t per,
                                                     const float* __restrict__ mean,
-                                                    uint32_t* __restrict__ status) {
+                                                    uint32_t* __restrict__ status,
+                                                    const float* __restrict__ lg) {
     __shared__ uint64_t mt[kWords];
     __shared__ uint64_t carry[2];
     __shared__ float res_lds[4];
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(64) void k_poisson_vop(EngineState* __restrict__ st
     if (per / 4 == 0) return;
     Eng e;
     eng_load(e, mt, states + g, lane);
-    op_poisson_vop(e, carry, res_lds, out + g * per, per, mean + g * per, status, lane);
+    op_poisson_vop(e, carry, res_lds, out + g * per, per, mean + g * per, status, lg, lane);
     eng_store(e, states + g, lane);
 }
 
@@ -136,7 +138,9 @@ hipError_t launch_fill_gamma_vop_f32(EngineState* st, int64_t n, float* out, int
 hipError_t launch_fill_poisson_vop_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
                                        const float* mean, uint32_t* status, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_poisson_vop, dim3((unsigned)n), dim3(64), 0, s, st, out, per, mean, status);
+    const float* lg = lgamma_table(s);
+    if (!lg) return hipErrorOutOfMemory;
+    hipLaunchKernelGGL(k_poisson_vop, dim3((unsigned)n), dim3(64), 0, s, st, out, per, mean, status, lg);
     return hipGetLastError();
 }
 
@@ -162,8 +166,10 @@ hipError_t launch_fill_binomial_vop_i32(EngineState* st, int64_t n, int32_t* out
                                         const int32_t* trials, const float* p, uint32_t* status,
                                         hipStream_t s) {
     if (n <= 0) return hipSuccess;
+    const float* lg = lgamma_table(s);
+    if (!lg) return hipErrorOutOfMemory;
     hipLaunchKernelGGL(k_binomial_vop, dim3((unsigned)n), dim3(64), 0, s, st, out, per, trials, p,
-                       status);
+                       status, lg);
     return hipGetLastError();
 }
 

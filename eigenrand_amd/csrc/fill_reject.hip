@@ -19,12 +19,43 @@
 // lanes whose packet is not complete yet wait in the LDS rest queue. The fill
 // stops at the candidate that emits the last packet, and the engine is left
 // exactly after that candidate's slots, as the reference leaves it.
+#include <mutex>
 #include "erd_device.hpp"
 #include "erd_dists.hpp"
 #include "erd_kernels.hpp"
 #include "erd_ops.hpp"
 
 namespace erd {
+
+// ----------------------------------------------------- plgamma table ---
+__global__ __launch_bounds__(256) void k_lgamma_table(float* __restrict__ lg) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < kLgTab) lg[k] = plgamma_approx((float)(k + 1));
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_lg_mutex;
+const float* g_lg[kMaxDevices] = {};
+}  // namespace
+
+const float* lgamma_table(hipStream_t s) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_lg_mutex);
+    if (!g_lg[dev]) {
+        float* lg = nullptr;
+        if (hipMalloc(&lg, sizeof(float) * kLgTab) != hipSuccess) return nullptr;
+        hipLaunchKernelGGL(k_lgamma_table, dim3(kLgTab / 256), dim3(256), 0, s, lg);
+        // the table is complete before any stream of this device can use it
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+            (void)hipFree(lg);
+            return nullptr;
+        }
+        g_lg[dev] = lg;
+    }
+    return g_lg[dev];
+}
 
 // gs (optional): per-engine queues of a reused generator object.
 template <class P>
@@ -66,15 +97,19 @@ hipError_t launch_fill_gamma_reject_f32(EngineState* st, int64_t n, float* out, 
 hipError_t launch_fill_poisson_reject_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
                                           const PoissonParams& pp, uint32_t* status, hipStream_t s,
                                           GenState* gs) {
-    PoissonTangent pol{(float)pp.sqrt_tmean, (float)pp.mean, (float)pp.log_mean, (float)pp.g1};
+    const float* lg = lgamma_table(s);
+    if (!lg) return hipErrorOutOfMemory;
+    PoissonTangent pol{(float)pp.sqrt_tmean, (float)pp.mean, (float)pp.log_mean, (float)pp.g1, lg};
     return launch_reject(st, n, out, per, pol, status, gs, s);
 }
 
 hipError_t launch_fill_binomial_reject_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
                                            const BinomialParams& bp, uint32_t* status, hipStream_t s,
                                            GenState* gs) {
+    const float* lg = lgamma_table(s);
+    if (!lg) return hipErrorOutOfMemory;
     BinomialTangent pol{(float)bp.trials, (float)bp.sqrt_v, (float)bp.mean, (float)bp.log_small_p,
-                        (float)bp.log_small_q, (float)bp.g1, bp.trials, bp.p == bp.small_p ? 0 : 1};
+                        (float)bp.log_small_q, (float)bp.g1, bp.trials, bp.p == bp.small_p ? 0 : 1, lg};
     return launch_reject(st, n, out, per, pol, status, gs, s);
 }
 

@@ -184,7 +184,7 @@ __device__ int32_t knuth_packet_lanes(Eng& e, uint64_t* carry, const float* thr4
 // rejection whose candidates share the gamma's CompressMask queue. A fresh
 // generator per fill.
 __device__ void run_negbin(Eng& e, ProgramLds& s, BetaLds& bl, int32_t* out, int64_t n, float alpha, float beta,
-                           uint32_t* status, int lane) {
+                           uint32_t* status, const float* lg, int lane) {
     GenState* q = &bl.q[0];
     if (lane == 0) q->count = 0;
     __syncthreads();
@@ -218,14 +218,15 @@ __device__ void run_negbin(Eng& e, ProgramLds& s, BetaLds& bl, int32_t* out, int
                 bl.c[2][lane] = m * lm - plgamma_approx(m + 1.0f);
             }
             __syncthreads();
-            NegBinTangent pol{mean, bl.c[0], bl.c[1], bl.c[2]};
+            NegBinTangent pol{mean, bl.c[0], bl.c[1], bl.c[2], lg};
             op_reject(e, s.nbt, out + 4 * pk, 4, pol, status, lane, q);
         }
         __syncthreads();
     }
 }
 
-__device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& sg, uint32_t* status, int lane) {
+__device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& sg, uint32_t* status,
+                            const float* lg, int lane) {
     const int64_t n = sg.count;
     switch (sg.kind) {
         case SEG_U64: {
@@ -282,7 +283,7 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
             if (sg.d[0] < 12) {
                 op_knuth(e, s.knuth.carry, s.knuth.u, (int32_t*)sg.out, n, (float)sg.d[1], 0, 0, status, lane);
             } else {
-                PoissonTangent pol{(float)sg.d[2], (float)sg.d[0], (float)sg.d[3], (float)sg.d[4]};
+                PoissonTangent pol{(float)sg.d[2], (float)sg.d[0], (float)sg.d[3], (float)sg.d[4], lg};
                 op_reject(e, s.pt, (int32_t*)sg.out, n, pol, status, lane);
             }
             break;
@@ -297,14 +298,14 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
                 op_knuth(e, s.knuth.carry, s.knuth.u, (int32_t*)sg.out, n, (float)sg.d[7], trials, flip, status, lane);
             } else {
                 BinomialTangent pol{(float)trials, (float)sg.d[4], (float)sg.d[2], (float)sg.d[5],
-                                    (float)sg.d[6], (float)sg.d[3], trials, flip};
+                                    (float)sg.d[6], (float)sg.d[3], trials, flip, lg};
                 op_reject(e, s.bt, (int32_t*)sg.out, n, pol, status, lane);
             }
             break;
         }
         case SEG_BINOMIAL_VOP_I32:
             op_binomial_vop(e, s.vop.carry, s.vop.res, (int32_t*)sg.out, n, (const int32_t*)sg.in0,
-                            (const float*)sg.in1, status, lane);
+                            (const float*)sg.in1, status, lg, lane);
             break;
         case SEG_GAMMA_VOP_F32:
             op_gamma_vop(e, s.vop.carry, (float*)sg.out, n, (const float*)sg.in0,
@@ -312,7 +313,7 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
             break;
         case SEG_POISSON_VOP_I32:
             op_poisson_vop(e, s.vop.carry, s.vop.res, (int32_t*)sg.out, n, (const float*)sg.in0, status,
-                           lane);
+                           lg, lane);
             break;
         case SEG_DISCARD:
             run_discard(e, n, lane);
@@ -349,7 +350,7 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
             break;
         }
         case SEG_NEGATIVE_BINOMIAL_I32:
-            run_negbin(e, s, bl, (int32_t*)sg.out, n, (float)sg.d[0], (float)sg.d[1], status, lane);
+            run_negbin(e, s, bl, (int32_t*)sg.out, n, (float)sg.d[0], (float)sg.d[1], status, lg, lane);
             break;
         case SEG_BETA_F32:
             run_beta(e, s, bl, (float*)sg.out, n, (float)sg.d[0], (float)sg.d[1], (float)sg.d[2], status, lane);
@@ -365,7 +366,7 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
             int32_t* o = (int32_t*)sg.out;
             for (int64_t row = 0; row + 1 < K; ++row)
                 op_binomial_row8(e, s.vop.carry, s.vop.res, o + row, K, rr, row == 0 ? sg.i0 : -1, n, prow[row],
-                                 status, lane);
+                                 status, lg, lane);
             for (int64_t i = lane; i < n; i += 64) o[i * K + (K - 1)] = K == 1 ? sg.i0 : rr[i];
             break;
         }
@@ -381,7 +382,8 @@ __global__ __launch_bounds__(64) void k_program(EngineState* __restrict__ states
                                                 const int32_t* __restrict__ engine_of,
                                                 const int32_t* __restrict__ begin,
                                                 const Segment* __restrict__ segs,
-                                                uint32_t* __restrict__ status) {
+                                                uint32_t* __restrict__ status,
+                                                const float* __restrict__ lg) {
     __shared__ uint64_t mt[kWords];
     __shared__ ProgramLds s;
     __shared__ BetaLds bl;
@@ -390,7 +392,7 @@ __global__ __launch_bounds__(64) void k_program(EngineState* __restrict__ states
     EngineState* st = states + engine_of[b];
     Eng e;
     eng_load(e, mt, st, lane);
-    for (int k = begin[b]; k < begin[b + 1]; ++k) run_segment(e, s, bl, segs[k], status, lane);
+    for (int k = begin[b]; k < begin[b + 1]; ++k) run_segment(e, s, bl, segs[k], status, lg, lane);
     eng_store(e, st, lane);
 }
 
@@ -398,8 +400,10 @@ hipError_t launch_program(EngineState* st, int32_t n_groups, const int32_t* d_en
                           const int32_t* d_begin, const Segment* d_segs, uint32_t* status,
                           hipStream_t s) {
     if (n_groups <= 0) return hipSuccess;
+    const float* lg = lgamma_table(s);
+    if (!lg) return hipErrorOutOfMemory;
     hipLaunchKernelGGL(k_program, dim3((unsigned)n_groups), dim3(64), 0, s, st, d_engine_of, d_begin,
-                       d_segs, status);
+                       d_segs, status, lg);
     return hipGetLastError();
 }
 

@@ -109,6 +109,37 @@ def test_poisson_knuth_prefix_many_windows(er, orc, mean):
         np.testing.assert_array_equal(urng.snapshot(), ref.raw)
 
 
+@pytest.mark.parametrize("kind,a,b", [("poisson", 1.03e6, 0), ("poisson", 1048000.0, 0),
+                                      ("poisson", 3.0e6, 0), ("binomial", 2 ** 21 + 5, 0.5),
+                                      ("binomial", 3000000, 0.4), ("binomial", 2100000, 0.97)])
+def test_tangent_beyond_lgamma_table(er, orc, kind, a, b):
+    """Tangent candidates whose plgamma arguments straddle or exceed the 2^20
+    entries of the device table (computed exactly there)."""
+    import torch
+    urng = er.Vmt19937_64(seeds=SEEDS)
+    ref = orc.Engines(seeds=SEEDS)
+    n = 2048
+    if kind == "poisson":
+        assert_same(flat(er.poisson(n, 1, urng, a)), orc.fill_poisson_i32(ref, n, a))
+    else:
+        assert_same(flat(er.binomial(n, 1, urng, a, b)), orc.fill_binomial_i32(ref, n, a, b))
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+    # the same parameters per lane in the VoP forms
+    m = len(SEEDS) * 512
+    if kind == "poisson":
+        mean = np.full(m, a, np.float32)
+        mean[::3] = np.float32(20.0)
+        out = er.poisson_vop(urng, torch.from_numpy(mean).cuda())
+        assert_same(out.cpu().numpy(), orc.fill_poisson_vop_i32(ref, mean))
+    else:
+        trials = np.full(m, a, np.int32)
+        trials[::3] = 100
+        pr = np.full(m, b, np.float32)
+        out = er.binomial_vop(urng, torch.from_numpy(trials).cuda(), torch.from_numpy(pr).cuda())
+        assert_same(out.cpu().numpy(), orc.fill_binomial_vop_i32(ref, trials, pr))
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+
+
 @pytest.mark.parametrize("trials,p", [(1, 0.5), (3, 0.7), (10, 0.5), (25, 0.3), (50, 0.05), (19, 0.9)])
 def test_negative_binomial_matches_oracle(er, orc, trials, p):
     """NegativeBinomialGen: gamma means on half draws (exponential, integer
