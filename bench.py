@@ -210,7 +210,7 @@ def serving_workload(n_req, n_clients, seed=5):
     return reqs, covs
 
 
-def serving(n_req=300, n_clients=64, engines_per_client=4096, engine_work=0):
+def serving(n_req=1000, n_clients=64, engines_per_client=4096, engine_work=0):
     """Serving configuration on one GPU: the batched loop (continuous batching
     over 8 streams) against per-request launches (one request per launch, one
     at a time) on the same server code, engines and outputs. Returns a dict."""
@@ -287,7 +287,10 @@ def serving(n_req=300, n_clients=64, engines_per_client=4096, engine_work=0):
     res = {"requests": n_req, "clients": n_clients, "engines_per_client": engines_per_client,
            "engine_work": engine_work or 65536,
            "elements": int(total_elems), "hbm_budget_bytes": 64 << 30}
-    for name, cfg in modes.items():
+    # per-request first: its open-loop p99 is the reference the batched loop's
+    # rate sweep is held to (throughput at equal p99)
+    for name in ("per_request", "batched"):
+        cfg = modes[name]
         with Server(max_clients=n_clients, engines_per_client=engines_per_client,
                     hbm_budget_bytes=64 << 30, seed_salt=0x5EED, engine_work=engine_work, **cfg) as srv:
             warm(srv)
@@ -295,6 +298,19 @@ def serving(n_req=300, n_clients=64, engines_per_client=4096, engine_work=0):
             sat_rate = n_req / dt
             load = 0.7 * sat_rate
             ol = open_loop(srv, load)
+            if name == "batched":
+                # highest offered rate whose p99 stays within the per-request
+                # loop's p99 at its own 70% load
+                ref_p99 = res["per_request"]["p99_ms"]
+                sweep_pts, best = [], None
+                for frac in (0.4, 0.5, 0.6, 0.7, 0.8, 0.9, 1.0):
+                    o2 = open_loop(srv, frac * sat_rate, seed=29)
+                    sweep_pts.append({"offered_per_s": round(frac * sat_rate, 1),
+                                      "p50_ms": round(o2["p50_ms"], 3), "p99_ms": round(o2["p99_ms"], 3)})
+                    if o2["p99_ms"] <= ref_p99:
+                        best = frac * sat_rate
+                res["equal_p99"] = {"reference_p99_ms": ref_p99, "batched_sweep": sweep_pts,
+                                    "batched_rate_per_s": None if best is None else round(best, 1)}
             res[name] = {
                 "saturated_gsamples_per_s": round(total_elems / dt / 1e9, 3),
                 "saturated_requests_per_s": round(sat_rate, 1),
@@ -312,6 +328,9 @@ def serving(n_req=300, n_clients=64, engines_per_client=4096, engine_work=0):
               file=sys.stderr, flush=True)
     b, p = res["batched"], res["per_request"]
     res["throughput_ratio"] = round(b["saturated_gsamples_per_s"] / p["saturated_gsamples_per_s"], 2)
+    eq = res["equal_p99"]
+    eq["ratio_vs_per_request"] = (None if eq["batched_rate_per_s"] is None
+                                  else round(eq["batched_rate_per_s"] / p["open_loop_rate_per_s"], 2))
     return res
 
 
@@ -401,7 +420,7 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="secondary per-distribution measurements")
     ap.add_argument("--mv", action="store_true", help="only the batched multivariate measurements")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (p50/p99) leg")
-    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--serve-only", action="store_true", help="only the serving leg (one JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy (HBM counter calibration) and exit")

@@ -928,7 +928,7 @@ __device__ void op_binomial_small(Eng& e, uint64_t* carry, int32_t* o, int64_t p
 // are evaluated 16 at a time (thread 4i+j runs iteration i of lane j); each
 // open lane takes its first accepted candidate and the packet consumes the
 // iterations up to the slowest lane, as the sequential loop does.
-__device__ void op_binomial_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t* o, int64_t per,
+__device__ void op_binomial_vop(Eng& e, uint64_t* carry, int32_t* o, int64_t per,
                                 const int32_t* trials, const float* prob, uint32_t* status,
                                 const float* lg, int lane) {
     const int64_t T = per / 4;
@@ -1022,12 +1022,10 @@ __device__ void op_binomial_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t
                     }
                 }
                 if (!all) need = ni;
+                // lane j's first accepted value, from the thread that drew it
                 const uint64_t mine = A & jmask;
-                const bool open = !(done & (1ull << j));
-                if (open && mine && i == ((__ffsll((long long)mine) - 1) >> 2)) res_lds[j] = fres;
-                __syncthreads();
-                if (open && mine) valid_res = res_lds[j];
-                __syncthreads();
+                const float got = __shfl(fres, mine ? (int)(__ffsll((long long)mine) - 1) : lane);
+                if (!(done & (1ull << j)) && mine) valid_res = got;
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                     if (A & (0x1111111111111111ull << q)) done |= 1ull << q;
@@ -1167,7 +1165,7 @@ __device__ void op_gamma_vop(Eng& e, uint64_t* carry, float* o, int64_t per, con
 // chains end within a batch: small means are mostly well below 1). The
 // per-lane constants are computed for 16 packets at once (one packet-lane per
 // thread) and fetched by bpermute; op_binomial_vop does the same.
-__device__ void op_poisson_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t* o, int64_t per,
+__device__ void op_poisson_vop(Eng& e, uint64_t* carry, int32_t* o, int64_t per,
                                const float* mean, uint32_t* status, const float* lg, int lane) {
     const int64_t T = per / 4;
     if (T == 0) return;
@@ -1262,12 +1260,10 @@ __device__ void op_poisson_vop(Eng& e, uint64_t* carry, float* res_lds, int32_t*
                     }
                 }
                 if (!all) need = ni;
+                // lane j's first accepted value, from the thread that drew it
                 const uint64_t mine = A & jmask;
-                const bool open = !(done & (1ull << j));
-                if (open && mine && i == ((__ffsll((long long)mine) - 1) >> 2)) res_lds[j] = fres;
-                __syncthreads();
-                if (open && mine) res = res_lds[j];
-                __syncthreads();
+                const float got = __shfl(fres, mine ? (int)(__ffsll((long long)mine) - 1) : lane);
+                if (!(done & (1ull << j)) && mine) res = got;
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                     if (A & (0x1111111111111111ull << q)) done |= 1ull << q;
@@ -1416,10 +1412,14 @@ __device__ void op_multinomial_small(Eng& e, int32_t* out, int64_t n, int32_t T,
 // per-row sampler of MultinomialGen's large-trials path (Multinomial.h:74-91):
 // trials come from r[] (int32), p is one float for the row, and the row
 // result t goes to out[i*stride] while r[i] -= t. Full draws; thread 8i+j
-// runs iteration i of lane j (8 iterations per step).
-__device__ void op_binomial_row8(Eng& e, uint64_t* carry, float* res_lds, int32_t* out, int64_t stride,
-                                 int32_t* r, int32_t t0, int64_t n, float p, uint32_t* status,
-                                 const float* lg, int lane) {
+// runs iteration i of lane j (8 iterations per step). REG (n <= 64): thread
+// 8i+j keeps sample 8i+j's remaining trials in `rreg` instead of r[].
+// The row constants (p, pmin(p, 1-p) and the logs of the latter and of its
+// complement) come from the caller, which computes them for many rows at once.
+template <bool REG>
+__device__ void op_binomial_row8(Eng& e, uint64_t* carry, int32_t* out, int64_t stride, int32_t* r,
+                                 int32_t& rreg, int32_t t0, int64_t n, float p, float psmall, float plp,
+                                 float plq, uint32_t* status, const float* lg, int lane) {
     const int64_t T = n / 8;
     if (T == 0) return;
     const int i = lane >> 3, j = lane & 7;
@@ -1431,14 +1431,12 @@ __device__ void op_binomial_row8(Eng& e, uint64_t* carry, float* res_lds, int32_
     cur.used = false;
     cur.twisted = &e.twisted;
     const uint64_t jmask = 0x0101010101010101ull << j;
-    const float psmall = (p < 1.0f - p) ? p : 1.0f - p;
-    const float plp = plog(psmall);
-    const float plq = plog(1.0f - psmall);
     for (int64_t pk = 0; pk < T; ++pk) {
-        const int32_t nt = t0 >= 0 ? t0 : r[8 * pk + j];  // t0: first row, every sample has t0 trials
+        // t0: first row, every sample has t0 trials
+        const int32_t nt = t0 >= 0 ? t0 : REG ? __shfl(rreg, 8 * (int)pk + j) : r[8 * pk + j];
         const float ptr = (float)nt;
         const float pmean = ptr * psmall;
-        const float pg1 = plgamma_approx(ptr + 1.0f);
+        const float pg1 = plgamma_int(lg, ptr + 1.0f);  // trials + 1 is integer-valued
         const float psq = sqrtf((2.0f * pmean) * (1.0f - psmall));
         const bool valid = pmean < 1.0f;
         float valid_res = 0.0f;
@@ -1500,12 +1498,10 @@ __device__ void op_binomial_row8(Eng& e, uint64_t* carry, float* res_lds, int32_
                     }
                 }
                 if (!all) need = ni;
+                // lane j's first accepted value, from the thread that drew it
                 const uint64_t mine = A & jmask;
-                const bool open = !(done & (1ull << j));
-                if (open && mine && i == ((__ffsll((long long)mine) - 1) >> 3)) res_lds[j] = fres;
-                __syncthreads();
-                if (open && mine) valid_res = res_lds[j];
-                __syncthreads();
+                const float got = __shfl(fres, mine ? (int)(__ffsll((long long)mine) - 1) : lane);
+                if (!(done & (1ull << j)) && mine) valid_res = got;
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
                     if (A & (0x0101010101010101ull << q)) done |= 1ull << q;
@@ -1519,9 +1515,10 @@ __device__ void op_binomial_row8(Eng& e, uint64_t* carry, float* res_lds, int32_
         }
         const int32_t k = (int32_t)valid_res;
         const int32_t t = (p == psmall) ? k : nt - k;
+        if (REG && i == pk) rreg = nt - t;
         if (lane < 8) {
             out[(8 * pk + j) * stride] = t;
-            r[8 * pk + j] = nt - t;
+            if (!REG) r[8 * pk + j] = nt - t;
         }
     }
     __syncthreads();

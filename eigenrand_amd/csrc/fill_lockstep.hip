@@ -81,13 +81,12 @@ __global__ __launch_bounds__(64) void k_binomial_vop(EngineState* __restrict__ s
                                                      const float* __restrict__ lg) {
     __shared__ uint64_t mt[kWords];
     __shared__ uint64_t carry[2];
-    __shared__ float res_lds[4];
     const int lane = threadIdx.x;
     const int64_t g = blockIdx.x;
     if (per / 4 == 0) return;
     Eng e;
     eng_load(e, mt, states + g, lane);
-    op_binomial_vop(e, carry, res_lds, out + g * per, per, trials + g * per, prob + g * per, status, lg,
+    op_binomial_vop(e, carry, out + g * per, per, trials + g * per, prob + g * per, status, lg,
                     lane);
     eng_store(e, states + g, lane);
 }
@@ -116,13 +115,12 @@ __global__ __launch_bounds__(64) void k_poisson_vop(EngineState* __restrict__ st
                                                     const float* __restrict__ lg) {
     __shared__ uint64_t mt[kWords];
     __shared__ uint64_t carry[2];
-    __shared__ float res_lds[4];
     const int lane = threadIdx.x;
     const int64_t g = blockIdx.x;
     if (per / 4 == 0) return;
     Eng e;
     eng_load(e, mt, states + g, lane);
-    op_poisson_vop(e, carry, res_lds, out + g * per, per, mean + g * per, status, lg, lane);
+    op_poisson_vop(e, carry, out + g * per, per, mean + g * per, status, lg, lane);
     eng_store(e, states + g, lane);
 }
 

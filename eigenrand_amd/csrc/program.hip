@@ -25,7 +25,6 @@ union ProgramLds {
     RejectLds<BinomialTangent> bt;
     struct {
         uint64_t carry[2 * kLanes];
-        float res[8];
     } vop;
     struct {
         uint64_t carry[2 * kLanes];
@@ -304,7 +303,7 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
             break;
         }
         case SEG_BINOMIAL_VOP_I32:
-            op_binomial_vop(e, s.vop.carry, s.vop.res, (int32_t*)sg.out, n, (const int32_t*)sg.in0,
+            op_binomial_vop(e, s.vop.carry, (int32_t*)sg.out, n, (const int32_t*)sg.in0,
                             (const float*)sg.in1, status, lg, lane);
             break;
         case SEG_GAMMA_VOP_F32:
@@ -312,7 +311,7 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
                          (const float*)sg.in1, status, lane);
             break;
         case SEG_POISSON_VOP_I32:
-            op_poisson_vop(e, s.vop.carry, s.vop.res, (int32_t*)sg.out, n, (const float*)sg.in0, status,
+            op_poisson_vop(e, s.vop.carry, (int32_t*)sg.out, n, (const float*)sg.in0, status,
                            lg, lane);
             break;
         case SEG_DISCARD:
@@ -364,10 +363,35 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
             const float* prow = (const float*)sg.in1;
             int32_t* rr = (int32_t*)sg.in0;
             int32_t* o = (int32_t*)sg.out;
-            for (int64_t row = 0; row + 1 < K; ++row)
-                op_binomial_row8(e, s.vop.carry, s.vop.res, o + row, K, rr, row == 0 ? sg.i0 : -1, n, prow[row],
-                                 status, lg, lane);
-            for (int64_t i = lane; i < n; i += 64) o[i * K + (K - 1)] = K == 1 ? sg.i0 : rr[i];
+            // Row constants for 64 rows at a time, one row per thread, read
+            // per row with readlane.
+            int32_t rreg = sg.i0;  // n <= 64: thread i keeps sample i's remaining trials
+            float c_p = 0.f, c_ps = 0.f, c_lp = 0.f, c_lq = 0.f;
+            for (int64_t row = 0; row + 1 < K; ++row) {
+                const int q = (int)(row & 63);
+                if (q == 0) {
+                    c_p = row + lane + 1 < K ? prow[row + lane] : 0.5f;
+                    c_ps = (c_p < 1.0f - c_p) ? c_p : 1.0f - c_p;  // pmin(p, 1-p)
+                    c_lp = plog(c_ps);
+                    c_lq = plog(1.0f - c_ps);
+                }
+                const float rp = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c_p), q));
+                const float rps = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c_ps), q));
+                const float rlp = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c_lp), q));
+                const float rlq = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c_lq), q));
+                const int32_t t0 = row == 0 ? sg.i0 : -1;
+                if (n <= 64)
+                    op_binomial_row8<true>(e, s.vop.carry, o + row, K, rr, rreg, t0, n, rp, rps, rlp, rlq, status,
+                                           lg, lane);
+                else
+                    op_binomial_row8<false>(e, s.vop.carry, o + row, K, rr, rreg, t0, n, rp, rps, rlp, rlq,
+                                            status, lg, lane);
+            }
+            if (n <= 64) {
+                if (lane < n) o[lane * K + (K - 1)] = rreg;
+            } else {
+                for (int64_t i = lane; i < n; i += 64) o[i * K + (K - 1)] = K == 1 ? sg.i0 : rr[i];
+            }
             break;
         }
         default:

@@ -218,6 +218,7 @@ def test_discrete_f_fill_matches_oracle(er, orc, K):
     [(1000, 4, 64), (5000, 100, 16), (3001, 3, 8)],                # BinomialVGen<float> rows
     [(50, 1000, 8), (1, 20, 16), (250, 12, 40), (2000, 16, 24)],  # mixed
     [(60000, 1500, 16), (5000, 2, 8)],                              # long row chains
+    [(3000, 50, 72), (10000, 30, 200), (400, 9, 64)],               # more than 64 samples per engine
 ])
 def test_multinomial_batch_matches_oracle(er, orc, cases):
     rng = np.random.default_rng(len(cases))
