@@ -548,3 +548,289 @@ def test_negative_binomial_moments(orc, trials, p):
     assert x.min() >= 0
     assert abs(x.mean() - m) < 5 * np.sqrt(v / x.size) + 1e-3
     assert abs(x.var() / v - 1) < 0.05
+
+
+# ------------------------------------------------------------ DiscreteGen ---
+def _discrete_cat(words, w, prob=None, alias=None):
+    """packetOp category of 32-bit words (Discrete.h:446-471): K < 16 counts
+    the CDF thresholds above the 31-bit value (signed compares); otherwise the
+    alias test (word >> 1) < prob[word & bitmask] picks the slot or its alias."""
+    words = words.astype(np.uint32)
+    K = len(w)
+    if K < 16:
+        c = np.cumsum(np.asarray(w, np.float64))
+        cdf = (c / c[-1] * 0x80000000).astype(np.uint64).astype(np.uint32).view(np.int32)
+        rx = (words & np.uint32(0x7FFFFFFF)).view(np.int32)
+        ret = np.full(words.shape, K - 1, np.int32)
+        for i in range(K - 1):
+            ret -= (rx < cdf[i]).astype(np.int32)
+        return ret
+    albit = (words & np.uint32(prob.size - 1)).astype(np.int64)
+    c = (words >> np.uint32(1)).view(np.int32) < prob[albit]
+    return np.where(c, albit, alias[albit]).astype(np.int32)
+
+
+@pytest.mark.parametrize("K,per", [(5, 1024), (15, 1020), (16, 1024), (100, 1028), (1000, 512)])
+def test_discrete_of_the_stream(orc, K, per):
+    # AVX2: one full draw gives a Packet8i; packet 2k is its low half, packet
+    # 2k+1 the cached high half; a fill of 4 mod 8 ends on a low half and
+    # drops the cached one with the generator.
+    rng = np.random.default_rng(K)
+    w = rng.uniform(0, 1, K)
+    w[::7] = 0.0
+    e = orc.Engines(n=2, seed0=1 + K)
+    got = orc.fill_discrete_i32(e.copy(), per, w).reshape(2, per)
+    draws = (per + 7) // 8
+    words = full_words(orc, e, draws).reshape(2, draws * 8)[:, :per]
+    prob = alias = None
+    if K >= 16:
+        prob, alias = orc.alias_table(w)
+    np.testing.assert_array_equal(got, _discrete_cat(words, w, prob, alias))
+    e2 = e.copy()
+    orc.fill_discrete_i32(e2, per, w)
+    e3 = e.copy()
+    orc.fill_u64(e3, draws)
+    np.testing.assert_array_equal(e2.raw, e3.raw)
+
+
+# ------------------------------------------------------- BinomialVGen<int> ---
+def _binomial_v_packet(orc, h, s, n, p):
+    """BinomialVGen<int32_t>::packetOp (Discrete.h:1124-1190) on half draws
+    h[s:], four lanes; returns (values, half draws consumed)."""
+    return _binomial_v_lanes(orc, lambda q: ur(h[q]), s, n, p)
+
+
+def _binomial_v_lanes(orc, u, s, n, p):
+    """The BinomialVGen packetOp body (Discrete.h:1124-1190 for int32 lanes,
+    1257-1327 for float lanes, the same steps): u(q) gives the uniforms of
+    draw q for all lanes. Returns (values, draws consumed)."""
+    ps = np.minimum(p, f32(1) - p)
+    tr = n.astype(f32)
+    pmean = tr * ps
+    pg1 = orc.plgamma(tr + f32(1))
+    psq = np.sqrt(f32(2) * pmean * (f32(1) - ps)).astype(f32)
+    plp = orc.plog(ps)
+    plq = orc.plog(f32(1) - ps)
+    valid = pmean < f32(1)
+    L = n.size
+    res = np.zeros(L, f32)
+    s0 = s
+    if valid.any():
+        nem = orc.pexp(-pmean)
+        val = np.ones(L, f32)
+        cnt = np.zeros(L, np.int32)
+        while True:
+            val = (val * u(s)).astype(f32)
+            s += 1
+            c = (nem < val) & valid
+            if not c.any():
+                break
+            cnt += c
+        res = cnt.astype(f32)
+    if not valid.all():
+        while True:
+            sn, cs = orc.psincos(f32(math.pi) * u(s))
+            ys = (sn / cs).astype(f32)
+            fres = np.trunc(psq * ys + pmean).astype(f32)
+            p1 = (f32(1.2) * psq) * (f32(1) + ys * ys)
+            with np.errstate(invalid="ignore"):  # p = 0 or 1 lanes: resolved by the Knuth loop
+                p2 = orc.pexp((((pg1 - orc.plgamma(fres + f32(1))) - orc.plgamma((tr + f32(1)) - fres))
+                               + fres * plp) + (tr - fres) * plq)
+            c1 = (f32(0) <= fres) & (fres <= tr)
+            c2 = u(s + 1) <= (p1 * p2).astype(f32)
+            s += 2
+            fv = c1 & c2
+            res = np.where(fv & ~valid, fres, res)
+            valid = valid | fv
+            if valid.all():
+                break
+    k = np.trunc(res).astype(np.int32)
+    return np.where(p == ps, k, n - k), s - s0
+
+
+def test_binomial_vop_of_the_stream(orc):
+    rng = np.random.default_rng(31)
+    per = 256
+    n = rng.integers(1, 1001, 2 * per).astype(np.int32)
+    p = rng.uniform(0, 1, 2 * per).astype(f32)
+    n[::5] = rng.integers(1, 30, n[::5].size)       # small means: the Knuth lockstep
+    p[::7] = f32(0.003)
+    n[::11] = 0
+    p[::13] = f32(1.0)
+    e = orc.Engines(n=2, seed0=2024)
+    got = orc.fill_binomial_vop_i32(e.copy(), n, p).reshape(2, per)
+    h = half_words(orc, e, 40 * per)
+    for g in range(2):
+        s = 0
+        for k in range(per // 4):
+            sl = slice(g * per + 4 * k, g * per + 4 * k + 4)
+            v, used = _binomial_v_packet(orc, h[g], s, n[sl], p[sl])
+            s += used
+            np.testing.assert_array_equal(got[g, 4 * k:4 * k + 4], v, err_msg=f"engine {g} packet {k}")
+
+
+# ----------------------------------------------------------- MultinomialGen ---
+def _multinomial_replay(orc, e, trials, w, samples):
+    """MultinomialGen<int32_t>::generate(urng, samples) (Multinomial.h:60-133)
+    on the raw full-draw stream of engine e; returns (K, samples)."""
+    K = len(w)
+    w = np.asarray(w, np.float64)
+    probs = w / w.sum()
+    words = full_words(orc, e, 200000)[0]  # (draws, 8) u32
+    pos = [0]
+    out = np.zeros((K, samples), np.int32)
+    if trials < max(30 * (K - 1), 100):
+        prob = alias = None
+        if K >= 16:
+            prob, alias = orc.alias_table(w)
+        cache = [None]
+
+        def packet4():  # AVX2: a Packet8i per draw, the high half cached
+            if cache[0] is not None:
+                c, cache[0] = cache[0], None
+                return c
+            cats = _discrete_cat(words[pos[0]], w, prob, alias)
+            pos[0] += 1
+            cache[0] = cats[4:]
+            return cats[:4]
+
+        def scalar():  # operator() (Discrete.h:428-441): word 0 of a fresh draw
+            rx = np.int32(words[pos[0], 0].view(np.int32))
+            pos[0] += 1
+            if K < 16:
+                c = np.cumsum(w)
+                cdf = (c / c[-1] * 0x80000000).astype(np.uint64)
+                r = int(rx) & 0x7FFFFFFF
+                return int(np.searchsorted(cdf[:-1], r, side="left"))
+            albit = int(rx) & (prob.size - 1)
+            alx = (int(rx) >> 1) & 0xFFFFFFFF  # arithmetic shift, then as uint32
+            return albit if alx < int(prob[albit]) else int(alias[albit])
+
+        for s in range(samples):
+            i = 0
+            while i < (trials & ~15):
+                buf = np.concatenate([packet4() for _ in range(4)])
+                np.add.at(out[:, s], buf, 1)
+                i += 16
+            r = trials - i
+            buf, j = [], 0
+            while j + 4 <= r:
+                buf += list(packet4())
+                j += 4
+            while j < r:
+                buf.append(scalar())
+                j += 1
+            np.add.at(out[:, s], np.asarray(buf, np.int64), 1)
+        return out
+    r = np.full(samples, trials, np.int32)
+    rest_p = 1 - probs[0]
+    for row in range(K - 1):
+        p = f32(probs[0] if row == 0 else probs[row] / rest_p)
+        t = np.zeros(samples, np.int32)
+        for q in range(0, samples, 8):
+            t[q:q + 8], used = _binomial_v_lanes(orc, lambda d: ur(words[d]), pos[0], r[q:q + 8],
+                                                 np.full(8, p, f32))
+            pos[0] += used
+        r -= t
+        out[row] = t
+        if row > 0:
+            rest_p -= probs[row]
+    out[K - 1] = r
+    return out
+
+
+@pytest.mark.parametrize("trials,K,samples", [(20, 4, 8), (99, 10, 16), (7, 3, 8), (50, 20, 8),
+                                              (200, 40, 8), (1000, 4, 16), (5000, 12, 24), (3001, 3, 8)])
+def test_multinomial_of_the_stream(orc, trials, K, samples):
+    rng = np.random.default_rng(trials + K)
+    w = rng.uniform(0.05, 1, K)
+    e = orc.Engines(n=1, seed0=trials)
+    got = orc.multinomial(e.copy(), trials, [w], samples)[0]
+    np.testing.assert_array_equal(got, _multinomial_replay(orc, e, trials, w, samples))
+
+
+# ---------------------------------------------------- NegativeBinomialGen ---
+def _cvtt(x):
+    """cvttps2dq: truncation; NaN and out-of-range give INT32_MIN."""
+    x = np.asarray(x, np.float64)
+    ok = np.isfinite(x) & (x < 2.0 ** 31) & (x >= -2.0 ** 31)
+    return np.where(ok, np.trunc(np.where(ok, x, 0)), -2.0 ** 31).astype(np.int64).astype(np.int32)
+
+
+@pytest.mark.parametrize("trials,p", [(1, 0.5), (3, 0.7), (10, 0.5), (19, 0.9), (25, 0.3), (50, 0.05)])
+def test_negative_binomial_of_the_stream(orc, trials, p):
+    """NegativeBinomialGen::packetOp (GammaPoisson.h:58-105) at Packet4i on
+    half draws: the GammaGen packet (NormalExp.h:527-602: exponential for
+    alpha = 1, a product of nonzero uniforms for integer alpha < 20, else the
+    tangent step through its CompressMask queue), then Knuth on all lanes if
+    no mean exceeds 12, else Poisson tangent candidates appended to the SAME
+    queue (the gamma's rest buffer) and emitted through cvttps."""
+    per = 256
+    e = orc.Engines(n=1, seed0=trials * 7 + 1)
+    got = orc.fill_negative_binomial_i32(e.copy(), per, trials, p)
+    h = half_words(orc, e, 60000)[0]
+    alpha, beta = f32(trials), f32((1 - p) / p)
+    push = _queue(4)  # the gamma's queue, shared with the Poisson tangent stage
+    s = 0
+    out = []
+    eps8 = f32(2.0 ** -26)
+    while len(out) < per:
+        # GammaGen<float>::packetOp<Packet4f>
+        if alpha == f32(1):
+            mean = beta * -(orc.plog(f32(1) - ur(h[s])) / f32(1))
+            s += 1
+        elif trials < 20:
+            yx = ur(h[s]) + eps8
+            s += 1
+            for _ in range(trials - 1):
+                yx = (yx * (ur(h[s]) + eps8)).astype(f32)
+                s += 1
+            mean = -(beta * orc.plog(yx))
+        else:
+            a1 = alpha - f32(1)
+            sq = np.sqrt(f32(2) * alpha - f32(1)).astype(f32)
+            while True:
+                ys, yc = orc.psincos(f32(math.pi) * ur(h[s]))
+                yx = (ys / yc).astype(f32)
+                xx = sq * yx + a1
+                c = f32(0) < xx
+                ux = ur(h[s + 1])
+                s += 2
+                with np.errstate(invalid="ignore", divide="ignore"):
+                    ub = (yx * yx + f32(1)) * orc.pexp(a1 * orc.plog(xx / a1) - yx * sq)
+                r = push(list(beta * xx), list(c & (ux <= ub)))
+                if r is not None:
+                    mean = np.array(r, f32)
+                    break
+        mean = mean.astype(f32)
+        # the Poisson stage
+        nem = orc.pexp(-mean)
+        if not (f32(12) < mean).any():
+            val = np.ones(4, f32)
+            res = np.zeros(4, np.int32)
+            while True:
+                val = (val * ur(h[s])).astype(f32)
+                s += 1
+                c = nem < val
+                if not c.any():
+                    break
+                res += c
+            out += list(res)
+        else:
+            psq = np.sqrt(f32(2) * mean).astype(f32)
+            plm = orc.plog(mean)
+            pg1 = mean * plm - orc.plgamma(mean + f32(1))
+            while True:
+                sn, cs = orc.psincos(f32(math.pi) * ur(h[s]))
+                yx = (sn / cs).astype(f32)
+                fres = np.trunc(psq * yx + mean).astype(f32)
+                p1 = (yx * yx + f32(1)) * f32(0.9)
+                with np.errstate(invalid="ignore", over="ignore"):
+                    p2 = orc.pexp((fres * plm - orc.plgamma(fres + f32(1))) - pg1)
+                acc = (f32(0) <= fres) & (ur(h[s + 1]) <= p1 * p2)
+                s += 2
+                r = push(list(fres), list(acc))
+                if r is not None:
+                    out += list(_cvtt(np.array(r, f32)))
+                    break
+    np.testing.assert_array_equal(got, np.array(out[:per], np.int32))
