@@ -834,3 +834,103 @@ def test_negative_binomial_of_the_stream(orc, trials, p):
                     out += list(_cvtt(np.array(r, f32)))
                     break
     np.testing.assert_array_equal(got, np.array(out[:per], np.int32))
+
+
+# ------------------------------------------------- gamma / Poisson VoP ---
+# Build-defined (the reference has no GammaVGen/PoissonVGen): BinomialVGen's
+# lockstep design with per-lane parameters; each lane keeps its first
+# accepted candidate and a packet consumes the iterations of its slowest lane.
+
+def test_gamma_vop_of_the_stream(orc):
+    rng = np.random.default_rng(77)
+    per = 128
+    alpha = np.exp(rng.uniform(np.log(0.1), np.log(50), 2 * per)).astype(f32)
+    alpha[::9] = f32(1.0)
+    beta = rng.uniform(0.5, 2, 2 * per).astype(f32)
+    e = orc.Engines(n=2, seed0=5)
+    got = orc.fill_gamma_vop_f32(e.copy(), alpha, beta).reshape(2, per)
+    w = full_words(orc, e, 6000)
+    eps8 = f32(2.0 ** -26)
+    for g in range(2):
+        d = 0
+        for k in range(per // 8):
+            a = alpha[g * per + 8 * k: g * per + 8 * k + 8]
+            b = beta[g * per + 8 * k: g * per + 8 * k + 8]
+            px = f32(math.e) / (a + f32(math.e))
+            inv = f32(1) / a
+            am1 = a - f32(1)
+            with np.errstate(invalid="ignore"):  # alpha < 0.5: unused (GS lanes)
+                sq = np.sqrt(f32(2) * a - f32(1)).astype(f32)
+            res = np.zeros(8, f32)
+            done = np.zeros(8, bool)
+            while not done.all():
+                u1, u2, u3 = ur(w[g, d]), ur(w[g, d + 1]), ur(w[g, d + 2])
+                d += 3
+                with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+                    lv = orc.plog(u2 + eps8)                    # GS lanes (alpha <= 1)
+                    x1 = orc.pexp(inv * lv)
+                    q1 = orc.pexp(-x1)
+                    x2 = f32(1) - lv
+                    q2 = orc.pexp(orc.plog(x2) * am1)
+                    c = u1 < px
+                    xg = np.where(c, x1, x2)
+                    ag = u3 < np.where(c, q1, q2)
+                    ys, yc = orc.psincos(f32(math.pi) * u1)     # tangent lanes
+                    yx = (ys / yc).astype(f32)
+                    xt = sq * yx + am1
+                    ub = (yx * yx + f32(1)) * orc.pexp(am1 * orc.plog(xt / am1) - yx * sq)
+                    at = (f32(0) < xt) & (u2 <= ub)
+                gs = a <= f32(1)
+                x = np.where(gs, xg, xt)
+                acc = np.where(gs, ag, at)
+                new = acc & ~done
+                res = np.where(new, b * x, res)
+                done |= acc
+            np.testing.assert_array_equal(got[g, 8 * k:8 * k + 8], res, err_msg=f"engine {g} packet {k}")
+
+
+def test_poisson_vop_of_the_stream(orc):
+    rng = np.random.default_rng(78)
+    per = 256
+    mean = np.exp(rng.uniform(np.log(0.01), np.log(1e4), 2 * per)).astype(f32)
+    mean[::11] = f32(12.0)
+    mean[::13] = f32(11.99)
+    e = orc.Engines(n=2, seed0=6)
+    got = orc.fill_poisson_vop_i32(e.copy(), mean).reshape(2, per)
+    h = half_words(orc, e, 20000)
+    for g in range(2):
+        s = 0
+        for k in range(per // 4):
+            m = mean[g * per + 4 * k: g * per + 4 * k + 4]
+            small = m < f32(12)
+            nem = orc.pexp(-m)
+            psq = np.sqrt(f32(2) * m).astype(f32)
+            plm = orc.plog(m)
+            pg1 = m * plm - orc.plgamma(m + f32(1))
+            res = np.zeros(4, f32)
+            if small.any():
+                val = np.ones(4, f32)
+                cnt = np.zeros(4, np.int32)
+                while True:
+                    val = (val * ur(h[g, s])).astype(f32)
+                    s += 1
+                    c = (nem < val) & small
+                    if not c.any():
+                        break
+                    cnt += c
+                res = cnt.astype(f32)
+            if not small.all():
+                done = small.copy()
+                while not done.all():
+                    sn, cs = orc.psincos(f32(math.pi) * ur(h[g, s]))
+                    yx = (sn / cs).astype(f32)
+                    fres = np.trunc(psq * yx + m).astype(f32)
+                    p1 = (yx * yx + f32(1)) * f32(0.9)
+                    with np.errstate(invalid="ignore", over="ignore"):
+                        p2 = orc.pexp((fres * plm - orc.plgamma(fres + f32(1))) - pg1)
+                    acc = (f32(0) <= fres) & (ur(h[g, s + 1]) <= p1 * p2)
+                    s += 2
+                    res = np.where(acc & ~done, fres, res)
+                    done |= acc
+            np.testing.assert_array_equal(got[g, 4 * k:4 * k + 4], res.astype(np.int32),
+                                          err_msg=f"engine {g} packet {k}")
