@@ -68,6 +68,25 @@ __global__ __launch_bounds__(64) void k_fill_normal_gen(EngineState* __restrict_
     eng_store(e, states + g, lane);
 }
 
+// DiscreteGen<int32_t, int32_t>::packetOp under AVX2 (Discrete.h:443-483):
+// one draw gives a Packet8i of categories, the low half now and the high half
+// from the cache on the next call; int j of the draw is 32-bit word j. A fill
+// of 4 mod 8 ends on a low half and drops the cached one with the generator.
+struct DiscreteI32 {
+    static constexpr int CALLS = 1;
+    int32_t* out;
+    DiscreteTable t;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return (n + 7) / 8; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int64_t i = 8 * u + 2 * l + h;
+            if (i < per) out[i] = discrete_packet(t, h ? hi32(w[0]) : lo32(w[0]));
+        }
+    }
+};
+
 // ------------------------------------------------------------- launchers ---
 template <class D>
 static hipError_t launch_fixed(EngineState* states, int64_t n_engines, int64_t per, const D& d,
@@ -101,6 +120,12 @@ hipError_t launch_fill_uniform_f64(EngineState* st, int64_t n, double* out, int6
                                    double b, int affine, hipStream_t s) {
     UniformF64 d{out, a, b - a, affine};
     return launch_fixed(st, n, per, d, s);
+}
+
+hipError_t launch_fill_discrete_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, const uint32_t* cdf,
+                                   const int32_t* prob, const int32_t* alias, int32_t ncdf, uint32_t bitmask,
+                                   hipStream_t s) {
+    return launch_fixed(st, n, per, DiscreteI32{out, DiscreteTable{cdf, prob, alias, ncdf, bitmask}}, s);
 }
 
 hipError_t launch_fill_normal_f64(EngineState* st, int64_t n, double* out, int64_t per, double mean,

@@ -663,20 +663,17 @@ int erd_fill_discrete_i32(erd_engines* e, int32_t* out, int64_t per, const doubl
     int rc = build_table(weights, K, t);
     if (rc) return rc;
     Scratch sc(s);
-    std::vector<Segment> segs;
-    std::vector<int32_t> eng;
-    Segment proto{};
-    proto.kind = SEG_DISCRETE_I32;
-    proto.count = per;
-    rc = upload_table(t, proto, sc);
-    if (rc) return rc;
-    for (int64_t g = 0; g < e->count; ++g) {
-        Segment sg = proto;
-        sg.out = out + g * per;
-        segs.push_back(sg);
-        eng.push_back((int32_t)g);
+    uint32_t* cdf = nullptr;
+    int32_t *prob = nullptr, *alias = nullptr;
+    if (!t.cdf.empty()) {
+        if (sc.upload(&cdf, t.cdf) != hipSuccess) return ERD_ERR_HIP;
+    } else if (sc.upload(&prob, t.prob) != hipSuccess || sc.upload(&alias, t.alias) != hipSuccess) {
+        return ERD_ERR_HIP;
     }
-    return run_segments(e, segs, eng, s);
+    return launch_fill_discrete_i32(e->states, e->count, out, per, cdf, prob, alias, (int32_t)t.cdf.size(),
+                                    t.bitmask, s) == hipSuccess
+               ? ERD_OK
+               : ERD_ERR_HIP;
 }
 
 int erd_fill_discrete_f_i32(erd_engines* e, int32_t* out, int64_t per, const double* weights, int64_t K,

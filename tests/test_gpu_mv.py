@@ -185,14 +185,14 @@ def test_alias_table_matches_oracle(er, orc):
             np.testing.assert_array_equal(t[2], alias)
 
 
-@pytest.mark.parametrize("K", [4, 16, 37, 1000])
+@pytest.mark.parametrize("K", [4, 15, 16, 37, 1000])
 def test_discrete_fill_matches_oracle(er, orc, K):
     rng = np.random.default_rng(K)
     w = rng.uniform(0, 1, K)
     seeds = [1, 2, 3]
     urng = er.Vmt19937_64(seeds=seeds)
     ref = orc.Engines(seeds=seeds)
-    for n in (4, 8, 1004, 12):
+    for n in (4, 8, 1004, 12, 5004):
         out = er.discrete(n, 1, urng, w)
         assert_same(out.reshape(len(seeds), n).cpu().numpy(), orc.fill_discrete_i32(ref, n, w))
     np.testing.assert_array_equal(urng.snapshot(), ref.raw)
