@@ -108,8 +108,8 @@ def sweep():
         ("poisson(0.05) i32 [Knuth]", lambda: er.poisson(PER_ENGINE, 1, urng, 0.05, out=outi)),
         ("poisson(0.5) i32 [Knuth]", lambda: er.poisson(PER_ENGINE, 1, urng, 0.5, out=outi)),
         ("poisson(1) i32 [Knuth]", lambda: er.poisson(PER_ENGINE, 1, urng, 1.0, out=outi)),
-        ("poisson(2) i32 [Knuth, prefix]", lambda: er.poisson(PER_ENGINE, 1, urng, 2.0, out=outi)),
-        ("poisson(3) i32 [Knuth, prefix]", lambda: er.poisson(PER_ENGINE, 1, urng, 3.0, out=outi)),
+        ("poisson(2) i32 [Knuth]", lambda: er.poisson(PER_ENGINE, 1, urng, 2.0, out=outi)),
+        ("poisson(3) i32 [Knuth]", lambda: er.poisson(PER_ENGINE, 1, urng, 3.0, out=outi)),
         ("poisson(6) i32 [Knuth, prefix]", lambda: er.poisson(PER_ENGINE, 1, urng, 6.0, out=outi)),
         ("poisson(10) i32 [Knuth, prefix]", lambda: er.poisson(PER_ENGINE, 1, urng, 10.0, out=outi)),
         ("binomial(10,0.3) i32 [n<25]", lambda: er.binomial(PER_ENGINE, 1, urng, 10, 0.3, out=outi)),

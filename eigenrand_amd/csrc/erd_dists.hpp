@@ -530,6 +530,54 @@ struct GammaIntF32 {
 };
 
 
+// BetaGen's gamma-pair regime (NormalExp.h:970-975) when both shapes are
+// integers in [1, 20): GammaGen<float>(a, 1) and (b, 1) then take a fixed
+// number of full draws (NormalExp.h:561-578: one for shape 1, the exponential;
+// `shape` nonzero uniforms multiplied otherwise), so packet k of the fill is
+// draws [k(a+b), k(a+b)+a) for p1 and the next b for p2, and packets are
+// independent. FisherFGen (ratio != 0) maps x to ratio x / (1 - x).
+struct BetaIntF32 {
+    float* out;
+    int a, b;
+    float ratio;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 8; }
+    template <class F>
+    __device__ void emit(int64_t per, int64_t u, int l, F fetch) const {
+        float p[2][2];
+        int c = 0;
+#pragma unroll
+        for (int gi = 0; gi < 2; ++gi) {
+            const int shape = gi ? b : a;
+            if (shape == 1) {
+                const uint64_t w = fetch(c++);
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    p[gi][h] = 1.0f * -(plog_pos(1.0f - ur_f32(h ? hi32(w) : lo32(w))) / 1.0f);
+            } else {
+                float y[2];
+                for (int k = 0; k < shape; ++k) {
+                    const uint64_t w = fetch(c++);
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const float v = nzur_f32(h ? hi32(w) : lo32(w));
+                        y[h] = k == 0 ? v : y[h] * v;
+                    }
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h) p[gi][h] = -(1.0f * plog(y[h]));
+            }
+        }
+        float v[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const float x = p[0][h] / (p[0][h] + p[1][h]);
+            v[h] = ratio != 0.0f ? (ratio * x) / (1.0f - x) : x;
+        }
+        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
+    }
+};
+
 // ====================================================== rejection policies ===
 // W: lanes per packet; SW: words per slot (4 full draw, 2 half draw);
 // K: slots per candidate; eval(w, h, v, acc): lane 2p+h of a candidate from

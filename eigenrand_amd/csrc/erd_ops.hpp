@@ -102,6 +102,43 @@ __device__ void op_fixed(Eng& e, uint64_t* carry, int64_t per, const D& d, int l
     e.idx = pos;
 }
 
+// op_fixed with a per-launch draw count C (1 <= C <= MAXC + 1): D::emit gets
+// fetch(c), the tempered word of draw c of its unit, instead of a word array.
+template <int MAXC, class D>
+__device__ void op_fixed_rt(Eng& e, uint64_t* carry, int64_t per, int C, const D& d, int lane) {
+    const int64_t units = D::units(per);
+    if (units == 0) return;
+    int pos = e.idx;
+    int nc = 0;
+    int64_t done = 0;
+    for (;;) {
+        if (pos >= kN) {
+            twist_lds(e.mt, lane);
+            pos = 0;
+            e.twisted = true;
+        }
+        const int avail = nc + (kN - pos);
+        const int64_t can = (int64_t)(avail / C);
+        const int nu = (int)(units - done < can ? units - done : can);
+        for (int t = lane; t < nu * kLanes; t += 64) {
+            const int u = t >> 2, l = t & 3;
+            d.emit(per, done + u, l, [&](int c) { return slot_word<kLanes>(e.mt, carry, nc, pos, u * C + c, l); });
+        }
+        const int consumed = nu * C;
+        done += nu;
+        if (done == units) {
+            pos = pos + consumed - nc;
+            break;
+        }
+        // Keep the draws of the unit that straddles the block boundary.
+        carry_tail<kLanes, MAXC>(e.mt, carry, nc, pos, consumed, avail, lane);
+        nc = avail - consumed;
+        pos = kN;
+    }
+    __syncthreads();
+    e.idx = pos;
+}
+
 // ------------------------------------------------------------- half state ---
 // Position of a half-draw consumer: block slot `pos` (in half draws) plus an
 // optional carried (already tempered) slot holding the engine's cached half.
@@ -615,9 +652,10 @@ constexpr int kKnuthStage = kKnuthSlots * 4;
 constexpr int kLpSlots = 128;
 constexpr int kLpFrac = 19;
 constexpr int32_t kLpMargin = 256;
-// Thresholds e^-mean below this (mean above about 2.3, where a packet spans
-// about seven slots or more) take the prefix path.
-constexpr float kKnuthPrefixThr = 0.1f;
+// Thresholds e^-mean below this (mean above 4, where a packet spans about
+// nine slots or more and the wide path's 64 speculative starts are mostly
+// wasted; measured crossover of the two paths) take the prefix path.
+constexpr float kKnuthPrefixThr = 0.0183156389f;  // e^-4
 
 // Tempered word q of window slot v: the carried half (v < nc), block slot
 // pos + v - nc, or past the block end a slot of the next block, computed
@@ -863,25 +901,47 @@ __device__ void op_knuth(Eng& e, uint64_t* carry, float* ustage, int32_t* o, int
             }
             const int m = max(max(s0, s1), max(s2, s3));
             const int cons = m < iters ? m + 1 : 0;  // 0: runs past the window
-            uint64_t chain = 0;
-            int at = 0;
-            int64_t n = 0;
-            const int64_t room = T - pk;
-            slow = false;
-            while (at < min(kKnuthStarts, S) && n < room) {
-                const int c = __builtin_amdgcn_readlane(cons, at);
-                if (c == 0) {
-                    slow = true;
-                    break;
-                }
-                chain |= 1ull << at;
-                ++n;
-                at += c;
+            // The chain from start 0, by pointer jumping instead of a walk.
+            // Position v < L (L = starts in this window) is a start whose next
+            // start is v + cons[v]; positions >= L end the chain, and so does a
+            // start whose packet runs past the window (marked kOvf: the walk
+            // stops before it and the one-slot loop finishes that packet).
+            // Thread h ends with pos = next^h(0), the start of chain packet h
+            // (next^a and next^b commute, so the bits of h apply in any order).
+            constexpr int kOvf = 0x100;
+            const int L = min(kKnuthStarts, S);
+            auto lookup = [&](int table, int v) {  // table[v] for a start v; v itself otherwise
+                const int t = __builtin_amdgcn_ds_bpermute(4 * (v & 63), table);
+                return v < L ? t : v;
+            };
+            int jmp = lane < L ? (cons ? lane + cons : (lane | kOvf)) : lane;  // next^(2^k)[lane]
+            int pos = 0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int moved = lookup(jmp, pos);
+                if ((lane >> k) & 1) pos = moved;
+                jmp = lookup(jmp, jmp);
             }
-            if ((chain >> lane) & 1ull) {
-                const int64_t q = pk + __popcll(chain & lanemask_lt(lane));
-                *reinterpret_cast<int4*>(o + 4 * q) =
-                    flip ? make_int4(trials - s0, trials - s1, trials - s2, trials - s3) : make_int4(s0, s1, s2, s3);
+            // a chain packet is a start whose own packet fits (a start with
+            // cons 0 is where the walk stops); the live ones form a prefix
+            const int cpos = __builtin_amdgcn_ds_bpermute(4 * (pos & 63), cons);
+            const int64_t room = T - pk;
+            const int nch = __popcll(__ballot(pos < L && cpos > 0));
+            const int n = (int)(room < nch ? room : nch);
+            int at = 0;  // slots consumed: just past chain packet n-1
+            if (n > 0) {
+                const int last = __builtin_amdgcn_readlane(pos, n - 1);
+                at = last + __builtin_amdgcn_readlane(cons, last);
+            }
+            // the walk meets a start whose packet runs past the window
+            slow = n < room && at < L && __builtin_amdgcn_readlane(cons, at) == 0;
+            // chain packet h takes the counts of the thread at its start
+            const int packed = s0 | (s1 << 8) | (s2 << 16) | (s3 << 24);
+            const int got = __builtin_amdgcn_ds_bpermute(4 * (pos & 63), packed);
+            if (lane < n) {
+                const int g0 = got & 0xFF, g1 = (got >> 8) & 0xFF, g2 = (got >> 16) & 0xFF, g3 = (got >> 24) & 0xFF;
+                *reinterpret_cast<int4*>(o + 4 * (pk + lane)) =
+                    flip ? make_int4(trials - g0, trials - g1, trials - g2, trials - g3) : make_int4(g0, g1, g2, g3);
             }
             pk += n;
             __syncthreads();  // ustage is rewritten by the next window

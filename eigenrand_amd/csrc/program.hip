@@ -13,6 +13,8 @@
 
 namespace erd {
 
+constexpr int kBetaMaxCarry = 37;  // a + b - 1 draws of an integer-shape beta unit
+
 union ProgramLds {
     uint64_t carry_fixed[18 * kLanes];  // GammaIntF32<19>: 18 straddling draws
     RejectLds<GammaGS> gs;
@@ -26,6 +28,7 @@ union ProgramLds {
     struct {
         uint64_t carry[2 * kLanes];
     } vop;
+    uint64_t carry_beta[kBetaMaxCarry * kLanes];  // integer-shape beta: draws straddling a twist
     struct {
         uint64_t carry[2 * kLanes];
         alignas(16) float u[kKnuthStage];
@@ -98,6 +101,12 @@ __device__ void run_beta(Eng& e, ProgramLds& s, BetaLds& bl, float* out, int64_t
     if (a < 1.0f && b < 1.0f) {
         BetaJohnk pol{1 / a, 1 / b, ratio};
         op_reject(e, s.gt_beta, out, n, pol, status, lane);
+        return;
+    }
+    const int ia = (int)a, ib = (int)b;
+    if ((float)ia == a && (float)ib == b && ia >= 1 && ib >= 1 && ia < 20 && ib < 20) {
+        // both gammas take a fixed number of draws: packets are independent
+        op_fixed_rt<kBetaMaxCarry>(e, s.carry_beta, n, ia + ib, BetaIntF32{out, ia, ib, ratio}, lane);
         return;
     }
     if (lane < 2) bl.q[lane].count = 0;

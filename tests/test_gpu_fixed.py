@@ -367,7 +367,10 @@ def test_uniform_int_rejects_bad_ranges(er):
 
 
 @pytest.mark.parametrize("a,b,fisher", [(0.4, 0.7, False), (2.0, 3.0, False), (0.5, 4.5, False), (1.0, 1.0, False),
-                                        (3.3, 0.2, False), (7.0, 2.5, True), (0.6, 1.2, True)])
+                                        (3.3, 0.2, False), (7.0, 2.5, True), (0.6, 1.2, True),
+                                        # integer shapes: fixed draws per packet (a + b up to 38)
+                                        (1.0, 4.0, False), (19.0, 19.0, False), (5.0, 1.0, False),
+                                        (4.0, 6.0, True), (2.0, 38.0, True)])
 def test_beta_fisher_f_match_oracle(er, orc, a, b, fisher):
     """BetaGen in both regimes (Johnk; GammaGen pairs in GS, integer,
     exponential and tangent regimes) and FisherFGen."""
