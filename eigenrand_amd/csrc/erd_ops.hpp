@@ -956,32 +956,48 @@ __device__ void op_knuth(Eng& e, uint64_t* carry, float* ustage, int32_t* o, int
     cursor_end(e, cur);
 }
 
-// BinomialGen, trials < 25 (Discrete.h:996-1006): up to 16 independent half
-// draws compared at once, counted per lane with a ballot.
+// BinomialGen, trials < 25 (Discrete.h:996-1006): packet k counts, per lane,
+// the half draws [k*n, k*n+n) below p, so packets are independent: each thread
+// takes a packet's two lanes of one u64 word. A packet straddling a twist
+// keeps up to 23 slots in the carry area.
+constexpr int kBinSmallMaxCarry = 23;
 __device__ void op_binomial_small(Eng& e, uint64_t* carry, int32_t* o, int64_t per, int32_t trials,
                                   float pp, int lane) {
     const int64_t T = per / 4;
+    const int S = trials;  // half draws per packet, < 25
     if (T == 0) return;
-    const int i = lane >> 2, j = lane & 3;
-    HalfCursor cur = cursor_begin(e, carry, lane);
-    const uint64_t jmask = 0x1111111111111111ull << j;
-    for (int64_t pk = 0; pk < T; ++pk) {
-        int32_t res = trials;
-        int left = trials;
-        while (left > 0) {
-            cur.ensure(1, lane);
-            int take = left < 16 ? left : 16;
-            const int a = cur.avail();
-            take = take < a ? take : a;
-            bool c = false;
-            if (i < take) c = pp <= ur_f32(cur.word32(i, j));
-            res -= __popcll(__ballot(c) & jmask);
-            cur.advance(take);
-            left -= take;
-        }
-        if (lane < 4) o[4 * pk + lane] = res;
+    if (S == 0) {  // no draws: every count is 0
+        for (int64_t i = lane; i < 4 * T; i += 64) o[i] = 0;
+        return;
     }
-    cursor_end(e, cur);
+    Cursor<2, kBinSmallMaxCarry> cur;
+    cur.mt = e.mt;
+    cur.carry = carry;
+    cur.used = false;
+    cur.twisted = &e.twisted;
+    half_begin(e, carry, cur.pos, cur.nc, lane);
+    int64_t done = 0;
+    while (done < T) {
+        cur.ensure(S, lane);
+        const int64_t can = cur.avail() / S;
+        const int take = (int)(T - done < can ? T - done : can);
+        // packet q takes slots [q*S, q*S+S); thread v counts lanes 2l, 2l+1
+        for (int v = lane; v < 2 * take; v += 64) {
+            const int q = v >> 1, l = v & 1;
+            int32_t r0 = S, r1 = S;
+            for (int k = 0; k < S; ++k) {
+                const uint64_t w = cur.word(q * S + k, l);
+                r0 -= (pp <= ur_f32(lo32(w))) ? 1 : 0;
+                r1 -= (pp <= ur_f32(hi32(w))) ? 1 : 0;
+            }
+            *reinterpret_cast<int2*>(o + 4 * (done + q) + 2 * l) = make_int2(r0, r1);
+        }
+        __syncthreads();  // carried slots are read before ensure() rewrites them
+        cur.advance(take * S);
+        done += take;
+    }
+    __syncthreads();
+    half_end(e, cur.pos, cur.nc, cur.used);
 }
 
 // BinomialVGen<int32_t>::packetOp (Discrete.h:1124-1192). Tangent iterations

@@ -62,7 +62,7 @@ __global__ __launch_bounds__(64) void k_binomial_small(EngineState* __restrict__
                                                        int32_t* __restrict__ out, int64_t per,
                                                        int32_t trials, float pp) {
     __shared__ uint64_t mt[kWords];
-    __shared__ uint64_t carry[2];
+    __shared__ uint64_t carry[2 * kBinSmallMaxCarry];
     const int lane = threadIdx.x;
     const int64_t g = blockIdx.x;
     if (per / 4 == 0) return;

@@ -28,7 +28,8 @@ union ProgramLds {
     struct {
         uint64_t carry[2 * kLanes];
     } vop;
-    uint64_t carry_beta[kBetaMaxCarry * kLanes];  // integer-shape beta: draws straddling a twist
+    // integer-shape beta: draws straddling a twist; small-n binomial: up to 23 half slots
+    uint64_t carry_beta[kBetaMaxCarry * kLanes];
     struct {
         uint64_t carry[2 * kLanes];
         alignas(16) float u[kKnuthStage];
@@ -301,7 +302,7 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
             const int32_t trials = sg.i0;
             const int flip = (sg.d[0] == sg.d[1]) ? 0 : 1;
             if (trials < 25) {
-                op_binomial_small(e, s.vop.carry, (int32_t*)sg.out, n, trials, (float)sg.d[0], lane);
+                op_binomial_small(e, s.carry_beta, (int32_t*)sg.out, n, trials, (float)sg.d[0], lane);
             } else if (sg.d[2] < 1.0) {
                 op_knuth(e, s.knuth.carry, s.knuth.u, (int32_t*)sg.out, n, (float)sg.d[7], trials, flip, status, lane);
             } else {
