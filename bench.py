@@ -363,9 +363,14 @@ def mv_sweep(emit_line):
         torch.cuda.synchronize()
         return r, a.elapsed_time(b)
 
+    def flat_dev(mats):
+        return torch.from_numpy(np.concatenate([m.reshape(-1, order="F") for m in mats])).cuda()
+
     dims = rng.integers(4, 257, 4096)
     covs = covs_for(dims)
-    gen, ms_f = once(lambda: er.MvNormalBatch(covs))
+    _, ms_fh = once(lambda: er.MvNormalBatch(covs))
+    covs_dev = flat_dev(covs)
+    gen, ms_f = once(lambda: er.MvNormalBatch(covs_dev, dims=dims))
     urng = er.Vmt19937_64(0, n=4096, stride=4)
     n = 1024
     out = torch.empty(int(dims.sum()) * n, dtype=torch.float32, device="cuda")
@@ -376,14 +381,14 @@ def mv_sweep(emit_line):
                "gsamples_per_s": round(elems / ms / 1e6, 3),
                "hbm_gbs": round((4 * elems + read) / ms / 1e6, 1),
                "hbm_frac": round((4 * elems + read) / ms / 1e6 / HBM_PEAK_GBS, 4),
-               "factorise_ms": round(ms_f, 3)})
-    del gen, out, covs
+               "factorise_ms": round(ms_f, 3), "factorise_from_host_ms": round(ms_fh, 3)})
+    del gen, out, covs, covs_dev
     wd = rng.integers(4, 257, 512)
-    scales = covs_for(wd)
+    scales = flat_dev(covs_for(wd))
     wrng = er.Vmt19937_64(1 << 20, n=512, stride=4)
     wout = torch.empty(int((wd.astype(np.int64) ** 2).sum()) * 64, dtype=torch.float32, device="cuda")
     for name, inverse in (("Wishart", False), ("InvWishart", True)):
-        wgen, ms_f = once(lambda: er.WishartBatch(scales, wd + 3, inverse=inverse))
+        wgen, ms_f = once(lambda: er.WishartBatch(scales, wd + 3, inverse=inverse, dims=wd))
         ms = timed(lambda: wgen.generate(wrng, 64, out=wout, flat=True), reps=3)
         elems = wout.numel()
         emit_line({"sweep": f"{name} batch generate", "instances": 512, "elements": elems, "ms": round(ms, 4),

@@ -637,10 +637,21 @@ class MvFactors:
     computed once on the device like the generators' constructors
     (MvNormal.h:20-38, 86-98, 196-215) and reused by every generate()."""
 
-    def __init__(self, mats, stream=None, inverse=False):
-        n = len(mats)
-        self.dims = np.array([np.asarray(c).shape[0] for c in mats], dtype=np.int32)
-        a = _concat_f64(mats)
+    def __init__(self, mats, stream=None, inverse=False, dims=None):
+        if dims is None:
+            n = len(mats)
+            self.dims = np.array([np.asarray(c).shape[0] for c in mats], dtype=np.int32)
+            a = _concat_f64(mats)
+        else:
+            # mats: one flat float64 CUDA tensor, the d_i x d_i matrices back
+            # to back, each column-major (no host copy)
+            self.dims = np.asarray(dims, dtype=np.int32).ravel().copy()
+            n = int(self.dims.size)
+            need = int((self.dims.astype(np.int64) ** 2).sum())
+            if (not hasattr(mats, "is_cuda") or not mats.is_cuda or mats.dtype != _torch().float64
+                    or not mats.is_contiguous() or mats.numel() != need):
+                raise ValueError(f"flat covariances: a contiguous float64 CUDA tensor of {need} elements")
+            a = mats
         self.info = np.zeros(n, dtype=np.int32)
         h = C.c_void_p()
         _lib.call("erd_mv_factors_create_inverse" if inverse else "erd_mv_factors_create", C.byref(h), n,
@@ -667,8 +678,8 @@ class MvNormalBatch:
     """Many MvNormalGen<float>(mean_i, cov_i) at once; generate() runs
     instance i on engine i of `urng`, reusing the factors."""
 
-    def __init__(self, covs, means=None, stream=None):
-        self.factors = MvFactors(covs, stream)
+    def __init__(self, covs, means=None, stream=None, dims=None):
+        self.factors = MvFactors(covs, stream, dims=dims)
         self.mean = None
         if means is not None:
             self.mean = _torch().from_numpy(
@@ -703,10 +714,10 @@ class WishartBatch:
     """Many WishartGen<float>(df_i, scale_i) at once (factors reused);
     InvWishartGen with inverse=True (factors of the inverse scales)."""
 
-    def __init__(self, scales, df, stream=None, inverse=False):
+    def __init__(self, scales, df, stream=None, inverse=False, dims=None):
         self.inverse = inverse
-        self.factors = MvFactors(scales, stream, inverse=inverse)
-        self.df = np.broadcast_to(np.asarray(df, dtype=np.int64), (len(scales),)).copy()
+        self.factors = MvFactors(scales, stream, inverse=inverse, dims=dims)
+        self.df = np.broadcast_to(np.asarray(df, dtype=np.int64), (len(self.factors),)).copy()
 
     def generate(self, urng, samples, out=None, stream=None, flat=False):
         torch = _torch()

@@ -25,8 +25,11 @@ struct MvTile {
     int64_t col0;
 };
 
+// FP64 Cholesky of n instances; dmax = the largest d (host value): up to
+// kCholBlkMaxDim the blocked LDS-panel kernel, above it the unblocked one.
+constexpr int kCholBlkMaxDim = 512;
 hipError_t launch_cholesky_f64(double* a, const int64_t* off, const int32_t* dims, int32_t* info,
-                               int n_inst, hipStream_t s);
+                               int n_inst, int dmax, hipStream_t s);
 hipError_t launch_split_l(const double* l64, const int64_t* off, const int32_t* dims,
                           const int64_t* split_off, uint16_t* hi, uint16_t* lo, float* l32, int n_inst,
                           int max_dp, hipStream_t s);

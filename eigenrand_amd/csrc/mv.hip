@@ -62,6 +62,101 @@ __global__ __launch_bounds__(256) void k_cholesky_f64(double* __restrict__ a,
     if (tid == 0) info[inst] = bad;
 }
 
+// Blocked left-looking Cholesky for d <= kCholBlkMaxDim: 32-column panels in
+// LDS. For panel j0: load A[j0:d, j0:j0+32]; subtract L[j0:d, 0:j0] *
+// L[j0:j0+32, 0:j0]^T (thread t takes panel row t with 32 FP64 accumulators;
+// the 32 x 32 tiles of the other factor are staged in LDS); factor the
+// tall-skinny panel in LDS (pivot, column scale, rank-1 updates of the panel's
+// later columns); write it back with the strictly upper part zeroed. FP64
+// throughout, the same factor as k_cholesky_f64 up to summation order.
+// Dynamic LDS: (dmax * 32 + 32 * 32) doubles.
+__global__ __launch_bounds__(256) void k_cholesky_f64_blk(double* __restrict__ a,
+                                                          const int64_t* __restrict__ off,
+                                                          const int32_t* __restrict__ dims,
+                                                          int32_t* __restrict__ info, int dmax) {
+    extern __shared__ double lds[];
+    __shared__ double ljj_s;
+    __shared__ int bad;
+    constexpr int NB = 32;
+    const int inst = blockIdx.x;
+    const int d = dims[inst];
+    double* A = a + off[inst];
+    double* P = lds;                    // panel, column-major, leading dimension R
+    double* B = lds + (int64_t)dmax * NB;  // [kk][c] tile of L[j0 + c][k0 + kk]
+    const int tid = threadIdx.x;
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int j0 = 0; j0 < d && !bad; j0 += NB) {
+        const int w = min(NB, d - j0), R = d - j0;
+        for (int t = tid; t < w * R; t += 256) {
+            const int c = t / R, i = t - c * R;
+            P[c * R + i] = A[(int64_t)(j0 + c) * d + j0 + i];
+        }
+        // update from the finished columns k < j0
+        for (int i0 = 0; i0 < R; i0 += 256) {
+            const int i = i0 + tid;
+            const bool have = i < R;
+            double acc[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) acc[c] = 0.0;
+            for (int k0 = 0; k0 < j0; k0 += NB) {
+                const int kw = min(NB, j0 - k0);
+                __syncthreads();
+                for (int t = tid; t < kw * NB; t += 256) {
+                    const int kk = t / NB, c = t - kk * NB;
+                    B[kk * NB + c] = c < w ? A[(int64_t)(k0 + kk) * d + j0 + c] : 0.0;
+                }
+                __syncthreads();
+                for (int kk = 0; kk < kw; ++kk) {
+                    const double av = have ? A[(int64_t)(k0 + kk) * d + j0 + i] : 0.0;
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) acc[c] = fma(av, B[kk * NB + c], acc[c]);
+                }
+            }
+            __syncthreads();
+            if (have) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+                    if (c < w) P[c * R + i] -= acc[c];
+            }
+        }
+        __syncthreads();
+        // factor the panel
+        for (int c = 0; c < w; ++c) {
+            if (tid == 0) {
+                const double ajj = P[c * R + c];
+                if (!(ajj > 0.0)) {
+                    bad = j0 + c + 1;
+                    ljj_s = 1.0;
+                } else {
+                    ljj_s = sqrt(ajj);
+                    P[c * R + c] = ljj_s;
+                }
+            }
+            __syncthreads();
+            if (bad) break;
+            const double inv = 1.0 / ljj_s;
+            for (int i = c + 1 + tid; i < R; i += 256) P[c * R + i] *= inv;
+            __syncthreads();
+            const int rest = w - c - 1;
+            for (int t = tid; t < rest * R; t += 256) {
+                const int m = c + 1 + t / R, i = t % R;
+                if (i >= m) P[m * R + i] -= P[c * R + i] * P[c * R + m];
+            }
+            __syncthreads();
+        }
+        if (bad) break;
+        // write back: column j0 + c, rows j0 + c .. d-1 from the panel, rows above zeroed
+        for (int t = tid; t < w * d; t += 256) {
+            const int c = t / d, row = t - c * d;
+            const int col = j0 + c;
+            A[(int64_t)col * d + row] = row < col ? 0.0 : P[c * R + (row - j0)];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) info[inst] = bad;
+}
+
 // -------------------------------------------------------- split-BF16 ---
 __device__ __forceinline__ uint16_t bf16_rne(float x) {
     const uint32_t u = __float_as_uint(x);
@@ -224,8 +319,16 @@ hipError_t launch_lz_gemm(int variant, int grid, const MvTile* tiles, int32_t n_
 }
 
 hipError_t launch_cholesky_f64(double* a, const int64_t* off, const int32_t* dims, int32_t* info,
-                               int n_inst, hipStream_t s) {
+                               int n_inst, int dmax, hipStream_t s) {
     if (n_inst <= 0) return hipSuccess;
+    if (dmax <= kCholBlkMaxDim) {
+        const size_t lds = sizeof(double) * ((size_t)dmax * 32 + 32 * 32);
+        hipError_t err = hipFuncSetAttribute((const void*)k_cholesky_f64_blk,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL(k_cholesky_f64_blk, dim3(n_inst), dim3(256), lds, s, a, off, dims, info, dmax);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_cholesky_f64, dim3(n_inst), dim3(256), 0, s, a, off, dims, info);
     return hipGetLastError();
 }

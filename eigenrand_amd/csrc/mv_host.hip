@@ -175,7 +175,9 @@ int factor_and_split(const std::vector<int32_t>& dims, double* d_a, const std::v
     int64_t* d_off;
     int32_t *d_dims, *d_info;
     if (sc.upload(&d_off, off) || sc.upload(&d_dims, dims) || sc.alloc(&d_info, n)) return ERD_ERR_HIP;
-    if (launch_cholesky_f64(d_a, d_off, d_dims, d_info, n, s) != hipSuccess) return ERD_ERR_HIP;
+    int dmax = 0;
+    for (int32_t dd : dims) dmax = dd > dmax ? dd : dmax;
+    if (launch_cholesky_f64(d_a, d_off, d_dims, d_info, n, dmax, s) != hipSuccess) return ERD_ERR_HIP;
     std::vector<int32_t> info(n);
     if (hipMemcpyAsync(info.data(), d_info, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)

@@ -36,6 +36,31 @@ def test_cholesky_reports_not_psd(er):
     assert rc == 3 and info[0] == 0 and info[1] == 2
 
 
+def test_cholesky_unblocked_path_above_512(er):
+    # a batch whose largest d exceeds 512 takes the unblocked kernel for all
+    rng = np.random.default_rng(5)
+    mats = [rand_cov(rng, d) for d in (600, 33)]
+    ls, info, rc = er.cholesky_batch(mats)
+    assert rc == 0 and not info.any()
+    for m, l in zip(mats, ls):
+        ref = np.linalg.cholesky(m)
+        assert np.allclose(l, ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+        assert np.allclose(np.triu(l, 1), 0)
+
+
+def test_cholesky_reports_late_pivot(er):
+    # M = U D U^T with U unit lower triangular: the j-th Cholesky pivot is D[j],
+    # so D[49] = -1 fails at column 50 (1-based), in the second 32-wide panel
+    rng = np.random.default_rng(6)
+    d = 70
+    u = np.tril(rng.normal(0, 0.3, (d, d)), -1) + np.eye(d)
+    dg = np.full(d, 2.0)
+    dg[49] = -1.0
+    m = u @ np.diag(dg) @ u.T
+    _, info, rc = er.cholesky_batch([rand_cov(rng, 40), m])
+    assert rc == 3 and info[0] == 0 and info[1] == 50
+
+
 # split-BF16 (hi+lo) with FP32 accumulation: bf16 keeps 8 significant bits,
 # so |x - hi| <= 2^-8 |x| and the hi+lo residual is <= 2^-16 |x|; the two
 # residuals and the dropped lo*lo term give <= 3 * 2^-16 per product, so per
@@ -263,6 +288,27 @@ def test_factors_once_generate_many(er):
     w2 = er.wishart_batch(b, covs[:2], [d + 3 for d in dims[:2]], 8)
     for x, y in zip(w1, w2):
         assert_same(x.cpu().numpy(), y.cpu().numpy())
+
+
+def test_factors_from_device_resident_covariances(er):
+    """A flat float64 CUDA tensor plus dims gives the same factors (and so the
+    same samples) as the list of host matrices."""
+    import torch
+    rng = np.random.default_rng(78)
+    dims = [3, 33, 70]
+    covs = [rand_cov(rng, d) for d in dims]
+    flat = torch.from_numpy(np.concatenate([c.reshape(-1, order="F") for c in covs])).cuda()
+    a = er.Vmt19937_64(seeds=[1, 2, 3])
+    b = er.Vmt19937_64(seeds=[1, 2, 3])
+    for x, y in zip(er.MvNormalBatch(flat, dims=dims).generate(a, 24), er.MvNormalBatch(covs).generate(b, 24)):
+        assert_same(x.cpu().numpy(), y.cpu().numpy())
+    for inverse in (False, True):
+        w1 = er.WishartBatch(flat, [d + 2 for d in dims], inverse=inverse, dims=dims).generate(a, 8)
+        w2 = er.WishartBatch(covs, [d + 2 for d in dims], inverse=inverse).generate(b, 8)
+        for x, y in zip(w1, w2):
+            assert_same(x.cpu().numpy(), y.cpu().numpy())
+    with pytest.raises(ValueError):
+        er.MvNormalBatch(flat[:-1], dims=dims)
 
 
 def test_factors_report_not_psd(er):
