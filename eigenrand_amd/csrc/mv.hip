@@ -546,7 +546,7 @@ __global__ __launch_bounds__(256) void k_trtri_panel(const SyrkTile* __restrict_
                                                      const MvInstance* __restrict__ inst, float* __restrict__ u_out,
                                                      const int64_t* __restrict__ u_off) {
     __shared__ float X[kTrtriMaxDim][33];
-    __shared__ float Tb[32][33];
+    __shared__ float Tb[2][32][33];
     const SyrkTile tl = tiles[blockIdx.x];  // ta = panel J
     const MvInstance in = inst[tl.inst];
     const int d = in.d;
@@ -554,42 +554,70 @@ __global__ __launch_bounds__(256) void k_trtri_panel(const SyrkTile* __restrict_
     float* U = u_out + u_off[tl.inst] + (int64_t)tl.sample * d * d;
     const int J = tl.ta, nb = (d + 31) / 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // column tx, rows ty*4..ty*4+3
+    // Tiles are used in the order (I, K) = (J, J), (J+1, J), (J+1, J+1), ...:
+    // for each block row I the updates K = J..I-1, then the diagonal block
+    // (identity-padded past d). The next tile is fetched into registers while
+    // the current one is in use; Tb is double-buffered, so one barrier per
+    // tile keeps writers and readers apart.
+    float pre[4];
+    auto fetch = [&](int I, int K) {
+#pragma unroll
+        for (int q4 = 0; q4 < 4; ++q4) {
+            const int q = threadIdx.x + 256 * q4;
+            const int r = q & 31, kk = q >> 5;
+            const int row = I * 32 + r, col = K * 32 + kk;
+            pre[q4] = (row < d && col < d) ? T[(int64_t)col * d + row] : ((I == K && r == kk) ? 1.0f : 0.0f);
+        }
+    };
+    fetch(J, J);
+    int buf = 0;
     for (int I = J; I < nb; ++I) {
         float acc[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc[q] = (I == J && ty * 4 + q == tx) ? 1.0f : 0.0f;
-        for (int K = J; K < I; ++K) {
-            for (int q = threadIdx.x; q < 32 * 32; q += 256) {
-                const int r = q & 31, kk = q >> 5;
-                const int row = I * 32 + r, col = K * 32 + kk;
-                Tb[r][kk] = (row < d && col < d) ? T[(int64_t)col * d + row] : 0.0f;
+        for (int K = J; K <= I; ++K) {
+            float(*tb)[33] = Tb[buf];
+#pragma unroll
+            for (int q4 = 0; q4 < 4; ++q4) {
+                const int q = threadIdx.x + 256 * q4;
+                tb[q & 31][q >> 5] = pre[q4];
+            }
+            if (K == I) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) X[I * 32 + ty * 4 + q][tx] = acc[q];
             }
             __syncthreads();
+            const int nI = K < I ? I : I + 1, nK = K < I ? K + 1 : J;
+            if (nI < nb) fetch(nI, nK);
+            if (K < I) {
 #pragma unroll 8
-            for (int kk = 0; kk < 32; ++kk) {
-                const float xv = X[K * 32 + kk][tx];
+                for (int kk = 0; kk < 32; ++kk) {
+                    const float xv = X[K * 32 + kk][tx];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[q] = fmaf(-Tb[ty * 4 + q][kk], xv, acc[q]);
-            }
-            __syncthreads();
-        }
-        // diagonal block solve: column tx by forward substitution
-        for (int q = threadIdx.x; q < 32 * 32; q += 256) {
-            const int r = q & 31, kk = q >> 5;
-            const int row = I * 32 + r, col = I * 32 + kk;
-            Tb[r][kk] = (row < d && col < d) ? T[(int64_t)col * d + row] : (r == kk ? 1.0f : 0.0f);
-        }
+                    for (int q = 0; q < 4; ++q) acc[q] = fmaf(-tb[ty * 4 + q][kk], xv, acc[q]);
+                }
+            } else {
+                // diagonal block: column tx by forward substitution, in
+                // registers and fully unrolled (only the broadcast T reads
+                // touch LDS)
+                if (threadIdx.x < 32) {
+                    float v[32];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) X[I * 32 + ty * 4 + q][tx] = acc[q];
-        __syncthreads();
-        if (threadIdx.x < 32) {
-            for (int r = 0; r < 32; ++r) {
-                float v = X[I * 32 + r][tx];
-                for (int k = 0; k < r; ++k) v -= Tb[r][k] * X[I * 32 + k][tx];
-                X[I * 32 + r][tx] = v / Tb[r][r];
+                    for (int r = 0; r < 32; ++r) v[r] = X[I * 32 + r][tx];
+#pragma unroll
+                    for (int r = 0; r < 32; ++r) {
+                        float t = v[r];
+#pragma unroll
+                        for (int k = 0; k < r; ++k) t -= tb[r][k] * v[k];
+                        v[r] = t / tb[r][r];
+                    }
+#pragma unroll
+                    for (int r = 0; r < 32; ++r) X[I * 32 + r][tx] = v[r];
+                }
+                __syncthreads();
             }
+            buf ^= 1;
         }
-        __syncthreads();
     }
     // store the panel's columns (column-major U; zeros above the panel's rows)
     for (int q = threadIdx.x; q < 32 * d; q += 256) {
