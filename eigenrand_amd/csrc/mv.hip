@@ -467,12 +467,17 @@ __global__ __launch_bounds__(64) void k_dirichlet_norm(const int32_t* __restrict
     const int64_t s = s0 + lane;
     const int ns = (int)min<int64_t>(kDirTile, n - s0);
     float sum = 0.0f;
-    if (lane < ns)
+    // the loads are unrolled ahead of the adds, which stay in category order
+    if (lane < ns) {
+#pragma unroll 8
         for (int k = 0; k < K; ++k) sum += G[(int64_t)k * n + s];
+    }
     for (int k0 = 0; k0 < K; k0 += kDirTile) {
         const int nk = min(kDirTile, K - k0);
-        if (lane < ns)
+        if (lane < ns) {
+#pragma unroll 8
             for (int i = 0; i < nk; ++i) tile[i][lane] = G[(int64_t)(k0 + i) * n + s] / sum;
+        }
         __syncthreads();
         if (lane < nk)
             for (int j = 0; j < ns; ++j) O[(s0 + j) * K + k0 + lane] = tile[lane][j];
