@@ -53,7 +53,7 @@ hipError_t launch_spd_inverse_f64(const double* l, double* w, double* out, const
                                   int n_inst, hipStream_t s);
 // U = T^-1 per (instance, sample, 32-column panel ta); d <= kTrtriMaxDim.
 constexpr int kTrtriMaxDim = 512;
-hipError_t launch_trtri_panel(const SyrkTile* tiles, int n_tiles, const MvInstance* inst, float* u,
+hipError_t launch_trtri_panel(const SyrkTile* tiles, int n_tiles, int max_rows, const MvInstance* inst, float* u,
                               const int64_t* u_off, hipStream_t s);
 hipError_t launch_dirichlet_norm(const int32_t* dims, const int64_t* samples, const int64_t* g_off,
                                  const float* g, float* out, int n_inst, int64_t max_n, hipStream_t s);

@@ -550,7 +550,9 @@ __global__ __launch_bounds__(256) void k_spd_inverse_f64(const double* __restric
 __global__ __launch_bounds__(256) void k_trtri_panel(const SyrkTile* __restrict__ tiles,
                                                      const MvInstance* __restrict__ inst, float* __restrict__ u_out,
                                                      const int64_t* __restrict__ u_off) {
-    __shared__ float X[kTrtriMaxDim][33];
+    // X: the panel's rows J*32 .. nb*32-1 (row-relative), 33 floats a row;
+    // dynamic LDS sized for the launch's largest nb - J
+    extern __shared__ float X[];
     __shared__ float Tb[2][32][33];
     const SyrkTile tl = tiles[blockIdx.x];  // ta = panel J
     const MvInstance in = inst[tl.inst];
@@ -589,7 +591,7 @@ __global__ __launch_bounds__(256) void k_trtri_panel(const SyrkTile* __restrict_
             }
             if (K == I) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) X[I * 32 + ty * 4 + q][tx] = acc[q];
+                for (int q = 0; q < 4; ++q) X[((I - J) * 32 + ty * 4 + q) * 33 + tx] = acc[q];
             }
             __syncthreads();
             const int nI = K < I ? I : I + 1, nK = K < I ? K + 1 : J;
@@ -597,7 +599,7 @@ __global__ __launch_bounds__(256) void k_trtri_panel(const SyrkTile* __restrict_
             if (K < I) {
 #pragma unroll 8
                 for (int kk = 0; kk < 32; ++kk) {
-                    const float xv = X[K * 32 + kk][tx];
+                    const float xv = X[((K - J) * 32 + kk) * 33 + tx];
 #pragma unroll
                     for (int q = 0; q < 4; ++q) acc[q] = fmaf(-tb[ty * 4 + q][kk], xv, acc[q]);
                 }
@@ -608,7 +610,7 @@ __global__ __launch_bounds__(256) void k_trtri_panel(const SyrkTile* __restrict_
                 if (threadIdx.x < 32) {
                     float v[32];
 #pragma unroll
-                    for (int r = 0; r < 32; ++r) v[r] = X[I * 32 + r][tx];
+                    for (int r = 0; r < 32; ++r) v[r] = X[((I - J) * 32 + r) * 33 + tx];
 #pragma unroll
                     for (int r = 0; r < 32; ++r) {
                         float t = v[r];
@@ -617,7 +619,7 @@ __global__ __launch_bounds__(256) void k_trtri_panel(const SyrkTile* __restrict_
                         v[r] = t / tb[r][r];
                     }
 #pragma unroll
-                    for (int r = 0; r < 32; ++r) X[I * 32 + r][tx] = v[r];
+                    for (int r = 0; r < 32; ++r) X[((I - J) * 32 + r) * 33 + tx] = v[r];
                 }
                 __syncthreads();
             }
@@ -628,7 +630,7 @@ __global__ __launch_bounds__(256) void k_trtri_panel(const SyrkTile* __restrict_
     for (int q = threadIdx.x; q < 32 * d; q += 256) {
         const int row = q % d, c = q / d;
         const int col = J * 32 + c;
-        if (col < d) U[(int64_t)col * d + row] = row < J * 32 ? 0.0f : X[row][c];
+        if (col < d) U[(int64_t)col * d + row] = row < J * 32 ? 0.0f : X[(row - J * 32) * 33 + c];
     }
 }
 
@@ -639,10 +641,15 @@ hipError_t launch_spd_inverse_f64(const double* l, double* w, double* out, const
     return hipGetLastError();
 }
 
-hipError_t launch_trtri_panel(const SyrkTile* tiles, int n_tiles, const MvInstance* inst, float* u,
+hipError_t launch_trtri_panel(const SyrkTile* tiles, int n_tiles, int max_rows, const MvInstance* inst, float* u,
                               const int64_t* u_off, hipStream_t s) {
     if (n_tiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_trtri_panel, dim3((unsigned)n_tiles), dim3(256), 0, s, tiles, inst, u, u_off);
+    if (max_rows <= 0 || max_rows > kTrtriMaxDim || max_rows % 32) return hipErrorInvalidValue;
+    const size_t lds = sizeof(float) * (size_t)max_rows * 33;
+    hipError_t err = hipFuncSetAttribute((const void*)k_trtri_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)lds);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_trtri_panel, dim3((unsigned)n_tiles), dim3(256), lds, s, tiles, inst, u, u_off);
     return hipGetLastError();
 }
 

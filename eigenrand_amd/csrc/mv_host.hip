@@ -433,16 +433,25 @@ static int wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64
     if (inverse) {
         // InvWishartGen (MvNormal.h:427-440): U_j = T_j^-1 into amat, then
         // W_j = U_j^T U_j from U.
+        // panel p of an instance with nb blocks solves nb - p block rows: the
+        // longest panels go first, and the LDS panel is sized for them
         std::vector<SyrkTile> panels;
-        for (int64_t i = 0; i < n_inst; ++i)
+        int max_rows = 0;
+        for (int64_t i = 0; i < n_inst; ++i) {
+            const int nb = (vd[i] + 31) / 32;
+            max_rows = std::max(max_rows, 32 * nb);
             for (int64_t j = 0; j < samples[i]; ++j)
-                for (int p = 0; p < (vd[i] + 31) / 32; ++p) panels.push_back(SyrkTile{(int32_t)i, (int32_t)j, p, 0});
+                for (int p = 0; p < nb; ++p) panels.push_back(SyrkTile{(int32_t)i, (int32_t)j, p, 0});
+        }
+        std::stable_sort(panels.begin(), panels.end(), [&](const SyrkTile& a, const SyrkTile& b) {
+            return (vd[a.inst] + 31) / 32 - a.ta > (vd[b.inst] + 31) / 32 - b.ta;
+        });
         std::vector<MvInstance> uinst = inst;
         for (int64_t i = 0; i < n_inst; ++i) uinst[i].x = amat + aoff[i];
         SyrkTile* d_panels;
         MvInstance* d_uinst;
         if (sc.upload(&d_panels, panels) || sc.upload(&d_uinst, uinst)) return ERD_ERR_HIP;
-        if (launch_trtri_panel(d_panels, (int)panels.size(), d_inst, amat, d_aoff, s) != hipSuccess)
+        if (launch_trtri_panel(d_panels, (int)panels.size(), max_rows, d_inst, amat, d_aoff, s) != hipSuccess)
             return ERD_ERR_HIP;
         if (launch_syrk_tt(d_tiles, (int)tiles.size(), d_uinst, out, d_aoff, s, true) != hipSuccess)
             return ERD_ERR_HIP;

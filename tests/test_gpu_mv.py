@@ -151,6 +151,24 @@ def test_invwishart_batch_against_fp64(er, orc):
             assert np.array_equal(got, got.T)
 
 
+def test_invwishart_large_dims(er, orc):
+    """Panels up to 16 blocks (d = 300, 512) next to small ones in one launch."""
+    import orc_mv
+    rng = np.random.default_rng(12)
+    dims = [300, 5, 512]
+    scales = [rand_cov(rng, d) for d in dims]
+    df = [d + 10 for d in dims]
+    seeds = [31, 41, 51]
+    ws = er.invwishart_batch(er.Vmt19937_64(seeds=seeds), scales, df, 8)
+    for i, d in enumerate(dims):
+        w64, _ = orc_mv.invwishart(orc.Engines(seeds=[seeds[i]]), df[i], scales[i], 8)
+        w = ws[i].cpu().numpy().astype(np.float64)
+        for j in (0, 7):
+            ref = w64[:, j * d:(j + 1) * d]
+            got = w[:, j * d:(j + 1) * d]
+            assert np.abs(got - ref).max() <= 2.0 ** -10 * np.abs(ref).max(), (d, j)
+
+
 def test_invwishart_mean(er):
     """E[W] = scale / (df - d - 1) for the inverse Wishart, within 10%."""
     rng = np.random.default_rng(9)
