@@ -233,29 +233,63 @@ __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tile
         const uint16_t* Lh = in.lhi;
         const uint16_t* Ll = in.llo;
         const float* Z = in.z;
-        for (int k0 = 0; k0 < kmax; k0 += KC) {
-            // stage L[row0 + r, k0:k0+32] (row-major bf16, 16 B per thread-step)
-            for (int q = tid; q < R * 16 * (KC / 8); q += 256) {
-                const int r = q / (KC / 8), kk = (q % (KC / 8)) * 8;
-                uint4 vh = {0, 0, 0, 0}, vl = {0, 0, 0, 0};
-                if (k0 + kk < dp) {
-                    vh = *reinterpret_cast<const uint4*>(Lh + (int64_t)(row0 + r) * dp + k0 + kk);
-                    vl = *reinterpret_cast<const uint4*>(Ll + (int64_t)(row0 + r) * dp + k0 + kk);
+        // The next chunk's L and Z are loaded into registers while the
+        // current chunk's MFMAs run; the registers go to LDS (Z split into
+        // hi/lo) between the two barriers of a step.
+        constexpr int LQ = (MAXR * 16 * (KC / 8) + 255) / 256;  // L uint4 pairs per thread
+        constexpr int ZQ = (NT * (KC / 8)) / 256;                // Z 8-float runs per thread
+        static_assert((NT * (KC / 8)) % 256 == 0, "Z staging must cover the tile exactly");
+        const int lq_n = R * 16 * (KC / 8);
+        uint4 ph[LQ], pl[LQ];
+        float pz[ZQ][8];
+        auto fetch = [&](int k0) {
+#pragma unroll
+            for (int i = 0; i < LQ; ++i) {
+                const int q = tid + 256 * i;
+                ph[i] = uint4{0, 0, 0, 0};
+                pl[i] = uint4{0, 0, 0, 0};
+                if (q < lq_n) {
+                    const int r = q / (KC / 8), kk = (q % (KC / 8)) * 8;
+                    if (k0 + kk < dp) {
+                        ph[i] = *reinterpret_cast<const uint4*>(Lh + (int64_t)(row0 + r) * dp + k0 + kk);
+                        pl[i] = *reinterpret_cast<const uint4*>(Ll + (int64_t)(row0 + r) * dp + k0 + kk);
+                    }
                 }
-                *reinterpret_cast<uint4*>(&la[0][r * KPAD + kk]) = vh;
-                *reinterpret_cast<uint4*>(&la[1][r * KPAD + kk]) = vl;
             }
-            // stage Z[k0:k0+32, c0:c0+NT] split into hi/lo, layout [col][k]
-            for (int q = tid; q < NT * (KC / 8); q += 256) {
+#pragma unroll
+            for (int i = 0; i < ZQ; ++i) {
+                const int q = tid + 256 * i;
                 const int c = q / (KC / 8), kk = (q % (KC / 8)) * 8;
                 const int64_t col = c0 + c;
-                uint16_t h[8], lw[8];
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const int k = k0 + kk + e;
-                    const float v = (k < d && col < n) ? Z[col * d + k] : 0.0f;
-                    h[e] = bf16_rne(v);
-                    lw[e] = bf16_rne(v - bf16_to_f(h[e]));
+                    pz[i][e] = (k < d && col < n) ? Z[col * d + k] : 0.0f;
+                }
+            }
+        };
+        fetch(0);
+        for (int k0 = 0; k0 < kmax; k0 += KC) {
+            // L[row0 + r, k0:k0+32] (row-major bf16) and Z[k0:k0+32, c0:c0+NT]
+            // split into hi/lo, layout [col][k]
+#pragma unroll
+            for (int i = 0; i < LQ; ++i) {
+                const int q = tid + 256 * i;
+                if (q < lq_n) {
+                    const int r = q / (KC / 8), kk = (q % (KC / 8)) * 8;
+                    *reinterpret_cast<uint4*>(&la[0][r * KPAD + kk]) = ph[i];
+                    *reinterpret_cast<uint4*>(&la[1][r * KPAD + kk]) = pl[i];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < ZQ; ++i) {
+                const int q = tid + 256 * i;
+                const int c = q / (KC / 8), kk = (q % (KC / 8)) * 8;
+                uint16_t h[8], lw[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    h[e] = bf16_rne(pz[i][e]);
+                    lw[e] = bf16_rne(pz[i][e] - bf16_to_f(h[e]));
                 }
                 uint4 vh, vl;
                 vh.x = h[0] | ((uint32_t)h[1] << 16); vh.y = h[2] | ((uint32_t)h[3] << 16);
@@ -266,6 +300,7 @@ __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tile
                 *reinterpret_cast<uint4*>(&zb[1][c * KPAD + kk]) = vl;
             }
             __syncthreads();
+            if (k0 + KC < kmax) fetch(k0 + KC);
 #pragma unroll
             for (int cb = 0; cb < NCB; ++cb) {
                 const int col = (wave * NCB + cb) * 16 + l16;
