@@ -41,6 +41,15 @@ bool autotune_enabled() {
     return !(v && v[0] == '0');
 }
 
+// ERD_LZ_VARIANT=k (0..kLzVariants-1) runs variant k for every bucket,
+// bypassing the tuner and its cache (read per call; tests use it to compare
+// the variants).
+int forced_variant() {
+    const char* v = std::getenv("ERD_LZ_VARIANT");
+    if (!v || v[0] < '0' || v[0] >= '0' + kLzVariants || v[1] != 0) return -1;
+    return v[0] - '0';
+}
+
 int bucket_of(int d) { return ((d + 15) & ~15) <= 64 ? 0 : ((d + 15) & ~15) <= 128 ? 1 : ((d + 15) & ~15) <= 256 ? 2 : 3; }
 
 void build_tiles(const std::vector<MvInstance>& inst, const std::vector<int>& which, int variant,
@@ -84,6 +93,7 @@ int plan_lz(const std::vector<MvInstance>& inst, std::vector<LzLaunch>& plan) {
             auto it = g_tuned.find(bkt);
             if (it != g_tuned.end()) v = it->second;
         }
+        if (forced_variant() >= 0) v = forced_variant();
         if (v < 0) {
             if (autotune_enabled()) return ERD_ERR_UNSUPPORTED;
             v = default_variant(bkt == 0 ? 64 : bkt == 1 ? 128 : 256);
@@ -114,6 +124,8 @@ int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t 
             auto it = g_tuned.find(kv.first);
             v = it != g_tuned.end() ? it->second : -1;
         }
+        const int forced = forced_variant();
+        if (forced >= 0) v = forced;
         if (v < 0 && autotune_enabled()) {
             float best = 1e30f;
             int best_v = default_variant(kv.first == 0 ? 64 : kv.first == 1 ? 128 : 256);

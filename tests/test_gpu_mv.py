@@ -93,6 +93,23 @@ def test_mvnormal_batch_against_fp64(er, orc, dims):
     print("max |X-X64| / (|L||Z|) =", worst)
 
 
+def test_lz_variants_give_identical_results(er, monkeypatch):
+    """The L*Z tuner may pick any of the four tile variants per dimension
+    bucket: each must give the same bits (same per-element k order)."""
+    rng = np.random.default_rng(21)
+    dims = [3, 16, 40, 100, 130, 256, 300]
+    covs = [rand_cov(rng, d) for d in dims]
+    means = [rng.normal(size=d).astype(np.float32) for d in dims]
+    outs = []
+    for v in range(4):
+        monkeypatch.setenv("ERD_LZ_VARIANT", str(v))
+        xs = er.mvnormal_batch(er.Vmt19937_64(seeds=list(range(7))), covs, [40, 8, 72, 64, 24, 16, 8], means)
+        outs.append([x.cpu().numpy() for x in xs])
+    for v in range(1, 4):
+        for a, b in zip(outs[0], outs[v]):
+            assert_same(a, b)
+
+
 def test_mvnormal_distribution_9d(er):
     """test_mv.cpp:98-122: 9-d normal, sample mean/cov within 10%."""
     rng = np.random.default_rng(9)
