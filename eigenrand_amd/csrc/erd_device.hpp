@@ -147,6 +147,28 @@ __device__ __forceinline__ void twist_lds(uint64_t* mt, int lane) {
     __syncthreads();
 }
 
+// twist_lds for a workgroup of several waves: every thread reaches the
+// barriers and wave 0 (tid < 64) does the work.
+__device__ __forceinline__ void twist_lds_wg(uint64_t* mt, int tid) {
+    constexpr int P1 = kM * kLanes;
+    constexpr int P2 = (kN - 1) * kLanes - P1;
+    if (tid < 64) {
+        twist_batch<0, 5>(mt, 0, P1, P1, tid);
+        twist_batch<5, 5>(mt, 0, P1, P1, tid);
+    }
+    __syncthreads();
+    if (tid < 64) {
+        twist_batch<0, 5>(mt, P1, P2, -P1, tid);
+        twist_batch<5, 5>(mt, P1, P2, -P1, tid);
+    }
+    __syncthreads();
+    if (tid < kLanes) {
+        int t = (kN - 1) * kLanes + tid;
+        mt[t] = twist_word(mt[t], mt[tid], mt[t - kM * kLanes]);
+    }
+    __syncthreads();
+}
+
 // Coalesced 16-byte copies of the 9984-byte word block between HBM and LDS.
 __device__ __forceinline__ void load_state(uint64_t* lds, const EngineState* st, int lane) {
     const uint4* src = reinterpret_cast<const uint4*>(st->mt);

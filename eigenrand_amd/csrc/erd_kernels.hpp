@@ -26,6 +26,15 @@ hipError_t launch_seed(EngineState* st, const uint64_t* d_seeds, int64_t n, hipS
 hipError_t launch_discard(EngineState* st, int64_t n, int64_t calls, hipStream_t s);
 
 // fill_fixed.hip
+// One engine's fill of `count` floats into `out` (count a multiple of 8).
+struct FillJob {
+    int32_t engine;
+    int32_t pad;
+    int64_t count;
+    float* out;
+};
+constexpr int kJobThreads = 256;
+hipError_t launch_std_normal_jobs(EngineState* st, const FillJob* jobs, int n_jobs, hipStream_t s);
 hipError_t launch_fill_u64(EngineState* st, int64_t n, uint64_t* out, int64_t per, hipStream_t s);
 // P8 adaptor fills: n Vmt engines (n/2 adaptors), per_v outputs per Vmt
 // engine; mode 0 u64, 1 u32, 2 uniform_real() floats.

@@ -102,6 +102,63 @@ __device__ void op_fixed(Eng& e, uint64_t* carry, int64_t per, const D& d, int l
     e.idx = pos;
 }
 
+// op_fixed for one engine served by a workgroup of NT threads (NT/64
+// waves): the units of a block are shared by all threads, wave 0 twists and
+// keeps the carried words. The items and their words are those of op_fixed,
+// so the outputs and the final engine state are identical.
+template <int NT, class D>
+__device__ void op_fixed_wg(Eng& e, uint64_t* carry, int64_t per, const D& d, int tid) {
+    constexpr int C = D::CALLS;
+    constexpr int MAXC = C > 1 ? C - 1 : 1;
+    const int64_t units = D::units(per);
+    if (units == 0) return;
+    const int wl = tid < 64 ? tid : (1 << 20);  // carry_tail lane: only wave 0 moves words
+    int pos = e.idx;
+    int nc = 0;
+    int64_t done = 0;
+    for (;;) {
+        if (pos >= kN) {
+            twist_lds_wg(e.mt, tid);
+            pos = 0;
+            e.twisted = true;
+        }
+        const int avail = nc + (kN - pos);
+        const int64_t can = (int64_t)(avail / C);
+        const int nu = (int)(units - done < can ? units - done : can);
+        int t = tid;
+        if (nc > 0) {
+            if (t < nu * kLanes) {
+                const int u = t >> 2, l = t & 3;
+                uint64_t w[C];
+#pragma unroll
+                for (int c = 0; c < C; ++c) w[c] = slot_word<kLanes>(e.mt, carry, nc, pos, u * C + c, l);
+                d.emit(per, done + u, l, w);
+            }
+            t += NT;
+        }
+        for (; t < nu * kLanes; t += NT) {
+            const int u = t >> 2, l = t & 3;
+            uint64_t w[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) w[c] = temper(e.mt[(pos - nc + u * C + c) * kLanes + l]);
+            d.emit(per, done + u, l, w);
+        }
+        const int consumed = nu * C;
+        done += nu;
+        if (done == units) {
+            pos = pos + consumed - nc;
+            break;
+        }
+        // carry_tail's barriers also end every wave's reads of this block
+        // before the next twist
+        carry_tail<kLanes, MAXC>(e.mt, carry, nc, pos, consumed, avail, wl);
+        nc = avail - consumed;
+        pos = kN;
+    }
+    __syncthreads();
+    e.idx = pos;
+}
+
 // op_fixed with a per-launch draw count C (1 <= C <= MAXC + 1): D::emit gets
 // fetch(c), the tempered word of draw c of its unit, instead of a word array.
 template <int MAXC, class D>

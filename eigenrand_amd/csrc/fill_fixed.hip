@@ -87,6 +87,31 @@ struct DiscreteI32 {
     }
 };
 
+// StdNormalGen fills of a list of engines, one workgroup of kJobThreads per
+// engine (job), so that one long fill on one engine (a multivariate
+// instance's Z) is not held to one wave's rate. Outputs and engine states
+// equal k_fill_fixed<NormalF32<0>> on each engine.
+__global__ __launch_bounds__(kJobThreads) void k_std_normal_jobs(EngineState* __restrict__ states,
+                                                                 const FillJob* __restrict__ jobs) {
+    __shared__ uint64_t mt[kWords];
+    __shared__ uint64_t carry[kLanes];
+    const int tid = threadIdx.x;
+    const FillJob jb = jobs[blockIdx.x];
+    if (NormalF32<0>::units(jb.count) == 0) return;
+    const int wl = tid < 64 ? tid : (1 << 20);  // state copies by wave 0
+    Eng e;
+    eng_load(e, mt, states + jb.engine, wl);
+    NormalF32<0> d{jb.out, 0.f, 1.f, nullptr, nullptr};
+    op_fixed_wg<kJobThreads>(e, carry, jb.count, d, tid);
+    eng_store(e, states + jb.engine, wl);
+}
+
+hipError_t launch_std_normal_jobs(EngineState* st, const FillJob* jobs, int n_jobs, hipStream_t s) {
+    if (n_jobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_std_normal_jobs, dim3((unsigned)n_jobs), dim3(kJobThreads), 0, s, st, jobs);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------- launchers ---
 template <class D>
 static hipError_t launch_fixed(EngineState* states, int64_t n_engines, int64_t per, const D& d,

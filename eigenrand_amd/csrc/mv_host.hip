@@ -226,6 +226,16 @@ int release_factors(erd_mv_factors* f, hipStream_t s) {
     return e == hipSuccess ? ERD_OK : ERD_ERR_HIP;
 }
 
+// StdNormalGen fills of one engine each, a workgroup per engine (the
+// longest first); same outputs and engine states as the segment program.
+int run_std_normal_jobs(erd_engines* e, std::vector<FillJob> jobs, hipStream_t s, Scratch& sc) {
+    if (jobs.empty()) return ERD_OK;
+    std::stable_sort(jobs.begin(), jobs.end(), [](const FillJob& a, const FillJob& b) { return a.count > b.count; });
+    FillJob* d_jobs;
+    if (sc.upload(&d_jobs, jobs) != hipSuccess) return ERD_ERR_HIP;
+    return launch_std_normal_jobs(e->states, d_jobs, (int)jobs.size(), s) == hipSuccess ? ERD_OK : ERD_ERR_HIP;
+}
+
 }  // namespace erd
 
 using namespace erd;
@@ -339,17 +349,10 @@ int erd_mvnormal_generate(erd_engines* e, const erd_mv_factors* f, const int64_t
     // Z_i: the StdNormalGen fill of d_i x n_i on engine i (MvNormal.h:103-108).
     float* z = z_out;
     if (!z && sc.alloc(&z, zoff[n_inst]) != hipSuccess) return ERD_ERR_OOM;
-    std::vector<Segment> segs;
-    std::vector<int32_t> engine_of;
-    for (int64_t i = 0; i < n_inst; ++i) {
-        Segment sg{};
-        sg.kind = SEG_STD_NORMAL_F32;
-        sg.count = (int64_t)vd[i] * samples[i];
-        sg.out = z + zoff[i];
-        segs.push_back(sg);
-        engine_of.push_back((int32_t)i);
-    }
-    int rc = run_segments(e, segs, engine_of, s);
+    std::vector<FillJob> jobs;
+    for (int64_t i = 0; i < n_inst; ++i)
+        jobs.push_back(FillJob{(int32_t)i, 0, (int64_t)vd[i] * samples[i], z + zoff[i]});
+    int rc = run_std_normal_jobs(e, jobs, s, sc);
     if (rc) return rc;
     std::vector<MvInstance> inst(n_inst);
     for (int64_t i = 0; i < n_inst; ++i)
@@ -396,16 +399,14 @@ static int wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64
         return ERD_ERR_OOM;
     // Engine program per instance: the normal fill, then chi^2(df - i) for
     // i = 0..d-1 (ChiSquaredGen = GammaGen(n/2, 2)), as generate() draws them.
+    // The normal fills run first, one workgroup per engine; the chi^2
+    // segments follow on the same stream.
+    std::vector<FillJob> jobs;
     std::vector<Segment> segs;
     std::vector<int32_t> engine_of;
     for (int64_t i = 0; i < n_inst; ++i) {
         const int64_t d = vd[i], n = samples[i];
-        Segment sg{};
-        sg.kind = SEG_STD_NORMAL_F32;
-        sg.count = n * d * (d - 1) / 2;
-        sg.out = nrm + nrm_off[i];
-        segs.push_back(sg);
-        engine_of.push_back((int32_t)i);
+        jobs.push_back(FillJob{(int32_t)i, 0, n * d * (d - 1) / 2, nrm + nrm_off[i]});
         for (int64_t k = 0; k < d; ++k) {
             Segment g{};
             g.kind = SEG_GAMMA_F32;
@@ -418,7 +419,9 @@ static int wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64
             engine_of.push_back((int32_t)i);
         }
     }
-    int rc = run_segments(e, segs, engine_of, s);
+    int rc = run_std_normal_jobs(e, jobs, s, sc);
+    if (rc) return rc;
+    rc = run_segments(e, segs, engine_of, s);
     if (rc) return rc;
     std::vector<MvInstance> inst(n_inst);
     for (int64_t i = 0; i < n_inst; ++i)
