@@ -316,7 +316,8 @@ int erd_generator_fill(erd_generator* g, erd_engines* e, void* out, int64_t per_
  * reference's float LLT). `a`: DEVICE, n_inst column-major d_i x d_i matrices
  * back to back, factored in place (L, upper part zeroed). dims/info: HOST.
  * Returns ERD_ERR_NOT_PSD if some info[i] != 0 (the reference then falls back
- * to an eigen-decomposition; this boundary reports it instead). */
+ * to an eigen-decomposition; this boundary reports it instead); the matrix of
+ * an instance with info[i] != 0 is left partly factored. */
 int erd_cholesky_f64_batch(int64_t n_inst, const int32_t* dims, double* a, int32_t* info, void* stream);
 
 /* MvNormalGen<float>(mean_i, cov_i).generate(engine_i, samples_i) for
