@@ -23,7 +23,6 @@ HIP_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-fno-fast-math",
     "-fhip-fp32-correctly-rounded-divide-sqrt",
@@ -38,9 +37,28 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def build_library(verbose=False):
-    cmd = [HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-           "-o", LIB + ".tmp", *sources()]
+OBJ_DIR = os.path.join(HERE, "_build")
+
+
+def _compile(src, verbose):
+    obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
+    cmd = [HIPCC, *HIP_FLAGS, "-c", "-I", os.path.join(ROOT, "include"), "-I", CSRC, "-o", obj, src]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return obj
+
+
+def build_library(verbose=False, jobs=None):
+    """Each source compiles to its own object (in parallel), then one link.
+    No device code is shared across sources (each kernel is launched from the
+    file that defines it), so no relocatable device code is needed."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    jobs = jobs or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(lambda f: _compile(f, verbose), sources()))
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
