@@ -6,6 +6,12 @@ Vmt19937_64 engines seeded 0, 4, ..., 16380 (16384 MT19937-64 streams seeded
 `Rand::normal<ArrayXf>(65536, 1, engine_g)` per engine. One step = one launch
 of the fill over the whole 1 GiB output, which stays resident in HBM.
 
+The headline runs the fast precision mode (hardware log2/sqrt/sin/cos, the
+same engine draws and output order as the bit-exact mode, each value within
+4 ulp + 2^-20 r of it); `--precision exact` makes the bit-exact mode the
+headline. The other mode and a write-only store kernel of the same launch
+shape (the practical HBM-write ceiling) are measured beside it.
+
 Prints one JSON line (rank 0) with Gsamples/s, the HBM-write roofline of the
 fill kernel (algorithmic bytes = 4 B written per sample, timed with HIP events
 on the launch stream), the CPU baseline (the oracle restatement of the
@@ -28,6 +34,12 @@ N_ENGINES = 4096
 PER_ENGINE = 65536
 N_SAMPLES = N_ENGINES * PER_ENGINE  # 2^28
 HBM_PEAK_GBS = 8000.0
+KERNEL = {"fast": "k_fill_fixed2<NormalF32Fast>", "exact": "k_fill_fixed<NormalF32<0>>"}
+ACCURACY = {
+    "exact": "bit-exact vs the oracle restatement of the reference's float op sequence",
+    "fast": "same engine draws, pairing and order as exact; |z - z_exact| <= 4 ulp(z_exact) + 2^-20 r "
+            "(r = sqrt(-2 ln u1) <= 5.65), asserted at 2^28 by tests/test_gpu_fast.py",
+}
 
 
 def cpu_baseline(max_threads=16):
@@ -406,12 +418,14 @@ def mv_sweep(emit_line):
                "hbm_frac": round(4 * elems / ms / 1e6 / HBM_PEAK_GBS, 4)})
 
 
-def load_traffic():
+def load_traffic(mode):
+    """HBM bytes per launch of the normal fill in `mode`, from the committed
+    FETCH_SIZE / WRITE_SIZE counter passes (profiles/)."""
     p = os.path.join(ROOT, "profiles", "normal_f32_2p28_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        return json.load(f).get(mode, {}).get("hbm_bytes_per_launch")
 
 
 def main():
@@ -421,6 +435,8 @@ def main():
     # the default warm-up covers that, and 200 timed launches take ~75 ms.
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=150)
+    ap.add_argument("--precision", choices=("fast", "exact"), default="fast",
+                    help="normal fill mode of the headline line (the other is reported beside it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--sweep", action="store_true", help="secondary per-distribution measurements")
     ap.add_argument("--mv", action="store_true", help="only the batched multivariate measurements")
@@ -459,9 +475,31 @@ def main():
     stream = torch.cuda.current_stream()
     urng = er.Vmt19937_64(0, n=N_ENGINES, stride=4)
     out = torch.empty(N_SAMPLES, dtype=torch.float32, device="cuda")
+    alg_bytes = 4 * N_SAMPLES
+
+    def event_ms(fn, reps):
+        """Average HIP-event time of fn() over reps launches on `stream`."""
+        a = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+        b = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+        for i in range(reps):
+            a[i].record(stream)
+            fn()
+            b[i].record(stream)
+        torch.cuda.synchronize()
+        return sum(x.elapsed_time(y) for x, y in zip(a, b)) / reps
+
+    # Side measurements, run before the headline's own warm-up (they also
+    # bring the clocks to their steady state): the plain-store ceiling of the
+    # same 4096 x 256 KiB block pattern, and the other precision mode of the
+    # same fill on the same engines.
+    other = "exact" if args.precision == "fast" else "fast"
+    event_ms(lambda: er.store_ceiling(N_ENGINES, PER_ENGINE, out), 20)
+    ceil_ms = event_ms(lambda: er.store_ceiling(N_ENGINES, PER_ENGINE, out), 100)
+    event_ms(lambda: er.normal(PER_ENGINE, 1, urng, out=out, precision=other), 20)
+    other_ms = event_ms(lambda: er.normal(PER_ENGINE, 1, urng, out=out, precision=other), 100)
 
     for _ in range(args.warmup):
-        er.normal(PER_ENGINE, 1, urng, out=out)
+        er.normal(PER_ENGINE, 1, urng, out=out, precision=args.precision)
     torch.cuda.synchronize()
 
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
@@ -470,15 +508,20 @@ def main():
     t0 = time.perf_counter()
     for i in range(args.steps):
         starts[i].record(stream)
-        er.normal(PER_ENGINE, 1, urng, out=out)
+        er.normal(PER_ENGINE, 1, urng, out=out, precision=args.precision)
         ends[i].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kernel_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
     ms_per_step = wall * 1e3 / args.steps
     value = N_SAMPLES / (wall / args.steps) / 1e9
-    alg_bytes = 4 * N_SAMPLES
     achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+
+    def mode_info(mode, ms):
+        return {"kernel": KERNEL[mode], "kernel_ms": round(ms, 4),
+                "gsamples_per_s": round(N_SAMPLES / ms / 1e6, 1),
+                "achieved_gbs": round(alg_bytes / ms / 1e6, 1),
+                "frac": round(alg_bytes / ms / 1e6 / HBM_PEAK_GBS, 4), "accuracy": ACCURACY[mode]}
 
     line = {
         "metric": METRIC,
@@ -495,7 +538,8 @@ def main():
         "data": "synthetic (engine-generated; no input data)",
         "config": {
             "workload": "normal(0,1) f32, 2^28 samples: 4096 Vmt19937_64 engines seeded 0,4,..,16380 "
-                        "x 65536 contiguous samples each",
+                        f"x 65536 contiguous samples each; precision mode {args.precision}",
+            "precision": args.precision,
             "n_samples": N_SAMPLES,
             "engines": N_ENGINES,
             "mt_streams": 4 * N_ENGINES,
@@ -507,11 +551,18 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic(),
-            "kernel": "k_fill_fixed<NormalF32<0>>",
+            "traffic": load_traffic(args.precision),
+            "kernel": KERNEL[args.precision],
             "kernel_ms": round(kernel_ms, 4),
             "algorithmic_bytes_per_launch": alg_bytes,
+            "store_ceiling": {"kernel": "k_store_blocks (16-byte stores, 4096 x 64 threads)",
+                              "kernel_ms": round(ceil_ms, 4),
+                              "achieved_gbs": round(alg_bytes / ceil_ms / 1e6, 1),
+                              "frac": round(alg_bytes / ceil_ms / 1e6 / HBM_PEAK_GBS, 4),
+                              "fill_vs_ceiling": round(ceil_ms / kernel_ms, 4)},
         },
+        "precision": {"mode": args.precision, "accuracy": ACCURACY[args.precision],
+                      other: mode_info(other, other_ms)},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(),
     }
     if not args.no_serve:

@@ -262,14 +262,34 @@ def uniform_real_f64(rows, cols, urng, min=0.0, max=1.0, out=None, stream=None):
                  float(min), float(max))
 
 
-def normal(rows, cols, urng, mean=None, stdev=None, out=None, stream=None):
+def normal(rows, cols, urng, mean=None, stdev=None, out=None, stream=None, precision="exact"):
     """Rand::normal<MatrixXf>(rows, cols, urng) (StdNormalGen) or, with
-    mean/stdev, Rand::normal<MatrixXf>(rows, cols, urng, mean, stdev) (NormalGen)."""
+    mean/stdev, Rand::normal<MatrixXf>(rows, cols, urng, mean, stdev) (NormalGen).
+
+    precision="exact": the reference's float op sequence, bit for bit.
+    precision="fast": the same engine draws, pairing and output order with the
+    hardware log2/sqrt/sin/cos; each value within 4 ulp + 2^-20 * r of the
+    exact mode's (r = sqrt(-2 ln u1) <= 5.65)."""
     torch = _torch()
+    if precision not in ("exact", "fast"):
+        raise InvalidArgument("normal: precision", 1)
+    sfx = "_fast" if precision == "fast" else ""
     if mean is None and stdev is None:
-        return _fill("erd_fill_std_normal_f32", urng, rows, cols, torch.float32, out, stream)
-    return _fill("erd_fill_normal_f32", urng, rows, cols, torch.float32, out, stream,
+        return _fill("erd_fill_std_normal_f32" + sfx, urng, rows, cols, torch.float32, out, stream)
+    return _fill("erd_fill_normal_f32" + sfx, urng, rows, cols, torch.float32, out, stream,
                  float(0.0 if mean is None else mean), float(1.0 if stdev is None else stdev))
+
+
+def store_ceiling(n_blocks, per_block, out, stream=None):
+    """Write-only 16-byte stores of 1.0f into n_blocks contiguous blocks of
+    per_block floats (one 64-thread workgroup per block): the HBM-write
+    ceiling the fills are measured against. No reference counterpart."""
+    torch = _torch()
+    if out.dtype != torch.float32 or not out.is_cuda or not out.is_contiguous() \
+            or out.numel() < int(n_blocks) * int(per_block):
+        raise InvalidArgument("store_ceiling: output tensor", 1)
+    _lib.call("erd_store_ceiling_f32", _ptr(out), int(n_blocks), int(per_block), _stream_ptr(stream))
+    return out
 
 
 def normal_f64(rows, cols, urng, mean=None, stdev=None, out=None, stream=None):

@@ -64,6 +64,9 @@ _SIGS = {
     "erd_fill_exponential_f64": [P, P, I64, F64, P],
     "erd_fill_cauchy_f64": [P, P, I64, F64, F64, P],
     "erd_fill_normal_f32": [P, P, I64, F32, F32, P],
+    "erd_fill_std_normal_f32_fast": [P, P, I64, P],
+    "erd_fill_normal_f32_fast": [P, P, I64, F32, F32, P],
+    "erd_store_ceiling_f32": [P, I64, I64, P],
     "erd_fill_normal_vop_f32": [P, P, I64, P, P, P],
     "erd_fill_exponential_f32": [P, P, I64, F32, P],
     "erd_fill_cauchy_f32": [P, P, I64, F32, F32, P],

@@ -252,6 +252,24 @@ int erd_fill_normal_f32(erd_engines* e, float* out, int64_t per, float mean, flo
     return hip_status(erd::launch_fill_normal_f32(e->states, e->count, out, per, mean, sd, 1, S(stream)));
 }
 
+int erd_fill_std_normal_f32_fast(erd_engines* e, float* out, int64_t per, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_normal_f32_fast(e->states, e->count, out, per, 0.f, 1.f, 0, S(stream)));
+}
+
+int erd_fill_normal_f32_fast(erd_engines* e, float* out, int64_t per, float mean, float sd, void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    return hip_status(erd::launch_fill_normal_f32_fast(e->states, e->count, out, per, mean, sd, 1, S(stream)));
+}
+
+int erd_store_ceiling_f32(float* out, int64_t n_blocks, int64_t per_block, void* stream) {
+    if (n_blocks < 0 || per_block < 0 || per_block % 4 != 0 || n_blocks > 0x7fffffff) return ERD_ERR_INVALID;
+    if (n_blocks > 0 && per_block > 0 && !out) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_store_blocks(out, n_blocks, per_block, S(stream)));
+}
+
 int erd_fill_normal_vop_f32(erd_engines* e, float* out, int64_t per, const float* mean,
                             const float* sd, void* stream) {
     int rc = check_fill(e, out, per, 8);

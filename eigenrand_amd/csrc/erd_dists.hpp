@@ -35,6 +35,15 @@ struct UniformF32 {  // Basic.h:584-590 / 626-634; float j of a call = word j
         if (affine) { u0 = u0 * slope + a; u1 = u1 * slope + a; }
         *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(u0, u1);
     }
+    // floats 4k .. 4k+3 of the call (lanes 2k, 2k+1) in one 16-byte store
+    __device__ void emit2(int64_t per, int64_t u, int k, const uint64_t (&w)[1][2]) const {
+        float v[4] = {ur_f32(lo32(w[0][0])), ur_f32(hi32(w[0][0])), ur_f32(lo32(w[0][1])), ur_f32(hi32(w[0][1]))};
+        if (affine) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] = v[i] * slope + a;
+        }
+        *reinterpret_cast<float4*>(out + 8 * u + 4 * k) = make_float4(v[0], v[1], v[2], v[3]);
+    }
 };
 
 struct UniformF64 {  // arch/AVX/RandUtils.h:212-225: 4 doubles per call
@@ -194,6 +203,47 @@ struct NormalF32 {
         }
         *reinterpret_cast<float2*>(out + base) = make_float2(sn[0], sn[1]);
         if (full) *reinterpret_cast<float2*>(out + base + 8) = make_float2(cs[0], cs[1]);
+    }
+};
+
+// Box-Muller with the hardware transcendentals (the fast precision mode):
+// the same engine words, the same pairing and the same output order as
+// NormalF32 (float j of draw 2m gives u1 = 1 - U, float j of draw 2m+1 gives
+// u2; out[16m + j] = r sin, out[16m + 8 + j] = r cos), but
+//   r = v_sqrt(-2 ln2 * v_log(u1))           (v_log is log2)
+//   sin, cos = v_sin, v_cos of 1 + u2        (they take revolutions and have
+//                                             period 1, so 1 + u2, which is
+//                                             exact, stands for 2 pi u2)
+// and four outputs per 16-byte store. The stated error bound against the
+// bit-exact mode is asserted by the tests.
+struct NormalF32Fast {
+    static constexpr int CALLS = 2;
+    float* out;
+    float mean, sd;
+    int affine;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return (n + 15) / 16; }
+    __device__ void emit2(int64_t per, int64_t u, int k, const uint64_t (&w)[2][2]) const {
+        float sn[4], cs[4];
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t a = h ? hi32(w[0][j]) : lo32(w[0][j]);
+                const uint32_t b = h ? hi32(w[1][j]) : lo32(w[1][j]);
+                const float u1 = one_minus_ur_f32(a);
+                const float rev = as_f((b & 0x7FFFFFu) | 0x3F800000u);
+                const float r = __builtin_amdgcn_sqrtf(-1.38629436111989061f * __builtin_amdgcn_logf(u1));
+                sn[2 * j + h] = r * __builtin_amdgcn_sinf(rev);
+                cs[2 * j + h] = r * __builtin_amdgcn_cosf(rev);
+            }
+        if (affine) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) { sn[i] = sn[i] * sd + mean; cs[i] = cs[i] * sd + mean; }
+        }
+        float* o = out + 16 * u + 4 * k;
+        *reinterpret_cast<float4*>(o) = make_float4(sn[0], sn[1], sn[2], sn[3]);
+        if (16 * u + 16 <= per) *reinterpret_cast<float4*>(o + 8) = make_float4(cs[0], cs[1], cs[2], cs[3]);
     }
 };
 

@@ -46,6 +46,11 @@ hipError_t launch_fill_uniform_f64(EngineState* st, int64_t n, double* out, int6
                                    double b, int affine, hipStream_t s);
 hipError_t launch_fill_normal_f32(EngineState* st, int64_t n, float* out, int64_t per, float mean,
                                   float sd, int affine, hipStream_t s);
+// The fast precision mode of normal<float> (hardware log2/sqrt/sin/cos).
+hipError_t launch_fill_normal_f32_fast(EngineState* st, int64_t n, float* out, int64_t per, float mean,
+                                       float sd, int affine, hipStream_t s);
+// Write-only stores of n_blocks contiguous blocks of `per` floats.
+hipError_t launch_store_blocks(float* out, int64_t n_blocks, int64_t per, hipStream_t s);
 // DiscreteGen<int32_t, int32_t> fill: CDF (ncdf = K < 16) or alias table.
 hipError_t launch_fill_discrete_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, const uint32_t* cdf,
                                    const int32_t* prob, const int32_t* alias, int32_t ncdf, uint32_t bitmask,

@@ -34,6 +34,34 @@ __global__ __launch_bounds__(64) void k_fill_fixed(EngineState* __restrict__ sta
     eng_store(e, states + g, lane);
 }
 
+// k_fill_fixed with two-lane items and 16-byte stores (op_fixed2).
+template <class D>
+__global__ __launch_bounds__(64) void k_fill_fixed2(EngineState* __restrict__ states,
+                                                    int64_t per, D d) {
+    constexpr int C = D::CALLS;
+    __shared__ uint64_t mt[kWords];
+    __shared__ uint64_t carry[(C > 1 ? C - 1 : 1) * kLanes];
+    const int lane = threadIdx.x;
+    const int64_t g = blockIdx.x;
+    if (D::units(per) == 0) return;
+    Eng e;
+    eng_load(e, mt, states + g, lane);
+    d.rebase(g * per);
+    op_fixed2(e, carry, per, d, lane);
+    eng_store(e, states + g, lane);
+}
+
+// Plain 16-byte stores of a constant, one workgroup of 64 threads per
+// contiguous block of `per` floats: the store pattern of the fills without
+// their arithmetic (the practical HBM-write ceiling they are measured
+// against). per is a multiple of 4.
+__global__ __launch_bounds__(64) void k_store_blocks(float* __restrict__ out, int64_t per, float v) {
+    float4* o = reinterpret_cast<float4*>(out + (int64_t)blockIdx.x * per);
+    const int64_t n4 = per / 4;
+    const float4 x = make_float4(v, v, v, v);
+    for (int64_t i = threadIdx.x; i < n4; i += 64) o[i] = x;
+}
+
 // Reused normal generator: the cached cos packet (count 8) is emitted first,
 // in the call's affine form; a trailing half unit leaves its cos packet in gs.
 template <int MODE>
@@ -135,10 +163,30 @@ hipError_t launch_fill_p8(EngineState* st, int64_t n, void* out, int64_t per_v, 
     }
 }
 
+template <class D>
+static hipError_t launch_fixed2(EngineState* states, int64_t n_engines, int64_t per, const D& d,
+                                hipStream_t stream) {
+    if (n_engines <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_fixed2<D>, dim3((unsigned)n_engines), dim3(64), 0, stream, states, per, d);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_uniform_f32(EngineState* st, int64_t n, float* out, int64_t per, float a,
                                    float b, int affine, hipStream_t s) {
     UniformF32 d{out, a, b - a, affine};
-    return launch_fixed(st, n, per, d, s);
+    return launch_fixed2(st, n, per, d, s);
+}
+
+hipError_t launch_fill_normal_f32_fast(EngineState* st, int64_t n, float* out, int64_t per, float mean,
+                                       float sd, int affine, hipStream_t s) {
+    NormalF32Fast d{out, mean, sd, affine};
+    return launch_fixed2(st, n, per, d, s);
+}
+
+hipError_t launch_store_blocks(float* out, int64_t n_blocks, int64_t per, hipStream_t s) {
+    if (n_blocks <= 0 || per <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_store_blocks, dim3((unsigned)n_blocks), dim3(64), 0, s, out, per, 1.0f);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_uniform_f64(EngineState* st, int64_t n, double* out, int64_t per, double a,

@@ -133,6 +133,17 @@ int erd_fill_normal_f32(erd_engines* e, float* out, int64_t per_engine, float me
  * free function 1294-1335). mean/stdev are device arrays of n*per_engine. */
 int erd_fill_normal_vop_f32(erd_engines* e, float* out, int64_t per_engine, const float* mean,
                             const float* stdev, void* stream);
+/* normal<float> in the FAST precision mode (StdNormalGen / NormalGen,
+ * NormalExp.h:60-82, 127-135, with other arithmetic): the same engine draws,
+ * the same Box-Muller pairing and output order, and the same final engine
+ * state as erd_fill_std_normal_f32 / erd_fill_normal_f32, but r and the angle
+ * use the hardware log2, square root, sine and cosine instead of the
+ * reference's cephes plog and psincos. Each output z stays within
+ *   |z - z_exact| <= 4 ulp(z_exact) + 2^-20 * r,   r = sqrt(-2 ln u1) <= 5.65,
+ * of the bit-exact mode's value (asserted by the tests). */
+int erd_fill_std_normal_f32_fast(erd_engines* e, float* out, int64_t per_engine, void* stream);
+int erd_fill_normal_f32_fast(erd_engines* e, float* out, int64_t per_engine, float mean, float stdev,
+                             void* stream);
 /* lognormal<float>(…, mean, stdev): LognormalGen (NormalExp.h:171-210), the
  * exp of NormalGen's packets (same cos-packet caching). */
 int erd_fill_lognormal_f32(erd_engines* e, float* out, int64_t per_engine, float mean, float stdev,
@@ -476,6 +487,14 @@ int erd_server_stats(erd_server* srv, erd_server_stats_t* stats);
 int erd_server_reset_stats(erd_server* srv);
 /* Non-owning view of a client's engines (after its first request ran). */
 int erd_server_client_engines(erd_server* srv, uint64_t client, erd_engines** view);
+
+/* ---------------------------------------------------------- measurement --- */
+/* No reference counterpart: writes 1.0f to n_blocks contiguous blocks of
+ * per_block floats (a multiple of 4) with one 64-thread workgroup per block
+ * and 16-byte stores, the store pattern of the fills without their
+ * arithmetic. It is the practical HBM-write ceiling the fills are compared
+ * with. */
+int erd_store_ceiling_f32(float* out, int64_t n_blocks, int64_t per_block, void* stream);
 
 /* -------------------------------------------------------- device memory --- */
 /* Small helpers so callers without another GPU runtime can use the library. */
