@@ -40,6 +40,8 @@ __all__ = [
 ]
 
 STATE_BYTES = 10016  # erd_engine_state_bytes()
+STATUS_ITER_CAP = 1  # ERD_STATUS_ITER_CAP
+STATUS_BAD_SEGMENT = 2  # ERD_STATUS_BAD_SEGMENT
 
 
 def _torch():
@@ -126,9 +128,27 @@ class Vmt19937_64:
         _lib.call("erd_engines_discard", self._h, int(calls), _stream_ptr(stream))
 
     def status(self, stream=None):
+        """Reads and clears the kernel status bits of this set (synchronises
+        `stream`): STATUS_ITER_CAP when a sampler gave up at its cap."""
         v = C.c_uint32()
         _lib.call("erd_engines_status", self._h, C.byref(v), _stream_ptr(stream))
         return int(v.value)
+
+    def check(self, stream=None):
+        """Raises ErdError(ERD_ERR_ITER_CAP) if a launch on this set gave up at
+        an iteration cap since the last status read (the outputs of that
+        launch are unspecified); clears the bits."""
+        bits = self.status(stream)
+        if bits & STATUS_ITER_CAP:
+            raise ErdError("sampler iteration cap", 4)
+        if bits:
+            raise ErdError("malformed request program", 1)
+
+    def set_iteration_caps(self, reject_cap=0, lockstep_cap=0):
+        """Caps of later launches on this set: a rejection fill gives up after
+        reject_cap candidates without output, a lockstep packet after
+        lockstep_cap iterations (0: the defaults 2^22 and 2^20)."""
+        _lib.call("erd_engines_set_iteration_caps", self._h, int(reject_cap), int(lockstep_cap))
 
 
 class P8_mt19937_64(Vmt19937_64):
@@ -227,6 +247,19 @@ def _alloc(urng, per_engine, dtype, out):
     return out
 
 
+def _param(t, dtype, name):
+    """A per-element parameter tensor, checked before any launch: it must be
+    a torch tensor of exactly `dtype` on the current CUDA device (where the
+    output lives). A host tensor would hand the kernel a host pointer, a
+    narrower dtype would make it read past the allocation, and a wider one
+    would be misread."""
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or t.dtype != dtype or not t.is_cuda \
+            or t.device.index != torch.cuda.current_device():
+        raise InvalidArgument(name, 1)
+    return t.contiguous()
+
+
 def _shape(flat, urng, rows, cols):
     n = len(urng)
     if n == 1:
@@ -306,8 +339,8 @@ def normal_vop(urng, mean, stdev, out=None, stream=None):
     """Rand::normal(urng, mean_array, stdev_array) (NormalVGen); mean/stdev are
     CUDA float32 tensors of n*per_engine elements (engine g uses its block)."""
     torch = _torch()
-    mean = mean.contiguous()
-    stdev = stdev.contiguous()
+    mean = _param(mean, torch.float32, "erd_fill_normal_vop_f32: mean")
+    stdev = _param(stdev, torch.float32, "erd_fill_normal_vop_f32: stdev")
     if mean.numel() != stdev.numel() or mean.numel() % len(urng):
         raise InvalidArgument("erd_fill_normal_vop_f32", 1)
     per = mean.numel() // len(urng)
@@ -436,7 +469,7 @@ def _vop(name, urng, params, out, stream):
     """Vectorize-over-params fill: params are float32 CUDA tensors of
     n*per_engine elements (engine g uses its block)."""
     torch = _torch()
-    params = [p.contiguous() for p in params]
+    params = [_param(p, torch.float32, name) for p in params]
     if any(p.numel() != params[0].numel() for p in params) or params[0].numel() % len(urng):
         raise InvalidArgument(name, 1)
     per = params[0].numel() // len(urng)
@@ -494,8 +527,8 @@ def binomial_vop(urng, trials, p, out=None, stream=None):
     """Rand::binomial(urng, trials_array, p_array) (BinomialVGen<int32_t>);
     trials int32 and p float32 CUDA tensors of n*per_engine elements."""
     torch = _torch()
-    trials = trials.contiguous()
-    p = p.contiguous()
+    trials = _param(trials, torch.int32, "erd_fill_binomial_vop_i32: trials")
+    p = _param(p, torch.float32, "erd_fill_binomial_vop_i32: p")
     if trials.numel() != p.numel() or trials.numel() % len(urng):
         raise InvalidArgument("erd_fill_binomial_vop_i32", 1)
     per = trials.numel() // len(urng)
@@ -509,8 +542,8 @@ def gamma_vop(urng, alpha, beta, out=None, stream=None):
     """Per-element gamma(alpha[i], beta[i]) (float32 CUDA tensors); the
     lockstep vectorize-over-params design of BinomialVGen applied to GammaGen."""
     torch = _torch()
-    alpha = alpha.contiguous()
-    beta = beta.contiguous()
+    alpha = _param(alpha, torch.float32, "erd_fill_gamma_vop_f32: alpha")
+    beta = _param(beta, torch.float32, "erd_fill_gamma_vop_f32: beta")
     if alpha.numel() != beta.numel() or alpha.numel() % len(urng):
         raise InvalidArgument("erd_fill_gamma_vop_f32", 1)
     per = alpha.numel() // len(urng)
@@ -523,7 +556,7 @@ def gamma_vop(urng, alpha, beta, out=None, stream=None):
 def poisson_vop(urng, mean, out=None, stream=None):
     """Per-element poisson(mean[i]) (float32 CUDA tensor) -> int32."""
     torch = _torch()
-    mean = mean.contiguous()
+    mean = _param(mean, torch.float32, "erd_fill_poisson_vop_i32: mean")
     if mean.numel() % len(urng):
         raise InvalidArgument("erd_fill_poisson_vop_i32", 1)
     per = mean.numel() // len(urng)
@@ -536,8 +569,15 @@ def run_requests(urng, requests, stream=None):
     """Run a batch of heterogeneous fills (erd_run_requests). `requests` is a
     list of dicts: dist (name), engine, count, out (tensor), optional in0/in1
     (tensors), a, b."""
+    torch = _torch()
     arr = (_lib.Request * len(requests))()
     for i, r in enumerate(requests):
+        for key in ("out", "in0", "in1"):
+            t = r.get(key)
+            if t is not None and (not isinstance(t, torch.Tensor) or not t.is_cuda
+                                  or t.device.index != torch.cuda.current_device()
+                                  or not t.is_contiguous()):
+                raise InvalidArgument(f"erd_run_requests: request {i} {key}", 1)
         arr[i].dist = _lib.DIST[r["dist"]]
         arr[i].engine = int(r.get("engine", 0))
         arr[i].count = int(r["count"])

@@ -51,6 +51,7 @@ _SIGS = {
     "erd_engines_copy": [P, P, P],
     "erd_engines_discard": [P, I64, P],
     "erd_engines_status": [P, C.POINTER(C.c_uint32), P],
+    "erd_engines_set_iteration_caps": [P, C.c_int32, C.c_int32],
     "erd_fill_u64": [P, P, I64, P],
     "erd_engines_create_p8": [C.POINTER(P), P, I64, P],
     "erd_p8_fill_u64": [P, P, I64, P],

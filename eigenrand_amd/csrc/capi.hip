@@ -100,6 +100,8 @@ int erd_engines_view(erd_engines** out, erd_engines* parent, int64_t first, int6
     if (!e) return ERD_ERR_OOM;
     e->states = parent->states + first;
     e->status = parent->status;
+    e->reject_cap = parent->reject_cap;
+    e->lockstep_cap = parent->lockstep_cap;
     e->count = n;
     e->owns = false;
     *out = e;
@@ -154,6 +156,13 @@ int erd_engines_copy(erd_engines* dst, const erd_engines* src, void* stream) {
 int erd_engines_discard(erd_engines* e, int64_t calls, void* stream) {
     if (!valid_set(e) || calls < 0) return ERD_ERR_INVALID;
     return hip_status(erd::launch_discard(e->states, e->count, calls, S(stream)));
+}
+
+int erd_engines_set_iteration_caps(erd_engines* e, int32_t reject_cap, int32_t lockstep_cap) {
+    if (!valid_set(e) || reject_cap < 0 || lockstep_cap < 0) return ERD_ERR_INVALID;
+    e->reject_cap = reject_cap > 0 ? reject_cap : erd::kDefaultRejectCap;
+    e->lockstep_cap = lockstep_cap > 0 ? lockstep_cap : erd::kDefaultLockstepCap;
+    return ERD_OK;
 }
 
 int erd_engines_status(erd_engines* e, uint32_t* status_out, void* stream) {
@@ -329,7 +338,7 @@ int fill_gamma(erd_engines* e, float* out, int64_t per, float alpha, float beta,
     if (alpha == 1.0f || ((float)(int)alpha == alpha && alpha < 20.0f))
         return hip_status(erd::launch_fill_gamma_fixed_f32(e->states, e->count, out, per, alpha, beta, S(stream)));
     return hip_status(erd::launch_fill_gamma_reject_f32(e->states, e->count, out, per, alpha, beta,
-                                                        e->status, S(stream), gs));
+                                                        guard_of(e), S(stream), gs));
 }
 
 int fill_poisson(erd_engines* e, int32_t* out, int64_t per, double mean, erd::GenState* gs, void* stream) {
@@ -338,7 +347,7 @@ int fill_poisson(erd_engines* e, int32_t* out, int64_t per, double mean, erd::Ge
     if (!(mean >= 0.0) || !std::isfinite(mean)) return ERD_ERR_INVALID;
     if (mean < 12) {
         return hip_status(erd::launch_fill_poisson_knuth_i32(e->states, e->count, out, per, mean, 0, 0,
-                                                             e->status, S(stream)));
+                                                             guard_of(e), S(stream)));
     }
     // PoissonGen constructor (Discrete.h:807-813), in double with libm.
     erd::PoissonParams pp;
@@ -347,7 +356,7 @@ int fill_poisson(erd_engines* e, int32_t* out, int64_t per, double mean, erd::Ge
     pp.sqrt_tmean = std::sqrt(2 * mean);
     pp.log_mean = std::log(mean);
     pp.g1 = mean * pp.log_mean - std::lgamma(mean + 1);
-    return hip_status(erd::launch_fill_poisson_reject_i32(e->states, e->count, out, per, pp, e->status,
+    return hip_status(erd::launch_fill_poisson_reject_i32(e->states, e->count, out, per, pp, guard_of(e),
                                                           S(stream), gs));
 }
 
@@ -369,12 +378,12 @@ int fill_binomial(erd_engines* e, int32_t* out, int64_t per, int32_t trials, dou
     const int flip = (p == bp.small_p) ? 0 : 1;
     if (bp.mean < 1.0)
         return hip_status(erd::launch_fill_poisson_knuth_i32(e->states, e->count, out, per, bp.mean,
-                                                             trials, flip, e->status, S(stream)));
+                                                             trials, flip, guard_of(e), S(stream)));
     bp.g1 = std::lgamma(trials + 1);
     bp.sqrt_v = std::sqrt(2 * bp.mean * (1 - bp.small_p));
     bp.log_small_p = std::log(bp.small_p);
     bp.log_small_q = std::log(1 - bp.small_p);
-    return hip_status(erd::launch_fill_binomial_reject_i32(e->states, e->count, out, per, bp, e->status,
+    return hip_status(erd::launch_fill_binomial_reject_i32(e->states, e->count, out, per, bp, guard_of(e),
                                                            S(stream), gs));
 }
 
@@ -481,7 +490,7 @@ int erd_fill_binomial_vop_i32(erd_engines* e, int32_t* out, int64_t per, const i
     if (rc) return rc;
     if (per > 0 && (!trials || !p)) return ERD_ERR_INVALID;
     return hip_status(erd::launch_fill_binomial_vop_i32(e->states, e->count, out, per, trials, p,
-                                                        e->status, S(stream)));
+                                                        guard_of(e), S(stream)));
 }
 
 int erd_fill_chisquared_f32(erd_engines* e, float* out, int64_t per, float n, void* stream) {
@@ -517,7 +526,7 @@ int erd_fill_gamma_vop_f32(erd_engines* e, float* out, int64_t per, const float*
     if (rc) return rc;
     if (per > 0 && (!alpha || !beta)) return ERD_ERR_INVALID;
     return hip_status(erd::launch_fill_gamma_vop_f32(e->states, e->count, out, per, alpha, beta,
-                                                     e->status, S(stream)));
+                                                     guard_of(e), S(stream)));
 }
 
 int erd_fill_poisson_vop_i32(erd_engines* e, int32_t* out, int64_t per, const float* mean,
@@ -525,7 +534,7 @@ int erd_fill_poisson_vop_i32(erd_engines* e, int32_t* out, int64_t per, const fl
     int rc = check_fill(e, out, per, 4);
     if (rc) return rc;
     if (per > 0 && !mean) return ERD_ERR_INVALID;
-    return hip_status(erd::launch_fill_poisson_vop_i32(e->states, e->count, out, per, mean, e->status,
+    return hip_status(erd::launch_fill_poisson_vop_i32(e->states, e->count, out, per, mean, guard_of(e),
                                                        S(stream)));
 }
 
@@ -732,8 +741,10 @@ int make_segment(const erd_request& r, Segment& s) {
             break;
         case ERD_DIST_BINOMIAL_I32: {
             width = 4;
+            // an integer in [0, INT32_MAX] before the cast (NaN fails the first test)
+            if (!(r.a >= 0.0 && r.a <= 2147483647.0) || std::floor(r.a) != r.a || !(r.b >= 0.0 && r.b <= 1.0))
+                return ERD_ERR_INVALID;
             const int32_t trials = (int32_t)r.a;
-            if (trials < 0 || (double)trials != r.a || !(r.b >= 0.0 && r.b <= 1.0)) return ERD_ERR_INVALID;
             s = seg_binomial((int32_t*)r.out, r.count, trials, r.b);
             break;
         }
@@ -797,7 +808,7 @@ int run_segments(erd_engines* e, const std::vector<Segment>& segs, const std::ve
     if (err == hipSuccess)
         err = launch_program(e->states, groups, (const int32_t*)(dbuf + seg_bytes),
                              (const int32_t*)(dbuf + seg_bytes) + groups, (const Segment*)dbuf,
-                             e->status, stream);
+                             guard_of(e), stream);
     hipError_t e2 = hipFreeAsync(dbuf, stream);
     if (err == hipSuccess) err = e2;
     return err == hipSuccess ? ERD_OK : ERD_ERR_HIP;

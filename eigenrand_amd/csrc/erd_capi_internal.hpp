@@ -9,7 +9,12 @@ struct erd_engines {
     uint32_t* status = nullptr;          // device status word (kernel error bits)
     int64_t count = 0;
     bool owns = false;                   // views share the parent's memory
+    int32_t reject_cap = erd::kDefaultRejectCap;      // per-launch iteration caps (erd::Guard)
+    int32_t lockstep_cap = erd::kDefaultLockstepCap;
 };
+
+// Status word and iteration caps of the launches on an engine set.
+inline erd::Guard guard_of(const erd_engines* e) { return {e->status, e->reject_cap, e->lockstep_cap}; }
 
 // Factors of a batch of MvNormalGen / WishartGen instances: the FP64 Cholesky
 // of each covariance, kept as bf16 hi/lo planes (row-major, padded to a

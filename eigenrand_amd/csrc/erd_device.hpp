@@ -53,6 +53,18 @@ static_assert(sizeof(GenState) == 40, "generator state layout");
 constexpr uint32_t kStatusIterCap = 1u;
 constexpr uint32_t kStatusBadSegment = 2u;
 
+// Error reporting and iteration limits of one launch. The reference's
+// rejection and lockstep loops run until they succeed (RandUtils.h:342-346);
+// here a rejection op gives up after reject_cap candidates without an output
+// and a lockstep packet after lockstep_cap iterations, raising kStatusIterCap.
+struct Guard {
+    uint32_t* status;
+    int32_t reject_cap;
+    int32_t lockstep_cap;
+};
+constexpr int32_t kDefaultRejectCap = 1 << 22;
+constexpr int32_t kDefaultLockstepCap = 1 << 20;
+
 __device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t as_u(float f) { return __float_as_uint(f); }
 

@@ -9,6 +9,7 @@ namespace erd {
 struct EngineState;
 struct Segment;
 struct GenState;
+struct Guard;
 
 // PoissonGen constructor constants, computed on the host in double with libm
 // (Discrete.h:807-813).
@@ -111,13 +112,13 @@ hipError_t launch_fill_uniform_int_i32(EngineState* st, int64_t n, int32_t* out,
 // fill_reject.hip (CompressMask rejection samplers); gs: optional per-engine
 // queues of a reused generator object.
 hipError_t launch_fill_gamma_reject_f32(EngineState* st, int64_t n, float* out, int64_t per,
-                                        float alpha, float beta, uint32_t* status, hipStream_t s,
+                                        float alpha, float beta, Guard gd, hipStream_t s,
                                         GenState* gs = nullptr);
 hipError_t launch_fill_poisson_reject_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
-                                          const PoissonParams& pp, uint32_t* status, hipStream_t s,
+                                          const PoissonParams& pp, Guard gd, hipStream_t s,
                                           GenState* gs = nullptr);
 hipError_t launch_fill_binomial_reject_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
-                                           const BinomialParams& bp, uint32_t* status, hipStream_t s,
+                                           const BinomialParams& bp, Guard gd, hipStream_t s,
                                            GenState* gs = nullptr);
 // Reused StdNormalGen / NormalGen (mode 0 / 1), NormalVGen (mode 2) or
 // LognormalGen (mode 3):
@@ -128,19 +129,19 @@ hipError_t launch_fill_normal_gen_f32(EngineState* st, int64_t n, float* out, in
 
 // fill_lockstep.hip (packet-lockstep samplers on half draws)
 hipError_t launch_fill_poisson_knuth_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
-                                         double mean, int32_t trials, int flip, uint32_t* status,
+                                         double mean, int32_t trials, int flip, Guard gd,
                                          hipStream_t s);
 hipError_t launch_fill_binomial_small_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
                                           int32_t trials, double p, hipStream_t s);
 hipError_t launch_fill_binomial_vop_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
-                                        const int32_t* trials, const float* p, uint32_t* status,
+                                        const int32_t* trials, const float* p, Guard gd,
                                         hipStream_t s);
 
 hipError_t launch_fill_gamma_vop_f32(EngineState* st, int64_t n, float* out, int64_t per,
-                                     const float* alpha, const float* beta, uint32_t* status,
+                                     const float* alpha, const float* beta, Guard gd,
                                      hipStream_t s);
 hipError_t launch_fill_poisson_vop_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
-                                       const float* mean, uint32_t* status, hipStream_t s);
+                                       const float* mean, Guard gd, hipStream_t s);
 
 // program.hip (segment programs)
 // Device table of plgamma_approx(k), k = 1..kLgTab, for the current device:
@@ -148,7 +149,7 @@ hipError_t launch_fill_poisson_vop_i32(EngineState* st, int64_t n, int32_t* out,
 // kept until the process ends. NULL if it cannot be allocated.
 const float* lgamma_table(hipStream_t s);
 hipError_t launch_program(EngineState* st, int32_t n_groups, const int32_t* d_engine_of,
-                          const int32_t* d_begin, const Segment* d_segs, uint32_t* status,
+                          const int32_t* d_begin, const Segment* d_segs, Guard gd,
                           hipStream_t s);
 Segment seg_poisson(int32_t* out, int64_t count, double mean);
 Segment seg_binomial(int32_t* out, int64_t count, int32_t trials, double p);

@@ -296,7 +296,8 @@ __device__ __forceinline__ void half_end(Eng& e, int pos, int nc, bool used) {
 }
 
 // ---------------------------------------------------------------- reject ---
-constexpr int kRejectCap = 1 << 22;  // candidates without output before giving up
+// The rejection cap (candidates without output before giving up) is per
+// launch: Guard::reject_cap.
 
 // Scratch a rejection op needs in LDS.
 template <class P>
@@ -321,7 +322,7 @@ struct RejectLds {
 // leftovers are dropped, as the reference's free functions do).
 template <class P>
 __device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t per, const P& pol,
-                          uint32_t* status, int lane, GenState* gs = nullptr) {
+                          Guard gd, int lane, GenState* gs = nullptr) {
     constexpr int W = P::W, SW = P::SW, K = P::K;
     constexpr int IPC = W / 2;     // threads per candidate (two lanes each)
     constexpr int CPS = 64 / IPC;  // candidates per wave step
@@ -456,12 +457,12 @@ __device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t p
                 since_emit = 0;
             } else {
                 since_emit += CPS;
-                if (since_emit > kRejectCap) break;
+                if (since_emit > gd.reject_cap) break;
             }
         }
         if (end_v >= 0) break;
-        if (since_emit > kRejectCap) {
-            if (lane == 0) atomicOr(status, kStatusIterCap);
+        if (since_emit > gd.reject_cap) {
+            if (lane == 0) atomicOr(gd.status, kStatusIterCap);
             end_v = ncand * K;
             break;
         }
@@ -490,7 +491,8 @@ __device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t p
 
 // -------------------------------------------------------------- lockstep ---
 constexpr int kHalfSlots = kWords / 2;  // half draws per block
-constexpr int kLockstepCap = 1 << 20;   // iterations per packet before giving up
+// The lockstep cap (iterations of one packet before giving up) is per launch:
+// Guard::lockstep_cap.
 
 // Cursor over the slot sequence of a lockstep op: SW words per slot (4 full
 // draw, 2 half draw), at most MAXC carried slots across a twist.
@@ -722,7 +724,7 @@ __device__ void op_uniform_int(Eng& e, uint64_t* carry, int32_t* ring, int32_t* 
 
 // Knuth loop of PoissonGen::packetOp (Discrete.h:860-873); returns lane j's count.
 __device__ __forceinline__ int32_t knuth_packet(HalfCursor& cur, float thr, int lane,
-                                                uint32_t* status) {
+                                                Guard gd) {
     const int j = lane & 3;
     float val = 1.0f;
     int32_t res = 0;
@@ -734,8 +736,8 @@ __device__ __forceinline__ int32_t knuth_packet(HalfCursor& cur, float thr, int 
         const bool c = thr < val;
         if (!(__ballot(c) & 0xFull)) break;
         res += c ? 1 : 0;
-        if (it > kLockstepCap) {
-            if (lane == 0) atomicOr(status, kStatusIterCap);
+        if (it > gd.lockstep_cap) {
+            if (lane == 0) atomicOr(gd.status, kStatusIterCap);
             break;
         }
     }
@@ -898,7 +900,7 @@ __device__ __forceinline__ LpLane lp_lane(uint32_t pt, uint32_t c, bool in) {
 // and the next starts after it. A packet whose stop lies past the window
 // restarts the window at its first slot.
 __device__ void op_knuth_lp(Eng& e, uint64_t* carry, int32_t* o, int64_t T, float thr, int32_t trials, int flip,
-                            uint32_t* status, int lane) {
+                            Guard gd, int lane) {
     HalfCursor cur = cursor_begin(e, carry, lane);
     const uint32_t tfix = (uint32_t)__builtin_amdgcn_readfirstlane(
         __float2int_rn(__builtin_amdgcn_logf(thr) * (float)(1 << kLpFrac)));
@@ -952,7 +954,7 @@ __device__ void op_knuth_lp(Eng& e, uint64_t* carry, int32_t* o, int64_t T, floa
             if (lane < 4 * nb) o[4 * flushed + lane] = flip ? trials - obuf : obuf;
             flushed += nb;
             nb = 0;
-            const int32_t r = knuth_packet(cur, thr, lane, status);
+            const int32_t r = knuth_packet(cur, thr, lane, gd);
             if (lane < 4) o[4 * flushed + lane] = flip ? trials - r : r;
             ++flushed;
         }
@@ -980,11 +982,11 @@ __device__ void op_knuth_lp(Eng& e, uint64_t* carry, int32_t* o, int64_t T, floa
 // sequential loop. Larger means (thr < kKnuthPrefixThr), where most of the 64
 // speculative starts are wasted, take op_knuth_lp.
 __device__ void op_knuth(Eng& e, uint64_t* carry, float* ustage, int32_t* o, int64_t per, float thr,
-                         int32_t trials, int flip, uint32_t* status, int lane) {
+                         int32_t trials, int flip, Guard gd, int lane) {
     const int64_t T = per / 4;
     if (T == 0) return;
     if (thr < kKnuthPrefixThr) {
-        op_knuth_lp(e, carry, o, T, thr, trials, flip, status, lane);
+        op_knuth_lp(e, carry, o, T, thr, trials, flip, gd, lane);
         return;
     }
     HalfCursor cur = cursor_begin(e, carry, lane);
@@ -1067,7 +1069,7 @@ __device__ void op_knuth(Eng& e, uint64_t* carry, float* ustage, int32_t* o, int
             cur.advance(at);
         }
         if (slow && pk < T) {
-            const int32_t r = knuth_packet(cur, thr, lane, status);
+            const int32_t r = knuth_packet(cur, thr, lane, gd);
             if (lane < 4) o[4 * pk + lane] = flip ? trials - r : r;
             ++pk;
         }
@@ -1124,7 +1126,7 @@ __device__ void op_binomial_small(Eng& e, uint64_t* carry, int32_t* o, int64_t p
 // open lane takes its first accepted candidate and the packet consumes the
 // iterations up to the slowest lane, as the sequential loop does.
 __device__ void op_binomial_vop(Eng& e, uint64_t* carry, int32_t* o, int64_t per,
-                                const int32_t* trials, const float* prob, uint32_t* status,
+                                const int32_t* trials, const float* prob, Guard gd,
                                 const float* lg, int lane) {
     const int64_t T = per / 4;
     if (T == 0) return;
@@ -1171,8 +1173,8 @@ __device__ void op_binomial_vop(Eng& e, uint64_t* carry, int32_t* o, int64_t per
                 const bool c = (nem < val) && valid;
                 if (!(__ballot(c) & 0xFull)) break;
                 res += c ? 1 : 0;
-                if (it > kLockstepCap) {
-                    if (lane == 0) atomicOr(status, kStatusIterCap);
+                if (it > gd.lockstep_cap) {
+                    if (lane == 0) atomicOr(gd.status, kStatusIterCap);
                     break;
                 }
             }
@@ -1226,8 +1228,8 @@ __device__ void op_binomial_vop(Eng& e, uint64_t* carry, int32_t* o, int64_t per
                     if (A & (0x1111111111111111ull << q)) done |= 1ull << q;
                 cur.advance(2 * need);
                 if (all) break;
-                if (it > kLockstepCap) {
-                    if (lane == 0) atomicOr(status, kStatusIterCap);
+                if (it > gd.lockstep_cap) {
+                    if (lane == 0) atomicOr(gd.status, kStatusIterCap);
                     break;
                 }
             }
@@ -1252,7 +1254,7 @@ __device__ void op_binomial_vop(Eng& e, uint64_t* carry, int32_t* o, int64_t per
 // that stores it through one bpermute. The per-lane constants are computed for
 // eight packets at once (one packet-lane per thread) and fetched by bpermute.
 __device__ void op_gamma_vop(Eng& e, uint64_t* carry, float* o, int64_t per, const float* alpha,
-                             const float* beta, uint32_t* status, int lane) {
+                             const float* beta, Guard gd, int lane) {
     constexpr uint64_t kLaneBits = 0x0101010101010101ull;  // threads of float lane 0
     const int64_t T = per / 8;
     if (T == 0) return;
@@ -1336,8 +1338,8 @@ __device__ void op_gamma_vop(Eng& e, uint64_t* carry, float* o, int64_t per, con
             done |= newly;
             cur.advance(3 * need);
             if (all) break;
-            if (it > kLockstepCap) {
-                if (lane == 0) atomicOr(status, kStatusIterCap);
+            if (it > gd.lockstep_cap) {
+                if (lane == 0) atomicOr(gd.status, kStatusIterCap);
                 break;
             }
         }
@@ -1361,7 +1363,7 @@ __device__ void op_gamma_vop(Eng& e, uint64_t* carry, float* o, int64_t per, con
 // per-lane constants are computed for 16 packets at once (one packet-lane per
 // thread) and fetched by bpermute; op_binomial_vop does the same.
 __device__ void op_poisson_vop(Eng& e, uint64_t* carry, int32_t* o, int64_t per,
-                               const float* mean, uint32_t* status, const float* lg, int lane) {
+                               const float* mean, Guard gd, const float* lg, int lane) {
     const int64_t T = per / 4;
     if (T == 0) return;
     const int i = lane >> 2, j = lane & 3;
@@ -1412,8 +1414,8 @@ __device__ void op_poisson_vop(Eng& e, uint64_t* carry, int32_t* o, int64_t per,
                 }
                 cur.advance(B);
                 base += B;
-                if (base > kLockstepCap) {
-                    if (lane == 0) atomicOr(status, kStatusIterCap);
+                if (base > gd.lockstep_cap) {
+                    if (lane == 0) atomicOr(gd.status, kStatusIterCap);
                     first = first < 0 ? base : first;
                     break;
                 }
@@ -1464,8 +1466,8 @@ __device__ void op_poisson_vop(Eng& e, uint64_t* carry, int32_t* o, int64_t per,
                     if (A & (0x1111111111111111ull << q)) done |= 1ull << q;
                 cur.advance(2 * need);
                 if (all) break;
-                if (it > kLockstepCap) {
-                    if (lane == 0) atomicOr(status, kStatusIterCap);
+                if (it > gd.lockstep_cap) {
+                    if (lane == 0) atomicOr(gd.status, kStatusIterCap);
                     break;
                 }
             }
@@ -1614,7 +1616,7 @@ __device__ void op_multinomial_small(Eng& e, int32_t* out, int64_t n, int32_t T,
 template <bool REG>
 __device__ void op_binomial_row8(Eng& e, uint64_t* carry, int32_t* out, int64_t stride, int32_t* r,
                                  int32_t& rreg, int32_t t0, int64_t n, float p, float psmall, float plp,
-                                 float plq, uint32_t* status, const float* lg, int lane) {
+                                 float plq, Guard gd, const float* lg, int lane) {
     const int64_t T = n / 8;
     if (T == 0) return;
     const int i = lane >> 3, j = lane & 7;
@@ -1647,8 +1649,8 @@ __device__ void op_binomial_row8(Eng& e, uint64_t* carry, int32_t* out, int64_t 
                 const bool c = (nem < val) && valid;
                 if (!(__ballot(c) & 0xFFull)) break;
                 res = res + (c ? 1.0f : 0.0f);
-                if (it > kLockstepCap) {
-                    if (lane == 0) atomicOr(status, kStatusIterCap);
+                if (it > gd.lockstep_cap) {
+                    if (lane == 0) atomicOr(gd.status, kStatusIterCap);
                     break;
                 }
             }
@@ -1702,8 +1704,8 @@ __device__ void op_binomial_row8(Eng& e, uint64_t* carry, int32_t* out, int64_t 
                     if (A & (0x0101010101010101ull << q)) done |= 1ull << q;
                 cur.advance(2 * need);
                 if (all) break;
-                if (it > kLockstepCap) {
-                    if (lane == 0) atomicOr(status, kStatusIterCap);
+                if (it > gd.lockstep_cap) {
+                    if (lane == 0) atomicOr(gd.status, kStatusIterCap);
                     break;
                 }
             }

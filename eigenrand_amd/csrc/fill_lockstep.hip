@@ -27,7 +27,7 @@ namespace erd {
 __global__ __launch_bounds__(64) void k_poisson_knuth(EngineState* __restrict__ states,
                                                       int32_t* __restrict__ out, int64_t per,
                                                       float thr, int32_t trials, int flip,
-                                                      uint32_t* __restrict__ status) {
+                                                      Guard gd) {
     __shared__ uint64_t mt[kWords];
     __shared__ uint64_t carry[2];
     __shared__ __align__(16) float ustage[kKnuthStage];
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(64) void k_poisson_knuth(EngineState* __restrict__ 
     if (per / 4 == 0) return;
     Eng e;
     eng_load(e, mt, states + g, lane);
-    op_knuth(e, carry, ustage, out + g * per, per, thr, trials, flip, status, lane);
+    op_knuth(e, carry, ustage, out + g * per, per, thr, trials, flip, gd, lane);
     eng_store(e, states + g, lane);
 }
 
@@ -45,7 +45,7 @@ __global__ __launch_bounds__(64) void k_poisson_knuth(EngineState* __restrict__ 
 __global__ __launch_bounds__(64) void k_poisson_knuth_lp(EngineState* __restrict__ states,
                                                          int32_t* __restrict__ out, int64_t per,
                                                          float thr, int32_t trials, int flip,
-                                                         uint32_t* __restrict__ status) {
+                                                         Guard gd) {
     __shared__ uint64_t mt[kWords];
     __shared__ uint64_t carry[2];
     const int lane = threadIdx.x;
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(64) void k_poisson_knuth_lp(EngineState* __restrict
     if (per / 4 == 0) return;
     Eng e;
     eng_load(e, mt, states + g, lane);
-    op_knuth_lp(e, carry, out + g * per, per / 4, thr, trials, flip, status, lane);
+    op_knuth_lp(e, carry, out + g * per, per / 4, thr, trials, flip, gd, lane);
     eng_store(e, states + g, lane);
 }
 
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(64) void k_binomial_vop(EngineState* __restrict__ s
                                                      int32_t* __restrict__ out, int64_t per,
                                                      const int32_t* __restrict__ trials,
                                                      const float* __restrict__ prob,
-                                                     uint32_t* __restrict__ status,
+                                                     Guard gd,
                                                      const float* __restrict__ lg) {
     __shared__ uint64_t mt[kWords];
     __shared__ uint64_t carry[2];
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(64) void k_binomial_vop(EngineState* __restrict__ s
     if (per / 4 == 0) return;
     Eng e;
     eng_load(e, mt, states + g, lane);
-    op_binomial_vop(e, carry, out + g * per, per, trials + g * per, prob + g * per, status, lg,
+    op_binomial_vop(e, carry, out + g * per, per, trials + g * per, prob + g * per, gd, lg,
                     lane);
     eng_store(e, states + g, lane);
 }
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(64) void k_gamma_vop(EngineState* __restrict__ stat
                                                   float* __restrict__ out, int64_t per,
                                                   const float* __restrict__ alpha,
                                                   const float* __restrict__ beta,
-                                                  uint32_t* __restrict__ status) {
+                                                  Guard gd) {
     __shared__ uint64_t mt[kWords];
     __shared__ uint64_t carry[2 * kLanes];
     const int lane = threadIdx.x;
@@ -104,14 +104,14 @@ __global__ __launch_bounds__(64) void k_gamma_vop(EngineState* __restrict__ stat
     if (per / 8 == 0) return;
     Eng e;
     eng_load(e, mt, states + g, lane);
-    op_gamma_vop(e, carry, out + g * per, per, alpha + g * per, beta + g * per, status, lane);
+    op_gamma_vop(e, carry, out + g * per, per, alpha + g * per, beta + g * per, gd, lane);
     eng_store(e, states + g, lane);
 }
 
 __global__ __launch_bounds__(64) void k_poisson_vop(EngineState* __restrict__ states,
                                                     int32_t* __restrict__ out, int64_t per,
                                                     const float* __restrict__ mean,
-                                                    uint32_t* __restrict__ status,
+                                                    Guard gd,
                                                     const float* __restrict__ lg) {
     __shared__ uint64_t mt[kWords];
     __shared__ uint64_t carry[2];
@@ -120,35 +120,35 @@ __global__ __launch_bounds__(64) void k_poisson_vop(EngineState* __restrict__ st
     if (per / 4 == 0) return;
     Eng e;
     eng_load(e, mt, states + g, lane);
-    op_poisson_vop(e, carry, out + g * per, per, mean + g * per, status, lg, lane);
+    op_poisson_vop(e, carry, out + g * per, per, mean + g * per, gd, lg, lane);
     eng_store(e, states + g, lane);
 }
 
 hipError_t launch_fill_gamma_vop_f32(EngineState* st, int64_t n, float* out, int64_t per,
-                                     const float* alpha, const float* beta, uint32_t* status,
+                                     const float* alpha, const float* beta, Guard gd,
                                      hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_gamma_vop, dim3((unsigned)n), dim3(64), 0, s, st, out, per, alpha, beta,
-                       status);
+                       gd);
     return hipGetLastError();
 }
 
 hipError_t launch_fill_poisson_vop_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
-                                       const float* mean, uint32_t* status, hipStream_t s) {
+                                       const float* mean, Guard gd, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const float* lg = lgamma_table(s);
     if (!lg) return hipErrorOutOfMemory;
-    hipLaunchKernelGGL(k_poisson_vop, dim3((unsigned)n), dim3(64), 0, s, st, out, per, mean, status, lg);
+    hipLaunchKernelGGL(k_poisson_vop, dim3((unsigned)n), dim3(64), 0, s, st, out, per, mean, gd, lg);
     return hipGetLastError();
 }
 
 hipError_t launch_fill_poisson_knuth_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
-                                         double mean, int32_t trials, int flip, uint32_t* status,
+                                         double mean, int32_t trials, int flip, Guard gd,
                                          hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const float thr = (float)std::exp(-mean);
     hipLaunchKernelGGL(thr < kKnuthPrefixThr ? k_poisson_knuth_lp : k_poisson_knuth, dim3((unsigned)n), dim3(64), 0,
-                       s, st, out, per, thr, trials, flip, status);
+                       s, st, out, per, thr, trials, flip, gd);
     return hipGetLastError();
 }
 
@@ -161,13 +161,13 @@ hipError_t launch_fill_binomial_small_i32(EngineState* st, int64_t n, int32_t* o
 }
 
 hipError_t launch_fill_binomial_vop_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
-                                        const int32_t* trials, const float* p, uint32_t* status,
+                                        const int32_t* trials, const float* p, Guard gd,
                                         hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const float* lg = lgamma_table(s);
     if (!lg) return hipErrorOutOfMemory;
     hipLaunchKernelGGL(k_binomial_vop, dim3((unsigned)n), dim3(64), 0, s, st, out, per, trials, p,
-                       status, lg);
+                       gd, lg);
     return hipGetLastError();
 }
 

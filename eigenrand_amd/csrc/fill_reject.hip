@@ -61,7 +61,7 @@ const float* lgamma_table(hipStream_t s) {
 template <class P>
 __global__ __launch_bounds__(64) void k_fill_reject(EngineState* __restrict__ states,
                                                     typename P::Out* __restrict__ out, int64_t per,
-                                                    P pol, uint32_t* __restrict__ status,
+                                                    P pol, Guard gd,
                                                     GenState* __restrict__ gs) {
     __shared__ uint64_t mt[kWords];
     __shared__ RejectLds<P> s;
@@ -70,47 +70,47 @@ __global__ __launch_bounds__(64) void k_fill_reject(EngineState* __restrict__ st
     if (per / P::W == 0) return;
     Eng e;
     eng_load(e, mt, states + g, lane);
-    op_reject(e, s, out + g * per, per, pol, status, lane, gs ? gs + g : nullptr);
+    op_reject(e, s, out + g * per, per, pol, gd, lane, gs ? gs + g : nullptr);
     eng_store(e, states + g, lane);
 }
 
 // ------------------------------------------------------------- launchers ---
 template <class P>
 static hipError_t launch_reject(EngineState* st, int64_t n, typename P::Out* out, int64_t per,
-                                const P& pol, uint32_t* status, GenState* gs, hipStream_t s) {
+                                const P& pol, Guard gd, GenState* gs, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_reject<P>, dim3((unsigned)n), dim3(64), 0, s, st, out, per, pol, status, gs);
+    hipLaunchKernelGGL(k_fill_reject<P>, dim3((unsigned)n), dim3(64), 0, s, st, out, per, pol, gd, gs);
     return hipGetLastError();
 }
 
 hipError_t launch_fill_gamma_reject_f32(EngineState* st, int64_t n, float* out, int64_t per,
-                                        float alpha, float beta, uint32_t* status, hipStream_t s,
+                                        float alpha, float beta, Guard gd, hipStream_t s,
                                         GenState* gs) {
     if (alpha < 1.0f) {
         GammaGS pol{beta, (float)(kE / (alpha + kE)), 1 / alpha, alpha - 1};
-        return launch_reject(st, n, out, per, pol, status, gs, s);
+        return launch_reject(st, n, out, per, pol, gd, gs, s);
     }
     GammaTangent pol{beta, sqrtf(2 * alpha - 1), alpha - 1};
-    return launch_reject(st, n, out, per, pol, status, gs, s);
+    return launch_reject(st, n, out, per, pol, gd, gs, s);
 }
 
 hipError_t launch_fill_poisson_reject_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
-                                          const PoissonParams& pp, uint32_t* status, hipStream_t s,
+                                          const PoissonParams& pp, Guard gd, hipStream_t s,
                                           GenState* gs) {
     const float* lg = lgamma_table(s);
     if (!lg) return hipErrorOutOfMemory;
     PoissonTangent pol{(float)pp.sqrt_tmean, (float)pp.mean, (float)pp.log_mean, (float)pp.g1, lg};
-    return launch_reject(st, n, out, per, pol, status, gs, s);
+    return launch_reject(st, n, out, per, pol, gd, gs, s);
 }
 
 hipError_t launch_fill_binomial_reject_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
-                                           const BinomialParams& bp, uint32_t* status, hipStream_t s,
+                                           const BinomialParams& bp, Guard gd, hipStream_t s,
                                            GenState* gs) {
     const float* lg = lgamma_table(s);
     if (!lg) return hipErrorOutOfMemory;
     BinomialTangent pol{(float)bp.trials, (float)bp.sqrt_v, (float)bp.mean, (float)bp.log_small_p,
                         (float)bp.log_small_q, (float)bp.g1, bp.trials, bp.p == bp.small_p ? 0 : 1, lg};
-    return launch_reject(st, n, out, per, pol, status, gs, s);
+    return launch_reject(st, n, out, per, pol, gd, gs, s);
 }
 
 }  // namespace erd

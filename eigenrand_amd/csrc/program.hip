@@ -45,10 +45,10 @@ __device__ void run_gamma_int(Eng& e, ProgramLds& s, float* out, int64_t n, floa
 // gs: the generator's CompressMask queue carried across calls (BetaGen's two
 // GammaGen members), or NULL for a fresh generator.
 __device__ void run_gamma(Eng& e, ProgramLds& s, float* out, int64_t n, float alpha, float beta,
-                          uint32_t* status, int lane, GenState* gs = nullptr) {
+                          Guard gd, int lane, GenState* gs = nullptr) {
     if (alpha < 1.0f) {
         GammaGS pol{beta, (float)(kE / (alpha + kE)), 1 / alpha, alpha - 1};
-        op_reject(e, s.gs, out, n, pol, status, lane, gs);
+        op_reject(e, s.gs, out, n, pol, gd, lane, gs);
         return;
     }
     if (alpha == 1.0f) {
@@ -80,7 +80,7 @@ __device__ void run_gamma(Eng& e, ProgramLds& s, float* out, int64_t n, float al
         }
     }
     GammaTangent pol{beta, sqrtf(2 * alpha - 1), alpha - 1};
-    op_reject(e, s.gt, out, n, pol, status, lane, gs);
+    op_reject(e, s.gt, out, n, pol, gd, lane, gs);
 }
 
 // Per-segment state that must outlive the ops' LDS union: BetaGen's two
@@ -98,10 +98,10 @@ struct BetaLds {
 // with its own queue, combined as p1 / (p1 + p2). Generators are fresh per
 // fill: the queues start empty and their leftovers are dropped.
 __device__ void run_beta(Eng& e, ProgramLds& s, BetaLds& bl, float* out, int64_t n, float a, float b, float ratio,
-                         uint32_t* status, int lane) {
+                         Guard gd, int lane) {
     if (a < 1.0f && b < 1.0f) {
         BetaJohnk pol{1 / a, 1 / b, ratio};
-        op_reject(e, s.gt_beta, out, n, pol, status, lane);
+        op_reject(e, s.gt_beta, out, n, pol, gd, lane);
         return;
     }
     const int ia = (int)a, ib = (int)b;
@@ -113,9 +113,9 @@ __device__ void run_beta(Eng& e, ProgramLds& s, BetaLds& bl, float* out, int64_t
     if (lane < 2) bl.q[lane].count = 0;
     __syncthreads();
     for (int64_t pk = 0; pk < n / 8; ++pk) {
-        run_gamma(e, s, bl.x[0], 8, a, 1.0f, status, lane, &bl.q[0]);
+        run_gamma(e, s, bl.x[0], 8, a, 1.0f, gd, lane, &bl.q[0]);
         __syncthreads();
-        run_gamma(e, s, bl.x[1], 8, b, 1.0f, status, lane, &bl.q[1]);
+        run_gamma(e, s, bl.x[1], 8, b, 1.0f, gd, lane, &bl.q[1]);
         __syncthreads();
         if (lane < 8) {
             const float p1 = bl.x[0][lane], p2 = bl.x[1][lane];
@@ -165,7 +165,7 @@ __device__ void gamma_fixed_w4(Eng& e, uint64_t* carry, float* dst4, float alpha
 
 // Knuth's product loop with a threshold per lane (NegativeBinomialGen,
 // GammaPoisson.h:57-67); returns lane j's count.
-__device__ int32_t knuth_packet_lanes(Eng& e, uint64_t* carry, const float* thr4, uint32_t* status, int lane) {
+__device__ int32_t knuth_packet_lanes(Eng& e, uint64_t* carry, const float* thr4, Guard gd, int lane) {
     HalfCursor cur = cursor_begin(e, carry, lane);
     const int j = lane & 3;
     const float thr = thr4[j];
@@ -178,8 +178,8 @@ __device__ int32_t knuth_packet_lanes(Eng& e, uint64_t* carry, const float* thr4
         const bool c = thr < val;
         if (!(__ballot(c) & 0xFull)) break;
         res += c ? 1 : 0;
-        if (it > kLockstepCap) {
-            if (lane == 0) atomicOr(status, kStatusIterCap);
+        if (it > gd.lockstep_cap) {
+            if (lane == 0) atomicOr(gd.status, kStatusIterCap);
             break;
         }
     }
@@ -193,7 +193,7 @@ __device__ int32_t knuth_packet_lanes(Eng& e, uint64_t* carry, const float* thr4
 // rejection whose candidates share the gamma's CompressMask queue. A fresh
 // generator per fill.
 __device__ void run_negbin(Eng& e, ProgramLds& s, BetaLds& bl, int32_t* out, int64_t n, float alpha, float beta,
-                           uint32_t* status, const float* lg, int lane) {
+                           Guard gd, const float* lg, int lane) {
     GenState* q = &bl.q[0];
     if (lane == 0) q->count = 0;
     __syncthreads();
@@ -202,12 +202,12 @@ __device__ void run_negbin(Eng& e, ProgramLds& s, BetaLds& bl, int32_t* out, int
         float* mean = bl.x[0];
         if (alpha < 1.0f) {
             GammaGSW<4> pol{beta, (float)(kE / (alpha + kE)), 1 / alpha, alpha - 1};
-            op_reject(e, s.gs4, mean, 4, pol, status, lane, q);
+            op_reject(e, s.gs4, mean, 4, pol, gd, lane, q);
         } else if (alpha == 1.0f || ((float)count == alpha && count < 20)) {
             gamma_fixed_w4(e, s.vop.carry, mean, alpha, beta, lane);
         } else {
             GammaTangentW<4> pol{beta, sqrtf(2 * alpha - 1), alpha - 1};
-            op_reject(e, s.gt4, mean, 4, pol, status, lane, q);
+            op_reject(e, s.gt4, mean, 4, pol, gd, lane, q);
         }
         __syncthreads();
         bool big = false;
@@ -216,7 +216,7 @@ __device__ void run_negbin(Eng& e, ProgramLds& s, BetaLds& bl, int32_t* out, int
         if (!big) {
             if (lane < 4) bl.x[1][lane] = pexp(-mean[lane]);
             __syncthreads();
-            const int32_t r = knuth_packet_lanes(e, s.vop.carry, bl.x[1], status, lane);
+            const int32_t r = knuth_packet_lanes(e, s.vop.carry, bl.x[1], gd, lane);
             if (lane < 4) out[4 * pk + lane] = r;
         } else {
             if (lane < 4) {
@@ -228,13 +228,13 @@ __device__ void run_negbin(Eng& e, ProgramLds& s, BetaLds& bl, int32_t* out, int
             }
             __syncthreads();
             NegBinTangent pol{mean, bl.c[0], bl.c[1], bl.c[2], lg};
-            op_reject(e, s.nbt, out + 4 * pk, 4, pol, status, lane, q);
+            op_reject(e, s.nbt, out + 4 * pk, 4, pol, gd, lane, q);
         }
         __syncthreads();
     }
 }
 
-__device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& sg, uint32_t* status,
+__device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& sg, Guard gd,
                             const float* lg, int lane) {
     const int64_t n = sg.count;
     switch (sg.kind) {
@@ -285,15 +285,15 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
             break;
         }
         case SEG_GAMMA_F32:
-            run_gamma(e, s, (float*)sg.out, n, (float)sg.d[0], (float)sg.d[1], status, lane);
+            run_gamma(e, s, (float*)sg.out, n, (float)sg.d[0], (float)sg.d[1], gd, lane);
             break;
         case SEG_POISSON_I32: {
             // d: mean, ne_mean, sqrt_tmean, log_mean, g1
             if (sg.d[0] < 12) {
-                op_knuth(e, s.knuth.carry, s.knuth.u, (int32_t*)sg.out, n, (float)sg.d[1], 0, 0, status, lane);
+                op_knuth(e, s.knuth.carry, s.knuth.u, (int32_t*)sg.out, n, (float)sg.d[1], 0, 0, gd, lane);
             } else {
                 PoissonTangent pol{(float)sg.d[2], (float)sg.d[0], (float)sg.d[3], (float)sg.d[4], lg};
-                op_reject(e, s.pt, (int32_t*)sg.out, n, pol, status, lane);
+                op_reject(e, s.pt, (int32_t*)sg.out, n, pol, gd, lane);
             }
             break;
         }
@@ -304,24 +304,24 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
             if (trials < 25) {
                 op_binomial_small(e, s.carry_beta, (int32_t*)sg.out, n, trials, (float)sg.d[0], lane);
             } else if (sg.d[2] < 1.0) {
-                op_knuth(e, s.knuth.carry, s.knuth.u, (int32_t*)sg.out, n, (float)sg.d[7], trials, flip, status, lane);
+                op_knuth(e, s.knuth.carry, s.knuth.u, (int32_t*)sg.out, n, (float)sg.d[7], trials, flip, gd, lane);
             } else {
                 BinomialTangent pol{(float)trials, (float)sg.d[4], (float)sg.d[2], (float)sg.d[5],
                                     (float)sg.d[6], (float)sg.d[3], trials, flip, lg};
-                op_reject(e, s.bt, (int32_t*)sg.out, n, pol, status, lane);
+                op_reject(e, s.bt, (int32_t*)sg.out, n, pol, gd, lane);
             }
             break;
         }
         case SEG_BINOMIAL_VOP_I32:
             op_binomial_vop(e, s.vop.carry, (int32_t*)sg.out, n, (const int32_t*)sg.in0,
-                            (const float*)sg.in1, status, lg, lane);
+                            (const float*)sg.in1, gd, lg, lane);
             break;
         case SEG_GAMMA_VOP_F32:
             op_gamma_vop(e, s.vop.carry, (float*)sg.out, n, (const float*)sg.in0,
-                         (const float*)sg.in1, status, lane);
+                         (const float*)sg.in1, gd, lane);
             break;
         case SEG_POISSON_VOP_I32:
-            op_poisson_vop(e, s.vop.carry, (int32_t*)sg.out, n, (const float*)sg.in0, status,
+            op_poisson_vop(e, s.vop.carry, (int32_t*)sg.out, n, (const float*)sg.in0, gd,
                            lg, lane);
             break;
         case SEG_DISCARD:
@@ -359,10 +359,10 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
             break;
         }
         case SEG_NEGATIVE_BINOMIAL_I32:
-            run_negbin(e, s, bl, (int32_t*)sg.out, n, (float)sg.d[0], (float)sg.d[1], status, lg, lane);
+            run_negbin(e, s, bl, (int32_t*)sg.out, n, (float)sg.d[0], (float)sg.d[1], gd, lg, lane);
             break;
         case SEG_BETA_F32:
-            run_beta(e, s, bl, (float*)sg.out, n, (float)sg.d[0], (float)sg.d[1], (float)sg.d[2], status, lane);
+            run_beta(e, s, bl, (float*)sg.out, n, (float)sg.d[0], (float)sg.d[1], (float)sg.d[2], gd, lane);
             break;
         case SEG_MULTINOMIAL_ROWS: {
             // MultinomialGen's large-trials path (Multinomial.h:74-91) for the
@@ -391,11 +391,11 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
                 const float rlq = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, c_lq), q));
                 const int32_t t0 = row == 0 ? sg.i0 : -1;
                 if (n <= 64)
-                    op_binomial_row8<true>(e, s.vop.carry, o + row, K, rr, rreg, t0, n, rp, rps, rlp, rlq, status,
+                    op_binomial_row8<true>(e, s.vop.carry, o + row, K, rr, rreg, t0, n, rp, rps, rlp, rlq, gd,
                                            lg, lane);
                 else
                     op_binomial_row8<false>(e, s.vop.carry, o + row, K, rr, rreg, t0, n, rp, rps, rlp, rlq,
-                                            status, lg, lane);
+                                            gd, lg, lane);
             }
             if (n <= 64) {
                 if (lane < n) o[lane * K + (K - 1)] = rreg;
@@ -405,7 +405,7 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
             break;
         }
         default:
-            if (lane == 0) atomicOr(status, kStatusBadSegment);
+            if (lane == 0) atomicOr(gd.status, kStatusBadSegment);
             break;
     }
     __syncthreads();
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(64) void k_program(EngineState* __restrict__ states
                                                 const int32_t* __restrict__ engine_of,
                                                 const int32_t* __restrict__ begin,
                                                 const Segment* __restrict__ segs,
-                                                uint32_t* __restrict__ status,
+                                                Guard gd,
                                                 const float* __restrict__ lg) {
     __shared__ uint64_t mt[kWords];
     __shared__ ProgramLds s;
@@ -426,18 +426,18 @@ __global__ __launch_bounds__(64) void k_program(EngineState* __restrict__ states
     EngineState* st = states + engine_of[b];
     Eng e;
     eng_load(e, mt, st, lane);
-    for (int k = begin[b]; k < begin[b + 1]; ++k) run_segment(e, s, bl, segs[k], status, lg, lane);
+    for (int k = begin[b]; k < begin[b + 1]; ++k) run_segment(e, s, bl, segs[k], gd, lg, lane);
     eng_store(e, st, lane);
 }
 
 hipError_t launch_program(EngineState* st, int32_t n_groups, const int32_t* d_engine_of,
-                          const int32_t* d_begin, const Segment* d_segs, uint32_t* status,
+                          const int32_t* d_begin, const Segment* d_segs, Guard gd,
                           hipStream_t s) {
     if (n_groups <= 0) return hipSuccess;
     const float* lg = lgamma_table(s);
     if (!lg) return hipErrorOutOfMemory;
     hipLaunchKernelGGL(k_program, dim3((unsigned)n_groups), dim3(64), 0, s, st, d_engine_of, d_begin,
-                       d_segs, status, lg);
+                       d_segs, gd, lg);
     return hipGetLastError();
 }
 

@@ -272,6 +272,7 @@ struct Server {
     std::vector<char> stream_free;
     std::vector<void*> pin;        // per stream: pinned staging buffer
     std::vector<size_t> pin_cap;
+    uint32_t* pin_status = nullptr;  // per stream: the last batch's device status bits (pinned)
     std::deque<Batch*> inflight;
     std::unordered_map<uint64_t, std::unique_ptr<Model>> models;     // id -> model
     std::map<std::pair<uint64_t, uint64_t>, uint64_t> model_by_hash;  // (client, contents) -> id
@@ -373,6 +374,9 @@ int Server::prepare_and_launch(Batch* b) {
     const int E = cfg.engines_per_client;
     Stage st;
     std::vector<Reloc> relocs;
+    // This batch's own status word (zeroed by the upload), so that a cap hit
+    // is traced to its batch.
+    const size_t off_status = st.reserve(sizeof(uint32_t));
     // Engines of clients seen for the first time:
     // engine g of client c = Vmt19937_64(splitmix64(c ^ salt) + 4g).
     std::vector<std::pair<int, size_t>> seed_jobs;  // (slot, staged seeds)
@@ -411,8 +415,11 @@ int Server::prepare_and_launch(Batch* b) {
                     (void)hipFree(fhi);
                     return ERD_ERR_OOM;
                 }
+                // The planes are complete before they are published: a later
+                // request of the model may run on another stream.
                 if (hipMemcpyAsync(fhi, hi, sizeof(uint16_t) * dp * dp, hipMemcpyDeviceToDevice, s) != hipSuccess ||
-                    hipMemcpyAsync(flo, lo, sizeof(uint16_t) * dp * dp, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+                    hipMemcpyAsync(flo, lo, sizeof(uint16_t) * dp * dp, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                    hipStreamSynchronize(s) != hipSuccess) {
                     (void)hipFree(fhi);
                     (void)hipFree(flo);
                     return ERD_ERR_HIP;
@@ -558,7 +565,10 @@ int Server::prepare_and_launch(Batch* b) {
             return ERD_ERR_HIP;
     if (!segs.empty() &&
         launch_program(pool->states, (int32_t)engine_list.size(), (const int32_t*)(dev + off_eng),
-                       (const int32_t*)(dev + off_begin), (const Segment*)(dev + off_segs), pool->status,
+                       (const int32_t*)(dev + off_begin), (const Segment*)(dev + off_segs),
+                       Guard{(uint32_t*)(dev + off_status),
+                             cfg.reject_cap > 0 ? cfg.reject_cap : kDefaultRejectCap,
+                             cfg.lockstep_cap > 0 ? cfg.lockstep_cap : kDefaultLockstepCap},
                        s) != hipSuccess)
         return ERD_ERR_HIP;
     if (!mv.empty()) {
@@ -574,6 +584,10 @@ int Server::prepare_and_launch(Batch* b) {
             if (rc) return rc;
         }
     }
+    // the status word comes back with the batch (read once its event is done)
+    if (hipMemcpyAsync(pin_status + 16 * si, dev + off_status, sizeof(uint32_t), hipMemcpyDeviceToHost, s) !=
+        hipSuccess)
+        return ERD_ERR_HIP;
     if (hipEventRecord(b->ev, s) != hipSuccess) return ERD_ERR_HIP;
     return ERD_OK;
 }
@@ -670,6 +684,10 @@ void Server::completer_loop() {
             }
             const hipError_t q = hipEventQuery(c->ev);
             if (q == hipSuccess) {
+                // kernel-raised errors of this batch (its own status word)
+                const uint32_t bits = reinterpret_cast<volatile uint32_t*>(pin_status)[16 * c->stream_idx];
+                if (bits & kStatusIterCap) c->status = ERD_ERR_ITER_CAP;
+                else if (bits) c->status = ERD_ERR_INVALID;
                 b = c;
                 break;
             }
@@ -693,6 +711,15 @@ void Server::completer_loop() {
                 q->t_done = t;
                 q->state = b->status == ERD_OK ? 2 : 3;
                 q->status = b->status;
+                if (b->status != ERD_OK) {
+                    // A failed request returns no output: its block (which may
+                    // hold another request's earlier data where the sampler did
+                    // not write) goes back to the pool. Engines it was to seed
+                    // are seeded again by the client's next request.
+                    blocks->put(q->out);
+                    q->out = nullptr;
+                    if (q->seed_first) slot_seeded[q->slot] = 0;
+                }
                 for_held(q, [&](int32_t g) {
                     if (owner[g] == b->serial) owner[g] = 0;
                 });
@@ -751,6 +778,13 @@ int erd_server_create(erd_server** out, const erd_server_config* cfg) {
         S.stream_free.push_back(1);
         S.pin.push_back(nullptr);
         S.pin_cap.push_back(0);
+    }
+    if (hipHostMalloc((void**)&S.pin_status, sizeof(uint32_t) * 16 * (size_t)cfg->max_inflight,
+                      hipHostMallocDefault) != hipSuccess) {
+        for (hipStream_t st : S.streams) (void)hipStreamDestroy(st);
+        erd_engines_destroy(S.pool);
+        delete srv;
+        return ERD_ERR_OOM;
     }
     S.launcher = std::thread([&S] { S.launcher_loop(); });
     S.completer = std::thread([&S] { S.completer_loop(); });
@@ -817,7 +851,10 @@ int erd_server_submit(erd_server* srv, const erd_serve_request* r, uint64_t* tic
         q->out_elems = (int64_t)d * q->per * q->e_req;
         q->out_bytes = sizeof(float) * (size_t)q->out_elems;
     } else if (r->kind == ERD_SERVE_MULTINOMIAL) {
-        if (r->dim <= 0 || !r->host_param || r->a < 0) return ERD_ERR_INVALID;
+        // trials: an integer in [0, INT32_MAX] (it is cast to int32 at launch)
+        if (r->dim <= 0 || !r->host_param || !(r->a >= 0.0 && r->a <= 2147483647.0) ||
+            std::floor(r->a) != r->a)
+            return ERD_ERR_INVALID;
         q->param.assign(r->host_param, r->host_param + r->dim);
         double sum = 0;
         for (double v : q->param) {
@@ -972,6 +1009,7 @@ int erd_server_destroy(erd_server* srv) {
     for (hipStream_t st : S.streams) (void)hipStreamDestroy(st);
     for (void* p : S.pin)
         if (p) (void)hipHostFree(p);
+    if (S.pin_status) (void)hipHostFree(S.pin_status);
     erd_engines_destroy(S.pool);
     delete srv;
     return ERD_OK;

@@ -24,7 +24,8 @@ class ServerConfig(C.Structure):
     _fields_ = [("max_clients", C.c_int64), ("engines_per_client", C.c_int32),
                 ("max_inflight", C.c_int32), ("hbm_budget_bytes", C.c_int64),
                 ("max_batch_requests", C.c_int64), ("seed_salt", C.c_uint64),
-                ("max_batch_elements", C.c_int64), ("engine_work", C.c_int64)]
+                ("max_batch_elements", C.c_int64), ("engine_work", C.c_int64),
+                ("reject_cap", C.c_int32), ("lockstep_cap", C.c_int32)]
 
 
 class ServeRequest(C.Structure):
@@ -105,10 +106,10 @@ def kind_layout(kind):
 class Server:
     def __init__(self, max_clients=64, engines_per_client=256, max_inflight=8,
                  hbm_budget_bytes=64 << 30, max_batch_requests=256, seed_salt=0,
-                 max_batch_elements=0, engine_work=0):
+                 max_batch_elements=0, engine_work=0, reject_cap=0, lockstep_cap=0):
         self.cfg = ServerConfig(max_clients, engines_per_client, max_inflight, hbm_budget_bytes,
                                 max_batch_requests, seed_salt & ((1 << 64) - 1), max_batch_elements,
-                                engine_work)
+                                engine_work, reject_cap, lockstep_cap)
         h = C.c_void_p()
         _lib.call("erd_server_create", C.byref(h), C.byref(self.cfg))
         self._h = h

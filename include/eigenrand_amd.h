@@ -72,8 +72,26 @@ int erd_engines_restore(erd_engines* e, const void* host_src, void* stream);
 int erd_engines_copy(erd_engines* dst, const erd_engines* src, void* stream);
 /* MersenneTwister::discard(calls) (PacketRandomEngine.h:317-323) on every engine. */
 int erd_engines_discard(erd_engines* e, int64_t calls, void* stream);
-/* Read and clear the device status bits raised by kernels on this set (sync). */
+/* Read and clear the device status bits raised by kernels on this set (sync):
+ * ERD_STATUS_ITER_CAP when a sampler gave up at an iteration cap (the
+ * outputs of that launch are then unspecified), ERD_STATUS_BAD_SEGMENT for a
+ * malformed request program. Fills return ERD_OK when enqueued; this call is
+ * how a caller learns that one of them gave up. */
+#define ERD_STATUS_ITER_CAP 1u
+#define ERD_STATUS_BAD_SEGMENT 2u
 int erd_engines_status(erd_engines* e, uint32_t* status_out, void* stream);
+/* Iteration caps of the launches on this set (views created from it later
+ * copy them). The reference's rejection and lockstep loops run until they
+ * succeed (RandUtils.h:342-346; Discrete.h:860-873, 1139-1185); here a
+ * rejection fill gives up after reject_cap candidates without an output
+ * packet, and a lockstep packet after lockstep_cap iterations, and the launch
+ * raises ERD_STATUS_ITER_CAP. The caps are checked once per wave step (16
+ * candidates of a rejection fill; 1 to 16 iterations of a lockstep packet,
+ * by sampler; the windowed Knuth fills finish a packet inside its window of
+ * at most 64 slots and check the cap only for a packet that runs past it),
+ * so a packet may run up to one step past its cap. 0 selects the default
+ * (2^22 and 2^20); negative values are invalid. */
+int erd_engines_set_iteration_caps(erd_engines* e, int32_t reject_cap, int32_t lockstep_cap);
 
 /* ---------------------------------------------------- raw engine draws --- */
 /* Vmt19937_64::operator() (PacketRandomEngine.h:295-310): out[4k+l] = lane l
@@ -448,6 +466,12 @@ typedef struct erd_server_config {
     int64_t max_batch_elements;  /* cap on one batch's outputs (0: none); a batch takes >= 1 request */
     int64_t engine_work;         /* normal-equivalent samples per engine when a request is spread
                                     over its client's engines (0: 65536) */
+    /* Iteration caps of the batches (0: the defaults, as for
+     * erd_engines_set_iteration_caps). A batch whose sampler gives up fails all
+     * its requests with ERD_ERR_ITER_CAP and returns no output for them
+     * (erd_server_wait gives out = NULL). */
+    int32_t reject_cap;
+    int32_t lockstep_cap;
 } erd_server_config;
 
 typedef struct erd_serve_request {

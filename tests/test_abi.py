@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from conftest import ROOT
@@ -56,3 +57,27 @@ def test_cpp_front_end_compiles():
         r = subprocess.run([cxx, "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
                             "-I", os.path.join(ROOT, "include"), path], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+def test_vop_parameter_tensors_are_checked_before_launch():
+    """Host tensors and wrong dtypes raise InvalidArgument before any engine,
+    allocation or launch is touched (so no GPU is needed to see it)."""
+    import torch
+    import eigenrand_amd as er
+    f32 = torch.ones(8, dtype=torch.float32)       # host memory
+    i32 = torch.ones(8, dtype=torch.int32)
+    cases = [
+        lambda: er.gamma_vop(None, f32, f32),
+        lambda: er.poisson_vop(None, f32),
+        lambda: er.normal_vop(None, f32, f32),
+        lambda: er.binomial_vop(None, i32, f32),
+        lambda: er.exponential_vop(None, f32),
+        lambda: er.gamma_vop(None, f32.half(), f32),
+        lambda: er.poisson_vop(None, f32.double()),
+        lambda: er.binomial_vop(None, i32.long(), f32),
+        lambda: er.binomial_vop(None, i32.float(), f32),
+        lambda: er.gamma_vop(None, np.ones(8, np.float32), f32),
+    ]
+    for fn in cases:
+        with pytest.raises(er.InvalidArgument):
+            fn()
