@@ -106,6 +106,7 @@ _SIGS = {
     "erd_generator_reset": [P, P],
     "erd_generator_fill": [P, P, P, I64, P],
     "erd_run_requests": [P, P, I64, P],
+    "erd_run_requests_noalloc": [P, P, I64, P],
     "erd_cholesky_f64_batch": [I64, P, P, P, P],
     "erd_mvnormal_batch": [P, I64, P, P, P, P, P, P, P],
     "erd_wishart_batch": [P, I64, P, P, P, P, P, P],

@@ -148,6 +148,8 @@ hipError_t launch_fill_poisson_vop_i32(EngineState* st, int64_t n, int32_t* out,
 // built on first use (one kernel on `s`, then a synchronisation of `s`) and
 // kept until the process ends. NULL if it cannot be allocated.
 const float* lgamma_table(hipStream_t s);
+// The current device's table if it has been built, else NULL (builds nothing).
+const float* lgamma_table_built();
 hipError_t launch_program(EngineState* st, int32_t n_groups, const int32_t* d_engine_of,
                           const int32_t* d_begin, const Segment* d_segs, Guard gd,
                           hipStream_t s);

@@ -57,6 +57,13 @@ const float* lgamma_table(hipStream_t s) {
     return g_lg[dev];
 }
 
+const float* lgamma_table_built() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_lg_mutex);
+    return g_lg[dev];
+}
+
 // gs (optional): per-engine queues of a reused generator object.
 template <class P>
 __global__ __launch_bounds__(64) void k_fill_reject(EngineState* __restrict__ states,

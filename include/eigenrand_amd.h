@@ -319,6 +319,20 @@ typedef struct erd_request {
 
 int erd_run_requests(erd_engines* e, const erd_request* requests, int64_t n_requests, void* stream);
 
+/* No counterpart in the reference: erd_run_requests for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same requests, kernels and results, but
+ * the segment program goes into a fixed device array of the library (about
+ * 290 KB, held on each GPU it runs on until the process ends) instead of a
+ * stream-ordered allocation, so at most ERD_NOALLOC_MAX_REQUESTS requests per
+ * call (more: ERD_ERR_INVALID). It allocates nothing; the engine set must have
+ * been created on the current device (its creation builds the table the
+ * tangent samplers read; without it: ERD_ERR_UNSUPPORTED). Calls that overlap
+ * on one GPU (other streams or threads) share the array and race; calls on
+ * one stream are ordered. */
+#define ERD_NOALLOC_MAX_REQUESTS 4096
+int erd_run_requests_noalloc(erd_engines* e, const erd_request* requests, int64_t n_requests, void* stream);
+
 /* ---------------------------------------------------- generator objects --- */
 /* A reused generator object on n engines: `auto gen = Rand::makeXxxGen(a, b);`
  * followed by several gen.generate(per_engine, 1, engine_g) calls. The
