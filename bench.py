@@ -42,30 +42,63 @@ ACCURACY = {
 }
 
 
-def cpu_baseline(max_threads=16):
-    """Oracle restatement (C++, lane-emulated AVX2 layout) on the host: one
-    engine per thread over the same 4096-engine workload."""
+def _host_isa():
+    try:
+        with open("/proc/cpuinfo") as f:
+            flags = next((l for l in f if l.startswith("flags")), "")
+    except OSError:
+        return "unknown"
+    return "avx512f" if " avx512f" in flags else "avx2" if " avx2" in flags else "sse"
+
+
+def cpu_baseline(max_threads=16, reps=3):
+    """CPU baselines on this host, timed like the reference's BenchmarkHelper
+    (wall clock, repeated, mean and stdev; benchmark/utils.hpp:10-56):
+    - the oracle restatement of the reference's packet algorithm (C++, AVX2
+      lane layout) over the whole workload, one engine per thread at a time,
+      on every core this process may use (the GPU box grants 16);
+    - the same restatement on one thread over a 256-engine sample;
+    - the reference benchmark's own comparator, std::mt19937_64 +
+      std::normal_distribution<float> (benchmark/benchmark.cpp:531-543), on
+      the same threads over 2^28 samples and on one thread over 2^24.
+    The reference itself cannot be built (Eigen is absent)."""
+    import statistics
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    threads = max(1, min(max_threads, os.cpu_count() or 1))
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    threads = max(1, min(max_threads, avail))
+
+    def rate(fn, n):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        m = statistics.mean(ts)
+        return {"gsamples_per_s": round(n / m / 1e9, 4), "mean_s": round(m, 4),
+                "stdev_s": round(statistics.stdev(ts), 4) if len(ts) > 1 else 0.0}
+
     e = pyoracle.Engines(n=N_ENGINES, seed0=0, stride=4)
-    t0 = time.perf_counter()
-    pyoracle.fill_normal_f32(e, PER_ENGINE, threads=threads)
-    dt = time.perf_counter() - t0
-    # single-thread rate on a 64-engine sample of the same workload
-    e1 = pyoracle.Engines(n=64, seed0=0, stride=4)
-    t1 = time.perf_counter()
-    pyoracle.fill_normal_f32(e1, PER_ENGINE, threads=1)
-    dt1 = time.perf_counter() - t1
+    port_all = rate(lambda: pyoracle.fill_normal_f32(e, PER_ENGINE, threads=threads), N_SAMPLES)
+    e1 = pyoracle.Engines(n=256, seed0=0, stride=4)
+    port_one = rate(lambda: pyoracle.fill_normal_f32(e1, PER_ENGINE, threads=1), 256 * PER_ENGINE)
+    std_all = rate(lambda: pyoracle.std_normal_f32(N_SAMPLES, 0, threads), N_SAMPLES)
+    std_one = rate(lambda: pyoracle.std_normal_f32(1 << 24, 0, 1), 1 << 24)
     return {
-        "value": N_SAMPLES / dt / 1e9,
+        "value": port_all["gsamples_per_s"],
         "unit": "Gsamples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"full workload: 4096 engines x 65536 normals, {threads} threads (one engine per "
-                  f"thread at a time); reference itself not buildable (Eigen absent)",
-        "single_thread_value": 64 * PER_ENGINE / dt1 / 1e9,
-        "single_thread_sample": "64 engines x 65536 normals on 1 thread",
+        "sample": f"whole workload (4096 engines x 65536 normals) on {threads} threads, one engine per thread "
+                  f"at a time, {reps} repeats; the reference itself is not buildable (Eigen absent)",
+        "host_cpus": os.cpu_count(),
+        "usable_cpus": avail,
+        "isa": _host_isa(),
+        "restatement_all_threads": port_all,
+        "restatement_one_thread": dict(port_one, sample="256 engines x 65536 normals, 1 thread"),
+        "std_comparator_all_threads": dict(std_all, sample=f"2^28 normals, std::mt19937_64 + "
+                                                           f"std::normal_distribution<float>, {threads} threads"),
+        "std_comparator_one_thread": dict(std_one, sample="2^24 normals, 1 thread"),
     }
 
 

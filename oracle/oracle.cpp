@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <memory>
 #include <limits>
+#include <random>
 #include "orc_math.h"
 
 using namespace orc;
@@ -1572,4 +1573,24 @@ int orc_fill_discrete_i32(void* states, int64_t n_engines, int32_t* out, int64_t
     }
     return 0;
 }
+}
+
+// The reference benchmark's own comparator (benchmark/benchmark.cpp:531-543):
+// scalar std::mt19937_64 and std::normal_distribution<float>. It is not part
+// of the restatement and no check uses it; it is timed beside the restatement
+// as a CPU baseline. Thread t fills the t-th contiguous slice from its own
+// std::mt19937_64(seed + t).
+extern "C" void orc_std_normal_f32(float* out, int64_t n, uint64_t seed, int threads) {
+    threads = std::max(1, threads);
+    std::vector<std::thread> pool;
+    const int64_t chunk = (n + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+        const int64_t lo = std::min<int64_t>(n, t * chunk), hi = std::min<int64_t>(n, lo + chunk);
+        pool.emplace_back([=] {
+            std::mt19937_64 rng(seed + (uint64_t)t);
+            std::normal_distribution<float> nd;
+            for (int64_t i = lo; i < hi; ++i) out[i] = nd(rng);
+        });
+    }
+    for (auto& th : pool) th.join();
 }

@@ -415,3 +415,13 @@ class Generators:
         if getattr(self, "h", None) is not None and _lib is not None:
             _lib.orc_gen_destroy(self.h)
             self.h = None
+
+
+def std_normal_f32(n, seed=0, threads=1):
+    """std::mt19937_64 + std::normal_distribution<float> (the reference
+    benchmark's comparator), `threads` contiguous slices."""
+    out = np.empty(int(n), dtype=np.float32)
+    f = lib().orc_std_normal_f32
+    f.restype = None
+    f(_p(out), I64(int(n)), C.c_uint64(seed), C.c_int(threads))
+    return out
