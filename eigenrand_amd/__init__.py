@@ -337,8 +337,25 @@ def normal_f64(rows, cols, urng, mean=None, stdev=None, out=None, stream=None):
 
 def normal_vop(urng, mean, stdev, out=None, stream=None):
     """Rand::normal(urng, mean_array, stdev_array) (NormalVGen); mean/stdev are
-    CUDA float32 tensors of n*per_engine elements (engine g uses its block)."""
+    CUDA float32 tensors of n*per_engine elements (engine g uses its block).
+    One of them may be a Python number: the reference's mixed overloads
+    normal(urng, a, b_scalar) / normal(urng, a_scalar, b) (NormalExp.h:
+    1304-1335)."""
     torch = _torch()
+    if isinstance(stdev, (int, float)) and not isinstance(mean, (int, float)):
+        mean = _param(mean, torch.float32, "erd_fill_normal_vop_vs_f32: mean")
+        per = _vop_per(urng, mean.numel(), "erd_fill_normal_vop_vs_f32")
+        buf = _alloc(urng, per, torch.float32, out)
+        _lib.call("erd_fill_normal_vop_vs_f32", urng.handle, _ptr(buf), per, _ptr(mean), float(stdev),
+                  _stream_ptr(stream))
+        return buf
+    if isinstance(mean, (int, float)) and not isinstance(stdev, (int, float)):
+        stdev = _param(stdev, torch.float32, "erd_fill_normal_vop_sv_f32: stdev")
+        per = _vop_per(urng, stdev.numel(), "erd_fill_normal_vop_sv_f32")
+        buf = _alloc(urng, per, torch.float32, out)
+        _lib.call("erd_fill_normal_vop_sv_f32", urng.handle, _ptr(buf), per, float(mean), _ptr(stdev),
+                  _stream_ptr(stream))
+        return buf
     mean = _param(mean, torch.float32, "erd_fill_normal_vop_f32: mean")
     stdev = _param(stdev, torch.float32, "erd_fill_normal_vop_f32: stdev")
     if mean.numel() != stdev.numel() or mean.numel() % len(urng):
@@ -348,6 +365,12 @@ def normal_vop(urng, mean, stdev, out=None, stream=None):
     _lib.call("erd_fill_normal_vop_f32", urng.handle, _ptr(buf), per, _ptr(mean), _ptr(stdev),
               _stream_ptr(stream))
     return buf
+
+
+def _vop_per(urng, numel, name):
+    if numel % len(urng):
+        raise InvalidArgument(name, 1)
+    return numel // len(urng)
 
 
 def exponential(rows, cols, urng, lmbda=1.0, out=None, stream=None):
@@ -525,8 +548,24 @@ def extreme_value_vop(urng, a, b, out=None, stream=None):
 
 def binomial_vop(urng, trials, p, out=None, stream=None):
     """Rand::binomial(urng, trials_array, p_array) (BinomialVGen<int32_t>);
-    trials int32 and p float32 CUDA tensors of n*per_engine elements."""
+    trials int32 and p float32 CUDA tensors of n*per_engine elements. One of
+    them may be a Python number: the mixed overloads binomial(urng, trials, p)
+    with a scalar p or scalar trials (Discrete.h:1852-1897)."""
     torch = _torch()
+    if isinstance(p, (int, float)) and not isinstance(trials, (int, float)):
+        trials = _param(trials, torch.int32, "erd_fill_binomial_vop_vs_i32: trials")
+        per = _vop_per(urng, trials.numel(), "erd_fill_binomial_vop_vs_i32")
+        buf = _alloc(urng, per, torch.int32, out)
+        _lib.call("erd_fill_binomial_vop_vs_i32", urng.handle, _ptr(buf), per, _ptr(trials), float(p),
+                  _stream_ptr(stream))
+        return buf
+    if isinstance(trials, (int, np.integer)) and not isinstance(p, (int, float)):
+        p = _param(p, torch.float32, "erd_fill_binomial_vop_sv_i32: p")
+        per = _vop_per(urng, p.numel(), "erd_fill_binomial_vop_sv_i32")
+        buf = _alloc(urng, per, torch.int32, out)
+        _lib.call("erd_fill_binomial_vop_sv_i32", urng.handle, _ptr(buf), per, int(trials), _ptr(p),
+                  _stream_ptr(stream))
+        return buf
     trials = _param(trials, torch.int32, "erd_fill_binomial_vop_i32: trials")
     p = _param(p, torch.float32, "erd_fill_binomial_vop_i32: p")
     if trials.numel() != p.numel() or trials.numel() % len(urng):
@@ -875,12 +914,14 @@ def alias_table(weights):
 
 def discrete(rows, cols, urng, weights, out=None, stream=None, precision="int32"):
     """Rand::discrete<MatrixXi>(rows, cols, urng, weights) (int32 precision);
-    precision="float" is Rand::discreteF (DiscreteGen<int32_t, float>)."""
+    precision="float" is Rand::discreteF (DiscreteGen<int32_t, float>) and
+    precision="double" Rand::discreteD (DiscreteGen<int32_t, double>)."""
     torch = _torch()
     w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64))
     per = int(rows) * int(cols)
     buf = _alloc(urng, per, torch.int32, out)
-    name = {"int32": "erd_fill_discrete_i32", "float": "erd_fill_discrete_f_i32"}[precision]
+    name = {"int32": "erd_fill_discrete_i32", "float": "erd_fill_discrete_f_i32",
+            "double": "erd_fill_discrete_d_i32"}[precision]
     _lib.call(name, urng.handle, _ptr(buf), per, w.ctypes.data_as(C.c_void_p), w.size, _stream_ptr(stream))
     return _shape(buf, urng, rows, cols) if out is None else buf
 

@@ -65,6 +65,11 @@ _SIGS = {
     "erd_fill_exponential_f64": [P, P, I64, F64, P],
     "erd_fill_cauchy_f64": [P, P, I64, F64, F64, P],
     "erd_fill_normal_f32": [P, P, I64, F32, F32, P],
+    "erd_fill_normal_vop_vs_f32": [P, P, I64, P, F32, P],
+    "erd_fill_discrete_d_i32": [P, P, I64, P, I64, P],
+    "erd_fill_normal_vop_sv_f32": [P, P, I64, F32, P, P],
+    "erd_fill_binomial_vop_vs_i32": [P, P, I64, P, F32, P],
+    "erd_fill_binomial_vop_sv_i32": [P, P, I64, INT, P, P],
     "erd_fill_std_normal_f32_fast": [P, P, I64, P],
     "erd_fill_normal_f32_fast": [P, P, I64, F32, F32, P],
     "erd_store_ceiling_f32": [P, I64, I64, P],

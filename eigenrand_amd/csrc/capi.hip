@@ -287,6 +287,24 @@ int erd_fill_normal_vop_f32(erd_engines* e, float* out, int64_t per, const float
     return hip_status(erd::launch_fill_normal_vop_f32(e->states, e->count, out, per, mean, sd, S(stream)));
 }
 
+int erd_fill_normal_vop_vs_f32(erd_engines* e, float* out, int64_t per, const float* mean, float sd,
+                               void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    if (per > 0 && !mean) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_fill_normal_vop_f32(e->states, e->count, out, per, mean, nullptr, S(stream),
+                                                      0.f, sd));
+}
+
+int erd_fill_normal_vop_sv_f32(erd_engines* e, float* out, int64_t per, float mean, const float* sd,
+                               void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    if (per > 0 && !sd) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_fill_normal_vop_f32(e->states, e->count, out, per, nullptr, sd, S(stream),
+                                                      mean, 1.f));
+}
+
 int erd_fill_exponential_f32(erd_engines* e, float* out, int64_t per, float lambda, void* stream) {
     int rc = check_fill(e, out, per, 8);
     if (rc) return rc;
@@ -489,8 +507,29 @@ int erd_fill_binomial_vop_i32(erd_engines* e, int32_t* out, int64_t per, const i
     int rc = check_fill(e, out, per, 4);
     if (rc) return rc;
     if (per > 0 && (!trials || !p)) return ERD_ERR_INVALID;
-    return hip_status(erd::launch_fill_binomial_vop_i32(e->states, e->count, out, per, trials, p,
-                                                        guard_of(e), S(stream)));
+    return hip_status(erd::launch_fill_binomial_vop_i32(e->states, e->count, out, per,
+                                                        erd::Param<int32_t>{trials, 0},
+                                                        erd::Param<float>{p, 0.f}, guard_of(e), S(stream)));
+}
+
+int erd_fill_binomial_vop_vs_i32(erd_engines* e, int32_t* out, int64_t per, const int32_t* trials, float p,
+                                 void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    if (per > 0 && !trials) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_fill_binomial_vop_i32(e->states, e->count, out, per,
+                                                        erd::Param<int32_t>{trials, 0},
+                                                        erd::Param<float>{nullptr, p}, guard_of(e), S(stream)));
+}
+
+int erd_fill_binomial_vop_sv_i32(erd_engines* e, int32_t* out, int64_t per, int32_t trials, const float* p,
+                                 void* stream) {
+    int rc = check_fill(e, out, per, 4);
+    if (rc) return rc;
+    if (per > 0 && !p) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_fill_binomial_vop_i32(e->states, e->count, out, per,
+                                                        erd::Param<int32_t>{nullptr, trials},
+                                                        erd::Param<float>{p, 0.f}, guard_of(e), S(stream)));
 }
 
 int erd_fill_chisquared_f32(erd_engines* e, float* out, int64_t per, float n, void* stream) {

@@ -49,6 +49,17 @@ struct GenState {
 };
 static_assert(sizeof(GenState) == 40, "generator state layout");
 
+// A per-element parameter array, or one value for every element: the
+// reference's mixed overloads pass a scalar as a constant array of the same
+// shape (Discrete.h:1852-1897, NormalExp.h:1312-1335), so element i reads v.
+template <class T>
+struct Param {
+    const T* p;  // NULL: every element is v
+    T v;
+    __device__ T operator[](int64_t i) const { return p ? p[i] : v; }
+    __device__ Param operator+(int64_t off) const { return {p ? p + off : nullptr, v}; }
+};
+
 // Error bits a kernel can raise (OR-ed into a device status word).
 constexpr uint32_t kStatusIterCap = 1u;
 constexpr uint32_t kStatusBadSegment = 2u;

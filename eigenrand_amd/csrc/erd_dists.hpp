@@ -159,8 +159,13 @@ struct NormalF32 {
     GenState* gs;  // reused generator: receives the cos packet of a trailing half unit
     __device__ void rebase(int64_t off) {
         out += off;
-        if (MODE == 2 || MODE == 4) { vmean += off; vsd += off; }
+        if (MODE == 2 || MODE == 4) {  // NULL: the scalar mean / sd for every element
+            if (vmean) vmean += off;
+            if (vsd) vsd += off;
+        }
     }
+    __device__ float vm(int64_t i) const { return vmean ? vmean[i] : mean; }
+    __device__ float vs(int64_t i) const { return vsd ? vsd[i] : sd; }
     __device__ static int64_t units(int64_t n) { return (n + 15) / 16; }
     __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
         float sn[2], cs[2];
@@ -193,8 +198,8 @@ struct NormalF32 {
         } else if (MODE == 2 || MODE == 4) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                sn[h] = sn[h] * vsd[base + h] + vmean[base + h];
-                if (full) cs[h] = cs[h] * vsd[base + 8 + h] + vmean[base + 8 + h];
+                sn[h] = sn[h] * vs(base + h) + vm(base + h);
+                if (full) cs[h] = cs[h] * vs(base + 8 + h) + vm(base + 8 + h);
                 if (MODE == 4) {
                     sn[h] = pexp(sn[h]);
                     if (full) cs[h] = pexp(cs[h]);

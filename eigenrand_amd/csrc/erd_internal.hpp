@@ -79,6 +79,8 @@ struct HostTable {
     uint32_t bitmask = 0;
     // DiscreteGen<int32_t, float> (Discrete.h:492-533): the same tables in float
     std::vector<float> fcdf, fprob;
+    // DiscreteGen<int32_t, double> (Discrete.h:628-649): in double
+    std::vector<double> dcdf, dprob;
 };
 
 // Segment programs (capi.hip).

@@ -10,6 +10,7 @@ struct EngineState;
 struct Segment;
 struct GenState;
 struct Guard;
+template <class T> struct Param;
 
 // PoissonGen constructor constants, computed on the host in double with libm
 // (Discrete.h:807-813).
@@ -62,8 +63,10 @@ hipError_t launch_fill_exponential_f64(EngineState* st, int64_t n, double* out, 
                                        double lambda, hipStream_t s);
 hipError_t launch_fill_cauchy_f64(EngineState* st, int64_t n, double* out, int64_t per, double a,
                                   double b, hipStream_t s);
+// NormalVGen; a NULL mean or sd array stands for mean0 or sd0 at every element.
 hipError_t launch_fill_normal_vop_f32(EngineState* st, int64_t n, float* out, int64_t per,
-                                      const float* mean, const float* sd, hipStream_t s);
+                                      const float* mean, const float* sd, hipStream_t s, float mean0 = 0.f,
+                                      float sd0 = 1.f);
 hipError_t launch_fill_exponential_f32(EngineState* st, int64_t n, float* out, int64_t per,
                                        float lambda, hipStream_t s);
 hipError_t launch_fill_cauchy_f32(EngineState* st, int64_t n, float* out, int64_t per, float a,
@@ -102,6 +105,10 @@ hipError_t launch_fill_bernoulli_i32(EngineState* st, int64_t n, int32_t* out, i
 hipError_t launch_fill_geometric_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, float rlog_q,
                                      hipStream_t s);
 hipError_t launch_fill_randbits_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, hipStream_t s);
+// DiscreteGen<int32_t, double>: a double cdf (ncdf = K < 16) or the double alias table.
+hipError_t launch_fill_discrete_d_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, const double* cdf,
+                                      int32_t ncdf, const double* prob, const int32_t* alias, uint32_t bitmask,
+                                      hipStream_t s);
 // DiscreteGen<int32_t, float>: cdf (k > 0, alias NULL) or float prob + alias tables.
 hipError_t launch_fill_discrete_f_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, const float* tab,
                                       const int32_t* alias, int32_t k, uint32_t bitmask, hipStream_t s);
@@ -134,7 +141,7 @@ hipError_t launch_fill_poisson_knuth_i32(EngineState* st, int64_t n, int32_t* ou
 hipError_t launch_fill_binomial_small_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
                                           int32_t trials, double p, hipStream_t s);
 hipError_t launch_fill_binomial_vop_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
-                                        const int32_t* trials, const float* p, Guard gd,
+                                        Param<int32_t> trials, Param<float> p, Guard gd,
                                         hipStream_t s);
 
 hipError_t launch_fill_gamma_vop_f32(EngineState* st, int64_t n, float* out, int64_t per,

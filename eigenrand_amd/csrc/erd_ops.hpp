@@ -1126,7 +1126,7 @@ __device__ void op_binomial_small(Eng& e, uint64_t* carry, int32_t* o, int64_t p
 // open lane takes its first accepted candidate and the packet consumes the
 // iterations up to the slowest lane, as the sequential loop does.
 __device__ void op_binomial_vop(Eng& e, uint64_t* carry, int32_t* o, int64_t per,
-                                const int32_t* trials, const float* prob, Guard gd,
+                                Param<int32_t> trials, Param<float> prob, Guard gd,
                                 const float* lg, int lane) {
     const int64_t T = per / 4;
     if (T == 0) return;

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(64) void k_fill_normal_gen(EngineState* __restrict_
             float x = z;
             if (MODE == 1) x = z * d.sd + d.mean;
             if (MODE == 3) x = pexp(z * d.sd + d.mean);
-            if (MODE == 2) x = z * d.vsd[lane] + d.vmean[lane];
+            if (MODE == 2) x = z * d.vs(lane) + d.vm(lane);
             d.out[lane] = x;
         }
         d.rebase(8);
@@ -114,6 +114,128 @@ struct DiscreteI32 {
         }
     }
 };
+
+// DiscreteGen<int32_t, double>::packetOp at Packet4i under AVX with Eigen 3.x
+// (Discrete.h:699-779, the EIGEN_VECTORIZE_AVX branches).
+// K < 16: one full draw per packet gives 4 doubles (bit_to_ur_double of the
+// u64 lanes) and lane l = K - #{c in cdf : rx_l < c}.
+struct DiscreteDCdfI32 {
+    static constexpr int CALLS = 1;
+    int32_t* out;
+    const double* cdf;
+    int32_t k;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 4; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        const double rx = ur_f64(w[0]);
+        int32_t r = k;
+        for (int c = 0; c < k; ++c) r -= rx < cdf[c] ? 1 : 0;
+        out[4 * u + l] = r;
+    }
+};
+
+// K >= 16: a packet takes idx = the 4 int32 words of half() & bitmask, then a
+// full draw's 4 doubles u, and lane l is idx_l if u_l < prob[idx_l] (the
+// double alias table) else alias[idx_l]. half() serves the low half of a new
+// draw and caches its high half for the next packet, so with an empty cache
+// two packets take three draws: the split draw, then one full draw each.
+struct DiscreteDAlias {
+    const double* prob;
+    const int32_t* alias;
+    uint32_t bitmask;
+    __device__ int32_t pick(uint32_t word, uint64_t full) const {
+        const uint32_t idx = word & bitmask;
+        return ur_f64(full) < prob[idx] ? (int32_t)idx : alias[idx];
+    }
+    // int32 word l of the half made of u64 lanes (h0, h1) of the split draw,
+    // whose lane l sits in thread (quad base + l) of a unit's four threads
+    __device__ static uint32_t half_word(uint64_t mine, int l, int first_lane) {
+        const int src = (int)(threadIdx.x & ~3u) + first_lane + (l >> 1);
+        const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)mine, src);
+        const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(mine >> 32), src);
+        return (l & 1) ? hi : lo;
+    }
+};
+
+// The first packet when the engine holds a cached half (cache words c0, c1).
+struct DiscreteDFirst {
+    int32_t* out;
+    DiscreteDAlias t;
+    uint64_t c0, c1;
+    __device__ static int64_t units(int64_t n) { return n / 4; }
+    template <class F>
+    __device__ void emit(int64_t per, int64_t u, int l, F fetch) const {
+        const uint64_t h = (l >> 1) ? c1 : c0;
+        out[l] = t.pick((l & 1) ? (uint32_t)(h >> 32) : (uint32_t)h, fetch(0));
+    }
+};
+
+// Packet pairs: draws (split, full A, full B) per unit. The last unit's split
+// draw leaves its high half in hc (the reference keeps it in the engine's
+// cache member even once served).
+struct DiscreteDPair {
+    int32_t* out;
+    DiscreteDAlias t;
+    uint64_t* hc;  // LDS, 2 words
+    int64_t n_units;
+    __device__ static int64_t units(int64_t n) { return n / 8; }
+    template <class F>
+    __device__ void emit(int64_t per, int64_t u, int l, F fetch) const {
+        const uint64_t s = fetch(0);
+        out[8 * u + l] = t.pick(DiscreteDAlias::half_word(s, l, 0), fetch(1));
+        out[8 * u + 4 + l] = t.pick(DiscreteDAlias::half_word(s, l, 2), fetch(2));
+        if (u == n_units - 1 && l >= 2) hc[l - 2] = s;
+    }
+};
+
+// A trailing single packet with an empty cache: (split, full); the split
+// draw's high half becomes the engine's valid cache (hc).
+struct DiscreteDLast {
+    int32_t* out;
+    DiscreteDAlias t;
+    uint64_t* hc;
+    __device__ static int64_t units(int64_t n) { return n / 4; }
+    template <class F>
+    __device__ void emit(int64_t per, int64_t u, int l, F fetch) const {
+        const uint64_t s = fetch(0);
+        out[l] = t.pick(DiscreteDAlias::half_word(s, l, 0), fetch(1));
+        if (l >= 2) hc[l - 2] = s;
+    }
+};
+
+__global__ __launch_bounds__(64) void k_fill_discrete_d_alias(EngineState* __restrict__ states,
+                                                              int32_t* __restrict__ out, int64_t per,
+                                                              DiscreteDAlias t) {
+    __shared__ uint64_t mt[kWords];
+    __shared__ uint64_t carry[2 * kLanes];
+    __shared__ uint64_t hc[2];
+    const int lane = threadIdx.x;
+    const int64_t g = blockIdx.x;
+    const int64_t T = per / 4;
+    if (T == 0) return;
+    Eng e;
+    eng_load(e, mt, states + g, lane);
+    int32_t* o = out + g * per;
+    int64_t k0 = 0;
+    if (e.half_valid) {
+        op_fixed_rt<2>(e, carry, 4, 1, DiscreteDFirst{o, t, e.half_cache[0], e.half_cache[1]}, lane);
+        e.half_valid = 0;
+        k0 = 1;
+    }
+    const int64_t pairs = (T - k0) / 2;
+    if (pairs > 0) {
+        op_fixed_rt<2>(e, carry, 8 * pairs, 3, DiscreteDPair{o + 4 * k0, t, hc, pairs}, lane);
+        e.half_cache[0] = hc[0];  // op_fixed_rt ends with a barrier
+        e.half_cache[1] = hc[1];
+    }
+    if ((T - k0) & 1) {
+        op_fixed_rt<2>(e, carry, 4, 2, DiscreteDLast{o + 4 * (k0 + 2 * pairs), t, hc}, lane);
+        e.half_cache[0] = hc[0];
+        e.half_cache[1] = hc[1];
+        e.half_valid = 1;
+    }
+    eng_store(e, states + g, lane);
+}
 
 // StdNormalGen fills of a list of engines, one workgroup of kJobThreads per
 // engine (job), so that one long fill on one engine (a multivariate
@@ -201,6 +323,16 @@ hipError_t launch_fill_discrete_i32(EngineState* st, int64_t n, int32_t* out, in
     return launch_fixed(st, n, per, DiscreteI32{out, DiscreteTable{cdf, prob, alias, ncdf, bitmask}}, s);
 }
 
+hipError_t launch_fill_discrete_d_i32(EngineState* st, int64_t n, int32_t* out, int64_t per, const double* cdf,
+                                      int32_t ncdf, const double* prob, const int32_t* alias, uint32_t bitmask,
+                                      hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (cdf) return launch_fixed(st, n, per, DiscreteDCdfI32{out, cdf, ncdf}, s);
+    hipLaunchKernelGGL(k_fill_discrete_d_alias, dim3((unsigned)n), dim3(64), 0, s, st, out, per,
+                       DiscreteDAlias{prob, alias, bitmask});
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_normal_f64(EngineState* st, int64_t n, double* out, int64_t per, double mean,
                                   double sd, int affine, hipStream_t s) {
     if (affine) return launch_fixed(st, n, per, NormalF64<1>{out, mean, sd}, s);
@@ -228,8 +360,9 @@ hipError_t launch_fill_normal_f32(EngineState* st, int64_t n, float* out, int64_
 }
 
 hipError_t launch_fill_normal_vop_f32(EngineState* st, int64_t n, float* out, int64_t per,
-                                      const float* mean, const float* sd, hipStream_t s) {
-    NormalF32<2> d{out, 0.f, 1.f, mean, sd};
+                                      const float* mean, const float* sd, hipStream_t s, float mean0,
+                                      float sd0) {
+    NormalF32<2> d{out, mean0, sd0, mean, sd};
     return launch_fixed(st, n, per, d, s);
 }
 

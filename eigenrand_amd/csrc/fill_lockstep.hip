@@ -75,8 +75,7 @@ __global__ __launch_bounds__(64) void k_binomial_small(EngineState* __restrict__
 // BinomialVGen<int32_t>, per-lane float parameters.
 __global__ __launch_bounds__(64) void k_binomial_vop(EngineState* __restrict__ states,
                                                      int32_t* __restrict__ out, int64_t per,
-                                                     const int32_t* __restrict__ trials,
-                                                     const float* __restrict__ prob,
+                                                     Param<int32_t> trials, Param<float> prob,
                                                      Guard gd,
                                                      const float* __restrict__ lg) {
     __shared__ uint64_t mt[kWords];
@@ -161,7 +160,7 @@ hipError_t launch_fill_binomial_small_i32(EngineState* st, int64_t n, int32_t* o
 }
 
 hipError_t launch_fill_binomial_vop_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
-                                        const int32_t* trials, const float* p, Guard gd,
+                                        Param<int32_t> trials, Param<float> p, Guard gd,
                                         hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const float* lg = lgamma_table(s);

@@ -577,6 +577,7 @@ int build_table(const double* w, int64_t K, HostTable& t) {
         for (double p : c) {
             t.cdf.push_back((uint32_t)(p / c.back() * 0x80000000));
             t.fcdf.push_back((float)(p / c.back()));
+            t.dcdf.push_back(p / c.back());
         }
         return ERD_OK;
     }
@@ -585,6 +586,7 @@ int build_table(const double* w, int64_t K, HostTable& t) {
     const size_t psize = (size_t)1 << nb;
     t.prob.assign(psize, 0);
     t.fprob.assign(psize, 0.0f);
+    t.dprob.assign(psize, 0.0);
     t.alias.assign(psize, 0);
     t.bitmask = (uint32_t)(psize - 1);
     const double mean = sum / psize;
@@ -598,6 +600,7 @@ int build_table(const double* w, int64_t K, HostTable& t) {
     while (over < psize && under < psize) {
         t.prob[under] = (int32_t)(f[under] * (imax + 1.0));
         t.fprob[under] = (float)f[under];
+        t.dprob[under] = f[under];
         t.alias[under] = (int32_t)over;
         f[over] += f[under] - 1;
         if (f[over] >= 1 || mm <= over) {
@@ -613,16 +616,19 @@ int build_table(const double* w, int64_t K, HostTable& t) {
         if (f[over] >= 1) {
             t.prob[over] = imax;
             t.fprob[over] = 1.0f;
+            t.dprob[over] = 1.0;
             t.alias[over] = (int32_t)over;
         }
     if (under < psize) {
         t.prob[under] = imax;
         t.fprob[under] = 1.0f;
+        t.dprob[under] = 1.0;
         t.alias[under] = (int32_t)under;
         for (under = mm; under < msize; ++under)
             if (f[under] < 1) {
                 t.prob[under] = imax;
                 t.fprob[under] = 1.0f;
+                t.dprob[under] = 1.0;
                 t.alias[under] = (int32_t)under;
             }
     }
@@ -719,6 +725,27 @@ int erd_fill_discrete_f_i32(erd_engines* e, int32_t* out, int64_t per, const dou
     }
     return launch_fill_discrete_f_i32(e->states, e->count, out, per, tab, alias, (int32_t)t.fcdf.size(), t.bitmask,
                                       s) == hipSuccess
+               ? ERD_OK
+               : ERD_ERR_HIP;
+}
+
+int erd_fill_discrete_d_i32(erd_engines* e, int32_t* out, int64_t per, const double* weights, int64_t K,
+                            void* stream) {
+    if (!e || !e->states || per < 0 || per % 4 != 0 || (per > 0 && !out) || !weights) return ERD_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    HostTable t;
+    int rc = build_table(weights, K, t);
+    if (rc) return rc;
+    Scratch sc(s);
+    double *cdf = nullptr, *prob = nullptr;
+    int32_t* alias = nullptr;
+    if (!t.dcdf.empty()) {
+        if (sc.upload(&cdf, t.dcdf) != hipSuccess) return ERD_ERR_HIP;
+    } else if (sc.upload(&prob, t.dprob) != hipSuccess || sc.upload(&alias, t.alias) != hipSuccess) {
+        return ERD_ERR_HIP;
+    }
+    return launch_fill_discrete_d_i32(e->states, e->count, out, per, cdf, (int32_t)t.dcdf.size(), prob, alias,
+                                      t.bitmask, s) == hipSuccess
                ? ERD_OK
                : ERD_ERR_HIP;
 }

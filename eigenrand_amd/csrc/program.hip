@@ -313,8 +313,8 @@ __device__ void run_segment(Eng& e, ProgramLds& s, BetaLds& bl, const Segment& s
             break;
         }
         case SEG_BINOMIAL_VOP_I32:
-            op_binomial_vop(e, s.vop.carry, (int32_t*)sg.out, n, (const int32_t*)sg.in0,
-                            (const float*)sg.in1, gd, lg, lane);
+            op_binomial_vop(e, s.vop.carry, (int32_t*)sg.out, n, Param<int32_t>{(const int32_t*)sg.in0, 0},
+                            Param<float>{(const float*)sg.in1, 0.f}, gd, lg, lane);
             break;
         case SEG_GAMMA_VOP_F32:
             op_gamma_vop(e, s.vop.carry, (float*)sg.out, n, (const float*)sg.in0,

@@ -162,6 +162,14 @@ int erd_fill_normal_vop_f32(erd_engines* e, float* out, int64_t per_engine, cons
 int erd_fill_std_normal_f32_fast(erd_engines* e, float* out, int64_t per_engine, void* stream);
 int erd_fill_normal_f32_fast(erd_engines* e, float* out, int64_t per_engine, float mean, float stdev,
                              void* stream);
+/* normal(urng, mean_array, stdev) and normal(urng, mean, stdev_array): the
+ * reference's mixed overloads (NormalExp.h:1304-1335), NormalVGen with the
+ * scalar as a constant array of the same shape, so each equals
+ * erd_fill_normal_vop_f32 with that array. */
+int erd_fill_normal_vop_vs_f32(erd_engines* e, float* out, int64_t per_engine, const float* mean, float stdev,
+                               void* stream);
+int erd_fill_normal_vop_sv_f32(erd_engines* e, float* out, int64_t per_engine, float mean, const float* stdev,
+                               void* stream);
 /* lognormal<float>(…, mean, stdev): LognormalGen (NormalExp.h:171-210), the
  * exp of NormalGen's packets (same cos-packet caching). */
 int erd_fill_lognormal_f32(erd_engines* e, float* out, int64_t per_engine, float mean, float stdev,
@@ -266,6 +274,14 @@ int erd_fill_binomial_i32(erd_engines* e, int32_t* out, int64_t per_engine, int3
  * arrays of n*per_engine elements. */
 int erd_fill_binomial_vop_i32(erd_engines* e, int32_t* out, int64_t per_engine, const int32_t* trials,
                               const float* p, void* stream);
+/* binomial(urng, trials_array, p) and binomial(urng, trials, p_array): the
+ * mixed overloads (Discrete.h:1852-1897), BinomialVGen<int32_t> with the
+ * scalar as a constant array; each equals erd_fill_binomial_vop_i32 with that
+ * array. */
+int erd_fill_binomial_vop_vs_i32(erd_engines* e, int32_t* out, int64_t per_engine, const int32_t* trials, float p,
+                                 void* stream);
+int erd_fill_binomial_vop_sv_i32(erd_engines* e, int32_t* out, int64_t per_engine, int32_t trials, const float* p,
+                                 void* stream);
 
 /* Vectorize-over-params gamma and Poisson. The reference has no GammaVGen or
  * PoissonVGen (its VoP generators stop at BinomialVGen); these follow the
@@ -439,6 +455,14 @@ int erd_fill_discrete_i32(erd_engines* e, int32_t* out, int64_t per_engine, cons
  * CDF; otherwise the alias table in float, index bits from one half draw and
  * the uniform from the next. weights: HOST doubles (K). */
 int erd_fill_discrete_f_i32(erd_engines* e, int32_t* out, int64_t per_engine, const double* weights, int64_t K,
+                            void* stream);
+/* discreteD<int32>(…, weights): DiscreteGen<int32_t, double> (Discrete.h:
+ * 606-779) at Packet4i under AVX (Eigen 3.x). K < 16: one full draw per
+ * packet gives four doubles counted against the double CDF; otherwise the
+ * alias table in double, index bits from one half() draw and the uniform
+ * doubles from the next full draw (so two packets take three draws when the
+ * engine's half cache starts empty). weights: HOST doubles (K). */
+int erd_fill_discrete_d_i32(erd_engines* e, int32_t* out, int64_t per_engine, const double* weights, int64_t K,
                             void* stream);
 
 /* MultinomialGen<int32_t>(trials_i, weights_i).generate(engine_i, samples_i)

@@ -1364,6 +1364,7 @@ namespace {
 struct AliasTable {
     std::vector<int32_t> prob, alias;
     std::vector<float> fprob;  // AliasMethod<float, int32_t>: (float)f, or 1
+    std::vector<double> dprob; // AliasMethod<double, int32_t>: f, or 1
     size_t bitsize = 0, bitmask = 0;
 };
 
@@ -1375,6 +1376,7 @@ bool build_alias(const double* w, size_t msize, AliasTable& t) {
     const size_t psize = (size_t)1 << nbsize;
     t.prob.assign(psize, 0);
     t.fprob.assign(psize, 0.0f);
+    t.dprob.assign(psize, 0.0);
     t.alias.assign(psize, 0);
     t.bitsize = nbsize;
     t.bitmask = ((size_t)(-1)) >> (sizeof(size_t) * 8 - nbsize);
@@ -1389,6 +1391,7 @@ bool build_alias(const double* w, size_t msize, AliasTable& t) {
     while (over < psize && under < psize) {
         t.prob[under] = (int32_t)(f[under] * (imax + 1.0));
         t.fprob[under] = (float)f[under];
+        t.dprob[under] = f[under];
         t.alias[under] = (int32_t)over;
         f[over] += f[under] - 1;
         if (f[over] >= 1 || mm <= over) {
@@ -1403,17 +1406,20 @@ bool build_alias(const double* w, size_t msize, AliasTable& t) {
         if (f[over] >= 1) {
             t.prob[over] = imax;
             t.fprob[over] = 1.0f;
+            t.dprob[over] = 1.0;
             t.alias[over] = (int32_t)over;
         }
     }
     if (under < psize) {
         t.prob[under] = imax;
         t.fprob[under] = 1.0f;
+        t.dprob[under] = 1.0;
         t.alias[under] = (int32_t)under;
         for (under = mm; under < msize; ++under) {
             if (f[under] < 1) {
                 t.prob[under] = imax;
                 t.fprob[under] = 1.0f;
+                t.dprob[under] = 1.0;
                 t.alias[under] = (int32_t)under;
             }
         }
@@ -1540,6 +1546,49 @@ int orc_alias_table(const double* w, int64_t K, int32_t* prob, int32_t* alias) {
 }
 
 // AliasMethod<float, int32_t> (float branch of buildTable, Discrete.h:119-183).
+int orc_alias_table_f64(const double* w, int64_t K, double* prob, int32_t* alias) {
+    AliasTable t;
+    if (!build_alias(w, (size_t)K, t)) return -1;
+    std::memcpy(prob, t.dprob.data(), 8 * t.dprob.size());
+    std::memcpy(alias, t.alias.data(), 4 * t.alias.size());
+    return (int)t.bitsize;
+}
+
+// DiscreteGen<int32_t, double>::packetOp at Packet4i under AVX with Eigen 3.x
+// (Discrete.h:699-779, the EIGEN_VECTORIZE_AVX branches). K < 16: one full
+// draw gives 4 doubles rx (StdUniformRealGen<double>, bit_to_ur_double of the
+// u64 lanes) and out = K - #{c in cdf : rx < c}, cdf_i = c_i / c_K in double.
+// Otherwise idx = the 4 int32 words of a half() draw & bitmask, then one full
+// draw's 4 doubles u: idx if u < prob[idx] (AliasMethod<double, int32_t>)
+// else alias[idx].
+void orc_fill_discrete_d_i32(void* states, int64_t n_engines, int32_t* out, int64_t per_engine, const double* cdf,
+                             int32_t k, const double* prob, const int32_t* alias, uint32_t bitmask) {
+    Vmt* e = (Vmt*)states;
+    for (int64_t g = 0; g < n_engines; ++g) {
+        int32_t* o = out + g * per_engine;
+        for (int64_t i = 0; i < per_engine; i += 4) {
+            uint64_t p[4];
+            if (cdf) {
+                vmt_draw(e[g], p);
+                for (int j = 0; j < 4; ++j) {
+                    const double rx = bit_to_ur_double(p[j]);
+                    int32_t r = k;
+                    for (int c = 0; c < k; ++c) r -= rx < cdf[c] ? 1 : 0;
+                    o[i + j] = r;
+                }
+            } else {
+                uint64_t h[2];
+                vmt_half(e[g], h);
+                uint32_t idx[4];
+                for (int j = 0; j < 4; ++j) idx[j] = (uint32_t)(h[j >> 1] >> (32 * (j & 1))) & bitmask;
+                vmt_draw(e[g], p);
+                for (int j = 0; j < 4; ++j)
+                    o[i + j] = bit_to_ur_double(p[j]) < prob[idx[j]] ? (int32_t)idx[j] : alias[idx[j]];
+            }
+        }
+    }
+}
+
 int orc_alias_table_f32(const double* w, int64_t K, float* prob, int32_t* alias) {
     AliasTable t;
     if (!build_alias(w, (size_t)K, t)) return -1;

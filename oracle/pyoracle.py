@@ -425,3 +425,24 @@ def std_normal_f32(n, seed=0, threads=1):
     f.restype = None
     f(_p(out), I64(int(n)), C.c_uint64(seed), C.c_int(threads))
     return out
+
+
+def fill_discrete_d_i32(e, per, w):
+    """DiscreteGen<int32_t, double>: double CDF for K < 16 (cdf_i = c_i / c_K,
+    Discrete.h:631-645), else AliasMethod<double, int32_t>."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    out = np.empty(len(e) * per, dtype=np.int32)
+    f = lib().orc_fill_discrete_d_i32
+    f.restype = None
+    if w.size < 16:
+        c = np.cumsum(w)
+        cdf = np.ascontiguousarray(c / c[-1])
+        f(_p(e.raw), I64(len(e)), _p(out), I64(per), _p(cdf), C.c_int32(w.size), None, None, C.c_uint32(0))
+        return out
+    size = 1 << int(np.ceil(np.log2(w.size)))
+    prob = np.zeros(size, dtype=np.float64)
+    alias = np.zeros(size, dtype=np.int32)
+    bits = lib().orc_alias_table_f64(_p(w), I64(w.size), _p(prob), _p(alias))
+    assert bits >= 0
+    f(_p(e.raw), I64(len(e)), _p(out), I64(per), None, C.c_int32(0), _p(prob), _p(alias), C.c_uint32(size - 1))
+    return out

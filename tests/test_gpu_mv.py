@@ -350,3 +350,25 @@ def test_factors_report_not_psd(er):
     bad = -np.eye(6)
     with pytest.raises(er.ErdError):
         er.MvFactors([np.eye(3), bad])
+
+
+@pytest.mark.parametrize("K", [3, 15, 16, 37, 1000, 10000])
+def test_discrete_d_fill_matches_oracle(er, orc, K):
+    """discreteD (double tables): K < 16 one full draw per packet; otherwise
+    half() index bits and one full draw of doubles per packet. Fills of odd
+    packet counts leave the half cache set, also for the next fill; one fill
+    starts on a cache left by a discreteF fill."""
+    rng = np.random.default_rng(K + 2)
+    w = rng.uniform(0, 1, K)
+    seeds = [1, 2, 3]
+    urng = er.Vmt19937_64(seeds=seeds)
+    ref = orc.Engines(seeds=seeds)
+    for n in (4, 12, 1004, 8, 4, 4, 2500):
+        out = er.discrete(n, 1, urng, w, precision="double")
+        assert_same(out.reshape(len(seeds), n).cpu().numpy(), orc.fill_discrete_d_i32(ref, n, w))
+        np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+    er.discrete(12, 1, urng, w, precision="float")  # three packets of half draws
+    orc.fill_discrete_f_i32(ref, 12, w)
+    out = er.discrete(1000, 1, urng, w, precision="double")
+    assert_same(out.reshape(len(seeds), 1000).cpu().numpy(), orc.fill_discrete_d_i32(ref, 1000, w))
+    np.testing.assert_array_equal(urng.snapshot(), ref.raw)
