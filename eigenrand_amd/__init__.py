@@ -199,6 +199,31 @@ def p8_uniform_real(count, urng, out=None, stream=None):
     return _p8_fill("erd_p8_fill_uniform_f32", count, urng, _torch().float32, out, stream)
 
 
+P8_DISTS = ("uniform", "std_normal", "normal", "exponential", "cauchy", "student_t", "gamma",
+            "lognormal", "weibull", "extreme_value", "poisson", "binomial")
+
+
+def p8_fill(dist, count, urng, a=0.0, b=0.0, out=None, stream=None):
+    """`Rand::dist<ArrayXf or ArrayXi>(count, 1, p8_k, a, b)` with each
+    P8_mt19937_64 adaptor k of `urng` as the engine (test/test.cpp:117 and
+    on): the packet code packs the adaptor's scalar draws into 256-bit words
+    (RawbitsMaker, arch/AVX/RandUtils.h:59-144) in Clang's argument order.
+    (n, count) float32, or int32 for poisson/binomial. a, b as in
+    `Request` (binomial: a = trials, b = p)."""
+    torch = _torch()
+    if dist not in P8_DISTS:
+        raise ValueError(f"p8_fill: {dist!r} is not drawn from a P8 adaptor; one of {P8_DISTS}")
+    dt = torch.int32 if dist in ("poisson", "binomial") else torch.float32
+    n = len(urng) // 2
+    if out is None:
+        out = torch.empty(n * int(count), dtype=dt, device="cuda")
+    elif out.dtype != dt or not out.is_cuda or not out.is_contiguous() or out.numel() != n * count:
+        raise InvalidArgument("output tensor", 1)
+    _lib.call("erd_p8_fill_dist", urng.handle, _lib.DIST[dist], _ptr(out), int(count), float(a), float(b),
+              _stream_ptr(stream))
+    return out.view(n, int(count))
+
+
 class Generator:
     """A reused generator object on `n_engines` engines: the GPU form of
     `auto gen = Rand::makeXxxGen(a, b)` followed by several

@@ -57,6 +57,7 @@ _SIGS = {
     "erd_p8_fill_u64": [P, P, I64, P],
     "erd_p8_fill_u32": [P, P, I64, P],
     "erd_p8_fill_uniform_f32": [P, P, I64, P],
+    "erd_p8_fill_dist": [P, C.c_int32, P, I64, F64, F64, P],
     "erd_fill_uniform_f32": [P, P, I64, F32, F32, P],
     "erd_fill_uniform_f64": [P, P, I64, F64, F64, P],
     "erd_fill_std_normal_f32": [P, P, I64, P],

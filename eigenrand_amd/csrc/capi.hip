@@ -199,8 +199,50 @@ static int p8_fill(erd_engines* e, void* out, int64_t per, int64_t width, int mo
     return hip_status(erd::launch_fill_p8(e->states, e->count, out, per / 2, mode, S(stream)));
 }
 
+// One segment per adaptor on the P8 program kernel: the adaptor's scalar
+// stream packed as the distribution's packet code takes it (mode -1), or raw
+// draws (mode 0).
+static int p8_dist(erd_engines* e, erd_request r, int64_t per, int mode, void* stream) {
+    if (!e || !e->states || e->count < 2 || e->count % 2 != 0 || per < 0 || (per > 0 && !r.out))
+        return ERD_ERR_INVALID;
+    if (per == 0) return ERD_OK;
+    const int64_t n_p8 = e->count / 2;
+    int elem = 4;
+    switch (r.dist) {
+        case ERD_DIST_U64: elem = 8; break;
+        default: break;
+    }
+    std::vector<erd::Segment> segs((size_t)n_p8);
+    std::vector<int32_t> eng((size_t)n_p8);
+    for (int64_t k = 0; k < n_p8; ++k) {
+        erd_request q = r;
+        q.engine = (int32_t)k;
+        q.count = per;
+        q.out = (char*)r.out + (size_t)k * (size_t)per * (size_t)elem;
+        int rc = erd::make_segment(q, segs[(size_t)k]);
+        if (rc) return rc;
+        if (mode < 0 && erd::p8_feed_mode(segs[(size_t)k].kind) < 0) return ERD_ERR_UNSUPPORTED;
+        eng[(size_t)k] = (int32_t)k;
+    }
+    return erd::run_segments(e, segs, eng, S(stream), true, mode);
+}
+
+int erd_p8_fill_dist(erd_engines* e, int32_t dist, void* out, int64_t per_p8, double a, double b, void* stream) {
+    erd_request r{};
+    r.dist = dist;
+    r.out = out;
+    r.a = a;
+    r.b = b;
+    if (dist == ERD_DIST_U64) return ERD_ERR_UNSUPPORTED;  // erd_p8_fill_u64
+    return p8_dist(e, r, per_p8, -1, stream);
+}
+
 int erd_p8_fill_u64(erd_engines* e, uint64_t* out, int64_t per, void* stream) {
-    return p8_fill(e, out, per, 8, 0, stream);
+    // the scalar stream itself, from the operator() buffer's position
+    erd_request r{};
+    r.dist = ERD_DIST_U64;
+    r.out = out;
+    return p8_dist(e, r, per, 0, stream);
 }
 
 int erd_p8_fill_u32(erd_engines* e, uint32_t* out, int64_t per, void* stream) {
@@ -821,7 +863,7 @@ int make_segment(const erd_request& r, Segment& s) {
 // it. Device buffers come from the stream-ordered allocator and are released
 // on the same stream.
 int run_segments(erd_engines* e, const std::vector<Segment>& segs, const std::vector<int32_t>& seg_engine,
-                 hipStream_t stream) {
+                 hipStream_t stream, bool p8, int p8_mode) {
     const int64_t n = (int64_t)segs.size();
     if (n == 0) return ERD_OK;
     std::vector<int32_t> engine_of, begin;
@@ -844,7 +886,11 @@ int run_segments(erd_engines* e, const std::vector<Segment>& segs, const std::ve
     std::memcpy(host.data() + seg_bytes + sizeof(int32_t) * (size_t)groups, begin.data(),
                 sizeof(int32_t) * (size_t)(groups + 1));
     err = upload_async(dbuf, host.data(), host.size(), stream);
-    if (err == hipSuccess)
+    if (err == hipSuccess && p8)
+        err = launch_program_p8(e->states, groups, (const int32_t*)(dbuf + seg_bytes),
+                                (const int32_t*)(dbuf + seg_bytes) + groups, (const Segment*)dbuf, p8_mode,
+                                guard_of(e), stream);
+    else if (err == hipSuccess)
         err = launch_program(e->states, groups, (const int32_t*)(dbuf + seg_bytes),
                              (const int32_t*)(dbuf + seg_bytes) + groups, (const Segment*)dbuf,
                              guard_of(e), stream);

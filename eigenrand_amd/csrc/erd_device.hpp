@@ -107,6 +107,20 @@ __device__ __forceinline__ uint64_t temper(uint64_t y) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Inverse of temper: a word w placed in the state block as untemper(w) is
+// read back as w by every consumer (they temper what they read).
+__device__ __forceinline__ uint64_t untemper(uint64_t y) {
+    y ^= y >> 43;                                  // its own inverse (43 >= 32)
+    y ^= (y << 37) & 0xfff7eee000000000ull;        // likewise (37 >= 32)
+    uint64_t x = y;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) x = y ^ ((x << 17) & 0x71d67fffeda60000ull);
+    y = x;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) x = y ^ ((x >> 29) & 0x5555555555555555ull);
+    return x;
+}
+
 __device__ __forceinline__ uint64_t twist_word(uint64_t cur, uint64_t next, uint64_t far) {
     // y = (cur & UM) | (next & LM); far ^ (y >> 1) ^ (y odd ? A : 0), with
     // UM = upper 33 bits, on 32-bit halves. y's parity is next's (bit 0 is in

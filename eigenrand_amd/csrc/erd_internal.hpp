@@ -84,8 +84,10 @@ struct HostTable {
 };
 
 // Segment programs (capi.hip).
+// p8: engines are P8_mt19937_64 adaptors (erd_engines_create_p8); p8_mode >= 0
+// forces the scalar packing (P8Feed), else each segment kind's own.
 int run_segments(erd_engines* e, const std::vector<Segment>& segs, const std::vector<int32_t>& engine_of,
-                 hipStream_t stream);
+                 hipStream_t stream, bool p8 = false, int p8_mode = -1);
 int make_segment(const erd_request& r, Segment& s);
 
 // Multivariate helpers (mv_host.hip).

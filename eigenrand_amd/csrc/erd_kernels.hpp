@@ -160,6 +160,10 @@ const float* lgamma_table_built();
 hipError_t launch_program(EngineState* st, int32_t n_groups, const int32_t* d_engine_of,
                           const int32_t* d_begin, const Segment* d_segs, Guard gd,
                           hipStream_t s);
+// The same on P8 adaptors (group b = adaptor engine_of[b], records 2a, 2a+1);
+// mode_override >= 0 forces the scalar packing, else each kind's own.
+hipError_t launch_program_p8(EngineState* st, int32_t n_groups, const int32_t* d_engine_of, const int32_t* d_begin,
+                             const Segment* d_segs, int mode_override, Guard gd, hipStream_t s);
 Segment seg_poisson(int32_t* out, int64_t count, double mean);
 Segment seg_binomial(int32_t* out, int64_t count, int32_t trials, double p);
 

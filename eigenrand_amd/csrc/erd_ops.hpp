@@ -15,13 +15,139 @@
 
 namespace erd {
 
+struct P8Feed;
+
 struct Eng {
     uint64_t* mt;            // LDS, kWords
     int idx;                 // next draw in the block (kN: twist first)
     int half_valid;
     uint64_t half_cache[2];  // tempered upper half of the last split draw
     bool twisted;            // LDS block differs from the HBM record
+    P8Feed* feed;            // non-NULL: the block is refilled from a scalar engine
 };
+
+// A P8_mt19937_64 (ParallelRandomEngineAdaptor<uint64_t, Vmt19937_64, 8>,
+// PacketRandomEngine.h:429-517) used as a scalar full-bit engine by packet
+// distributions. Its scalar stream is out[8r + j] = mt19937_64(seed + j)[r]:
+// refill r is one draw of Vmt (seed) then one of Vmt (seed + 4). The packet
+// code packs scalars into packets with RawbitsMaker<..., scalar_fullbit>; the
+// argument evaluation order of _mm*_set*_epi64x(rng(), ...) is unspecified in
+// C++, and this follows Clang's left-to-right order (the reference's default
+// toolchain preset, CMakePresets.json:16-17):
+//  kP8Raw: operator() itself, 4 scalars per draw in order;
+//  kP8F34: Packet8f uniform_real (rawbits_34, arch/AVX/RandUtils.h:75-131):
+//          scalars a, b, c give the 32-bit words [a.lo, a.hi, b.lo, X1,
+//          b.hi, c.lo, c.hi, X2], X1 = the top bytes of a.lo, a.hi, b.lo,
+//          b.lo and X2 those of b.hi, c.lo, c.hi, c.hi;
+//  kP8H2:  Packet4i rawbits (arch/SSE/RandUtils.h:24-41): _mm_set_epi64x of
+//          two scalars, so u64 lane 0 is the second scalar, lane 1 the first.
+// The engine block in LDS is filled with such packed draws (stored
+// untempered), so the ordinary ops consume them; `refills` counts the blocks.
+constexpr int kP8Raw = 0, kP8F34 = 1, kP8H2 = 2;
+struct P8Feed {
+    uint64_t* v0;   // LDS state of Vmt (seed)
+    uint64_t* v1;   // LDS state of Vmt (seed + 4)
+    int idx;        // draws taken from both in their block (kN: twist first)
+    int b;          // scalars of refill idx - 1 already consumed (0: none pending)
+    int mode;
+    int refills;
+};
+
+// Scalars per engine block of packed draws.
+__device__ __forceinline__ int p8_block_scalars(int mode) {
+    return mode == kP8F34 ? 3 * kN : kWords;
+}
+
+// Scalar stream position (scalars into the current block of both Vmts) and
+// back: refill r of the block is scalars [8r, 8r + 8).
+__device__ __forceinline__ int p8_offset(int idx, int b) { return b > 0 ? 8 * (idx - 1) + b : 8 * idx; }
+
+// Refill the packed block `mt` with the next p8_block_scalars(mode) scalars
+// of the feed (twisting both Vmts once if the block runs into their next one).
+__device__ __attribute__((noinline)) void p8_refill(P8Feed& f, uint64_t* mt, int lane) {
+    const int n = p8_block_scalars(f.mode);
+    const int o0 = p8_offset(f.idx, f.b);
+    constexpr int kBlock = 8 * kN;  // scalars per block of both Vmts
+    // scalars still in the current block of the Vmts
+    for (int q = lane; q < n; q += 64) {
+        const int o = o0 + q;
+        if (o < kBlock) {
+            const uint64_t* v = ((o & 7) < 4) ? f.v0 : f.v1;
+            mt[q] = temper(v[(o >> 3) * kLanes + (o & 3)]);
+        }
+    }
+    __syncthreads();
+    int o_end = o0 + n;
+    if (o_end > kBlock) {
+        twist_lds(f.v0, lane);
+        twist_lds(f.v1, lane);
+        for (int q = lane; q < n; q += 64) {
+            const int o = o0 + q - kBlock;
+            if (o >= 0) {
+                const uint64_t* v = ((o & 7) < 4) ? f.v0 : f.v1;
+                mt[q] = temper(v[(o >> 3) * kLanes + (o & 3)]);
+            }
+        }
+        o_end -= kBlock;
+        __syncthreads();
+    }
+    f.idx = (o_end + 7) >> 3;
+    f.b = o_end & 7;
+    ++f.refills;
+    // pack the scalars (now in mt[0, n)) into draws, stored untempered
+    if (f.mode == kP8F34) {
+        uint64_t w[5][4];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const int v = lane + 64 * k;  // draw v takes scalars 3v, 3v+1, 3v+2
+            if (v < kN) {
+                const uint64_t a = mt[3 * v], bb = mt[3 * v + 1], c = mt[3 * v + 2];
+                const uint32_t al = (uint32_t)a, ah = (uint32_t)(a >> 32), bl = (uint32_t)bb,
+                               bh = (uint32_t)(bb >> 32), cl = (uint32_t)c, ch = (uint32_t)(c >> 32);
+                const uint32_t x1 = (al >> 24) | ((ah >> 24) << 8) | ((bl >> 24) << 16) | ((bl >> 24) << 24);
+                const uint32_t x2 = (bh >> 24) | ((cl >> 24) << 8) | ((ch >> 24) << 16) | ((ch >> 24) << 24);
+                w[k][0] = a;
+                w[k][1] = (uint64_t)bl | ((uint64_t)x1 << 32);
+                w[k][2] = (uint64_t)bh | ((uint64_t)cl << 32);
+                w[k][3] = (uint64_t)ch | ((uint64_t)x2 << 32);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const int v = lane + 64 * k;
+            if (v < kN)
+#pragma unroll
+                for (int l = 0; l < 4; ++l) mt[v * kLanes + l] = untemper(w[k][l]);
+        }
+    } else {
+        // raw: draw v = scalars 4v..4v+3 in order; H2: half draw h = scalars
+        // (2h, 2h+1) with lane 0 the second
+        constexpr int K = (kWords + 63) / 64;
+        uint64_t x[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int q = lane + 64 * k;
+            if (q < kWords) x[k] = (f.mode == kP8H2) ? mt[q ^ 1] : mt[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int q = lane + 64 * k;
+            if (q < kWords) mt[q] = untemper(x[k]);
+        }
+    }
+    __syncthreads();
+}
+
+// The next block of draws: the MT twist, or a refill from a scalar engine.
+__device__ __forceinline__ void eng_refill(Eng& e, int lane) {
+    if (e.feed) {
+        p8_refill(*e.feed, e.mt, lane);
+    } else {
+        twist_lds(e.mt, lane);
+    }
+}
 
 __device__ __forceinline__ void eng_load(Eng& e, uint64_t* mt, const EngineState* st, int lane) {
     load_state(mt, st, lane);
@@ -31,6 +157,7 @@ __device__ __forceinline__ void eng_load(Eng& e, uint64_t* mt, const EngineState
     e.half_cache[0] = st->half_cache[0];
     e.half_cache[1] = st->half_cache[1];
     e.twisted = false;
+    e.feed = nullptr;
 }
 
 __device__ __forceinline__ void eng_store(const Eng& e, EngineState* st, int lane) {
@@ -58,7 +185,7 @@ __device__ void op_fixed(Eng& e, uint64_t* carry, int64_t per, const D& d, int l
     int64_t done = 0;
     for (;;) {
         if (pos >= kN) {
-            twist_lds(e.mt, lane);
+            eng_refill(e, lane);
             pos = 0;
             e.twisted = true;
         }
@@ -118,7 +245,7 @@ __device__ void op_fixed2(Eng& e, uint64_t* carry, int64_t per, const D& d, int 
     int64_t done = 0;
     for (;;) {
         if (pos >= kN) {
-            twist_lds(e.mt, lane);
+            eng_refill(e, lane);
             pos = 0;
             e.twisted = true;
         }
@@ -232,7 +359,7 @@ __device__ void op_fixed_rt(Eng& e, uint64_t* carry, int64_t per, int C, const D
     int64_t done = 0;
     for (;;) {
         if (pos >= kN) {
-            twist_lds(e.mt, lane);
+            eng_refill(e, lane);
             pos = 0;
             e.twisted = true;
         }
@@ -349,7 +476,7 @@ __device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t p
     const int cl = lane / IPC, p = lane % IPC;
     for (;;) {
         if (pos >= SLOTS) {
-            twist_lds(e.mt, lane);
+            eng_refill(e, lane);
             pos = 0;
             e.twisted = true;
             used = true;
@@ -504,7 +631,7 @@ struct Cursor {
     int pos;
     int nc;
     bool used;
-    bool* twisted;
+    Eng* eng;  // its refill and twisted flag
 
     __device__ int avail() const { return nc + (kSlots - pos); }
     __device__ uint64_t word(int v, int p) const { return slot_word<SW>(mt, carry, nc, pos, v, p); }
@@ -531,9 +658,9 @@ struct Cursor {
         if (a >= need) return;
         carry_tail<SW, MAXC>(mt, carry, nc, pos, 0, a, lane);
         nc = a;
-        twist_lds(mt, lane);
+        eng_refill(*eng, lane);
         pos = 0;
-        *twisted = true;
+        eng->twisted = true;
     }
 };
 using HalfCursor = Cursor<2, 1>;
@@ -543,7 +670,7 @@ __device__ __forceinline__ HalfCursor cursor_begin(Eng& e, uint64_t* carry, int 
     c.mt = e.mt;
     c.carry = carry;
     c.used = false;
-    c.twisted = &e.twisted;
+    c.eng = &e;
     half_begin(e, carry, c.pos, c.nc, lane);
     return c;
 }
@@ -873,9 +1000,9 @@ __device__ __forceinline__ void lp_advance(HalfCursor& cur, int n, int lane) {
     cur.advance(n);
     if (cur.pos > kHalfSlots) {
         __syncthreads();
-        twist_lds(cur.mt, lane);
+        eng_refill(*cur.eng, lane);
         cur.pos -= kHalfSlots;
-        *cur.twisted = true;
+        cur.eng->twisted = true;
     }
 }
 
@@ -1095,7 +1222,7 @@ __device__ void op_binomial_small(Eng& e, uint64_t* carry, int32_t* o, int64_t p
     cur.mt = e.mt;
     cur.carry = carry;
     cur.used = false;
-    cur.twisted = &e.twisted;
+    cur.eng = &e;
     half_begin(e, carry, cur.pos, cur.nc, lane);
     int64_t done = 0;
     while (done < T) {
@@ -1265,7 +1392,7 @@ __device__ void op_gamma_vop(Eng& e, uint64_t* carry, float* o, int64_t per, con
     cur.pos = e.idx;
     cur.nc = 0;
     cur.used = false;
-    cur.twisted = &e.twisted;
+    cur.eng = &e;
     float g_al = 0.f, g_be = 0.f, g_px = 0.f, g_inv = 0.f, g_sq = 0.f;
     for (int64_t pk = 0; pk < T; ++pk) {
         if ((pk & 7) == 0) {
@@ -1568,7 +1695,7 @@ __device__ void op_multinomial_small(Eng& e, int32_t* out, int64_t n, int32_t T,
         int32_t done = 0;
         while (done < calls) {
             if (pos >= kN) {
-                twist_lds(e.mt, lane);
+                eng_refill(e, lane);
                 pos = 0;
                 e.twisted = true;
             }
@@ -1593,7 +1720,7 @@ __device__ void op_multinomial_small(Eng& e, int32_t* out, int64_t n, int32_t T,
         }
         for (int32_t k = 0; k < S; ++k) {
             if (pos >= kN) {
-                twist_lds(e.mt, lane);
+                eng_refill(e, lane);
                 pos = 0;
                 e.twisted = true;
             }
@@ -1626,7 +1753,7 @@ __device__ void op_binomial_row8(Eng& e, uint64_t* carry, int32_t* out, int64_t 
     cur.pos = e.idx;
     cur.nc = 0;
     cur.used = false;
-    cur.twisted = &e.twisted;
+    cur.eng = &e;
     const uint64_t jmask = 0x0101010101010101ull << j;
     for (int64_t pk = 0; pk < T; ++pk) {
         // t0: first row, every sample has t0 trials

@@ -38,6 +38,20 @@ enum SegKind : int32_t {
     SEG_KIND_COUNT
 };
 
+// How a segment kind takes draws from a P8_mt19937_64 scalar engine (the
+// packings are described at P8Feed in erd_ops.hpp): 0 raw draws, 1 Packet8f
+// uniform_real packing (three scalars per draw), 2 Packet4i packing (two
+// scalars per half draw), -1 not served from a scalar engine.
+constexpr int p8_feed_mode(int32_t kind) {
+    return kind == SEG_U64 ? 0
+         : (kind == SEG_UNIFORM_F32 || kind == SEG_STD_NORMAL_F32 || kind == SEG_NORMAL_F32 ||
+            kind == SEG_EXPONENTIAL_F32 || kind == SEG_CAUCHY_F32 || kind == SEG_STUDENT_T_F32 ||
+            kind == SEG_GAMMA_F32 || kind == SEG_LOGNORMAL_F32 || kind == SEG_WEIBULL_F32 ||
+            kind == SEG_EXTREME_VALUE_F32) ? 1
+         : (kind == SEG_POISSON_I32 || kind == SEG_BINOMIAL_I32) ? 2
+         : -1;
+}
+
 struct Segment {
     int32_t kind;
     int32_t i0;        // integer parameter (trials, affine flag, ...)

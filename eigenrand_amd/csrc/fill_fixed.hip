@@ -276,13 +276,25 @@ hipError_t launch_fill_u64(EngineState* st, int64_t n, uint64_t* out, int64_t pe
     return launch_fixed(st, n, per, d, s);
 }
 
+// Whole-refill fills drop the scalars left in an adaptor's operator() buffer:
+// the buffer position (record 2k's `reserved`) goes back to 0.
+__global__ void k_p8_clear_position(EngineState* __restrict__ states, int64_t n_p8) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_p8) states[2 * k].reserved = 0;
+}
+
 hipError_t launch_fill_p8(EngineState* st, int64_t n, void* out, int64_t per_v, int mode,
                           hipStream_t s) {
+    hipError_t rc;
     switch (mode) {
-        case 0: return launch_fixed(st, n, per_v, P8Fill<0>{out, per_v, 0}, s);
-        case 1: return launch_fixed(st, n, per_v, P8Fill<1>{out, per_v, 0}, s);
-        default: return launch_fixed(st, n, per_v, P8Fill<2>{out, per_v, 0}, s);
+        case 0: rc = launch_fixed(st, n, per_v, P8Fill<0>{out, per_v, 0}, s); break;
+        case 1: rc = launch_fixed(st, n, per_v, P8Fill<1>{out, per_v, 0}, s); break;
+        default: rc = launch_fixed(st, n, per_v, P8Fill<2>{out, per_v, 0}, s); break;
     }
+    if (rc != hipSuccess || n < 2) return rc;
+    const int64_t n_p8 = n / 2;
+    hipLaunchKernelGGL(k_p8_clear_position, dim3((unsigned)((n_p8 + 255) / 256)), dim3(256), 0, s, st, n_p8);
+    return hipGetLastError();
 }
 
 template <class D>

@@ -430,6 +430,93 @@ __global__ __launch_bounds__(64) void k_program(EngineState* __restrict__ states
     eng_store(e, st, lane);
 }
 
+// Reload a P8 adaptor's two Vmt states from their records and advance the
+// scalar stream by `used` scalars from the records' position (the position
+// the feed reached is past that: it fills whole blocks of draws ahead).
+__device__ void p8_seek(P8Feed& f, const EngineState* st0, const EngineState* st1, int64_t used, int lane) {
+    __syncthreads();
+    load_state(f.v0, st0, lane);
+    load_state(f.v1, st1, lane);
+    int64_t o = p8_offset(st0->idx, (int)st0->reserved) + used;
+    while (((o + 7) >> 3) > kN) {
+        twist_lds(f.v0, lane);
+        twist_lds(f.v1, lane);
+        o -= 8 * kN;
+    }
+    f.idx = (int)((o + 7) >> 3);
+    f.b = (int)(o & 7);
+}
+
+// Segment programs on P8_mt19937_64 adaptors: group b runs on adaptor
+// engine_of[b] = records 2a (Vmt seed) and 2a + 1 (Vmt seed + 4). The
+// segment's draws come packed from the adaptor's scalar stream (P8Feed;
+// mode >= 0 forces a packing, else the kind's own); afterwards the adaptor
+// is left exactly as far along its stream as the scalars consumed. Record 2a
+// keeps the operator() buffer position (scalars of the last refill used) in
+// its `reserved` word.
+__global__ __launch_bounds__(64) void k_program_p8(EngineState* __restrict__ states,
+                                                   const int32_t* __restrict__ engine_of,
+                                                   const int32_t* __restrict__ begin,
+                                                   const Segment* __restrict__ segs, int mode_override,
+                                                   Guard gd, const float* __restrict__ lg) {
+    __shared__ uint64_t mt[kWords];
+    __shared__ uint64_t vst[2][kWords];
+    __shared__ ProgramLds s;
+    __shared__ BetaLds bl;
+    const int lane = threadIdx.x;
+    const int b = blockIdx.x;
+    EngineState* st0 = states + 2 * (int64_t)engine_of[b];
+    EngineState* st1 = st0 + 1;
+    P8Feed f;
+    f.v0 = &vst[0][0];
+    f.v1 = &vst[1][0];
+    f.mode = kP8Raw;
+    f.refills = 0;
+    int64_t used = 0;  // scalars consumed by the segments so far
+    p8_seek(f, st0, st1, 0, lane);
+    __syncthreads();
+    for (int k = begin[b]; k < begin[b + 1]; ++k) {
+        const int mode = mode_override >= 0 ? mode_override : p8_feed_mode(segs[k].kind);
+        if (mode < 0) {
+            if (lane == 0) atomicOr(gd.status, kStatusBadSegment);
+            continue;
+        }
+        Eng e;
+        e.mt = mt;
+        e.idx = kN;  // the first draw refills the block from the feed
+        e.half_valid = 0;
+        e.half_cache[0] = e.half_cache[1] = 0;
+        e.twisted = false;
+        e.feed = &f;
+        f.mode = mode;
+        f.refills = 0;
+        run_segment(e, s, bl, segs[k], gd, lg, lane);
+        // slots of the last packed block used: full draws, or half draws
+        // (an odd count leaves the last draw's upper half as the cache)
+        const int64_t last = mode == kP8H2 ? 2 * (2 * (int64_t)e.idx - e.half_valid)
+                           : mode == kP8F34 ? 3 * (int64_t)e.idx : 4 * (int64_t)e.idx;
+        if (f.refills > 0) used += (int64_t)(f.refills - 1) * p8_block_scalars(mode) + last;
+        p8_seek(f, st0, st1, used, lane);  // the next segment continues from there
+        __syncthreads();
+    }
+    store_state(st0, f.v0, lane);
+    store_state(st1, f.v1, lane);
+    if (lane == 0) {
+        st0->idx = st1->idx = f.idx;
+        st0->reserved = (uint64_t)f.b;
+    }
+}
+
+hipError_t launch_program_p8(EngineState* st, int32_t n_groups, const int32_t* d_engine_of, const int32_t* d_begin,
+                             const Segment* d_segs, int mode_override, Guard gd, hipStream_t s) {
+    if (n_groups <= 0) return hipSuccess;
+    const float* lg = lgamma_table(s);
+    if (!lg) return hipErrorOutOfMemory;
+    hipLaunchKernelGGL(k_program_p8, dim3((unsigned)n_groups), dim3(64), 0, s, st, d_engine_of, d_begin, d_segs,
+                       mode_override, gd, lg);
+    return hipGetLastError();
+}
+
 hipError_t launch_program(EngineState* st, int32_t n_groups, const int32_t* d_engine_of,
                           const int32_t* d_begin, const Segment* d_segs, Guard gd,
                           hipStream_t s) {

@@ -102,20 +102,44 @@ int erd_fill_u64(erd_engines* e, uint64_t* out, int64_t per_engine, void* stream
 /* P8_mt19937_64 = ParallelRandomEngineAdaptor<uint64_t, Vmt19937_64, 8>
  * (PacketRandomEngine.h:429-517, 600-609): adaptor k = the Vmt pair
  * (seeds[k], seeds[k]+4), so its 8 u64 lanes are mt19937_64(seed .. seed+7)
- * on any ISA. A set of n adaptors is an erd_engines set of 2n Vmt engines
- * (adaptor k = engines 2k, 2k+1); snapshot/copy/discard work on it as usual.
- * The adaptor's output buffers are empty between calls because every fill
- * is a whole number of refills. */
+ * on any ISA: its scalar stream is out[8r + j] = mt19937_64(seed + j)[r]. A
+ * set of n adaptors is an erd_engines set of 2n Vmt engines (adaptor k =
+ * engines 2k, 2k+1); snapshot/copy/discard work on it as usual. The record of
+ * engine 2k keeps the position in the operator() buffer (scalars of the last
+ * refill already returned) in its last word. */
 int erd_engines_create_p8(erd_engines** out, const uint64_t* seeds, int64_t n_p8, void* stream);
-/* operator() of P8_mt19937_64: out[8k + j] = mt19937_64(seed + j)[k];
- * per_p8 is a multiple of 8. */
+/* operator() of P8_mt19937_64, continuing from the buffer position:
+ * per_p8 scalars (a multiple of 4) per adaptor. */
 int erd_p8_fill_u64(erd_engines* e, uint64_t* out, int64_t per_p8, void* stream);
 /* operator() of P8_mt19937_64_32: 16 uint32 per refill, Vmt i's 4 lanes as
- * low/high word pairs at [8i, 8i+8); per_p8 is a multiple of 16. */
+ * low/high word pairs at [8i, 8i+8); per_p8 is a multiple of 16. Whole
+ * refills: it starts with a fresh refill, dropping any scalars an earlier u64
+ * or distribution fill left in the operator() buffer, and leaves the buffer
+ * position at 0. */
 int erd_p8_fill_u32(erd_engines* e, uint32_t* out, int64_t per_p8, void* stream);
 /* uniform_real() of the adaptor: bit_to_ur_float of the same 16 words per
- * refill; per_p8 is a multiple of 16. */
+ * refill; per_p8 is a multiple of 16; whole refills as above. */
 int erd_p8_fill_uniform_f32(erd_engines* e, float* out, int64_t per_p8, void* stream);
+/* A distribution fill with P8_mt19937_64 as its engine, as the reference's
+ * own tests draw (test/test.cpp:117 and on): `Rand::xxx<ArrayXf>(per_p8, 1,
+ * p8_k, a, b)` for each adaptor k, out = n blocks of per_p8. The packet code
+ * takes the adaptor as a scalar full-bit engine and packs its scalars with
+ * RawbitsMaker<..., scalar_fullbit> (arch/AVX/RandUtils.h:59-144,
+ * arch/SSE/RandUtils.h:22-67). Those packings call _mm*_set*_epi64x(rng(),
+ * rng(), ...), whose argument evaluation order C++ leaves unspecified; this
+ * follows Clang's left-to-right order (the reference's default toolchain
+ * preset, CMakePresets.json:16-17):
+ *  - float samplers (Packet8f uniform_real, rawbits_34): three scalars a, b, c
+ *    per draw give the 32-bit words [a.lo, a.hi, b.lo, X1, b.hi, c.lo, c.hi,
+ *    X2], X1 = the top bytes of a.lo, a.hi, b.lo, b.lo, X2 those of b.hi,
+ *    c.lo, c.hi, c.hi. dist: uniform, normal (std or mean/stdev), exponential,
+ *    Cauchy, Student-t, gamma, lognormal, Weibull, extreme value (float out);
+ *  - int32 samplers (Packet4i rawbits, _mm_set_epi64x(rng(), rng())): two
+ *    scalars per half draw, the second in u64 lane 0. dist: Poisson,
+ *    binomial (int32 out).
+ * Other kinds return ERD_ERR_UNSUPPORTED. a, b as in erd_request; per_p8 a
+ * multiple of the packet width (8 floats, 4 int32). */
+int erd_p8_fill_dist(erd_engines* e, int32_t dist, void* out, int64_t per_p8, double a, double b, void* stream);
 
 /* ---------------------------------------------------------- univariate --- */
 /* uniformReal<float>: StdUniformRealGen (Basic.h:584-590); with (a,b) it is

@@ -86,7 +86,14 @@ inline uint64_t temper(uint64_t y) {
     return y;
 }
 
+// A record with half_valid < 0 stands for a scalar engine fed into packet
+// code (P8_mt19937_64 through RawbitsMaker, below); `reserved` points to it.
+struct P8Scalar;
+void p8s_full(P8Scalar* f, uint64_t out[4]);
+void p8s_half(P8Scalar* f, uint64_t out[2]);
+
 inline void vmt_draw(Vmt& e, uint64_t out[4]) {
+    if (e.half_valid < 0) { p8s_full(reinterpret_cast<P8Scalar*>(e.reserved), out); return; }
     if (e.idx >= kN) { vmt_twist(e); e.idx = 0; }
     for (int l = 0; l < 4; ++l) out[l] = temper(e.mt[e.idx * 4 + l]);
     e.idx++;
@@ -94,6 +101,7 @@ inline void vmt_draw(Vmt& e, uint64_t out[4]) {
 
 // half(): low 128 bits now, high 128 bits cached (PacketRandomEngine.h:325-336).
 inline void vmt_half(Vmt& e, uint64_t out[2]) {
+    if (e.half_valid < 0) { p8s_half(reinterpret_cast<P8Scalar*>(e.reserved), out); return; }
     if (e.half_valid) {
         e.half_valid = 0;
         out[0] = e.half_cache[0]; out[1] = e.half_cache[1];
@@ -1642,4 +1650,106 @@ extern "C" void orc_std_normal_f32(float* out, int64_t n, uint64_t seed, int thr
         });
     }
     for (auto& th : pool) th.join();
+}
+
+
+// ------------------------------------------------ scalar-engine (P8) feeds ---
+// P8_mt19937_64 (ParallelRandomEngineAdaptor<uint64_t, Vmt19937_64, 8>,
+// PacketRandomEngine.h:429-517) as a scalar full-bit engine: operator()
+// returns buf[cnt++], a refill drawing Vmt (seed) then Vmt (seed + 4). Packet
+// code takes it through RawbitsMaker<..., scalar_fullbit> with the argument
+// order of Clang (left to right):
+//  mode 0, raw: operator() in order, four scalars per draw;
+//  mode 1, Packet8f rawbits_34 (arch/AVX/RandUtils.h:75-131):
+//          _mm256_setr_epi64x(a, b, c, 0), the 8x32 permutation (0,1,2,7,3,
+//          4,5,7) and the byte shuffle (0..11, 3, 7, 11, 11) per 128 bits;
+//  mode 2, Packet4i rawbits (arch/SSE/RandUtils.h:24-41):
+//          _mm_set_epi64x(first, second), lane 0 = the second scalar.
+namespace {
+struct P8Scalar {
+    Vmt* v0;
+    Vmt* v1;
+    uint64_t buf[8];
+    int cnt;   // 8: empty
+    int mode;
+    uint64_t next() {
+        if (cnt >= 8) {
+            vmt_draw(*v0, buf);
+            vmt_draw(*v1, buf + 4);
+            cnt = 0;
+        }
+        return buf[cnt++];
+    }
+};
+
+void p8s_full(P8Scalar* f, uint64_t out[4]) {
+    if (f->mode == 0) {
+        for (int l = 0; l < 4; ++l) out[l] = f->next();
+        return;
+    }
+    // rawbits_34: bytes of the 8 32-bit words after set/permute/shuffle
+    const uint64_t a = f->next(), b = f->next(), c = f->next();
+    uint32_t w[8] = {(uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b, 0,
+                     (uint32_t)(b >> 32), (uint32_t)c, (uint32_t)(c >> 32), 0};
+    auto top = [](uint32_t x) { return x >> 24; };
+    w[3] = top(w[0]) | (top(w[1]) << 8) | (top(w[2]) << 16) | (top(w[2]) << 24);
+    w[7] = top(w[4]) | (top(w[5]) << 8) | (top(w[6]) << 16) | (top(w[6]) << 24);
+    for (int l = 0; l < 4; ++l) out[l] = (uint64_t)w[2 * l] | ((uint64_t)w[2 * l + 1] << 32);
+}
+
+void p8s_half(P8Scalar* f, uint64_t out[2]) {
+    const uint64_t first = f->next(), second = f->next();
+    out[0] = second;
+    out[1] = first;
+}
+
+}  // namespace
+
+// dist (erd_dist codes) filled per adaptor with P8 adaptor k (records 2k, 2k+1)
+// as its engine; mode_override >= 0 forces a packing (0 for operator()).
+// Record 2k's `reserved` keeps the buffer position (scalars of the last
+// refill returned; its draws are the records' last ones).
+extern "C" int orc_p8_fill(void* states, int64_t n_p8, int dist, int mode_override, void* out, int64_t per,
+                           double a, double b) {
+    Vmt* st = (Vmt*)states;
+    for (int64_t k = 0; k < n_p8; ++k) {
+        P8Scalar f;
+        f.v0 = &st[2 * k];
+        f.v1 = &st[2 * k + 1];
+        const int b0 = (int)f.v0->reserved;
+        f.cnt = 8;
+        if (b0 > 0) {
+            for (int l = 0; l < 4; ++l) {
+                f.buf[l] = temper(f.v0->mt[(f.v0->idx - 1) * 4 + l]);
+                f.buf[4 + l] = temper(f.v1->mt[(f.v1->idx - 1) * 4 + l]);
+            }
+            f.cnt = b0;
+        }
+        const bool ints = dist == 10 || dist == 11;
+        f.mode = mode_override >= 0 ? mode_override : dist == 0 ? 0 : ints ? 2 : 1;
+        Vmt v;
+        std::memset(&v, 0, sizeof(v));
+        v.half_valid = -1;
+        v.reserved = (uint64_t)(uintptr_t)&f;
+        float* of = (float*)out + k * per;
+        int32_t* oi = (int32_t*)out + k * per;
+        switch (dist) {
+            case 0: orc_fill_u64(&v, 1, (uint64_t*)out + k * per, per / 4, 1); break;
+            case 1: orc_fill_uniform_f32(&v, 1, of, per, (float)a, (float)b, !(a == 0.0 && b == 1.0), 1); break;
+            case 3: orc_fill_normal_f32(&v, 1, of, per, 0.f, 1.f, 0, 1); break;
+            case 4: orc_fill_normal_f32(&v, 1, of, per, (float)a, (float)b, 1, 1); break;
+            case 6: orc_fill_exponential_f32(&v, 1, of, per, (float)a, 1); break;
+            case 7: orc_fill_cauchy_f32(&v, 1, of, per, (float)a, (float)b, 1); break;
+            case 8: orc_fill_studentt_f32(&v, 1, of, per, (float)a, 1); break;
+            case 9: orc_fill_gamma_f32(&v, 1, of, per, (float)a, (float)b, 1); break;
+            case 10: orc_fill_poisson_i32(&v, 1, oi, per, a, 1); break;
+            case 11: orc_fill_binomial_i32(&v, 1, oi, per, (int32_t)a, b, 1); break;
+            case 16: orc_fill_lognormal_f32(&v, 1, of, per, (float)a, (float)b, 1); break;
+            case 17: orc_fill_weibull_f32(&v, 1, of, per, (float)a, (float)b, 1); break;
+            case 18: orc_fill_extreme_value_f32(&v, 1, of, per, (float)a, (float)b, 1); break;
+            default: return -1;
+        }
+        f.v0->reserved = f.cnt >= 8 ? 0 : (uint64_t)f.cnt;
+    }
+    return 0;
 }

@@ -446,3 +446,27 @@ def fill_discrete_d_i32(e, per, w):
     assert bits >= 0
     f(_p(e.raw), I64(len(e)), _p(out), I64(per), None, C.c_int32(0), _p(prob), _p(alias), C.c_uint32(size - 1))
     return out
+
+
+P8_DIST = {"u64": 0, "uniform": 1, "std_normal": 3, "normal": 4, "exponential": 6, "cauchy": 7,
+           "student_t": 8, "gamma": 9, "poisson": 10, "binomial": 11, "lognormal": 16, "weibull": 17,
+           "extreme_value": 18}
+
+
+def p8_engines(seeds):
+    """Records of P8_mt19937_64 adaptors: adaptor k = Vmt (seed_k), Vmt (seed_k + 4)."""
+    return Engines(seeds=[s + d for s in seeds for d in (0, 4)])
+
+
+def p8_fill(e, dist, per, a=0.0, b=0.0, mode=-1):
+    """`dist` with each P8 adaptor of `e` (records 2k, 2k+1) as its engine;
+    mode 0 forces operator() draws (the raw stream)."""
+    code = P8_DIST[dist]
+    dt = np.uint64 if dist == "u64" else np.int32 if dist in ("poisson", "binomial") else np.float32
+    out = np.empty((len(e) // 2) * per, dtype=dt)
+    f = lib().orc_p8_fill
+    f.restype = C.c_int
+    rc = f(_p(e.raw), I64(len(e) // 2), C.c_int(code), C.c_int(mode), _p(out), I64(per), C.c_double(a),
+           C.c_double(b))
+    assert rc == 0
+    return out

@@ -214,7 +214,7 @@ def test_p8_adaptor_streams(er, orc, mt_golden):
 def test_p8_rejects_bad_sizes(er):
     urng = er.P8_mt19937_64(5)
     with pytest.raises(er.InvalidArgument):
-        er.p8_randbits_u64(12, urng)
+        er.p8_randbits_u64(10, urng)
     with pytest.raises(er.InvalidArgument):
         er.p8_uniform_real(8, urng)
     odd = er.Vmt19937_64(1, n=3)
