@@ -602,6 +602,28 @@ def binomial_vop(urng, trials, p, out=None, stream=None):
     return buf
 
 
+def binomial_vop_float(urng, trials, p, out=None, stream=None):
+    """Rand::binomial(urng, trials, p) with float-typed trials
+    (BinomialVGen<float>, Discrete.h:1196-1331): Packet8f lanes on full draws.
+    The reference keeps trials and results as int32 bit patterns in float
+    lanes; here trials is an int32 tensor and the result is int32. p may be a
+    Python number (the scalar-p form MultinomialGen uses)."""
+    torch = _torch()
+    trials = _param(trials, torch.int32, "erd_fill_binomial_vop_f32: trials")
+    per = _vop_per(urng, trials.numel(), "erd_fill_binomial_vop_f32")
+    buf = _alloc(urng, per, torch.int32, out)
+    if isinstance(p, (int, float)):
+        _lib.call("erd_fill_binomial_vop_vs_f32", urng.handle, _ptr(buf), per, _ptr(trials), float(p),
+                  _stream_ptr(stream))
+        return buf
+    p = _param(p, torch.float32, "erd_fill_binomial_vop_f32: p")
+    if p.numel() != trials.numel():
+        raise InvalidArgument("erd_fill_binomial_vop_f32", 1)
+    _lib.call("erd_fill_binomial_vop_f32", urng.handle, _ptr(buf), per, _ptr(trials), _ptr(p),
+              _stream_ptr(stream))
+    return buf
+
+
 def gamma_vop(urng, alpha, beta, out=None, stream=None):
     """Per-element gamma(alpha[i], beta[i]) (float32 CUDA tensors); the
     lockstep vectorize-over-params design of BinomialVGen applied to GammaGen."""

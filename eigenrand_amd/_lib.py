@@ -70,6 +70,8 @@ _SIGS = {
     "erd_fill_discrete_d_i32": [P, P, I64, P, I64, P],
     "erd_fill_normal_vop_sv_f32": [P, P, I64, F32, P, P],
     "erd_fill_binomial_vop_vs_i32": [P, P, I64, P, F32, P],
+    "erd_fill_binomial_vop_f32": [P, P, I64, P, P, P],
+    "erd_fill_binomial_vop_vs_f32": [P, P, I64, P, F32, P],
     "erd_fill_binomial_vop_sv_i32": [P, P, I64, INT, P, P],
     "erd_fill_std_normal_f32_fast": [P, P, I64, P],
     "erd_fill_normal_f32_fast": [P, P, I64, F32, F32, P],

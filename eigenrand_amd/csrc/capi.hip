@@ -574,6 +574,26 @@ int erd_fill_binomial_vop_sv_i32(erd_engines* e, int32_t* out, int64_t per, int3
                                                         erd::Param<float>{p, 0.f}, guard_of(e), S(stream)));
 }
 
+int erd_fill_binomial_vop_f32(erd_engines* e, int32_t* out, int64_t per, const int32_t* trials, const float* p,
+                              void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    if (per > 0 && (!trials || !p)) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_fill_binomial_vop_f32(e->states, e->count, out, per,
+                                                        erd::Param<int32_t>{trials, 0},
+                                                        erd::Param<float>{p, 0.f}, guard_of(e), S(stream)));
+}
+
+int erd_fill_binomial_vop_vs_f32(erd_engines* e, int32_t* out, int64_t per, const int32_t* trials, float p,
+                                 void* stream) {
+    int rc = check_fill(e, out, per, 8);
+    if (rc) return rc;
+    if (per > 0 && !trials) return ERD_ERR_INVALID;
+    return hip_status(erd::launch_fill_binomial_vop_f32(e->states, e->count, out, per,
+                                                        erd::Param<int32_t>{trials, 0},
+                                                        erd::Param<float>{nullptr, p}, guard_of(e), S(stream)));
+}
+
 int erd_fill_chisquared_f32(erd_engines* e, float* out, int64_t per, float n, void* stream) {
     return erd_fill_gamma_f32(e, out, per, n * 0.5f, 2.0f, stream);
 }

@@ -143,6 +143,9 @@ hipError_t launch_fill_binomial_small_i32(EngineState* st, int64_t n, int32_t* o
 hipError_t launch_fill_binomial_vop_i32(EngineState* st, int64_t n, int32_t* out, int64_t per,
                                         Param<int32_t> trials, Param<float> p, Guard gd,
                                         hipStream_t s);
+hipError_t launch_fill_binomial_vop_f32(EngineState* st, int64_t n, int32_t* out, int64_t per,
+                                        Param<int32_t> trials, Param<float> p, Guard gd,
+                                        hipStream_t s);
 
 hipError_t launch_fill_gamma_vop_f32(EngineState* st, int64_t n, float* out, int64_t per,
                                      const float* alpha, const float* beta, Guard gd,

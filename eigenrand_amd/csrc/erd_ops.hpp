@@ -1732,19 +1732,18 @@ __device__ void op_multinomial_small(Eng& e, int32_t* out, int64_t n, int32_t T,
     e.idx = pos;
 }
 
-// BinomialVGen<float>::packetOp at Packet8f (Discrete.h:1251-1330), the
-// per-row sampler of MultinomialGen's large-trials path (Multinomial.h:74-91):
-// trials come from r[] (int32), p is one float for the row, and the row
-// result t goes to out[i*stride] while r[i] -= t. Full draws; thread 8i+j
-// runs iteration i of lane j (8 iterations per step). REG (n <= 64): thread
-// 8i+j keeps sample 8i+j's remaining trials in `rreg` instead of r[].
-// The row constants (p, pmin(p, 1-p) and the logs of the latter and of its
-// complement) come from the caller, which computes them for many rows at once.
-template <bool REG>
-__device__ void op_binomial_row8(Eng& e, uint64_t* carry, int32_t* out, int64_t stride, int32_t* r,
-                                 int32_t& rreg, int32_t t0, int64_t n, float p, float psmall, float plp,
-                                 float plq, Guard gd, const float* lg, int lane) {
-    const int64_t T = n / 8;
+// BinomialVGen<float>::packetOp at Packet8f (Discrete.h:1251-1330): full
+// draws, lockstep Knuth for the lanes with mean < 1, then lockstep tangent
+// rejection keeping each lane's first accepted candidate; thread 8i+j runs
+// iteration i of lane j (8 iterations per step). T packets; the source S
+// gives each packet's lane parameters and takes its results:
+//   S::params(pk, lane, nt, p, psmall, plp, plq): lane j = lane & 7 of packet
+//     pk (called by all 64 threads; nt = trials, psmall = pmin(p, 1 - p),
+//     plp/plq = plog(psmall) / plog(1 - psmall));
+//   S::result(pk, lane, nt, t): the lane's result t (called by all threads).
+template <class S>
+__device__ void op_binomial_v8(Eng& e, uint64_t* carry, int64_t T, S& src, Guard gd, const float* lg,
+                               int lane) {
     if (T == 0) return;
     const int i = lane >> 3, j = lane & 7;
     Cursor<4, 1> cur;
@@ -1756,8 +1755,9 @@ __device__ void op_binomial_row8(Eng& e, uint64_t* carry, int32_t* out, int64_t 
     cur.eng = &e;
     const uint64_t jmask = 0x0101010101010101ull << j;
     for (int64_t pk = 0; pk < T; ++pk) {
-        // t0: first row, every sample has t0 trials
-        const int32_t nt = t0 >= 0 ? t0 : REG ? __shfl(rreg, 8 * (int)pk + j) : r[8 * pk + j];
+        int32_t nt;
+        float p, psmall, plp, plq;
+        src.params(pk, lane, nt, p, psmall, plp, plq);
         const float ptr = (float)nt;
         const float pmean = ptr * psmall;
         const float pg1 = plgamma_int(lg, ptr + 1.0f);  // trials + 1 is integer-valued
@@ -1838,15 +1838,83 @@ __device__ void op_binomial_row8(Eng& e, uint64_t* carry, int32_t* out, int64_t 
             }
         }
         const int32_t k = (int32_t)valid_res;
-        const int32_t t = (p == psmall) ? k : nt - k;
+        src.result(pk, lane, nt, (p == psmall) ? k : nt - k);
+    }
+    __syncthreads();
+    e.idx = cur.pos;
+}
+
+// The per-row source of MultinomialGen's large-trials path
+// (Multinomial.h:74-91): trials come from r[] (int32), p is one float for the
+// row, and the row result t goes to out[i*stride] while r[i] -= t. REG
+// (n <= 64): thread 8i+j keeps sample 8i+j's remaining trials in `rreg`
+// instead of r[]. t0 >= 0: the first row, every sample has t0 trials. The row
+// constants come from the caller, which computes them for many rows at once.
+template <bool REG>
+struct BinomialRowSrc {
+    int32_t* out;
+    int64_t stride;
+    int32_t* r;
+    int32_t& rreg;
+    int32_t t0;
+    float p, psmall, plp, plq;
+    __device__ void params(int64_t pk, int lane, int32_t& nt, float& pp, float& ps, float& lp, float& lq) {
+        const int j = lane & 7;
+        nt = t0 >= 0 ? t0 : REG ? __shfl(rreg, 8 * (int)pk + j) : r[8 * pk + j];
+        pp = p;
+        ps = psmall;
+        lp = plp;
+        lq = plq;
+    }
+    __device__ void result(int64_t pk, int lane, int32_t nt, int32_t t) {
+        const int i = lane >> 3, j = lane & 7;
         if (REG && i == pk) rreg = nt - t;
         if (lane < 8) {
             out[(8 * pk + j) * stride] = t;
             if (!REG) r[8 * pk + j] = nt - t;
         }
     }
-    __syncthreads();
-    e.idx = cur.pos;
+};
+
+template <bool REG>
+__device__ void op_binomial_row8(Eng& e, uint64_t* carry, int32_t* out, int64_t stride, int32_t* r,
+                                 int32_t& rreg, int32_t t0, int64_t n, float p, float psmall, float plp,
+                                 float plq, Guard gd, const float* lg, int lane) {
+    BinomialRowSrc<REG> src{out, stride, r, rreg, t0, p, psmall, plp, plq};
+    op_binomial_v8(e, carry, n / 8, src, gd, lg, lane);
 }
+
+// binomial(urng, trials, p) with BinomialVGen<float> (Discrete.h:1196-1331,
+// free functions 1824-1897): element k has trials[k] and p[k]; the
+// reference carries the int32 trials and results as bit patterns in float
+// lanes, here they are int32. The constants of 8 packets are computed at a
+// time, thread t for element 8*pk + t, and read per packet with a shuffle.
+struct BinomialVopFSrc {
+    int32_t* out;
+    Param<int32_t> trials;
+    Param<float> prob;
+    int64_t n;  // elements (8 per packet)
+    int32_t g_n;
+    float g_p, g_ps, g_lp, g_lq;
+    __device__ void params(int64_t pk, int lane, int32_t& nt, float& pp, float& ps, float& lp, float& lq) {
+        if ((pk & 7) == 0) {
+            const bool in = 8 * pk + lane < n;
+            g_n = in ? trials[8 * pk + lane] : 1;
+            g_p = in ? prob[8 * pk + lane] : 0.5f;
+            g_ps = (g_p < 1.0f - g_p) ? g_p : 1.0f - g_p;  // pmin(p, 1-p)
+            g_lp = plog(g_ps);
+            g_lq = plog(1.0f - g_ps);
+        }
+        const int src = 8 * (int)(pk & 7) + (lane & 7);
+        nt = __shfl(g_n, src);
+        pp = __shfl(g_p, src);
+        ps = __shfl(g_ps, src);
+        lp = __shfl(g_lp, src);
+        lq = __shfl(g_lq, src);
+    }
+    __device__ void result(int64_t pk, int lane, int32_t, int32_t t) {
+        if (lane < 8) out[8 * pk + lane] = t;
+    }
+};
 
 }  // namespace erd

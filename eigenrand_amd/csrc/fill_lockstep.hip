@@ -90,6 +90,25 @@ __global__ __launch_bounds__(64) void k_binomial_vop(EngineState* __restrict__ s
     eng_store(e, states + g, lane);
 }
 
+// BinomialVGen<float>: Packet8f lanes on full draws, per-element trials and p.
+__global__ __launch_bounds__(64) void k_binomial_vopf(EngineState* __restrict__ states,
+                                                      int32_t* __restrict__ out, int64_t per,
+                                                      Param<int32_t> trials, Param<float> prob,
+                                                      Guard gd,
+                                                      const float* __restrict__ lg) {
+    __shared__ uint64_t mt[kWords];
+    __shared__ uint64_t carry[2 * kLanes];
+    const int lane = threadIdx.x;
+    const int64_t g = blockIdx.x;
+    const int64_t T = per / 8;
+    if (T == 0) return;
+    Eng e;
+    eng_load(e, mt, states + g, lane);
+    BinomialVopFSrc src{out + g * per, trials + g * per, prob + g * per, per, 0, 0.f, 0.f, 0.f, 0.f};
+    op_binomial_v8(e, carry, T, src, gd, lg, lane);
+    eng_store(e, states + g, lane);
+}
+
 // Vectorize-over-params gamma (Packet8f) and Poisson (Packet4f).
 __global__ __launch_bounds__(64) void k_gamma_vop(EngineState* __restrict__ states,
                                                   float* __restrict__ out, int64_t per,
@@ -166,6 +185,17 @@ hipError_t launch_fill_binomial_vop_i32(EngineState* st, int64_t n, int32_t* out
     const float* lg = lgamma_table(s);
     if (!lg) return hipErrorOutOfMemory;
     hipLaunchKernelGGL(k_binomial_vop, dim3((unsigned)n), dim3(64), 0, s, st, out, per, trials, p,
+                       gd, lg);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_binomial_vop_f32(EngineState* st, int64_t n, int32_t* out, int64_t per,
+                                        Param<int32_t> trials, Param<float> p, Guard gd,
+                                        hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const float* lg = lgamma_table(s);
+    if (!lg) return hipErrorOutOfMemory;
+    hipLaunchKernelGGL(k_binomial_vopf, dim3((unsigned)n), dim3(64), 0, s, st, out, per, trials, p,
                        gd, lg);
     return hipGetLastError();
 }

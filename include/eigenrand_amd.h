@@ -306,6 +306,16 @@ int erd_fill_binomial_vop_vs_i32(erd_engines* e, int32_t* out, int64_t per_engin
                                  void* stream);
 int erd_fill_binomial_vop_sv_i32(erd_engines* e, int32_t* out, int64_t per_engine, int32_t trials, const float* p,
                                  void* stream);
+/* binomial(urng, trials, p) with float-typed trials: BinomialVGen<float>
+ * (Discrete.h:1196-1331; free functions 1824-1848 and the scalar-p form
+ * 1851-1881 that MultinomialGen uses). Packet8f lanes on full draws, so the
+ * stream differs from the int32 form's. The reference carries trials and
+ * results as int32 bit patterns in float lanes; here they are int32 arrays.
+ * per_engine is a multiple of 8. */
+int erd_fill_binomial_vop_f32(erd_engines* e, int32_t* out, int64_t per_engine, const int32_t* trials,
+                              const float* p, void* stream);
+int erd_fill_binomial_vop_vs_f32(erd_engines* e, int32_t* out, int64_t per_engine, const int32_t* trials, float p,
+                                 void* stream);
 
 /* Vectorize-over-params gamma and Poisson. The reference has no GammaVGen or
  * PoissonVGen (its VoP generators stop at BinomialVGen); these follow the

@@ -1129,6 +1129,18 @@ void orc_fill_binomial_vop_i32(void* states, int64_t n_engines, int32_t* out, in
     });
 }
 
+// binomial(urng, trials, p) with BinomialVGen<float> (Discrete.h:1196-1331):
+// Packet8f lanes on full draws, per-element trials and p.
+void orc_fill_binomial_vopf_i32(void* states, int64_t n_engines, int32_t* out, int64_t per_engine,
+                                const int32_t* trials, const float* p, int threads) {
+    Vmt* e = (Vmt*)states;
+    for_engines(n_engines, threads, [&](int64_t g) {
+        int64_t base = g * per_engine;
+        for (int64_t i = 0; i < per_engine; i += 8)
+            binomialv<8>(e[g], trials + base + i, p + base + i, out + base + i);
+    });
+}
+
 // Reused generator objects (`auto gen = Rand::makeXxxGen(...)` followed by
 // several gen.generate() calls): per-engine generators that keep the normal
 // generator's cached cos packet and the rejection samplers' CompressMask

@@ -271,6 +271,18 @@ def fill_binomial_vop_i32(e, trials, p, threads=1):
     return out
 
 
+def fill_binomial_vopf_i32(e, trials, p, threads=1):
+    """BinomialVGen<float>: Packet8f lanes on full draws."""
+    trials = np.ascontiguousarray(trials, dtype=np.int32)
+    p = np.ascontiguousarray(np.broadcast_to(np.float32(p), trials.shape) if np.isscalar(p) else p,
+                             dtype=np.float32)
+    per = trials.size // len(e)
+    out = np.empty(trials.size, dtype=np.int32)
+    lib().orc_fill_binomial_vopf_i32(_p(e.raw), I64(len(e)), _p(out), I64(per), _p(trials), _p(p),
+                                     C.c_int(threads))
+    return out
+
+
 def _elementwise(name, x, nout=1, dtype=np.float32):
     x = np.ascontiguousarray(x, dtype=dtype)
     outs = [np.empty_like(x) for _ in range(nout)]

@@ -48,3 +48,28 @@ def test_normal_mixed_overloads(er, orc):
         got = er.normal_vop(a, m_arg, s_arg).cpu().numpy()
         assert_same(got, orc.fill_normal_vop_f32(ref, mu, s))
         np.testing.assert_array_equal(a.snapshot(), ref.raw)
+
+
+def test_binomial_float_form_matches_oracle(er, orc):
+    """BinomialVGen<float> (Discrete.h:1196-1331): Packet8f lanes on full
+    draws, per-element (trials, p) including the edges (n = 0, p = 0 and 1,
+    the Knuth regime), then the scalar-p form on the continuing engines."""
+    import torch
+    rng = np.random.default_rng(8)
+    n = len(SEEDS) * PER
+    trials = rng.integers(1, 1001, n).astype(np.int32)
+    p = rng.uniform(0, 1, n).astype(np.float32)
+    trials[::5] = rng.integers(1, 30, trials[::5].size)
+    p[::7] = np.float32(0.003)
+    trials[::11] = 0
+    p[::13] = np.float32(1.0)
+    p[::17] = np.float32(0.0)
+    a = er.Vmt19937_64(seeds=SEEDS)
+    ref = orc.Engines(seeds=SEEDS)
+    t_dev = torch.from_numpy(trials).cuda()
+    got = er.binomial_vop_float(a, t_dev, torch.from_numpy(p).cuda()).cpu().numpy()
+    assert_same(got, orc.fill_binomial_vopf_i32(ref, trials, p))
+    got = er.binomial_vop_float(a, t_dev, 0.37).cpu().numpy()
+    assert_same(got, orc.fill_binomial_vopf_i32(ref, trials, np.full(n, 0.37, np.float32)))
+    np.testing.assert_array_equal(a.snapshot(), ref.raw)
+    assert a.status() == 0

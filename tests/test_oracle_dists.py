@@ -111,3 +111,26 @@ def test_binomial_vop_means(orc):
         m, v = nn * pp, nn * pp * (1 - pp)
         assert abs(x[g].mean() - m) < 5 * math.sqrt(v / n_per)
         assert abs(x[g].var() / v - 1) < 0.15
+
+
+@pytest.mark.parametrize("fill", ["fill_binomial_vop_i32", "fill_binomial_vopf_i32"])
+def test_binomial_vop_randomised_pit(orc, fill):
+    """Per-element (n_i, p_i) as in the VoP configuration (n ~ U{1..1000},
+    p ~ U(0, 1)): the randomised PIT F(k-1) + V (F(k) - F(k-1)) is U(0, 1),
+    for the int32 (Packet4i, half draws) and float (Packet8f, full draws)
+    forms of BinomialVGen."""
+    rng = np.random.default_rng(11)
+    per = 4096
+    n_eng = 32
+    trials = rng.integers(1, 1001, n_eng * per).astype(np.int32)
+    p = rng.uniform(0, 1, n_eng * per).astype(np.float32)
+    e = orc.Engines(n=n_eng, seed0=99)
+    k = getattr(orc, fill)(e, trials, p, threads=8).astype(np.float64)
+    pp = p.astype(np.float64)
+    lo = stats.binom.cdf(k - 1, trials, pp)
+    hi = stats.binom.cdf(k, trials, pp)
+    u = lo + np.random.default_rng(5).uniform(size=k.size) * (hi - lo)
+    assert stats.kstest(u, "uniform").pvalue > 1e-4
+    small = trials.astype(np.float64) * np.minimum(pp, 1 - pp) < 1  # the Knuth regime
+    assert stats.kstest(u[small], "uniform").pvalue > 1e-4
+    assert stats.kstest(u[~small], "uniform").pvalue > 1e-4

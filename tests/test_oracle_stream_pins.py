@@ -669,6 +669,29 @@ def test_binomial_vop_of_the_stream(orc):
             np.testing.assert_array_equal(got[g, 4 * k:4 * k + 4], v, err_msg=f"engine {g} packet {k}")
 
 
+def test_binomial_vop_float_of_the_stream(orc):
+    """BinomialVGen<float> (Discrete.h:1251-1330): the same packet steps on
+    Packet8f lanes of full draws, per-element trials and p."""
+    rng = np.random.default_rng(37)
+    per = 256
+    n = rng.integers(1, 1001, 2 * per).astype(np.int32)
+    p = rng.uniform(0, 1, 2 * per).astype(f32)
+    n[::5] = rng.integers(1, 30, n[::5].size)
+    p[::7] = f32(0.003)
+    n[::11] = 0
+    p[::13] = f32(1.0)
+    e = orc.Engines(n=2, seed0=4048)
+    got = orc.fill_binomial_vopf_i32(e.copy(), n, p).reshape(2, per)
+    words = full_words(orc, e, 40 * per)
+    for g in range(2):
+        s = 0
+        for k in range(per // 8):
+            sl = slice(g * per + 8 * k, g * per + 8 * k + 8)
+            v, used = _binomial_v_lanes(orc, lambda q: ur(words[g][q]), s, n[sl], p[sl])
+            s += used
+            np.testing.assert_array_equal(got[g, 8 * k:8 * k + 8], v, err_msg=f"engine {g} packet {k}")
+
+
 # ----------------------------------------------------------- MultinomialGen ---
 def _multinomial_replay(orc, e, trials, w, samples):
     """MultinomialGen<int32_t>::generate(urng, samples) (Multinomial.h:60-133)
