@@ -68,17 +68,20 @@ def build_library(verbose=False, jobs=None):
 
 CXX = os.environ.get("CXX", "g++")
 SHIM_EXAMPLE = os.path.join(ROOT, "tests", "cpp", "shim_example")
+EIGEN_EXAMPLE = os.path.join(ROOT, "tests", "cpp", "eigen_frontend_example")
 
 
 def build_shim_example(verbose=False):
-    """The C++ front end's example program, linked against the library."""
-    src = SHIM_EXAMPLE + ".cpp"
-    cmd = [CXX, "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), src,
-           "-L", HERE, "-lerd_amd", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../eigenrand_amd",
-           "-o", SHIM_EXAMPLE]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+    """The C++ front ends' example programs, linked against the library: the
+    erd::rand header (C++17) and the Eigen::Rand header (built as C++11, the
+    standard the reference requires)."""
+    for exe, std in ((SHIM_EXAMPLE, "c++17"), (EIGEN_EXAMPLE, "c++11")):
+        cmd = [CXX, f"-std={std}", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), exe + ".cpp",
+               "-L", HERE, "-lerd_amd", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../eigenrand_amd",
+               "-o", exe]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
     return SHIM_EXAMPLE
 
 

@@ -59,6 +59,22 @@ def test_cpp_front_end_compiles():
         assert r.returncode == 0, r.stderr
 
 
+@pytest.mark.parametrize("std", ["c++11", "c++14", "c++17", "c++20"])
+def test_eigen_front_end_compiles(std):
+    """include/eigenrand_amd/EigenRand and its example program (written the
+    way EigenRand code is written, over an Eigen-like array type) build
+    warning-free as C++11 (the reference's standard) and later standards."""
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("no host C++ compiler")
+    src = os.path.join(ROOT, "tests", "cpp", "eigen_frontend_example.cpp")
+    r = subprocess.run([cxx, f"-std={std}", "-Wall", "-Wextra", "-pedantic", "-Werror", "-fsyntax-only",
+                        "-I", os.path.join(ROOT, "include"), src], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_vop_parameter_tensors_are_checked_before_launch():
     """Host tensors and wrong dtypes raise InvalidArgument before any engine,
     allocation or launch is touched (so no GPU is needed to see it)."""
