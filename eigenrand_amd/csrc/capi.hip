@@ -898,7 +898,7 @@ int run_segments(erd_engines* e, const std::vector<Segment>& segs, const std::ve
     const size_t seg_bytes = sizeof(Segment) * (size_t)n;
     const size_t idx_bytes = sizeof(int32_t) * (size_t)(2 * groups + 1);
     char* dbuf = nullptr;
-    hipError_t err = hipMallocAsync((void**)&dbuf, seg_bytes + idx_bytes, stream);
+    hipError_t err = scratch_alloc((void**)&dbuf, seg_bytes + idx_bytes, stream);
     if (err != hipSuccess) return err == hipErrorOutOfMemory ? ERD_ERR_OOM : ERD_ERR_HIP;
     std::vector<char> host(seg_bytes + idx_bytes);
     std::memcpy(host.data(), segs.data(), seg_bytes);
@@ -914,8 +914,7 @@ int run_segments(erd_engines* e, const std::vector<Segment>& segs, const std::ve
         err = launch_program(e->states, groups, (const int32_t*)(dbuf + seg_bytes),
                              (const int32_t*)(dbuf + seg_bytes) + groups, (const Segment*)dbuf,
                              guard_of(e), stream);
-    hipError_t e2 = hipFreeAsync(dbuf, stream);
-    if (err == hipSuccess) err = e2;
+    scratch_free(dbuf, seg_bytes + idx_bytes, stream);
     return err == hipSuccess ? ERD_OK : ERD_ERR_HIP;
 }
 

@@ -16,6 +16,18 @@ namespace erd {
 // does not wait for the stream.
 hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t s);
 
+// Per-call device scratch (staging.hip). hipFreeAsync of a large block can
+// hold up the calling thread for up to milliseconds, and the multivariate
+// calls free several GB-sized blocks per call, so released scratch goes to a
+// per-device cache instead: a block is tagged with an event recorded on its
+// stream at release and is handed out again, without a driver call, to the
+// same stream at once (stream order makes that safe) or to another stream
+// once its event has completed. Blocks come in size classes 2^k * {4,5,6,7}/4
+// (small ones: 4 KiB). Cached blocks stay allocated until an allocation
+// fails, which releases every idle block and retries.
+hipError_t scratch_alloc(void** p, size_t bytes, hipStream_t s);
+void scratch_free(void* p, size_t bytes, hipStream_t s);
+
 // A source of device blocks that are recycled by their owner rather than
 // freed through the driver (the serving loop's block pool).
 struct Recycler {
@@ -31,13 +43,13 @@ protected:
 // owner does only once the stream's work on them has completed.
 struct Scratch {
     hipStream_t s;
-    std::vector<void*> ptrs;
+    std::vector<std::pair<void*, size_t>> ptrs;
     Recycler* rec = nullptr;
     explicit Scratch(hipStream_t st, Recycler* r = nullptr) : s(st), rec(r) {}
     ~Scratch() {
-        for (void* p : ptrs) {
-            if (rec) rec->put(p);
-            else (void)hipFreeAsync(p, s);
+        for (auto& pb : ptrs) {
+            if (rec) rec->put(pb.first);
+            else scratch_free(pb.first, pb.second, s);
         }
     }
     template <class T>
@@ -49,15 +61,16 @@ struct Scratch {
             *p = (T*)rec->get(sizeof(T) * count);
             e = *p ? hipSuccess : hipErrorOutOfMemory;
         } else {
-            e = hipMallocAsync((void**)p, sizeof(T) * count, s);
+            e = scratch_alloc((void**)p, sizeof(T) * count, s);
         }
-        if (e == hipSuccess) ptrs.push_back(*p);
+        if (e == hipSuccess) ptrs.emplace_back(*p, sizeof(T) * count);
         return e;
     }
-    // Keep an allocation past this scope (the caller frees it later).
+    // Keep an allocation past this scope: the caller releases it later with
+    // scratch_free (same size) or hipFreeAsync.
     void keep(const void* p) {
         for (size_t i = 0; i < ptrs.size(); ++i)
-            if (ptrs[i] == p) {
+            if (ptrs[i].first == p) {
                 ptrs.erase(ptrs.begin() + (long)i);
                 return;
             }
@@ -97,6 +110,7 @@ struct LzLaunch {
     int variant = 0;
     int grid = 0;
     std::vector<MvTile> tiles;
+    std::vector<int32_t> qbeg;  // kLzQueues + 1 queue bounds in tiles
 };
 int plan_lz(const std::vector<MvInstance>& inst, std::vector<LzLaunch>& plan);
 int factor_and_split(const std::vector<int32_t>& dims, double* d_a, const std::vector<int64_t>& off,

@@ -35,7 +35,12 @@ hipError_t launch_split_l(const double* l64, const int64_t* off, const int32_t* 
                           int max_dp, hipStream_t s);
 // variant: 0 = (1 col block per wave, 256-row tiles), 1 = (2, 128-row), 2 = (4, 64-row),
 // 3 = (1, 64-row). Any dimension works with any variant (rows are tiled).
-hipError_t launch_lz_gemm(int variant, int grid, const MvTile* tiles, int32_t n_tiles, int32_t* counter,
+// The tiles come in kLzQueues queues, tiles[qbeg[q], qbeg[q+1]); workgroup b
+// serves queue b % kLzQueues first (the XCD it is dispatched to: workgroups
+// go to the 8 XCDs round robin), then takes what is left in the others.
+// counter: kLzQueues zeroed ints.
+constexpr int kLzQueues = 8;
+hipError_t launch_lz_gemm(int variant, int grid, const MvTile* tiles, const int32_t* qbeg, int32_t* counter,
                           const MvInstance* inst, hipStream_t s);
 
 struct SyrkTile {

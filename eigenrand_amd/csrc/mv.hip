@@ -191,6 +191,18 @@ __global__ void k_split_l(const double* __restrict__ l64, const int64_t* __restr
 // ------------------------------------------------------- MFMA L * Z ---
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// Four floats at a 4-byte-aligned address (column starts are d floats
+// apart, d arbitrary): one 16-byte global access.
+typedef float f32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+// Global-memory views of the instance pointers: global (not flat)
+// instructions, so outstanding loads do not hold up the LDS waits of the
+// MFMA loop.
+#define ERD_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ ERD_GLOBAL T* gmem(T* p) {
+    return (ERD_GLOBAL T*)p;
+}
 
 constexpr int KC = 32;       // k per chunk (one v_mfma_f32_16x16x32_bf16)
 constexpr int KPAD = 40;     // LDS row pitch in bf16 (80 B: keeps 16 B alignment)
@@ -202,7 +214,8 @@ constexpr int KPAD = 40;     // LDS row pitch in bf16 (80 B: keeps 16 B alignmen
 // in LDS; each wave issues hi*hi + hi*lo + lo*hi MFMAs, skipping row blocks
 // above the diagonal (L is lower triangular).
 template <int NCB, int MAXR>
-__global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tiles, int32_t n_tiles,
+__global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tiles,
+                                                 const int32_t* __restrict__ qbeg,
                                                  int32_t* __restrict__ counter,
                                                  const MvInstance* __restrict__ inst) {
     constexpr int NT = 64 * NCB;
@@ -211,12 +224,26 @@ __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tile
     __shared__ int tile_s;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int l16 = lane & 15, g = lane >> 4;
+    // thread 0: the queue it takes tiles from, and how many it found empty
+    int q = (int)(blockIdx.x % kLzQueues), emptied = 0;
     for (;;) {
-        if (tid == 0) tile_s = atomicAdd(counter, 1);
+        if (tid == 0) {
+            int t = -1;
+            while (emptied < kLzQueues) {
+                const int k = qbeg[q] + atomicAdd(counter + q, 1);
+                if (k < qbeg[q + 1]) {
+                    t = k;
+                    break;
+                }
+                q = q + 1 == kLzQueues ? 0 : q + 1;  // a queue found empty stays empty
+                ++emptied;
+            }
+            tile_s = t;
+        }
         __syncthreads();
         const int t = tile_s;
         __syncthreads();
-        if (t >= n_tiles) break;
+        if (t < 0) break;
         const MvTile tl = tiles[t];
         const MvInstance in = inst[tl.inst];
         const int d = in.d, dp = (d + 15) & ~15;
@@ -230,9 +257,10 @@ __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tile
         for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
             for (int rb = 0; rb < MAXR; ++rb) acc[cb][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
-        const uint16_t* Lh = in.lhi;
-        const uint16_t* Ll = in.llo;
-        const float* Z = in.z;
+        ERD_GLOBAL const uint16_t* Lh = gmem(in.lhi);
+        ERD_GLOBAL const uint16_t* Ll = gmem(in.llo);
+        ERD_GLOBAL const float* Z = gmem(in.z);
+        ERD_GLOBAL const float* mean = in.mean ? gmem(in.mean) : nullptr;
         // The next chunk's L and Z are loaded into registers while the
         // current chunk's MFMAs run; the registers go to LDS (Z split into
         // hi/lo) between the two barriers of a step.
@@ -251,8 +279,10 @@ __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tile
                 if (q < lq_n) {
                     const int r = q / (KC / 8), kk = (q % (KC / 8)) * 8;
                     if (k0 + kk < dp) {
-                        ph[i] = *reinterpret_cast<const uint4*>(Lh + (int64_t)(row0 + r) * dp + k0 + kk);
-                        pl[i] = *reinterpret_cast<const uint4*>(Ll + (int64_t)(row0 + r) * dp + k0 + kk);
+                        const u32x4 h = *reinterpret_cast<ERD_GLOBAL const u32x4*>(Lh + (int64_t)(row0 + r) * dp + k0 + kk);
+                        const u32x4 l = *reinterpret_cast<ERD_GLOBAL const u32x4*>(Ll + (int64_t)(row0 + r) * dp + k0 + kk);
+                        ph[i] = uint4{h[0], h[1], h[2], h[3]};
+                        pl[i] = uint4{l[0], l[1], l[2], l[3]};
                     }
                 }
             }
@@ -261,10 +291,18 @@ __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tile
                 const int q = tid + 256 * i;
                 const int c = q / (KC / 8), kk = (q % (KC / 8)) * 8;
                 const int64_t col = c0 + c;
+                if (col < n && k0 + kk + 8 <= d) {
+                    // 8 consecutive floats of column col: two 16-byte loads
+                    const f32x4_a4 a = *reinterpret_cast<ERD_GLOBAL const f32x4_a4*>(Z + col * d + k0 + kk);
+                    const f32x4_a4 b = *reinterpret_cast<ERD_GLOBAL const f32x4_a4*>(Z + col * d + k0 + kk + 4);
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int k = k0 + kk + e;
-                    pz[i][e] = (k < d && col < n) ? Z[col * d + k] : 0.0f;
+                    for (int e = 0; e < 4; ++e) pz[i][e] = a[e], pz[i][4 + e] = b[e];
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const int k = k0 + kk + e;
+                        pz[i][e] = (k < d && col < n) ? Z[col * d + k] : 0.0f;
+                    }
                 }
             }
         };
@@ -321,7 +359,7 @@ __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tile
             __syncthreads();
         }
         // epilogue: X[i][j] = acc + mean[i]; lane holds rows 4g..4g+3 of column l16
-        float* X = in.x;
+        ERD_GLOBAL float* X = gmem(in.x);
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) {
             const int64_t col = c0 + (wave * NCB + cb) * 16 + l16;
@@ -329,12 +367,21 @@ __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tile
 #pragma unroll
             for (int rb = 0; rb < MAXR; ++rb) {
                 if (rb >= R) continue;
+                const int rowb = row0 + rb * 16 + 4 * g;
+                if (rowb + 4 <= d) {
+                    // rows rowb..rowb+3 of column col: one 16-byte store
+                    f32x4_a4 v;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = row0 + rb * 16 + 4 * g + r;
-                    if (row < d) {
-                        const float mu = in.mean ? in.mean[row] : 0.0f;
-                        X[col * d + row] = acc[cb][rb][r] + mu;
+                    for (int r = 0; r < 4; ++r) v[r] = acc[cb][rb][r] + (mean ? mean[rowb + r] : 0.0f);
+                    *reinterpret_cast<ERD_GLOBAL f32x4_a4*>(X + col * d + rowb) = v;
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = rowb + r;
+                        if (row < d) {
+                            const float mu = mean ? mean[row] : 0.0f;
+                            X[col * d + row] = acc[cb][rb][r] + mu;
+                        }
                     }
                 }
             }
@@ -342,13 +389,14 @@ __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tile
     }
 }
 
-hipError_t launch_lz_gemm(int variant, int grid, const MvTile* tiles, int32_t n_tiles, int32_t* counter,
+hipError_t launch_lz_gemm(int variant, int grid, const MvTile* tiles, const int32_t* qbeg, int32_t* counter,
                           const MvInstance* inst, hipStream_t s) {
+    if (grid <= 0) return hipSuccess;
     switch (variant) {
-        case 0: hipLaunchKernelGGL((k_lz_gemm<1, 16>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter, inst); break;
-        case 1: hipLaunchKernelGGL((k_lz_gemm<2, 8>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter, inst); break;
-        case 2: hipLaunchKernelGGL((k_lz_gemm<4, 4>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter, inst); break;
-        default: hipLaunchKernelGGL((k_lz_gemm<1, 4>), dim3(grid), dim3(256), 0, s, tiles, n_tiles, counter, inst); break;
+        case 0: hipLaunchKernelGGL((k_lz_gemm<1, 16>), dim3(grid), dim3(256), 0, s, tiles, qbeg, counter, inst); break;
+        case 1: hipLaunchKernelGGL((k_lz_gemm<2, 8>), dim3(grid), dim3(256), 0, s, tiles, qbeg, counter, inst); break;
+        case 2: hipLaunchKernelGGL((k_lz_gemm<4, 4>), dim3(grid), dim3(256), 0, s, tiles, qbeg, counter, inst); break;
+        default: hipLaunchKernelGGL((k_lz_gemm<1, 4>), dim3(grid), dim3(256), 0, s, tiles, qbeg, counter, inst); break;
     }
     return hipGetLastError();
 }

@@ -52,22 +52,37 @@ int forced_variant() {
 
 int bucket_of(int d) { return ((d + 15) & ~15) <= 64 ? 0 : ((d + 15) & ~15) <= 128 ? 1 : ((d + 15) & ~15) <= 256 ? 2 : 3; }
 
+// Tile queues of one L*Z launch. Each instance goes whole to one of the
+// kLzQueues queues (largest work first, to the least loaded queue), so the
+// workgroups of one XCD read its L planes and Z columns through that XCD's
+// L2 only. In a queue: instances largest first (the tail of the launch is
+// short), and per instance the row tiles of a column block one after the
+// other (they read the same Z columns).
 void build_tiles(const std::vector<MvInstance>& inst, const std::vector<int>& which, int variant,
-                 std::vector<MvTile>& tiles) {
+                 std::vector<MvTile>& tiles, std::vector<int32_t>& qbeg) {
     const int rows = lz_rows(variant), cols = lz_cols(variant);
-    // Largest instances first (longest k loops), so the tail of the launch is short.
     std::vector<int> order(which);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return inst[a].d > inst[b].d; });
-    size_t total = tiles.size();
+    auto work = [&](int i) {
+        const int64_t dp = (inst[i].d + 15) & ~15;
+        return dp * dp * inst[i].n;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+    std::vector<std::vector<int>> queue(kLzQueues);
+    std::vector<int64_t> load(kLzQueues, 0);
     for (int i : order) {
-        const int dp = (inst[i].d + 15) & ~15;
-        total += (size_t)((dp + rows - 1) / rows) * (size_t)((inst[i].n + cols - 1) / cols);
+        const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        queue[q].push_back(i);
+        load[q] += work(i);
     }
-    tiles.reserve(total);
-    for (int i : order) {
-        const int dp = (inst[i].d + 15) & ~15;
-        for (int r = 0; r < dp; r += rows)
-            for (int64_t c = 0; c < inst[i].n; c += cols) tiles.push_back(MvTile{i, r, c});
+    tiles.clear();
+    qbeg.assign(1, 0);
+    for (int q = 0; q < kLzQueues; ++q) {
+        for (int i : queue[q]) {
+            const int dp = (inst[i].d + 15) & ~15;
+            for (int64_t c = 0; c < inst[i].n; c += cols)
+                for (int r = 0; r < dp; r += rows) tiles.push_back(MvTile{i, r, c});
+        }
+        qbeg.push_back((int32_t)tiles.size());
     }
 }
 
@@ -104,7 +119,7 @@ int plan_lz(const std::vector<MvInstance>& inst, std::vector<LzLaunch>& plan) {
     for (auto& kv : by_variant) {
         LzLaunch l;
         l.variant = kv.first;
-        build_tiles(inst, kv.second, kv.first, l.tiles);
+        build_tiles(inst, kv.second, kv.first, l.tiles, l.qbeg);
         l.grid = grid_for(kv.first, l.tiles.size());
         plan.push_back(std::move(l));
     }
@@ -131,19 +146,21 @@ int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t 
             int best_v = default_variant(kv.first == 0 ? 64 : kv.first == 1 ? 128 : 256);
             for (int cand = 0; cand < kLzVariants; ++cand) {
                 std::vector<MvTile> tiles;
-                build_tiles(inst, kv.second, cand, tiles);
+                std::vector<int32_t> qbeg;
+                build_tiles(inst, kv.second, cand, tiles, qbeg);
                 MvTile* d_tiles;
-                int32_t* d_cnt;
-                if (sc.upload(&d_tiles, tiles) != hipSuccess || sc.alloc(&d_cnt, 1) != hipSuccess) return ERD_ERR_HIP;
+                int32_t *d_qbeg, *d_cnt;
+                if (sc.upload(&d_tiles, tiles) != hipSuccess || sc.upload(&d_qbeg, qbeg) != hipSuccess ||
+                    sc.alloc(&d_cnt, kLzQueues) != hipSuccess)
+                    return ERD_ERR_HIP;
                 hipEvent_t a = nullptr, b = nullptr;
                 float ms = 0;
                 hipError_t e = hipEventCreate(&a);
                 if (e == hipSuccess) e = hipEventCreate(&b);
-                if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, sizeof(int32_t), s);
+                if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, sizeof(int32_t) * kLzQueues, s);
                 if (e == hipSuccess) e = hipEventRecord(a, s);
                 if (e == hipSuccess)
-                    e = launch_lz_gemm(cand, grid_for(cand, tiles.size()), d_tiles, (int32_t)tiles.size(), d_cnt,
-                                       d_inst, s);
+                    e = launch_lz_gemm(cand, grid_for(cand, tiles.size()), d_tiles, d_qbeg, d_cnt, d_inst, s);
                 if (e == hipSuccess) e = hipEventRecord(b, s);
                 if (e == hipSuccess) e = hipEventSynchronize(b);
                 if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
@@ -165,13 +182,15 @@ int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t 
     }
     for (auto& kv : by_variant) {
         std::vector<MvTile> tiles;
-        build_tiles(inst, kv.second, kv.first, tiles);
+        std::vector<int32_t> qbeg;
+        build_tiles(inst, kv.second, kv.first, tiles, qbeg);
         MvTile* d_tiles;
-        int32_t* d_cnt;
-        if (sc.upload(&d_tiles, tiles) != hipSuccess || sc.alloc(&d_cnt, 1) != hipSuccess) return ERD_ERR_HIP;
-        if (hipMemsetAsync(d_cnt, 0, sizeof(int32_t), s) != hipSuccess) return ERD_ERR_HIP;
-        hipError_t e = launch_lz_gemm(kv.first, grid_for(kv.first, tiles.size()), d_tiles, (int32_t)tiles.size(),
-                                      d_cnt, d_inst, s);
+        int32_t *d_qbeg, *d_cnt;
+        if (sc.upload(&d_tiles, tiles) != hipSuccess || sc.upload(&d_qbeg, qbeg) != hipSuccess ||
+            sc.alloc(&d_cnt, kLzQueues) != hipSuccess)
+            return ERD_ERR_HIP;
+        if (hipMemsetAsync(d_cnt, 0, sizeof(int32_t) * kLzQueues, s) != hipSuccess) return ERD_ERR_HIP;
+        hipError_t e = launch_lz_gemm(kv.first, grid_for(kv.first, tiles.size()), d_tiles, d_qbeg, d_cnt, d_inst, s);
         if (e != hipSuccess) return ERD_ERR_HIP;
     }
     return ERD_OK;
@@ -219,11 +238,11 @@ int factor_and_split(const std::vector<int32_t>& dims, double* d_a, const std::v
 // Stream-ordered release of MvNormal/Wishart factor planes after the work
 // on `s`.
 int release_factors(erd_mv_factors* f, hipStream_t s) {
-    hipError_t e = hipSuccess;
-    if (f->lhi) e = hipFreeAsync(f->lhi, s);
-    if (f->llo && e == hipSuccess) e = hipFreeAsync(f->llo, s);
+    const size_t bytes = sizeof(uint16_t) * (size_t)(f->split_off.empty() ? 0 : f->split_off.back());
+    scratch_free(f->lhi, bytes, s);
+    scratch_free(f->llo, bytes, s);
     delete f;
-    return e == hipSuccess ? ERD_OK : ERD_ERR_HIP;
+    return ERD_OK;
 }
 
 // StdNormalGen fills of one engine each, a workgroup per engine (the

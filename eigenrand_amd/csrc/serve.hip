@@ -528,12 +528,13 @@ int Server::prepare_and_launch(Batch* b) {
     const size_t off_eng = st.put(engine_list);
     const size_t off_begin = st.put(begin);
     const size_t off_mv = st.reserve(sizeof(MvInstance) * mv.size());
-    std::vector<size_t> off_tiles, off_cnt;
+    std::vector<size_t> off_tiles, off_qbeg, off_cnt;
     if (staged_lz) {
-        const int32_t zero = 0;
+        const int32_t zero[kLzQueues] = {};
         for (const LzLaunch& l : plan) {
             off_tiles.push_back(st.put(l.tiles));
-            off_cnt.push_back(st.put(&zero, sizeof(zero)));
+            off_qbeg.push_back(st.put(l.qbeg));
+            off_cnt.push_back(st.put(zero, sizeof(zero)));
         }
     }
     char* dev;
@@ -576,7 +577,7 @@ int Server::prepare_and_launch(Batch* b) {
         if (staged_lz) {
             for (size_t k = 0; k < plan.size(); ++k)
                 if (launch_lz_gemm(plan[k].variant, plan[k].grid, (const MvTile*)(dev + off_tiles[k]),
-                                   (int32_t)plan[k].tiles.size(), (int32_t*)(dev + off_cnt[k]), d_inst,
+                                   (const int32_t*)(dev + off_qbeg[k]), (int32_t*)(dev + off_cnt[k]), d_inst,
                                    s) != hipSuccess)
                     return ERD_ERR_HIP;
         } else {

@@ -124,6 +124,111 @@ std::map<int, PinnedRing*> g_rings;  // per device; kept until the process ends
 
 }  // namespace
 
+namespace {
+
+size_t size_class(size_t b) {
+    if (b <= 4096) return 4096;
+    size_t base = 4096;
+    while (base * 2 < b) base *= 2;  // base < b <= 2 * base
+    const size_t step = base / 4;
+    return base + (b - base + step - 1) / step * step;
+}
+
+class ScratchCache {
+public:
+    hipError_t get(void** p, size_t bytes, hipStream_t s) {
+        const size_t cls = size_class(bytes);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            for (size_t i = 0; i < free_.size(); ++i) {
+                Blk& b = free_[i];
+                if (b.cls != cls) continue;
+                if (b.s != s && hipEventQuery(b.ev) != hipSuccess) continue;
+                *p = b.p;
+                events_.push_back(b.ev);
+                free_[i] = free_.back();
+                free_.pop_back();
+                return hipSuccess;
+            }
+        }
+        hipError_t e = hipMallocAsync(p, cls, s);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            release_idle();
+            e = hipMallocAsync(p, cls, s);
+        }
+        return e;
+    }
+
+    void put(void* p, size_t bytes, hipStream_t s) {
+        std::lock_guard<std::mutex> lk(mu_);
+        hipEvent_t ev = nullptr;
+        if (!events_.empty()) {
+            ev = events_.back();
+            events_.pop_back();
+        } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            ev = nullptr;
+        }
+        if (!ev || hipEventRecord(ev, s) != hipSuccess) {
+            // cannot track its stream position: give it back to the driver
+            if (ev) events_.push_back(ev);
+            (void)hipFreeAsync(p, s);
+            return;
+        }
+        free_.push_back(Blk{p, size_class(bytes), s, ev});
+    }
+
+private:
+    struct Blk {
+        void* p;
+        size_t cls;
+        hipStream_t s;
+        hipEvent_t ev;
+    };
+
+    // Frees every cached block (after its last use has completed).
+    void release_idle() {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (Blk& b : free_) {
+            (void)hipEventSynchronize(b.ev);
+            (void)hipFree(b.p);
+            events_.push_back(b.ev);
+        }
+        free_.clear();
+    }
+
+    std::mutex mu_;
+    std::vector<Blk> free_;
+    std::vector<hipEvent_t> events_;
+};
+
+std::mutex g_cache_mu;
+std::map<int, ScratchCache*> g_caches;  // per device; kept until the process ends
+
+ScratchCache* cache_of_current_device() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    ScratchCache*& c = g_caches[dev];
+    if (!c) c = new ScratchCache();
+    return c;
+}
+
+}  // namespace
+
+hipError_t scratch_alloc(void** p, size_t bytes, hipStream_t s) {
+    ScratchCache* c = cache_of_current_device();
+    if (!c) return hipErrorInvalidDevice;
+    return c->get(p, bytes, s);
+}
+
+void scratch_free(void* p, size_t bytes, hipStream_t s) {
+    if (!p) return;
+    ScratchCache* c = cache_of_current_device();
+    if (c) c->put(p, bytes, s);
+    else (void)hipFreeAsync(p, s);
+}
+
 hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t s) {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
