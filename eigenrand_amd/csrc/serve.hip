@@ -142,6 +142,7 @@ struct Req {
     int status = ERD_OK;
     int slot = -1;
     bool seed_first = false;    // first request of its client: seeds the engines
+    int wclass = 0;             // latency class: batches hold requests of one class
 };
 
 struct Batch {
@@ -160,6 +161,15 @@ void spread(const erd_serve_request& r, int E, int width, double work, Req* q) {
     const int64_t chunk = std::max<int64_t>(chunk_for(r, work), width);  // at least one packet
     q->e_req = (int)std::min<int64_t>(E, (r.count + chunk - 1) / chunk);
     q->per = round_up((r.count + q->e_req - 1) / q->e_req, width);
+    // Time on one engine, in units of `work` (engines run in parallel, so
+    // this is the request's time in a launch): up to 1 when the request has
+    // enough engines; more when it needs more than E of them, or when one
+    // output alone costs more than `work` (large multinomial rows). Class
+    // 0 for up to 1, then one class per factor of 4.
+    const double t = (double)q->per / (double)std::max<int64_t>(chunk_for(r, work), 1);
+    int c = 0;
+    for (double u = 1.0; t > u && c < 16; u *= 4.0) ++c;
+    q->wclass = c;
 }
 
 // Device blocks for request outputs and batch scratch, recycled by the server
@@ -606,6 +616,10 @@ void Server::launcher_loop() {
                 if (free_idx >= 0 && !pending.empty()) {
                     // Oldest admissible requests; a client blocked once in this
                     // pass stays blocked so that its order is kept.
+                    // A batch holds requests of one latency class (that of its
+                    // oldest admissible request): a request completes with
+                    // its batch, so a short request must not wait for a long
+                    // one; the other classes go to the next launch.
                     std::vector<char> blocked(slot_client.size() + 1, 0);
                     std::vector<Req*> take;
                     int64_t take_elems = 0;
@@ -613,10 +627,15 @@ void Server::launcher_loop() {
                     for (auto it = pending.begin(); it != pending.end();) {
                         Req* q = *it;
                         if ((int64_t)take.size() >= cfg.max_batch_requests) break;
+                        const int sl = q->slot;
+                        if (!take.empty() && q->wclass != take.front()->wclass) {
+                            blocked[sl] = 1;  // the client's later requests stay behind it
+                            ++it;
+                            continue;
+                        }
                         if (!take.empty() && cfg.max_batch_elements > 0 &&
                             take_elems + q->out_elems > cfg.max_batch_elements)
                             break;
-                        const int sl = q->slot;
                         bool ok = !blocked[sl] && budget_used + (int64_t)q->out_bytes <= cfg.hbm_budget_bytes;
                         if (ok) {
                             q->seed_first = !slot_seeded[sl];
