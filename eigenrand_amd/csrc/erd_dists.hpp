@@ -584,6 +584,29 @@ struct GammaIntF32 {
     }
 };
 
+// GammaIntF32 with alpha given at run time (for op_fixed_rt): the same
+// product of alpha nonzero uniforms in the same order.
+struct GammaIntRtF32 {
+    float* out;
+    float beta;
+    int alpha;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 8; }
+    template <class F>
+    __device__ void emit(int64_t per, int64_t u, int l, F&& fetch) const {
+        float v[2];
+        const uint64_t w0 = fetch(0);
+        float y[2] = {nzur_f32(lo32(w0)), nzur_f32(hi32(w0))};
+        for (int c = 1; c < alpha; ++c) {
+            const uint64_t w = fetch(c);
+            y[0] = y[0] * nzur_f32(lo32(w));
+            y[1] = y[1] * nzur_f32(hi32(w));
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) v[h] = -(beta * plog(y[h]));
+        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
+    }
+};
 
 // BetaGen's gamma-pair regime (NormalExp.h:970-975) when both shapes are
 // integers in [1, 20): GammaGen<float>(a, 1) and (b, 1) then take a fixed

@@ -37,6 +37,19 @@ struct FillJob {
 };
 constexpr int kJobThreads = 256;
 hipError_t launch_std_normal_jobs(EngineState* st, const FillJob* jobs, int n_jobs, hipStream_t s);
+// Consecutive GammaGen fills (fresh generators, scale 1) on one engine: `rows`
+// rows of `count` gammas, row k of shape alpha[alpha_off + k] at
+// out + k * count (a Dirichlet instance's category rows).
+struct GammaRowGroup {
+    int32_t engine;
+    int32_t rows;
+    int64_t count;
+    float* out;
+    int64_t alpha_off;
+};
+// One workgroup of four waves per group.
+hipError_t launch_gamma_rows(EngineState* st, const GammaRowGroup* rows, int32_t n_groups, const float* alpha,
+                             Guard gd, hipStream_t s);
 hipError_t launch_fill_u64(EngineState* st, int64_t n, uint64_t* out, int64_t per, hipStream_t s);
 // P8 adaptor fills: n Vmt engines (n/2 adaptors), per_v outputs per Vmt
 // engine; mode 0 u64, 1 u32, 2 uniform_real() floats.

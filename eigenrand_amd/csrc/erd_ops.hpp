@@ -385,6 +385,46 @@ __device__ void op_fixed_rt(Eng& e, uint64_t* carry, int64_t per, int C, const D
     e.idx = pos;
 }
 
+// op_fixed_rt for one engine served by a workgroup of NT threads (as
+// op_fixed_wg: the units of a block are shared by all threads, wave 0 twists
+// and keeps the carried words); same items, outputs and final state.
+template <int NT, int MAXC, class D>
+__device__ void op_fixed_rt_wg(Eng& e, uint64_t* carry, int64_t per, int C, const D& d, int tid) {
+    const int64_t units = D::units(per);
+    if (units == 0) return;
+    const int wl = tid < 64 ? tid : (1 << 20);  // carry_tail lane: only wave 0 moves words
+    int pos = e.idx;
+    int nc = 0;
+    int64_t done = 0;
+    for (;;) {
+        if (pos >= kN) {
+            twist_lds_wg(e.mt, tid);
+            pos = 0;
+            e.twisted = true;
+        }
+        const int avail = nc + (kN - pos);
+        const int64_t can = (int64_t)(avail / C);
+        const int nu = (int)(units - done < can ? units - done : can);
+        for (int t = tid; t < nu * kLanes; t += NT) {
+            const int u = t >> 2, l = t & 3;
+            d.emit(per, done + u, l, [&](int c) { return slot_word<kLanes>(e.mt, carry, nc, pos, u * C + c, l); });
+        }
+        const int consumed = nu * C;
+        done += nu;
+        if (done == units) {
+            pos = pos + consumed - nc;
+            break;
+        }
+        // carry_tail's barriers also end every wave's reads of this block
+        // before the next twist
+        carry_tail<kLanes, MAXC>(e.mt, carry, nc, pos, consumed, avail, wl);
+        nc = avail - consumed;
+        pos = kN;
+    }
+    __syncthreads();
+    e.idx = pos;
+}
+
 // ------------------------------------------------------------- half state ---
 // Position of a half-draw consumer: block slot `pos` (in half draws) plus an
 // optional carried (already tempered) slot holding the engine's cached half.
@@ -614,6 +654,190 @@ __device__ void op_reject(Eng& e, RejectLds<P>& s, typename P::Out* o, int64_t p
     } else {
         half_end(e, bpos, rem, used);
     }
+}
+
+// op_reject for one engine served by NW waves (a workgroup of 64 * NW
+// threads, full-draw policies, a fresh generator): a step evaluates NW * CPS
+// consecutive candidates, wave w the w-th run of CPS, and each wave places
+// its queue items and bypassing packets after those of the waves before it,
+// so the outputs, their order and the final engine position are op_reject's.
+// The serial chain of one engine's rejection fill (a Dirichlet instance's
+// category rows) then runs at NW waves' rate. Wave 0 twists and carries.
+template <class P, int NW>
+struct RejectWgLds {
+    uint64_t carry[(P::K - 1) * P::SW];
+    float rest[P::W];
+    int32_t tab[NW * (64 / (P::W / 2)) + 1];
+    int32_t wtot[NW], wfc[NW], wlast[NW];
+};
+
+template <class P, int NW>
+__device__ void op_reject_wg(Eng& e, RejectWgLds<P, NW>& s, typename P::Out* o, int64_t per, const P& pol,
+                             Guard gd, int tid) {
+    static_assert(P::SW == 4, "full-draw policies only");
+    constexpr int W = P::W, SW = P::SW, K = P::K;
+    constexpr int IPC = W / 2;         // threads per candidate (two lanes each)
+    constexpr int CPS = 64 / IPC;      // candidates per wave step
+    constexpr int CPSW = NW * CPS;     // candidates per workgroup step
+    constexpr int SLOTS = SlotMode<SW>::kSlots;
+    constexpr int MAXC = K - 1;
+    const int64_t T = per / W;
+    if (T == 0) return;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wl = tid < 64 ? tid : (1 << 20);  // carry_tail lane: only wave 0 moves words
+    uint64_t* carry = s.carry;
+    int pos = e.idx, nc = 0;
+    int64_t full_cnt = 0, fifo = 0, since_emit = 0;
+    int end_v = -1;
+    const int cl = lane / IPC, p = lane % IPC;
+    constexpr int kFirst = IPC == 4 ? 0x00 : 0xA0;  // quad_perm 0,0,0,0 / 0,0,2,2
+    for (;;) {
+        if (pos >= SLOTS) {
+            twist_lds_wg(e.mt, tid);
+            pos = 0;
+            e.twisted = true;
+        }
+        const int avail = nc + (SLOTS - pos);
+        const int ncand = avail / K;
+        for (int c0 = 0; c0 < ncand; c0 += CPSW) {
+            const int c = c0 + wave * CPS + cl;
+            const bool live = c < ncand;
+            float v[2] = {0.f, 0.f};
+            bool a[2] = {false, false};
+            if (live) {
+                uint64_t w[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) w[k] = slot_word<SW>(e.mt, carry, nc, pos, c * K + k, p);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) pol.eval(w, h, v[h], a[h], 2 * p + h);
+            }
+            const int64_t m0 = fifo / W;
+            const int r0 = (int)(fifo - m0 * W);
+            const int64_t pbase = full_cnt + m0;
+            const int64_t prem = T - pbase;  // local packets < prem exist
+            const int lim = prem < 0x40000000 ? (int)prem : 0x40000000;
+            const int cb = cl * IPC;
+            const uint64_t cmask = (IPC == 4 ? 0xFull : 0x3ull) << cb;
+            // this wave's counts, as op_reject's plan() computes them
+            bool full, q0, q1;
+            uint64_t FC;
+            int f0l, fbl, acnt, tot_w;
+            auto count = [&](bool lv) {
+                const uint64_t B0 = __ballot(lv && a[0]), B1 = __ballot(lv && a[1]);
+                full = lv && ((B0 & B1 & cmask) == cmask);
+                q0 = lv && a[0] && !full;
+                q1 = lv && a[1] && !full;
+                const uint64_t A0 = __ballot(q0), A1 = __ballot(q1);
+                FC = __ballot(full && p == 0);
+                f0l = (int)mbcnt64(A1, mbcnt64(A0, 0));
+                fbl = (int)mbcnt64(FC, 0);
+                tot_w = __popcll(A0) + __popcll(A1);
+                acnt = (q0 ? 1 : 0) + (q1 ? 1 : 0);  // summed over the candidate's threads
+                acnt += __builtin_amdgcn_mov_dpp(acnt, 0xB1, 0xF, 0xF, false);       // quad_perm 1,0,3,2
+                if (IPC == 4) acnt += __builtin_amdgcn_mov_dpp(acnt, 0x4E, 0xF, 0xF, false);  // 2,3,0,1
+            };
+            count(live);
+            if (lane == 0) {
+                s.wtot[wave] = tot_w;
+                s.wfc[wave] = __popcll(FC);
+            }
+            __syncthreads();
+            int tb = 0, fcb = 0;  // queue items and bypassing packets of the waves before this one
+            for (int w2 = 0; w2 < wave; ++w2) {
+                tb += s.wtot[w2];
+                fcb += s.wfc[w2];
+            }
+            int f0, fstart, emit;
+            auto place = [&](bool lv) {
+                f0 = r0 + tb + f0l;
+                fstart = __builtin_amdgcn_mov_dpp(f0, kFirst, 0xF, 0xF, false);
+                const int fb = __builtin_amdgcn_mov_dpp(fcb + fbl, kFirst, 0xF, 0xF, false);
+                emit = -1;
+                if (lv && (full || (acnt > 0 && (unsigned)(fstart + acnt) / W > (unsigned)fstart / W)))
+                    emit = fb + (int)((unsigned)fstart / W);
+            };
+            place(live);
+            if (!full && emit >= 0 && emit < lim && p == 0) s.tab[(unsigned)fstart / W] = (int32_t)(pbase + emit);
+            const uint64_t LAST = __ballot(emit == lim - 1 && p == 0);
+            const int clast = LAST ? (int)(__ffsll((long long)LAST) - 1) / IPC : -1;
+            if (lane == 0) s.wlast[wave] = clast + 1;
+            __syncthreads();
+            int lw = NW;  // the wave whose candidate emits the last packet (NW: none)
+            for (int w2 = NW - 1; w2 >= 0; --w2)
+                if (s.wlast[w2] > 0) lw = w2;
+            const int lw_c = lw < NW ? s.wlast[lw] : 0;  // candidates of wave lw still drawn
+            if (lw < NW) {
+                // Candidates after the one emitting the last packet are never
+                // drawn: drop them so that the queue ends as the reference's.
+                const bool lv = live && (wave < lw || (wave == lw && cl < lw_c));
+                count(lv);
+                place(lv);
+                if (lane == 0) {
+                    s.wtot[wave] = tot_w;
+                    s.wfc[wave] = __popcll(FC);
+                }
+                __syncthreads();
+            }
+            int tot = 0, nfull = 0;
+            for (int w2 = 0; w2 < NW; ++w2) {
+                tot += s.wtot[w2];
+                nfull += s.wfc[w2];
+            }
+            const int m_end = (r0 + tot) / W;  // queued packets completed in this step
+            if (full && emit < lim) {
+                const int64_t at = (pbase + emit) * W + 2 * p;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) pol.store(o, at + h, v[h]);
+            }
+            if (m_end > 0 && tid < r0) {
+                const int64_t ex = s.tab[0];
+                if (ex < T) pol.store(o, ex * W + tid, s.rest[tid]);
+            }
+            float park[2] = {0.f, 0.f};
+            int park_at[2] = {-1, -1};
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const bool q = h ? q1 : q0;
+                const unsigned f = (unsigned)(f0 + (h ? (q0 ? 1 : 0) : 0));
+                if (!q) continue;
+                const int m = (int)(f / W);
+                if (m < m_end) {
+                    const int64_t ex = s.tab[m];
+                    if (ex < T) pol.store(o, ex * W + (f % W), v[h]);
+                } else {
+                    park[h] = v[h];
+                    park_at[h] = (int)(f % W);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (park_at[h] >= 0) s.rest[park_at[h]] = park[h];
+            full_cnt += nfull;
+            fifo += tot;
+            if (lw < NW) {
+                end_v = (c0 + lw * CPS + lw_c) * K;
+                break;
+            }
+            if (nfull + m_end > 0) {
+                since_emit = 0;
+            } else {
+                since_emit += CPSW;
+                if (since_emit > gd.reject_cap) break;
+            }
+        }
+        if (end_v >= 0) break;
+        if (since_emit > gd.reject_cap) {
+            if (tid == 0) atomicOr(gd.status, kStatusIterCap);
+            end_v = ncand * K;
+            break;
+        }
+        carry_tail<SW, MAXC>(e.mt, carry, nc, pos, ncand * K, avail, wl);
+        nc = avail - ncand * K;
+        pos = SLOTS;
+    }
+    __syncthreads();
+    e.idx = end_v >= nc ? pos + end_v - nc : pos;
 }
 
 // -------------------------------------------------------------- lockstep ---

@@ -24,6 +24,7 @@
 #include "erd_dists.hpp"
 #include "erd_kernels.hpp"
 #include "erd_ops.hpp"
+#include "erd_segment.hpp"
 
 namespace erd {
 
@@ -118,6 +119,68 @@ hipError_t launch_fill_binomial_reject_i32(EngineState* st, int64_t n, int32_t* 
     BinomialTangent pol{(float)bp.trials, (float)bp.sqrt_v, (float)bp.mean, (float)bp.log_small_p,
                         (float)bp.log_small_q, (float)bp.g1, bp.trials, bp.p == bp.small_p ? 0 : 1, lg};
     return launch_reject(st, n, out, per, pol, gd, gs, s);
+}
+
+// ------------------------------------------- gamma rows on one engine ---
+// GammaGen fills (fresh generators) that run one after another on one engine
+// (the category rows of a Dirichlet instance), served by a workgroup of
+// kRowWaves waves: op_reject_wg for the rejection regimes, op_fixed_wg and
+// op_fixed_rt_wg for alpha = 1 and integer alpha < 20. The outputs and the
+// final engine state are those of the same fills on one wave (run_gamma of
+// the segment program); a large-K instance's serial chain runs about twice
+// as fast, and the kernel keeps only the gamma paths, so it has no spills.
+constexpr int kRowWaves = 4;
+constexpr int kRowThreads = 64 * kRowWaves;
+
+union GammaRowsLds {
+    RejectWgLds<GammaGS, kRowWaves> gs;
+    RejectWgLds<GammaTangent, kRowWaves> gt;
+    uint64_t carry_fixed[kLanes * 18];  // up to 19 draws per unit
+};
+
+__device__ void run_gamma_rows(Eng& e, GammaRowsLds& s, float* out, int64_t n, float alpha, float beta, Guard gd,
+                               int tid) {
+    if (alpha < 1.0f) {
+        GammaGS pol{beta, (float)(kE / (alpha + kE)), 1 / alpha, alpha - 1};
+        op_reject_wg<GammaGS, kRowWaves>(e, s.gs, out, n, pol, gd, tid);
+        return;
+    }
+    if (alpha == 1.0f) {
+        ExponentialF32 d{out, 1.0f, beta, 1};
+        op_fixed_wg<kRowThreads>(e, s.carry_fixed, n, d, tid);
+        return;
+    }
+    const int count = (int)alpha;
+    if ((float)count == alpha && count < 20) {
+        op_fixed_rt_wg<kRowThreads, 18>(e, s.carry_fixed, n, count, GammaIntRtF32{out, beta, count}, tid);
+        return;
+    }
+    GammaTangent pol{beta, sqrtf(2 * alpha - 1), alpha - 1};
+    op_reject_wg<GammaTangent, kRowWaves>(e, s.gt, out, n, pol, gd, tid);
+}
+
+// Workgroup b runs the rows of group rows[b] on its engine: row k is
+// `count` gammas of shape alpha[alpha_off + k] and scale 1 at out + k * count.
+__global__ __launch_bounds__(kRowThreads) void k_gamma_rows(EngineState* __restrict__ states,
+                                                            const GammaRowGroup* __restrict__ rows,
+                                                            const float* __restrict__ alpha, Guard gd) {
+    __shared__ uint64_t mt[kWords];
+    __shared__ GammaRowsLds s;
+    const int tid = threadIdx.x;
+    const GammaRowGroup gr = rows[blockIdx.x];
+    const int wl = tid < 64 ? tid : (1 << 20);  // state copies by wave 0
+    Eng e;
+    eng_load(e, mt, states + gr.engine, wl);
+    for (int k = 0; k < gr.rows; ++k)
+        run_gamma_rows(e, s, gr.out + (int64_t)k * gr.count, gr.count, alpha[gr.alpha_off + k], 1.0f, gd, tid);
+    eng_store(e, states + gr.engine, wl);
+}
+
+hipError_t launch_gamma_rows(EngineState* st, const GammaRowGroup* rows, int32_t n_groups, const float* alpha,
+                             Guard gd, hipStream_t s) {
+    if (n_groups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gamma_rows, dim3((unsigned)n_groups), dim3(kRowThreads), 0, s, st, rows, alpha, gd);
+    return hipGetLastError();
 }
 
 }  // namespace erd

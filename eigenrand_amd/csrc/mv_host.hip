@@ -547,30 +547,33 @@ int erd_dirichlet_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, con
         if (vd[i] <= 0 || vs[i] < 0 || vs[i] % 8 != 0) return ERD_ERR_INVALID;
         goff[i + 1] = goff[i] + (int64_t)vd[i] * vs[i];
         max_n = std::max(max_n, vs[i]);
+        aoff += vd[i];
     }
     Scratch sc(s);
     float* g;
     if (sc.alloc(&g, goff[n_inst]) != hipSuccess) return ERD_ERR_OOM;
-    // One GammaGen(alpha_k) fill of n per category, in order (Multinomial.h:190-199).
-    std::vector<Segment> segs;
-    std::vector<int32_t> engine_of;
+    // One GammaGen(alpha_k) fill of n per category, in order (Multinomial.h:190-199),
+    // instance i on engine i: a workgroup of four waves per instance (the
+    // serial chain of a large-K instance runs at four waves' rate), the
+    // longest chains first.
+    for (int64_t j = 0; j < aoff; ++j)
+        if (!(alpha[j] > 0.0f) || !std::isfinite(alpha[j])) return ERD_ERR_INVALID;
+    std::vector<GammaRowGroup> rows((size_t)n_inst);
+    int64_t a0 = 0;
     for (int64_t i = 0; i < n_inst; ++i) {
-        for (int k = 0; k < vd[i]; ++k) {
-            const float a = alpha[aoff + k];
-            if (!(a > 0.0f) || !std::isfinite(a)) return ERD_ERR_INVALID;
-            Segment sg{};
-            sg.kind = SEG_GAMMA_F32;
-            sg.count = vs[i];
-            sg.out = g + goff[i] + (int64_t)k * vs[i];
-            sg.d[0] = a;
-            sg.d[1] = 1.0;
-            segs.push_back(sg);
-            engine_of.push_back((int32_t)i);
-        }
-        aoff += vd[i];
+        rows[(size_t)i] = GammaRowGroup{(int32_t)i, vd[i], vs[i], g + goff[i], a0};
+        a0 += vd[i];
     }
-    int rc = run_segments(e, segs, engine_of, s);
-    if (rc) return rc;
+    std::stable_sort(rows.begin(), rows.end(), [](const GammaRowGroup& x, const GammaRowGroup& y) {
+        return (int64_t)x.rows * x.count > (int64_t)y.rows * y.count;
+    });
+    GammaRowGroup* d_rows;
+    float* d_alpha;
+    if (sc.upload(&d_rows, rows) || sc.alloc(&d_alpha, (size_t)aoff) ||
+        upload_async(d_alpha, alpha, sizeof(float) * (size_t)aoff, s))
+        return ERD_ERR_HIP;
+    if (launch_gamma_rows(e->states, d_rows, (int32_t)n_inst, d_alpha, guard_of(e), s) != hipSuccess)
+        return ERD_ERR_HIP;
     int32_t* d_dims;
     int64_t *d_samples, *d_goff;
     if (sc.upload(&d_dims, vd) || sc.upload(&d_samples, vs) || sc.upload(&d_goff, goff)) return ERD_ERR_HIP;

@@ -372,3 +372,34 @@ def test_discrete_d_fill_matches_oracle(er, orc, K):
     out = er.discrete(1000, 1, urng, w, precision="double")
     assert_same(out.reshape(len(seeds), 1000).cpu().numpy(), orc.fill_discrete_d_i32(ref, 1000, w))
     np.testing.assert_array_equal(urng.snapshot(), ref.raw)
+
+
+def test_dirichlet_gamma_rows_are_exact(er, orc):
+    """The category rows of a Dirichlet instance run as one serial chain of
+    GammaGen fills on the instance's engine, served by a four-wave workgroup
+    (every regime: GS for alpha < 1, the exponential for 1, the integer
+    products up to 19, tangent above, and 20 as tangent). The gamma rows must
+    be the oracle's bit for bit: with the float column sums taken in category
+    order, the normalised values are compared exactly, and the engine records
+    must equal the oracle's after the same draws."""
+    import orc_mv
+    rng = np.random.default_rng(21)
+    alphas = [np.float32([0.3, 1.0, 2.0, 0.05, 19.0, 20.0, 3.5, 1.0, 7.0, 0.9]),
+              np.exp(rng.uniform(np.log(0.01), np.log(10), 1000)).astype(np.float32),
+              np.exp(rng.uniform(np.log(0.01), np.log(10), 300)).astype(np.float32),
+              np.float32([4.0, 0.5])]
+    samples = [64, 1024, 40, 8]
+    seeds = [31, 32, 33, 34]
+    urng = er.Vmt19937_64(seeds=seeds)
+    print("dirichlet rows", [len(a) for a in alphas], samples, flush=True)
+    res = er.dirichlet_batch(urng, alphas, samples)
+    snap = urng.snapshot().reshape(len(seeds), -1)
+    for i, a in enumerate(alphas):
+        ref = orc.Engines(seeds=[seeds[i]])
+        _, g = orc_mv.dirichlet(ref, a, samples[i])
+        s = np.zeros(samples[i], np.float32)
+        for k in range(len(a)):
+            s = (s + g[k]).astype(np.float32)
+        want = (g / s).astype(np.float32)
+        assert_same(res[i].cpu().numpy(), want)
+        np.testing.assert_array_equal(snap[i], ref.raw)
