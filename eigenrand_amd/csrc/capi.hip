@@ -390,6 +390,12 @@ namespace {
 // The fills of GammaGen, PoissonGen and BinomialGen; gs (optional) holds the
 // CompressMask queues of a reused generator object (rejection regimes only:
 // the other regimes have no state).
+// Up to this many engines, a rejection-regime gamma fill serves each engine
+// with a workgroup of four waves (op_reject_wg) instead of one wave: with
+// fewer waves than SIMDs to spread them over, each engine's serial chain is
+// the bound.
+constexpr int64_t kGammaRowsMaxEngines = 1024;
+
 int fill_gamma(erd_engines* e, float* out, int64_t per, float alpha, float beta, erd::GenState* gs,
                void* stream) {
     int rc = check_fill(e, out, per, 8);
@@ -397,6 +403,20 @@ int fill_gamma(erd_engines* e, float* out, int64_t per, float alpha, float beta,
     if (!(alpha > 0.0f) || !std::isfinite(alpha)) return ERD_ERR_INVALID;
     if (alpha == 1.0f || ((float)(int)alpha == alpha && alpha < 20.0f))
         return hip_status(erd::launch_fill_gamma_fixed_f32(e->states, e->count, out, per, alpha, beta, S(stream)));
+    if (!gs && e->count <= kGammaRowsMaxEngines) {
+        // few engines: a four-wave workgroup per engine runs each serial fill
+        std::vector<erd::GammaRowGroup> rows((size_t)e->count);
+        for (int64_t g = 0; g < e->count; ++g)
+            rows[(size_t)g] = erd::GammaRowGroup{(int32_t)g, 1, per, out + g * per, 0, beta, 0};
+        erd::Scratch sc(S(stream));
+        erd::GammaRowGroup* d_rows;
+        float* d_alpha;
+        if (sc.upload(&d_rows, rows) || sc.alloc(&d_alpha, 1) ||
+            erd::upload_async(d_alpha, &alpha, sizeof(float), S(stream)))
+            return ERD_ERR_HIP;
+        return hip_status(erd::launch_gamma_rows(e->states, d_rows, (int32_t)e->count, d_alpha, guard_of(e),
+                                                 S(stream)));
+    }
     return hip_status(erd::launch_fill_gamma_reject_f32(e->states, e->count, out, per, alpha, beta,
                                                         guard_of(e), S(stream), gs));
 }

@@ -37,15 +37,17 @@ struct FillJob {
 };
 constexpr int kJobThreads = 256;
 hipError_t launch_std_normal_jobs(EngineState* st, const FillJob* jobs, int n_jobs, hipStream_t s);
-// Consecutive GammaGen fills (fresh generators, scale 1) on one engine: `rows`
-// rows of `count` gammas, row k of shape alpha[alpha_off + k] at
-// out + k * count (a Dirichlet instance's category rows).
+// Consecutive GammaGen fills (fresh generators) on one engine: `rows` rows
+// of `count` gammas, row k of shape alpha[alpha_off + k] and scale `beta` at
+// out + k * count (a Dirichlet instance's category rows, or one fill).
 struct GammaRowGroup {
     int32_t engine;
     int32_t rows;
     int64_t count;
     float* out;
     int64_t alpha_off;
+    float beta;
+    int32_t pad;
 };
 // One workgroup of four waves per group.
 hipError_t launch_gamma_rows(EngineState* st, const GammaRowGroup* rows, int32_t n_groups, const float* alpha,

@@ -172,7 +172,7 @@ __global__ __launch_bounds__(kRowThreads) void k_gamma_rows(EngineState* __restr
     Eng e;
     eng_load(e, mt, states + gr.engine, wl);
     for (int k = 0; k < gr.rows; ++k)
-        run_gamma_rows(e, s, gr.out + (int64_t)k * gr.count, gr.count, alpha[gr.alpha_off + k], 1.0f, gd, tid);
+        run_gamma_rows(e, s, gr.out + (int64_t)k * gr.count, gr.count, alpha[gr.alpha_off + k], gr.beta, gd, tid);
     eng_store(e, states + gr.engine, wl);
 }
 

@@ -561,7 +561,7 @@ int erd_dirichlet_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, con
     std::vector<GammaRowGroup> rows((size_t)n_inst);
     int64_t a0 = 0;
     for (int64_t i = 0; i < n_inst; ++i) {
-        rows[(size_t)i] = GammaRowGroup{(int32_t)i, vd[i], vs[i], g + goff[i], a0};
+        rows[(size_t)i] = GammaRowGroup{(int32_t)i, vd[i], vs[i], g + goff[i], a0, 1.0f, 0};
         a0 += vd[i];
     }
     std::stable_sort(rows.begin(), rows.end(), [](const GammaRowGroup& x, const GammaRowGroup& y) {

@@ -25,6 +25,22 @@ def test_gamma_reject_matches_oracle(er, orc, alpha, beta):
     assert urng.status() == 0
 
 
+@pytest.mark.parametrize("alpha,beta", [(0.5, 1.0), (3.7, 2.0)])
+def test_gamma_reject_one_wave_per_engine_matches_oracle(er, orc, alpha, beta):
+    """Above 1024 engines the rejection fill runs one wave per engine (up to
+    1024, a four-wave workgroup per engine: test_gamma_reject_matches_oracle);
+    sampled engine blocks and records against the oracle."""
+    n_eng, per = 1028, 2504
+    urng = er.Vmt19937_64(77, n=n_eng, stride=4)
+    out = er.gamma(per, 1, urng, alpha, beta).reshape(n_eng, per).cpu().numpy()
+    snap = urng.snapshot().reshape(n_eng, -1)
+    for g in (0, 1, 513, n_eng - 1):
+        ref = orc.Engines(seeds=[77 + 4 * g])
+        assert_same(out[g], orc.fill_gamma_f32(ref, per, alpha, beta))
+        np.testing.assert_array_equal(snap[g], ref.raw)
+    assert urng.status() == 0
+
+
 @pytest.mark.parametrize("mean", [0.3, 1.0, 4.0, 11.5, 12.0, 16.0, 100.0, 3000.0])
 def test_poisson_matches_oracle(er, orc, mean):
     urng = er.Vmt19937_64(seeds=SEEDS)
