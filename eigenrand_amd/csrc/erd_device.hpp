@@ -184,19 +184,46 @@ __device__ __forceinline__ void twist_lds(uint64_t* mt, int lane) {
     __syncthreads();
 }
 
-// twist_lds for a workgroup of several waves: every thread reaches the
-// barriers and wave 0 (tid < 64) does the work.
+// twist_lds for a workgroup of NT threads (NT/64 waves), all of which share
+// the work: in each of the two large phases every thread loads its words
+// (and their next and far partners), the workgroup waits, then every thread
+// stores its new words, so each old word is read before any is rewritten.
+template <int NT>
 __device__ __forceinline__ void twist_lds_wg(uint64_t* mt, int tid) {
-    constexpr int P1 = kM * kLanes;
-    constexpr int P2 = (kN - 1) * kLanes - P1;
-    if (tid < 64) {
-        twist_batch<0, 5>(mt, 0, P1, P1, tid);
-        twist_batch<5, 5>(mt, 0, P1, P1, tid);
+    constexpr int P1 = kM * kLanes;              // 624 words
+    constexpr int P2 = (kN - 1) * kLanes - P1;   // 620 words
+    constexpr int TR = (P1 + NT - 1) / NT;       // words per thread and phase
+    uint64_t c[TR], n[TR], f[TR];
+#pragma unroll
+    for (int i = 0; i < TR; ++i) {
+        const int r = tid + NT * i;
+        if (r < P1) {
+            c[i] = mt[r];
+            n[i] = mt[r + kLanes];
+            f[i] = mt[r + P1];
+        }
     }
     __syncthreads();
-    if (tid < 64) {
-        twist_batch<0, 5>(mt, P1, P2, -P1, tid);
-        twist_batch<5, 5>(mt, P1, P2, -P1, tid);
+#pragma unroll
+    for (int i = 0; i < TR; ++i) {
+        const int r = tid + NT * i;
+        if (r < P1) mt[r] = twist_word(c[i], n[i], f[i]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TR; ++i) {
+        const int r = tid + NT * i;
+        if (r < P2) {
+            c[i] = mt[P1 + r];
+            n[i] = mt[P1 + r + kLanes];
+            f[i] = mt[r];  // word r of phase 1, already new
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TR; ++i) {
+        const int r = tid + NT * i;
+        if (r < P2) mt[P1 + r] = twist_word(c[i], n[i], f[i]);
     }
     __syncthreads();
     if (tid < kLanes) {

@@ -307,7 +307,7 @@ __device__ void op_fixed_wg(Eng& e, uint64_t* carry, int64_t per, const D& d, in
     int64_t done = 0;
     for (;;) {
         if (pos >= kN) {
-            twist_lds_wg(e.mt, tid);
+            twist_lds_wg<NT>(e.mt, tid);
             pos = 0;
             e.twisted = true;
         }
@@ -398,7 +398,7 @@ __device__ void op_fixed_rt_wg(Eng& e, uint64_t* carry, int64_t per, int C, cons
     int64_t done = 0;
     for (;;) {
         if (pos >= kN) {
-            twist_lds_wg(e.mt, tid);
+            twist_lds_wg<NT>(e.mt, tid);
             pos = 0;
             e.twisted = true;
         }
@@ -693,7 +693,7 @@ __device__ void op_reject_wg(Eng& e, RejectWgLds<P, NW>& s, typename P::Out* o, 
     constexpr int kFirst = IPC == 4 ? 0x00 : 0xA0;  // quad_perm 0,0,0,0 / 0,0,2,2
     for (;;) {
         if (pos >= SLOTS) {
-            twist_lds_wg(e.mt, tid);
+            twist_lds_wg<64 * NW>(e.mt, tid);
             pos = 0;
             e.twisted = true;
         }
