@@ -28,6 +28,13 @@ hipError_t upload_async(void* dst, const void* src, size_t bytes, hipStream_t s)
 hipError_t scratch_alloc(void** p, size_t bytes, hipStream_t s);
 void scratch_free(void* p, size_t bytes, hipStream_t s);
 
+// A second stream of the current device (created on first use, default
+// priority, not synchronising with the null stream) for work a call splits
+// in two; stream_after(to, from) makes `to` wait for what `from` holds so
+// far (an event recorded on `from`).
+hipError_t side_stream(hipStream_t* out);
+hipError_t stream_after(hipStream_t to, hipStream_t from);
+
 // A source of device blocks that are recycled by their owner rather than
 // freed through the driver (the serving loop's block pool).
 struct Recycler {

@@ -542,11 +542,10 @@ int erd_dirichlet_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, con
     hipStream_t s = (hipStream_t)stream;
     std::vector<int32_t> vd(dims, dims + n_inst);
     std::vector<int64_t> goff(n_inst + 1, 0), vs(samples, samples + n_inst);
-    int64_t aoff = 0, max_n = 0;
+    int64_t aoff = 0;
     for (int64_t i = 0; i < n_inst; ++i) {
         if (vd[i] <= 0 || vs[i] < 0 || vs[i] % 8 != 0) return ERD_ERR_INVALID;
         goff[i + 1] = goff[i] + (int64_t)vd[i] * vs[i];
-        max_n = std::max(max_n, vs[i]);
         aoff += vd[i];
     }
     Scratch sc(s);
@@ -567,18 +566,56 @@ int erd_dirichlet_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, con
     std::stable_sort(rows.begin(), rows.end(), [](const GammaRowGroup& x, const GammaRowGroup& y) {
         return (int64_t)x.rows * x.count > (int64_t)y.rows * y.count;
     });
+    // The longest chains end last: their instances (work >= kLongChain of
+    // the longest) run on the caller's stream; the rest run on a second
+    // stream at the same time and are normalised there as soon as their rows
+    // are done, in the time the long chains still take. Then the long ones
+    // are normalised, and the caller's stream waits for the second.
+    constexpr double kLongChain = 0.7;
+    const double w_max = (double)rows[0].rows * (double)rows[0].count;
+    size_t n_long = 0;
+    while (n_long < rows.size() && (double)rows[n_long].rows * (double)rows[n_long].count >= kLongChain * w_max)
+        ++n_long;
+    hipStream_t s2 = nullptr;
+    if (n_long == rows.size() || side_stream(&s2) != hipSuccess) n_long = rows.size();
+    struct Part {
+        std::vector<int32_t> d;
+        std::vector<int64_t> n, off;
+        int64_t max_n = 0;
+        int32_t* d_d = nullptr;
+        int64_t *d_n = nullptr, *d_off = nullptr;
+    } part[2];
+    for (size_t i = 0; i < rows.size(); ++i) {
+        Part& q = part[i < n_long ? 0 : 1];
+        q.d.push_back(rows[i].rows);
+        q.n.push_back(rows[i].count);
+        q.off.push_back(rows[i].out - g);
+        q.max_n = std::max(q.max_n, rows[i].count);
+    }
     GammaRowGroup* d_rows;
     float* d_alpha;
     if (sc.upload(&d_rows, rows) || sc.alloc(&d_alpha, (size_t)aoff) ||
         upload_async(d_alpha, alpha, sizeof(float) * (size_t)aoff, s))
         return ERD_ERR_HIP;
-    if (launch_gamma_rows(e->states, d_rows, (int32_t)n_inst, d_alpha, guard_of(e), s) != hipSuccess)
+    for (Part& q : part)
+        if (!q.d.empty() && (sc.upload(&q.d_d, q.d) || sc.upload(&q.d_n, q.n) || sc.upload(&q.d_off, q.off)))
+            return ERD_ERR_HIP;
+    // the long chains are enqueued first, so that their workgroups are
+    // dispatched ahead of the short ones
+    if (n_long < rows.size() && stream_after(s2, s) != hipSuccess) return ERD_ERR_HIP;
+    const Part& q = part[0];
+    if (launch_gamma_rows(e->states, d_rows, (int32_t)n_long, d_alpha, guard_of(e), s) != hipSuccess ||
+        launch_dirichlet_norm(q.d_d, q.d_n, q.d_off, g, out, (int)q.d.size(), q.max_n, s) != hipSuccess)
         return ERD_ERR_HIP;
-    int32_t* d_dims;
-    int64_t *d_samples, *d_goff;
-    if (sc.upload(&d_dims, vd) || sc.upload(&d_samples, vs) || sc.upload(&d_goff, goff)) return ERD_ERR_HIP;
-    if (launch_dirichlet_norm(d_dims, d_samples, d_goff, g, out, (int)n_inst, max_n, s) != hipSuccess)
-        return ERD_ERR_HIP;
+    if (n_long < rows.size()) {
+        // second stream: the shorter chains, then their normalisation
+        const Part& r = part[1];
+        if (launch_gamma_rows(e->states, d_rows + n_long, (int32_t)(rows.size() - n_long), d_alpha, guard_of(e),
+                              s2) != hipSuccess ||
+            launch_dirichlet_norm(r.d_d, r.d_n, r.d_off, g, out, (int)r.d.size(), r.max_n, s2) != hipSuccess)
+            return ERD_ERR_HIP;
+    }
+    if (n_long < rows.size() && stream_after(s, s2) != hipSuccess) return ERD_ERR_HIP;
     return ERD_OK;
 }
 

@@ -214,7 +214,37 @@ ScratchCache* cache_of_current_device() {
     return c;
 }
 
+std::mutex g_side_mu;
+std::map<int, hipStream_t> g_side;  // per device; kept until the process ends
+
 }  // namespace
+
+hipError_t side_stream(hipStream_t* out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(g_side_mu);
+    hipStream_t& st = g_side[dev];
+    if (!st) {
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            st = nullptr;
+            return e;
+        }
+    }
+    *out = st;
+    return hipSuccess;
+}
+
+hipError_t stream_after(hipStream_t to, hipStream_t from) {
+    hipEvent_t ev;
+    hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(ev, from);
+    if (e == hipSuccess) e = hipStreamWaitEvent(to, ev, 0);
+    const hipError_t d = hipEventDestroy(ev);  // released once the record completes
+    return e != hipSuccess ? e : d;
+}
 
 hipError_t scratch_alloc(void** p, size_t bytes, hipStream_t s) {
     ScratchCache* c = cache_of_current_device();

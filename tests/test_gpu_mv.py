@@ -403,3 +403,36 @@ def test_dirichlet_gamma_rows_are_exact(er, orc):
         want = (g / s).astype(np.float32)
         assert_same(res[i].cpu().numpy(), want)
         np.testing.assert_array_equal(snap[i], ref.raw)
+
+
+def test_dirichlet_split_batches_match_oracle(er, orc):
+    """A Dirichlet batch runs its longest chains of category rows (work at
+    least 0.7 of the longest) and their normalisation on the caller's
+    stream, and the other instances on a second stream at the same time;
+    the caller's stream then waits for the second. Each instance must be the
+    oracle's gamma rows divided by their category-order column sums, bit
+    for bit, with its engine record advanced as the oracle's, whichever
+    stream ran it: a short 2048-sample instance beside a long one, a
+    2056-sample instance beside a short one, and a 2100-category instance
+    beside a short one."""
+    import orc_mv
+    rng = np.random.default_rng(23)
+    long_k = np.exp(rng.uniform(np.log(0.05), np.log(8), 40)).astype(np.float32)
+    wide_k = np.exp(rng.uniform(np.log(0.05), np.log(8), 2100)).astype(np.float32)
+    cases = [([np.float32([0.7, 2.0, 5.5]), long_k], [2048, 1024]),
+             ([np.float32([0.7, 2.0, 5.5]), np.float32([1.0, 0.2, 3.0, 12.5, 0.05])], [2056, 16]),
+             ([wide_k, np.float32([0.7, 2.0, 5.5])], [24, 16])]
+    for alphas, samples in cases:
+        seeds = [61, 62]
+        urng = er.Vmt19937_64(seeds=seeds)
+        print("dirichlet samples", samples, [len(a) for a in alphas], flush=True)
+        res = er.dirichlet_batch(urng, alphas, samples)
+        snap = urng.snapshot().reshape(len(seeds), -1)
+        for i, a in enumerate(alphas):
+            ref = orc.Engines(seeds=[seeds[i]])
+            _, g = orc_mv.dirichlet(ref, a, samples[i])
+            s = np.zeros(samples[i], np.float32)
+            for k in range(len(a)):
+                s = (s + g[k]).astype(np.float32)
+            assert_same(res[i].cpu().numpy(), (g / s).astype(np.float32))
+            np.testing.assert_array_equal(snap[i], ref.raw)
