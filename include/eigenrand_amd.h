@@ -468,7 +468,10 @@ int erd_invwishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, co
 
 /* DirichletGen<float>(alpha_i).generate(engine_i, samples_i)
  * (Multinomial.h:159-217): out_i is K_i x samples_i column-major. alpha: HOST
- * floats (K_i each, back to back); samples_i must be a multiple of 8. */
+ * floats (K_i each, back to back); samples_i must be a multiple of 8. The
+ * call may split its work over `stream` and an internal second stream of the
+ * device (event fork and join): on return, everything it enqueued is ordered
+ * before later work on `stream`. */
 int erd_dirichlet_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* samples,
                         const float* alpha, float* out, void* stream);
 
