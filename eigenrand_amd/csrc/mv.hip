@@ -165,6 +165,19 @@ __device__ __forceinline__ uint16_t bf16_rne(float x) {
 }
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// Split of two floats into packed bf16 hi and lo words (element 0 in the low
+// half): hi = bf16(x), lo = bf16(x - hi), both round-to-nearest-even by
+// v_cvt_pk_bf16_f32, which equals bf16_rne bit for bit on finite inputs
+// below the bf16 overflow threshold (all that L and Z hold).
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split_bf16x2(float a, float b, uint32_t& hi, uint32_t& lo) {
+    hi = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+    const float ra = a - __uint_as_float(hi << 16);
+    const float rb = b - __uint_as_float(hi & 0xFFFF0000u);
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){ra, rb}, bf16x2));
+}
+
 // L (FP64, column-major d x d) -> row-major bf16 hi/lo planes of a padded
 // dp x dp tile (dp = d rounded up to 16), plus the float L (column-major)
 // that float consumers use. hi = bf16(l), lo = bf16(l - hi) with l = float(L).
@@ -323,17 +336,11 @@ __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tile
             for (int i = 0; i < ZQ; ++i) {
                 const int q = tid + 256 * i;
                 const int c = q / (KC / 8), kk = (q % (KC / 8)) * 8;
-                uint16_t h[8], lw[8];
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    h[e] = bf16_rne(pz[i][e]);
-                    lw[e] = bf16_rne(pz[i][e] - bf16_to_f(h[e]));
-                }
                 uint4 vh, vl;
-                vh.x = h[0] | ((uint32_t)h[1] << 16); vh.y = h[2] | ((uint32_t)h[3] << 16);
-                vh.z = h[4] | ((uint32_t)h[5] << 16); vh.w = h[6] | ((uint32_t)h[7] << 16);
-                vl.x = lw[0] | ((uint32_t)lw[1] << 16); vl.y = lw[2] | ((uint32_t)lw[3] << 16);
-                vl.z = lw[4] | ((uint32_t)lw[5] << 16); vl.w = lw[6] | ((uint32_t)lw[7] << 16);
+                split_bf16x2(pz[i][0], pz[i][1], vh.x, vl.x);
+                split_bf16x2(pz[i][2], pz[i][3], vh.y, vl.y);
+                split_bf16x2(pz[i][4], pz[i][5], vh.z, vl.z);
+                split_bf16x2(pz[i][6], pz[i][7], vh.w, vl.w);
                 *reinterpret_cast<uint4*>(&zb[0][c * KPAD + kk]) = vh;
                 *reinterpret_cast<uint4*>(&zb[1][c * KPAD + kk]) = vl;
             }
