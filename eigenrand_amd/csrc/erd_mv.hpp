@@ -40,6 +40,8 @@ hipError_t launch_split_l(const double* l64, const int64_t* off, const int32_t* 
 // go to the 8 XCDs round robin), then takes what is left in the others.
 // counter: kLzQueues zeroed ints.
 constexpr int kLzQueues = 8;
+// Workgroups of L*Z variant v resident per CU (from its register and LDS use).
+int lz_blocks_per_cu(int variant);
 hipError_t launch_lz_gemm(int variant, int grid, const MvTile* tiles, const int32_t* qbeg, int32_t* counter,
                           const MvInstance* inst, hipStream_t s);
 

@@ -227,7 +227,7 @@ constexpr int KPAD = 40;     // LDS row pitch in bf16 (80 B: keeps 16 B alignmen
 // in LDS; each wave issues hi*hi + hi*lo + lo*hi MFMAs, skipping row blocks
 // above the diagonal (L is lower triangular).
 template <int NCB, int MAXR>
-__global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tiles,
+__global__ __launch_bounds__(256, NCB == 4 ? 2 : 3) void k_lz_gemm(const MvTile* __restrict__ tiles,
                                                  const int32_t* __restrict__ qbeg,
                                                  int32_t* __restrict__ counter,
                                                  const MvInstance* __restrict__ inst) {
@@ -394,6 +394,21 @@ __global__ __launch_bounds__(256) void k_lz_gemm(const MvTile* __restrict__ tile
             }
         }
     }
+}
+
+int lz_blocks_per_cu(int variant) {
+    static int cache[kLzVariants] = {};
+    if (variant < 0 || variant >= kLzVariants) return 1;
+    if (cache[variant] == 0) {
+        const void* f = variant == 0   ? (const void*)k_lz_gemm<1, 16>
+                        : variant == 1 ? (const void*)k_lz_gemm<2, 8>
+                        : variant == 2 ? (const void*)k_lz_gemm<4, 4>
+                                       : (const void*)k_lz_gemm<1, 4>;
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, f, 256, 0) != hipSuccess || b < 1) b = 2;
+        cache[variant] = b;
+    }
+    return cache[variant];
 }
 
 hipError_t launch_lz_gemm(int variant, int grid, const MvTile* tiles, const int32_t* qbeg, int32_t* counter,

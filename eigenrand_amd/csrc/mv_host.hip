@@ -91,8 +91,8 @@ int grid_for(int variant, size_t n_tiles) {
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         cus = 256;
-    const int per_cu = variant == 0 ? 3 : 4;
-    return (int)std::min<size_t>(n_tiles, (size_t)cus * per_cu);
+    // a persistent grid of the workgroups that fit at once
+    return (int)std::min<size_t>(n_tiles, (size_t)cus * lz_blocks_per_cu(variant));
 }
 
 // Tile lists of run_lz without launching, for callers that stage their own
