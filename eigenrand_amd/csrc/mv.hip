@@ -467,12 +467,16 @@ __global__ void k_wishart_assemble(const MvInstance* __restrict__ inst, const in
     const int64_t total = (int64_t)d * d * n;
     const int64_t per_sample = (int64_t)d * (d - 1) / 2;
     float* A = const_cast<float*>(in.z);
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t col = t / d;
-        const int row = (int)(t % d);
-        const int64_t j = col / d;
-        const int i = (int)(col % d);
+    // element t = (j * d + i) * d + row: the divisions once, then the grid
+    // stride S advances (row, i, j) by (S mod d, (S / d) mod d, S / d^2) with
+    // carries
+    const int64_t S = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int row = (int)(t0 % d), i = (int)((t0 / d) % d);
+    int64_t j = t0 / d / d;
+    const int s_row = (int)(S % d), s_i = (int)((S / d) % d);
+    const int64_t s_j = S / d / d;
+    for (int64_t t = t0; t < total; t += S) {
         float v = 0.0f;
         if (row == i) {
             v = sqrtf(chi[chi_off[i_inst] + (int64_t)i * n + j]);
@@ -482,68 +486,119 @@ __global__ void k_wishart_assemble(const MvInstance* __restrict__ inst, const in
             v = nrm[nrm_off[i_inst] + j * per_sample + before + (row - i - 1)];
         }
         A[t] = v;
+        row += s_row;
+        int carry = 0;
+        if (row >= d) row -= d, carry = 1;
+        i += s_i + carry;
+        j += s_j;
+        if (i >= d) i -= d, ++j;
     }
 }
 
-// W_j = T_j T_j^T (MvNormal.h:271-275) in FP32. One workgroup per (instance,
-// sample, 32x32 output tile with tile row >= tile column); T is lower
-// triangular so the k loop stops at the smaller tile's last row.
+// W_j = T_j T_j^T (WishartGen, MvNormal.h:271-275) or U_j^T U_j (U = T^-1,
+// InvWishartGen, MvNormal.h:433-439) in FP32, per sample, over 64 x 64 output tiles with
+// ta >= tb (tile units of 64). A thread keeps a 4 x 4 block of the tile in
+// registers; per 32-wide k chunk the workgroup stages the tile's 64 rows of
+// both operands in LDS ([k][row], so a thread's four rows are one 16-byte
+// read). Every output is one fmaf chain over k ascending, from 0 (Wishart:
+// T[b][k] = 0 beyond k = b) or from the tile's first row (inverse: U[k][a]
+// = 0 for k < a); the extra terms the tile bounds add are exact zeros. Both
+// halves of W are written from the same accumulator, so W is exactly
+// symmetric; diagonal tiles write only their lower half.
+constexpr int kSyT = 64, kSyK = 32, kSyPitch = kSyT + 4;
 
-// UTU: W_j = U_j^T U_j instead (InvWishartGen, MvNormal.h:433-439), with U
-// lower triangular in `in.x`: the k loop runs from the larger tile's first
-// row to d, and the tiles read columns of U.
 template <bool UTU>
 __global__ __launch_bounds__(256) void k_syrk_tt(const SyrkTile* __restrict__ tiles,
                                                  const MvInstance* __restrict__ inst,
                                                  float* __restrict__ w, const int64_t* __restrict__ w_off) {
-    __shared__ float ta_s[32][33];
-    __shared__ float tb_s[32][33];
+    __shared__ __align__(16) float sm[2][kSyK][kSyPitch];
+    float(&ta_s)[kSyK][kSyPitch] = sm[0];
+    float(&tb_s)[kSyK][kSyPitch] = sm[1];
     const SyrkTile tl = tiles[blockIdx.x];
+    if (tl.inst < 0) return;  // padding of the per-XCD tile order
     const MvInstance in = inst[tl.inst];
     const int d = in.d;
     const float* T = in.x + (int64_t)tl.sample * d * d;  // d x d column-major
     float* W = w + w_off[tl.inst] + (int64_t)tl.sample * d * d;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-    const int a0 = tl.ta * 32, b0 = tl.tb * 32;
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    // T[b][k] = 0 for k > b; U[k][a] = 0 for k < a (a0 >= b0)
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;  // 16 x 16 threads, 4 x 4 outputs each
+    const int a0 = tl.ta * kSyT, b0 = tl.tb * kSyT;
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
     const int kbeg = UTU ? a0 : 0;
-    const int kend = UTU ? d : (b0 + 32 < d ? b0 + 32 : d);
-    for (int k0 = kbeg; k0 < kend; k0 += 32) {
-        for (int q = threadIdx.x; q < 32 * 32; q += 256) {
+    const int kend = UTU ? d : (b0 + kSyT < d ? b0 + kSyT : d);
+    // element q = tid + 256 i of a chunk's staging: row r, k offset kk
+    constexpr int kSyQ = kSyT * kSyK / 256;
+    auto rk = [&](int i, int& r, int& kk) {
+        const int q = tid + 256 * i;
+        if (UTU) {  // consecutive threads walk down a column of U
+            kk = q & (kSyK - 1);
+            r = q / kSyK;
+        } else {    // consecutive threads walk down a column of T
+            r = q & (kSyT - 1);
+            kk = q / kSyT;
+        }
+    };
+    // the next chunk's operands are loaded into registers while the current
+    // chunk's products run
+    float pa[kSyQ], pb[kSyQ];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < kSyQ; ++i) {
             int r, kk;
-            if (UTU) {  // consecutive threads walk down a column of U
-                kk = q & 31;
-                r = q >> 5;
-            } else {
-                r = q & 31;
-                kk = q >> 5;
-            }
+            rk(i, r, kk);
             const int k = k0 + kk;
-            if (UTU) {
-                ta_s[r][kk] = (a0 + r < d && k < d) ? T[(int64_t)(a0 + r) * d + k] : 0.0f;
-                tb_s[r][kk] = (b0 + r < d && k < d) ? T[(int64_t)(b0 + r) * d + k] : 0.0f;
-            } else {
-                ta_s[r][kk] = (a0 + r < d && k < d) ? T[(int64_t)k * d + a0 + r] : 0.0f;
-                tb_s[r][kk] = (b0 + r < d && k < d) ? T[(int64_t)k * d + b0 + r] : 0.0f;
+            pa[i] = 0.0f;
+            pb[i] = 0.0f;
+            if (k < d) {
+                if (a0 + r < d) pa[i] = UTU ? T[(int64_t)(a0 + r) * d + k] : T[(int64_t)k * d + a0 + r];
+                if (b0 + r < d) pb[i] = UTU ? T[(int64_t)(b0 + r) * d + k] : T[(int64_t)k * d + b0 + r];
             }
         }
-        __syncthreads();
-#pragma unroll 8
-        for (int kk = 0; kk < 32; ++kk) {
-            const float bv = tb_s[tx][kk];
+    };
+    if (kbeg < kend) fetch(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += kSyK) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] = fmaf(ta_s[ty * 4 + q][kk], bv, acc[q]);
+        for (int i = 0; i < kSyQ; ++i) {
+            int r, kk;
+            rk(i, r, kk);
+            ta_s[kk][r] = pa[i];
+            tb_s[kk][r] = pb[i];
+        }
+        __syncthreads();
+        if (k0 + kSyK < kend) fetch(k0 + kSyK);
+#pragma unroll 8
+        for (int kk = 0; kk < kSyK; ++kk) {
+            const float4 av = *reinterpret_cast<const float4*>(&ta_s[kk][4 * ty]);
+            const float4 bv = *reinterpret_cast<const float4*>(&tb_s[kk][4 * tx]);
+            const float ar[4] = {av.x, av.y, av.z, av.w}, br[4] = {bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(ar[i], br[j], acc[i][j]);
         }
         __syncthreads();
     }
+    // the tile through LDS (the operand buffers are free after the last
+    // barrier), so that both halves of W are written along columns
+    static_assert(sizeof(sm) >= sizeof(float) * kSyT * (kSyT + 1), "the tile fits the operand LDS");
+    float(*ot)[kSyT + 1] = reinterpret_cast<float(*)[kSyT + 1]>(&sm[0][0][0]);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int ra = a0 + ty * 4 + q, rb = b0 + tx;
-        if (ra < d && rb < d) {
-            W[(int64_t)rb * d + ra] = acc[q];
-            W[(int64_t)ra * d + rb] = acc[q];
-        }
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ot[4 * ty + i][4 * tx + j] = acc[i][j];
+    __syncthreads();
+    const bool diag = tl.ta == tl.tb;
+    const int c = tid & (kSyT - 1), r0 = tid >> 6;  // 64 consecutive threads along a column
+    for (int r = r0; r < kSyT; r += 4) {
+        // column b0 + r of W, rows a0 + c: W(a, b) = ot[a - a0][b - b0]
+        const int ra = a0 + c, rb = b0 + r;
+        if (ra < d && rb < d && (!diag || ra >= rb)) W[(int64_t)rb * d + ra] = ot[c][r];
+        // column a0 + r of W, rows b0 + c: W(b, a) = ot[a - a0][b - b0]
+        const int sa = a0 + r, sb = b0 + c;
+        if (sa < d && sb < d && (!diag || sa >= sb)) W[(int64_t)sa * d + sb] = ot[r][c];
     }
 }
 
@@ -660,6 +715,7 @@ __global__ __launch_bounds__(256) void k_trtri_panel(const SyrkTile* __restrict_
     extern __shared__ float X[];
     __shared__ float Tb[2][32][33];
     const SyrkTile tl = tiles[blockIdx.x];  // ta = panel J
+    if (tl.inst < 0) return;               // padding of the per-XCD panel order
     const MvInstance in = inst[tl.inst];
     const int d = in.d;
     const float* T = in.x + (int64_t)tl.sample * d * d;  // d x d column-major

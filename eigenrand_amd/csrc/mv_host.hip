@@ -455,13 +455,32 @@ static int wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64
         return ERD_ERR_HIP;
     rc = run_lz(inst, d_inst, s, sc);
     if (rc) return rc;
-    std::vector<SyrkTile> tiles;
-    for (int64_t i = 0; i < n_inst; ++i) {
-        const int nb = (vd[i] + 31) / 32;
-        for (int64_t j = 0; j < samples[i]; ++j)
-            for (int a = 0; a < nb; ++a)
-                for (int b = 0; b <= a; ++b) tiles.push_back(SyrkTile{(int32_t)i, (int32_t)j, a, b});
+    // 64 x 64 output tiles of W (ta >= tb). All tiles of one sample go to
+    // one XCD, so that its L2 fetches the sample's T once: workgroups are
+    // dispatched to the eight XCDs round-robin, so tile list position p runs
+    // on XCD p mod 8. Samples are dealt to the XCDs by least load (work of a
+    // tile ~ its k range), and the eight lists are interleaved, padded with
+    // empty tiles (inst -1) to equal length.
+    std::vector<std::vector<SyrkTile>> bins(kXcds);
+    {
+        std::vector<double> load(kXcds, 0.0);
+        for (int64_t i = 0; i < n_inst; ++i) {
+            const int nb = (vd[i] + kSyrkTile - 1) / kSyrkTile;
+            for (int64_t j = 0; j < samples[i]; ++j) {
+                const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                for (int a = 0; a < nb; ++a)
+                    for (int b = 0; b <= a; ++b) {
+                        bins[x].push_back(SyrkTile{(int32_t)i, (int32_t)j, a, b});
+                        load[x] += inverse ? (double)(vd[i] - a * kSyrkTile) : (double)std::min(vd[i], (b + 1) * kSyrkTile);
+                    }
+            }
+        }
     }
+    size_t per_bin = 0;
+    for (const auto& b : bins) per_bin = std::max(per_bin, b.size());
+    std::vector<SyrkTile> tiles(per_bin * kXcds, SyrkTile{-1, 0, 0, 0});
+    for (int x = 0; x < kXcds; ++x)
+        for (size_t m = 0; m < bins[x].size(); ++m) tiles[m * kXcds + x] = bins[x][m];
     SyrkTile* d_tiles;
     if (sc.upload(&d_tiles, tiles) != hipSuccess) return ERD_ERR_HIP;
     if (inverse) {
@@ -469,17 +488,32 @@ static int wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64
         // W_j = U_j^T U_j from U.
         // panel p of an instance with nb blocks solves nb - p block rows: the
         // longest panels go first, and the LDS panel is sized for them
-        std::vector<SyrkTile> panels;
+        // (all panels of a sample on one XCD, as the syrk tiles: they all
+        // read the sample's T)
+        std::vector<std::vector<SyrkTile>> pbins(kXcds);
+        std::vector<double> pload(kXcds, 0.0);
         int max_rows = 0;
         for (int64_t i = 0; i < n_inst; ++i) {
             const int nb = (vd[i] + 31) / 32;
             max_rows = std::max(max_rows, 32 * nb);
-            for (int64_t j = 0; j < samples[i]; ++j)
-                for (int p = 0; p < nb; ++p) panels.push_back(SyrkTile{(int32_t)i, (int32_t)j, p, 0});
+            for (int64_t j = 0; j < samples[i]; ++j) {
+                const int x = (int)(std::min_element(pload.begin(), pload.end()) - pload.begin());
+                for (int p = 0; p < nb; ++p) {
+                    pbins[x].push_back(SyrkTile{(int32_t)i, (int32_t)j, p, 0});
+                    pload[x] += (double)(nb - p);
+                }
+            }
         }
-        std::stable_sort(panels.begin(), panels.end(), [&](const SyrkTile& a, const SyrkTile& b) {
-            return (vd[a.inst] + 31) / 32 - a.ta > (vd[b.inst] + 31) / 32 - b.ta;
-        });
+        size_t per_pbin = 0;
+        for (auto& b : pbins) {
+            std::stable_sort(b.begin(), b.end(), [&](const SyrkTile& a, const SyrkTile& c) {
+                return (vd[a.inst] + 31) / 32 - a.ta > (vd[c.inst] + 31) / 32 - c.ta;
+            });
+            per_pbin = std::max(per_pbin, b.size());
+        }
+        std::vector<SyrkTile> panels(per_pbin * kXcds, SyrkTile{-1, 0, 0, 0});
+        for (int x = 0; x < kXcds; ++x)
+            for (size_t m = 0; m < pbins[x].size(); ++m) panels[m * kXcds + x] = pbins[x][m];
         std::vector<MvInstance> uinst = inst;
         for (int64_t i = 0; i < n_inst; ++i) uinst[i].x = amat + aoff[i];
         SyrkTile* d_panels;
