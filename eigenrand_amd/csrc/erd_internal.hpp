@@ -111,7 +111,8 @@ int run_segments(erd_engines* e, const std::vector<Segment>& segs, const std::ve
 int make_segment(const erd_request& r, Segment& s);
 
 // Multivariate helpers (mv_host.hip).
-int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t s, Scratch& sc);
+int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t s, Scratch& sc,
+           bool bartlett = false);
 // One persistent L*Z launch: variant, grid and its tile queue.
 struct LzLaunch {
     int variant = 0;

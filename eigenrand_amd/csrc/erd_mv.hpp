@@ -15,6 +15,12 @@ struct MvInstance {
     const float* z;        // d x n, column-major
     float* x;              // d x n, column-major
     const float* mean;     // d floats, or null
+    // Wishart's Bartlett matrices built in the product's operand fetch (no A
+    // in memory): with chi set, column i of sample j of Z (n = d * samples
+    // columns) is 0 above row i, sqrt(chi[i * samples + j]) at row i, and
+    // below it the next normals of z (sample j's d(d-1)/2 normals, column by
+    // column), as WishartGen's rand_mat (MvNormal.h:240-266)
+    const float* chi;
 };
 
 // A work item of the persistent transform: rows [row0, row0 + 16*MAXR) x
@@ -43,7 +49,7 @@ constexpr int kLzQueues = 8;
 // Workgroups of L*Z variant v resident per CU (from its register and LDS use).
 int lz_blocks_per_cu(int variant);
 hipError_t launch_lz_gemm(int variant, int grid, const MvTile* tiles, const int32_t* qbeg, int32_t* counter,
-                          const MvInstance* inst, hipStream_t s);
+                          const MvInstance* inst, hipStream_t s, bool bartlett = false);
 
 struct SyrkTile {
     int32_t inst;

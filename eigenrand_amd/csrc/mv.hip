@@ -226,8 +226,8 @@ constexpr int KPAD = 40;     // LDS row pitch in bf16 (80 B: keeps 16 B alignmen
 // stages L[:, k0:k0+32] (hi, lo) and Z[k0:k0+32, cols] (split on the fly)
 // in LDS; each wave issues hi*hi + hi*lo + lo*hi MFMAs, skipping row blocks
 // above the diagonal (L is lower triangular).
-template <int NCB, int MAXR>
-__global__ __launch_bounds__(256, NCB == 4 ? 2 : 3) void k_lz_gemm(const MvTile* __restrict__ tiles,
+template <int NCB, int MAXR, bool BART>
+__global__ __launch_bounds__(256, NCB == 4 || BART ? 2 : 3) void k_lz_gemm(const MvTile* __restrict__ tiles,
                                                  const int32_t* __restrict__ qbeg,
                                                  int32_t* __restrict__ counter,
                                                  const MvInstance* __restrict__ inst) {
@@ -283,6 +283,21 @@ __global__ __launch_bounds__(256, NCB == 4 ? 2 : 3) void k_lz_gemm(const MvTile*
         const int lq_n = R * 16 * (KC / 8);
         uint4 ph[LQ], pl[LQ];
         float pz[ZQ][8];
+        // BART: the sample j and column i of each of this thread's Z columns,
+        // and where the column's normals start in z (element k > i at z[zn + k])
+        int zi[ZQ];
+        int64_t zj[ZQ], zn[ZQ];
+        if (BART) {
+            const int64_t per_sample = (int64_t)d * (d - 1) / 2;
+#pragma unroll
+            for (int i = 0; i < ZQ; ++i) {
+                const int c = (tid + 256 * i) / (KC / 8);
+                const int64_t col = c0 + c < n ? c0 + c : 0;
+                zj[i] = col / d;
+                zi[i] = (int)(col - zj[i] * d);
+                zn[i] = zj[i] * per_sample + (int64_t)zi[i] * (d - 1) - (int64_t)zi[i] * (zi[i] - 1) / 2 - zi[i] - 1;
+            }
+        }
         auto fetch = [&](int k0) {
 #pragma unroll
             for (int i = 0; i < LQ; ++i) {
@@ -304,6 +319,30 @@ __global__ __launch_bounds__(256, NCB == 4 ? 2 : 3) void k_lz_gemm(const MvTile*
                 const int q = tid + 256 * i;
                 const int c = q / (KC / 8), kk = (q % (KC / 8)) * 8;
                 const int64_t col = c0 + c;
+                if (BART) {
+                    const int kb = k0 + kk;
+                    if (col < n && kb > zi[i] && kb + 8 <= d) {
+                        // all eight below the diagonal: consecutive normals
+                        const f32x4_a4 a = *reinterpret_cast<ERD_GLOBAL const f32x4_a4*>(Z + zn[i] + kb);
+                        const f32x4_a4 b = *reinterpret_cast<ERD_GLOBAL const f32x4_a4*>(Z + zn[i] + kb + 4);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) pz[i][e] = a[e], pz[i][4 + e] = b[e];
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) {
+                            const int k = kb + e;
+                            float v = 0.0f;
+                            if (col < n && k < d) {
+                                if (k == zi[i])
+                                    v = sqrtf(in.chi[(int64_t)zi[i] * (n / d) + zj[i]]);
+                                else if (k > zi[i])
+                                    v = Z[zn[i] + k];
+                            }
+                            pz[i][e] = v;
+                        }
+                    }
+                    continue;
+                }
                 if (col < n && k0 + kk + 8 <= d) {
                     // 8 consecutive floats of column col: two 16-byte loads
                     const f32x4_a4 a = *reinterpret_cast<ERD_GLOBAL const f32x4_a4*>(Z + col * d + k0 + kk);
@@ -400,10 +439,10 @@ int lz_blocks_per_cu(int variant) {
     static int cache[kLzVariants] = {};
     if (variant < 0 || variant >= kLzVariants) return 1;
     if (cache[variant] == 0) {
-        const void* f = variant == 0   ? (const void*)k_lz_gemm<1, 16>
-                        : variant == 1 ? (const void*)k_lz_gemm<2, 8>
-                        : variant == 2 ? (const void*)k_lz_gemm<4, 4>
-                                       : (const void*)k_lz_gemm<1, 4>;
+        const void* f = variant == 0   ? (const void*)k_lz_gemm<1, 16, false>
+                        : variant == 1 ? (const void*)k_lz_gemm<2, 8, false>
+                        : variant == 2 ? (const void*)k_lz_gemm<4, 4, false>
+                                       : (const void*)k_lz_gemm<1, 4, false>;
         int b = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, f, 256, 0) != hipSuccess || b < 1) b = 2;
         cache[variant] = b;
@@ -411,15 +450,24 @@ int lz_blocks_per_cu(int variant) {
     return cache[variant];
 }
 
-hipError_t launch_lz_gemm(int variant, int grid, const MvTile* tiles, const int32_t* qbeg, int32_t* counter,
-                          const MvInstance* inst, hipStream_t s) {
-    if (grid <= 0) return hipSuccess;
+template <bool BART>
+static void launch_lz_variant(int variant, int grid, const MvTile* tiles, const int32_t* qbeg, int32_t* counter,
+                              const MvInstance* inst, hipStream_t s) {
     switch (variant) {
-        case 0: hipLaunchKernelGGL((k_lz_gemm<1, 16>), dim3(grid), dim3(256), 0, s, tiles, qbeg, counter, inst); break;
-        case 1: hipLaunchKernelGGL((k_lz_gemm<2, 8>), dim3(grid), dim3(256), 0, s, tiles, qbeg, counter, inst); break;
-        case 2: hipLaunchKernelGGL((k_lz_gemm<4, 4>), dim3(grid), dim3(256), 0, s, tiles, qbeg, counter, inst); break;
-        default: hipLaunchKernelGGL((k_lz_gemm<1, 4>), dim3(grid), dim3(256), 0, s, tiles, qbeg, counter, inst); break;
+        case 0: hipLaunchKernelGGL((k_lz_gemm<1, 16, BART>), dim3(grid), dim3(256), 0, s, tiles, qbeg, counter, inst); break;
+        case 1: hipLaunchKernelGGL((k_lz_gemm<2, 8, BART>), dim3(grid), dim3(256), 0, s, tiles, qbeg, counter, inst); break;
+        case 2: hipLaunchKernelGGL((k_lz_gemm<4, 4, BART>), dim3(grid), dim3(256), 0, s, tiles, qbeg, counter, inst); break;
+        default: hipLaunchKernelGGL((k_lz_gemm<1, 4, BART>), dim3(grid), dim3(256), 0, s, tiles, qbeg, counter, inst); break;
     }
+}
+
+hipError_t launch_lz_gemm(int variant, int grid, const MvTile* tiles, const int32_t* qbeg, int32_t* counter,
+                          const MvInstance* inst, hipStream_t s, bool bartlett) {
+    if (grid <= 0) return hipSuccess;
+    if (bartlett)
+        launch_lz_variant<true>(variant, grid, tiles, qbeg, counter, inst, s);
+    else
+        launch_lz_variant<false>(variant, grid, tiles, qbeg, counter, inst, s);
     return hipGetLastError();
 }
 

@@ -128,7 +128,7 @@ int plan_lz(const std::vector<MvInstance>& inst, std::vector<LzLaunch>& plan) {
 
 // X = L*Z (+mean) for every instance: group instances by tuned variant, one
 // persistent launch per group.
-int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t s, Scratch& sc) {
+int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t s, Scratch& sc, bool bartlett) {
     std::map<int, std::vector<int>> by_bucket;
     for (int i = 0; i < (int)inst.size(); ++i) by_bucket[bucket_of(inst[i].d)].push_back(i);
     std::map<int, std::vector<int>> by_variant;
@@ -160,7 +160,8 @@ int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t 
                 if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, sizeof(int32_t) * kLzQueues, s);
                 if (e == hipSuccess) e = hipEventRecord(a, s);
                 if (e == hipSuccess)
-                    e = launch_lz_gemm(cand, grid_for(cand, tiles.size()), d_tiles, d_qbeg, d_cnt, d_inst, s);
+                    e = launch_lz_gemm(cand, grid_for(cand, tiles.size()), d_tiles, d_qbeg, d_cnt, d_inst, s,
+                                       bartlett);
                 if (e == hipSuccess) e = hipEventRecord(b, s);
                 if (e == hipSuccess) e = hipEventSynchronize(b);
                 if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
@@ -190,7 +191,8 @@ int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t 
             sc.alloc(&d_cnt, kLzQueues) != hipSuccess)
             return ERD_ERR_HIP;
         if (hipMemsetAsync(d_cnt, 0, sizeof(int32_t) * kLzQueues, s) != hipSuccess) return ERD_ERR_HIP;
-        hipError_t e = launch_lz_gemm(kv.first, grid_for(kv.first, tiles.size()), d_tiles, d_qbeg, d_cnt, d_inst, s);
+        hipError_t e =
+            launch_lz_gemm(kv.first, grid_for(kv.first, tiles.size()), d_tiles, d_qbeg, d_cnt, d_inst, s, bartlett);
         if (e != hipSuccess) return ERD_ERR_HIP;
     }
     return ERD_OK;
@@ -412,48 +414,47 @@ static int wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64
         chi_off[i + 1] = chi_off[i] + d * n;
     }
     Scratch sc(s);
-    float *nrm, *chi, *amat, *tmat;
-    if (sc.alloc(&nrm, nrm_off[n_inst]) || sc.alloc(&chi, chi_off[n_inst]) || sc.alloc(&amat, aoff[n_inst]) ||
-        sc.alloc(&tmat, aoff[n_inst]))
+    float *nrm, *chi, *amat = nullptr, *tmat;
+    if (sc.alloc(&nrm, nrm_off[n_inst]) || sc.alloc(&chi, chi_off[n_inst]) ||
+        (inverse && sc.alloc(&amat, aoff[n_inst])) || sc.alloc(&tmat, aoff[n_inst]))
         return ERD_ERR_OOM;
     // Engine program per instance: the normal fill, then chi^2(df - i) for
     // i = 0..d-1 (ChiSquaredGen = GammaGen(n/2, 2)), as generate() draws them.
-    // The normal fills run first, one workgroup per engine; the chi^2
-    // segments follow on the same stream.
+    // The normal fills run first, one workgroup per engine; the chi^2 rows
+    // follow on the same stream, the d fills of an instance in order in one
+    // four-wave workgroup (row k: GammaGen((df - k) / 2, 2) over the samples).
     std::vector<FillJob> jobs;
-    std::vector<Segment> segs;
-    std::vector<int32_t> engine_of;
+    std::vector<GammaRowGroup> rows((size_t)n_inst);
+    std::vector<float> alpha;
     for (int64_t i = 0; i < n_inst; ++i) {
         const int64_t d = vd[i], n = samples[i];
         jobs.push_back(FillJob{(int32_t)i, 0, n * d * (d - 1) / 2, nrm + nrm_off[i]});
+        rows[(size_t)i] = GammaRowGroup{(int32_t)i, (int32_t)d, n, chi + chi_off[i], (int64_t)alpha.size(), 2.0f, 0};
         for (int64_t k = 0; k < d; ++k) {
-            Segment g{};
-            g.kind = SEG_GAMMA_F32;
-            g.count = n;
-            g.out = chi + chi_off[i] + k * n;
             const float nu = (float)(df[i] - k);
-            g.d[0] = nu * 0.5f;
-            g.d[1] = 2.0f;
-            segs.push_back(g);
-            engine_of.push_back((int32_t)i);
+            alpha.push_back(nu * 0.5f);
         }
     }
+    std::stable_sort(rows.begin(), rows.end(), [](const GammaRowGroup& x, const GammaRowGroup& y) {
+        return (int64_t)x.rows * x.count > (int64_t)y.rows * y.count;
+    });
     int rc = run_std_normal_jobs(e, jobs, s, sc);
     if (rc) return rc;
-    rc = run_segments(e, segs, engine_of, s);
-    if (rc) return rc;
+    GammaRowGroup* d_rows;
+    float* d_alpha;
+    if (sc.upload(&d_rows, rows) || sc.upload(&d_alpha, alpha)) return ERD_ERR_HIP;
+    if (launch_gamma_rows(e->states, d_rows, (int32_t)n_inst, d_alpha, guard_of(e), s) != hipSuccess)
+        return ERD_ERR_HIP;
+    // T = L A with A built from the normals and chi-squares in the product's
+    // operand fetch
     std::vector<MvInstance> inst(n_inst);
     for (int64_t i = 0; i < n_inst; ++i)
         inst[i] = MvInstance{vd[i], 0, (int64_t)vd[i] * samples[i], f->lhi + f->split_off[i],
-                             f->llo + f->split_off[i], amat + aoff[i], tmat + aoff[i], nullptr};
+                             f->llo + f->split_off[i], nrm + nrm_off[i], tmat + aoff[i], nullptr, chi + chi_off[i]};
     MvInstance* d_inst;
-    int64_t *d_nrm_off, *d_chi_off, *d_aoff;
-    if (sc.upload(&d_inst, inst) || sc.upload(&d_nrm_off, nrm_off) || sc.upload(&d_chi_off, chi_off) ||
-        sc.upload(&d_aoff, aoff))
-        return ERD_ERR_HIP;
-    if (launch_wishart_assemble(d_inst, d_nrm_off, d_chi_off, nrm, chi, (int)n_inst, s) != hipSuccess)
-        return ERD_ERR_HIP;
-    rc = run_lz(inst, d_inst, s, sc);
+    int64_t* d_aoff;
+    if (sc.upload(&d_inst, inst) || sc.upload(&d_aoff, aoff)) return ERD_ERR_HIP;
+    rc = run_lz(inst, d_inst, s, sc, true);
     if (rc) return rc;
     // 64 x 64 output tiles of W (ta >= tb). All tiles of one sample go to
     // one XCD, so that its L2 fetches the sample's T once: workgroups are
