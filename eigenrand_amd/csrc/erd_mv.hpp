@@ -56,8 +56,6 @@ struct SyrkTile {
     int32_t sample;
     int32_t ta, tb;
 };
-hipError_t launch_wishart_assemble(const MvInstance* inst, const int64_t* nrm_off, const int64_t* chi_off,
-                                   const float* nrm, const float* chi, int n_inst, hipStream_t s);
 // W = T T^T per sample (Wishart), or W = U^T U with U lower triangular (utu).
 constexpr int kXcds = 8;  // XCDs of the device; workgroups are dispatched to them round-robin
 constexpr int kSyrkTile = 64;  // output tile edge of launch_syrk_tt (SyrkTile ta, tb in these units)

@@ -501,48 +501,6 @@ hipError_t launch_split_l(const double* l64, const int64_t* off, const int32_t* 
 namespace erd {
 
 // --------------------------------------------------------------- Wishart ---
-// rand_mat of WishartGen::generate (MvNormal.h:240-266) for every sample:
-// column i of sample j holds normals in rows i+1..d-1 (taken in order from the
-// instance's normal fill), sqrt(chi^2_i) on the diagonal, zeros above.
-// a: d x (d*n) column-major; nrm: n*d(d-1)/2 normals; chi: d rows of n.
-__global__ void k_wishart_assemble(const MvInstance* __restrict__ inst, const int64_t* __restrict__ nrm_off,
-                                   const int64_t* __restrict__ chi_off, const float* __restrict__ nrm,
-                                   const float* __restrict__ chi) {
-    const int i_inst = blockIdx.y;
-    const MvInstance in = inst[i_inst];
-    const int d = in.d;
-    const int64_t n = in.n / d;  // columns of a = d * samples
-    const int64_t total = (int64_t)d * d * n;
-    const int64_t per_sample = (int64_t)d * (d - 1) / 2;
-    float* A = const_cast<float*>(in.z);
-    // element t = (j * d + i) * d + row: the divisions once, then the grid
-    // stride S advances (row, i, j) by (S mod d, (S / d) mod d, S / d^2) with
-    // carries
-    const int64_t S = (int64_t)gridDim.x * blockDim.x;
-    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int row = (int)(t0 % d), i = (int)((t0 / d) % d);
-    int64_t j = t0 / d / d;
-    const int s_row = (int)(S % d), s_i = (int)((S / d) % d);
-    const int64_t s_j = S / d / d;
-    for (int64_t t = t0; t < total; t += S) {
-        float v = 0.0f;
-        if (row == i) {
-            v = sqrtf(chi[chi_off[i_inst] + (int64_t)i * n + j]);
-        } else if (row > i) {
-            // normals before column i of sample j: sum_{c<i} (d-1-c)
-            const int64_t before = (int64_t)i * (d - 1) - (int64_t)i * (i - 1) / 2;
-            v = nrm[nrm_off[i_inst] + j * per_sample + before + (row - i - 1)];
-        }
-        A[t] = v;
-        row += s_row;
-        int carry = 0;
-        if (row >= d) row -= d, carry = 1;
-        i += s_i + carry;
-        j += s_j;
-        if (i >= d) i -= d, ++j;
-    }
-}
-
 // W_j = T_j T_j^T (WishartGen, MvNormal.h:271-275) or U_j^T U_j (U = T^-1,
 // InvWishartGen, MvNormal.h:433-439) in FP32, per sample, over 64 x 64 output tiles with
 // ta >= tb (tile units of 64). A thread keeps a 4 x 4 block of the tile in
@@ -691,14 +649,6 @@ __global__ __launch_bounds__(64) void k_dirichlet_norm(const int32_t* __restrict
             for (int j = 0; j < ns; ++j) O[(s0 + j) * K + k0 + lane] = tile[lane][j];
         __syncthreads();
     }
-}
-
-hipError_t launch_wishart_assemble(const MvInstance* inst, const int64_t* nrm_off, const int64_t* chi_off,
-                                   const float* nrm, const float* chi, int n_inst, hipStream_t s) {
-    if (n_inst <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_wishart_assemble, dim3(256, (unsigned)n_inst), dim3(256), 0, s, inst, nrm_off,
-                       chi_off, nrm, chi);
-    return hipGetLastError();
 }
 
 hipError_t launch_syrk_tt(const SyrkTile* tiles, int n_tiles, const MvInstance* inst, float* w,

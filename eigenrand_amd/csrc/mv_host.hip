@@ -491,27 +491,36 @@ static int wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64
         // longest panels go first, and the LDS panel is sized for them
         // (all panels of a sample on one XCD, as the syrk tiles: they all
         // read the sample's T)
+        // Each bin lists its panels longest first (nb - p block rows): the
+        // samples are dealt to the bins first, then the panels are emitted by
+        // length.
         std::vector<std::vector<SyrkTile>> pbins(kXcds);
         std::vector<double> pload(kXcds, 0.0);
-        int max_rows = 0;
+        std::vector<int64_t> sbase(n_inst + 1, 0);
+        int max_nb = 0;
+        for (int64_t i = 0; i < n_inst; ++i) {
+            sbase[i + 1] = sbase[i] + samples[i];
+            max_nb = std::max(max_nb, (vd[i] + 31) / 32);
+        }
+        std::vector<uint8_t> xbin((size_t)sbase[n_inst]);
         for (int64_t i = 0; i < n_inst; ++i) {
             const int nb = (vd[i] + 31) / 32;
-            max_rows = std::max(max_rows, 32 * nb);
             for (int64_t j = 0; j < samples[i]; ++j) {
                 const int x = (int)(std::min_element(pload.begin(), pload.end()) - pload.begin());
-                for (int p = 0; p < nb; ++p) {
-                    pbins[x].push_back(SyrkTile{(int32_t)i, (int32_t)j, p, 0});
-                    pload[x] += (double)(nb - p);
-                }
+                xbin[(size_t)(sbase[i] + j)] = (uint8_t)x;
+                pload[x] += (double)nb * (nb + 1) / 2;
             }
         }
+        for (int len = max_nb; len >= 1; --len)
+            for (int64_t i = 0; i < n_inst; ++i) {
+                const int nb = (vd[i] + 31) / 32;
+                if (nb < len) continue;
+                for (int64_t j = 0; j < samples[i]; ++j)
+                    pbins[xbin[(size_t)(sbase[i] + j)]].push_back(SyrkTile{(int32_t)i, (int32_t)j, nb - len, 0});
+            }
+        const int max_rows = 32 * max_nb;
         size_t per_pbin = 0;
-        for (auto& b : pbins) {
-            std::stable_sort(b.begin(), b.end(), [&](const SyrkTile& a, const SyrkTile& c) {
-                return (vd[a.inst] + 31) / 32 - a.ta > (vd[c.inst] + 31) / 32 - c.ta;
-            });
-            per_pbin = std::max(per_pbin, b.size());
-        }
+        for (const auto& b : pbins) per_pbin = std::max(per_pbin, b.size());
         std::vector<SyrkTile> panels(per_pbin * kXcds, SyrkTile{-1, 0, 0, 0});
         for (int x = 0; x < kXcds; ++x)
             for (size_t m = 0; m < pbins[x].size(); ++m) panels[m * kXcds + x] = pbins[x][m];
