@@ -117,6 +117,14 @@ def timed(fn, reps=5, warmup=1):
     return a.elapsed_time(b) / reps
 
 
+def sweep_line(name, n, ms, bytes_per, **extra):
+    """One JSON line of the secondary measurements."""
+    print(json.dumps({"sweep": name, "elements": n, **extra, "ms": round(ms, 4),
+                      "gsamples_per_s": round(n / ms / 1e6, 3),
+                      "hbm_gbs": round(n * bytes_per / ms / 1e6, 1),
+                      "hbm_frac": round(n * bytes_per / ms / 1e6 / HBM_PEAK_GBS, 4)}), flush=True)
+
+
 def sweep():
     """Secondary measurements: the univariate sweep (2^28 f32 per distribution,
     4096 engines), vectorize-over-params (2^24 elements) and the batched
@@ -125,11 +133,7 @@ def sweep():
     import torch
     import eigenrand_amd as er
 
-    def emit(name, n, ms, bytes_per):
-        print(json.dumps({"sweep": name, "elements": n, "ms": round(ms, 4),
-                          "gsamples_per_s": round(n / ms / 1e6, 3),
-                          "hbm_gbs": round(n * bytes_per / ms / 1e6, 1),
-                          "hbm_frac": round(n * bytes_per / ms / 1e6 / HBM_PEAK_GBS, 4)}), flush=True)
+    emit = sweep_line
 
     urng = er.Vmt19937_64(0, n=N_ENGINES, stride=4)
     out = torch.empty(N_SAMPLES, dtype=torch.float32, device="cuda")
@@ -190,7 +194,19 @@ def sweep():
         emit(name, N_SAMPLES, timed(fn, reps=3), 8)
     del outd
 
-    # vectorize-over-params, 2^24 elements, parameters from mt19937_64(2024)
+    vop_sweep(emit)
+    torch.cuda.empty_cache()
+    mv_sweep(lambda d: print(json.dumps(d), flush=True))
+
+
+def vop_sweep(emit):
+    """Vectorize-over-params configuration: 2^24 elements with per-element
+    parameters from numpy MT19937(2024), on 4096 engines (4096 elements
+    each; the configuration does not fix the engine count). One line per
+    fill through emit(name, n, ms, bytes_per_element, engines=...)."""
+    import numpy as np
+    import torch
+    import eigenrand_amd as er
     nv = 1 << 24
     rng = np.random.Generator(np.random.MT19937(2024))
     vp = er.Vmt19937_64(0, n=N_ENGINES, stride=4)
@@ -203,13 +219,14 @@ def sweep():
     pm = torch.from_numpy(np.exp(rng.uniform(np.log(0.01), np.log(1e4), nv)).astype(np.float32)).cuda()
     vo = torch.empty(nv, dtype=torch.float32, device="cuda")
     vi = torch.empty(nv, dtype=torch.int32, device="cuda")
-    emit("VoP normal f32", nv, timed(lambda: er.normal_vop(vp, mu, sd, out=vo), reps=3), 12)
-    emit("VoP gamma f32", nv, timed(lambda: er.gamma_vop(vp, shape, ones, out=vo), reps=3), 12)
-    emit("VoP binomial i32", nv, timed(lambda: er.binomial_vop(vp, bn, bp, out=vi), reps=3), 12)
-    emit("VoP poisson i32", nv, timed(lambda: er.poisson_vop(vp, pm, out=vi), reps=3), 8)
-    del mu, sd, shape, ones, bn, bp, pm, vo, vi
-    torch.cuda.empty_cache()
-    mv_sweep(lambda d: print(json.dumps(d), flush=True))
+    for _ in range(100):  # clocks to their steady state before the first row
+        er.normal_vop(vp, mu, sd, out=vo)
+    torch.cuda.synchronize()
+    emit("VoP normal f32", nv, timed(lambda: er.normal_vop(vp, mu, sd, out=vo), reps=3), 12, engines=N_ENGINES)
+    emit("VoP gamma f32", nv, timed(lambda: er.gamma_vop(vp, shape, ones, out=vo), reps=3), 12, engines=N_ENGINES)
+    emit("VoP binomial i32", nv, timed(lambda: er.binomial_vop(vp, bn, bp, out=vi), reps=3), 12,
+         engines=N_ENGINES)
+    emit("VoP poisson i32", nv, timed(lambda: er.poisson_vop(vp, pm, out=vi), reps=3), 8, engines=N_ENGINES)
 
 
 def serving_workload(n_req, n_clients, seed=5):
@@ -489,6 +506,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--sweep", action="store_true", help="secondary per-distribution measurements")
     ap.add_argument("--mv", action="store_true", help="only the batched multivariate measurements")
+    ap.add_argument("--vop", action="store_true", help="only the vectorize-over-params measurements")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving (p50/p99) leg")
     ap.add_argument("--serve-requests", type=int, default=5000)
     ap.add_argument("--serve-only", action="store_true", help="only the serving leg (one JSON line)")
@@ -517,6 +535,9 @@ def main():
         return
     if args.mv:
         mv_sweep(lambda d: print(json.dumps(d), flush=True))
+        return
+    if args.vop:
+        vop_sweep(sweep_line)
         return
     if args.serve_only:
         print(json.dumps({"serving": serving(n_req=args.serve_requests)}), flush=True)

@@ -76,6 +76,16 @@ struct Guard {
 constexpr int32_t kDefaultRejectCap = 1 << 22;
 constexpr int32_t kDefaultLockstepCap = 1 << 20;
 
+// Engine (or output block) of workgroup b in a launch of n workgroups, one
+// per engine. Workgroups are dealt to the eight XCDs round-robin (b on XCD
+// b mod 8); when n is a multiple of 8, XCD x takes the contiguous engines
+// [x*n/8, (x+1)*n/8), so each XCD writes one contiguous region. Measured for
+// 4096 blocks of 256 KiB: plain 16-byte stores 71% -> 74% of 8 TB/s, the
+// fast normal fill 58.5% -> 59.8%.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t n) {
+    return (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3);
+}
+
 __device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t as_u(float f) { return __float_as_uint(f); }
 

@@ -25,7 +25,7 @@ __global__ __launch_bounds__(64) void k_fill_fixed(EngineState* __restrict__ sta
     __shared__ uint64_t mt[kWords];
     __shared__ uint64_t carry[(C > 1 ? C - 1 : 1) * kLanes];
     const int lane = threadIdx.x;
-    const int64_t g = blockIdx.x;
+    const int64_t g = xcd_block(blockIdx.x, gridDim.x);
     if (D::units(per) == 0) return;
     Eng e;
     eng_load(e, mt, states + g, lane);
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(64) void k_fill_fixed2(EngineState* __restrict__ st
     __shared__ uint64_t mt[kWords];
     __shared__ uint64_t carry[(C > 1 ? C - 1 : 1) * kLanes];
     const int lane = threadIdx.x;
-    const int64_t g = blockIdx.x;
+    const int64_t g = xcd_block(blockIdx.x, gridDim.x);
     if (D::units(per) == 0) return;
     Eng e;
     eng_load(e, mt, states + g, lane);
@@ -52,11 +52,12 @@ __global__ __launch_bounds__(64) void k_fill_fixed2(EngineState* __restrict__ st
 }
 
 // Plain 16-byte stores of a constant, one workgroup of 64 threads per
-// contiguous block of `per` floats: the store pattern of the fills without
-// their arithmetic (the practical HBM-write ceiling they are measured
-// against). per is a multiple of 4.
+// contiguous block of `per` floats, blocks placed on the XCDs as the fills
+// place their engines: the store pattern of the fills without their
+// arithmetic (the practical HBM-write ceiling they are measured against).
+// per is a multiple of 4.
 __global__ __launch_bounds__(64) void k_store_blocks(float* __restrict__ out, int64_t per, float v) {
-    float4* o = reinterpret_cast<float4*>(out + (int64_t)blockIdx.x * per);
+    float4* o = reinterpret_cast<float4*>(out + xcd_block(blockIdx.x, gridDim.x) * per);
     const int64_t n4 = per / 4;
     const float4 x = make_float4(v, v, v, v);
     for (int64_t i = threadIdx.x; i < n4; i += 64) o[i] = x;
