@@ -66,11 +66,17 @@ class Vmt19937_64:
     Engine g is `Vmt19937_64(seed + g*stride)`: its 4 MT19937-64 lanes are
     seeded seed+g*stride .. seed+g*stride+3 (PacketRandomEngine.h:230-239).
     With the default stride 4, n engines cover lanes seed .. seed+4n-1.
+
+    `layout` is the float packet width of the reference build being matched:
+    8 (AVX2, the default), 4 (SSE2/NEON: the 2-lane Vmt19937_64 seeded x0,
+    x0+1, i.e. lanes 0-1 of these engines) or 16 (AVX-512: packets of two
+    calls). At 4 and 16 only the fills named at erd_engines_set_layout have
+    a form; the others raise ErdError(ERD_ERR_UNSUPPORTED).
     """
 
     default_seed = 5489
 
-    def __init__(self, seed=default_seed, n=1, stride=4, seeds=None, stream=None, _handle=None,
+    def __init__(self, seed=default_seed, n=1, stride=4, seeds=None, stream=None, layout=8, _handle=None,
                  _parent=None):
         self._parent = _parent
         if _handle is not None:
@@ -85,6 +91,16 @@ class Vmt19937_64:
             _lib.call("erd_engines_create_range", C.byref(h), int(seed) % (1 << 64),
                       int(stride) % (1 << 64), int(n), _stream_ptr(stream))
         self._h = h
+        if layout != 8:
+            self.set_layout(layout)
+
+    @property
+    def layout(self):
+        """Float packet width of the fills on this set (4, 8 or 16)."""
+        return int(_lib.load().erd_engines_layout(self._h))
+
+    def set_layout(self, layout):
+        _lib.call("erd_engines_set_layout", self._h, int(layout))
 
     @property
     def handle(self):
@@ -120,7 +136,7 @@ class Vmt19937_64:
 
     def copy(self, stream=None):
         """An independent engine set in the same state (`auto c = urng;`)."""
-        out = Vmt19937_64(seed=0, n=len(self), stream=stream)
+        out = Vmt19937_64(seed=0, n=len(self), stream=stream, layout=self.layout)
         _lib.call("erd_engines_copy", out._h, self._h, _stream_ptr(stream))
         return out
 
@@ -300,9 +316,10 @@ def _fill(name, urng, rows, cols, dtype, out, stream, *params):
 
 
 def randbits_u64(draws, urng, out=None, stream=None):
-    """`draws` calls of Vmt19937_64::operator(); returns (n, 4*draws) uint64."""
+    """`draws` calls of Vmt19937_64::operator(); returns (n, 4*draws) uint64
+    ((n, 2*draws) at layout 4, whose calls are two lanes wide)."""
     torch = _torch()
-    per = 4 * int(draws)
+    per = (2 if urng.layout == 4 else 4) * int(draws)
     buf = _alloc(urng, per, torch.uint64, out)
     _lib.call("erd_fill_u64", urng.handle, _ptr(buf), per, _stream_ptr(stream))
     return buf.view(len(urng), per)

@@ -45,6 +45,7 @@ _SIGS = {
     "erd_engines_view": [C.POINTER(P), P, I64, I64],
     "erd_engines_destroy": [P],
     "erd_engines_count": [P],
+    "erd_engines_layout": [P],
     "erd_engines_device_ptr": [P],
     "erd_engines_snapshot": [P, P, P],
     "erd_engines_restore": [P, P, P],
@@ -52,6 +53,7 @@ _SIGS = {
     "erd_engines_discard": [P, I64, P],
     "erd_engines_status": [P, C.POINTER(C.c_uint32), P],
     "erd_engines_set_iteration_caps": [P, C.c_int32, C.c_int32],
+    "erd_engines_set_layout": [P, C.c_int],
     "erd_fill_u64": [P, P, I64, P],
     "erd_engines_create_p8": [C.POINTER(P), P, I64, P],
     "erd_p8_fill_u64": [P, P, I64, P],

@@ -30,9 +30,26 @@ inline bool valid_set(const erd_engines* e) { return e && e->states && e->count 
 // and a device output pointer whenever something is written.
 inline int check_fill(const erd_engines* e, const void* out, int64_t per, int64_t width) {
     if (!valid_set(e)) return ERD_ERR_INVALID;
+    if (e->layout != 8) return ERD_ERR_UNSUPPORTED;
     if (per < 0 || per % width != 0) return ERD_ERR_INVALID;
     if (per > 0 && !out) return ERD_ERR_INVALID;
     return ERD_OK;
+}
+
+// The same for a fill with a form at every lane layout: its packet width at
+// layouts 4, 8 and 16.
+inline int check_fill_l(const erd_engines* e, const void* out, int64_t per, int64_t w4, int64_t w8, int64_t w16) {
+    if (!valid_set(e)) return ERD_ERR_INVALID;
+    const int64_t width = e->layout == 4 ? w4 : e->layout == 16 ? w16 : w8;
+    if (per < 0 || per % width != 0) return ERD_ERR_INVALID;
+    if (per > 0 && !out) return ERD_ERR_INVALID;
+    return ERD_OK;
+}
+
+// A fill at layout 4, or at layout 16 where its element order differs.
+inline int layout_fill(erd_engines* e, int kind, void* out, int64_t per, double a, double b, int mode,
+                       void* stream) {
+    return hip_status(erd::launch_fill_layout(e->states, e->count, e->layout, kind, out, per, a, b, mode, S(stream)));
 }
 
 }  // namespace
@@ -102,6 +119,7 @@ int erd_engines_view(erd_engines** out, erd_engines* parent, int64_t first, int6
     e->status = parent->status;
     e->reject_cap = parent->reject_cap;
     e->lockstep_cap = parent->lockstep_cap;
+    e->layout = parent->layout;
     e->count = n;
     e->owns = false;
     *out = e;
@@ -176,9 +194,18 @@ int erd_engines_status(erd_engines* e, uint32_t* status_out, void* stream) {
     return hip_status(err);
 }
 
+int erd_engines_set_layout(erd_engines* e, int layout) {
+    if (!valid_set(e) || (layout != 4 && layout != 8 && layout != 16)) return ERD_ERR_INVALID;
+    e->layout = layout;
+    return ERD_OK;
+}
+
+int erd_engines_layout(const erd_engines* e) { return valid_set(e) ? e->layout : 0; }
+
 int erd_fill_u64(erd_engines* e, uint64_t* out, int64_t per, void* stream) {
-    int rc = check_fill(e, out, per, 4);
+    int rc = check_fill_l(e, out, per, 2, 4, 4);
     if (rc) return rc;
+    if (e->layout == 4) return layout_fill(e, erd::LK_U64, out, per, 0, 0, 0, stream);
     return hip_status(erd::launch_fill_u64(e->states, e->count, out, per, S(stream)));
 }
 
@@ -254,52 +281,60 @@ int erd_p8_fill_uniform_f32(erd_engines* e, float* out, int64_t per, void* strea
 }
 
 int erd_fill_uniform_f32(erd_engines* e, float* out, int64_t per, float a, float b, void* stream) {
-    int rc = check_fill(e, out, per, 8);
+    int rc = check_fill_l(e, out, per, 4, 8, 16);
     if (rc) return rc;
     const int affine = !(a == 0.0f && b == 1.0f);
+    if (e->layout == 4) return layout_fill(e, erd::LK_UNIFORM_F32, out, per, a, b, affine, stream);
     return hip_status(erd::launch_fill_uniform_f32(e->states, e->count, out, per, a, b, affine, S(stream)));
 }
 
 int erd_fill_uniform_f64(erd_engines* e, double* out, int64_t per, double a, double b, void* stream) {
-    int rc = check_fill(e, out, per, 4);
+    int rc = check_fill_l(e, out, per, 2, 4, 8);
     if (rc) return rc;
     const int affine = !(a == 0.0 && b == 1.0);
+    if (e->layout == 4) return layout_fill(e, erd::LK_UNIFORM_F64, out, per, a, b, affine, stream);
     return hip_status(erd::launch_fill_uniform_f64(e->states, e->count, out, per, a, b, affine, S(stream)));
 }
 
 int erd_fill_std_normal_f64(erd_engines* e, double* out, int64_t per, void* stream) {
-    int rc = check_fill(e, out, per, 4);
+    int rc = check_fill_l(e, out, per, 2, 4, 8);
     if (rc) return rc;
+    if (e->layout != 8) return layout_fill(e, erd::LK_NORMAL_F64, out, per, 0.0, 1.0, 0, stream);
     return hip_status(erd::launch_fill_normal_f64(e->states, e->count, out, per, 0.0, 1.0, 0, S(stream)));
 }
 
 int erd_fill_normal_f64(erd_engines* e, double* out, int64_t per, double mean, double sd, void* stream) {
-    int rc = check_fill(e, out, per, 4);
+    int rc = check_fill_l(e, out, per, 2, 4, 8);
     if (rc) return rc;
+    if (e->layout != 8) return layout_fill(e, erd::LK_NORMAL_F64, out, per, mean, sd, 1, stream);
     return hip_status(erd::launch_fill_normal_f64(e->states, e->count, out, per, mean, sd, 1, S(stream)));
 }
 
 int erd_fill_exponential_f64(erd_engines* e, double* out, int64_t per, double lambda, void* stream) {
-    int rc = check_fill(e, out, per, 4);
+    int rc = check_fill_l(e, out, per, 2, 4, 8);
     if (rc) return rc;
+    if (e->layout == 4) return layout_fill(e, erd::LK_EXPONENTIAL_F64, out, per, lambda, 0, 0, stream);
     return hip_status(erd::launch_fill_exponential_f64(e->states, e->count, out, per, lambda, S(stream)));
 }
 
 int erd_fill_cauchy_f64(erd_engines* e, double* out, int64_t per, double a, double b, void* stream) {
-    int rc = check_fill(e, out, per, 4);
+    int rc = check_fill_l(e, out, per, 2, 4, 8);
     if (rc) return rc;
+    if (e->layout == 4) return layout_fill(e, erd::LK_CAUCHY_F64, out, per, a, b, 0, stream);
     return hip_status(erd::launch_fill_cauchy_f64(e->states, e->count, out, per, a, b, S(stream)));
 }
 
 int erd_fill_std_normal_f32(erd_engines* e, float* out, int64_t per, void* stream) {
-    int rc = check_fill(e, out, per, 8);
+    int rc = check_fill_l(e, out, per, 4, 8, 16);
     if (rc) return rc;
+    if (e->layout != 8) return layout_fill(e, erd::LK_NORMAL_F32, out, per, 0.0, 1.0, 0, stream);
     return hip_status(erd::launch_fill_normal_f32(e->states, e->count, out, per, 0.f, 1.f, 0, S(stream)));
 }
 
 int erd_fill_normal_f32(erd_engines* e, float* out, int64_t per, float mean, float sd, void* stream) {
-    int rc = check_fill(e, out, per, 8);
+    int rc = check_fill_l(e, out, per, 4, 8, 16);
     if (rc) return rc;
+    if (e->layout != 8) return layout_fill(e, erd::LK_NORMAL_F32, out, per, mean, sd, 1, stream);
     return hip_status(erd::launch_fill_normal_f32(e->states, e->count, out, per, mean, sd, 1, S(stream)));
 }
 
@@ -348,38 +383,44 @@ int erd_fill_normal_vop_sv_f32(erd_engines* e, float* out, int64_t per, float me
 }
 
 int erd_fill_exponential_f32(erd_engines* e, float* out, int64_t per, float lambda, void* stream) {
-    int rc = check_fill(e, out, per, 8);
+    int rc = check_fill_l(e, out, per, 4, 8, 16);
     if (rc) return rc;
+    if (e->layout == 4) return layout_fill(e, erd::LK_EXPONENTIAL_F32, out, per, lambda, 0, 0, stream);
     return hip_status(erd::launch_fill_exponential_f32(e->states, e->count, out, per, lambda, S(stream)));
 }
 
 int erd_fill_lognormal_f32(erd_engines* e, float* out, int64_t per, float mean, float sd, void* stream) {
-    int rc = check_fill(e, out, per, 8);
+    int rc = check_fill_l(e, out, per, 4, 8, 16);
     if (rc) return rc;
+    if (e->layout != 8) return layout_fill(e, erd::LK_NORMAL_F32, out, per, mean, sd, 3, stream);
     return hip_status(erd::launch_fill_lognormal_f32(e->states, e->count, out, per, mean, sd, S(stream)));
 }
 
 int erd_fill_weibull_f32(erd_engines* e, float* out, int64_t per, float a, float b, void* stream) {
-    int rc = check_fill(e, out, per, 8);
+    int rc = check_fill_l(e, out, per, 4, 8, 16);
     if (rc) return rc;
+    if (e->layout == 4) return layout_fill(e, erd::LK_WEIBULL_F32, out, per, a, b, 0, stream);
     return hip_status(erd::launch_fill_weibull_f32(e->states, e->count, out, per, a, b, S(stream)));
 }
 
 int erd_fill_extreme_value_f32(erd_engines* e, float* out, int64_t per, float a, float b, void* stream) {
-    int rc = check_fill(e, out, per, 8);
+    int rc = check_fill_l(e, out, per, 4, 8, 16);
     if (rc) return rc;
+    if (e->layout == 4) return layout_fill(e, erd::LK_EXTREME_VALUE_F32, out, per, a, b, 0, stream);
     return hip_status(erd::launch_fill_extreme_value_f32(e->states, e->count, out, per, a, b, S(stream)));
 }
 
 int erd_fill_cauchy_f32(erd_engines* e, float* out, int64_t per, float a, float b, void* stream) {
-    int rc = check_fill(e, out, per, 8);
+    int rc = check_fill_l(e, out, per, 4, 8, 16);
     if (rc) return rc;
+    if (e->layout == 4) return layout_fill(e, erd::LK_CAUCHY_F32, out, per, a, b, 0, stream);
     return hip_status(erd::launch_fill_cauchy_f32(e->states, e->count, out, per, a, b, S(stream)));
 }
 
 int erd_fill_studentt_f32(erd_engines* e, float* out, int64_t per, float n, void* stream) {
-    int rc = check_fill(e, out, per, 8);
+    int rc = check_fill_l(e, out, per, 4, 8, 16);
     if (rc) return rc;
+    if (e->layout != 8) return layout_fill(e, erd::LK_STUDENT_T_F32, out, per, n, 0, 0, stream);
     return hip_status(erd::launch_fill_studentt_f32(e->states, e->count, out, per, n, S(stream)));
 }
 
@@ -532,6 +573,7 @@ int erd_generator_reset(erd_generator* g, void* stream) {
 }
 
 int erd_generator_fill(erd_generator* g, erd_engines* e, void* out, int64_t per, void* stream) {
+    if (layout_unsupported(e)) return ERD_ERR_UNSUPPORTED;
     if (!g || !e || e->count != g->n) return ERD_ERR_INVALID;
     switch (g->dist) {
         case ERD_DIST_STD_NORMAL_F32:
@@ -963,5 +1005,6 @@ int run_requests(erd_engines* e, const erd_request* reqs, int64_t n, hipStream_t
 }  // namespace erd
 
 extern "C" int erd_run_requests(erd_engines* e, const erd_request* requests, int64_t n, void* stream) {
+    if (layout_unsupported(e)) return ERD_ERR_UNSUPPORTED;
     return erd::run_requests(e, requests, n, S(stream));
 }

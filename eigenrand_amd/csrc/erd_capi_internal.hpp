@@ -11,7 +11,11 @@ struct erd_engines {
     bool owns = false;                   // views share the parent's memory
     int32_t reject_cap = erd::kDefaultRejectCap;      // per-launch iteration caps (erd::Guard)
     int32_t lockstep_cap = erd::kDefaultLockstepCap;
+    int32_t layout = 8;  // float packet width of the fills: 4 (SSE2/NEON), 8 (AVX2) or 16 (AVX-512)
 };
+
+// Calls that exist only at the AVX2 layout refuse a set with another one.
+inline bool layout_unsupported(const erd_engines* e) { return e && e->layout != 8; }
 
 // Status word and iteration caps of the launches on an engine set.
 inline erd::Guard guard_of(const erd_engines* e) { return {e->status, e->reject_cap, e->lockstep_cap}; }

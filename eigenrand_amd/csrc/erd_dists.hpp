@@ -18,6 +18,7 @@ struct RawU64 {  // out[4u + l] = lane l of call u (PacketRandomEngine.h:295-310
     uint64_t* out;
     __device__ void rebase(int64_t off) { out += off; }
     __device__ static int64_t units(int64_t n) { return n / 4; }
+    __device__ uint64_t value(uint64_t w) const { return w; }
     __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
         out[4 * u + l] = w[0];
     }
@@ -30,10 +31,12 @@ struct UniformF32 {  // Basic.h:584-590 / 626-634; float j of a call = word j
     int affine;
     __device__ void rebase(int64_t off) { out += off; }
     __device__ static int64_t units(int64_t n) { return n / 8; }
+    __device__ float value(uint32_t x) const {
+        const float v = ur_f32(x);
+        return affine ? v * slope + a : v;
+    }
     __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
-        float u0 = ur_f32(lo32(w[0])), u1 = ur_f32(hi32(w[0]));
-        if (affine) { u0 = u0 * slope + a; u1 = u1 * slope + a; }
-        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(u0, u1);
+        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(value(lo32(w[0])), value(hi32(w[0])));
     }
     // floats 4k .. 4k+3 of the call (lanes 2k, 2k+1) in one 16-byte store
     __device__ void emit2(int64_t per, int64_t u, int k, const uint64_t (&w)[1][2]) const {
@@ -53,10 +56,12 @@ struct UniformF64 {  // arch/AVX/RandUtils.h:212-225: 4 doubles per call
     int affine;
     __device__ void rebase(int64_t off) { out += off; }
     __device__ static int64_t units(int64_t n) { return n / 4; }
+    __device__ double value(uint64_t w) const {
+        const double v = ur_f64(w);
+        return affine ? v * slope + a : v;
+    }
     __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
-        double v = ur_f64(w[0]);
-        if (affine) v = v * slope + a;
-        out[4 * u + l] = v;
+        out[4 * u + l] = value(w[0]);
     }
 };
 
@@ -93,8 +98,9 @@ struct ExponentialF64 {  // NormalExp.h:391-398 at Packet4d
     double lambda;
     __device__ void rebase(int64_t off) { out += off; }
     __device__ static int64_t units(int64_t n) { return n / 4; }
+    __device__ double value(uint64_t w) const { return -(plog_pos_f64(1.0 - ur_f64(w)) / lambda); }
     __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
-        out[4 * u + l] = -(plog_pos_f64(1.0 - ur_f64(w[0])) / lambda);
+        out[4 * u + l] = value(w[0]);
     }
 };
 
@@ -104,10 +110,13 @@ struct CauchyF64 {  // NormalExp.h:843-855 at Packet4d
     double a, b;
     __device__ void rebase(int64_t off) { out += off; }
     __device__ static int64_t units(int64_t n) { return n / 4; }
-    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+    __device__ double value(uint64_t w) const {
         double s, c;
-        psincos_f64(kPi * (ur_f64(w[0]) - 0.5), s, c);
-        out[4 * u + l] = a + b * (s / c);
+        psincos_f64(kPi * (ur_f64(w) - 0.5), s, c);
+        return a + b * (s / c);
+    }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        out[4 * u + l] = value(w[0]);
     }
 };
 
@@ -258,14 +267,12 @@ struct WeibullF32 {  // NormalExp.h:649-655: b * exp(log(-log(1 - U)) * (1/a))
     float inv_a, b;  // 1/a rounded in float, as the reference's pset1(1 / a)
     __device__ void rebase(int64_t off) { out += off; }
     __device__ static int64_t units(int64_t n) { return n / 8; }
+    __device__ float value(uint32_t w) const {
+        const float x = 1.0f - ur_f32(w);
+        return pexp(plog(-plog(x)) * inv_a) * b;
+    }
     __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
-        float v[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const float x = 1.0f - ur_f32(h ? hi32(w[0]) : lo32(w[0]));
-            v[h] = pexp(plog(-plog(x)) * inv_a) * b;
-        }
-        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
+        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(value(lo32(w[0])), value(hi32(w[0])));
     }
 };
 
@@ -275,14 +282,9 @@ struct ExtremeValueF32 {  // NormalExp.h:725-731: a - log(-log(U + eps/8)) * b
     float a, b;
     __device__ void rebase(int64_t off) { out += off; }
     __device__ static int64_t units(int64_t n) { return n / 8; }
+    __device__ float value(uint32_t w) const { return a - plog(-plog(nzur_f32(w))) * b; }
     __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
-        float v[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const float x = nzur_f32(h ? hi32(w[0]) : lo32(w[0]));
-            v[h] = a - plog(-plog(x)) * b;
-        }
-        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
+        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(value(lo32(w[0])), value(hi32(w[0])));
     }
 };
 
@@ -294,14 +296,12 @@ struct ExponentialF32 {  // NormalExp.h:391-398
     int gamma;
     __device__ void rebase(int64_t off) { out += off; }
     __device__ static int64_t units(int64_t n) { return n / 8; }
+    __device__ float value(uint32_t w) const {
+        const float x = -(plog_pos(1.0f - ur_f32(w)) / lambda);
+        return gamma ? scale * x : x;
+    }
     __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
-        float v[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            float x = -(plog_pos(1.0f - ur_f32(h ? hi32(w[0]) : lo32(w[0]))) / lambda);
-            v[h] = gamma ? scale * x : x;
-        }
-        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
+        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(value(lo32(w[0])), value(hi32(w[0])));
     }
 };
 
@@ -311,15 +311,13 @@ struct CauchyF32 {  // NormalExp.h:843-855
     float a, b;
     __device__ void rebase(int64_t off) { out += off; }
     __device__ static int64_t units(int64_t n) { return n / 8; }
+    __device__ float value(uint32_t w) const {
+        float s, c;
+        psincos(ERD_F(kPi) * (ur_f32(w) - 0.5f), s, c);
+        return a + b * (s / c);
+    }
     __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
-        float v[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            float s, c;
-            psincos(ERD_F(kPi) * (ur_f32(h ? hi32(w[0]) : lo32(w[0])) - 0.5f), s, c);
-            v[h] = a + b * (s / c);
-        }
-        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
+        *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(value(lo32(w[0])), value(hi32(w[0])));
     }
 };
 
@@ -341,6 +339,161 @@ struct StudentTF32 {  // NormalExp.h:285-301 (sine from the reference's psincos)
             v[h] = radius * s;
         }
         *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(v[0], v[1]);
+    }
+};
+
+// ------------------------------------------------------------ lane layouts ---
+// The reference's packet width follows the ISA it is built for
+// (PacketRandomEngine.h:585-588; arch/SSE/RandUtils.h:70-115;
+// arch/AVX512/RandUtils.h:37-46). Besides the AVX2 layout above (W = 8
+// floats per packet, one 4-lane call), two more are built:
+//  - W = 4, SSE2 / NEON: Vmt19937_64 = Pmt19937_64<Packet4i>, a 2-lane engine
+//    seeded x0, x0+1 = lanes 0-1 of the 4-lane record seeded x0. A call is
+//    u64 lanes 0-1 of a draw (lanes 2-3 are stepped and never read): 4
+//    floats, 2 doubles or 2 raw u64 per packet.
+//  - W = 16, AVX-512: the AVX2 engine; a Packet16f / Packet8d is two calls,
+//    the first in the low half (`_mm512_inserti64x4(cast(rng()), rng(), 1)`
+//    with its arguments evaluated left to right, as the reference's default
+//    Clang preset does). Element-wise fills therefore equal the AVX2 ones;
+//    only samplers that pair two packets (Box-Muller, Student-t) differ.
+// Items read u64 lanes 0..LANES-1 of each call (op_fixed).
+template <int W>
+struct Layout {
+    static constexpr int LANES = W == 4 ? 2 : 4;  // u64 lanes read per call
+    static constexpr int FPC = 2 * LANES;         // floats per call
+    static constexpr int PC = W / FPC;            // calls per float packet
+};
+
+// Element-wise float distribution E (E::value of a 32-bit word) at W = 4:
+// floats 4k..4k+3 of a block are the words of call k.
+template <class E>
+struct ElemF32W4 {
+    static constexpr int CALLS = 1, LANES = 2;
+    E d;
+    __device__ void rebase(int64_t off) { d.out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 4; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        *reinterpret_cast<float2*>(d.out + 4 * u + 2 * l) = make_float2(d.value(lo32(w[0])), d.value(hi32(w[0])));
+    }
+};
+
+// Element-wise 64-bit distribution E (raw u64 or a double per u64 lane) at
+// W = 4: elements 2k, 2k+1 of a block are lanes 0-1 of call k.
+template <class E>
+struct ElemU64W4 {
+    static constexpr int CALLS = 1, LANES = 2;
+    E d;
+    __device__ void rebase(int64_t off) { d.out += off; }
+    __device__ static int64_t units(int64_t n) { return n / 2; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        d.out[2 * u + l] = d.value(w[0]);
+    }
+};
+
+// Box-Muller pair of the exact normal fills: words a (u1 = 1 - U) and b (u2).
+__device__ __forceinline__ void box_muller_f32(uint32_t a, uint32_t b, float& sn, float& cs) {
+    const float u1 = one_minus_ur_f32(a);
+    const float u2 = ur_f32(b);
+    const float radius = sqrt_cr_normal(-2.0f * plog_pos(u1));
+    float s, c;
+    psincos_pos(ERD_F(2 * kPi) * u2, s, c);
+    sn = radius * s;
+    cs = radius * c;
+}
+
+// StdNormalGen / NormalGen / LognormalGen (NormalExp.h:60-82, 127-135,
+// 197-210) at W = 4 or 16: u1 is the packet of calls [0, PC), u2 that of
+// calls [PC, 2PC); a unit is the sine packet then the cached cosine packet
+// (2W outputs), a trailing half unit the sine packet alone.
+// MODE 0 standard, 1 z * sd + mean, 3 exp(z * sd + mean).
+template <int W, int MODE>
+struct NormalF32L {
+    using LY = Layout<W>;
+    static constexpr int CALLS = 2 * LY::PC, LANES = LY::LANES;
+    float* out;
+    float mean, sd;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return (n + 2 * W - 1) / (2 * W); }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        const bool full = 2 * W * u + 2 * W <= per;
+#pragma unroll
+        for (int c = 0; c < LY::PC; ++c) {
+            float sn[2], cs[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                box_muller_f32(h ? hi32(w[c]) : lo32(w[c]), h ? hi32(w[LY::PC + c]) : lo32(w[LY::PC + c]), sn[h],
+                               cs[h]);
+                if (MODE == 1) {
+                    sn[h] = sn[h] * sd + mean;
+                    cs[h] = cs[h] * sd + mean;
+                } else if (MODE == 3) {
+                    sn[h] = pexp(sn[h] * sd + mean);
+                    cs[h] = pexp(cs[h] * sd + mean);
+                }
+            }
+            float* o = out + 2 * W * u + LY::FPC * c + 2 * l;
+            *reinterpret_cast<float2*>(o) = make_float2(sn[0], sn[1]);
+            if (full) *reinterpret_cast<float2*>(o + W) = make_float2(cs[0], cs[1]);
+        }
+    }
+};
+
+// StudentTGen (NormalExp.h:285-301) at W = 4 or 16: u1 from calls [0, PC),
+// u2 from [PC, 2PC), W outputs per unit, nothing cached.
+template <int W>
+struct StudentTF32L {
+    using LY = Layout<W>;
+    static constexpr int CALLS = 2 * LY::PC, LANES = LY::LANES;
+    float* out;
+    float n, k;  // k = -2/n in float
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n_) { return n_ / W; }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+#pragma unroll
+        for (int c = 0; c < LY::PC; ++c) {
+            float v[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const float u1 = 1.0f - ur_f32(h ? hi32(w[c]) : lo32(w[c]));
+                const float u2 = ur_f32(h ? hi32(w[LY::PC + c]) : lo32(w[LY::PC + c]));
+                const float radius = sqrtf(n * (pexp(plog_pos(u1) * k) - 1.0f));
+                float s, cc;
+                psincos_pos(ERD_F(2 * kPi) * u2, s, cc);
+                v[h] = radius * s;
+            }
+            *reinterpret_cast<float2*>(out + W * u + LY::FPC * c + 2 * l) = make_float2(v[0], v[1]);
+        }
+    }
+};
+
+// StdNormalGen / NormalGen<double> at W = 4 (Packet2d) or 16 (Packet8d):
+// PD = W/2 doubles per packet, LANES per call; u1 from calls [0, PC), u2
+// from [PC, 2PC); a unit is 2PD outputs (sine packet, cosine packet).
+template <int W, int MODE>
+struct NormalF64L {
+    static constexpr int LANES = Layout<W>::LANES, PD = W / 2, PC = PD / LANES, CALLS = 2 * PC;
+    double* out;
+    double mean, sd;
+    __device__ void rebase(int64_t off) { out += off; }
+    __device__ static int64_t units(int64_t n) { return (n + 2 * PD - 1) / (2 * PD); }
+    __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
+        const bool full = 2 * PD * u + 2 * PD <= per;
+#pragma unroll
+        for (int c = 0; c < PC; ++c) {
+            const double u1 = 1.0 - ur_f64(w[c]);  // in [2^-52, 1], exact
+            const double u2 = ur_f64(w[PC + c]);
+            const double radius = sqrt(-2.0 * plog_pos_f64(u1));
+            double s, cc;
+            psincos_f64((2.0 * kPi) * u2, s, cc);
+            double sn = radius * s, cs = radius * cc;
+            if (MODE == 1) {
+                sn = sn * sd + mean;
+                cs = cs * sd + mean;
+            }
+            const int64_t i = 2 * PD * u + LANES * c + l;
+            out[i] = sn;
+            if (full) out[i + PD] = cs;
+        }
     }
 };
 

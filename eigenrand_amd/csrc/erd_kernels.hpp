@@ -53,6 +53,25 @@ struct GammaRowGroup {
 hipError_t launch_gamma_rows(EngineState* st, const GammaRowGroup* rows, int32_t n_groups, const float* alpha,
                              Guard gd, hipStream_t s);
 hipError_t launch_fill_u64(EngineState* st, int64_t n, uint64_t* out, int64_t per, hipStream_t s);
+// Fills at the SSE2/NEON (W = 4) and AVX-512 (W = 16) lane layouts whose
+// element order differs from the AVX2 one. `mode`: uniform 0 = [0,1),
+// 1 = a + u (b - a); normal 0 standard, 1 z*b + a (mean a, sd b), 3 lognormal.
+enum LayoutKind {
+    LK_U64 = 0,
+    LK_UNIFORM_F32,
+    LK_UNIFORM_F64,
+    LK_NORMAL_F32,
+    LK_EXPONENTIAL_F32,
+    LK_CAUCHY_F32,
+    LK_WEIBULL_F32,
+    LK_EXTREME_VALUE_F32,
+    LK_STUDENT_T_F32,
+    LK_NORMAL_F64,
+    LK_EXPONENTIAL_F64,
+    LK_CAUCHY_F64
+};
+hipError_t launch_fill_layout(EngineState* st, int64_t n, int W, int kind, void* out, int64_t per, double a,
+                              double b, int mode, hipStream_t s);
 // P8 adaptor fills: n Vmt engines (n/2 adaptors), per_v outputs per Vmt
 // engine; mode 0 u64, 1 u32, 2 uniform_real() floats.
 hipError_t launch_fill_p8(EngineState* st, int64_t n, void* out, int64_t per_v, int mode,

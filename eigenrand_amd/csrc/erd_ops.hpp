@@ -173,10 +173,23 @@ __device__ __forceinline__ void eng_store(const Eng& e, EngineState* st, int lan
 
 // ---------------------------------------------------------------- fixed ---
 // D: CALLS draws per unit; units(per); emit(per, u, l, w) writes the outputs
-// of unit u from the tempered words w[0..CALLS) of lane l.
+// of unit u from the tempered words w[0..CALLS) of lane l. A distribution
+// that reads only u64 lanes 0..LANES-1 of each draw (D::LANES = 2: the SSE2 /
+// NEON layout, whose 2-lane engine is lanes 0-1 of the 4-lane record) gets
+// items for those lanes only; the draws advance as before.
+template <class D, class = void>
+struct LanesOf {
+    static constexpr int v = kLanes;
+};
+template <class D>
+struct LanesOf<D, decltype((void)D::LANES)> {
+    static constexpr int v = D::LANES;
+};
+
 template <class D>
 __device__ void op_fixed(Eng& e, uint64_t* carry, int64_t per, const D& d, int lane) {
     constexpr int C = D::CALLS;
+    constexpr int LU = LanesOf<D>::v;
     constexpr int MAXC = C > 1 ? C - 1 : 1;
     const int64_t units = D::units(per);
     if (units == 0) return;
@@ -196,8 +209,8 @@ __device__ void op_fixed(Eng& e, uint64_t* carry, int64_t per, const D& d, int l
         if (nc > 0) {
             // Only unit 0 can hold carried words (nc < C); the first pass of
             // the wave takes the general path.
-            if (t < nu * kLanes) {
-                const int u = t >> 2, l = t & 3;
+            if (t < nu * LU) {
+                const int u = t / LU, l = t % LU;
                 uint64_t w[C];
 #pragma unroll
                 for (int c = 0; c < C; ++c) w[c] = slot_word<kLanes>(e.mt, carry, nc, pos, u * C + c, l);
@@ -207,8 +220,8 @@ __device__ void op_fixed(Eng& e, uint64_t* carry, int64_t per, const D& d, int l
         }
         // Units past the carried words read the block directly: word c of
         // unit u is block draw pos - nc + u*C + c.
-        for (; t < nu * kLanes; t += 64) {
-            const int u = t >> 2, l = t & 3;
+        for (; t < nu * LU; t += 64) {
+            const int u = t / LU, l = t % LU;
             uint64_t w[C];
 #pragma unroll
             for (int c = 0; c < C; ++c) w[c] = temper(e.mt[(pos - nc + u * C + c) * kLanes + l]);

@@ -277,6 +277,50 @@ hipError_t launch_fill_u64(EngineState* st, int64_t n, uint64_t* out, int64_t pe
     return launch_fixed(st, n, per, d, s);
 }
 
+hipError_t launch_fill_layout(EngineState* st, int64_t n, int W, int kind, void* out, int64_t per, double a,
+                              double b, int mode, hipStream_t s) {
+    const float fa = (float)a, fb = (float)b;
+    if (W == 4) {
+        switch (kind) {
+            case LK_U64: return launch_fixed(st, n, per, ElemU64W4<RawU64>{RawU64{(uint64_t*)out}}, s);
+            case LK_UNIFORM_F32:
+                return launch_fixed(st, n, per, ElemF32W4<UniformF32>{UniformF32{(float*)out, fa, fb - fa, mode}}, s);
+            case LK_UNIFORM_F64:
+                return launch_fixed(st, n, per, ElemU64W4<UniformF64>{UniformF64{(double*)out, a, b - a, mode}}, s);
+            case LK_EXPONENTIAL_F32:
+                return launch_fixed(st, n, per, ElemF32W4<ExponentialF32>{ExponentialF32{(float*)out, fa, 1.0f, 0}}, s);
+            case LK_CAUCHY_F32: return launch_fixed(st, n, per, ElemF32W4<CauchyF32>{CauchyF32{(float*)out, fa, fb}}, s);
+            case LK_WEIBULL_F32:
+                return launch_fixed(st, n, per, ElemF32W4<WeibullF32>{WeibullF32{(float*)out, 1 / fa, fb}}, s);
+            case LK_EXTREME_VALUE_F32:
+                return launch_fixed(st, n, per, ElemF32W4<ExtremeValueF32>{ExtremeValueF32{(float*)out, fa, fb}}, s);
+            case LK_EXPONENTIAL_F64:
+                return launch_fixed(st, n, per, ElemU64W4<ExponentialF64>{ExponentialF64{(double*)out, a}}, s);
+            case LK_CAUCHY_F64: return launch_fixed(st, n, per, ElemU64W4<CauchyF64>{CauchyF64{(double*)out, a, b}}, s);
+            case LK_NORMAL_F32:
+                if (mode == 1) return launch_fixed(st, n, per, NormalF32L<4, 1>{(float*)out, fa, fb}, s);
+                if (mode == 3) return launch_fixed(st, n, per, NormalF32L<4, 3>{(float*)out, fa, fb}, s);
+                return launch_fixed(st, n, per, NormalF32L<4, 0>{(float*)out, fa, fb}, s);
+            case LK_STUDENT_T_F32: return launch_fixed(st, n, per, StudentTF32L<4>{(float*)out, fa, -2 / fa}, s);
+            case LK_NORMAL_F64:
+                if (mode == 1) return launch_fixed(st, n, per, NormalF64L<4, 1>{(double*)out, a, b}, s);
+                return launch_fixed(st, n, per, NormalF64L<4, 0>{(double*)out, a, b}, s);
+        }
+    } else if (W == 16) {
+        switch (kind) {
+            case LK_NORMAL_F32:
+                if (mode == 1) return launch_fixed(st, n, per, NormalF32L<16, 1>{(float*)out, fa, fb}, s);
+                if (mode == 3) return launch_fixed(st, n, per, NormalF32L<16, 3>{(float*)out, fa, fb}, s);
+                return launch_fixed(st, n, per, NormalF32L<16, 0>{(float*)out, fa, fb}, s);
+            case LK_STUDENT_T_F32: return launch_fixed(st, n, per, StudentTF32L<16>{(float*)out, fa, -2 / fa}, s);
+            case LK_NORMAL_F64:
+                if (mode == 1) return launch_fixed(st, n, per, NormalF64L<16, 1>{(double*)out, a, b}, s);
+                return launch_fixed(st, n, per, NormalF64L<16, 0>{(double*)out, a, b}, s);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
 // Whole-refill fills drop the scalars left in an adaptor's operator() buffer:
 // the buffer position (record 2k's `reserved`) goes back to 0.
 __global__ void k_p8_clear_position(EngineState* __restrict__ states, int64_t n_p8) {

@@ -355,6 +355,7 @@ int64_t erd_mv_factors_count(const erd_mv_factors* f) { return f ? (int64_t)f->d
 
 int erd_mvnormal_generate(erd_engines* e, const erd_mv_factors* f, const int64_t* samples, const float* mean,
                           float* out, float* z_out, void* stream) {
+    if (layout_unsupported(e)) return ERD_ERR_UNSUPPORTED;
     if (!e || !e->states || !f || !samples || !out) return ERD_ERR_INVALID;
     const int64_t n_inst = (int64_t)f->dims.size();
     if (n_inst > e->count) return ERD_ERR_INVALID;
@@ -386,6 +387,7 @@ int erd_mvnormal_generate(erd_engines* e, const erd_mv_factors* f, const int64_t
 
 int erd_mvnormal_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* samples,
                        const double* cov, const float* mean, float* out, float* z_out, void* stream) {
+    if (layout_unsupported(e)) return ERD_ERR_UNSUPPORTED;
     if (!e || !e->states || n_inst <= 0 || n_inst > e->count || !dims || !samples || !cov || !out)
         return ERD_ERR_INVALID;
     erd_mv_factors* f;
@@ -398,6 +400,7 @@ int erd_mvnormal_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, cons
 
 static int wishart_generate(erd_engines* e, const erd_mv_factors* f, const int64_t* df, const int64_t* samples,
                             float* out, void* stream, bool inverse) {
+    if (layout_unsupported(e)) return ERD_ERR_UNSUPPORTED;
     if (!e || !e->states || !f || !df || !samples || !out) return ERD_ERR_INVALID;
     const int64_t n_inst = (int64_t)f->dims.size();
     if (n_inst > e->count) return ERD_ERR_INVALID;
@@ -551,6 +554,7 @@ int erd_invwishart_generate(erd_engines* e, const erd_mv_factors* f, const int64
 
 int erd_invwishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* df,
                          const int64_t* samples, const double* scale, float* out, void* stream) {
+    if (layout_unsupported(e)) return ERD_ERR_UNSUPPORTED;
     if (!e || !e->states || n_inst <= 0 || n_inst > e->count || !dims || !df || !samples || !scale || !out)
         return ERD_ERR_INVALID;
     for (int64_t i = 0; i < n_inst; ++i) {
@@ -567,6 +571,7 @@ int erd_invwishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, co
 
 int erd_wishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* df,
                       const int64_t* samples, const double* scale, float* out, void* stream) {
+    if (layout_unsupported(e)) return ERD_ERR_UNSUPPORTED;
     if (!e || !e->states || n_inst <= 0 || n_inst > e->count || !dims || !df || !samples || !scale || !out)
         return ERD_ERR_INVALID;
     for (int64_t i = 0; i < n_inst; ++i)
@@ -581,6 +586,7 @@ int erd_wishart_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const
 
 int erd_dirichlet_batch(erd_engines* e, int64_t n_inst, const int32_t* dims, const int64_t* samples,
                         const float* alpha, float* out, void* stream) {
+    if (layout_unsupported(e)) return ERD_ERR_UNSUPPORTED;
     if (!e || !e->states || n_inst <= 0 || n_inst > e->count || !dims || !samples || !alpha || !out)
         return ERD_ERR_INVALID;
     hipStream_t s = (hipStream_t)stream;
@@ -792,6 +798,7 @@ int erd_alias_table(const double* weights, int64_t K, int32_t* prob, int32_t* al
 
 int erd_fill_discrete_i32(erd_engines* e, int32_t* out, int64_t per, const double* weights, int64_t K,
                           void* stream) {
+    if (layout_unsupported(e)) return ERD_ERR_UNSUPPORTED;
     if (!e || !e->states || per < 0 || per % 4 != 0 || (per > 0 && !out) || !weights) return ERD_ERR_INVALID;
     hipStream_t s = (hipStream_t)stream;
     HostTable t;
@@ -813,6 +820,7 @@ int erd_fill_discrete_i32(erd_engines* e, int32_t* out, int64_t per, const doubl
 
 int erd_fill_discrete_f_i32(erd_engines* e, int32_t* out, int64_t per, const double* weights, int64_t K,
                             void* stream) {
+    if (layout_unsupported(e)) return ERD_ERR_UNSUPPORTED;
     if (!e || !e->states || per < 0 || per % 4 != 0 || (per > 0 && !out) || !weights) return ERD_ERR_INVALID;
     hipStream_t s = (hipStream_t)stream;
     HostTable t;
@@ -834,6 +842,7 @@ int erd_fill_discrete_f_i32(erd_engines* e, int32_t* out, int64_t per, const dou
 
 int erd_fill_discrete_d_i32(erd_engines* e, int32_t* out, int64_t per, const double* weights, int64_t K,
                             void* stream) {
+    if (layout_unsupported(e)) return ERD_ERR_UNSUPPORTED;
     if (!e || !e->states || per < 0 || per % 4 != 0 || (per > 0 && !out) || !weights) return ERD_ERR_INVALID;
     hipStream_t s = (hipStream_t)stream;
     HostTable t;
@@ -855,6 +864,7 @@ int erd_fill_discrete_d_i32(erd_engines* e, int32_t* out, int64_t per, const dou
 
 int erd_multinomial_batch(erd_engines* e, int64_t n_inst, const int64_t* dims, const int32_t* trials,
                           const int64_t* samples, const double* weights, int32_t* out, void* stream) {
+    if (layout_unsupported(e)) return ERD_ERR_UNSUPPORTED;
     if (!e || !e->states || n_inst <= 0 || n_inst > e->count || !dims || !trials || !samples || !weights ||
         !out)
         return ERD_ERR_INVALID;

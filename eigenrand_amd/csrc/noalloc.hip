@@ -56,6 +56,7 @@ int scratch_of_current_device(char** out) {
 }  // namespace
 
 extern "C" int erd_run_requests_noalloc(erd_engines* e, const erd_request* reqs, int64_t n, void* stream) {
+    if (layout_unsupported(e)) return ERD_ERR_UNSUPPORTED;
     if (!e || !e->states || e->count <= 0 || n < 0 || (n > 0 && !reqs)) return ERD_ERR_INVALID;
     if (n > kMaxRequests) return ERD_ERR_INVALID;  // beyond the scratch array
     if (n == 0) return ERD_OK;

@@ -80,6 +80,29 @@ int erd_engines_discard(erd_engines* e, int64_t calls, void* stream);
 #define ERD_STATUS_ITER_CAP 1u
 #define ERD_STATUS_BAD_SEGMENT 2u
 int erd_engines_status(erd_engines* e, uint32_t* status_out, void* stream);
+/* Lane layout of the fills on this set: the float packet width the
+ * reference's build takes from its ISA (views created from the set later
+ * copy it).
+ *   8  AVX2, the default: Vmt19937_64 = Pmt19937_64<Packet8i>
+ *      (PacketRandomEngine.h:585-586), a call is 4 u64 lanes, Packet8f.
+ *   4  SSE2 / NEON: Vmt19937_64 = Pmt19937_64<Packet4i>
+ *      (PacketRandomEngine.h:587-588), a 2-lane engine seeded x0, x0+1,
+ *      which is u64 lanes 0-1 of this set's engines (lanes 2-3 are stepped
+ *      and not read); Packet4f / Packet2d of one call
+ *      (arch/SSE/RandUtils.h:70-115).
+ *  16  AVX-512: the AVX2 engine, a Packet16f / Packet8d is two calls with the
+ *      first in the low half (arch/AVX512/RandUtils.h:37-46, evaluated left
+ *      to right as the reference's default Clang preset does).
+ * Fills with a form at every layout: erd_fill_u64, erd_fill_uniform_f32/_f64,
+ * erd_fill_std_normal_f32/_f64, erd_fill_normal_f32/_f64,
+ * erd_fill_lognormal_f32, erd_fill_exponential_f32/_f64,
+ * erd_fill_cauchy_f32/_f64, erd_fill_weibull_f32, erd_fill_extreme_value_f32
+ * and erd_fill_studentt_f32; per_engine is then a multiple of the layout's
+ * packet width. Every other call that consumes draws returns
+ * ERD_ERR_UNSUPPORTED on a set whose layout is not 8. Engine records,
+ * snapshots, copies and discard do not depend on the layout. */
+int erd_engines_set_layout(erd_engines* e, int layout);
+int erd_engines_layout(const erd_engines* e); /* 0 for an invalid handle */
 /* Iteration caps of the launches on this set (views created from it later
  * copy them). The reference's rejection and lockstep loops run until they
  * succeed (RandUtils.h:342-346; Discrete.h:860-873, 1139-1185); here a

@@ -1765,3 +1765,179 @@ extern "C" int orc_p8_fill(void* states, int64_t n_p8, int dist, int mode_overri
     }
     return 0;
 }
+
+// ------------------------------------------------------------ lane layouts ---
+// Fills at the SSE2/NEON (W = 4) and AVX-512 (W = 16) layouts. W = 4: the
+// 2-lane Vmt19937_64 = Pmt19937_64<Packet4i> (PacketRandomEngine.h:587-588)
+// seeded x0 has lanes x0, x0+1, which are lanes 0-1 of the 4-lane engine
+// seeded x0; one call makes a Packet4f / Packet2d (arch/SSE/RandUtils.h:
+// 70-115). W = 16: a Packet16f / Packet8d is two calls of the AVX2 engine,
+// the first in the low half (arch/AVX512/RandUtils.h:37-46, arguments
+// evaluated left to right). The packet ops are those of the AVX2 fills at
+// the wider or narrower packet.
+namespace {
+// One raw packet at layout W: its u64 words (2, 4 or 8).
+inline int raw_packet(Vmt& e, int W, uint64_t* p) {
+    if (W == 4) {
+        uint64_t d[4];
+        vmt_draw(e, d);
+        p[0] = d[0];
+        p[1] = d[1];
+        return 2;
+    }
+    vmt_draw(e, p);
+    if (W == 16) {
+        vmt_draw(e, p + 4);
+        return 8;
+    }
+    return 4;
+}
+// Packet of W floats: float j = 32-bit word j of the packet, low word first.
+inline void ur_packet(Vmt& e, int W, float* u) {
+    uint64_t p[8];
+    raw_packet(e, W, p);
+    for (int j = 0; j < W; ++j) u[j] = bit_to_ur_float((uint32_t)(p[j >> 1] >> (32 * (j & 1))));
+}
+}  // namespace
+
+extern "C" {
+
+// Raw draws at layout W: calls of 2 u64 (W = 4) or 4 u64 (W = 8, 16).
+void orc_fill_u64_l(void* states, int64_t n_engines, uint64_t* out, int64_t per, int W) {
+    Vmt* e = (Vmt*)states;
+    const int width = W == 4 ? 2 : 4;
+    for (int64_t g = 0; g < n_engines; ++g)
+        for (int64_t i = 0; i < per; i += width) raw_packet(e[g], W == 4 ? 4 : 8, out + g * per + i);
+}
+
+// Float fills at layout W (per a multiple of W). kind: 0 uniformReal (mode 1:
+// a + u*(b-a)), 1 normal (StdNormalGen, sine packet then the cached cosine
+// packet; mode 0 standard, 1 z*b + a, 3 exp(z*b + a)), 2 exponential(a),
+// 3 Cauchy(a, b), 4 Weibull(a, b), 5 extreme value(a, b), 6 Student-t(a).
+void orc_fill_f32_l(void* states, int64_t n_engines, int W, int kind, float* out, int64_t per, float a, float b,
+                    int mode) {
+    Vmt* e = (Vmt*)states;
+    const float slope = b - a, inv_a = 1 / a, k = -2 / a;
+    for (int64_t g = 0; g < n_engines; ++g) {
+        float* o = out + g * per;
+        float u1[16], u2[16], cache[16];
+        bool valid = false;
+        for (int64_t i = 0; i < per; i += W) {
+            float* v = o + i;
+            switch (kind) {
+                case 0:
+                    ur_packet(e[g], W, u1);
+                    for (int j = 0; j < W; ++j) v[j] = mode ? u1[j] * slope + a : u1[j];
+                    break;
+                case 1:
+                    if (valid) {
+                        valid = false;
+                        for (int j = 0; j < W; ++j) v[j] = cache[j];
+                    } else {
+                        valid = true;
+                        ur_packet(e[g], W, u1);
+                        ur_packet(e[g], W, u2);
+                        for (int j = 0; j < W; ++j) {
+                            const float radius = std::sqrt(-2.0f * plog(1.0f - u1[j]));
+                            float sn, cs;
+                            psincos(F(2 * kPi) * u2[j], sn, cs);
+                            cache[j] = radius * cs;
+                            v[j] = radius * sn;
+                        }
+                    }
+                    if (mode == 1) for (int j = 0; j < W; ++j) v[j] = v[j] * b + a;
+                    if (mode == 3) for (int j = 0; j < W; ++j) v[j] = pexp(v[j] * b + a);
+                    break;
+                case 2:
+                    ur_packet(e[g], W, u1);
+                    for (int j = 0; j < W; ++j) v[j] = -(plog(1.0f - u1[j]) / a);
+                    break;
+                case 3:
+                    ur_packet(e[g], W, u1);
+                    for (int j = 0; j < W; ++j) {
+                        float sn, cs;
+                        psincos(kPiF * (u1[j] - F(0.5)), sn, cs);
+                        v[j] = a + b * (sn / cs);
+                    }
+                    break;
+                case 4:
+                    ur_packet(e[g], W, u1);
+                    for (int j = 0; j < W; ++j) v[j] = pexp(plog(-plog(1.0f - u1[j])) * inv_a) * b;
+                    break;
+                case 5:
+                    ur_packet(e[g], W, u1);
+                    for (int j = 0; j < W; ++j) v[j] = a - plog(-plog(u1[j] + 1.1920928955078125e-07f / 8)) * b;
+                    break;
+                case 6:
+                    ur_packet(e[g], W, u1);
+                    ur_packet(e[g], W, u2);
+                    for (int j = 0; j < W; ++j) {
+                        const float radius = std::sqrt(a * (pexp(plog(1.0f - u1[j]) * k) - 1.0f));
+                        float sn, cs;
+                        psincos(F(2 * kPi) * u2[j], sn, cs);
+                        v[j] = radius * sn;
+                    }
+                    break;
+            }
+        }
+    }
+}
+
+// Double fills at layout W (per a multiple of W/2 doubles). kind: 0
+// uniformReal (mode 1: a + u*(b-a)), 1 normal (mode 1: z*b + a), 2
+// exponential(a), 3 Cauchy(a, b).
+void orc_fill_f64_l(void* states, int64_t n_engines, int W, int kind, double* out, int64_t per, double a,
+                    double b, int mode) {
+    Vmt* e = (Vmt*)states;
+    const int PD = W / 2;
+    for (int64_t g = 0; g < n_engines; ++g) {
+        double* o = out + g * per;
+        uint64_t p1[8], p2[8];
+        double cache[8];
+        bool valid = false;
+        for (int64_t i = 0; i < per; i += PD) {
+            double* v = o + i;
+            switch (kind) {
+                case 0:
+                    raw_packet(e[g], W, p1);
+                    for (int j = 0; j < PD; ++j) {
+                        const double u = bit_to_ur_double(p1[j]);
+                        v[j] = mode ? u * (b - a) + a : u;
+                    }
+                    break;
+                case 1:
+                    if (valid) {
+                        valid = false;
+                        for (int j = 0; j < PD; ++j) v[j] = cache[j];
+                    } else {
+                        valid = true;
+                        raw_packet(e[g], W, p1);
+                        raw_packet(e[g], W, p2);
+                        for (int j = 0; j < PD; ++j) {
+                            const double radius = std::sqrt(-2.0 * plog_d(1.0 - bit_to_ur_double(p1[j])));
+                            double sn, cs;
+                            psincos_d((2 * kPi) * bit_to_ur_double(p2[j]), sn, cs);
+                            cache[j] = radius * cs;
+                            v[j] = radius * sn;
+                        }
+                    }
+                    if (mode == 1) for (int j = 0; j < PD; ++j) v[j] = v[j] * b + a;
+                    break;
+                case 2:
+                    raw_packet(e[g], W, p1);
+                    for (int j = 0; j < PD; ++j) v[j] = -(plog_d(1.0 - bit_to_ur_double(p1[j])) / a);
+                    break;
+                case 3:
+                    raw_packet(e[g], W, p1);
+                    for (int j = 0; j < PD; ++j) {
+                        double sn, cs;
+                        psincos_d(kPi * (bit_to_ur_double(p1[j]) - 0.5), sn, cs);
+                        v[j] = a + b * (sn / cs);
+                    }
+                    break;
+            }
+        }
+    }
+}
+
+}  // extern "C"

@@ -482,3 +482,31 @@ def p8_fill(e, dist, per, a=0.0, b=0.0, mode=-1):
            C.c_double(b))
     assert rc == 0
     return out
+
+
+# Lane layouts (W = 4: SSE2/NEON, W = 16: AVX-512; W = 8 is the AVX2 layout
+# of the functions above).
+LAYOUT_F32 = {"uniform": 0, "normal": 1, "exponential": 2, "cauchy": 3, "weibull": 4, "extreme_value": 5,
+              "studentt": 6}
+LAYOUT_F64 = {"uniform": 0, "normal": 1, "exponential": 2, "cauchy": 3}
+
+
+def fill_u64_layout(e, per, W):
+    """per raw u64 per engine at layout W (calls of 2 u64 at W = 4)."""
+    out = np.empty(len(e) * per, dtype=np.uint64)
+    lib().orc_fill_u64_l(_p(e.raw), I64(len(e)), _p(out), I64(per), C.c_int(W))
+    return out
+
+
+def fill_f32_layout(e, kind, per, W, a=0.0, b=1.0, mode=0):
+    out = np.empty(len(e) * per, dtype=np.float32)
+    lib().orc_fill_f32_l(_p(e.raw), I64(len(e)), C.c_int(W), C.c_int(LAYOUT_F32[kind]), _p(out), I64(per),
+                         C.c_float(a), C.c_float(b), C.c_int(mode))
+    return out
+
+
+def fill_f64_layout(e, kind, per, W, a=0.0, b=1.0, mode=0):
+    out = np.empty(len(e) * per, dtype=np.float64)
+    lib().orc_fill_f64_l(_p(e.raw), I64(len(e)), C.c_int(W), C.c_int(LAYOUT_F64[kind]), _p(out), I64(per),
+                         C.c_double(a), C.c_double(b), C.c_int(mode))
+    return out
