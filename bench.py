@@ -34,7 +34,7 @@ N_ENGINES = 4096
 PER_ENGINE = 65536
 N_SAMPLES = N_ENGINES * PER_ENGINE  # 2^28
 HBM_PEAK_GBS = 8000.0
-KERNEL = {"fast": "k_fill_fixed2<NormalF32Fast>", "exact": "k_fill_fixed<NormalF32<0>>"}
+KERNEL = {"fast": "k_fill_fixed2<NormalF32Fast<0>>", "exact": "k_fill_fixed<NormalF32<0>>"}
 ACCURACY = {
     "exact": "bit-exact vs the oracle restatement of the reference's float op sequence",
     "fast": "same engine draws, pairing and order as exact; |z - z_exact| <= 4 ulp(z_exact) + 2^-20 r "

@@ -38,8 +38,9 @@ struct UniformF32 {  // Basic.h:584-590 / 626-634; float j of a call = word j
     __device__ void emit(int64_t per, int64_t u, int l, const uint64_t* w) const {
         *reinterpret_cast<float2*>(out + 8 * u + 2 * l) = make_float2(value(lo32(w[0])), value(hi32(w[0])));
     }
+    __device__ static int64_t full_units(int64_t n) { return n / 8; }
     // floats 4k .. 4k+3 of the call (lanes 2k, 2k+1) in one 16-byte store
-    __device__ void emit2(int64_t per, int64_t u, int k, const uint64_t (&w)[1][2]) const {
+    __device__ void emit2(int64_t per, int64_t u, int k, const uint64_t (&w)[1][2], bool) const {
         float v[4] = {ur_f32(lo32(w[0][0])), ur_f32(hi32(w[0][0])), ur_f32(lo32(w[0][1])), ur_f32(hi32(w[0][1]))};
         if (affine) {
 #pragma unroll
@@ -230,14 +231,15 @@ struct NormalF32 {
 //                                             exact, stands for 2 pi u2)
 // and four outputs per 16-byte store. The stated error bound against the
 // bit-exact mode is asserted by the tests.
+template <int AFFINE>  // 1: z * sd + mean
 struct NormalF32Fast {
     static constexpr int CALLS = 2;
     float* out;
     float mean, sd;
-    int affine;
     __device__ void rebase(int64_t off) { out += off; }
     __device__ static int64_t units(int64_t n) { return (n + 15) / 16; }
-    __device__ void emit2(int64_t per, int64_t u, int k, const uint64_t (&w)[2][2]) const {
+    __device__ static int64_t full_units(int64_t n) { return n / 16; }
+    __device__ void emit2(int64_t per, int64_t u, int k, const uint64_t (&w)[2][2], bool full) const {
         float sn[4], cs[4];
 #pragma unroll
         for (int j = 0; j < 2; ++j)
@@ -251,13 +253,13 @@ struct NormalF32Fast {
                 sn[2 * j + h] = r * __builtin_amdgcn_sinf(rev);
                 cs[2 * j + h] = r * __builtin_amdgcn_cosf(rev);
             }
-        if (affine) {
+        if (AFFINE) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) { sn[i] = sn[i] * sd + mean; cs[i] = cs[i] * sd + mean; }
         }
         float* o = out + 16 * u + 4 * k;
         *reinterpret_cast<float4*>(o) = make_float4(sn[0], sn[1], sn[2], sn[3]);
-        if (16 * u + 16 <= per) *reinterpret_cast<float4*>(o + 8) = make_float4(cs[0], cs[1], cs[2], cs[3]);
+        if (full) *reinterpret_cast<float4*>(o + 8) = make_float4(cs[0], cs[1], cs[2], cs[3]);
     }
 };
 

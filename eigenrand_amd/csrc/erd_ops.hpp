@@ -245,17 +245,22 @@ __device__ void op_fixed(Eng& e, uint64_t* carry, int64_t per, const D& d, int l
 // op_fixed with items of two u64 lanes: item k of unit u reads lanes 2k and
 // 2k+1 of each of the unit's C draws (one 16-byte LDS read per draw), so a
 // float fill writes four consecutive outputs per draw and store. Same draws,
-// same order, same final engine state as op_fixed.
-// D::emit2(per, u, k, w): w[c][j] = tempered lane 2k+j of draw c of unit u.
+// same order, same final engine state as op_fixed. The wave takes the units
+// of a block 32 at a time (thread t: unit t/2, item t%2), with the loop and
+// its bounds uniform.
+// D::emit2(per, u, k, w, full): w[c][j] = tempered lane 2k+j of draw c of
+// unit u; full = unit u lies wholly inside the fill (D::full_units).
 template <class D>
 __device__ void op_fixed2(Eng& e, uint64_t* carry, int64_t per, const D& d, int lane) {
     constexpr int C = D::CALLS;
     constexpr int MAXC = C > 1 ? C - 1 : 1;
     const int64_t units = D::units(per);
     if (units == 0) return;
+    const int64_t full_units = D::full_units(per);
     int pos = e.idx;
     int nc = 0;
     int64_t done = 0;
+    const int ul = lane >> 1, k = lane & 1;
     for (;;) {
         if (pos >= kN) {
             eng_refill(e, lane);
@@ -265,30 +270,28 @@ __device__ void op_fixed2(Eng& e, uint64_t* carry, int64_t per, const D& d, int 
         const int avail = nc + (kN - pos);
         const int64_t can = (int64_t)(avail / C);
         const int nu = (int)(units - done < can ? units - done : can);
-        int t = lane;
-        if (nc > 0) {
-            // only unit 0 can hold carried words
-            if (t < nu * 2) {
-                const int u = t >> 1, k = t & 1;
+        // units of this block that are whole (only the fill's last unit can be partial)
+        const int nfull = (int)(full_units - done < (int64_t)nu ? full_units - done : (int64_t)nu);
+        for (int cb = 0; cb < nu; cb += 32) {
+            const int u = cb + ul;
+            if (u < nu) {
                 uint64_t w[C][2];
+                if (nc > 0 && cb == 0) {
+                    // only unit 0 can hold carried words
 #pragma unroll
-                for (int c = 0; c < C; ++c)
+                    for (int c = 0; c < C; ++c)
 #pragma unroll
-                    for (int j = 0; j < 2; ++j) w[c][j] = slot_word<kLanes>(e.mt, carry, nc, pos, u * C + c, 2 * k + j);
-                d.emit2(per, done + u, k, w);
+                        for (int j = 0; j < 2; ++j) w[c][j] = slot_word<kLanes>(e.mt, carry, nc, pos, u * C + c, 2 * k + j);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < C; ++c) {
+                        const ulonglong2 p = *reinterpret_cast<const ulonglong2*>(e.mt + (pos - nc + u * C + c) * kLanes + 2 * k);
+                        w[c][0] = temper(p.x);
+                        w[c][1] = temper(p.y);
+                    }
+                }
+                d.emit2(per, done + u, k, w, u < nfull);
             }
-            t += 64;
-        }
-        for (; t < nu * 2; t += 64) {
-            const int u = t >> 1, k = t & 1;
-            uint64_t w[C][2];
-#pragma unroll
-            for (int c = 0; c < C; ++c) {
-                const ulonglong2 p = *reinterpret_cast<const ulonglong2*>(e.mt + (pos - nc + u * C + c) * kLanes + 2 * k);
-                w[c][0] = temper(p.x);
-                w[c][1] = temper(p.y);
-            }
-            d.emit2(per, done + u, k, w);
         }
         const int consumed = nu * C;
         done += nu;

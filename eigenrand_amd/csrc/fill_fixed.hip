@@ -358,8 +358,8 @@ hipError_t launch_fill_uniform_f32(EngineState* st, int64_t n, float* out, int64
 
 hipError_t launch_fill_normal_f32_fast(EngineState* st, int64_t n, float* out, int64_t per, float mean,
                                        float sd, int affine, hipStream_t s) {
-    NormalF32Fast d{out, mean, sd, affine};
-    return launch_fixed2(st, n, per, d, s);
+    if (affine) return launch_fixed2(st, n, per, NormalF32Fast<1>{out, mean, sd}, s);
+    return launch_fixed2(st, n, per, NormalF32Fast<0>{out, mean, sd}, s);
 }
 
 hipError_t launch_store_blocks(float* out, int64_t n_blocks, int64_t per, hipStream_t s) {
