@@ -204,6 +204,7 @@ __global__ void k_split_l(const double* __restrict__ l64, const int64_t* __restr
 // ------------------------------------------------------- MFMA L * Z ---
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 // Four floats at a 4-byte-aligned address (column starts are d floats
 // apart, d arbitrary): one 16-byte global access.
 typedef float f32x4_a4 __attribute__((ext_vector_type(4), aligned(4)));
@@ -503,14 +504,14 @@ namespace erd {
 // --------------------------------------------------------------- Wishart ---
 // W_j = T_j T_j^T (WishartGen, MvNormal.h:271-275) or U_j^T U_j (U = T^-1,
 // InvWishartGen, MvNormal.h:433-439) in FP32, per sample, over 64 x 64 output tiles with
-// ta >= tb (tile units of 64). A thread keeps a 4 x 4 block of the tile in
-// registers; per 32-wide k chunk the workgroup stages the tile's 64 rows of
-// both operands in LDS ([k][row], so a thread's four rows are one 16-byte
-// read). Every output is one fmaf chain over k ascending, from 0 (Wishart:
-// T[b][k] = 0 beyond k = b) or from the tile's first row (inverse: U[k][a]
-// = 0 for k < a); the extra terms the tile bounds add are exact zeros. Both
-// halves of W are written from the same accumulator, so W is exactly
-// symmetric; diagonal tiles write only their lower half.
+// ta >= tb (tile units of 64). Per 32-wide k chunk the workgroup stages the
+// tile's 64 rows of both operands in LDS ([k][row]); each wave accumulates a
+// 32 x 32 quadrant with FP32 MFMA (v_mfma_f32_32x32x2_f32, FP32 products and
+// sums), over k from 0 (Wishart: T[b][k] = 0 beyond k = b) or from the tile's
+// first row (inverse: U[k][a] = 0 for k < a); the extra terms the tile
+// bounds add are exact zeros. Both halves of W are written from the same
+// accumulator, so W is exactly symmetric; diagonal tiles write only their
+// lower half.
 constexpr int kSyT = 64, kSyK = 32, kSyPitch = kSyT + 4;
 
 template <bool UTU>
@@ -526,13 +527,13 @@ __global__ __launch_bounds__(256) void k_syrk_tt(const SyrkTile* __restrict__ ti
     const int d = in.d;
     const float* T = in.x + (int64_t)tl.sample * d * d;  // d x d column-major
     float* W = w + w_off[tl.inst] + (int64_t)tl.sample * d * d;
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;  // 16 x 16 threads, 4 x 4 outputs each
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // wave w computes the 32 x 32 quadrant (rows qa.., columns qb..) of the tile
+    const int qa = (wave >> 1) * 32, qb = (wave & 1) * 32;
     const int a0 = tl.ta * kSyT, b0 = tl.tb * kSyT;
-    float acc[4][4];
+    f32x16 acc;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
     const int kbeg = UTU ? a0 : 0;
     const int kend = UTU ? d : (b0 + kSyT < d ? b0 + kSyT : d);
     // element q = tid + 256 i of a chunk's staging: row r, k offset kk
@@ -575,15 +576,13 @@ __global__ __launch_bounds__(256) void k_syrk_tt(const SyrkTile* __restrict__ ti
         }
         __syncthreads();
         if (k0 + kSyK < kend) fetch(k0 + kSyK);
-#pragma unroll 8
-        for (int kk = 0; kk < kSyK; ++kk) {
-            const float4 av = *reinterpret_cast<const float4*>(&ta_s[kk][4 * ty]);
-            const float4 bv = *reinterpret_cast<const float4*>(&tb_s[kk][4 * tx]);
-            const float ar[4] = {av.x, av.y, av.z, av.w}, br[4] = {bv.x, bv.y, bv.z, bv.w};
+        // D[i][j] += sum over k of A[i][k] B[k][j] on the matrix cores, two k per
+        // instruction: lane l holds A[l % 32][k + l / 32] and B[k + l / 32][l % 32]
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(ar[i], br[j], acc[i][j]);
+        for (int kk = 0; kk < kSyK; kk += 2) {
+            const float av = ta_s[kk + (lane >> 5)][qa + (lane & 31)];
+            const float bv = tb_s[kk + (lane >> 5)][qb + (lane & 31)];
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
         }
         __syncthreads();
     }
@@ -591,10 +590,10 @@ __global__ __launch_bounds__(256) void k_syrk_tt(const SyrkTile* __restrict__ ti
     // barrier), so that both halves of W are written along columns
     static_assert(sizeof(sm) >= sizeof(float) * kSyT * (kSyT + 1), "the tile fits the operand LDS");
     float(*ot)[kSyT + 1] = reinterpret_cast<float(*)[kSyT + 1]>(&sm[0][0][0]);
+    // D[i][j] of the quadrant is register r of lane l with j = l % 32,
+    // i = 8 (r / 4) + 4 (l / 32) + r % 4
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) ot[4 * ty + i][4 * tx + j] = acc[i][j];
+    for (int r = 0; r < 16; ++r) ot[qa + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)][qb + (lane & 31)] = acc[r];
     __syncthreads();
     const bool diag = tl.ta == tl.tb;
     const int c = tid & (kSyT - 1), r0 = tid >> 6;  // 64 consecutive threads along a column
