@@ -29,6 +29,8 @@ struct MvTile {
     int32_t inst;
     int32_t row0;
     int64_t col0;
+    int32_t kbeg;   // first k chunk (a multiple of 32)
+    int32_t ncols;  // columns of the tile: [col0, col0 + ncols)
 };
 
 // FP64 Cholesky of n instances; dmax = the largest d (host value): up to

@@ -265,6 +265,7 @@ __global__ __launch_bounds__(256, NCB == 4 || BART ? 2 : 3) void k_lz_gemm(const
         const int R = (dp - row0) / 16 < MAXR ? (dp - row0) / 16 : MAXR;
         const int kmax = row0 + 16 * R;  // L[i][k] = 0 for k > i
         const int64_t n = in.n;
+        const int64_t cend = tl.col0 + tl.ncols;  // the tile's columns are [c0, cend)
         const int64_t c0 = tl.col0;
         f32x4 acc[NCB][MAXR];
 #pragma unroll
@@ -293,7 +294,7 @@ __global__ __launch_bounds__(256, NCB == 4 || BART ? 2 : 3) void k_lz_gemm(const
 #pragma unroll
             for (int i = 0; i < ZQ; ++i) {
                 const int c = (tid + 256 * i) / (KC / 8);
-                const int64_t col = c0 + c < n ? c0 + c : 0;
+                const int64_t col = c0 + c < cend ? c0 + c : 0;
                 zj[i] = col / d;
                 zi[i] = (int)(col - zj[i] * d);
                 zn[i] = zj[i] * per_sample + (int64_t)zi[i] * (d - 1) - (int64_t)zi[i] * (zi[i] - 1) / 2 - zi[i] - 1;
@@ -322,7 +323,7 @@ __global__ __launch_bounds__(256, NCB == 4 || BART ? 2 : 3) void k_lz_gemm(const
                 const int64_t col = c0 + c;
                 if (BART) {
                     const int kb = k0 + kk;
-                    if (col < n && kb > zi[i] && kb + 8 <= d) {
+                    if (col < cend && kb > zi[i] && kb + 8 <= d) {
                         // all eight below the diagonal: consecutive normals
                         const f32x4_a4 a = *reinterpret_cast<ERD_GLOBAL const f32x4_a4*>(Z + zn[i] + kb);
                         const f32x4_a4 b = *reinterpret_cast<ERD_GLOBAL const f32x4_a4*>(Z + zn[i] + kb + 4);
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(256, NCB == 4 || BART ? 2 : 3) void k_lz_gemm(const
                         for (int e = 0; e < 8; ++e) {
                             const int k = kb + e;
                             float v = 0.0f;
-                            if (col < n && k < d) {
+                            if (col < cend && k < d) {
                                 if (k == zi[i])
                                     v = sqrtf(in.chi[(int64_t)zi[i] * (n / d) + zj[i]]);
                                 else if (k > zi[i])
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(256, NCB == 4 || BART ? 2 : 3) void k_lz_gemm(const
                     }
                     continue;
                 }
-                if (col < n && k0 + kk + 8 <= d) {
+                if (col < cend && k0 + kk + 8 <= d) {
                     // 8 consecutive floats of column col: two 16-byte loads
                     const f32x4_a4 a = *reinterpret_cast<ERD_GLOBAL const f32x4_a4*>(Z + col * d + k0 + kk);
                     const f32x4_a4 b = *reinterpret_cast<ERD_GLOBAL const f32x4_a4*>(Z + col * d + k0 + kk + 4);
@@ -354,13 +355,15 @@ __global__ __launch_bounds__(256, NCB == 4 || BART ? 2 : 3) void k_lz_gemm(const
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
                         const int k = k0 + kk + e;
-                        pz[i][e] = (k < d && col < n) ? Z[col * d + k] : 0.0f;
+                        pz[i][e] = (k < d && col < cend) ? Z[col * d + k] : 0.0f;
                     }
                 }
             }
         };
-        fetch(0);
-        for (int k0 = 0; k0 < kmax; k0 += KC) {
+        // k below the tile's first chunk contributes exact zeros (a Bartlett
+        // column i is zero above row i)
+        fetch(tl.kbeg);
+        for (int k0 = tl.kbeg; k0 < kmax; k0 += KC) {
             // L[row0 + r, k0:k0+32] (row-major bf16) and Z[k0:k0+32, c0:c0+NT]
             // split into hi/lo, layout [col][k]
 #pragma unroll
@@ -410,7 +413,7 @@ __global__ __launch_bounds__(256, NCB == 4 || BART ? 2 : 3) void k_lz_gemm(const
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) {
             const int64_t col = c0 + (wave * NCB + cb) * 16 + l16;
-            if (col >= n) continue;
+            if (col >= cend) continue;
 #pragma unroll
             for (int rb = 0; rb < MAXR; ++rb) {
                 if (rb >= R) continue;

@@ -58,8 +58,16 @@ int bucket_of(int d) { return ((d + 15) & ~15) <= 64 ? 0 : ((d + 15) & ~15) <= 1
 // L2 only. In a queue: instances largest first (the tail of the launch is
 // short), and per instance the row tiles of a column block one after the
 // other (they read the same Z columns).
+// Bartlett factors (T = L A for the Wishart samples, A lower triangular per
+// sample): column blocks stay inside one sample, the product of a block
+// whose columns start at i starts at the k chunk holding i (A[k][i] = 0
+// for k < i: the skipped terms are exact zeros, the results unchanged), and
+// row tiles wholly above the block's first column are not computed. Those
+// entries of T are zero and never read: the syrk and the triangular inverse
+// read T's upper part only inside 64-aligned diagonal blocks, and every
+// skipped tile lies above one (block columns start at multiples of 64).
 void build_tiles(const std::vector<MvInstance>& inst, const std::vector<int>& which, int variant,
-                 std::vector<MvTile>& tiles, std::vector<int32_t>& qbeg) {
+                 std::vector<MvTile>& tiles, std::vector<int32_t>& qbeg, bool bartlett = false) {
     const int rows = lz_rows(variant), cols = lz_cols(variant);
     std::vector<int> order(which);
     auto work = [&](int i) {
@@ -78,9 +86,18 @@ void build_tiles(const std::vector<MvInstance>& inst, const std::vector<int>& wh
     qbeg.assign(1, 0);
     for (int q = 0; q < kLzQueues; ++q) {
         for (int i : queue[q]) {
-            const int dp = (inst[i].d + 15) & ~15;
-            for (int64_t c = 0; c < inst[i].n; c += cols)
-                for (int r = 0; r < dp; r += rows) tiles.push_back(MvTile{i, r, c});
+            const int d = inst[i].d, dp = (d + 15) & ~15;
+            if (!bartlett) {
+                for (int64_t c = 0; c < inst[i].n; c += cols)
+                    for (int r = 0; r < dp; r += rows)
+                        tiles.push_back(MvTile{i, r, c, 0, (int32_t)std::min<int64_t>(cols, inst[i].n - c)});
+                continue;
+            }
+            for (int64_t j0 = 0; j0 < inst[i].n; j0 += d)  // sample j0 / d
+                for (int c = 0; c < d; c += cols)
+                    for (int r = 0; r < dp; r += rows)
+                        if (r + rows > c)
+                            tiles.push_back(MvTile{i, r, j0 + c, c & ~31, std::min(cols, d - c)});
         }
         qbeg.push_back((int32_t)tiles.size());
     }
@@ -147,7 +164,7 @@ int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t 
             for (int cand = 0; cand < kLzVariants; ++cand) {
                 std::vector<MvTile> tiles;
                 std::vector<int32_t> qbeg;
-                build_tiles(inst, kv.second, cand, tiles, qbeg);
+                build_tiles(inst, kv.second, cand, tiles, qbeg, bartlett);
                 MvTile* d_tiles;
                 int32_t *d_qbeg, *d_cnt;
                 if (sc.upload(&d_tiles, tiles) != hipSuccess || sc.upload(&d_qbeg, qbeg) != hipSuccess ||
@@ -184,7 +201,7 @@ int run_lz(const std::vector<MvInstance>& inst, MvInstance* d_inst, hipStream_t 
     for (auto& kv : by_variant) {
         std::vector<MvTile> tiles;
         std::vector<int32_t> qbeg;
-        build_tiles(inst, kv.second, kv.first, tiles, qbeg);
+        build_tiles(inst, kv.second, kv.first, tiles, qbeg, bartlett);
         MvTile* d_tiles;
         int32_t *d_qbeg, *d_cnt;
         if (sc.upload(&d_tiles, tiles) != hipSuccess || sc.upload(&d_qbeg, qbeg) != hipSuccess ||
