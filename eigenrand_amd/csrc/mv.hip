@@ -703,10 +703,72 @@ __global__ __launch_bounds__(256) void k_spd_inverse_f64(const double* __restric
 }
 
 // U = T^-1 for the lower-triangular T of each InvWishart sample
-// (MvNormal.h:433-438: t.triangularView<Lower>().solve(identity)), in FP32.
-// One workgroup per (instance, sample, 32-column panel J): forward
-// substitution by 32-row blocks I >= J, X_I = T_II^-1 (E_IJ - sum_{K=J}^{I-1}
-// T_IK X_K), with the panel X (d x 32) in LDS; rows above the panel are zero.
+// (MvNormal.h:433-438: t.triangularView<Lower>().solve(identity)), in FP32,
+// in two launches over the (instance, sample, 32-column panel J) list:
+// k_trtri_diag inverts every diagonal block, U_JJ = T_JJ^-1 (forward
+// substitution, one column per thread, independent across blocks), straight
+// into U; k_trtri_panel then forms each panel by 32-row blocks I > J,
+// X_I = U_II (E_IJ - sum_{K=J}^{I-1} T_IK X_K), with the panel X (d x 32) in
+// LDS: block products only, no serial solve inside a panel. Rows above the
+// panel are zero.
+
+// U_JJ = T_JJ^-1 of panel tile t, eight tiles per workgroup (32 threads each,
+// thread c: column c). Diagonal blocks past d are identity-padded.
+__global__ __launch_bounds__(256) void k_trtri_diag(const SyrkTile* __restrict__ tiles, int n_tiles,
+                                                    const MvInstance* __restrict__ inst, float* __restrict__ u_out,
+                                                    const int64_t* __restrict__ u_off) {
+    __shared__ float Tb[8][32][33];
+    const int grp = threadIdx.x >> 5, c = threadIdx.x & 31;
+    const int t = blockIdx.x * 8 + grp;
+    SyrkTile tl{-1, 0, 0, 0};
+    if (t < n_tiles) tl = tiles[t];
+    const bool live = tl.inst >= 0;  // padding of the per-XCD panel order
+    int d = 0, J = 0;
+    const float* T = nullptr;
+    float* U = nullptr;
+    if (live) {
+        const MvInstance in = inst[tl.inst];
+        d = in.d;
+        J = tl.ta;
+        T = in.x + (int64_t)tl.sample * d * d;  // d x d column-major
+        U = u_out + u_off[tl.inst] + (int64_t)tl.sample * d * d;
+        // column by column, thread c reading row c: 128 contiguous bytes a column
+        const int row = J * 32 + c;
+        for (int k = 0; k < 32; ++k) {
+            const int col = J * 32 + k;
+            Tb[grp][c][k] = (row < d && col < d) ? T[(int64_t)col * d + row] : (c == k ? 1.0f : 0.0f);
+        }
+    }
+    __syncthreads();
+    float v[32];
+#pragma unroll
+    for (int r = 0; r < 32; ++r) v[r] = r == c ? 1.0f : 0.0f;
+    if (live) {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            float x = v[r];
+#pragma unroll
+            for (int k = 0; k < r; ++k) x -= Tb[grp][r][k] * v[k];
+            v[r] = x / Tb[grp][r][r];
+        }
+    }
+    // back through LDS, so that the stores also run down the columns
+    __syncthreads();
+    if (live) {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) Tb[grp][r][c] = v[r];
+    }
+    __syncthreads();
+    if (!live) return;
+    const int row = J * 32 + c;
+    if (row < d) {
+        for (int k = 0; k < 32; ++k) {
+            const int col = J * 32 + k;
+            if (col < d) U[(int64_t)col * d + row] = Tb[grp][c][k];
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_trtri_panel(const SyrkTile* __restrict__ tiles,
                                                      const MvInstance* __restrict__ inst, float* __restrict__ u_out,
                                                      const int64_t* __restrict__ u_off) {
@@ -723,18 +785,20 @@ __global__ __launch_bounds__(256) void k_trtri_panel(const SyrkTile* __restrict_
     const int J = tl.ta, nb = (d + 31) / 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // column tx, rows ty*4..ty*4+3
     // Tiles are used in the order (I, K) = (J, J), (J+1, J), (J+1, J+1), ...:
-    // for each block row I the updates K = J..I-1, then the diagonal block
-    // (identity-padded past d). The next tile is fetched into registers while
-    // the current one is in use; Tb is double-buffered, so one barrier per
-    // tile keeps writers and readers apart.
+    // for each block row I the updates T_IK, K = J..I-1, then U_II (the
+    // diagonal block of U, from k_trtri_diag; identity-padded past d). The
+    // next tile is fetched into registers while the current one is in use;
+    // Tb is double-buffered, so one barrier per tile keeps writers and
+    // readers apart.
     float pre[4];
     auto fetch = [&](int I, int K) {
+        const float* src = I == K ? U : T;
 #pragma unroll
         for (int q4 = 0; q4 < 4; ++q4) {
             const int q = threadIdx.x + 256 * q4;
             const int r = q & 31, kk = q >> 5;
             const int row = I * 32 + r, col = K * 32 + kk;
-            pre[q4] = (row < d && col < d) ? T[(int64_t)col * d + row] : ((I == K && r == kk) ? 1.0f : 0.0f);
+            pre[q4] = (row < d && col < d) ? src[(int64_t)col * d + row] : ((I == K && r == kk) ? 1.0f : 0.0f);
         }
     };
     fetch(J, J);
@@ -765,33 +829,38 @@ __global__ __launch_bounds__(256) void k_trtri_panel(const SyrkTile* __restrict_
                     for (int q = 0; q < 4; ++q) acc[q] = fmaf(-tb[ty * 4 + q][kk], xv, acc[q]);
                 }
             } else {
-                // diagonal block: column tx by forward substitution, in
-                // registers and fully unrolled (only the broadcast T reads
-                // touch LDS)
-                if (threadIdx.x < 32) {
-                    float v[32];
+                // X_I = U_II R_I (U_II lower triangular; R_I = E_JJ for I = J,
+                // where X_J = U_JJ itself)
+                float o[4];
+                if (I == J) {
 #pragma unroll
-                    for (int r = 0; r < 32; ++r) v[r] = X[((I - J) * 32 + r) * 33 + tx];
+                    for (int q = 0; q < 4; ++q) o[q] = tb[ty * 4 + q][tx];
+                } else {
 #pragma unroll
-                    for (int r = 0; r < 32; ++r) {
-                        float t = v[r];
+                    for (int q = 0; q < 4; ++q) o[q] = 0.0f;
+#pragma unroll 8
+                    for (int kk = 0; kk < 32; ++kk) {
+                        const float xv = X[((I - J) * 32 + kk) * 33 + tx];
 #pragma unroll
-                        for (int k = 0; k < r; ++k) t -= tb[r][k] * v[k];
-                        v[r] = t / tb[r][r];
+                        for (int q = 0; q < 4; ++q) o[q] = fmaf(tb[ty * 4 + q][kk], xv, o[q]);
                     }
-#pragma unroll
-                    for (int r = 0; r < 32; ++r) X[((I - J) * 32 + r) * 33 + tx] = v[r];
                 }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < 4; ++q) X[((I - J) * 32 + ty * 4 + q) * 33 + tx] = o[q];
                 __syncthreads();
             }
             buf ^= 1;
         }
     }
     // store the panel's columns (column-major U; zeros above the panel's rows)
+    // (the diagonal block, written by k_trtri_diag, is left as it is: other
+    // panels read it)
     for (int q = threadIdx.x; q < 32 * d; q += 256) {
         const int row = q % d, c = q / d;
         const int col = J * 32 + c;
-        if (col < d) U[(int64_t)col * d + row] = row < J * 32 ? 0.0f : X[(row - J * 32) * 33 + c];
+        if (col < d && (row < J * 32 || row >= J * 32 + 32))
+            U[(int64_t)col * d + row] = row < J * 32 ? 0.0f : X[(row - J * 32) * 33 + c];
     }
 }
 
@@ -810,6 +879,8 @@ hipError_t launch_trtri_panel(const SyrkTile* tiles, int n_tiles, int max_rows, 
     hipError_t err = hipFuncSetAttribute((const void*)k_trtri_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)lds);
     if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(k_trtri_diag, dim3((unsigned)((n_tiles + 7) / 8)), dim3(256), 0, s, tiles, n_tiles, inst, u,
+                       u_off);
     hipLaunchKernelGGL(k_trtri_panel, dim3((unsigned)n_tiles), dim3(256), lds, s, tiles, inst, u, u_off);
     return hipGetLastError();
 }
