@@ -35,8 +35,11 @@ struct FillJob {
     int64_t count;
     float* out;
 };
+// Threads per job: 256, or 640 (`wide`, one item of a draw block per
+// thread) when the jobs are few enough to run in one round.
 constexpr int kJobThreads = 256;
-hipError_t launch_std_normal_jobs(EngineState* st, const FillJob* jobs, int n_jobs, hipStream_t s);
+constexpr int kJobThreadsWide = 640;
+hipError_t launch_std_normal_jobs(EngineState* st, const FillJob* jobs, int n_jobs, hipStream_t s, bool wide);
 // Consecutive GammaGen fills (fresh generators) on one engine: `rows` rows
 // of `count` gammas, row k of shape alpha[alpha_off + k] and scale `beta` at
 // out + k * count (a Dirichlet instance's category rows, or one fill).

@@ -238,12 +238,16 @@ __global__ __launch_bounds__(64) void k_fill_discrete_d_alias(EngineState* __res
     eng_store(e, states + g, lane);
 }
 
-// StdNormalGen fills of a list of engines, one workgroup of kJobThreads per
+// StdNormalGen fills of a list of engines, one workgroup of NT threads per
 // engine (job), so that one long fill on one engine (a multivariate
 // instance's Z) is not held to one wave's rate. Outputs and engine states
-// equal k_fill_fixed<NormalF32<0>> on each engine.
-__global__ __launch_bounds__(kJobThreads) void k_std_normal_jobs(EngineState* __restrict__ states,
-                                                                 const FillJob* __restrict__ jobs) {
+// equal k_fill_fixed<NormalF32<0>> on each engine. A block of 312 draws holds
+// 624 items (4 normals each): at 256 threads the busiest thread takes 3 per
+// block, at 640 threads (10 waves) one, which shortens a long serial fill
+// when there are few engines (the Bartlett normals of a Wishart batch).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_std_normal_jobs(EngineState* __restrict__ states,
+                                                        const FillJob* __restrict__ jobs) {
     __shared__ uint64_t mt[kWords];
     __shared__ uint64_t carry[kLanes];
     const int tid = threadIdx.x;
@@ -253,13 +257,18 @@ __global__ __launch_bounds__(kJobThreads) void k_std_normal_jobs(EngineState* __
     Eng e;
     eng_load(e, mt, states + jb.engine, wl);
     NormalF32<0> d{jb.out, 0.f, 1.f, nullptr, nullptr};
-    op_fixed_wg<kJobThreads>(e, carry, jb.count, d, tid);
+    op_fixed_wg<NT>(e, carry, jb.count, d, tid);
     eng_store(e, states + jb.engine, wl);
 }
 
-hipError_t launch_std_normal_jobs(EngineState* st, const FillJob* jobs, int n_jobs, hipStream_t s) {
+hipError_t launch_std_normal_jobs(EngineState* st, const FillJob* jobs, int n_jobs, hipStream_t s, bool wide) {
     if (n_jobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_std_normal_jobs, dim3((unsigned)n_jobs), dim3(kJobThreads), 0, s, st, jobs);
+    if (wide)
+        hipLaunchKernelGGL(k_std_normal_jobs<kJobThreadsWide>, dim3((unsigned)n_jobs), dim3(kJobThreadsWide), 0, s,
+                           st, jobs);
+    else
+        hipLaunchKernelGGL(k_std_normal_jobs<kJobThreads>, dim3((unsigned)n_jobs), dim3(kJobThreads), 0, s, st,
+                           jobs);
     return hipGetLastError();
 }
 

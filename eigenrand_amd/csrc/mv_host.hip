@@ -271,7 +271,14 @@ int run_std_normal_jobs(erd_engines* e, std::vector<FillJob> jobs, hipStream_t s
     std::stable_sort(jobs.begin(), jobs.end(), [](const FillJob& a, const FillJob& b) { return a.count > b.count; });
     FillJob* d_jobs;
     if (sc.upload(&d_jobs, jobs) != hipSuccess) return ERD_ERR_HIP;
-    return launch_std_normal_jobs(e->states, d_jobs, (int)jobs.size(), s) == hipSuccess ? ERD_OK : ERD_ERR_HIP;
+    // 640-thread workgroups when all of them are resident at once (three per
+    // CU), so the longest fills get ten waves each
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+    const bool wide = jobs.size() <= (size_t)cus * 3;
+    return launch_std_normal_jobs(e->states, d_jobs, (int)jobs.size(), s, wide) == hipSuccess ? ERD_OK : ERD_ERR_HIP;
 }
 
 }  // namespace erd
