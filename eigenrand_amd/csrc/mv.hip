@@ -240,24 +240,27 @@ __global__ __launch_bounds__(256, NCB == 4 || BART ? 2 : 3) void k_lz_gemm(const
     const int l16 = lane & 15, g = lane >> 4;
     // thread 0: the queue it takes tiles from, and how many it found empty
     int q = (int)(blockIdx.x % kLzQueues), emptied = 0;
-    for (;;) {
-        if (tid == 0) {
-            int t = -1;
-            while (emptied < kLzQueues) {
-                const int k = qbeg[q] + atomicAdd(counter + q, 1);
-                if (k < qbeg[q + 1]) {
-                    t = k;
-                    break;
-                }
-                q = q + 1 == kLzQueues ? 0 : q + 1;  // a queue found empty stays empty
-                ++emptied;
-            }
-            tile_s = t;
+    // thread 0: the next tile (-1: every queue empty), given the slot k
+    // already taken from queue q
+    auto next_tile = [&](int k) {
+        while (emptied < kLzQueues) {
+            if (k < qbeg[q + 1]) return k;
+            q = q + 1 == kLzQueues ? 0 : q + 1;  // a queue found empty stays empty
+            if (++emptied < kLzQueues) k = qbeg[q] + atomicAdd(counter + q, 1);
         }
+        return -1;
+    };
+    if (tid == 0) tile_s = next_tile(qbeg[q] + atomicAdd(counter + q, 1));
+    for (;;) {
         __syncthreads();
         const int t = tile_s;
         __syncthreads();
         if (t < 0) break;
+        // the slot for the tile after this one is taken now and looked at
+        // only when this tile is done, so the atomic's round trip overlaps
+        // the tile's chunks
+        int a_next = 0;
+        if (tid == 0 && emptied < kLzQueues) a_next = atomicAdd(counter + q, 1);
         const MvTile tl = tiles[t];
         const MvInstance in = inst[tl.inst];
         const int d = in.d, dp = (d + 15) & ~15;
@@ -436,6 +439,7 @@ __global__ __launch_bounds__(256, NCB == 4 || BART ? 2 : 3) void k_lz_gemm(const
                 }
             }
         }
+        if (tid == 0) tile_s = next_tile(emptied < kLzQueues ? qbeg[q] + a_next : 0);
     }
 }
 
