@@ -87,11 +87,13 @@ void build_tiles(const std::vector<MvInstance>& inst, const std::vector<int>& wh
     for (int q = 0; q < kLzQueues; ++q) {
         for (int i : queue[q]) {
             const int d = inst[i].d, dp = (d + 15) & ~15;
-            // Per-sample tiles skip a Bartlett factor's zero blocks; a sample
-            // narrower than half a tile's columns takes the tiles across
-            // samples instead (the zero blocks are then multiplied, with the
-            // same results), so that the tile's columns are used.
-            if (!bartlett || 2 * d < cols) {
+            // Per-sample tiles skip a Bartlett factor's zero blocks, which
+            // happens only in a sample's second and later column tiles (the
+            // first starts at k = 0 and skips no row tile); a sample no wider
+            // than one tile takes the tiles across samples instead (the zero
+            // blocks are then multiplied, with the same results), so that the
+            // tile's columns are used.
+            if (!bartlett || d <= cols) {
                 for (int64_t c = 0; c < inst[i].n; c += cols)
                     for (int r = 0; r < dp; r += rows)
                         tiles.push_back(MvTile{i, r, c, 0, (int32_t)std::min<int64_t>(cols, inst[i].n - c)});
